@@ -1,0 +1,435 @@
+! ref_harness.f90 -- drives the reference's own compiled FP64 routines
+! (n-parzuchowski/EOM-IMSRG_FULL, im-SRG/*.f90, built from the sources where
+! they lie by oracle/ref.mk into oracle/_ref/).
+!
+! TEST INFRASTRUCTURE ONLY.  It writes golden vectors and times the reference
+! CPU path; the product library never links it.
+!
+! Operators travel as raw FP64 streams in the reference's vectorize order
+! (basic_IMSRG.f90:3794-3854).  The basis comes from a text file:
+!   line 1: nsp ;  then nsp lines "n l 2j 2tz con"  (con=1 for holes)
+! which replaces read_sp_basis/find_holes (basic_IMSRG.f90:152-400) and
+! reproduces their side effects (hb4/pb4, holes/parts, states, store_6j).
+!
+! usage: ref_harness <basis> <command> args...
+!   terms  hL L.bin hR R.bin outprefix   -> per-term outputs of [L,R]
+!   comm   hL L.bin hR R.bin out.bin     -> full commutator incl. E0
+!   cc     h  A.bin out.bin              -> Pandya matrices of A (all blocks)
+!   bch    G.bin H.bin out.bin           -> BCH_EXPAND(HS,G,H)
+!   magexp DG.bin G.bin out.bin          -> MAGNUS_EXPAND(DG,G,AD)
+!   gen    kind H.bin out.bin            -> generator (white|atan|w2)
+!   normal H.bin out.bin                 -> normal_order(H)
+!   scal   h A.bin                       -> prints frob norm and mbpt2
+!   flow   kind HS.bin maxsteps outprefix -> magnus_decouple loop, full precision
+!   brute  hA A.bin hB B.bin points.txt out.txt -> brute-force J-coupled sums
+!   time   hL L.bin hR R.bin reps        -> wall seconds per full commutator
+program ref_harness
+  use basic_IMSRG
+  use cross_coupled
+  use commutators
+  use IMSRG_MAGNUS
+  use brute_force_testing
+  implicit none
+
+  type(spd) :: jbas
+  type(sq_op) :: HS
+  character(len=1024) :: basisfile, cmd
+  external :: build_gs_white, build_gs_w2, build_gs_atan
+
+  call getarg(1, basisfile)
+  call getarg(2, cmd)
+  checkpointing = .false.
+  OUTPUT_DIR = './'
+  prefix = 'ref_harness'
+
+  call setup_basis(trim(basisfile), jbas)
+  call allocate_blocks(jbas, HS)
+  HS%herm = 1
+  HS%neq = compute_neq(HS)
+  HS%hospace = 20.d0
+  HS%lawson_beta = 0.d0
+  HS%com_hw = 0.d0
+  HS%eMax = maxval(jbas%nshell)
+
+  select case (trim(cmd))
+  case ('terms')
+     call do_terms()
+  case ('comm')
+     call do_comm()
+  case ('cc')
+     call do_cc()
+  case ('bch')
+     call do_bch()
+  case ('magexp')
+     call do_magexp()
+  case ('gen')
+     call do_gen()
+  case ('normal')
+     call do_normal()
+  case ('scal')
+     call do_scal()
+  case ('flow')
+     call do_flow()
+  case ('brute')
+     call do_brute()
+  case ('time')
+     call do_time()
+  case default
+     print*, 'unknown command ', trim(cmd)
+     stop 2
+  end select
+
+contains
+
+  integer function compute_neq(op)
+    ! IMSRG_ODE.f90:36-44
+    type(sq_op) :: op
+    integer :: A, N, q, nh, np, nb
+    A = op%belowEF
+    N = op%Nsp
+    compute_neq = 1 + A*A + A*(N-A) + (N-A)**2
+    do q = 1, op%nblocks
+       nh = op%mat(q)%nhh ; np = op%mat(q)%npp ; nb = op%mat(q)%nph
+       compute_neq = compute_neq + (nh*nh+nh + nb*nb+nb + np*np+np)/2 + nb*np + nh*np + nh*nb
+    end do
+  end function compute_neq
+
+  subroutine setup_basis(fname, jbas)
+    character(len=*) :: fname
+    type(spd) :: jbas
+    integer :: ix, i, r, q, tzi, li, ji, r1, r2
+    open(unit=39, file=fname, status='old')
+    read(39,*) ix
+    jbas%total_orbits = ix
+    allocate(jbas%nn(ix), jbas%ll(ix), jbas%jj(ix), jbas%itzp(ix), jbas%nshell(ix))
+    allocate(jbas%e(ix), jbas%con(ix), hb4(ix), pb4(ix))
+    do i = 1, ix
+       read(39,*) jbas%nn(i), jbas%ll(i), jbas%jj(i), jbas%itzp(i), jbas%con(i)
+       jbas%nshell(i) = 2*jbas%nn(i) + jbas%ll(i)
+       jbas%e(i) = jbas%nshell(i)
+    end do
+    close(39)
+    ! find_holes side effects, basic_IMSRG.f90:370-397
+    allocate(jbas%holes(sum(jbas%con)))
+    allocate(jbas%parts(jbas%total_orbits - sum(jbas%con)))
+    r1 = 1 ; r2 = 1
+    do i = 1, jbas%total_orbits
+       hb4(i) = sum(jbas%con(1:i-1))
+       pb4(i) = i - hb4(i) - 1
+       if (jbas%con(i) == 1) then
+          jbas%holes(r1) = i ; r1 = r1 + 1
+       else
+          jbas%parts(r2) = i ; r2 = r2 + 1
+       end if
+    end do
+    ! read_sp_basis tail, basic_IMSRG.f90:304-340
+    jbas%Jtotal_max = maxval(jbas%jj)
+    jbas%lmax = maxval(jbas%ll)
+    call store_6j(jbas, 'n')
+    jbas%spblocks = (jbas%Jtotal_max + 1) * (jbas%lmax + 1)
+    allocate(jbas%states(jbas%spblocks))
+    q = 1
+    do tzi = -1, 1, 2
+       do li = 0, jbas%lmax
+          do ji = 1, jbas%Jtotal_max, 2
+             r = 0
+             do i = 1, jbas%total_orbits
+                if (jbas%jj(i) .ne. ji) cycle
+                if (jbas%ll(i) .ne. li) cycle
+                if (jbas%itzp(i) .ne. tzi) cycle
+                r = r + 1
+             end do
+             allocate(jbas%states(q)%Z(r))
+             r = 0
+             do i = 1, jbas%total_orbits
+                if (jbas%jj(i) .ne. ji) cycle
+                if (jbas%ll(i) .ne. li) cycle
+                if (jbas%itzp(i) .ne. tzi) cycle
+                r = r + 1
+                jbas%states(q)%Z(r) = i
+             end do
+             q = q + 1
+          end do
+       end do
+    end do
+  end subroutine setup_basis
+
+  subroutine read_op(fname, op)
+    character(len=*) :: fname
+    type(sq_op) :: op
+    real(8), allocatable :: v(:)
+    allocate(v(HS%neq))
+    open(unit=41, file=fname, access='stream', form='unformatted', status='old')
+    read(41) v
+    close(41)
+    call repackage(op, v)
+  end subroutine read_op
+
+  subroutine write_op(fname, op)
+    character(len=*) :: fname
+    type(sq_op) :: op
+    real(8), allocatable :: v(:)
+    allocate(v(HS%neq))
+    call vectorize(op, v)
+    open(unit=42, file=fname, access='stream', form='unformatted', status='replace')
+    write(42) v
+    close(42)
+  end subroutine write_op
+
+  integer function iarg(k)
+    integer :: k
+    character(len=64) :: s
+    call getarg(k, s)
+    read(s,*) iarg
+  end function iarg
+
+  character(len=1024) function sarg(k)
+    integer :: k
+    call getarg(k, sarg)
+  end function sarg
+
+  subroutine load_pair(L, R)
+    type(sq_op) :: L, R
+    call duplicate_sq_op(HS, L)
+    call duplicate_sq_op(HS, R)
+    L%herm = iarg(3)
+    call read_op(trim(sarg(4)), L)
+    R%herm = iarg(5)
+    call read_op(trim(sarg(6)), R)
+  end subroutine load_pair
+
+  subroutine do_terms()
+    type(sq_op) :: L, R, RES, w1, w2
+    type(cc_mat) :: LCC, RCC, WCC
+    character(len=1024) :: pfx
+    real(8) :: e110, e220
+    call load_pair(L, R)
+    pfx = sarg(7)
+    call duplicate_sq_op(HS, RES)
+    call duplicate_sq_op(HS, w1)
+    call duplicate_sq_op(HS, w2)
+    RES%herm = -1 * L%herm * R%herm
+    call init_ph_mat(L, LCC, jbas)
+    call duplicate_ph_mat(LCC, RCC)
+    call init_ph_wkspc(LCC, WCC)
+    e110 = commutator_110(L, R, jbas)
+    e220 = commutator_220(L, R, jbas)
+    open(unit=43, file=trim(pfx)//'_e0.txt', status='replace')
+    write(43,'(2es26.17)') e110, e220
+    close(43)
+    call clear_sq_op(RES) ; call commutator_111(L, R, RES, jbas)
+    call write_op(trim(pfx)//'_111.bin', RES)
+    call clear_sq_op(RES) ; call commutator_121(L, R, RES, jbas)
+    call write_op(trim(pfx)//'_121.bin', RES)
+    call clear_sq_op(RES) ; call commutator_122(L, R, RES, jbas)
+    call write_op(trim(pfx)//'_122.bin', RES)
+    call clear_sq_op(RES) ; call commutator_222_pp_hh(L, R, RES, w1, w2, jbas)
+    call write_op(trim(pfx)//'_222pphh.bin', RES)
+    call clear_sq_op(RES) ; call commutator_221(L, R, RES, w1, w2, jbas)
+    call write_op(trim(pfx)//'_221.bin', RES)
+    call calculate_cross_coupled(L, LCC, jbas)
+    call calculate_cross_coupled(R, RCC, jbas)
+    call clear_sq_op(RES) ; call commutator_222_ph(LCC, RCC, RES, WCC, jbas)
+    call write_op(trim(pfx)//'_222ph.bin', RES)
+  end subroutine do_terms
+
+  subroutine full_comm(L, R, RES)
+    ! the call sequence of IMSRG_MAGNUS.f90:207-219
+    type(sq_op) :: L, R, RES, w1, w2
+    type(cc_mat) :: LCC, RCC, WCC
+    call duplicate_sq_op(HS, w1)
+    call duplicate_sq_op(HS, w2)
+    call init_ph_mat(L, LCC, jbas)
+    call duplicate_ph_mat(LCC, RCC)
+    call init_ph_wkspc(LCC, WCC)
+    call calculate_cross_coupled(R, RCC, jbas)
+    call calculate_cross_coupled(L, LCC, jbas)
+    RES%E0 = commutator_110(L, R, jbas) + commutator_220(L, R, jbas)
+    call commutator_111(L, R, RES, jbas)
+    call commutator_121(L, R, RES, jbas)
+    call commutator_122(L, R, RES, jbas)
+    call commutator_222_pp_hh(L, R, RES, w1, w2, jbas)
+    call commutator_221(L, R, RES, w1, w2, jbas)
+    call commutator_222_ph(LCC, RCC, RES, WCC, jbas)
+  end subroutine full_comm
+
+  subroutine do_comm()
+    type(sq_op) :: L, R, RES
+    call load_pair(L, R)
+    call duplicate_sq_op(HS, RES)
+    RES%herm = -1 * L%herm * R%herm
+    call full_comm(L, R, RES)
+    call write_op(trim(sarg(7)), RES)
+  end subroutine do_comm
+
+  subroutine do_cc()
+    type(sq_op) :: A
+    type(cc_mat) :: ACC
+    integer :: q
+    call duplicate_sq_op(HS, A)
+    A%herm = iarg(3)
+    call read_op(trim(sarg(4)), A)
+    call init_ph_mat(A, ACC, jbas)
+    call calculate_cross_coupled(A, ACC, jbas)
+    open(unit=44, file=trim(sarg(5)), access='stream', form='unformatted', status='replace')
+    do q = 1, ACC%nblocks
+       if (ACC%rlen(q) * ACC%nph(q) == 0) cycle
+       write(44) ACC%CCX(q)%X
+    end do
+    close(44)
+  end subroutine do_cc
+
+  subroutine do_bch()
+    type(sq_op) :: G, H0, HOUT
+    call duplicate_sq_op(HS, G)
+    call duplicate_sq_op(HS, H0)
+    call duplicate_sq_op(HS, HOUT)
+    G%herm = -1
+    H0%herm = 1
+    call read_op(trim(sarg(3)), G)
+    call read_op(trim(sarg(4)), H0)
+    call BCH_EXPAND(HOUT, G, H0, jbas, 'n')
+    call write_op(trim(sarg(5)), HOUT)
+  end subroutine do_bch
+
+  subroutine do_magexp()
+    type(sq_op) :: DG, G, AD
+    call duplicate_sq_op(HS, DG)
+    call duplicate_sq_op(HS, G)
+    call duplicate_sq_op(HS, AD)
+    DG%herm = -1
+    G%herm = -1
+    call read_op(trim(sarg(3)), DG)
+    call read_op(trim(sarg(4)), G)
+    call MAGNUS_EXPAND(DG, G, AD, jbas)
+    call write_op(trim(sarg(5)), DG)
+  end subroutine do_magexp
+
+  subroutine run_gen(kind, H, ETA)
+    character(len=*) :: kind
+    type(sq_op) :: H, ETA
+    select case (kind)
+    case ('white')
+       call build_gs_white(H, ETA, jbas)
+    case ('atan')
+       call build_gs_atan(H, ETA, jbas)
+    case ('w2')
+       call build_gs_w2(H, ETA, jbas)
+    case default
+       stop 'bad generator'
+    end select
+  end subroutine run_gen
+
+  subroutine do_gen()
+    type(sq_op) :: H, ETA
+    call duplicate_sq_op(HS, H)
+    call duplicate_sq_op(HS, ETA)
+    H%herm = 1
+    call read_op(trim(sarg(4)), H)
+    ETA%herm = -1
+    call run_gen(trim(sarg(3)), H, ETA)
+    call write_op(trim(sarg(5)), ETA)
+  end subroutine do_gen
+
+  subroutine do_normal()
+    type(sq_op) :: H
+    call duplicate_sq_op(HS, H)
+    H%herm = 1
+    call read_op(trim(sarg(3)), H)
+    call normal_order(H, jbas)
+    call write_op(trim(sarg(4)), H)
+  end subroutine do_normal
+
+  subroutine do_scal()
+    type(sq_op) :: A
+    call duplicate_sq_op(HS, A)
+    A%herm = iarg(3)
+    call read_op(trim(sarg(4)), A)
+    write(*,'(2es26.17)') mat_frob_norm(A), mbpt2(A, jbas)
+  end subroutine do_scal
+
+  subroutine do_flow()
+    ! body of magnus_decouple, IMSRG_MAGNUS.f90:15-128, with full-precision output
+    type(sq_op) :: H, G, AD, DG, H0
+    character(len=64) :: kind
+    integer :: steps, maxsteps
+    real(8) :: s, ds, E_mbpt2, crit
+    kind = sarg(3)
+    maxsteps = iarg(5)
+    call duplicate_sq_op(HS, H0)
+    H0%herm = 1
+    call read_op(trim(sarg(4)), H0)
+    call copy_sq_op(H0, HS)
+    call duplicate_sq_op(HS, H)
+    call duplicate_sq_op(HS, G)
+    call duplicate_sq_op(HS, AD)
+    call duplicate_sq_op(HS, DG)
+    G%herm = -1
+    DG%herm = -1
+    call run_gen(trim(kind), HS, DG)
+    call copy_sq_op(HS, H)
+    s = 0.d0
+    ds = 0.5d0
+    steps = 0
+    E_mbpt2 = mbpt2(HS, jbas)
+    crit = abs(E_mbpt2)
+    open(unit=45, file=trim(sarg(6))//'_steps.txt', status='replace')
+    do while (crit > 1e-6)
+       if (steps >= maxsteps) exit
+       steps = steps + 1
+       call MAGNUS_EXPAND(DG, G, AD, jbas)
+       call euler_step(G, DG, s, ds)
+       call BCH_EXPAND(HS, G, H, jbas, 'n')
+       call run_gen(trim(kind), HS, DG)
+       E_mbpt2 = mbpt2(HS, jbas)
+       crit = abs(E_mbpt2)
+       write(45,'(I6,4es26.17)') steps, s, HS%E0, HS%E0 + E_mbpt2, crit
+       flush(45)
+    end do
+    close(45)
+    call write_op(trim(sarg(6))//'_HS.bin', HS)
+    call write_op(trim(sarg(6))//'_G.bin', G)
+  end subroutine do_flow
+
+  subroutine do_brute()
+    type(sq_op) :: A, B
+    integer :: u, kind, a1, b1, c1, d1, J, ios
+    real(8) :: val
+    call load_pair(A, B)
+    open(unit=46, file=trim(sarg(7)), status='old')
+    open(unit=47, file=trim(sarg(8)), status='replace')
+    do
+       read(46,*,iostat=ios) kind, a1, b1, c1, d1, J
+       if (ios /= 0) exit
+       select case (kind)
+       case (0)
+          val = scalar_scalar_0body_comm(A, B, jbas)
+       case (1)
+          val = scalar_scalar_1body_comm(A, B, a1, b1, jbas)
+       case default
+          val = scalar_scalar_2body_comm(A, B, a1, b1, c1, d1, J, jbas)
+       end select
+       write(47,'(6I6,es26.17)') kind, a1, b1, c1, d1, J, val
+    end do
+    close(46)
+    close(47)
+    u = 0
+  end subroutine do_brute
+
+  subroutine do_time()
+    type(sq_op) :: L, R, RES
+    integer :: reps, k
+    real(8) :: t0, t1, omp_get_wtime
+    call load_pair(L, R)
+    reps = iarg(7)
+    call duplicate_sq_op(HS, RES)
+    RES%herm = -1 * L%herm * R%herm
+    t0 = omp_get_wtime()
+    do k = 1, reps
+       call full_comm(L, R, RES)
+    end do
+    t1 = omp_get_wtime()
+    write(*,'(A,es16.8)') 'SECONDS_PER_COMMUTATOR ', (t1 - t0) / reps
+  end subroutine do_time
+
+end program ref_harness
