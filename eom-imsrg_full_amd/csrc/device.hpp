@@ -1,0 +1,70 @@
+// device.hpp -- device-side view of the layout and the kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace imsrg {
+
+// Read-only tables resident in HBM, passed by value to every kernel.
+struct DevLayout {
+  int nsp, nhole, npart, nblocks, ncc, jtot_max, nhalf, nJ, npairs, nob, nphp, nccol;
+  const int *jj, *l, *tz, *con;
+  const int *grp_of, *grp_start, *grp_cnt, *grp_list;
+  const int *bJ, *bnp, *bnb, *bnh, *bdim;
+  const long long *boff;
+  const int *pbase, *pair_a, *pair_b, *pair_q;
+  const int *pidx;
+  const int *nb_start, *nb_pos, *nb_x, *nb_i;
+  const double *nb_coef;
+  const int *ob_p, *ob_q;
+  const int *php_a, *php_i;  // same-(l,j,tz) (particle, hole) pairs
+  const int *ccJ, *ccrlen, *ccnph;
+  const long long *ccoff, *cwoff;
+  const int *ccrow, *ccrbase, *ccrow_a, *ccrow_b, *cccbase, *cccol_p, *cccol_h, *cccol_q;
+  const double *sixj;
+  long long f_off, v_off, op_size, vsize;
+};
+
+struct GemmProb {
+  int M, N, K, lda, ldb, ldc;
+  long long a_off, b_off, c_off;
+};
+
+// term bits (mirror the C ABI)
+enum : unsigned {
+  T110 = 1u << 0, T111 = 1u << 1, T121 = 1u << 2, T122 = 1u << 3,
+  T220 = 1u << 4, T221 = 1u << 5, T222PPHH = 1u << 6, T222PH = 1u << 7
+};
+
+// ---- launchers (kernels.hip) ----
+void launch_fill(hipStream_t s, double *p, long long n, double v);
+void launch_axpby(hipStream_t s, long long n, double a, const double *A, double b, const double *B,
+                  double *C);
+void launch_gemm_f64(hipStream_t s, bool a_kmajor, const GemmProb *probs, const int4 *tiles,
+                     int ntiles, const double *A, const double *B, double *C);
+void launch_pandya(hipStream_t s, const DevLayout &L, const double *op, double *cc);
+void launch_122(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES);
+void launch_clear_upper(hipStream_t s, const DevLayout &L, double *RES);
+void launch_pphh_combine(hipStream_t s, const DevLayout &L, const double *Wa, const double *Wb,
+                         double *RES, double pm);
+void launch_ph_gather(hipStream_t s, const DevLayout &L, const double *W, double *RES, double hL,
+                      double hR);
+void launch_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm);
+void launch_onebody(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop,
+                    const double *Wa, const double *Wb, double *RES, int hL, int hR, int hres,
+                    unsigned mask);
+// E0 = 110 + 220 into RES[0]; partial buffer of >= 1024 doubles
+void launch_e0(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES,
+               int hL, int hR, unsigned mask, double *partials);
+void launch_sumsq(hipStream_t s, const DevLayout &L, const double *op, double *partials, double *out);
+void launch_mbpt2(hipStream_t s, const DevLayout &L, const double *op, double *partials, double *out);
+void launch_monopole(hipStream_t s, const DevLayout &L, const double *H, double *mono);
+void launch_eta(hipStream_t s, const DevLayout &L, const double *Hden, const double *Hnum,
+                const double *mono, double *ETA, int kind);
+void launch_normal_order(hipStream_t s, const DevLayout &L, const double *T, double *H,
+                         double *partials);
+
+constexpr int kPartials = 1024;
+
+}  // namespace imsrg

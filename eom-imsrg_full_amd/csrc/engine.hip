@@ -1,0 +1,614 @@
+// engine.hip -- contexts, device-resident operators, the commutator
+// orchestration and the Magnus/BCH drivers behind the C ABI (imsrg_mi355x.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/imsrg_mi355x.h"
+#include "device.hpp"
+#include "layout.hpp"
+
+using namespace imsrg;
+
+static thread_local std::string g_err;
+static int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+#define HIPCHK(call)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(IMSRG_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+struct imsrg_op_s;
+
+struct imsrg_ctx_s {
+  Layout lay;
+  DevLayout dl{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<void *> dallocs;
+  double *wpphh = nullptr;  // W1 | W2, 2 * vsize
+  double *cc[2] = {nullptr, nullptr};
+  double *wph = nullptr;
+  double *partials = nullptr;
+  double *scal = nullptr;   // device scalars
+  double *hscal = nullptr;  // pinned host scalars
+  double *mono = nullptr;
+  double *hstage = nullptr; // pinned op_size staging
+  GemmProb *pp_probs = nullptr, *ph_probs = nullptr;
+  int4 *pp_tiles = nullptr, *ph_tiles = nullptr;
+  int n_pp_tiles = 0, n_ph_tiles = 0;
+  int prec = IMSRG_PREC_FP64;
+  imsrg_op_s *tmp[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+};
+
+struct imsrg_op_s {
+  imsrg_ctx_s *ctx = nullptr;
+  double *d = nullptr;
+  int herm = 1;
+};
+
+const char *imsrg_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------- helpers
+template <class T>
+static int upload_table(imsrg_ctx_s *c, const std::vector<T> &v, const T **out) {
+  size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) return fail(IMSRG_ENOMEM, "hipMalloc table failed");
+  c->dallocs.push_back(p);
+  if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  *out = (const T *)p;
+  return 0;
+}
+static int dalloc(imsrg_ctx_s *c, size_t ndouble, double **out) {
+  void *p = nullptr;
+  if (hipMalloc(&p, std::max<size_t>(ndouble, 1) * sizeof(double)) != hipSuccess)
+    return fail(IMSRG_ENOMEM, "hipMalloc workspace failed (" + std::to_string(ndouble * 8) + " bytes)");
+  c->dallocs.push_back(p);
+  *out = (double *)p;
+  return 0;
+}
+
+static void build_tiles(const std::vector<GemmProb> &probs, std::vector<int4> &tiles) {
+  tiles.clear();
+  std::vector<int> order(probs.size());
+  for (size_t i = 0; i < probs.size(); i++) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return probs[a].K > probs[b].K; });
+  for (int pi : order) {
+    const GemmProb &p = probs[pi];
+    int tm = (p.M + 63) / 64, tn = (p.N + 63) / 64;
+    for (int i = 0; i < tm; i++)
+      for (int j = 0; j < tn; j++) tiles.push_back(make_int4(pi, i, j, 0));
+  }
+}
+
+static int op_new(imsrg_ctx_s *c, int herm, imsrg_op_s **out) {
+  imsrg_op_s *op = new imsrg_op_s;
+  op->ctx = c;
+  op->herm = herm;
+  if (hipMalloc(&op->d, sizeof(double) * c->lay.op_size) != hipSuccess) {
+    delete op;
+    return fail(IMSRG_ENOMEM, "hipMalloc operator failed");
+  }
+  launch_fill(c->stream, op->d, c->lay.op_size, 0.0);
+  *out = op;
+  return 0;
+}
+static imsrg_op_s *tmp_op(imsrg_ctx_s *c, int slot, int herm) {
+  if (!c->tmp[slot]) {
+    if (op_new(c, herm, &c->tmp[slot]) != 0) return nullptr;
+  }
+  c->tmp[slot]->herm = herm;
+  return c->tmp[slot];
+}
+static void op_copy(imsrg_op_s *src, imsrg_op_s *dst) {
+  imsrg_ctx_s *c = src->ctx;
+  (void)hipMemcpyAsync(dst->d, src->d, sizeof(double) * c->lay.op_size, hipMemcpyDeviceToDevice, c->stream);
+  dst->herm = src->herm;
+}
+static int read_scalar(imsrg_ctx_s *c, int idx, double *out) {
+  HIPCHK(hipMemcpyAsync(c->hscal + idx, c->scal + idx, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *out = c->hscal[idx];
+  return 0;
+}
+static int op_norm(imsrg_op_s *op, double *out) {
+  imsrg_ctx_s *c = op->ctx;
+  launch_sumsq(c->stream, c->dl, op->d, c->partials, c->scal);
+  return read_scalar(c, 0, out);
+}
+
+// ---------------------------------------------------------------- context
+int imsrg_ctx_create(int nsp, const int *n, const int *l, const int *jj, const int *tz, const int *con,
+                     int device, imsrg_ctx *out) {
+  if (!out || !n || !l || !jj || !tz || !con) return fail(IMSRG_EINVAL, "null argument");
+  imsrg_ctx_s *c = new imsrg_ctx_s;
+  std::string err = c->lay.build(nsp, n, l, jj, tz, con);
+  if (!err.empty()) {
+    delete c;
+    return fail(IMSRG_EINVAL, err);
+  }
+  if (device >= 0) {
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) {
+      delete c;
+      return fail(IMSRG_EDEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    }
+  }
+  (void)hipGetDevice(&c->device);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(IMSRG_EDEVICE, "hipStreamCreate failed");
+  }
+  const Layout &Ly = c->lay;
+  DevLayout &D = c->dl;
+  D.nsp = Ly.nsp; D.nhole = Ly.nhole; D.npart = Ly.npart; D.nblocks = Ly.nblocks; D.ncc = Ly.ncc;
+  D.jtot_max = Ly.jtot_max; D.nhalf = Ly.nhalf; D.nJ = Ly.nJ; D.npairs = Ly.npairs;
+  D.nob = (int)Ly.ob_p.size();
+  D.f_off = Ly.f_off; D.v_off = Ly.v_off; D.op_size = Ly.op_size; D.vsize = Ly.vsize;
+  std::vector<int> php_a, php_i;
+  for (int a : Ly.parts)
+    for (int i : Ly.holes)
+      if (Ly.jj[a] == Ly.jj[i] && Ly.l[a] == Ly.l[i] && Ly.tz[a] == Ly.tz[i]) {
+        php_a.push_back(a);
+        php_i.push_back(i);
+      }
+  D.nphp = (int)php_a.size();
+  std::vector<int> cccol_q(Ly.cccol_p.size());
+  for (int q = 0; q < Ly.ncc; q++)
+    for (int k = Ly.cccbase[q]; k < Ly.cccbase[q + 1]; k++) cccol_q[k] = q;
+  D.nccol = (int)cccol_q.size();
+  std::vector<long long> boff(Ly.boff.begin(), Ly.boff.end()), ccoff(Ly.ccoff.begin(), Ly.ccoff.end()),
+      cwoff(Ly.cwoff.begin(), Ly.cwoff.end());
+  int rc = 0;
+#define UP(vec, field) \
+  if (!rc) rc = upload_table(c, vec, &D.field)
+  UP(Ly.jj, jj); UP(Ly.l, l); UP(Ly.tz, tz); UP(Ly.con, con);
+  UP(Ly.grp_of, grp_of); UP(Ly.grp_start, grp_start); UP(Ly.grp_cnt, grp_cnt); UP(Ly.grp_list, grp_list);
+  UP(Ly.bJ, bJ); UP(Ly.bnp, bnp); UP(Ly.bnb, bnb); UP(Ly.bnh, bnh); UP(Ly.bdim, bdim); UP(boff, boff);
+  UP(Ly.pbase, pbase); UP(Ly.pair_a, pair_a); UP(Ly.pair_b, pair_b); UP(Ly.pair_q, pair_q);
+  UP(Ly.pidx, pidx);
+  UP(Ly.nb_start, nb_start); UP(Ly.nb_pos, nb_pos); UP(Ly.nb_x, nb_x); UP(Ly.nb_i, nb_i);
+  UP(Ly.nb_coef, nb_coef);
+  UP(Ly.ob_p, ob_p); UP(Ly.ob_q, ob_q); UP(php_a, php_a); UP(php_i, php_i);
+  UP(Ly.ccJ, ccJ); UP(Ly.ccrlen, ccrlen); UP(Ly.ccnph, ccnph); UP(ccoff, ccoff); UP(cwoff, cwoff);
+  UP(Ly.ccrow, ccrow); UP(Ly.ccrbase, ccrbase); UP(Ly.ccrow_a, ccrow_a); UP(Ly.ccrow_b, ccrow_b);
+  UP(Ly.cccbase, cccbase); UP(Ly.cccol_p, cccol_p); UP(Ly.cccol_h, cccol_h); UP(cccol_q, cccol_q);
+  UP(Ly.sixj, sixj);
+#undef UP
+  if (!rc) rc = dalloc(c, 2 * (size_t)Ly.vsize, &c->wpphh);
+  if (!rc) rc = dalloc(c, Ly.ccsize, &c->cc[0]);
+  if (!rc) rc = dalloc(c, Ly.ccsize, &c->cc[1]);
+  if (!rc) rc = dalloc(c, Ly.cwsize, &c->wph);
+  if (!rc) rc = dalloc(c, kPartials, &c->partials);
+  if (!rc) rc = dalloc(c, 64, &c->scal);
+  if (!rc) rc = dalloc(c, (size_t)Ly.nsp * Ly.nsp, &c->mono);
+  if (!rc && hipHostMalloc((void **)&c->hscal, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    rc = fail(IMSRG_ENOMEM, "hipHostMalloc failed");
+  if (!rc && hipHostMalloc((void **)&c->hstage, Ly.op_size * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    rc = fail(IMSRG_ENOMEM, "hipHostMalloc staging failed");
+  // GEMM problem tables
+  if (!rc) {
+    std::vector<GemmProb> pp;
+    for (int q = 0; q < Ly.nblocks; q++) {
+      int dim = Ly.bdim[q];
+      if (dim == 0) continue;
+      int np = Ly.bnp[q], nb = Ly.bnb[q], nh = Ly.bnh[q];
+      long long bo = Ly.boff[q];
+      pp.push_back(GemmProb{dim, dim, np, dim, dim, dim, bo, bo, bo});
+      pp.push_back(GemmProb{dim, dim, nh, dim, dim, dim, bo + np + nb, bo + (long long)(np + nb) * dim,
+                            Ly.vsize + bo});
+    }
+    std::vector<int4> tiles;
+    build_tiles(pp, tiles);
+    c->n_pp_tiles = (int)tiles.size();
+    rc = upload_table(c, pp, (const GemmProb **)&c->pp_probs);
+    if (!rc) rc = upload_table(c, tiles, (const int4 **)&c->pp_tiles);
+  }
+  if (!rc) {
+    std::vector<GemmProb> ph;
+    for (int q = 0; q < Ly.ncc; q++) {
+      int r = Ly.ccrlen[q];
+      if (r == 0) continue;
+      ph.push_back(GemmProb{r, r, Ly.ccnph[q], r, r, r, Ly.ccoff[q], Ly.ccoff[q], Ly.cwoff[q]});
+    }
+    std::vector<int4> tiles;
+    build_tiles(ph, tiles);
+    c->n_ph_tiles = (int)tiles.size();
+    rc = upload_table(c, ph, (const GemmProb **)&c->ph_probs);
+    if (!rc) rc = upload_table(c, tiles, (const int4 **)&c->ph_tiles);
+  }
+  if (!rc && Ly.cwsize > 0) launch_fill(c->stream, c->wph, Ly.cwsize, 0.0);
+  if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(IMSRG_EDEVICE, "context setup failed");
+  if (rc) {
+    imsrg_ctx_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+int imsrg_ctx_destroy(imsrg_ctx c) {
+  if (!c) return 0;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto *t : c->tmp)
+    if (t) {
+      (void)hipFree(t->d);
+      delete t;
+    }
+  for (void *p : c->dallocs) (void)hipFree(p);
+  if (c->hscal) (void)hipHostFree(c->hscal);
+  if (c->hstage) (void)hipHostFree(c->hstage);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+int64_t imsrg_ctx_neq(imsrg_ctx c) { return c ? c->lay.neq : -1; }
+
+int imsrg_ctx_info(imsrg_ctx c, double *info) {
+  if (!c || !info) return fail(IMSRG_EINVAL, "null argument");
+  const Layout &L = c->lay;
+  int maxdim = 0, maxr = 0;
+  for (int d : L.bdim) maxdim = std::max(maxdim, d);
+  for (int r : L.ccrlen) maxr = std::max(maxr, r);
+  double v[16] = {(double)L.nsp, (double)L.nhole, (double)L.nblocks, (double)L.ncc, (double)L.npairs,
+                  (double)maxdim, (double)maxr, 8.0 * L.op_size, (double)L.vsize, (double)L.ccsize,
+                  (double)L.cwsize, L.flop_pphh, L.flop_ph, 0, 0, 0};
+  std::memcpy(info, v, sizeof(v));
+  return 0;
+}
+
+int imsrg_ctx_set_precision(imsrg_ctx c, int prec) {
+  if (!c) return fail(IMSRG_EINVAL, "null context");
+  if (prec != IMSRG_PREC_FP64 && prec != IMSRG_PREC_SPLIT_FP32) return fail(IMSRG_EINVAL, "unknown precision");
+  c->prec = prec;
+  return 0;
+}
+int imsrg_ctx_synchronize(imsrg_ctx c) {
+  if (!c) return fail(IMSRG_EINVAL, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+void *imsrg_ctx_stream(imsrg_ctx c) { return c ? (void *)c->stream : nullptr; }
+
+// ---------------------------------------------------------------- operators
+int imsrg_op_create(imsrg_ctx c, int herm, imsrg_op *out) {
+  if (!c || !out) return fail(IMSRG_EINVAL, "null argument");
+  if (herm != 1 && herm != -1) return fail(IMSRG_EINVAL, "herm must be +1 or -1");
+  return op_new(c, herm, out);
+}
+int imsrg_op_destroy(imsrg_op op) {
+  if (!op) return 0;
+  (void)hipStreamSynchronize(op->ctx->stream);
+  (void)hipFree(op->d);
+  delete op;
+  return 0;
+}
+int imsrg_op_set_herm(imsrg_op op, int herm) {
+  if (!op || (herm != 1 && herm != -1)) return fail(IMSRG_EINVAL, "bad herm");
+  op->herm = herm;
+  return 0;
+}
+int imsrg_op_get_herm(imsrg_op op, int *herm) {
+  if (!op || !herm) return fail(IMSRG_EINVAL, "null argument");
+  *herm = op->herm;
+  return 0;
+}
+int imsrg_op_upload(imsrg_op op, const double *vec, int64_t neq) {
+  if (!op || !vec) return fail(IMSRG_EINVAL, "null argument");
+  imsrg_ctx_s *c = op->ctx;
+  if (neq != c->lay.neq) return fail(IMSRG_EINVAL, "neq mismatch");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->lay.unpack_vec(vec, op->herm, c->hstage);
+  HIPCHK(hipMemcpyAsync(op->d, c->hstage, sizeof(double) * c->lay.op_size, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+int imsrg_op_download(imsrg_op op, double *vec, int64_t neq) {
+  if (!op || !vec) return fail(IMSRG_EINVAL, "null argument");
+  imsrg_ctx_s *c = op->ctx;
+  if (neq != c->lay.neq) return fail(IMSRG_EINVAL, "neq mismatch");
+  HIPCHK(hipMemcpyAsync(c->hstage, op->d, sizeof(double) * c->lay.op_size, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->lay.pack_vec(c->hstage, vec);
+  return 0;
+}
+int imsrg_op_copy(imsrg_op src, imsrg_op dst) {
+  if (!src || !dst || src->ctx != dst->ctx) return fail(IMSRG_EINVAL, "bad operands");
+  op_copy(src, dst);
+  return 0;
+}
+int imsrg_op_clear(imsrg_op op) {
+  if (!op) return fail(IMSRG_EINVAL, "null operand");
+  launch_fill(op->ctx->stream, op->d, op->ctx->lay.op_size, 0.0);
+  return 0;
+}
+int imsrg_op_scale(imsrg_op op, double a) {
+  if (!op) return fail(IMSRG_EINVAL, "null operand");
+  launch_axpby(op->ctx->stream, op->ctx->lay.op_size, a, op->d, 0.0, op->d, op->d);
+  return 0;
+}
+int imsrg_op_add(imsrg_op A, double ax, imsrg_op B, double bx, imsrg_op C) {
+  if (!A || !B || !C || A->ctx != B->ctx || A->ctx != C->ctx) return fail(IMSRG_EINVAL, "bad operands");
+  launch_axpby(A->ctx->stream, A->ctx->lay.op_size, ax, A->d, bx, B->d, C->d);
+  return 0;
+}
+int imsrg_op_get_e0(imsrg_op op, double *e0) {
+  if (!op || !e0) return fail(IMSRG_EINVAL, "null argument");
+  imsrg_ctx_s *c = op->ctx;
+  HIPCHK(hipMemcpyAsync(c->hscal + 1, op->d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *e0 = c->hscal[1];
+  return 0;
+}
+int imsrg_op_set_e0(imsrg_op op, double e0) {
+  if (!op) return fail(IMSRG_EINVAL, "null operand");
+  launch_fill(op->ctx->stream, op->d, 1, e0);
+  return 0;
+}
+int imsrg_op_norm(imsrg_op op, double *out) {
+  if (!op || !out) return fail(IMSRG_EINVAL, "null argument");
+  return op_norm(op, out);
+}
+int imsrg_mbpt2(imsrg_op H, double *out) {
+  if (!H || !out) return fail(IMSRG_EINVAL, "null argument");
+  imsrg_ctx_s *c = H->ctx;
+  launch_mbpt2(c->stream, c->dl, H->d, c->partials, c->scal + 2);
+  return read_scalar(c, 2, out);
+}
+int imsrg_normal_order(imsrg_op H) {
+  if (!H) return fail(IMSRG_EINVAL, "null operand");
+  imsrg_ctx_s *c = H->ctx;
+  imsrg_op_s *T = tmp_op(c, 5, H->herm);
+  if (!T) return IMSRG_ENOMEM;
+  op_copy(H, T);
+  launch_normal_order(c->stream, c->dl, T->d, H->d, c->partials);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------- commutator
+// lcc_ready: cc[0] already holds the Pandya matrices of L (drivers reuse them
+// across the nested commutators of one BCH / Magnus series, where L = Omega).
+static int comm_impl(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *RES, unsigned mask,
+                     bool lcc_ready) {
+  hipStream_t s = c->stream;
+  const DevLayout &D = c->dl;
+  const int hL = L->herm, hR = R->herm, hres = RES->herm;
+  if (mask & (T110 | T220)) launch_e0(s, D, L->d, R->d, RES->d, hL, hR, mask, c->partials);
+  double *Wa = c->wpphh, *Wb = c->wpphh + c->lay.vsize;
+  if (mask & (T222PPHH | T221)) {
+    launch_gemm_f64(s, false, c->pp_probs, c->pp_tiles, c->n_pp_tiles, L->d + D.v_off, R->d + D.v_off,
+                    c->wpphh);
+  }
+  if (mask & T222PH) {
+    if (!lcc_ready) launch_pandya(s, D, L->d, c->cc[0]);
+    launch_pandya(s, D, R->d, c->cc[1]);
+    launch_gemm_f64(s, true, c->ph_probs, c->ph_tiles, c->n_ph_tiles, c->cc[0], c->cc[1], c->wph);
+  }
+  if (mask & T122) launch_122(s, D, L->d, R->d, RES->d);
+  else launch_clear_upper(s, D, RES->d);
+  if (mask & T222PPHH) launch_pphh_combine(s, D, Wa, Wb, RES->d, (double)(hL * hR));
+  if (mask & T222PH) launch_ph_gather(s, D, c->wph, RES->d, (double)hL, (double)hR);
+  launch_mirror(s, D, RES->d, hres);
+  launch_onebody(s, D, L->d, R->d, Wa, Wb, RES->d, hL, hR, hres, mask);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int imsrg_commutator(imsrg_ctx c, imsrg_op L, imsrg_op R, imsrg_op RES, unsigned mask) {
+  if (!c || !L || !R || !RES) return fail(IMSRG_EINVAL, "null argument");
+  if (L->ctx != c || R->ctx != c || RES->ctx != c) return fail(IMSRG_EINVAL, "operands of another context");
+  if (RES == L || RES == R) return fail(IMSRG_EINVAL, "RES must not alias L or R");
+  if (RES->herm != -L->herm * R->herm) return fail(IMSRG_EINVAL, "RES herm must be -herm(L)*herm(R)");
+  return comm_impl(c, L, R, RES, mask & IMSRG_TALL, false);
+}
+
+static int scalar_term(imsrg_op L, imsrg_op R, unsigned mask, double *out) {
+  if (!L || !R || !out || L->ctx != R->ctx) return fail(IMSRG_EINVAL, "bad operands");
+  imsrg_ctx_s *c = L->ctx;
+  launch_e0(c->stream, c->dl, L->d, R->d, c->scal + 3, L->herm, R->herm, mask, c->partials);
+  return read_scalar(c, 3, out);
+}
+int imsrg_commutator_110(imsrg_op L, imsrg_op R, double *out) { return scalar_term(L, R, T110, out); }
+int imsrg_commutator_220(imsrg_op L, imsrg_op R, double *out) { return scalar_term(L, R, T220, out); }
+
+int64_t imsrg_cross_coupled_size(imsrg_ctx c) {
+  if (!c) return -1;
+  int64_t n = 0;
+  for (int q = 0; q < c->lay.ncc; q++) n += (int64_t)c->lay.ccrlen[q] * c->lay.ccnph[q];
+  return n;
+}
+int imsrg_cross_coupled(imsrg_op op, double *host_out, int64_t n) {
+  if (!op || !host_out) return fail(IMSRG_EINVAL, "null argument");
+  imsrg_ctx_s *c = op->ctx;
+  if (n != imsrg_cross_coupled_size(c)) return fail(IMSRG_EINVAL, "size mismatch");
+  launch_pandya(c->stream, c->dl, op->d, c->cc[0]);
+  int64_t k = 0;
+  for (int q = 0; q < c->lay.ncc; q++) {
+    int64_t m = (int64_t)c->lay.ccrlen[q] * c->lay.ccnph[q];
+    if (m == 0) continue;
+    HIPCHK(hipMemcpyAsync(host_out + k, c->cc[0] + c->lay.ccoff[q], m * sizeof(double),
+                          hipMemcpyDeviceToHost, c->stream));
+    k += m;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// ---------------------------------------------------------------- generators
+static void gen_simple(imsrg_ctx_s *c, imsrg_op_s *Hden, imsrg_op_s *Hnum, imsrg_op_s *ETA, int kind) {
+  hipStream_t s = c->stream;
+  launch_fill(s, ETA->d + c->lay.f_off, c->lay.op_size - c->lay.f_off, 0.0);
+  launch_monopole(s, c->dl, Hden->d, c->mono);
+  launch_eta(s, c->dl, Hden->d, Hnum->d, c->mono, ETA->d, kind);
+  ETA->herm = -1;
+}
+static int gen_impl(imsrg_ctx_s *c, int kind, imsrg_op_s *H, imsrg_op_s *ETA) {
+  if (kind == IMSRG_GEN_WHITE || kind == IMSRG_GEN_ATAN) {
+    gen_simple(c, H, H, ETA, kind == IMSRG_GEN_WHITE ? 0 : 1);
+  } else if (kind == IMSRG_GEN_W2) {
+    // build_gs_w2 (generators.f90:109-160)
+    imsrg_op_s *E1 = tmp_op(c, 2, -1), *H2 = tmp_op(c, 3, 1), *E2 = tmp_op(c, 4, -1);
+    if (!E1 || !H2 || !E2) return IMSRG_ENOMEM;
+    gen_simple(c, H, H, E1, 1);
+    int rc = comm_impl(c, E1, H, H2, IMSRG_TALL, false);
+    if (rc) return rc;
+    gen_simple(c, H, H2, E2, 1);
+    launch_axpby(c->stream, c->lay.op_size, 1.5, E1->d, 0.5, E2->d, ETA->d);
+    ETA->herm = -1;
+  } else {
+    return fail(IMSRG_EINVAL, "unknown generator kind");
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+int imsrg_generator(imsrg_ctx c, int kind, imsrg_op H, imsrg_op ETA) {
+  if (!c || !H || !ETA || H == ETA) return fail(IMSRG_EINVAL, "bad operands");
+  return gen_impl(c, kind, H, ETA);
+}
+int imsrg_generator_atanxx(imsrg_ctx c, imsrg_op Hold, imsrg_op H, imsrg_op ETA) {
+  if (!c || !Hold || !H || !ETA || ETA == H || ETA == Hold) return fail(IMSRG_EINVAL, "bad operands");
+  gen_simple(c, Hold, H, ETA, 1);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------- Magnus / BCH
+static const double kConv = (double)1e-6f;   // real(8), parameter :: conv = 1e-6
+static const double kTiny = (double)1e-9f;   // fullnorm < 1e-9 (default-real literal)
+
+// BCH_EXPAND (IMSRG_MAGNUS.f90:149-234)
+static int bch_impl(imsrg_ctx_s *c, imsrg_op_s *HS, imsrg_op_s *G, imsrg_op_s *H, int *nterms) {
+  imsrg_op_s *AD = tmp_op(c, 0, 1), *INT2 = tmp_op(c, 1, 1);
+  if (!AD || !INT2) return IMSRG_ENOMEM;
+  op_copy(H, HS);
+  op_copy(H, AD);  // AD_1 = H
+  AD->herm = 1;
+  INT2->herm = 1;
+  launch_pandya(c->stream, c->dl, G->d, c->cc[0]);
+  double coef = 1.0;
+  int n = 0, iw;
+  for (iw = 2; iw <= 40; iw++) {
+    coef = coef / (iw - 1.0);
+    int rc = comm_impl(c, G, AD, INT2, IMSRG_TALL, true);
+    if (rc) return rc;
+    n++;
+    launch_axpby(c->stream, c->lay.op_size, 1.0, HS->d, coef, INT2->d, HS->d);
+    double nrm;
+    rc = op_norm(INT2, &nrm);
+    if (rc) return rc;
+    if (nrm * coef < kConv) break;
+    std::swap(c->tmp[0], c->tmp[1]);
+    AD = c->tmp[0];
+    INT2 = c->tmp[1];
+  }
+  if (nterms) *nterms = n;
+  if (iw > 40) return fail(IMSRG_ECLIP, "clipping BCH");
+  return 0;
+}
+int imsrg_bch_expand(imsrg_ctx c, imsrg_op HS, imsrg_op G, imsrg_op H, int *nterms) {
+  if (!c || !HS || !G || !H || HS == G || HS == H) return fail(IMSRG_EINVAL, "bad operands");
+  return bch_impl(c, HS, G, H, nterms);
+}
+
+// MAGNUS_EXPAND (IMSRG_MAGNUS.f90:622-693)
+static int magexp_impl(imsrg_ctx_s *c, imsrg_op_s *DG, imsrg_op_s *G, int *nterms) {
+  static const double cof[7] = {1.0, -0.5, .0833333333, 0.0, -0.00138888888, 0.0, 3.306878e-5};
+  double advals[7] = {0, 0, 0, 0, 0, 0, 0};
+  imsrg_op_s *cur = tmp_op(c, 0, DG->herm), *oth = tmp_op(c, 1, DG->herm);
+  if (!cur || !oth) return IMSRG_ENOMEM;
+  op_copy(DG, cur);
+  op_copy(DG, oth);
+  int rc = op_norm(cur, &advals[0]);
+  if (rc) return rc;
+  double fullnorm;
+  rc = op_norm(G, &fullnorm);
+  if (rc) return rc;
+  int n = 0;
+  if (nterms) *nterms = 0;
+  if (fullnorm < kTiny) return 0;
+  launch_pandya(c->stream, c->dl, G->d, c->cc[0]);
+  int q = 1;
+  for (int i = 2; i <= 7; i++) {
+    double adnorm = advals[i - q - 1];
+    q = (std::fabs(cof[i - 1]) > kConv) ? 1 : 2;
+    if (std::fabs(adnorm / fullnorm) < kConv) break;
+    rc = comm_impl(c, G, cur, oth, IMSRG_TALL & ~(T110 | T220), true);
+    if (rc) return rc;
+    n++;
+    launch_axpby(c->stream, c->lay.op_size, 1.0, DG->d, cof[i - 1], oth->d, DG->d);
+    double nrm;
+    rc = op_norm(oth, &nrm);
+    if (rc) return rc;
+    advals[i - 1] = nrm * std::fabs(cof[i - 1]);
+    std::swap(cur, oth);
+  }
+  if (nterms) *nterms = n;
+  return 0;
+}
+int imsrg_magnus_expand(imsrg_ctx c, imsrg_op DG, imsrg_op G, int *nterms) {
+  if (!c || !DG || !G || DG == G) return fail(IMSRG_EINVAL, "bad operands");
+  return magexp_impl(c, DG, G, nterms);
+}
+
+// magnus_decouple (IMSRG_MAGNUS.f90:15-128) without checkpointing
+int imsrg_magnus_flow(imsrg_ctx c, imsrg_op HS, imsrg_op G, int gen_kind, double ds, int max_steps,
+                      double *s_out, double *e0_out, double *e0mbpt2_out, double *crit_out,
+                      int *nmag_out, int *nbch_out, int *steps_out) {
+  if (!c || !HS || !G || HS == G) return fail(IMSRG_EINVAL, "bad operands");
+  imsrg_op_s *H0 = nullptr, *DG = nullptr;
+  int rc = op_new(c, 1, &H0);
+  if (!rc) rc = op_new(c, -1, &DG);
+  if (rc) {
+    if (H0) imsrg_op_destroy(H0);
+    return rc;
+  }
+  launch_fill(c->stream, G->d, c->lay.op_size, 0.0);
+  G->herm = -1;
+  DG->herm = -1;
+  rc = gen_impl(c, gen_kind, HS, DG);
+  op_copy(HS, H0);
+  double s = 0.0, E_mbpt2 = 0.0;
+  if (!rc) rc = imsrg_mbpt2(HS, &E_mbpt2);
+  double crit = std::fabs(E_mbpt2);
+  int steps = 0;
+  while (!rc && crit > kConv && steps < max_steps) {
+    steps++;
+    int nm = 0, nb = 0;
+    rc = magexp_impl(c, DG, G, &nm);
+    if (rc) break;
+    launch_axpby(c->stream, c->lay.op_size, 1.0, G->d, ds, DG->d, G->d);  // euler_step :756-766
+    s = s + ds;
+    rc = bch_impl(c, HS, G, H0, &nb);
+    if (rc) break;
+    rc = gen_impl(c, gen_kind, HS, DG);
+    if (rc) break;
+    rc = imsrg_mbpt2(HS, &E_mbpt2);
+    if (rc) break;
+    crit = std::fabs(E_mbpt2);
+    double e0 = 0.0;
+    rc = imsrg_op_get_e0(HS, &e0);
+    int k = steps - 1;
+    if (s_out) s_out[k] = s;
+    if (e0_out) e0_out[k] = e0;
+    if (e0mbpt2_out) e0mbpt2_out[k] = e0 + E_mbpt2;
+    if (crit_out) crit_out[k] = crit;
+    if (nmag_out) nmag_out[k] = nm;
+    if (nbch_out) nbch_out[k] = nb;
+  }
+  if (steps_out) *steps_out = steps;
+  imsrg_op_destroy(H0);
+  imsrg_op_destroy(DG);
+  return rc;
+}

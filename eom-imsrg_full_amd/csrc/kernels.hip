@@ -1,0 +1,684 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the IMSRG(2) commutator engine.
+//
+// Operators are device buffers [E0 | F (nsp x nsp) | V (full channel blocks)]
+// described in layout.hpp.  Every two-body kernel walks the upper triangle of
+// each channel block with one wavefront per row (row = one pair); the lower
+// triangle is written only by k_mirror as herm * upper, which reproduces the
+// reference's "upper-triangle then mirror" contract for square sub-blocks and
+// its implied herm * transpose for the non-square ones.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "device.hpp"
+
+namespace imsrg {
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+#define HIP_KERNEL_CHECK() ((void)0)
+
+__device__ __forceinline__ int sgnp(int e) { return (e & 1) ? -1 : 1; }  // (-1)^e
+__device__ __forceinline__ bool dtri(int a, int b, int J) { return J <= a + b && J >= abs(a - b); }
+__device__ __forceinline__ int dpos(const DevLayout &L, int J, int a, int b) {
+  return L.pidx[(size_t)(J >> 1) * L.nsp * L.nsp + (size_t)a * L.nsp + b];
+}
+__device__ __forceinline__ double dsixj(const DevLayout &L, int j1, int j2, int j3, int j4, int j5,
+                                        int j6) {
+  // table holds 0 for any failed triangle, matching sixj (basic_IMSRG.f90:2509-2555)
+  const int h = L.nhalf, nJ = L.nJ;
+  size_t idx = (((((size_t)(j1 >> 1) * h + (j2 >> 1)) * h + (j4 >> 1)) * h + (j5 >> 1)) * nJ +
+                (j3 >> 1)) * nJ + (j6 >> 1);
+  return L.sixj[idx];
+}
+
+// v_elem (basic_IMSRG.f90:1270-1385) on the full layout; V = operator + v_off
+__device__ double dv(const DevLayout &L, const double *V, int a, int b, int c, int d, int J) {
+  const int ja = L.jj[a], jb = L.jj[b], jc = L.jj[c], jd = L.jj[d];
+  if (!(dtri(ja, jb, J) && dtri(jc, jd, J))) return 0.0;
+  const int P = (L.l[a] + L.l[b]) & 1;
+  if (((L.l[c] + L.l[d]) & 1) != P) return 0.0;
+  const int T2 = L.tz[a] + L.tz[b];
+  if (L.tz[c] + L.tz[d] != T2) return 0.0;
+  const int q = 3 * J + 2 * (T2 / 2 + 1) + P;
+  double pre = 1.0;
+  int r, s;
+  if (a > b) {
+    r = dpos(L, J, b, a);
+    pre = sgnp(1 + ((ja + jb - J) >> 1));
+  } else {
+    if (a == b) pre = pre * M_SQRT2;
+    r = dpos(L, J, a, b);
+  }
+  if (c > d) {
+    s = dpos(L, J, d, c);
+    pre = pre * sgnp(1 + ((jc + jd - J) >> 1));
+  } else {
+    if (c == d) pre = pre * M_SQRT2;
+    s = dpos(L, J, c, d);
+  }
+  if ((r | s) < 0) return 0.0;  // unreachable for valid tables; never read out of bounds
+  return V[L.boff[q] + (long long)r * L.bdim[q] + s] * pre;
+}
+
+// ------------------------------------------------------------------ elementwise
+__global__ void k_fill(double *p, long long n, double v) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long st = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += st) p[i] = v;
+}
+void launch_fill(hipStream_t s, double *p, long long n, double v) {
+  if (n <= 0) return;
+  long long nb = (n + 255) / 256;
+  int grid = (int)(nb < 8192 ? nb : 8192);
+  hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, s, p, n, v);
+}
+
+// add_sq_op (basic_IMSRG.f90:3450-3478): C = A*ax + B*bx elementwise
+__global__ void k_axpby(long long n, double a, const double *A, double b, const double *B, double *C) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long st = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += st) C[i] = A[i] * a + B[i] * b;
+}
+void launch_axpby(hipStream_t s, long long n, double a, const double *A, double b, const double *B,
+                  double *C) {
+  if (n <= 0) return;
+  long long nb = (n + 255) / 256;
+  int grid = (int)(nb < 8192 ? nb : 8192);
+  hipLaunchKernelGGL(k_axpby, dim3(grid), dim3(256), 0, s, n, a, A, b, B, C);
+}
+
+// ------------------------------------------------------------------ grouped FP64 GEMM
+// C[M x N] = A[M x K] * B[K x N], one 64x64 output tile per workgroup, tiles of
+// all problems in one launch (tile list sorted heaviest first on the host).
+// A is row-major (lda) or, with A_KMAJOR, stored K x M row-major (lda).
+// B is K x N row-major (ldb).  4 wavefronts, each a 32x32 quadrant built from
+// 2x2 v_mfma_f64_16x16x4 tiles; K staged through LDS 16 at a time.
+constexpr int GBM = 64, GBN = 64, GBK = 16, GPAD = 1;
+
+template <bool A_KMAJOR>
+__global__ __launch_bounds__(256) void k_gemm_f64(const GemmProb *__restrict__ probs,
+                                                  const int4 *__restrict__ tiles, int ntiles,
+                                                  const double *__restrict__ A,
+                                                  const double *__restrict__ B,
+                                                  double *__restrict__ C) {
+  const int t = blockIdx.x;
+  if (t >= ntiles) return;
+  const int4 tl = tiles[t];
+  const GemmProb p = probs[tl.x];
+  const int m0 = tl.y * GBM, n0 = tl.z * GBN;
+  __shared__ double As[GBK][GBM + GPAD];
+  __shared__ double Bs[GBK][GBN + GPAD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  double4_t acc[2][2];
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  const double *Ap = A + p.a_off;
+  const double *Bp = B + p.b_off;
+  for (int k0 = 0; k0 < p.K; k0 += GBK) {
+    if (A_KMAJOR) {
+      const int kk = tid >> 4, mm = (tid & 15) * 4;
+      const int k = k0 + kk;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int m = m0 + mm + j;
+        As[kk][mm + j] = (m < p.M && k < p.K) ? Ap[(long long)k * p.lda + m] : 0.0;
+      }
+    } else {
+      const int mm = tid >> 2, kk = (tid & 3) * 4;
+      const int m = m0 + mm;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int k = k0 + kk + j;
+        As[kk + j][mm] = (m < p.M && k < p.K) ? Ap[(long long)m * p.lda + k] : 0.0;
+      }
+    }
+    {
+      const int kk = tid >> 4, nn = (tid & 15) * 4;
+      const int k = k0 + kk;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int n = n0 + nn + j;
+        Bs[kk][nn + j] = (n < p.N && k < p.K) ? Bp[(long long)k * p.ldb + n] : 0.0;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kq = 0; kq < GBK / 4; kq++) {
+      const int k = kq * 4 + (lane >> 4);
+      const double a0 = As[k][wm * 32 + (lane & 15)];
+      const double a1 = As[k][wm * 32 + 16 + (lane & 15)];
+      const double b0 = Bs[k][wn * 32 + (lane & 15)];
+      const double b1 = Bs[k][wn * 32 + 16 + (lane & 15)];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  double *Cp = C + p.c_off;
+#pragma unroll
+  for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+    for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int row = m0 + wm * 32 + ti * 16 + (lane >> 4) + 4 * v;  // D[i][j]: i = lane/16 + 4v
+        const int col = n0 + wn * 32 + tj * 16 + (lane & 15);
+        if (row < p.M && col < p.N) Cp[(long long)row * p.ldc + col] = acc[ti][tj][v];
+      }
+}
+
+void launch_gemm_f64(hipStream_t s, bool a_kmajor, const GemmProb *probs, const int4 *tiles,
+                     int ntiles, const double *A, const double *B, double *C) {
+  if (ntiles <= 0) return;
+  if (a_kmajor)
+    hipLaunchKernelGGL(k_gemm_f64<true>, dim3(ntiles), dim3(256), 0, s, probs, tiles, ntiles, A, B, C);
+  else
+    hipLaunchKernelGGL(k_gemm_f64<false>, dim3(ntiles), dim3(256), 0, s, probs, tiles, ntiles, A, B, C);
+}
+
+// ------------------------------------------------------------------ Pandya transform
+// calculate_cross_coupled (cross_coupled.f90:1190-1300): one workgroup per
+// Pandya column (p,h), threads over rows (a,b); output column-major per block.
+__global__ __launch_bounds__(256) void k_pandya(DevLayout L, const double *__restrict__ op,
+                                                double *__restrict__ cc) {
+  const int col = blockIdx.x;
+  if (col >= L.nccol) return;
+  const int q1 = L.cccol_q[col];
+  const int c = col - L.cccbase[q1];
+  const int p = L.cccol_p[col], h = L.cccol_h[col];
+  const int JC = L.ccJ[q1], rlen = L.ccrlen[q1];
+  const int jp = L.jj[p], jh = L.jj[h], lp = L.l[p], lh = L.l[h], tp = L.tz[p], th = L.tz[h];
+  const double *V = op + L.v_off;
+  double *out = cc + L.ccoff[q1] + (long long)c * rlen;
+  const double sq = sqrt(JC + 1.0);
+  for (int r = threadIdx.x; r < rlen; r += blockDim.x) {
+    const int a = L.ccrow_a[L.ccrbase[q1] + r], b = L.ccrow_b[L.ccrbase[q1] + r];
+    const int ja = L.jj[a], jb = L.jj[b];
+    double val = 0.0;
+    if ((((L.l[a] + lh) & 1) == ((L.l[b] + lp) & 1)) && (L.tz[a] + th == L.tz[b] + tp)) {
+      const int Jmin = max(abs(jp - jb), abs(ja - jh));
+      const int Jmax = min(jp + jb, ja + jh);
+      double sm = 0.0;
+      for (int JT = Jmin; JT <= Jmax; JT += 2)
+        sm = sm + (double)(sgnp(JT >> 1) * (JT + 1)) * dsixj(L, jp, jh, JC, ja, jb, JT) *
+                      dv(L, V, h, a, p, b, JT);
+      val = sm * sgnp((jh + jb + JC) >> 1) * sq;
+    }
+    out[r] = val;
+  }
+}
+void launch_pandya(hipStream_t s, const DevLayout &L, const double *op, double *cc) {
+  if (L.nccol <= 0) return;
+  hipLaunchKernelGGL(k_pandya, dim3(L.nccol), dim3(256), 0, s, L, op, cc);
+}
+
+// ------------------------------------------------------------------ 122
+// commutator_122 (commutators.f90:283-396), overwriting every upper element:
+// RES(r,s) = sum_{e in nb(r)} c_e [fL(x,i) R(r_e,s) - fR(x,i) L(r_e,s)]
+//          - sum_{e in nb(s)} c_e [fL(i,x) R(r,s_e) - fR(i,x) L(r,s_e)]
+__global__ __launch_bounds__(64) void k_122(DevLayout L, const double *__restrict__ Lop,
+                                            const double *__restrict__ Rop, double *__restrict__ RES) {
+  const int r = blockIdx.x;
+  if (r >= L.npairs) return;
+  const int q = L.pair_q[r], base = L.pbase[q], rl = r - base, dim = L.bdim[q], N = L.nsp;
+  const long long bo = L.v_off + L.boff[q];
+  const double *LV = Lop + bo, *RV = Rop + bo;
+  double *OV = RES + bo;
+  const double *FL = Lop + L.f_off, *FR = Rop + L.f_off;
+  const int e0 = L.nb_start[r], e1 = L.nb_start[r + 1];
+  for (int s = rl + (int)threadIdx.x; s < dim; s += 64) {
+    double val = 0.0;
+    for (int e = e0; e < e1; e++) {
+      const int pe = L.nb_pos[e], x = L.nb_x[e], i = L.nb_i[e];
+      const double ce = L.nb_coef[e];
+      val += ce * (FL[x * N + i] * RV[(long long)pe * dim + s] - FR[x * N + i] * LV[(long long)pe * dim + s]);
+    }
+    const int gs = base + s;
+    const int f0 = L.nb_start[gs], f1 = L.nb_start[gs + 1];
+    for (int e = f0; e < f1; e++) {
+      const int pe = L.nb_pos[e], x = L.nb_x[e], i = L.nb_i[e];
+      const double ce = L.nb_coef[e];
+      val -= ce * (FL[i * N + x] * RV[(long long)rl * dim + pe] - FR[i * N + x] * LV[(long long)rl * dim + pe]);
+    }
+    OV[(long long)rl * dim + s] = val;
+  }
+}
+void launch_122(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES) {
+  if (L.npairs <= 0) return;
+  hipLaunchKernelGGL(k_122, dim3(L.npairs), dim3(64), 0, s, L, Lop, Rop, RES);
+}
+
+__global__ __launch_bounds__(64) void k_clear_upper(DevLayout L, double *RES) {
+  const int r = blockIdx.x;
+  if (r >= L.npairs) return;
+  const int q = L.pair_q[r], rl = r - L.pbase[q], dim = L.bdim[q];
+  double *OV = RES + L.v_off + L.boff[q];
+  for (int s = rl + (int)threadIdx.x; s < dim; s += 64) OV[(long long)rl * dim + s] = 0.0;
+}
+void launch_clear_upper(hipStream_t s, const DevLayout &L, double *RES) {
+  if (L.npairs <= 0) return;
+  hipLaunchKernelGGL(k_clear_upper, dim3(L.npairs), dim3(64), 0, s, L, RES);
+}
+
+// ------------------------------------------------------------------ 222_pp_hh combine
+// RES += (W1 - W2) - pm (W1 - W2)^T on the upper triangle (commutators.f90:705-766)
+__global__ __launch_bounds__(64) void k_pphh_combine(DevLayout L, const double *__restrict__ Wa,
+                                                     const double *__restrict__ Wb,
+                                                     double *__restrict__ RES, double pm) {
+  const int r = blockIdx.x;
+  if (r >= L.npairs) return;
+  const int q = L.pair_q[r], rl = r - L.pbase[q], dim = L.bdim[q];
+  const double *A = Wa + L.boff[q], *B = Wb + L.boff[q];
+  double *OV = RES + L.v_off + L.boff[q];
+  for (int s = rl + (int)threadIdx.x; s < dim; s += 64) {
+    const long long rs = (long long)rl * dim + s, sr = (long long)s * dim + rl;
+    OV[rs] = OV[rs] + A[rs] - B[rs] - (A[sr] - B[sr]) * pm;
+  }
+}
+void launch_pphh_combine(hipStream_t s, const DevLayout &L, const double *Wa, const double *Wb,
+                         double *RES, double pm) {
+  if (L.npairs <= 0) return;
+  hipLaunchKernelGGL(k_pphh_combine, dim3(L.npairs), dim3(64), 0, s, L, Wa, Wb, RES, pm);
+}
+
+// ------------------------------------------------------------------ 222_ph recoupling gather
+// commutators.f90:804-923: for every upper element (ij,kl;J) two J' sums of
+// 6j-weighted W entries; W blocks row-major rlen x rlen (W(x,y) at x*rlen+y).
+__global__ __launch_bounds__(64) void k_ph_gather(DevLayout L, const double *__restrict__ W,
+                                                  double *__restrict__ RES, double hL, double hR) {
+  const int r = blockIdx.x;
+  if (r >= L.npairs) return;
+  const int q = L.pair_q[r], base = L.pbase[q], rl = r - base, dim = L.bdim[q];
+  const int N = L.nsp, JTM1 = L.jtot_max + 1, Jtot = L.bJ[q];
+  double *OV = RES + L.v_off + L.boff[q];
+  const int i = L.pair_a[r], j = L.pair_b[r];
+  const double pre = (i == j) ? .70710678118 : 1.0;
+  const int ji = L.jj[i], jj = L.jj[j], li = L.l[i], lj = L.l[j], ti = L.tz[i], tj = L.tz[j];
+  for (int s = rl + (int)threadIdx.x; s < dim; s += 64) {
+    const int k = L.pair_a[base + s], l = L.pair_b[base + s];
+    const double pre2 = (k == l) ? .70710678118 : 1.0;
+    const int jk = L.jj[k], jl = L.jj[l], lk = L.l[k], ll = L.l[l], tk = L.tz[k], tl = L.tz[l];
+    double sm = 0.0;
+    int Tz = abs(ti - tk) / 2;
+    if (abs(tl - tj) != Tz * 2) continue;
+    int PAR = (li + lk) % 2;
+    if ((ll + lj) % 2 != PAR) continue;
+    int jmin = max(abs(jj - jl), abs(ji - jk)), jmax = min(jj + jl, ji + jk);
+    int phase = sgnp((ji + jj + jk + jl) >> 1);
+    for (int JP = jmin; JP <= jmax; JP += 2) {
+      const int qx = (JP >> 1) + Tz * JTM1 + 2 * PAR * JTM1;
+      const int *rowt = L.ccrow + (size_t)qx * N * N;
+      const int rjl = rowt[j * N + l], rik = rowt[i * N + k], gjl = rowt[l * N + j], gik = rowt[k * N + i];
+      if ((rjl | rik | gjl | gik) < 0) continue;
+      const long long rlq = L.ccrlen[qx];
+      const double *Wq = W + L.cwoff[qx];
+      sm = sm - (double)sgnp((jl + jk + Jtot) >> 1) *
+                    ((Wq[gik * rlq + rjl] + phase * Wq[gjl * rlq + rik]) * hL -
+                     (phase * Wq[rik * rlq + gjl] + Wq[rjl * rlq + gik]) * hR) *
+                    dsixj(L, ji, jj, Jtot, jl, jk, JP);
+    }
+    Tz = abs(ti - tl) / 2;
+    if (abs(tk - tj) != Tz * 2) continue;
+    PAR = (li + ll) % 2;
+    if ((lk + lj) % 2 != PAR) continue;
+    jmin = max(abs(ji - jl), abs(jj - jk));
+    jmax = min(ji + jl, jj + jk);
+    phase = sgnp((ji + jj + jk + jl) >> 1);
+    for (int JP = jmin; JP <= jmax; JP += 2) {
+      const int qx = (JP >> 1) + Tz * JTM1 + 2 * PAR * JTM1;
+      const int *rowt = L.ccrow + (size_t)qx * N * N;
+      const int ril = rowt[i * N + l], rjk = rowt[j * N + k], gil = rowt[l * N + i], gjk = rowt[k * N + j];
+      if ((ril | rjk | gil | gjk) < 0) continue;
+      const long long rlq = L.ccrlen[qx];
+      const double *Wq = W + L.cwoff[qx];
+      sm = sm + ((Wq[gil * rlq + rjk] + phase * Wq[gjk * rlq + ril]) * hL -
+                 (phase * Wq[ril * rlq + gjk] + Wq[rjk * rlq + gil]) * hR) *
+                    dsixj(L, ji, jj, Jtot, jk, jl, JP);
+    }
+    const long long rs = (long long)rl * dim + s;
+    OV[rs] = OV[rs] + sm * pre * pre2;
+  }
+}
+void launch_ph_gather(hipStream_t s, const DevLayout &L, const double *W, double *RES, double hL,
+                      double hR) {
+  if (L.npairs <= 0) return;
+  hipLaunchKernelGGL(k_ph_gather, dim3(L.npairs), dim3(64), 0, s, L, W, RES, hL, hR);
+}
+
+// ------------------------------------------------------------------ mirror
+__global__ __launch_bounds__(64) void k_mirror(DevLayout L, double *RES, int herm) {
+  const int r = blockIdx.x;
+  if (r >= L.npairs) return;
+  const int q = L.pair_q[r], rl = r - L.pbase[q], dim = L.bdim[q];
+  double *OV = RES + L.v_off + L.boff[q];
+  for (int s = rl + (int)threadIdx.x; s < dim; s += 64) {
+    if (s == rl) {
+      if (herm == -1) OV[(long long)rl * dim + rl] = 0.0;
+    } else {
+      OV[(long long)s * dim + rl] = herm * OV[(long long)rl * dim + s];
+    }
+  }
+}
+void launch_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm) {
+  if (L.npairs <= 0) return;
+  hipLaunchKernelGGL(k_mirror, dim3(L.npairs), dim3(64), 0, s, L, RES, herm);
+}
+
+// ------------------------------------------------------------------ one-body (111 + 121 + 221)
+__device__ __forceinline__ bool same_ljt(const DevLayout &L, int a, int b) {
+  return L.jj[a] == L.jj[b] && L.l[a] == L.l[b] && L.tz[a] == L.tz[b];
+}
+// element of a full-layout W buffer at pair (c,x),(c,y) in block J with v_elem's
+// ordering phase and identical-orbit sqrt2 applied to both pairs
+__device__ __forceinline__ double w_pair(const DevLayout &L, const double *W, int c, int x, int y,
+                                         int J, double pm) {
+  const int jc = L.jj[c];
+  const int P = (L.l[c] + L.l[x]) & 1, T2 = L.tz[c] + L.tz[x];
+  const int q = 3 * J + 2 * (T2 / 2 + 1) + P;
+  double pre = 1.0;
+  int r, s;
+  if (c > x) {
+    r = dpos(L, J, x, c);
+    pre = sgnp(1 + ((jc + L.jj[x] - J) >> 1));
+  } else {
+    if (c == x) pre = pre * M_SQRT2;
+    r = dpos(L, J, c, x);
+  }
+  if (c > y) {
+    s = dpos(L, J, y, c);
+    pre = pre * sgnp(1 + ((jc + L.jj[y] - J) >> 1));
+  } else {
+    if (c == y) pre = pre * M_SQRT2;
+    s = dpos(L, J, c, y);
+  }
+  if ((r | s) < 0) return 0.0;
+  const double *Wq = W + L.boff[q];
+  const int dim = L.bdim[q];
+  return Wq[(long long)r * dim + s] * pre - pm * (Wq[(long long)s * dim + r] * pre);
+}
+
+__global__ __launch_bounds__(64) void k_onebody(DevLayout L, const double *__restrict__ Lop,
+                                                const double *__restrict__ Rop,
+                                                const double *__restrict__ Wa,
+                                                const double *__restrict__ Wb,
+                                                double *__restrict__ RES, int hL, int hR, int hres,
+                                                unsigned mask) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L.nob) return;
+  const int p = L.ob_p[t], qo = L.ob_q[t], N = L.nsp;
+  const double *FL = Lop + L.f_off, *FR = Rop + L.f_off;
+  const double *VL = Lop + L.v_off, *VR = Rop + L.v_off;
+  double val = 0.0;
+  if (mask & T111) {
+    double s = 0.0;
+    for (int k = 0; k < N; k++) s += FL[p * N + k] * FR[k * N + qo] - FR[p * N + k] * FL[k * N + qo];
+    val += s;
+  }
+  const bool same = same_ljt(L, p, qo);
+  if (same && (mask & T121)) {
+    const int jq = L.jj[qo];
+    double sm = 0.0;
+    for (int e = 0; e < L.nphp; e++) {
+      const int a = L.php_a[e], i = L.php_i[e], ji = L.jj[i];
+      double smx = 0.0, smy = 0.0, smx2 = 0.0, smy2 = 0.0;
+      for (int JT = abs(ji - jq); JT <= ji + jq; JT += 2) {
+        smx = smx + dv(L, VR, a, p, i, qo, JT) * (JT + 1);
+        smx2 = smx2 + dv(L, VL, a, p, i, qo, JT) * (JT + 1);
+        smy = smy + dv(L, VR, i, p, a, qo, JT) * (JT + 1);
+        smy2 = smy2 + dv(L, VL, i, p, a, qo, JT) * (JT + 1);
+      }
+      sm = sm + FL[a * N + i] * (hL * smx - smy) - FR[a * N + i] * (hR * smx2 - smy2);
+    }
+    val += sm / (L.jj[p] + 1.0);
+  }
+  if (same && (mask & T221)) {
+    const double pm = (double)(hL * hR);
+    const int ji = L.jj[p];
+    double sm = 0.0;
+    for (int pass = 0; pass < 2; pass++)
+      for (int c = 0; c < N; c++) {
+        if (L.con[c] != (pass == 0 ? 1 : 0)) continue;  // holes first, then particles
+        const double *W = (pass == 0) ? Wa : Wb;
+        const int jc = L.jj[c];
+        for (int JT = abs(jc - ji); JT <= jc + ji; JT += 2)
+          sm = sm + w_pair(L, W, c, p, qo, JT, pm) * (JT + 1);
+      }
+    val += sm / (ji + 1.0);
+  }
+  double *F = RES + L.f_off;
+  F[p * N + qo] = val;
+  if (p != qo) F[qo * N + p] = hres * val;
+  else if (hres == -1) F[p * N + p] = 0.0;
+}
+void launch_onebody(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop,
+                    const double *Wa, const double *Wb, double *RES, int hL, int hR, int hres,
+                    unsigned mask) {
+  if (L.nob <= 0) return;
+  hipLaunchKernelGGL(k_onebody, dim3((L.nob + 63) / 64), dim3(64), 0, s, L, Lop, Rop, Wa, Wb, RES, hL,
+                     hR, hres, mask);
+}
+
+// ------------------------------------------------------------------ reductions
+__device__ double block_sum256(double v, double *sh) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) sh[tid] += sh[tid + w];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// 220 per block (commutators.f90:399-432) into partials[q]; 110 into partials[nblocks]
+__global__ __launch_bounds__(256) void k_e0_partial(DevLayout L, const double *Lop, const double *Rop,
+                                                     int hL, int hR, double *partials) {
+  __shared__ double sh[256];
+  const int b = blockIdx.x;
+  if (b < L.nblocks) {
+    const int np = L.bnp[b], nb = L.bnb[b], nh = L.bnh[b], dim = L.bdim[b];
+    const double *A = Lop + L.v_off + L.boff[b], *B = Rop + L.v_off + L.boff[b];
+    double s = 0.0;
+    for (int e = threadIdx.x; e < np * nh; e += 256) {
+      const int ix = e / nh, jx = e % nh;
+      const long long o = (long long)ix * dim + np + nb + jx;
+      s = s - A[o] * B[o] * hR;
+    }
+    s = block_sum256(s, sh);
+    if (threadIdx.x == 0) partials[b] = s * (L.bJ[b] + 1);
+  } else {
+    const int N = L.nsp;
+    const double *FL = Lop + L.f_off, *FR = Rop + L.f_off;
+    double s = 0.0;
+    for (int e = threadIdx.x; e < N * N; e += 256) {
+      const int a = e / N, i = e % N;
+      if (L.con[a] != 0 || L.con[i] != 1) continue;
+      const double lf = FL[a * N + i], rf = FR[a * N + i];
+      s = s + (lf * rf * hL - rf * lf * hR) * (L.jj[i] + 1);
+    }
+    s = block_sum256(s, sh);
+    if (threadIdx.x == 0) partials[b] = s;
+  }
+}
+__global__ __launch_bounds__(256) void k_e0_final(int nblocks, const double *partials, double *RES,
+                                                   int use110, int use220) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int q = threadIdx.x; q < nblocks; q += 256) s += partials[q];
+  s = block_sum256(s, sh);
+  if (threadIdx.x == 0) {
+    double e = 0.0;
+    if (use110) e = partials[nblocks];
+    if (use220) e = e + s * 2.0;
+    RES[0] = e;
+  }
+}
+void launch_e0(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES,
+               int hL, int hR, unsigned mask, double *partials) {
+  hipLaunchKernelGGL(k_e0_partial, dim3(L.nblocks + 1), dim3(256), 0, s, L, Lop, Rop, hL, hR, partials);
+  const int use220 = (mask & T220) && (hL * hR == -1);
+  hipLaunchKernelGGL(k_e0_final, dim3(1), dim3(256), 0, s, L.nblocks, partials, RES,
+                     (mask & T110) ? 1 : 0, use220 ? 1 : 0);
+}
+
+// sum of squares of F and V = mat_frob_norm^2 (basic_IMSRG.f90:3921-3952)
+__global__ __launch_bounds__(256) void k_sumsq_partial(const double *x, long long n, double *partials) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    s += x[i] * x[i];
+  s = block_sum256(s, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+__global__ __launch_bounds__(256) void k_final_sum(const double *partials, int n, double *out, int do_sqrt) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += partials[i];
+  s = block_sum256(s, sh);
+  if (threadIdx.x == 0) out[0] = do_sqrt ? sqrt(s) : s;
+}
+void launch_sumsq(hipStream_t s, const DevLayout &L, const double *op, double *partials, double *out) {
+  const long long n = L.op_size - L.f_off;
+  hipLaunchKernelGGL(k_sumsq_partial, dim3(kPartials), dim3(256), 0, s, op + L.f_off, n, partials);
+  hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, s, partials, kPartials, out, 1);
+}
+
+// mbpt2 (basic_IMSRG.f90:4030-4064): one workgroup per block
+__global__ __launch_bounds__(256) void k_mbpt2_partial(DevLayout L, const double *H, double *partials) {
+  __shared__ double sh[256];
+  const int q = blockIdx.x;
+  const int np = L.bnp[q], nb = L.bnb[q], nh = L.bnh[q], dim = L.bdim[q], base = L.pbase[q], N = L.nsp;
+  const double *V = H + L.v_off + L.boff[q];
+  const double *F = H + L.f_off;
+  double s = 0.0;
+  for (int e = threadIdx.x; e < np * nh; e += 256) {
+    const int ii = e / nh, jx = e % nh;
+    const int i = L.pair_a[base + ii], j = L.pair_b[base + ii];
+    const int k = L.pair_a[base + np + nb + jx], l = L.pair_b[base + np + nb + jx];
+    const double eden = F[k * N + k] + F[l * N + l] - F[i * N + i] - F[j * N + j];
+    const double x = V[(long long)ii * dim + np + nb + jx];
+    s = s + x * x / eden;
+  }
+  s = block_sum256(s, sh);
+  if (threadIdx.x == 0) partials[q] = (L.bJ[q] + 1.0) * s;
+}
+void launch_mbpt2(hipStream_t s, const DevLayout &L, const double *op, double *partials, double *out) {
+  hipLaunchKernelGGL(k_mbpt2_partial, dim3(L.nblocks), dim3(256), 0, s, L, op, partials);
+  hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, s, partials, L.nblocks, out, 0);
+}
+
+// ------------------------------------------------------------------ generators
+// twobody_monopole (basic_IMSRG.f90:2031-2046) for every orbit pair
+__global__ void k_monopole(DevLayout L, const double *H, double *mono) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = L.nsp;
+  if (t >= N * N) return;
+  const int a = t / N, b = t % N, ja = L.jj[a], jb = L.jj[b];
+  const double *V = H + L.v_off;
+  double sm = 0.0;
+  for (int JT = abs(ja - jb); JT <= ja + jb; JT += 2) sm = sm + (JT + 1) * dv(L, V, a, b, a, b, JT);
+  mono[t] = sm / (ja + 1.0) / (jb + 1.0);
+}
+void launch_monopole(hipStream_t s, const DevLayout &L, const double *H, double *mono) {
+  const int n = L.nsp * L.nsp;
+  hipLaunchKernelGGL(k_monopole, dim3((n + 255) / 256), dim3(256), 0, s, L, H, mono);
+}
+
+// build_gs_white / build_gs_atan / build_gs_atanxx (generators.f90:4-374):
+// ETA must be zero on entry; kind 0 = H/Eden, 1 = atan(2H/Eden)/2
+__global__ void k_eta_onebody(DevLayout L, const double *Hden, const double *Hnum, double *ETA, int kind) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= L.nphp) return;
+  const int a = L.php_a[e], i = L.php_i[e], N = L.nsp, ji = L.jj[i];
+  const double *V = Hden + L.v_off, *Fd = Hden + L.f_off;
+  double Eden = 0.0;
+  for (int JT = 0; JT <= 2 * ji; JT += 2) Eden = Eden - (JT + 1) * dv(L, V, a, i, a, i, JT);
+  Eden = Eden / (ji + 1.0) / (ji + 1.0);
+  Eden = Eden + Fd[a * N + a] - Fd[i * N + i];
+  const double h = Hnum[L.f_off + a * N + i];
+  const double eta = (kind == 0) ? h / Eden : atan(2 * h / Eden) * 0.5;
+  ETA[L.f_off + a * N + i] = eta;
+  ETA[L.f_off + i * N + a] = -eta;
+}
+__global__ __launch_bounds__(256) void k_eta_twobody(DevLayout L, const double *Hden, const double *Hnum,
+                                                      const double *mono, double *ETA, int kind) {
+  const int q = blockIdx.x;
+  const int np = L.bnp[q], nb = L.bnb[q], nh = L.bnh[q], dim = L.bdim[q], base = L.pbase[q], N = L.nsp;
+  const double *Fd = Hden + L.f_off;
+  const double *Vn = Hnum + L.v_off + L.boff[q];
+  double *Ve = ETA + L.v_off + L.boff[q];
+  for (int e = threadIdx.x; e < np * nh; e += 256) {
+    const int ix = e / nh, jx = e % nh;
+    const int a = L.pair_a[base + ix], b = L.pair_b[base + ix];
+    const int i = L.pair_a[base + np + nb + jx], j = L.pair_b[base + np + nb + jx];
+    double Eden = 0.0;
+    Eden = Eden + mono[a * N + b];
+    Eden = Eden + mono[i * N + j];
+    Eden = Eden - mono[a * N + i];
+    Eden = Eden - mono[a * N + j];
+    Eden = Eden - mono[i * N + b];
+    Eden = Eden - mono[j * N + b];
+    Eden = Eden + Fd[a * N + a] + Fd[b * N + b] - Fd[i * N + i] - Fd[j * N + j];
+    const long long o = (long long)ix * dim + np + nb + jx;
+    const double h = Vn[o];
+    const double eta = (kind == 0) ? h / Eden : 0.5 * atan(2 * h / Eden);
+    Ve[o] = eta;
+    Ve[(long long)(np + nb + jx) * dim + ix] = -eta;
+  }
+}
+void launch_eta(hipStream_t s, const DevLayout &L, const double *Hden, const double *Hnum,
+                const double *mono, double *ETA, int kind) {
+  if (L.nphp > 0)
+    hipLaunchKernelGGL(k_eta_onebody, dim3((L.nphp + 63) / 64), dim3(64), 0, s, L, Hden, Hnum, ETA, kind);
+  hipLaunchKernelGGL(k_eta_twobody, dim3(L.nblocks), dim3(256), 0, s, L, Hden, Hnum, mono, ETA, kind);
+}
+
+// ------------------------------------------------------------------ normal ordering
+// normal_order (basic_IMSRG.f90:2375-2506); T is an unmodified copy of H
+__global__ __launch_bounds__(256) void k_no_e0(DevLayout L, const double *T, double *H) {
+  __shared__ double sh[256];
+  const int N = L.nsp;
+  const double *F = T + L.f_off, *V = T + L.v_off;
+  double s1 = 0.0;
+  for (int i = threadIdx.x; i < N; i += 256) s1 += F[i * N + i] * L.con[i] * (L.jj[i] + 1);
+  s1 = block_sum256(s1, sh);
+  double s2 = 0.0;
+  for (int e = threadIdx.x; e < N * N; e += 256) {
+    const int i = e / N, j = e % N;
+    if (L.con[i] * L.con[j] == 0) continue;
+    for (int JT = abs(L.jj[i] - L.jj[j]); JT <= L.jj[i] + L.jj[j]; JT += 2)
+      s2 = s2 + dv(L, V, i, j, i, j, JT) * (JT + 1);
+  }
+  s2 = block_sum256(s2, sh);
+  if (threadIdx.x == 0) H[0] = (s1 * 2.0 + s2) * 0.5;
+}
+__global__ void k_no_f(DevLayout L, const double *T, double *H, int herm) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L.nob) return;
+  const int a = L.ob_p[t], b = L.ob_q[t], N = L.nsp;
+  if (!same_ljt(L, a, b)) return;
+  const double *V = T + L.v_off;
+  double s2 = 0.0;
+  for (int ii = 0; ii < N; ii++) {
+    if (L.con[ii] != 1) continue;
+    for (int JT = abs(L.jj[ii] - L.jj[a]); JT <= L.jj[ii] + L.jj[a]; JT += 2)
+      s2 = s2 + (JT + 1.0) / (L.jj[a] + 1) * dv(L, V, a, ii, b, ii, JT);
+  }
+  const double val = T[L.f_off + a * N + b] + s2;
+  H[L.f_off + a * N + b] = val;
+  if (a != b) H[L.f_off + b * N + a] = herm * val;
+}
+void launch_normal_order(hipStream_t s, const DevLayout &L, const double *T, double *H, double *partials) {
+  (void)partials;
+  hipLaunchKernelGGL(k_no_e0, dim3(1), dim3(256), 0, s, L, T, H);
+  if (L.nob > 0) hipLaunchKernelGGL(k_no_f, dim3((L.nob + 63) / 64), dim3(64), 0, s, L, T, H, 1);
+}
+
+// split-FP32 GEMM lives in gemm_split.hip
+}  // namespace imsrg

@@ -1,0 +1,323 @@
+// layout.cpp -- builds the channel layout, index tables and 6j table.
+// Orbit/block/Pandya enumeration follows the reference's rules
+// (basic_IMSRG.f90:477-641, cross_coupled.f90:61-177); the storage itself is
+// this library's own (full square channel matrices, dense lookup tables).
+#include "layout.hpp"
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+namespace imsrg {
+
+static inline bool tri(int j1, int j2, int J) { return J <= j1 + j2 && J >= std::abs(j1 - j2); }
+static inline int64_t align32(int64_t x) { return (x + 31) & ~int64_t(31); }
+
+// ---------------------------------------------------------------- 6j
+static long double g_fact[256];
+static bool g_fact_ready = false;
+static void init_fact() {
+  if (g_fact_ready) return;
+  g_fact[0] = 1.0L;
+  for (int i = 1; i < 256; i++) g_fact[i] = g_fact[i - 1] * (long double)i;
+  g_fact_ready = true;
+}
+static bool tri_ok(int a, int b, int c) {
+  return a >= 0 && b >= 0 && c >= 0 && c <= a + b && c >= std::abs(a - b) && ((a + b + c) & 1) == 0;
+}
+static long double tri_delta(int a, int b, int c) {
+  return sqrtl(g_fact[(a + b - c) / 2] * g_fact[(a - b + c) / 2] * g_fact[(-a + b + c) / 2] /
+               g_fact[(a + b + c) / 2 + 1]);
+}
+// Racah's single-sum formula for {j1 j2 j3; j4 j5 j6}, arguments doubled.
+double racah6j(int j1, int j2, int j3, int j4, int j5, int j6) {
+  init_fact();
+  if (!tri_ok(j1, j2, j3) || !tri_ok(j1, j5, j6) || !tri_ok(j4, j2, j6) || !tri_ok(j4, j5, j3))
+    return 0.0;
+  const int a[4] = {(j1 + j2 + j3) / 2, (j1 + j5 + j6) / 2, (j4 + j2 + j6) / 2, (j4 + j5 + j3) / 2};
+  const int b[3] = {(j1 + j2 + j4 + j5) / 2, (j2 + j3 + j5 + j6) / 2, (j3 + j1 + j6 + j4) / 2};
+  int tmin = a[0], tmax = b[0];
+  for (int k = 1; k < 4; k++) tmin = a[k] > tmin ? a[k] : tmin;
+  for (int k = 1; k < 3; k++) tmax = b[k] < tmax ? b[k] : tmax;
+  long double s = 0.0L;
+  for (int t = tmin; t <= tmax; t++) {
+    long double den = g_fact[t - a[0]] * g_fact[t - a[1]] * g_fact[t - a[2]] * g_fact[t - a[3]] *
+                      g_fact[b[0] - t] * g_fact[b[1] - t] * g_fact[b[2] - t];
+    long double term = g_fact[t + 1] / den;
+    s += (t & 1) ? -term : term;
+  }
+  s *= tri_delta(j1, j2, j3) * tri_delta(j1, j5, j6) * tri_delta(j4, j2, j6) * tri_delta(j4, j5, j3);
+  return (double)s;
+}
+
+// ---------------------------------------------------------------- layout
+std::string Layout::build(int nsp_, const int *n_, const int *l_, const int *jj_, const int *tz_,
+                          const int *con_) {
+  if (nsp_ <= 0 || nsp_ > 4096) return "nsp out of range";
+  nsp = nsp_;
+  n.assign(n_, n_ + nsp);
+  l.assign(l_, l_ + nsp);
+  jj.assign(jj_, jj_ + nsp);
+  tz.assign(tz_, tz_ + nsp);
+  con.assign(con_, con_ + nsp);
+  for (int i = 0; i < nsp; i++) {
+    if (jj[i] <= 0 || (jj[i] & 1) == 0) return "2j must be a positive odd integer";
+    if (tz[i] != -1 && tz[i] != 1) return "2tz must be -1 or +1";
+    if (con[i] != 0 && con[i] != 1) return "con must be 0 or 1";
+    if (l[i] < 0 || n[i] < 0) return "n and l must be non-negative";
+  }
+  holes.clear();
+  parts.clear();
+  for (int i = 0; i < nsp; i++) (con[i] ? holes : parts).push_back(i);
+  nhole = (int)holes.size();
+  npart = (int)parts.size();
+  jtot_max = 0;
+  lmax = 0;
+  for (int i = 0; i < nsp; i++) {
+    jtot_max = jj[i] > jtot_max ? jj[i] : jtot_max;
+    lmax = l[i] > lmax ? l[i] : lmax;
+  }
+  if (jtot_max > 63) return "2j too large";
+  // groups
+  grp_of.assign(nsp, -1);
+  grp_start.clear();
+  grp_cnt.clear();
+  grp_list.clear();
+  for (int i = 0; i < nsp; i++) {
+    if (grp_of[i] >= 0) continue;
+    int g = (int)grp_start.size();
+    grp_start.push_back((int)grp_list.size());
+    grp_cnt.push_back(0);
+    for (int k = i; k < nsp; k++)
+      if (l[k] == l[i] && jj[k] == jj[i] && tz[k] == tz[i]) {
+        grp_of[k] = g;
+        grp_list.push_back(k);
+        grp_cnt[g]++;
+      }
+  }
+  // two-body blocks
+  const int JTM = jtot_max;
+  nblocks = (JTM + 1) * 6;
+  bJ.assign(nblocks, 0); bP.assign(nblocks, 0); bTz.assign(nblocks, 0);
+  bnp.assign(nblocks, 0); bnb.assign(nblocks, 0); bnh.assign(nblocks, 0); bdim.assign(nblocks, 0);
+  boff.assign(nblocks, 0);
+  pbase.assign(nblocks + 1, 0);
+  const size_t nn2 = (size_t)nsp * nsp;
+  pidx.assign((size_t)(JTM + 1) * nn2, -1);
+  pair_a.clear(); pair_b.clear(); pair_q.clear();
+  int q = 0;
+  int64_t off = 0;
+  for (int J = 0; J <= 2 * JTM; J += 2)
+    for (int T = -1; T <= 1; T++)
+      for (int P = 0; P <= 1; P++) {
+        bJ[q] = J; bP[q] = P; bTz[q] = T;
+        std::vector<int> lists[3];
+        for (int i = 0; i < nsp; i++)
+          for (int j = i; j < nsp; j++) {
+            if (T != (tz[i] + tz[j]) / 2) continue;
+            if (P != (l[i] + l[j]) % 2) continue;
+            if (!tri(jj[i], jj[j], J)) continue;
+            lists[con[i] + con[j]].push_back(i * nsp + j);
+          }
+        bnp[q] = (int)lists[0].size();
+        bnb[q] = (int)lists[1].size();
+        bnh[q] = (int)lists[2].size();
+        bdim[q] = bnp[q] + bnb[q] + bnh[q];
+        pbase[q] = (int)pair_a.size();
+        int pos = 0;
+        for (int c = 0; c < 3; c++)
+          for (int code : lists[c]) {
+            int a = code / nsp, b = code % nsp;
+            pidx[(size_t)(J / 2) * nn2 + (size_t)a * nsp + b] = pos++;
+            pair_a.push_back(a);
+            pair_b.push_back(b);
+            pair_q.push_back(q);
+          }
+        boff[q] = off;
+        off = align32(off + (int64_t)bdim[q] * bdim[q]);
+        q++;
+      }
+  pbase[nblocks] = (int)pair_a.size();
+  npairs = (int)pair_a.size();
+  vsize = off;
+  // one-body outputs (p,q) with p<=q in hh and pp, every (particle, hole)
+  ob_p.clear(); ob_q.clear();
+  for (int a = 0; a < nhole; a++)
+    for (int b = a; b < nhole; b++) { ob_p.push_back(holes[a]); ob_q.push_back(holes[b]); }
+  for (int a = 0; a < npart; a++)
+    for (int b = a; b < npart; b++) { ob_p.push_back(parts[a]); ob_q.push_back(parts[b]); }
+  for (int a = 0; a < npart; a++)
+    for (int b = 0; b < nhole; b++) { ob_p.push_back(parts[a]); ob_q.push_back(holes[b]); }
+  // 122 neighbour lists
+  nb_start.assign(npairs + 1, 0);
+  nb_pos.clear(); nb_x.clear(); nb_i.clear(); nb_coef.clear();
+  for (int r = 0; r < npairs; r++) {
+    nb_start[r] = (int)nb_pos.size();
+    int a = pair_a[r], b = pair_b[r], J = bJ[pair_q[r]];
+    double inv_old = (a == b) ? 1.0 / std::sqrt(2.0) : 1.0;
+    for (int side = 0; side < 2; side++) {
+      int x = side == 0 ? a : b;
+      int g = grp_of[x];
+      for (int t = 0; t < grp_cnt[g]; t++) {
+        int i = grp_list[grp_start[g] + t];
+        int u = side == 0 ? i : a, v = side == 0 ? b : i;  // new ordered pair (u,v)
+        double pre = 1.0;
+        int lo = u, hi = v;
+        if (u > v) {
+          lo = v; hi = u;
+          pre = (((jj[u] + jj[v] - J) / 2 + 1) % 2 == 0) ? 1.0 : -1.0;
+        } else if (u == v) {
+          pre = std::sqrt(2.0);
+        }
+        int p = pidx[(size_t)(J / 2) * nn2 + (size_t)lo * nsp + hi];
+        if (p < 0) return "internal: 122 neighbour outside block";
+        nb_pos.push_back(p);
+        nb_x.push_back(x);
+        nb_i.push_back(i);
+        nb_coef.push_back(pre * inv_old);
+      }
+    }
+  }
+  nb_start[npairs] = (int)nb_pos.size();
+  // Pandya blocks
+  ncc = (JTM + 1) * 4;
+  ccJ.assign(ncc, 0); ccrlen.assign(ncc, 0); ccnph.assign(ncc, 0); ccTz.assign(ncc, 0); ccP.assign(ncc, 0);
+  ccoff.assign(ncc, 0); cwoff.assign(ncc, 0);
+  ccrow.assign((size_t)ncc * nn2, -1);
+  ccrbase.assign(ncc + 1, 0); cccbase.assign(ncc + 1, 0);
+  ccrow_a.clear(); ccrow_b.clear(); cccol_p.clear(); cccol_h.clear();
+  int64_t coff = 0, woff = 0;
+  for (int q1 = 0; q1 < ncc; q1++) {
+    int JC = (q1 % (JTM + 1)) * 2;
+    int PAR = q1 / (2 * JTM + 2);
+    int TZ = (q1 % (2 * JTM + 2)) / (JTM + 1);
+    ccJ[q1] = JC; ccTz[q1] = TZ; ccP[q1] = PAR;
+    ccrbase[q1] = (int)ccrow_a.size();
+    cccbase[q1] = (int)cccol_p.size();
+    int r = 0, nb = 0;
+    for (int i = 0; i < nsp; i++)
+      for (int j = 0; j < nsp; j++) {
+        if (!tri(jj[i], jj[j], JC)) continue;
+        if ((l[i] + l[j]) % 2 != PAR) continue;
+        if (std::abs(tz[i] - tz[j]) / 2 != TZ) continue;
+        ccrow[((size_t)q1 * nsp + i) * nsp + j] = r++;
+        ccrow_a.push_back(i);
+        ccrow_b.push_back(j);
+        if (con[i] == 0 && con[j] == 1) {
+          nb++;
+          cccol_p.push_back(i);
+          cccol_h.push_back(j);
+        }
+      }
+    ccrlen[q1] = r;
+    ccnph[q1] = nb;
+    ccoff[q1] = coff;
+    cwoff[q1] = woff;
+    coff = align32(coff + (int64_t)r * nb);
+    woff = align32(woff + (int64_t)r * r);
+  }
+  ccrbase[ncc] = (int)ccrow_a.size();
+  cccbase[ncc] = (int)cccol_p.size();
+  ccsize = coff;
+  cwsize = woff;
+  // 6j table
+  nhalf = (JTM + 1) / 2;
+  nJ = JTM + 1;
+  {
+    size_t h = nhalf, J = nJ;
+    sixj.assign(h * h * h * h * J * J, 0.0);
+    for (size_t i1 = 0; i1 < h; i1++)
+      for (size_t i2 = 0; i2 < h; i2++)
+        for (size_t i4 = 0; i4 < h; i4++)
+          for (size_t i5 = 0; i5 < h; i5++)
+            for (size_t a3 = 0; a3 < J; a3++)
+              for (size_t a6 = 0; a6 < J; a6++)
+                sixj[((((i1 * h + i2) * h + i4) * h + i5) * J + a3) * J + a6] =
+                    racah6j((int)(2 * i1 + 1), (int)(2 * i2 + 1), (int)(2 * a3), (int)(2 * i4 + 1),
+                            (int)(2 * i5 + 1), (int)(2 * a6));
+  }
+  // neq: the length of the vectorize stream (IMSRG_ODE.f90:36-44)
+  {
+    int64_t A = nhole, Pp = npart;
+    neq = 1 + A * A + A * Pp + Pp * Pp;
+    for (int qq = 0; qq < nblocks; qq++) {
+      int64_t np = bnp[qq], nb = bnb[qq], nh = bnh[qq];
+      neq += (nh * nh + nh + nb * nb + nb + np * np + np) / 2 + nb * np + nh * np + nh * nb;
+    }
+  }
+  f_off = 32;
+  v_off = align32(f_off + (int64_t)nsp * nsp);
+  op_size = v_off + vsize;
+  flop_pphh = 0;
+  for (int qq = 0; qq < nblocks; qq++) flop_pphh += 2.0 * bdim[qq] * bdim[qq] * (bnp[qq] + bnh[qq]);
+  flop_ph = 0;
+  for (int q1 = 0; q1 < ncc; q1++) flop_ph += 2.0 * ccrlen[q1] * ccrlen[q1] * ccnph[q1];
+  return "";
+}
+
+// sub-block geometry of gam1..gam6 (basic_IMSRG.f90:118-134)
+static const int SEA1[6] = {1, 1, 1, 2, 3, 2};
+static const int SEA2[6] = {1, 2, 3, 2, 3, 3};
+static const bool SQS[6] = {true, false, false, true, true, false};
+
+// repackage (basic_IMSRG.f90:3857-3918) into the full layout
+void Layout::unpack_vec(const double *v, int herm, double *buf) const {
+  std::memset(buf, 0, sizeof(double) * op_size);
+  size_t k = 0;
+  buf[0] = v[k++];
+  double *F = buf + f_off;
+  for (int i = 0; i < nhole; i++)
+    for (int j = 0; j < nhole; j++) F[(size_t)holes[j] * nsp + holes[i]] = v[k++];  // fhh(j,i)
+  for (int i = 0; i < npart; i++)
+    for (int j = 0; j < npart; j++) F[(size_t)parts[j] * nsp + parts[i]] = v[k++];  // fpp(j,i)
+  for (int i = 0; i < nhole; i++)
+    for (int j = 0; j < npart; j++) {                                                 // fph(j,i)
+      double x = v[k++];
+      F[(size_t)parts[j] * nsp + holes[i]] = x;
+      F[(size_t)holes[i] * nsp + parts[j]] = herm * x;
+    }
+  double *V = buf + v_off;
+  for (int q = 0; q < nblocks; q++) {
+    int dim = bdim[q];
+    const int cls_off[4] = {0, bnp[q], bnp[q] + bnb[q], dim};
+    double *M = V + boff[q];
+    for (int g = 0; g < 6; g++) {
+      int r0 = cls_off[SEA1[g] - 1], nr = cls_off[SEA1[g]] - r0;
+      int c0 = cls_off[SEA2[g] - 1], nc = cls_off[SEA2[g]] - c0;
+      for (int i = 0; i < nc; i++)
+        for (int j = SQS[g] ? i : 0; j < nr; j++) {
+          double x = v[k++];
+          M[(size_t)(r0 + j) * dim + (c0 + i)] = x;          // X(j,i)
+          if (SQS[g]) M[(size_t)(r0 + i) * dim + (c0 + j)] = x * herm;   // X(i,j)
+          else M[(size_t)(c0 + i) * dim + (r0 + j)] = x * herm;          // implied lower block
+        }
+    }
+  }
+}
+
+// vectorize (basic_IMSRG.f90:3794-3854) from the full layout
+void Layout::pack_vec(const double *buf, double *v) const {
+  size_t k = 0;
+  v[k++] = buf[0];
+  const double *F = buf + f_off;
+  for (int i = 0; i < nhole; i++)
+    for (int j = 0; j < nhole; j++) v[k++] = F[(size_t)holes[j] * nsp + holes[i]];
+  for (int i = 0; i < npart; i++)
+    for (int j = 0; j < npart; j++) v[k++] = F[(size_t)parts[j] * nsp + parts[i]];
+  for (int i = 0; i < nhole; i++)
+    for (int j = 0; j < npart; j++) v[k++] = F[(size_t)parts[j] * nsp + holes[i]];
+  const double *V = buf + v_off;
+  for (int q = 0; q < nblocks; q++) {
+    int dim = bdim[q];
+    const int cls_off[4] = {0, bnp[q], bnp[q] + bnb[q], dim};
+    const double *M = V + boff[q];
+    for (int g = 0; g < 6; g++) {
+      int r0 = cls_off[SEA1[g] - 1], nr = cls_off[SEA1[g]] - r0;
+      int c0 = cls_off[SEA2[g] - 1], nc = cls_off[SEA2[g]] - c0;
+      for (int i = 0; i < nc; i++)
+        for (int j = SQS[g] ? i : 0; j < nr; j++) v[k++] = M[(size_t)(r0 + j) * dim + (c0 + i)];
+    }
+  }
+}
+
+}  // namespace imsrg

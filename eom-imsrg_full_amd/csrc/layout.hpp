@@ -1,0 +1,91 @@
+// layout.hpp -- host-side description of the device-resident operator layout.
+//
+// An operator lives in one contiguous FP64 device buffer:
+//   [E0 | pad to 32 doubles][F: nsp x nsp row-major one-body matrix][V]
+// V holds, for every (J,P,Tz) channel block q, the FULL dim x dim row-major
+// matrix over that block's pair basis, pairs ordered [pp | ph | hh] exactly as
+// the reference lists them (allocate_blocks, basic_IMSRG.f90:477-641).  The
+// reference keeps only six upper sub-blocks (gam1..gam6) and implies the rest
+// through herm; keeping the full square block makes every 222 contraction a
+// plain GEMM and every norm a flat sum.  Lower triangles are always
+// herm * upper (diagonal zero when herm = -1).
+//
+// Pandya (cross-coupled) matrices and the ph product W live in context
+// workspaces with the reference's block order (cross_coupled.f90:61-177).
+// All angular momenta are doubled; orbits are 0-based.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace imsrg {
+
+struct Layout {
+  // orbits
+  int nsp = 0, nhole = 0, npart = 0;
+  std::vector<int> n, l, jj, tz, con, holes, parts;
+  int jtot_max = 0, lmax = 0;
+  // (l, 2j, 2tz) groups, members in orbit order
+  std::vector<int> grp_of, grp_start, grp_cnt, grp_list;
+
+  // two-body (J,P,Tz) blocks: J slowest, Tz = -1..1, P fastest
+  int nblocks = 0;
+  std::vector<int> bJ, bP, bTz, bnp, bnb, bnh, bdim;
+  std::vector<int64_t> boff;    // element offset of block q inside V
+  int64_t vsize = 0;            // elements of V (sum dim^2, blocks 32-aligned)
+  std::vector<int> pbase;       // first global pair index of block q
+  int npairs = 0;               // sum of dims
+  std::vector<int> pair_a, pair_b, pair_q;  // per global pair (a <= b)
+  // pidx[(J/2)*nsp*nsp + a*nsp + b] (a <= b): position inside block's full matrix
+  std::vector<int> pidx;
+
+  // one-body "upper" outputs: (p,q) with p<=q inside hh/pp, all (particle,hole)
+  std::vector<int> ob_p, ob_q;
+
+  // 122 neighbours: for global pair r=(a,b), entries replacing a or b by an
+  // orbit of the same (l,j,tz) group.  nb_pos is the new pair's position in
+  // the same block; coef = exchange phase * norm(new)/norm(old).
+  std::vector<int> nb_start;    // npairs + 1
+  std::vector<int> nb_pos, nb_x, nb_i;
+  std::vector<double> nb_coef;
+
+  // Pandya blocks: JC fastest, |Tz|, parity slowest
+  int ncc = 0;
+  std::vector<int> ccJ, ccrlen, ccnph, ccTz, ccP;
+  std::vector<int64_t> ccoff;   // element offset of CC block (rlen x nph, column-major)
+  std::vector<int64_t> cwoff;   // element offset of W block (rlen x rlen, row-major)
+  int64_t ccsize = 0, cwsize = 0;
+  std::vector<int> ccrow;       // [(q*nsp + a)*nsp + b] -> row, -1
+  std::vector<int> ccrbase;     // first global row of block q
+  std::vector<int> ccrow_a, ccrow_b;  // per global row
+  std::vector<int> cccbase;     // first global column of block q
+  std::vector<int> cccol_p, cccol_h;  // per global column (particle, hole)
+
+  // dense 6j table {j1 j2 J3; j4 j5 J6}: j1,j2,j4,j5 half-integer, J3,J6 integer
+  int nhalf = 0, nJ = 0;
+  std::vector<double> sixj;
+
+  int64_t neq = 0;              // length of the reference vectorize stream
+  int64_t f_off = 32;           // F offset inside the operator buffer
+  int64_t v_off = 0;            // V offset inside the operator buffer
+  int64_t op_size = 0;          // doubles per operator buffer
+
+  // algorithmic work counts (per commutator)
+  double flop_pphh = 0, flop_ph = 0;
+
+  std::string build(int nsp_, const int *n_, const int *l_, const int *jj_, const int *tz_,
+                    const int *con_);
+
+  // host conversions between the reference vectorize stream and the full layout
+  void unpack_vec(const double *vec, int herm, double *opbuf) const;   // repackage
+  void pack_vec(const double *opbuf, double *vec) const;               // vectorize
+
+  int block_of(int J, int a, int b) const {  // block_index, basic_IMSRG.f90:1064-1070
+    int P = (l[a] + l[b]) % 2, T = (tz[a] + tz[b]) / 2;
+    return 3 * J + 2 * (T + 1) + P;  // J doubled
+  }
+};
+
+double racah6j(int j1, int j2, int j3, int j4, int j5, int j6);  // doubled arguments
+
+}  // namespace imsrg

@@ -1,0 +1,285 @@
+"""ctypes binding of libimsrg_mi355x.so (the C ABI in include/imsrg_mi355x.h).
+
+The shared library is loaded from this package's ``lib/`` directory, where
+``__graft_entry__.build()`` (or ``make -C eom-imsrg_full_amd/csrc``) puts it.
+There is no CPU fallback: if the library is missing this module raises on
+import, and every call that fails on the device raises ``ImsrgError``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from .basis import Basis
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libimsrg_mi355x.so")
+HEADER = os.path.abspath(os.path.join(HERE, "..", "..", "include", "imsrg_mi355x.h"))
+
+T110, T111, T121, T122, T220, T221, T222PPHH, T222PH = (1 << k for k in range(8))
+TALL = 0xFF
+PREC_FP64, PREC_SPLIT_FP32 = 0, 1
+GEN = {"white": 0, "atan": 1, "w2": 2}
+
+
+class ImsrgError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `make -C eom-imsrg_full_amd/csrc` "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    return C.CDLL(LIB_PATH)
+
+
+_lib = _load()
+_vp, _ip, _dp, _i64 = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int64
+_SIG = {
+    "imsrg_last_error": (C.c_char_p, []),
+    "imsrg_ctx_create": (C.c_int, [C.c_int, _ip, _ip, _ip, _ip, _ip, C.c_int, C.POINTER(_vp)]),
+    "imsrg_ctx_destroy": (C.c_int, [_vp]),
+    "imsrg_ctx_neq": (_i64, [_vp]),
+    "imsrg_ctx_info": (C.c_int, [_vp, _dp]),
+    "imsrg_ctx_set_precision": (C.c_int, [_vp, C.c_int]),
+    "imsrg_ctx_synchronize": (C.c_int, [_vp]),
+    "imsrg_ctx_stream": (_vp, [_vp]),
+    "imsrg_op_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
+    "imsrg_op_destroy": (C.c_int, [_vp]),
+    "imsrg_op_set_herm": (C.c_int, [_vp, C.c_int]),
+    "imsrg_op_get_herm": (C.c_int, [_vp, _ip]),
+    "imsrg_op_upload": (C.c_int, [_vp, _dp, _i64]),
+    "imsrg_op_download": (C.c_int, [_vp, _dp, _i64]),
+    "imsrg_op_copy": (C.c_int, [_vp, _vp]),
+    "imsrg_op_clear": (C.c_int, [_vp]),
+    "imsrg_op_scale": (C.c_int, [_vp, C.c_double]),
+    "imsrg_op_add": (C.c_int, [_vp, C.c_double, _vp, C.c_double, _vp]),
+    "imsrg_op_get_e0": (C.c_int, [_vp, _dp]),
+    "imsrg_op_set_e0": (C.c_int, [_vp, C.c_double]),
+    "imsrg_op_norm": (C.c_int, [_vp, _dp]),
+    "imsrg_mbpt2": (C.c_int, [_vp, _dp]),
+    "imsrg_normal_order": (C.c_int, [_vp]),
+    "imsrg_commutator": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint]),
+    "imsrg_commutator_110": (C.c_int, [_vp, _vp, _dp]),
+    "imsrg_commutator_220": (C.c_int, [_vp, _vp, _dp]),
+    "imsrg_cross_coupled": (C.c_int, [_vp, _dp, _i64]),
+    "imsrg_cross_coupled_size": (_i64, [_vp]),
+    "imsrg_generator": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "imsrg_generator_atanxx": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "imsrg_bch_expand": (C.c_int, [_vp, _vp, _vp, _vp, _ip]),
+    "imsrg_magnus_expand": (C.c_int, [_vp, _vp, _vp, _ip]),
+    "imsrg_magnus_flow": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_double, C.c_int, _dp, _dp, _dp, _dp,
+                                    _ip, _ip, _ip]),
+}
+for _name, (_res, _args) in _SIG.items():
+    _f = getattr(_lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+def header_symbols(path=HEADER):
+    """Every function name the C ABI header declares."""
+    txt = open(path).read()
+    return sorted(set(re.findall(r"\b(imsrg_\w+)\s*\(", txt)))
+
+
+def _chk(rc):
+    if rc != 0:
+        raise ImsrgError(f"imsrg error {rc}: {_lib.imsrg_last_error().decode()}")
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _iptr(a):
+    return a.ctypes.data_as(_ip)
+
+
+class Context:
+    """A device context bound to one basis (imsrg_ctx_create)."""
+
+    def __init__(self, basis: Basis, device: int = 0, precision: int = PREC_FP64):
+        self.basis = basis
+        t = np.ascontiguousarray(basis.orbit_table())
+        cols = [np.ascontiguousarray(t[:, k]) for k in range(5)]
+        h = _vp()
+        _chk(_lib.imsrg_ctx_create(len(t), *[_iptr(c) for c in cols], device, C.byref(h)))
+        self.h = h
+        self.neq = int(_lib.imsrg_ctx_neq(h))
+        if self.neq != basis.neq:
+            raise ImsrgError(f"neq mismatch {self.neq} vs {basis.neq}")
+        self.set_precision(precision)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.imsrg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_precision(self, prec):
+        _chk(_lib.imsrg_ctx_set_precision(self.h, int(prec)))
+        self.precision = prec
+
+    def info(self):
+        v = np.zeros(16)
+        _chk(_lib.imsrg_ctx_info(self.h, _dptr(v)))
+        keys = ["nsp", "nhole", "nblocks", "ncc", "npairs", "max_dim", "max_cc_rows", "op_bytes",
+                "vsize", "ccsize", "cwsize", "flop_pphh", "flop_ph"]
+        return dict(zip(keys, v[:len(keys)]))
+
+    def sync(self):
+        _chk(_lib.imsrg_ctx_synchronize(self.h))
+
+    @property
+    def stream(self):
+        return _lib.imsrg_ctx_stream(self.h)
+
+    def op(self, herm=1, vec=None):
+        o = Operator(self, herm)
+        if vec is not None:
+            o.upload(vec)
+        return o
+
+    # -- engine
+    def commutator(self, L, R, RES=None, mask=TALL):
+        if RES is None:
+            RES = Operator(self, -L.herm * R.herm)
+        _chk(_lib.imsrg_commutator(self.h, L.h, R.h, RES.h, mask))
+        return RES
+
+    def generator(self, kind, H, ETA=None):
+        ETA = ETA or Operator(self, -1)
+        _chk(_lib.imsrg_generator(self.h, GEN[kind], H.h, ETA.h))
+        return ETA
+
+    def generator_atanxx(self, Hold, H, ETA=None):
+        ETA = ETA or Operator(self, -1)
+        _chk(_lib.imsrg_generator_atanxx(self.h, Hold.h, H.h, ETA.h))
+        return ETA
+
+    def bch(self, G, H, HS=None):
+        HS = HS or Operator(self, 1)
+        n = C.c_int(0)
+        _chk(_lib.imsrg_bch_expand(self.h, HS.h, G.h, H.h, C.byref(n)))
+        return HS, n.value
+
+    def magnus_expand(self, DG, G):
+        n = C.c_int(0)
+        _chk(_lib.imsrg_magnus_expand(self.h, DG.h, G.h, C.byref(n)))
+        return DG, n.value
+
+    def magnus_flow(self, HS, kind="white", ds=0.5, max_steps=200, G=None):
+        G = G or Operator(self, -1)
+        s = np.zeros(max_steps); e0 = np.zeros(max_steps); emp2 = np.zeros(max_steps)
+        crit = np.zeros(max_steps)
+        nm = np.zeros(max_steps, np.int32); nb = np.zeros(max_steps, np.int32)
+        steps = C.c_int(0)
+        _chk(_lib.imsrg_magnus_flow(self.h, HS.h, G.h, GEN[kind], ds, max_steps, _dptr(s), _dptr(e0),
+                                    _dptr(emp2), _dptr(crit), _iptr(nm), _iptr(nb), C.byref(steps)))
+        n = steps.value
+        return dict(steps=n, s=s[:n], e0=e0[:n], e0_mbpt2=emp2[:n], crit=crit[:n], n_magnus=nm[:n],
+                    n_bch=nb[:n], HS=HS, G=G)
+
+    def cross_coupled(self, A):
+        n = int(_lib.imsrg_cross_coupled_size(self.h))
+        out = np.zeros(n)
+        _chk(_lib.imsrg_cross_coupled(A.h, _dptr(out), n))
+        return out
+
+
+class Operator:
+    """A device-resident sq_op (imsrg_op)."""
+
+    def __init__(self, ctx: Context, herm=1):
+        self.ctx = ctx
+        h = _vp()
+        _chk(_lib.imsrg_op_create(ctx.h, int(herm), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                _lib.imsrg_op_destroy(self.h)
+        except Exception:
+            pass
+
+    @property
+    def herm(self):
+        v = C.c_int(0)
+        _chk(_lib.imsrg_op_get_herm(self.h, C.byref(v)))
+        return v.value
+
+    @herm.setter
+    def herm(self, h):
+        _chk(_lib.imsrg_op_set_herm(self.h, int(h)))
+
+    def upload(self, vec):
+        v = np.ascontiguousarray(vec, dtype=np.float64)
+        _chk(_lib.imsrg_op_upload(self.h, _dptr(v), v.size))
+        return self
+
+    def download(self):
+        v = np.zeros(self.ctx.neq)
+        _chk(_lib.imsrg_op_download(self.h, _dptr(v), v.size))
+        return v
+
+    def copy_from(self, other):
+        _chk(_lib.imsrg_op_copy(other.h, self.h))
+        return self
+
+    def clear(self):
+        _chk(_lib.imsrg_op_clear(self.h))
+
+    def scale(self, a):
+        _chk(_lib.imsrg_op_scale(self.h, float(a)))
+
+    def add(self, ax, B, bx, out=None):
+        out = out or self
+        _chk(_lib.imsrg_op_add(self.h, float(ax), B.h, float(bx), out.h))
+        return out
+
+    @property
+    def e0(self):
+        v = C.c_double(0)
+        _chk(_lib.imsrg_op_get_e0(self.h, C.byref(v)))
+        return v.value
+
+    @e0.setter
+    def e0(self, x):
+        _chk(_lib.imsrg_op_set_e0(self.h, float(x)))
+
+    def norm(self):
+        v = C.c_double(0)
+        _chk(_lib.imsrg_op_norm(self.h, C.byref(v)))
+        return v.value
+
+    def mbpt2(self):
+        v = C.c_double(0)
+        _chk(_lib.imsrg_mbpt2(self.h, C.byref(v)))
+        return v.value
+
+    def normal_order(self):
+        _chk(_lib.imsrg_normal_order(self.h))
+        return self
+
+
+def commutator_110(L, R):
+    v = C.c_double(0)
+    _chk(_lib.imsrg_commutator_110(L.h, R.h, C.byref(v)))
+    return v.value
+
+
+def commutator_220(L, R):
+    v = C.c_double(0)
+    _chk(_lib.imsrg_commutator_220(L.h, R.h, C.byref(v)))
+    return v.value

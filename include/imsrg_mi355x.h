@@ -1,0 +1,134 @@
+/*
+ * imsrg_mi355x.h -- C ABI of the MI355X-native IMSRG(2) Magnus commutator engine.
+ *
+ * Drop-in for the scalar commutator engine of n-parzuchowski/EOM-IMSRG_FULL
+ * (im-SRG/commutators.f90, cross_coupled.f90, generators.f90,
+ * IMSRG_MAGNUS.f90).  Operators are device-resident; they cross the boundary
+ * only as FP64 streams in the reference's own vectorize order
+ * (basic_IMSRG.f90:3794-3854), so a Fortran caller moves an sq_op with
+ * vectorize/repackage and an ISO_C_BINDING interface.
+ *
+ * Conventions: angular momenta doubled (2j, 2tz); orbits 0-based on the C side;
+ * every function returns 0 on success and a negative IMSRG_E* code on failure,
+ * with a message from imsrg_last_error() (the reference STOPs instead).
+ * A context is bound to one device and one stream; calls on one context are
+ * stream-ordered and must come from one host thread at a time.  Different
+ * contexts are independent (no global mutable state besides the per-thread
+ * error string).
+ */
+#ifndef IMSRG_MI355X_H
+#define IMSRG_MI355X_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct imsrg_ctx_s *imsrg_ctx;
+typedef struct imsrg_op_s *imsrg_op;
+
+/* status codes */
+#define IMSRG_OK 0
+#define IMSRG_EINVAL -1     /* bad argument / size mismatch                  */
+#define IMSRG_EDEVICE -2    /* HIP runtime error                              */
+#define IMSRG_ENOMEM -3     /* device allocation failed                        */
+#define IMSRG_ECLIP -4      /* BCH series did not converge in 40 terms        */
+
+/* term mask bits for imsrg_commutator (one per reference routine) */
+#define IMSRG_T110 (1u << 0)      /* commutator_110      commutators.f90:10-33   */
+#define IMSRG_T111 (1u << 1)      /* commutator_111      commutators.f90:36-80   */
+#define IMSRG_T121 (1u << 2)      /* commutator_121      commutators.f90:83-280  */
+#define IMSRG_T122 (1u << 3)      /* commutator_122      commutators.f90:283-396 */
+#define IMSRG_T220 (1u << 4)      /* commutator_220      commutators.f90:399-432 */
+#define IMSRG_T221 (1u << 5)      /* commutator_221      commutators.f90:435-578 */
+#define IMSRG_T222PPHH (1u << 6)  /* commutator_222_pp_hh commutators.f90:581-768 */
+#define IMSRG_T222PH (1u << 7)    /* commutator_222_ph   commutators.f90:771-925 */
+#define IMSRG_TALL 0xFFu
+
+/* GEMM arithmetic of the 222 products */
+#define IMSRG_PREC_FP64 0
+#define IMSRG_PREC_SPLIT_FP32 1   /* 2-term split FP32 operands, FP32 MFMA, FP64 combine */
+
+/* generator kinds (generators.f90) */
+#define IMSRG_GEN_WHITE 0         /* build_gs_white  generators.f90:4-106   */
+#define IMSRG_GEN_ATAN 1          /* build_gs_atan   generators.f90:163-266 */
+#define IMSRG_GEN_W2 2            /* build_gs_w2     generators.f90:109-160 */
+
+const char *imsrg_last_error(void);
+
+/* ---- basis / context ----------------------------------------------------
+ * Replaces read_sp_basis + allocate_blocks + store_6j + allocate_CCMAT
+ * (basic_IMSRG.f90:152-641, 2183-2288; cross_coupled.f90:61-177): orbit
+ * quantum numbers (n, l, 2j, 2tz with protons -1) and con (1 = hole) in the
+ * caller's orbit order; builds the channel layout, index and 6j tables on
+ * the device.  device < 0 keeps the current device. */
+int imsrg_ctx_create(int nsp, const int *n, const int *l, const int *jj, const int *tz,
+                     const int *con, int device, imsrg_ctx *out);
+int imsrg_ctx_destroy(imsrg_ctx ctx);
+/* length of the vectorize stream (IMSRG_ODE.f90:36-44) */
+int64_t imsrg_ctx_neq(imsrg_ctx ctx);
+/* info[0..15]: nsp, nhole, nblocks, ncc, npairs, max block dim, max CC rows,
+ * device bytes per operator, vsize, ccsize, cwsize, flop_pphh, flop_ph
+ * (as doubles: algorithmic GEMM flop per commutator), 0,0,0 */
+int imsrg_ctx_info(imsrg_ctx ctx, double *info16);
+int imsrg_ctx_set_precision(imsrg_ctx ctx, int prec);
+int imsrg_ctx_synchronize(imsrg_ctx ctx);
+/* the hipStream_t the context launches on (as void*) */
+void *imsrg_ctx_stream(imsrg_ctx ctx);
+
+/* ---- operators (sq_op, basic_IMSRG.f90:50-76) ---------------------------- */
+int imsrg_op_create(imsrg_ctx ctx, int herm, imsrg_op *out);     /* duplicate_sq_op :2736 */
+int imsrg_op_destroy(imsrg_op op);
+int imsrg_op_set_herm(imsrg_op op, int herm);
+int imsrg_op_get_herm(imsrg_op op, int *herm);
+int imsrg_op_upload(imsrg_op op, const double *vec, int64_t neq);   /* repackage :3857-3918 */
+int imsrg_op_download(imsrg_op op, double *vec, int64_t neq);       /* vectorize :3794-3854 */
+int imsrg_op_copy(imsrg_op src, imsrg_op dst);                      /* copy_sq_op :2942-2969 */
+int imsrg_op_clear(imsrg_op op);                                    /* clear_sq_op :3537-3562 */
+int imsrg_op_scale(imsrg_op op, double a);                          /* scale_sq_op :3510-3534 */
+/* C = ax*A + bx*B (C may alias A or B)                                add_sq_op :3450-3478 */
+int imsrg_op_add(imsrg_op A, double ax, imsrg_op B, double bx, imsrg_op C);
+int imsrg_op_get_e0(imsrg_op op, double *e0);
+int imsrg_op_set_e0(imsrg_op op, double e0);
+int imsrg_op_norm(imsrg_op op, double *out);                        /* mat_frob_norm :3921-3952 */
+int imsrg_mbpt2(imsrg_op H, double *out);                           /* mbpt2 :4030-4064 */
+int imsrg_normal_order(imsrg_op H);                                 /* normal_order :2375-2506 */
+
+/* ---- commutator engine --------------------------------------------------
+ * RES = [L, R] restricted to the terms in mask, in the reference's call
+ * contract (IMSRG_MAGNUS.f90:207-219): one-body and two-body parts of RES are
+ * fully overwritten with the sum of the selected terms; when mask contains
+ * T110 or T220, RES E0 is assigned 110+220 (else left untouched, as in
+ * MAGNUS_EXPAND).  RES herm must be -herm(L)*herm(R).  RES must not alias L or R. */
+int imsrg_commutator(imsrg_ctx ctx, imsrg_op L, imsrg_op R, imsrg_op RES, unsigned mask);
+/* scalar terms (commutators.f90:10-33, 399-432) */
+int imsrg_commutator_110(imsrg_op L, imsrg_op R, double *out);
+int imsrg_commutator_220(imsrg_op L, imsrg_op R, double *out);
+/* Pandya transform of op (calculate_cross_coupled, cross_coupled.f90:1190-1300),
+ * copied to the host column-major per non-empty CC block in reference order */
+int imsrg_cross_coupled(imsrg_op op, double *host_out, int64_t n);
+int64_t imsrg_cross_coupled_size(imsrg_ctx ctx);
+
+/* ---- generators (external build_generator callbacks) ---------------------- */
+int imsrg_generator(imsrg_ctx ctx, int kind, imsrg_op H, imsrg_op ETA);
+/* build_gs_atanxx (generators.f90:271-374): denominators from Hold, numerators from H */
+int imsrg_generator_atanxx(imsrg_ctx ctx, imsrg_op Hold, imsrg_op H, imsrg_op ETA);
+
+/* ---- Magnus driver (IMSRG_MAGNUS.f90) -------------------------------------- */
+/* HS = e^G H e^-G by the nested-commutator series (BCH_EXPAND :149-234);
+ * *nterms = commutators evaluated.  Returns IMSRG_ECLIP after 39 terms. */
+int imsrg_bch_expand(imsrg_ctx ctx, imsrg_op HS, imsrg_op G, imsrg_op H, int *nterms);
+/* DG <- dOmega/ds series (MAGNUS_EXPAND :622-693) */
+int imsrg_magnus_expand(imsrg_ctx ctx, imsrg_op DG, imsrg_op G, int *nterms);
+/* full flow (magnus_decouple :15-128, no checkpointing): HS in = normal-ordered
+ * H(0), out = decoupled H(s); G out = Omega(s).  Per-step records (arrays of
+ * length max_steps, any may be NULL): s, E0, E0+MBPT2, |MBPT2|, #Magnus and
+ * #BCH commutators.  *steps = steps taken. */
+int imsrg_magnus_flow(imsrg_ctx ctx, imsrg_op HS, imsrg_op G, int gen_kind, double ds,
+                      int max_steps, double *s_out, double *e0_out, double *e0mbpt2_out,
+                      double *crit_out, int *nmag_out, int *nbch_out, int *steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

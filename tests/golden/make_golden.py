@@ -1,0 +1,111 @@
+"""Writes the golden fixtures under tests/golden/ from the reference's own
+compiled FP64 routines (oracle/_ref/ref_harness, built by oracle/ref.mk from
+the reference sources).  Run in the build container only:
+
+    make -f oracle/ref.mk && python tests/golden/make_golden.py
+
+The fixtures hold inputs and outputs (vectorize-order FP64 streams), never
+reference source.  Inputs are stored too, so nothing depends on a random
+number generator being stable across numpy versions.
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "eom-imsrg_full_amd"))
+sys.path.insert(0, ROOT)
+
+from imsrg_mi355x import Basis, random_operator, synthetic_hamiltonian  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def brute_points(B, rng):
+    pts = [(0, 0, 0, 0, 0, 0)]
+    N = B.nsp
+    for _ in range(25):
+        a, b = rng.integers(0, N, 2)
+        pts.append((1, a, b, 0, 0, 0))
+    n2 = 0
+    while n2 < 12:
+        a, b, c, d = rng.integers(0, N, 4)
+        if (B.l[a] + B.l[b]) % 2 != (B.l[c] + B.l[d]) % 2:
+            continue
+        if B.tz[a] + B.tz[b] != B.tz[c] + B.tz[d]:
+            continue
+        jmin = max(abs(B.jj[a] - B.jj[b]), abs(B.jj[c] - B.jj[d]))
+        jmax = min(B.jj[a] + B.jj[b], B.jj[c] + B.jj[d])
+        if jmin > jmax:
+            continue
+        for J in range(jmin, jmax + 1, 2):
+            pts.append((2, a, b, c, d, J))
+        n2 += 1
+    return np.array(pts, dtype=np.int64)
+
+
+def emax2():
+    B = Basis.ho(2, 2)
+    ref = O.RefRunner(B, threads=4)
+    d = {}
+    L1, R1 = random_operator(B, -1, 1), random_operator(B, 1, 2)
+    L2, R2 = random_operator(B, -1, 3), random_operator(B, -1, 4)
+    d["L1"], d["R1"], d["L2"], d["R2"] = L1, R1, L2, R2
+    for tag, (hL, L, hR, R) in {"a": (-1, L1, 1, R1), "b": (-1, L2, -1, R2)}.items():
+        t = ref.terms(hL, L, hR, R)
+        for k, v in t.items():
+            d[f"{tag}_{k}"] = np.atleast_1d(v)
+        d[f"{tag}_full"] = ref.comm(hL, L, hR, R)
+    d["cc_L1"] = ref.cc(-1, L1)
+    d["cc_R1"] = ref.cc(1, R1)
+    H0 = synthetic_hamiltonian(B, seed=0)
+    d["H0"] = H0
+    Hn = ref.normal(H0)
+    d["Hn"] = Hn
+    d["scal_Hn"] = np.array(ref.scal(1, Hn))
+    for kind in ("white", "atan", "w2"):
+        d[f"gen_{kind}"] = ref.gen(kind, Hn)
+    eta = d["gen_white"]
+    G = 0.5 * eta
+    d["G"] = G
+    d["bch"] = ref.bch(G, Hn)
+    d["magexp"] = ref.magexp(eta, G)
+    # one commutator [eta, H] and one Magnus step (the emax=2 CPU workload)
+    d["comm_eta_H"] = ref.comm(-1, eta, 1, Hn)
+    st1, hs1, g1 = ref.flow("white", Hn, 1)
+    d["step1_rows"], d["step1_HS"], d["step1_G"] = st1, hs1, g1
+    for kind in ("white", "w2"):
+        st, hs, g = ref.flow(kind, Hn, 200)
+        d[f"flow_{kind}_rows"], d[f"flow_{kind}_HS"], d[f"flow_{kind}_G"] = st, hs, g
+    pts = brute_points(B, np.random.Generator(np.random.PCG64(7)))
+    ref_pts = pts.copy()
+    ref_pts[:, 1:5] += 1  # the reference indexes orbits from 1
+    ref_pts[pts[:, 0] == 0, 1:5] = 0
+    ref_pts[pts[:, 0] == 1, 3:5] = 0
+    d["brute_pts"] = pts
+    d["brute_a"] = np.array(ref.brute(-1, L1, 1, R1, ref_pts))
+    d["brute_b"] = np.array(ref.brute(-1, L2, -1, R2, ref_pts))
+    np.savez_compressed(os.path.join(OUT, "emax2_k2.npz"), **d)
+    print("emax2_k2.npz", sum(v.nbytes for v in d.values()) / 1e6, "MB")
+
+
+def emax4():
+    B = Basis.ho(4, 3)
+    ref = O.RefRunner(B, threads=8)
+    d = {}
+    H0 = synthetic_hamiltonian(B, seed=0)
+    d["H0"] = H0
+    Hn = ref.normal(H0)
+    st, hs, _ = ref.flow("white", Hn, 200)
+    d["flow_white_rows"], d["flow_white_HS"] = st, hs
+    np.savez_compressed(os.path.join(OUT, "emax4_k3.npz"), **d)
+    print("emax4_k3.npz", sum(v.nbytes for v in d.values()) / 1e6, "MB")
+
+
+if __name__ == "__main__":
+    if not O.ref_available():
+        sys.exit("oracle/_ref/ref_harness is missing: run make -f oracle/ref.mk first")
+    emax2()
+    emax4()
