@@ -48,7 +48,41 @@ struct imsrg_ctx_s {
   int n_pp_tiles = 0, n_ph_tiles = 0;
   int prec = IMSRG_PREC_FP64;
   imsrg_op_s *tmp[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  // per-kernel-class HIP event timing on this context's stream
+  bool prof_on = false;
+  std::vector<hipEvent_t> evpool;
+  size_t evused = 0;
+  struct Rec { int cls; hipEvent_t a, b; };
+  std::vector<Rec> pend;
+  double prof_ms[IMSRG_PROF_NCLASS] = {};
+  long long prof_n[IMSRG_PROF_NCLASS] = {};
 };
+
+// Records the launch(es) in `stmt` as one timed interval of class `cls`
+// when profiling is on; events go on the context stream, nothing syncs.
+static hipEvent_t prof_event(imsrg_ctx_s *c) {
+  if (c->evused == c->evpool.size()) {
+    if (c->evpool.size() >= 400000) return nullptr;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->evpool.push_back(e);
+  }
+  return c->evpool[c->evused++];
+}
+#define PROF(c, cls, stmt)                                                   \
+  do {                                                                       \
+    hipEvent_t ea_ = nullptr, eb_ = nullptr;                                 \
+    if ((c)->prof_on) {                                                      \
+      ea_ = prof_event(c);                                                   \
+      eb_ = prof_event(c);                                                   \
+      if (ea_ && eb_) (void)hipEventRecord(ea_, (c)->stream);                \
+    }                                                                        \
+    stmt;                                                                    \
+    if (ea_ && eb_) {                                                        \
+      (void)hipEventRecord(eb_, (c)->stream);                                \
+      (c)->pend.push_back({(cls), ea_, eb_});                                \
+    }                                                                        \
+  } while (0)
 
 struct imsrg_op_s {
   imsrg_ctx_s *ctx = nullptr;
@@ -123,7 +157,7 @@ static int read_scalar(imsrg_ctx_s *c, int idx, double *out) {
 }
 static int op_norm(imsrg_op_s *op, double *out) {
   imsrg_ctx_s *c = op->ctx;
-  launch_sumsq(c->stream, c->dl, op->d, c->partials, c->scal);
+  PROF(c, IMSRG_PROF_NORM, launch_sumsq(c->stream, c->dl, op->d, c->partials, c->scal));
   return read_scalar(c, 0, out);
 }
 
@@ -245,6 +279,7 @@ int imsrg_ctx_destroy(imsrg_ctx c) {
       (void)hipFree(t->d);
       delete t;
     }
+  for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
   for (void *p : c->dallocs) (void)hipFree(p);
   if (c->hscal) (void)hipHostFree(c->hscal);
   if (c->hstage) (void)hipHostFree(c->hstage);
@@ -363,7 +398,7 @@ int imsrg_op_norm(imsrg_op op, double *out) {
 int imsrg_mbpt2(imsrg_op H, double *out) {
   if (!H || !out) return fail(IMSRG_EINVAL, "null argument");
   imsrg_ctx_s *c = H->ctx;
-  launch_mbpt2(c->stream, c->dl, H->d, c->partials, c->scal + 2);
+  PROF(c, IMSRG_PROF_NORM, launch_mbpt2(c->stream, c->dl, H->d, c->partials, c->scal + 2));
   return read_scalar(c, 2, out);
 }
 int imsrg_normal_order(imsrg_op H) {
@@ -385,23 +420,25 @@ static int comm_impl(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *R
   hipStream_t s = c->stream;
   const DevLayout &D = c->dl;
   const int hL = L->herm, hR = R->herm, hres = RES->herm;
-  if (mask & (T110 | T220)) launch_e0(s, D, L->d, R->d, RES->d, hL, hR, mask, c->partials);
+  if (mask & (T110 | T220)) PROF(c, IMSRG_PROF_E0, launch_e0(s, D, L->d, R->d, RES->d, hL, hR, mask, c->partials));
   double *Wa = c->wpphh, *Wb = c->wpphh + c->lay.vsize;
   if (mask & (T222PPHH | T221)) {
-    launch_gemm_f64(s, false, c->pp_probs, c->pp_tiles, c->n_pp_tiles, L->d + D.v_off, R->d + D.v_off,
-                    c->wpphh);
+    PROF(c, IMSRG_PROF_GEMM_PPHH,
+         launch_gemm_f64(s, false, c->pp_probs, c->pp_tiles, c->n_pp_tiles, L->d + D.v_off,
+                         R->d + D.v_off, c->wpphh));
   }
   if (mask & T222PH) {
-    if (!lcc_ready) launch_pandya(s, D, L->d, c->cc[0]);
-    launch_pandya(s, D, R->d, c->cc[1]);
-    launch_gemm_f64(s, true, c->ph_probs, c->ph_tiles, c->n_ph_tiles, c->cc[0], c->cc[1], c->wph);
+    if (!lcc_ready) PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, L->d, c->cc[0]));
+    PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, R->d, c->cc[1]));
+    PROF(c, IMSRG_PROF_GEMM_PH,
+         launch_gemm_f64(s, true, c->ph_probs, c->ph_tiles, c->n_ph_tiles, c->cc[0], c->cc[1], c->wph));
   }
-  if (mask & T122) launch_122(s, D, L->d, R->d, RES->d);
+  if (mask & T122) PROF(c, IMSRG_PROF_122, launch_122(s, D, L->d, R->d, RES->d));
   else launch_clear_upper(s, D, RES->d);
-  if (mask & T222PPHH) launch_pphh_combine(s, D, Wa, Wb, RES->d, (double)(hL * hR));
-  if (mask & T222PH) launch_ph_gather(s, D, c->wph, RES->d, (double)hL, (double)hR);
-  launch_mirror(s, D, RES->d, hres);
-  launch_onebody(s, D, L->d, R->d, Wa, Wb, RES->d, hL, hR, hres, mask);
+  if (mask & T222PPHH) PROF(c, IMSRG_PROF_COMBINE, launch_pphh_combine(s, D, Wa, Wb, RES->d, (double)(hL * hR)));
+  if (mask & T222PH) PROF(c, IMSRG_PROF_PH_GATHER, launch_ph_gather(s, D, c->wph, RES->d, (double)hL, (double)hR));
+  PROF(c, IMSRG_PROF_MIRROR, launch_mirror(s, D, RES->d, hres));
+  PROF(c, IMSRG_PROF_ONEBODY, launch_onebody(s, D, L->d, R->d, Wa, Wb, RES->d, hL, hR, hres, mask));
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -449,9 +486,11 @@ int imsrg_cross_coupled(imsrg_op op, double *host_out, int64_t n) {
 // ---------------------------------------------------------------- generators
 static void gen_simple(imsrg_ctx_s *c, imsrg_op_s *Hden, imsrg_op_s *Hnum, imsrg_op_s *ETA, int kind) {
   hipStream_t s = c->stream;
-  launch_fill(s, ETA->d + c->lay.f_off, c->lay.op_size - c->lay.f_off, 0.0);
-  launch_monopole(s, c->dl, Hden->d, c->mono);
-  launch_eta(s, c->dl, Hden->d, Hnum->d, c->mono, ETA->d, kind);
+  PROF(c, IMSRG_PROF_GENERATOR, {
+    launch_fill(s, ETA->d + c->lay.f_off, c->lay.op_size - c->lay.f_off, 0.0);
+    launch_monopole(s, c->dl, Hden->d, c->mono);
+    launch_eta(s, c->dl, Hden->d, Hnum->d, c->mono, ETA->d, kind);
+  });
   ETA->herm = -1;
 }
 static int gen_impl(imsrg_ctx_s *c, int kind, imsrg_op_s *H, imsrg_op_s *ETA) {
@@ -504,7 +543,7 @@ static int bch_impl(imsrg_ctx_s *c, imsrg_op_s *HS, imsrg_op_s *G, imsrg_op_s *H
     int rc = comm_impl(c, G, AD, INT2, IMSRG_TALL, true);
     if (rc) return rc;
     n++;
-    launch_axpby(c->stream, c->lay.op_size, 1.0, HS->d, coef, INT2->d, HS->d);
+    PROF(c, IMSRG_PROF_AXPBY, launch_axpby(c->stream, c->lay.op_size, 1.0, HS->d, coef, INT2->d, HS->d));
     double nrm;
     rc = op_norm(INT2, &nrm);
     if (rc) return rc;
@@ -547,7 +586,7 @@ static int magexp_impl(imsrg_ctx_s *c, imsrg_op_s *DG, imsrg_op_s *G, int *nterm
     rc = comm_impl(c, G, cur, oth, IMSRG_TALL & ~(T110 | T220), true);
     if (rc) return rc;
     n++;
-    launch_axpby(c->stream, c->lay.op_size, 1.0, DG->d, cof[i - 1], oth->d, DG->d);
+    PROF(c, IMSRG_PROF_AXPBY, launch_axpby(c->stream, c->lay.op_size, 1.0, DG->d, cof[i - 1], oth->d, DG->d));
     double nrm;
     rc = op_norm(oth, &nrm);
     if (rc) return rc;
@@ -562,53 +601,135 @@ int imsrg_magnus_expand(imsrg_ctx c, imsrg_op DG, imsrg_op G, int *nterms) {
   return magexp_impl(c, DG, G, nterms);
 }
 
-// magnus_decouple (IMSRG_MAGNUS.f90:15-128) without checkpointing
+// magnus_decouple (IMSRG_MAGNUS.f90:15-128) without checkpointing, split into
+// set-up (:31-96) and one pass of the step loop (:98-126)
+struct imsrg_flow_s {
+  imsrg_ctx_s *ctx = nullptr;
+  imsrg_op_s *HS = nullptr, *G = nullptr, *H0 = nullptr, *DG = nullptr;
+  int gen_kind = 0;
+  double ds = 0.5, s = 0.0, crit = 0.0, emp2 = 0.0;
+  int steps = 0;
+};
+
+int imsrg_flow_create(imsrg_ctx c, imsrg_op HS, imsrg_op G, int gen_kind, double ds, imsrg_flow *out) {
+  if (!c || !HS || !G || HS == G || !out) return fail(IMSRG_EINVAL, "bad operands");
+  if (HS->ctx != c || G->ctx != c) return fail(IMSRG_EINVAL, "operands of another context");
+  imsrg_flow_s *f = new imsrg_flow_s;
+  f->ctx = c; f->HS = HS; f->G = G; f->gen_kind = gen_kind; f->ds = ds;
+  int rc = op_new(c, 1, &f->H0);
+  if (!rc) rc = op_new(c, -1, &f->DG);
+  if (!rc) {
+    launch_fill(c->stream, G->d, c->lay.op_size, 0.0);
+    G->herm = -1;
+    rc = gen_impl(c, gen_kind, HS, f->DG);
+  }
+  if (!rc) {
+    op_copy(HS, f->H0);
+    rc = imsrg_mbpt2(HS, &f->emp2);
+    f->crit = std::fabs(f->emp2);
+  }
+  if (rc) {
+    imsrg_flow_destroy(f);
+    return rc;
+  }
+  *out = f;
+  return 0;
+}
+
+int imsrg_flow_step(imsrg_flow f, double *rec8) {
+  if (!f) return fail(IMSRG_EINVAL, "null flow");
+  imsrg_ctx_s *c = f->ctx;
+  int nm = 0, nb = 0;
+  int rc = magexp_impl(c, f->DG, f->G, &nm);
+  if (rc) return rc;
+  PROF(c, IMSRG_PROF_AXPBY,
+       launch_axpby(c->stream, c->lay.op_size, 1.0, f->G->d, f->ds, f->DG->d, f->G->d));  // euler_step :756-766
+  f->s = f->s + f->ds;
+  rc = bch_impl(c, f->HS, f->G, f->H0, &nb);
+  if (rc) return rc;
+  rc = gen_impl(c, f->gen_kind, f->HS, f->DG);
+  if (rc) return rc;
+  rc = imsrg_mbpt2(f->HS, &f->emp2);
+  if (rc) return rc;
+  f->crit = std::fabs(f->emp2);
+  double e0 = 0.0;
+  rc = imsrg_op_get_e0(f->HS, &e0);
+  if (rc) return rc;
+  f->steps++;
+  if (rec8) {
+    const double r[8] = {f->s, e0, e0 + f->emp2, f->crit, (double)nm, (double)nb, (double)f->steps, 0.0};
+    std::memcpy(rec8, r, sizeof(r));
+  }
+  return 0;
+}
+
+int imsrg_flow_converged(imsrg_flow f, int *yes) {
+  if (!f || !yes) return fail(IMSRG_EINVAL, "null argument");
+  *yes = !(f->crit > kConv);  // do while (crit > 1e-6)
+  return 0;
+}
+
+int imsrg_flow_destroy(imsrg_flow f) {
+  if (!f) return 0;
+  if (f->H0) imsrg_op_destroy(f->H0);
+  if (f->DG) imsrg_op_destroy(f->DG);
+  delete f;
+  return 0;
+}
+
 int imsrg_magnus_flow(imsrg_ctx c, imsrg_op HS, imsrg_op G, int gen_kind, double ds, int max_steps,
                       double *s_out, double *e0_out, double *e0mbpt2_out, double *crit_out,
                       int *nmag_out, int *nbch_out, int *steps_out) {
-  if (!c || !HS || !G || HS == G) return fail(IMSRG_EINVAL, "bad operands");
-  imsrg_op_s *H0 = nullptr, *DG = nullptr;
-  int rc = op_new(c, 1, &H0);
-  if (!rc) rc = op_new(c, -1, &DG);
-  if (rc) {
-    if (H0) imsrg_op_destroy(H0);
-    return rc;
-  }
-  launch_fill(c->stream, G->d, c->lay.op_size, 0.0);
-  G->herm = -1;
-  DG->herm = -1;
-  rc = gen_impl(c, gen_kind, HS, DG);
-  op_copy(HS, H0);
-  double s = 0.0, E_mbpt2 = 0.0;
-  if (!rc) rc = imsrg_mbpt2(HS, &E_mbpt2);
-  double crit = std::fabs(E_mbpt2);
-  int steps = 0;
-  while (!rc && crit > kConv && steps < max_steps) {
-    steps++;
-    int nm = 0, nb = 0;
-    rc = magexp_impl(c, DG, G, &nm);
+  imsrg_flow f = nullptr;
+  int rc = imsrg_flow_create(c, HS, G, gen_kind, ds, &f);
+  if (rc) return rc;
+  int steps = 0, done = 0;
+  imsrg_flow_converged(f, &done);
+  while (!done && steps < max_steps) {
+    double r[8];
+    rc = imsrg_flow_step(f, r);
     if (rc) break;
-    launch_axpby(c->stream, c->lay.op_size, 1.0, G->d, ds, DG->d, G->d);  // euler_step :756-766
-    s = s + ds;
-    rc = bch_impl(c, HS, G, H0, &nb);
-    if (rc) break;
-    rc = gen_impl(c, gen_kind, HS, DG);
-    if (rc) break;
-    rc = imsrg_mbpt2(HS, &E_mbpt2);
-    if (rc) break;
-    crit = std::fabs(E_mbpt2);
-    double e0 = 0.0;
-    rc = imsrg_op_get_e0(HS, &e0);
-    int k = steps - 1;
-    if (s_out) s_out[k] = s;
-    if (e0_out) e0_out[k] = e0;
-    if (e0mbpt2_out) e0mbpt2_out[k] = e0 + E_mbpt2;
-    if (crit_out) crit_out[k] = crit;
-    if (nmag_out) nmag_out[k] = nm;
-    if (nbch_out) nbch_out[k] = nb;
+    int k = steps++;
+    if (s_out) s_out[k] = r[0];
+    if (e0_out) e0_out[k] = r[1];
+    if (e0mbpt2_out) e0mbpt2_out[k] = r[2];
+    if (crit_out) crit_out[k] = r[3];
+    if (nmag_out) nmag_out[k] = (int)r[4];
+    if (nbch_out) nbch_out[k] = (int)r[5];
+    imsrg_flow_converged(f, &done);
   }
   if (steps_out) *steps_out = steps;
-  imsrg_op_destroy(H0);
-  imsrg_op_destroy(DG);
+  imsrg_flow_destroy(f);
   return rc;
+}
+
+// ---------------------------------------------------------------- profiling
+int imsrg_ctx_profile(imsrg_ctx c, int enable) {
+  if (!c) return fail(IMSRG_EINVAL, "null context");
+  c->prof_on = enable != 0;
+  return 0;
+}
+
+int imsrg_ctx_profile_read(imsrg_ctx c, double *ms, long long *launches, int nclass, int reset) {
+  if (!c || !ms || !launches) return fail(IMSRG_EINVAL, "null argument");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (auto &r : c->pend) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+      c->prof_ms[r.cls] += t;
+      c->prof_n[r.cls] += 1;
+    }
+  }
+  c->pend.clear();
+  c->evused = 0;
+  for (int k = 0; k < nclass && k < IMSRG_PROF_NCLASS; k++) {
+    ms[k] = c->prof_ms[k];
+    launches[k] = c->prof_n[k];
+  }
+  if (reset)
+    for (int k = 0; k < IMSRG_PROF_NCLASS; k++) {
+      c->prof_ms[k] = 0.0;
+      c->prof_n[k] = 0;
+    }
+  return 0;
 }

@@ -74,7 +74,15 @@ _SIG = {
     "imsrg_magnus_expand": (C.c_int, [_vp, _vp, _vp, _ip]),
     "imsrg_magnus_flow": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_double, C.c_int, _dp, _dp, _dp, _dp,
                                     _ip, _ip, _ip]),
+    "imsrg_flow_create": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_double, C.POINTER(_vp)]),
+    "imsrg_flow_step": (C.c_int, [_vp, _dp]),
+    "imsrg_flow_converged": (C.c_int, [_vp, _ip]),
+    "imsrg_flow_destroy": (C.c_int, [_vp]),
+    "imsrg_ctx_profile": (C.c_int, [_vp, C.c_int]),
+    "imsrg_ctx_profile_read": (C.c_int, [_vp, _dp, C.POINTER(C.c_longlong), C.c_int, C.c_int]),
 }
+PROF_CLASSES = ["gemm_pphh", "gemm_ph", "pandya", "k122", "pphh_combine", "ph_gather", "mirror",
+                "onebody", "e0", "norm", "axpby", "generator"]
 for _name, (_res, _args) in _SIG.items():
     _f = getattr(_lib, _name)
     _f.restype = _res
@@ -190,6 +198,20 @@ class Context:
         return dict(steps=n, s=s[:n], e0=e0[:n], e0_mbpt2=emp2[:n], crit=crit[:n], n_magnus=nm[:n],
                     n_bch=nb[:n], HS=HS, G=G)
 
+    def flow(self, HS, kind="white", ds=0.5, G=None):
+        return Flow(self, HS, kind, ds, G)
+
+    def profile(self, enable=True):
+        _chk(_lib.imsrg_ctx_profile(self.h, 1 if enable else 0))
+
+    def profile_read(self, reset=True):
+        """{kernel class: (total ms, launches)} since the last reset."""
+        n = 16
+        ms = np.zeros(n)
+        cnt = (C.c_longlong * n)()
+        _chk(_lib.imsrg_ctx_profile_read(self.h, _dptr(ms), cnt, n, 1 if reset else 0))
+        return {name: (float(ms[k]), int(cnt[k])) for k, name in enumerate(PROF_CLASSES)}
+
     def cross_coupled(self, A):
         n = int(_lib.imsrg_cross_coupled_size(self.h))
         out = np.zeros(n)
@@ -283,3 +305,38 @@ def commutator_220(L, R):
     v = C.c_double(0)
     _chk(_lib.imsrg_commutator_220(L.h, R.h, C.byref(v)))
     return v.value
+
+
+class Flow:
+    """magnus_decouple one step at a time (imsrg_flow_*)."""
+
+    def __init__(self, ctx, HS, kind="white", ds=0.5, G=None):
+        self.ctx = ctx
+        self.HS = HS
+        self.G = G or Operator(ctx, -1)
+        h = _vp()
+        _chk(_lib.imsrg_flow_create(ctx.h, HS.h, self.G.h, GEN[kind], float(ds), C.byref(h)))
+        self.h = h
+
+    def step(self):
+        r = np.zeros(8)
+        _chk(_lib.imsrg_flow_step(self.h, _dptr(r)))
+        return dict(s=r[0], e0=r[1], e0_mbpt2=r[2], crit=r[3], n_magnus=int(r[4]), n_bch=int(r[5]),
+                    step=int(r[6]))
+
+    @property
+    def converged(self):
+        v = C.c_int(0)
+        _chk(_lib.imsrg_flow_converged(self.h, C.byref(v)))
+        return bool(v.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.imsrg_flow_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -128,6 +128,38 @@ int imsrg_magnus_flow(imsrg_ctx ctx, imsrg_op HS, imsrg_op G, int gen_kind, doub
                       int max_steps, double *s_out, double *e0_out, double *e0mbpt2_out,
                       double *crit_out, int *nmag_out, int *nbch_out, int *steps);
 
+/* The same flow one step at a time: set-up of magnus_decouple (:31-96; Omega
+ * reset to 0, DG = generator(HS), H(0) kept, crit = |MBPT2(HS)|), then one
+ * pass of its step loop (:98-126) per imsrg_flow_step.  rec8 receives
+ * s, E0, E0+MBPT2, |MBPT2|, #Magnus, #BCH commutators, step index, 0.
+ * HS and G stay owned by the caller and must outlive the flow. */
+typedef struct imsrg_flow_s *imsrg_flow;
+int imsrg_flow_create(imsrg_ctx ctx, imsrg_op HS, imsrg_op G, int gen_kind, double ds, imsrg_flow *out);
+int imsrg_flow_step(imsrg_flow flow, double *rec8);
+int imsrg_flow_converged(imsrg_flow flow, int *yes);   /* |MBPT2| <= float32(1e-6) */
+int imsrg_flow_destroy(imsrg_flow flow);
+
+/* ---- measurement (no counterpart in the reference, which only prints
+ * omp_get_wtime, main_IMSRG.f90:46,308-309) ---------------------------------
+ * While enabled, every engine kernel launch is bracketed by HIP events on the
+ * context stream; profile_read synchronizes the stream and returns the summed
+ * milliseconds and launch counts per kernel class. */
+#define IMSRG_PROF_GEMM_PPHH 0
+#define IMSRG_PROF_GEMM_PH 1
+#define IMSRG_PROF_PANDYA 2
+#define IMSRG_PROF_122 3
+#define IMSRG_PROF_COMBINE 4
+#define IMSRG_PROF_PH_GATHER 5
+#define IMSRG_PROF_MIRROR 6
+#define IMSRG_PROF_ONEBODY 7
+#define IMSRG_PROF_E0 8
+#define IMSRG_PROF_NORM 9
+#define IMSRG_PROF_AXPBY 10
+#define IMSRG_PROF_GENERATOR 11
+#define IMSRG_PROF_NCLASS 16
+int imsrg_ctx_profile(imsrg_ctx ctx, int enable);
+int imsrg_ctx_profile_read(imsrg_ctx ctx, double *ms, long long *launches, int nclass, int reset);
+
 #ifdef __cplusplus
 }
 #endif
