@@ -23,6 +23,8 @@ struct DevLayout {
   const long long *ccoff, *cwoff;
   const int *ccrow, *ccrbase, *ccrow_a, *ccrow_b, *cccbase, *cccol_p, *cccol_h, *cccol_q;
   const double *sixj;
+  const unsigned *ph_list1, *ph_list2;  // upper elements sorted by (i,k) and by (i,l)
+  long long nupper;
   long long f_off, v_off, op_size, vsize;
 };
 
@@ -43,12 +45,14 @@ void launch_axpby(hipStream_t s, long long n, double a, const double *A, double 
                   double *C);
 void launch_gemm_f64(hipStream_t s, bool a_kmajor, const GemmProb *probs, const int4 *tiles,
                      int ntiles, const double *A, const double *B, double *C);
+void launch_gemm_z(hipStream_t s, const GemmProb *probs, const int4 *tiles, int ntiles, const double *A,
+                   const double *B, double *Z, double hL, double hR);
 void launch_pandya(hipStream_t s, const DevLayout &L, const double *op, double *cc);
 void launch_122(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES);
 void launch_clear_upper(hipStream_t s, const DevLayout &L, double *RES);
 void launch_pphh_combine(hipStream_t s, const DevLayout &L, const double *Wa, const double *Wb,
                          double *RES, double pm);
-void launch_ph_gather(hipStream_t s, const DevLayout &L, const double *W, double *RES, double hL,
+void launch_ph_gather(hipStream_t s, const DevLayout &L, const double *Z, double *RES, double hL,
                       double hR);
 void launch_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm);
 void launch_onebody(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop,

@@ -112,7 +112,9 @@ static int dalloc(imsrg_ctx_s *c, size_t ndouble, double **out) {
   return 0;
 }
 
-static void build_tiles(const std::vector<GemmProb> &probs, std::vector<int4> &tiles) {
+// tile schedule, heaviest problems first; with `pairs`, only tiles (tm <= tn)
+// (each workgroup then owns the mirrored tile too)
+static void build_tiles(const std::vector<GemmProb> &probs, std::vector<int4> &tiles, bool pairs = false) {
   tiles.clear();
   std::vector<int> order(probs.size());
   for (size_t i = 0; i < probs.size(); i++) order[i] = (int)i;
@@ -121,7 +123,7 @@ static void build_tiles(const std::vector<GemmProb> &probs, std::vector<int4> &t
     const GemmProb &p = probs[pi];
     int tm = (p.M + 63) / 64, tn = (p.N + 63) / 64;
     for (int i = 0; i < tm; i++)
-      for (int j = 0; j < tn; j++) tiles.push_back(make_int4(pi, i, j, 0));
+      for (int j = pairs ? i : 0; j < tn; j++) tiles.push_back(make_int4(pi, i, j, 0));
   }
 }
 
@@ -218,6 +220,8 @@ int imsrg_ctx_create(int nsp, const int *n, const int *l, const int *jj, const i
   UP(Ly.ccrow, ccrow); UP(Ly.ccrbase, ccrbase); UP(Ly.ccrow_a, ccrow_a); UP(Ly.ccrow_b, ccrow_b);
   UP(Ly.cccbase, cccbase); UP(Ly.cccol_p, cccol_p); UP(Ly.cccol_h, cccol_h); UP(cccol_q, cccol_q);
   UP(Ly.sixj, sixj);
+  UP(Ly.ph_list1, ph_list1); UP(Ly.ph_list2, ph_list2);
+  D.nupper = (long long)Ly.ph_list1.size();
 #undef UP
   if (!rc) rc = dalloc(c, 2 * (size_t)Ly.vsize, &c->wpphh);
   if (!rc) rc = dalloc(c, Ly.ccsize, &c->cc[0]);
@@ -256,7 +260,7 @@ int imsrg_ctx_create(int nsp, const int *n, const int *l, const int *jj, const i
       ph.push_back(GemmProb{r, r, Ly.ccnph[q], r, r, r, Ly.ccoff[q], Ly.ccoff[q], Ly.cwoff[q]});
     }
     std::vector<int4> tiles;
-    build_tiles(ph, tiles);
+    build_tiles(ph, tiles, true);
     c->n_ph_tiles = (int)tiles.size();
     rc = upload_table(c, ph, (const GemmProb **)&c->ph_probs);
     if (!rc) rc = upload_table(c, tiles, (const int4 **)&c->ph_tiles);
@@ -431,7 +435,8 @@ static int comm_impl(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *R
     if (!lcc_ready) PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, L->d, c->cc[0]));
     PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, R->d, c->cc[1]));
     PROF(c, IMSRG_PROF_GEMM_PH,
-         launch_gemm_f64(s, true, c->ph_probs, c->ph_tiles, c->n_ph_tiles, c->cc[0], c->cc[1], c->wph));
+         launch_gemm_z(s, c->ph_probs, c->ph_tiles, c->n_ph_tiles, c->cc[0], c->cc[1], c->wph,
+                       (double)hL, (double)hR));
   }
   if (mask & T122) PROF(c, IMSRG_PROF_122, launch_122(s, D, L->d, R->d, RES->d));
   else launch_clear_upper(s, D, RES->d);

@@ -96,26 +96,16 @@ void launch_axpby(hipStream_t s, long long n, double a, const double *A, double 
 // 2x2 v_mfma_f64_16x16x4 tiles; K staged through LDS 16 at a time.
 constexpr int GBM = 64, GBN = 64, GBK = 16, GPAD = 1;
 
+// one 64x64 tile of C = A*B into acc (per-lane 2x2 MFMA 16x16 quadrants)
 template <bool A_KMAJOR>
-__global__ __launch_bounds__(256) void k_gemm_f64(const GemmProb *__restrict__ probs,
-                                                  const int4 *__restrict__ tiles, int ntiles,
-                                                  const double *__restrict__ A,
-                                                  const double *__restrict__ B,
-                                                  double *__restrict__ C) {
-  const int t = blockIdx.x;
-  if (t >= ntiles) return;
-  const int4 tl = tiles[t];
-  const GemmProb p = probs[tl.x];
-  const int m0 = tl.y * GBM, n0 = tl.z * GBN;
-  __shared__ double As[GBK][GBM + GPAD];
-  __shared__ double Bs[GBK][GBN + GPAD];
+__device__ __forceinline__ void gemm_tile_f64(const GemmProb &p, const double *__restrict__ Ap,
+                                              const double *__restrict__ Bp, int m0, int n0,
+                                              double (*As)[GBM + GPAD], double (*Bs)[GBN + GPAD],
+                                              double4_t acc[2][2]) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  double4_t acc[2][2];
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
-  const double *Ap = A + p.a_off;
-  const double *Bp = B + p.b_off;
   for (int k0 = 0; k0 < p.K; k0 += GBK) {
     if (A_KMAJOR) {
       const int kk = tid >> 4, mm = (tid & 15) * 4;
@@ -158,6 +148,31 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmProb *__restrict__ p
     }
     __syncthreads();
   }
+}
+// local (row, col) of accumulator element (ti, tj, v) of this lane.
+// v_mfma_f64_16x16x4 D layout: D[i][j] with j = lane%16, i = lane/16 + 4v.
+__device__ __forceinline__ int acc_row(int ti, int v) {
+  return ((threadIdx.x >> 6) >> 1) * 32 + ti * 16 + ((threadIdx.x & 63) >> 4) + 4 * v;
+}
+__device__ __forceinline__ int acc_col(int tj) {
+  return ((threadIdx.x >> 6) & 1) * 32 + tj * 16 + (threadIdx.x & 15);
+}
+
+template <bool A_KMAJOR>
+__global__ __launch_bounds__(256) void k_gemm_f64(const GemmProb *__restrict__ probs,
+                                                  const int4 *__restrict__ tiles, int ntiles,
+                                                  const double *__restrict__ A,
+                                                  const double *__restrict__ B,
+                                                  double *__restrict__ C) {
+  const int t = blockIdx.x;
+  if (t >= ntiles) return;
+  const int4 tl = tiles[t];
+  const GemmProb p = probs[tl.x];
+  const int m0 = tl.y * GBM, n0 = tl.z * GBN;
+  __shared__ double As[GBK][GBM + GPAD];
+  __shared__ double Bs[GBK][GBN + GPAD];
+  double4_t acc[2][2];
+  gemm_tile_f64<A_KMAJOR>(p, A + p.a_off, B + p.b_off, m0, n0, As, Bs, acc);
   double *Cp = C + p.c_off;
 #pragma unroll
   for (int ti = 0; ti < 2; ti++)
@@ -165,10 +180,76 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmProb *__restrict__ p
     for (int tj = 0; tj < 2; tj++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int row = m0 + wm * 32 + ti * 16 + (lane >> 4) + 4 * v;  // D[i][j]: i = lane/16 + 4v
-        const int col = n0 + wn * 32 + tj * 16 + (lane & 15);
+        const int row = m0 + acc_row(ti, v), col = n0 + acc_col(tj);
         if (row < p.M && col < p.N) Cp[(long long)row * p.ldc + col] = acc[ti][tj][v];
       }
+}
+
+// Pandya product in antisymmetrised form: W = LCC * RCC^T and
+// Z = hL*W - hR*W^T written directly.  One workgroup per tile pair (tm <= tn):
+// it computes W(tm,tn) and W(tn,tm) and exchanges them through LDS, so Z costs
+// no more MFMA work than W and never needs a separate transpose pass.
+__global__ __launch_bounds__(256) void k_gemm_z(const GemmProb *__restrict__ probs,
+                                                const int4 *__restrict__ tiles, int ntiles,
+                                                const double *__restrict__ A,
+                                                const double *__restrict__ B,
+                                                double *__restrict__ Z, double hL, double hR) {
+  const int t = blockIdx.x;
+  if (t >= ntiles) return;
+  const int4 tl = tiles[t];
+  const GemmProb p = probs[tl.x];
+  const int m0 = tl.y * GBM, n0 = tl.z * GBN;
+  __shared__ double As[GBK][GBM + GPAD];
+  __shared__ double Bs[GBK][GBN + GPAD];
+  __shared__ double T[GBM][GBN + GPAD];
+  double4_t c1[2][2], c2[2][2];
+  const double *Ap = A + p.a_off, *Bp = B + p.b_off;
+  double *Zp = Z + p.c_off;
+  gemm_tile_f64<true>(p, Ap, Bp, m0, n0, As, Bs, c1);  // W(tm, tn)
+  const bool diag = (tl.y == tl.z);
+  if (!diag) gemm_tile_f64<true>(p, Ap, Bp, n0, m0, As, Bs, c2);  // W(tn, tm)
+  // Z(tm,tn)[r][c] = hL*W(tm,tn)[r][c] - hR*W(tn,tm)[c][r]
+#pragma unroll
+  for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+    for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) T[acc_row(ti, v)][acc_col(tj)] = diag ? c1[ti][tj][v] : c2[ti][tj][v];
+  __syncthreads();
+#pragma unroll
+  for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+    for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = acc_row(ti, v), c = acc_col(tj);
+        const int row = m0 + r, col = n0 + c;
+        if (row < p.M && col < p.N) Zp[(long long)row * p.ldc + col] = hL * c1[ti][tj][v] - hR * T[c][r];
+      }
+  if (diag) return;
+  __syncthreads();
+#pragma unroll
+  for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+    for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) T[acc_row(ti, v)][acc_col(tj)] = c1[ti][tj][v];
+  __syncthreads();
+#pragma unroll
+  for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+    for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = acc_row(ti, v), c = acc_col(tj);
+        const int row = n0 + r, col = m0 + c;
+        if (row < p.M && col < p.N) Zp[(long long)row * p.ldc + col] = hL * c2[ti][tj][v] - hR * T[c][r];
+      }
+}
+void launch_gemm_z(hipStream_t s, const GemmProb *probs, const int4 *tiles, int ntiles, const double *A,
+                   const double *B, double *Z, double hL, double hR) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_gemm_z, dim3(ntiles), dim3(256), 0, s, probs, tiles, ntiles, A, B, Z, hL, hR);
 }
 
 void launch_gemm_f64(hipStream_t s, bool a_kmajor, const GemmProb *probs, const int4 *tiles,
@@ -286,67 +367,68 @@ void launch_pphh_combine(hipStream_t s, const DevLayout &L, const double *Wa, co
 }
 
 // ------------------------------------------------------------------ 222_ph recoupling gather
-// commutators.f90:804-923: for every upper element (ij,kl;J) two J' sums of
-// 6j-weighted W entries; W blocks row-major rlen x rlen (W(x,y) at x*rlen+y).
-__global__ __launch_bounds__(64) void k_ph_gather(DevLayout L, const double *__restrict__ W,
-                                                  double *__restrict__ RES, double hL, double hR) {
-  const int r = blockIdx.x;
-  if (r >= L.npairs) return;
-  const int q = L.pair_q[r], base = L.pbase[q], rl = r - base, dim = L.bdim[q];
-  const int N = L.nsp, JTM1 = L.jtot_max + 1, Jtot = L.bJ[q];
-  double *OV = RES + L.v_off + L.boff[q];
-  const int i = L.pair_a[r], j = L.pair_b[r];
-  const double pre = (i == j) ? .70710678118 : 1.0;
-  const int ji = L.jj[i], jj = L.jj[j], li = L.l[i], lj = L.l[j], ti = L.tz[i], tj = L.tz[j];
-  for (int s = rl + (int)threadIdx.x; s < dim; s += 64) {
-    const int k = L.pair_a[base + s], l = L.pair_b[base + s];
-    const double pre2 = (k == l) ? .70710678118 : 1.0;
-    const int jk = L.jj[k], jl = L.jj[l], lk = L.l[k], ll = L.l[l], tk = L.tz[k], tl = L.tz[l];
-    double sm = 0.0;
-    int Tz = abs(ti - tk) / 2;
-    if (abs(tl - tj) != Tz * 2) continue;
-    int PAR = (li + lk) % 2;
-    if ((ll + lj) % 2 != PAR) continue;
-    int jmin = max(abs(jj - jl), abs(ji - jk)), jmax = min(jj + jl, ji + jk);
-    int phase = sgnp((ji + jj + jk + jl) >> 1);
+// commutators.f90:804-923 rewritten on Z = hL*W - hR*W^T (Z^T = sigma*Z with
+// sigma = -hL*hR): for an upper element (ij,kl;J)
+//   sum1 = -sum_J' (-1)^((jl+jk+J)/2) {ji jj J; jl jk J'} [Z(ki,jl) + ph*sigma*Z(ik,lj)]
+//   sum2 =  sum_J'                    {ji jj J; jk jl J'} [Z(li,jk) + ph*sigma*Z(il,kj)]
+// (Z(ab,cd) = Z[row(a,b)][row(c,d)] of Pandya block J').  Pass 1 adds sum1 over
+// elements sorted by (i,k), pass 2 adds sum2 sorted by (i,l): every Z row is
+// then read by one contiguous run of threads, so Z streams through L2 once.
+__global__ __launch_bounds__(256) void k_ph_pass(DevLayout L, const unsigned *__restrict__ list,
+                                                 long long n, const double *__restrict__ Z,
+                                                 double *__restrict__ RES, double sigma, int pass) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const unsigned code = list[t];
+  const int q = code >> 22, r = (code >> 11) & 2047, s = code & 2047;
+  const int base = L.pbase[q], dim = L.bdim[q], Jtot = L.bJ[q], N = L.nsp, JTM1 = L.jtot_max + 1;
+  const int i = L.pair_a[base + r], j = L.pair_b[base + r];
+  const int k = L.pair_a[base + s], l = L.pair_b[base + s];
+  const int ji = L.jj[i], jj = L.jj[j], jk = L.jj[k], jl = L.jj[l];
+  const int li = L.l[i], lj = L.l[j], lk = L.l[k], ll = L.l[l];
+  const int ti = L.tz[i], tj = L.tz[j], tk = L.tz[k], tl = L.tz[l];
+  const double ph = sgnp((ji + jj + jk + jl) >> 1) * sigma;
+  double sm = 0.0;
+  if (pass == 0) {
+    const int Tz = abs(ti - tk) / 2, PAR = (li + lk) % 2;
+    if (abs(tl - tj) != Tz * 2 || (ll + lj) % 2 != PAR) return;
+    const int jmin = max(abs(jj - jl), abs(ji - jk)), jmax = min(jj + jl, ji + jk);
+    const double sg = sgnp((jl + jk + Jtot) >> 1);
     for (int JP = jmin; JP <= jmax; JP += 2) {
       const int qx = (JP >> 1) + Tz * JTM1 + 2 * PAR * JTM1;
       const int *rowt = L.ccrow + (size_t)qx * N * N;
-      const int rjl = rowt[j * N + l], rik = rowt[i * N + k], gjl = rowt[l * N + j], gik = rowt[k * N + i];
-      if ((rjl | rik | gjl | gik) < 0) continue;
-      const long long rlq = L.ccrlen[qx];
-      const double *Wq = W + L.cwoff[qx];
-      sm = sm - (double)sgnp((jl + jk + Jtot) >> 1) *
-                    ((Wq[gik * rlq + rjl] + phase * Wq[gjl * rlq + rik]) * hL -
-                     (phase * Wq[rik * rlq + gjl] + Wq[rjl * rlq + gik]) * hR) *
-                    dsixj(L, ji, jj, Jtot, jl, jk, JP);
+      const int gik = rowt[k * N + i], rjl = rowt[j * N + l], rik = rowt[i * N + k], gjl = rowt[l * N + j];
+      if ((gik | rjl | rik | gjl) < 0) continue;
+      const long long rl = L.ccrlen[qx];
+      const double *Zq = Z + L.cwoff[qx];
+      sm = sm - sg * (Zq[gik * rl + rjl] + ph * Zq[rik * rl + gjl]) * dsixj(L, ji, jj, Jtot, jl, jk, JP);
     }
-    Tz = abs(ti - tl) / 2;
-    if (abs(tk - tj) != Tz * 2) continue;
-    PAR = (li + ll) % 2;
-    if ((lk + lj) % 2 != PAR) continue;
-    jmin = max(abs(ji - jl), abs(jj - jk));
-    jmax = min(ji + jl, jj + jk);
-    phase = sgnp((ji + jj + jk + jl) >> 1);
+  } else {
+    const int Tz = abs(ti - tl) / 2, PAR = (li + ll) % 2;
+    if (abs(tk - tj) != Tz * 2 || (lk + lj) % 2 != PAR) return;
+    const int jmin = max(abs(ji - jl), abs(jj - jk)), jmax = min(ji + jl, jj + jk);
     for (int JP = jmin; JP <= jmax; JP += 2) {
       const int qx = (JP >> 1) + Tz * JTM1 + 2 * PAR * JTM1;
       const int *rowt = L.ccrow + (size_t)qx * N * N;
-      const int ril = rowt[i * N + l], rjk = rowt[j * N + k], gil = rowt[l * N + i], gjk = rowt[k * N + j];
-      if ((ril | rjk | gil | gjk) < 0) continue;
-      const long long rlq = L.ccrlen[qx];
-      const double *Wq = W + L.cwoff[qx];
-      sm = sm + ((Wq[gil * rlq + rjk] + phase * Wq[gjk * rlq + ril]) * hL -
-                 (phase * Wq[ril * rlq + gjk] + Wq[rjk * rlq + gil]) * hR) *
-                    dsixj(L, ji, jj, Jtot, jk, jl, JP);
+      const int gil = rowt[l * N + i], rjk = rowt[j * N + k], ril = rowt[i * N + l], gjk = rowt[k * N + j];
+      if ((gil | rjk | ril | gjk) < 0) continue;
+      const long long rl = L.ccrlen[qx];
+      const double *Zq = Z + L.cwoff[qx];
+      sm = sm + (Zq[gil * rl + rjk] + ph * Zq[ril * rl + gjk]) * dsixj(L, ji, jj, Jtot, jk, jl, JP);
     }
-    const long long rs = (long long)rl * dim + s;
-    OV[rs] = OV[rs] + sm * pre * pre2;
   }
+  const double pre = (i == j) ? .70710678118 : 1.0, pre2 = (k == l) ? .70710678118 : 1.0;
+  double *OV = RES + L.v_off + L.boff[q];
+  const long long rs = (long long)r * dim + s;
+  OV[rs] = OV[rs] + sm * pre * pre2;
 }
-void launch_ph_gather(hipStream_t s, const DevLayout &L, const double *W, double *RES, double hL,
+void launch_ph_gather(hipStream_t s, const DevLayout &L, const double *Z, double *RES, double hL,
                       double hR) {
-  if (L.npairs <= 0) return;
-  hipLaunchKernelGGL(k_ph_gather, dim3(L.npairs), dim3(64), 0, s, L, W, RES, hL, hR);
+  if (L.nupper <= 0) return;
+  const double sigma = -hL * hR;
+  const int grid = (int)((L.nupper + 255) / 256);
+  hipLaunchKernelGGL(k_ph_pass, dim3(grid), dim3(256), 0, s, L, L.ph_list1, L.nupper, Z, RES, sigma, 0);
+  hipLaunchKernelGGL(k_ph_pass, dim3(grid), dim3(256), 0, s, L, L.ph_list2, L.nupper, Z, RES, sigma, 1);
 }
 
 // ------------------------------------------------------------------ mirror
@@ -401,28 +483,36 @@ __device__ __forceinline__ double w_pair(const DevLayout &L, const double *W, in
   return Wq[(long long)r * dim + s] * pre - pm * (Wq[(long long)s * dim + r] * pre);
 }
 
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// one wavefront per one-body output (p,q); lanes split the inner sums
 __global__ __launch_bounds__(64) void k_onebody(DevLayout L, const double *__restrict__ Lop,
                                                 const double *__restrict__ Rop,
                                                 const double *__restrict__ Wa,
                                                 const double *__restrict__ Wb,
                                                 double *__restrict__ RES, int hL, int hR, int hres,
                                                 unsigned mask) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x;
   if (t >= L.nob) return;
+  const int lane = threadIdx.x;
   const int p = L.ob_p[t], qo = L.ob_q[t], N = L.nsp;
   const double *FL = Lop + L.f_off, *FR = Rop + L.f_off;
   const double *VL = Lop + L.v_off, *VR = Rop + L.v_off;
   double val = 0.0;
   if (mask & T111) {
     double s = 0.0;
-    for (int k = 0; k < N; k++) s += FL[p * N + k] * FR[k * N + qo] - FR[p * N + k] * FL[k * N + qo];
-    val += s;
+    for (int k = lane; k < N; k += 64) s += FL[p * N + k] * FR[k * N + qo] - FR[p * N + k] * FL[k * N + qo];
+    val += wave_sum(s);
   }
   const bool same = same_ljt(L, p, qo);
   if (same && (mask & T121)) {
     const int jq = L.jj[qo];
     double sm = 0.0;
-    for (int e = 0; e < L.nphp; e++) {
+    for (int e = lane; e < L.nphp; e += 64) {
       const int a = L.php_a[e], i = L.php_i[e], ji = L.jj[i];
       double smx = 0.0, smy = 0.0, smx2 = 0.0, smy2 = 0.0;
       for (int JT = abs(ji - jq); JT <= ji + jq; JT += 2) {
@@ -433,33 +523,31 @@ __global__ __launch_bounds__(64) void k_onebody(DevLayout L, const double *__res
       }
       sm = sm + FL[a * N + i] * (hL * smx - smy) - FR[a * N + i] * (hR * smx2 - smy2);
     }
-    val += sm / (L.jj[p] + 1.0);
+    val += wave_sum(sm) / (L.jj[p] + 1.0);
   }
   if (same && (mask & T221)) {
     const double pm = (double)(hL * hR);
     const int ji = L.jj[p];
     double sm = 0.0;
-    for (int pass = 0; pass < 2; pass++)
-      for (int c = 0; c < N; c++) {
-        if (L.con[c] != (pass == 0 ? 1 : 0)) continue;  // holes first, then particles
-        const double *W = (pass == 0) ? Wa : Wb;
-        const int jc = L.jj[c];
-        for (int JT = abs(jc - ji); JT <= jc + ji; JT += 2)
-          sm = sm + w_pair(L, W, c, p, qo, JT, pm) * (JT + 1);
-      }
-    val += sm / (ji + 1.0);
+    for (int c = lane; c < N; c += 64) {
+      const double *W = L.con[c] ? Wa : Wb;  // holes: pp intermediates, particles: hh
+      const int jc = L.jj[c];
+      for (int JT = abs(jc - ji); JT <= jc + ji; JT += 2) sm = sm + w_pair(L, W, c, p, qo, JT, pm) * (JT + 1);
+    }
+    val += wave_sum(sm) / (ji + 1.0);
   }
-  double *F = RES + L.f_off;
-  F[p * N + qo] = val;
-  if (p != qo) F[qo * N + p] = hres * val;
-  else if (hres == -1) F[p * N + p] = 0.0;
+  if (lane == 0) {
+    double *F = RES + L.f_off;
+    F[p * N + qo] = val;
+    if (p != qo) F[qo * N + p] = hres * val;
+    else if (hres == -1) F[p * N + p] = 0.0;
+  }
 }
 void launch_onebody(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop,
                     const double *Wa, const double *Wb, double *RES, int hL, int hR, int hres,
                     unsigned mask) {
   if (L.nob <= 0) return;
-  hipLaunchKernelGGL(k_onebody, dim3((L.nob + 63) / 64), dim3(64), 0, s, L, Lop, Rop, Wa, Wb, RES, hL,
-                     hR, hres, mask);
+  hipLaunchKernelGGL(k_onebody, dim3(L.nob), dim3(64), 0, s, L, Lop, Rop, Wa, Wb, RES, hL, hR, hres, mask);
 }
 
 // ------------------------------------------------------------------ reductions

@@ -4,6 +4,7 @@
 // this library's own (full square channel matrices, dense lookup tables).
 #include "layout.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -220,6 +221,36 @@ std::string Layout::build(int nsp_, const int *n_, const int *l_, const int *jj_
   cccbase[ncc] = (int)cccol_p.size();
   ccsize = coff;
   cwsize = woff;
+  // element lists of the ph recoupling passes
+  {
+    int maxdim = 0;
+    for (int d : bdim) maxdim = std::max(maxdim, d);
+    if (maxdim >= 2048 || nblocks >= 1024) return "channel block too large for the ph element codes";
+    std::vector<std::pair<uint64_t, uint32_t>> e1, e2;
+    size_t nup = 0;
+    for (int qq = 0; qq < nblocks; qq++) nup += (size_t)bdim[qq] * (bdim[qq] + 1) / 2;
+    e1.reserve(nup);
+    e2.reserve(nup);
+    const uint64_t N = nsp;
+    for (int qq = 0; qq < nblocks; qq++) {
+      const int dim = bdim[qq], base = pbase[qq];
+      for (int r = 0; r < dim; r++) {
+        const uint64_t i = pair_a[base + r], j = pair_b[base + r];
+        for (int s2 = r; s2 < dim; s2++) {
+          const uint64_t k = pair_a[base + s2], l = pair_b[base + s2];
+          const uint32_t code = ((uint32_t)qq << 22) | ((uint32_t)r << 11) | (uint32_t)s2;
+          e1.push_back({((((i * N + k) * N + j) * N + l) * nblocks) + qq, code});
+          e2.push_back({((((i * N + l) * N + j) * N + k) * nblocks) + qq, code});
+        }
+      }
+    }
+    std::sort(e1.begin(), e1.end());
+    std::sort(e2.begin(), e2.end());
+    ph_list1.resize(e1.size());
+    ph_list2.resize(e2.size());
+    for (size_t t = 0; t < e1.size(); t++) ph_list1[t] = e1[t].second;
+    for (size_t t = 0; t < e2.size(); t++) ph_list2[t] = e2[t].second;
+  }
   // 6j table
   nhalf = (JTM + 1) / 2;
   nJ = JTM + 1;

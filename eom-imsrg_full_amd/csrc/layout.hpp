@@ -61,6 +61,10 @@ struct Layout {
   std::vector<int> cccbase;     // first global column of block q
   std::vector<int> cccol_p, cccol_h;  // per global column (particle, hole)
 
+  // upper-triangle elements (q<<22 | r<<11 | s) sorted for the two ph passes:
+  // list1 by (i,k,j,l), list2 by (i,l,j,k) for element (ij,kl) of block q
+  std::vector<uint32_t> ph_list1, ph_list2;
+
   // dense 6j table {j1 j2 J3; j4 j5 J6}: j1,j2,j4,j5 half-integer, J3,J6 integer
   int nhalf = 0, nJ = 0;
   std::vector<double> sixj;
