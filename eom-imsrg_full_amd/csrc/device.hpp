@@ -23,8 +23,14 @@ struct DevLayout {
   const long long *ccoff, *cwoff;
   const int *ccrow, *ccrbase, *ccrow_a, *ccrow_b, *cccbase, *cccol_p, *cccol_h, *cccol_q;
   const double *sixj;
-  const unsigned *ph_list1, *ph_list2;  // upper elements sorted by (i,k) and by (i,l)
-  long long nupper;
+  const unsigned *ph_quads;  // (i<<24|j<<16|k<<8|l) of every upper (ij,kl), sorted by (i,k,j,l)
+  long long nquad;
+  const unsigned *ph_terms;  // J' terms: quad<<6 | side<<5 | m, in quadruple order
+  long long nphterm;
+  const unsigned *ph_toff;   // per upper element (row-major per block): first T1 term of its quadruple
+  const long long *ph_ubase; // per global pair row r: index of its first upper element
+  const int4 *mirror_tiles;  // (block, tile row, tile col) with row <= col, 32x32 tiles
+  int nmirror;
   long long f_off, v_off, op_size, vsize;
 };
 
@@ -52,8 +58,8 @@ void launch_122(hipStream_t s, const DevLayout &L, const double *Lop, const doub
 void launch_clear_upper(hipStream_t s, const DevLayout &L, double *RES);
 void launch_pphh_combine(hipStream_t s, const DevLayout &L, const double *Wa, const double *Wb,
                          double *RES, double pm);
-void launch_ph_gather(hipStream_t s, const DevLayout &L, const double *Z, double *RES, double hL,
-                      double hR);
+void launch_ph_gather(hipStream_t s, const DevLayout &L, const double *Z, double *Tbuf, double *RES,
+                      double hL, double hR);
 void launch_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm);
 void launch_onebody(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop,
                     const double *Wa, const double *Wb, double *RES, int hL, int hR, int hres,

@@ -38,6 +38,7 @@ struct imsrg_ctx_s {
   double *wpphh = nullptr;  // W1 | W2, 2 * vsize
   double *cc[2] = {nullptr, nullptr};
   double *wph = nullptr;
+  double *tph = nullptr;    // ph J' terms
   double *partials = nullptr;
   double *scal = nullptr;   // device scalars
   double *hscal = nullptr;  // pinned host scalars
@@ -220,13 +221,29 @@ int imsrg_ctx_create(int nsp, const int *n, const int *l, const int *jj, const i
   UP(Ly.ccrow, ccrow); UP(Ly.ccrbase, ccrbase); UP(Ly.ccrow_a, ccrow_a); UP(Ly.ccrow_b, ccrow_b);
   UP(Ly.cccbase, cccbase); UP(Ly.cccol_p, cccol_p); UP(Ly.cccol_h, cccol_h); UP(cccol_q, cccol_q);
   UP(Ly.sixj, sixj);
-  UP(Ly.ph_list1, ph_list1); UP(Ly.ph_list2, ph_list2);
-  D.nupper = (long long)Ly.ph_list1.size();
+  UP(Ly.ph_quads, ph_quads);
+  D.nquad = (long long)Ly.ph_quads.size();
+  UP(Ly.ph_terms, ph_terms);
+  D.nphterm = (long long)Ly.ph_terms.size();
+  UP(Ly.ph_toff, ph_toff);
+  {
+    std::vector<long long> ub(Ly.ph_ubase.begin(), Ly.ph_ubase.end());
+    UP(ub, ph_ubase);
+  }
+  std::vector<int4> mtiles;
+  for (int q = 0; q < Ly.nblocks; q++) {
+    const int nt = (Ly.bdim[q] + 31) / 32;
+    for (int a = 0; a < nt; a++)
+      for (int b = a; b < nt; b++) mtiles.push_back(make_int4(q, a, b, 0));
+  }
+  D.nmirror = (int)mtiles.size();
+  UP(mtiles, mirror_tiles);
 #undef UP
   if (!rc) rc = dalloc(c, 2 * (size_t)Ly.vsize, &c->wpphh);
   if (!rc) rc = dalloc(c, Ly.ccsize, &c->cc[0]);
   if (!rc) rc = dalloc(c, Ly.ccsize, &c->cc[1]);
   if (!rc) rc = dalloc(c, Ly.cwsize, &c->wph);
+  if (!rc) rc = dalloc(c, Ly.ph_terms.size(), &c->tph);
   if (!rc) rc = dalloc(c, kPartials, &c->partials);
   if (!rc) rc = dalloc(c, 64, &c->scal);
   if (!rc) rc = dalloc(c, (size_t)Ly.nsp * Ly.nsp, &c->mono);
@@ -441,7 +458,7 @@ static int comm_impl(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *R
   if (mask & T122) PROF(c, IMSRG_PROF_122, launch_122(s, D, L->d, R->d, RES->d));
   else launch_clear_upper(s, D, RES->d);
   if (mask & T222PPHH) PROF(c, IMSRG_PROF_COMBINE, launch_pphh_combine(s, D, Wa, Wb, RES->d, (double)(hL * hR)));
-  if (mask & T222PH) PROF(c, IMSRG_PROF_PH_GATHER, launch_ph_gather(s, D, c->wph, RES->d, (double)hL, (double)hR));
+  if (mask & T222PH) PROF(c, IMSRG_PROF_PH_GATHER, launch_ph_gather(s, D, c->wph, c->tph, RES->d, (double)hL, (double)hR));
   PROF(c, IMSRG_PROF_MIRROR, launch_mirror(s, D, RES->d, hres));
   PROF(c, IMSRG_PROF_ONEBODY, launch_onebody(s, D, L->d, R->d, Wa, Wb, RES->d, hL, hR, hres, mask));
   HIPCHK(hipGetLastError());

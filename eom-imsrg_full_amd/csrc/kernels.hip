@@ -27,8 +27,8 @@ __device__ __forceinline__ double dsixj(const DevLayout &L, int j1, int j2, int 
                                         int j6) {
   // table holds 0 for any failed triangle, matching sixj (basic_IMSRG.f90:2509-2555)
   const int h = L.nhalf, nJ = L.nJ;
-  size_t idx = (((((size_t)(j1 >> 1) * h + (j2 >> 1)) * h + (j4 >> 1)) * h + (j5 >> 1)) * nJ +
-                (j3 >> 1)) * nJ + (j6 >> 1);
+  size_t idx = (((((size_t)(j1 >> 1) * h + (j2 >> 1)) * nJ + (j3 >> 1)) * h + (j4 >> 1)) * h +
+                (j5 >> 1)) * nJ + (j6 >> 1);
   return L.sixj[idx];
 }
 
@@ -367,87 +367,118 @@ void launch_pphh_combine(hipStream_t s, const DevLayout &L, const double *Wa, co
 }
 
 // ------------------------------------------------------------------ 222_ph recoupling gather
-// commutators.f90:804-923 rewritten on Z = hL*W - hR*W^T (Z^T = sigma*Z with
-// sigma = -hL*hR): for an upper element (ij,kl;J)
-//   sum1 = -sum_J' (-1)^((jl+jk+J)/2) {ji jj J; jl jk J'} [Z(ki,jl) + ph*sigma*Z(ik,lj)]
-//   sum2 =  sum_J'                    {ji jj J; jk jl J'} [Z(li,jk) + ph*sigma*Z(il,kj)]
-// (Z(ab,cd) = Z[row(a,b)][row(c,d)] of Pandya block J').  Pass 1 adds sum1 over
-// elements sorted by (i,k), pass 2 adds sum2 sorted by (i,l): every Z row is
-// then read by one contiguous run of threads, so Z streams through L2 once.
-__global__ __launch_bounds__(256) void k_ph_pass(DevLayout L, const unsigned *__restrict__ list,
-                                                 long long n, const double *__restrict__ Z,
-                                                 double *__restrict__ RES, double sigma, int pass) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const unsigned code = list[t];
-  const int q = code >> 22, r = (code >> 11) & 2047, s = code & 2047;
-  const int base = L.pbase[q], dim = L.bdim[q], Jtot = L.bJ[q], N = L.nsp, JTM1 = L.jtot_max + 1;
-  const int i = L.pair_a[base + r], j = L.pair_b[base + r];
-  const int k = L.pair_a[base + s], l = L.pair_b[base + s];
+// commutators.f90:804-923 rewritten on Z = hL*W - hR*W^T (Z^T = sigma*Z,
+// sigma = -hL*hR).  For an upper element (ij,kl;J), with Z(ab,cd) =
+// Z[row(a,b)][row(c,d)] of Pandya block J' and ph = (-1)^((ji+jj+jk+jl)/2):
+//   sum1 = -(-1)^((jl+jk+J)/2) sum_J' {ji jj J; jl jk J'} T1(J'),
+//          T1 = Z(ki,jl) + ph*sigma*Z(ik,lj)
+//   sum2 =  sum_J' {ji jj J; jk jl J'} T2(J'),
+//          T2 = sigma*Z(jk,li) + ph*Z(kj,il)           (symmetry applied)
+// T1/T2 depend on the (i,j,k,l) quadruple only, not on J.  Phase A gathers
+// every T once (one thread per J' term, terms ordered by quadruple sorted by
+// (i,k,j,l) so that runs of threads share Z rows); phase B adds the 6j-weighted
+// sums to every upper element, one wavefront per RES row.
+__global__ __launch_bounds__(256) void k_ph_terms(DevLayout L, const double *__restrict__ Z,
+                                                  double *__restrict__ Tbuf, double sigma) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= L.nphterm) return;
+  const unsigned tc = L.ph_terms[t];
+  const unsigned code = L.ph_quads[tc >> 6];
+  const int side = (tc >> 5) & 1, m = tc & 31;
+  const int i = code >> 24, j = (code >> 16) & 255, k = (code >> 8) & 255, l = code & 255;
+  const int N = L.nsp, JTM1 = L.jtot_max + 1;
   const int ji = L.jj[i], jj = L.jj[j], jk = L.jj[k], jl = L.jj[l];
-  const int li = L.l[i], lj = L.l[j], lk = L.l[k], ll = L.l[l];
-  const int ti = L.tz[i], tj = L.tz[j], tk = L.tz[k], tl = L.tz[l];
-  const double ph = sgnp((ji + jj + jk + jl) >> 1) * sigma;
-  double sm = 0.0;
-  if (pass == 0) {
-    const int Tz = abs(ti - tk) / 2, PAR = (li + lk) % 2;
-    if (abs(tl - tj) != Tz * 2 || (ll + lj) % 2 != PAR) return;
-    const int jmin = max(abs(jj - jl), abs(ji - jk)), jmax = min(jj + jl, ji + jk);
-    const double sg = sgnp((jl + jk + Jtot) >> 1);
-    for (int JP = jmin; JP <= jmax; JP += 2) {
-      const int qx = (JP >> 1) + Tz * JTM1 + 2 * PAR * JTM1;
-      const int *rowt = L.ccrow + (size_t)qx * N * N;
-      const int gik = rowt[k * N + i], rjl = rowt[j * N + l], rik = rowt[i * N + k], gjl = rowt[l * N + j];
-      if ((gik | rjl | rik | gjl) < 0) continue;
+  const double ph = sgnp((ji + jj + jk + jl) >> 1);
+  double v = 0.0;
+  if (side == 0) {
+    const int Tz = abs(L.tz[i] - L.tz[k]) / 2, P = (L.l[i] + L.l[k]) % 2;
+    const int JP = max(abs(jj - jl), abs(ji - jk)) + 2 * m;
+    const int qx = (JP >> 1) + Tz * JTM1 + 2 * P * JTM1;
+    const int *rowt = L.ccrow + (size_t)qx * N * N;
+    const int gik = rowt[k * N + i], rjl = rowt[j * N + l], rik = rowt[i * N + k], gjl = rowt[l * N + j];
+    if ((gik | rjl | rik | gjl) >= 0) {
       const long long rl = L.ccrlen[qx];
       const double *Zq = Z + L.cwoff[qx];
-      sm = sm - sg * (Zq[gik * rl + rjl] + ph * Zq[rik * rl + gjl]) * dsixj(L, ji, jj, Jtot, jl, jk, JP);
+      v = Zq[gik * rl + rjl] + ph * sigma * Zq[rik * rl + gjl];
     }
   } else {
-    const int Tz = abs(ti - tl) / 2, PAR = (li + ll) % 2;
-    if (abs(tk - tj) != Tz * 2 || (lk + lj) % 2 != PAR) return;
-    const int jmin = max(abs(ji - jl), abs(jj - jk)), jmax = min(ji + jl, jj + jk);
-    for (int JP = jmin; JP <= jmax; JP += 2) {
-      const int qx = (JP >> 1) + Tz * JTM1 + 2 * PAR * JTM1;
-      const int *rowt = L.ccrow + (size_t)qx * N * N;
-      const int gil = rowt[l * N + i], rjk = rowt[j * N + k], ril = rowt[i * N + l], gjk = rowt[k * N + j];
-      if ((gil | rjk | ril | gjk) < 0) continue;
+    const int Tz = abs(L.tz[i] - L.tz[l]) / 2, P = (L.l[i] + L.l[l]) % 2;
+    const int JP = max(abs(ji - jl), abs(jj - jk)) + 2 * m;
+    const int qx = (JP >> 1) + Tz * JTM1 + 2 * P * JTM1;
+    const int *rowt = L.ccrow + (size_t)qx * N * N;
+    const int rjk = rowt[j * N + k], gil = rowt[l * N + i], gjk = rowt[k * N + j], ril = rowt[i * N + l];
+    if ((rjk | gil | gjk | ril) >= 0) {
       const long long rl = L.ccrlen[qx];
       const double *Zq = Z + L.cwoff[qx];
-      sm = sm + (Zq[gil * rl + rjk] + ph * Zq[ril * rl + gjk]) * dsixj(L, ji, jj, Jtot, jk, jl, JP);
+      v = sigma * Zq[rjk * rl + gil] + ph * Zq[gjk * rl + ril];
     }
   }
-  const double pre = (i == j) ? .70710678118 : 1.0, pre2 = (k == l) ? .70710678118 : 1.0;
-  double *OV = RES + L.v_off + L.boff[q];
-  const long long rs = (long long)r * dim + s;
-  OV[rs] = OV[rs] + sm * pre * pre2;
+  Tbuf[t] = v;
 }
-void launch_ph_gather(hipStream_t s, const DevLayout &L, const double *Z, double *RES, double hL,
-                      double hR) {
-  if (L.nupper <= 0) return;
-  const double sigma = -hL * hR;
-  const int grid = (int)((L.nupper + 255) / 256);
-  hipLaunchKernelGGL(k_ph_pass, dim3(grid), dim3(256), 0, s, L, L.ph_list1, L.nupper, Z, RES, sigma, 0);
-  hipLaunchKernelGGL(k_ph_pass, dim3(grid), dim3(256), 0, s, L, L.ph_list2, L.nupper, Z, RES, sigma, 1);
+
+__global__ __launch_bounds__(64) void k_ph_apply(DevLayout L, const double *__restrict__ Tbuf,
+                                                 double *__restrict__ RES) {
+  const int r = blockIdx.x;
+  if (r >= L.npairs) return;
+  const int q = L.pair_q[r], base = L.pbase[q], rl = r - base, dim = L.bdim[q], J = L.bJ[q];
+  const int i = L.pair_a[r], j = L.pair_b[r];
+  const int ji = L.jj[i], jj = L.jj[j], li = L.l[i], lj = L.l[j], ti = L.tz[i], tj = L.tz[j];
+  const double pre = (i == j) ? .70710678118 : 1.0;
+  double *OV = RES + L.v_off + L.boff[q] + (long long)rl * dim;
+  const unsigned *toff = L.ph_toff + L.ph_ubase[r] - rl;
+  for (int s = rl + (int)threadIdx.x; s < dim; s += 64) {
+    const int k = L.pair_a[base + s], l = L.pair_b[base + s];
+    const int jk = L.jj[k], jl = L.jj[l], lk = L.l[k], ll = L.l[l], tk = L.tz[k], tl = L.tz[l];
+    const double *T1 = Tbuf + toff[s];
+    const int a1 = max(abs(jj - jl), abs(ji - jk)), b1 = min(jj + jl, ji + jk);
+    const int a2 = max(abs(ji - jl), abs(jj - jk)), b2 = min(ji + jl, jj + jk);
+    const double *T2 = T1 + (b1 >= a1 ? (b1 - a1) / 2 + 1 : 0);
+    double s1 = 0.0, s2 = 0.0;
+    if (abs(tl - tj) == abs(ti - tk) && (ll + lj) % 2 == (li + lk) % 2)
+      for (int JP = a1, m = 0; JP <= b1; JP += 2, m++) s1 += dsixj(L, ji, jj, J, jl, jk, JP) * T1[m];
+    if (abs(tk - tj) == abs(ti - tl) && (lk + lj) % 2 == (li + ll) % 2)
+      for (int JP = a2, m = 0; JP <= b2; JP += 2, m++) s2 += dsixj(L, ji, jj, J, jk, jl, JP) * T2[m];
+    const double out = -(double)sgnp((jl + jk + J) >> 1) * s1 + s2;
+    const double pre2 = (k == l) ? .70710678118 : 1.0;
+    OV[s] = OV[s] + out * pre * pre2;
+  }
+}
+void launch_ph_gather(hipStream_t s, const DevLayout &L, const double *Z, double *Tbuf, double *RES,
+                      double hL, double hR) {
+  if (L.nphterm <= 0) return;
+  hipLaunchKernelGGL(k_ph_terms, dim3((unsigned)((L.nphterm + 255) / 256)), dim3(256), 0, s, L, Z, Tbuf,
+                     -hL * hR);
+  hipLaunchKernelGGL(k_ph_apply, dim3(L.npairs), dim3(64), 0, s, L, Tbuf, RES);
 }
 
 // ------------------------------------------------------------------ mirror
-__global__ __launch_bounds__(64) void k_mirror(DevLayout L, double *RES, int herm) {
-  const int r = blockIdx.x;
-  if (r >= L.npairs) return;
-  const int q = L.pair_q[r], rl = r - L.pbase[q], dim = L.bdim[q];
+// lower = herm * upper (diagonal zeroed for herm = -1), 32x32 tiles through LDS
+// so both the upper reads and the lower writes are row-contiguous.
+__global__ __launch_bounds__(256) void k_mirror(DevLayout L, const int4 *__restrict__ tiles, double *RES,
+                                                int herm) {
+  const int4 t = tiles[blockIdx.x];  // x = block, y = tile row, z = tile col (y <= z)
+  const int q = t.x, dim = L.bdim[q];
   double *OV = RES + L.v_off + L.boff[q];
-  for (int s = rl + (int)threadIdx.x; s < dim; s += 64) {
-    if (s == rl) {
-      if (herm == -1) OV[(long long)rl * dim + rl] = 0.0;
-    } else {
-      OV[(long long)s * dim + rl] = herm * OV[(long long)rl * dim + s];
+  __shared__ double T[32][33];
+  const int r0 = t.y * 32, c0 = t.z * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int k = ty; k < 32; k += 8) {
+    const int r = r0 + k, c = c0 + tx;
+    T[k][tx] = (r < dim && c < dim) ? OV[(long long)r * dim + c] : 0.0;
+  }
+  __syncthreads();
+  // element (c, r) of the lower triangle takes herm * upper (r, c)
+  for (int k = ty; k < 32; k += 8) {
+    const int c = c0 + k, r = r0 + tx;  // write row c, column r
+    if (c < dim && r < dim) {
+      if (r < c) OV[(long long)c * dim + r] = herm * T[tx][k];
+      else if (r == c && herm == -1) OV[(long long)c * dim + r] = 0.0;
     }
   }
 }
 void launch_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm) {
-  if (L.npairs <= 0) return;
-  hipLaunchKernelGGL(k_mirror, dim3(L.npairs), dim3(64), 0, s, L, RES, herm);
+  if (L.nmirror <= 0) return;
+  hipLaunchKernelGGL(k_mirror, dim3(L.nmirror), dim3(256), 0, s, L, L.mirror_tiles, RES, herm);
 }
 
 // ------------------------------------------------------------------ one-body (111 + 121 + 221)

@@ -78,7 +78,7 @@ std::string Layout::build(int nsp_, const int *n_, const int *l_, const int *jj_
     jtot_max = jj[i] > jtot_max ? jj[i] : jtot_max;
     lmax = l[i] > lmax ? l[i] : lmax;
   }
-  if (jtot_max > 63) return "2j too large";
+  if (jtot_max > 31) return "2j > 31 is not supported (ph term codes hold 5-bit J' indices)";
   // groups
   grp_of.assign(nsp, -1);
   grp_start.clear();
@@ -221,35 +221,69 @@ std::string Layout::build(int nsp_, const int *n_, const int *l_, const int *jj_
   cccbase[ncc] = (int)cccol_p.size();
   ccsize = coff;
   cwsize = woff;
-  // element lists of the ph recoupling passes
+  // quadruples of the ph recoupling kernel: every (i<=j, k<=l) whose pairs
+  // share a block and whose (ij) precedes (kl) there (that order does not
+  // depend on J: class and in-class order of a pair are J-independent)
   {
-    int maxdim = 0;
-    for (int d : bdim) maxdim = std::max(maxdim, d);
-    if (maxdim >= 2048 || nblocks >= 1024) return "channel block too large for the ph element codes";
-    std::vector<std::pair<uint64_t, uint32_t>> e1, e2;
-    size_t nup = 0;
-    for (int qq = 0; qq < nblocks; qq++) nup += (size_t)bdim[qq] * (bdim[qq] + 1) / 2;
-    e1.reserve(nup);
-    e2.reserve(nup);
+    if (nsp > 256) return "more than 256 orbits: ph quadruple codes need 8-bit orbits";
+    std::vector<uint64_t> keys;
     const uint64_t N = nsp;
     for (int qq = 0; qq < nblocks; qq++) {
-      const int dim = bdim[qq], base = pbase[qq];
+      const int dim = bdim[qq], base = pbase[qq], J = bJ[qq];
       for (int r = 0; r < dim; r++) {
-        const uint64_t i = pair_a[base + r], j = pair_b[base + r];
+        const int i = pair_a[base + r], j = pair_b[base + r];
+        const int Jlo_ij = std::abs(jj[i] - jj[j]);
         for (int s2 = r; s2 < dim; s2++) {
-          const uint64_t k = pair_a[base + s2], l = pair_b[base + s2];
-          const uint32_t code = ((uint32_t)qq << 22) | ((uint32_t)r << 11) | (uint32_t)s2;
-          e1.push_back({((((i * N + k) * N + j) * N + l) * nblocks) + qq, code});
-          e2.push_back({((((i * N + l) * N + j) * N + k) * nblocks) + qq, code});
+          const int k = pair_a[base + s2], l = pair_b[base + s2];
+          // emit once per quadruple: at its lowest J
+          const int Jlo = std::max(Jlo_ij, std::abs(jj[k] - jj[l]));
+          if (J != Jlo) continue;
+          keys.push_back(((((uint64_t)i * N + k) * N + j) * N + l));
         }
       }
     }
-    std::sort(e1.begin(), e1.end());
-    std::sort(e2.begin(), e2.end());
-    ph_list1.resize(e1.size());
-    ph_list2.resize(e2.size());
-    for (size_t t = 0; t < e1.size(); t++) ph_list1[t] = e1[t].second;
-    for (size_t t = 0; t < e2.size(); t++) ph_list2[t] = e2[t].second;
+    std::sort(keys.begin(), keys.end());
+    ph_quads.resize(keys.size());
+    for (size_t t = 0; t < keys.size(); t++) {
+      uint64_t x = keys[t];
+      const uint32_t l = x % N; x /= N;
+      const uint32_t j = x % N; x /= N;
+      const uint32_t k = x % N; x /= N;
+      const uint32_t i = (uint32_t)x;
+      ph_quads[t] = (i << 24) | (j << 16) | (k << 8) | l;
+    }
+    if (ph_quads.size() >= (1u << 26)) return "too many ph quadruples for the term codes";
+    // J' terms in quadruple order; toff of each quadruple
+    std::vector<uint32_t> qtoff(ph_quads.size());
+    ph_terms.clear();
+    for (size_t t = 0; t < ph_quads.size(); t++) {
+      const uint32_t c = ph_quads[t];
+      const int i = c >> 24, j = (c >> 16) & 255, k = (c >> 8) & 255, l = c & 255;
+      const int a1 = std::max(std::abs(jj[j] - jj[l]), std::abs(jj[i] - jj[k])), b1 = std::min(jj[j] + jj[l], jj[i] + jj[k]);
+      const int a2 = std::max(std::abs(jj[i] - jj[l]), std::abs(jj[j] - jj[k])), b2 = std::min(jj[i] + jj[l], jj[j] + jj[k]);
+      qtoff[t] = (uint32_t)ph_terms.size();
+      for (int JP = a1, m = 0; JP <= b1; JP += 2, m++) ph_terms.push_back(((uint32_t)t << 6) | (uint32_t)m);
+      for (int JP = a2, m = 0; JP <= b2; JP += 2, m++) ph_terms.push_back(((uint32_t)t << 6) | 32u | (uint32_t)m);
+    }
+    if (ph_terms.size() >= 0xFFFFFFFFull) return "too many ph terms";
+    // per upper element (block-major, row-major upper triangle): its quadruple's toff
+    ph_ubase.assign(npairs + 1, 0);
+    ph_toff.clear();
+    for (int qq = 0; qq < nblocks; qq++) {
+      const int dim = bdim[qq], base = pbase[qq];
+      for (int r = 0; r < dim; r++) {
+        ph_ubase[base + r] = (int64_t)ph_toff.size();
+        const uint64_t i = pair_a[base + r], j = pair_b[base + r];
+        for (int s2 = r; s2 < dim; s2++) {
+          const uint64_t k = pair_a[base + s2], l = pair_b[base + s2];
+          const uint64_t key = ((i * N + k) * N + j) * N + l;
+          const size_t t = std::lower_bound(keys.begin(), keys.end(), key) - keys.begin();
+          if (t >= keys.size() || keys[t] != key) return "internal: ph quadruple missing";
+          ph_toff.push_back(qtoff[t]);
+        }
+      }
+    }
+    ph_ubase[npairs] = (int64_t)ph_toff.size();
   }
   // 6j table
   nhalf = (JTM + 1) / 2;
@@ -263,7 +297,7 @@ std::string Layout::build(int nsp_, const int *n_, const int *l_, const int *jj_
           for (size_t i5 = 0; i5 < h; i5++)
             for (size_t a3 = 0; a3 < J; a3++)
               for (size_t a6 = 0; a6 < J; a6++)
-                sixj[((((i1 * h + i2) * h + i4) * h + i5) * J + a3) * J + a6] =
+                sixj[((((i1 * h + i2) * J + a3) * h + i4) * h + i5) * J + a6] =
                     racah6j((int)(2 * i1 + 1), (int)(2 * i2 + 1), (int)(2 * a3), (int)(2 * i4 + 1),
                             (int)(2 * i5 + 1), (int)(2 * a6));
   }

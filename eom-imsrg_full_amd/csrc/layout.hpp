@@ -61,11 +61,16 @@ struct Layout {
   std::vector<int> cccbase;     // first global column of block q
   std::vector<int> cccol_p, cccol_h;  // per global column (particle, hole)
 
-  // upper-triangle elements (q<<22 | r<<11 | s) sorted for the two ph passes:
-  // list1 by (i,k,j,l), list2 by (i,l,j,k) for element (ij,kl) of block q
-  std::vector<uint32_t> ph_list1, ph_list2;
+  // (i,j,k,l) quadruples owning at least one upper element (ij,kl;J), packed
+  // i<<24|j<<16|k<<8|l and sorted by (i,k,j,l) for the ph recoupling kernel
+  std::vector<uint32_t> ph_quads;
+  std::vector<uint32_t> ph_terms;   // quad<<6 | side<<5 | m (side 0: T1, 1: T2)
+  std::vector<uint32_t> ph_toff;    // per upper element: offset of its quadruple's T1 terms
+  std::vector<int64_t> ph_ubase;    // per global pair row: first upper element
 
-  // dense 6j table {j1 j2 J3; j4 j5 J6}: j1,j2,j4,j5 half-integer, J3,J6 integer
+  // dense 6j table {j1 j2 J3; j4 j5 J6}: j1,j2,j4,j5 half-integer, J3,J6 integer,
+  // index order [j1][j2][J3][j4][j5][J6] so that the (j4,j5,J6) a wavefront
+  // varies over sit in one small contiguous slab
   int nhalf = 0, nJ = 0;
   std::vector<double> sixj;
 

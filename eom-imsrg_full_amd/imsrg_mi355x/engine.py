@@ -16,7 +16,7 @@ import numpy as np
 from .basis import Basis
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libimsrg_mi355x.so")
+LIB_PATH = os.environ.get("IMSRG_MI355X_LIB", os.path.join(HERE, "lib", "libimsrg_mi355x.so"))
 HEADER = os.path.abspath(os.path.join(HERE, "..", "..", "include", "imsrg_mi355x.h"))
 
 T110, T111, T121, T122, T220, T221, T222PPHH, T222PH = (1 << k for k in range(8))
