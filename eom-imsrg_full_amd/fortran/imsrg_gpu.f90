@@ -1,0 +1,425 @@
+! imsrg_gpu.f90 -- Fortran drop-in for the scalar commutator engine of
+! n-parzuchowski/EOM-IMSRG_FULL, forwarding to libimsrg_mi355x.so through
+! ISO_C_BINDING (the same pattern as the reference's own zlib binding,
+! im-SRG/gzipmod.f90:5-39).
+!
+! Every public routine takes the reference's own derived types (sq_op, spd,
+! cc_mat from basic_IMSRG / cross_coupled) and reproduces the contract of the
+! routine it replaces.  Operators cross the boundary as the reference's
+! vectorize/repackage streams (basic_IMSRG.f90:3794-3918); errors turn into
+! ERROR STOP, as the reference STOPs.
+!
+!   call gpu_init(jbas)                          ! once, after read_sp_basis
+!   call gpu_magnus_decouple(HS, G, jbas, GEN_W2)  ! replaces magnus_decouple
+!   call gpu_BCH_EXPAND(HS, G, H, jbas, 'n')     ! replaces BCH_EXPAND
+!   call gpu_commutator_111(L, R, RES, jbas) ...  ! per-routine replacements
+module imsrg_gpu
+  use iso_c_binding
+  use basic_IMSRG
+  use cross_coupled
+  implicit none
+  private
+
+  integer, parameter, public :: GEN_WHITE = 0, GEN_ATAN = 1, GEN_W2 = 2
+  integer(c_int), parameter :: T110 = 1, T111 = 2, T121 = 4, T122 = 8, T220 = 16, T221 = 32, &
+       T222PPHH = 64, T222PH = 128, TALL = 255
+
+  public :: gpu_init, gpu_finalize
+  public :: gpu_commutator, gpu_commutator_110, gpu_commutator_220, gpu_commutator_111, &
+       gpu_commutator_121, gpu_commutator_122, gpu_commutator_222_pp_hh, gpu_commutator_221, &
+       gpu_commutator_222_ph, gpu_calculate_cross_coupled
+  public :: gpu_BCH_EXPAND, gpu_MAGNUS_EXPAND, gpu_build_generator, gpu_magnus_decouple
+
+  type(c_ptr), save :: ctx = c_null_ptr
+  integer(c_int64_t), save :: neq = 0
+
+  interface
+     function c_last_error() bind(C, name='imsrg_last_error') result(p)
+       import :: c_ptr
+       type(c_ptr) :: p
+     end function c_last_error
+     integer(c_int) function c_ctx_create(nsp, n, l, jj, tz, con, device, out) &
+          bind(C, name='imsrg_ctx_create')
+       import :: c_int, c_ptr
+       integer(c_int), value :: nsp, device
+       integer(c_int) :: n(*), l(*), jj(*), tz(*), con(*)
+       type(c_ptr) :: out
+     end function c_ctx_create
+     integer(c_int) function c_ctx_destroy(c) bind(C, name='imsrg_ctx_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: c
+     end function c_ctx_destroy
+     integer(c_int64_t) function c_ctx_neq(c) bind(C, name='imsrg_ctx_neq')
+       import :: c_int64_t, c_ptr
+       type(c_ptr), value :: c
+     end function c_ctx_neq
+     integer(c_int) function c_op_create(c, herm, out) bind(C, name='imsrg_op_create')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: c
+       integer(c_int), value :: herm
+       type(c_ptr) :: out
+     end function c_op_create
+     integer(c_int) function c_op_destroy(op) bind(C, name='imsrg_op_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: op
+     end function c_op_destroy
+     integer(c_int) function c_op_upload(op, v, n) bind(C, name='imsrg_op_upload')
+       import :: c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: op
+       real(c_double) :: v(*)
+       integer(c_int64_t), value :: n
+     end function c_op_upload
+     integer(c_int) function c_op_download(op, v, n) bind(C, name='imsrg_op_download')
+       import :: c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: op
+       real(c_double) :: v(*)
+       integer(c_int64_t), value :: n
+     end function c_op_download
+     integer(c_int) function c_commutator(c, L, R, RES, mask) bind(C, name='imsrg_commutator')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: c, L, R, RES
+       integer(c_int), value :: mask
+     end function c_commutator
+     integer(c_int) function c_comm110(L, R, out) bind(C, name='imsrg_commutator_110')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: L, R
+       real(c_double) :: out
+     end function c_comm110
+     integer(c_int) function c_comm220(L, R, out) bind(C, name='imsrg_commutator_220')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: L, R
+       real(c_double) :: out
+     end function c_comm220
+     integer(c_int64_t) function c_cc_size(c) bind(C, name='imsrg_cross_coupled_size')
+       import :: c_int64_t, c_ptr
+       type(c_ptr), value :: c
+     end function c_cc_size
+     integer(c_int) function c_cross_coupled(op, out, n) bind(C, name='imsrg_cross_coupled')
+       import :: c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: op
+       real(c_double) :: out(*)
+       integer(c_int64_t), value :: n
+     end function c_cross_coupled
+     integer(c_int) function c_generator(c, kind, H, ETA) bind(C, name='imsrg_generator')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: c, H, ETA
+       integer(c_int), value :: kind
+     end function c_generator
+     integer(c_int) function c_bch(c, HS, G, H, nterms) bind(C, name='imsrg_bch_expand')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: c, HS, G, H
+       integer(c_int) :: nterms
+     end function c_bch
+     integer(c_int) function c_magexp(c, DG, G, nterms) bind(C, name='imsrg_magnus_expand')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: c, DG, G
+       integer(c_int) :: nterms
+     end function c_magexp
+     integer(c_int) function c_flow_create(c, HS, G, kind, ds, out) bind(C, name='imsrg_flow_create')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: c, HS, G
+       integer(c_int), value :: kind
+       real(c_double), value :: ds
+       type(c_ptr) :: out
+     end function c_flow_create
+     integer(c_int) function c_flow_step(f, rec) bind(C, name='imsrg_flow_step')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: f
+       real(c_double) :: rec(8)
+     end function c_flow_step
+     integer(c_int) function c_flow_converged(f, yes) bind(C, name='imsrg_flow_converged')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: f
+       integer(c_int) :: yes
+     end function c_flow_converged
+     integer(c_int) function c_flow_destroy(f) bind(C, name='imsrg_flow_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: f
+     end function c_flow_destroy
+     integer(c_size_t) function c_strlen(s) bind(C, name='strlen')
+       import :: c_size_t, c_ptr
+       type(c_ptr), value :: s
+     end function c_strlen
+  end interface
+
+contains
+
+  subroutine check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: what
+    character(kind=c_char), pointer :: msg(:)
+    type(c_ptr) :: p
+    integer :: n
+    if (rc == 0) return
+    p = c_last_error()
+    n = int(c_strlen(p))
+    call c_f_pointer(p, msg, [n])
+    write(*,*) 'imsrg_gpu: ', what, ' failed (', rc, '): ', msg(1:n)
+    error stop 'imsrg_gpu failure'
+  end subroutine check
+
+  ! -- context: read_sp_basis / allocate_blocks / store_6j replacement
+  subroutine gpu_init(jbas, device)
+    type(spd), intent(in) :: jbas
+    integer, intent(in), optional :: device
+    integer(c_int) :: dev
+    integer(c_int), allocatable :: n(:), l(:), jj(:), tz(:), con(:)
+    if (c_associated(ctx)) return
+    dev = 0
+    if (present(device)) dev = device
+    n = jbas%nn ; l = jbas%ll ; jj = jbas%jj ; tz = jbas%itzp ; con = jbas%con
+    call check(c_ctx_create(int(jbas%total_orbits, c_int), n, l, jj, tz, con, dev, ctx), 'imsrg_ctx_create')
+    neq = c_ctx_neq(ctx)
+  end subroutine gpu_init
+
+  subroutine gpu_finalize()
+    if (c_associated(ctx)) call check(c_ctx_destroy(ctx), 'imsrg_ctx_destroy')
+    ctx = c_null_ptr
+  end subroutine gpu_finalize
+
+  ! -- marshalling through the reference's vectorize/repackage
+  function push(op) result(h)
+    type(sq_op) :: op
+    type(c_ptr) :: h
+    real(8), allocatable :: v(:)
+    op%neq = int(neq)
+    allocate(v(neq))
+    call vectorize(op, v)
+    call check(c_op_create(ctx, int(op%herm, c_int), h), 'imsrg_op_create')
+    call check(c_op_upload(h, v, neq), 'imsrg_op_upload')
+  end function push
+
+  function fresh(herm) result(h)
+    integer, intent(in) :: herm
+    type(c_ptr) :: h
+    call check(c_op_create(ctx, int(herm, c_int), h), 'imsrg_op_create')
+  end function fresh
+
+  subroutine pull(h, op)
+    type(c_ptr) :: h
+    type(sq_op) :: op
+    real(8), allocatable :: v(:)
+    op%neq = int(neq)
+    allocate(v(neq))
+    call check(c_op_download(h, v, neq), 'imsrg_op_download')
+    call repackage(op, v)
+  end subroutine pull
+
+  subroutine drop(h)
+    type(c_ptr) :: h
+    call check(c_op_destroy(h), 'imsrg_op_destroy')
+  end subroutine drop
+
+  ! -- the full commutator sequence of IMSRG_MAGNUS.f90:207-219 (with_e0) or
+  !    :679-685 (MAGNUS_EXPAND, E0 untouched)
+  subroutine gpu_commutator(L, R, RES, with_e0)
+    type(sq_op) :: L, R, RES
+    logical, intent(in) :: with_e0
+    type(c_ptr) :: hl, hr, hres
+    real(8) :: e0keep
+    integer(c_int) :: mask
+    e0keep = RES%E0
+    hl = push(L) ; hr = push(R) ; hres = fresh(RES%herm)
+    mask = TALL
+    if (.not. with_e0) mask = TALL - T110 - T220
+    call check(c_commutator(ctx, hl, hr, hres, mask), 'imsrg_commutator')
+    call pull(hres, RES)
+    if (.not. with_e0) RES%E0 = e0keep
+    call drop(hl) ; call drop(hr) ; call drop(hres)
+  end subroutine gpu_commutator
+
+  real(8) function gpu_commutator_110(L, R, jbas)       ! commutators.f90:10-33
+    type(sq_op) :: L, R
+    type(spd) :: jbas
+    type(c_ptr) :: hl, hr
+    real(c_double) :: x
+    hl = push(L) ; hr = push(R)
+    call check(c_comm110(hl, hr, x), 'imsrg_commutator_110')
+    gpu_commutator_110 = x
+    call drop(hl) ; call drop(hr)
+  end function gpu_commutator_110
+
+  real(8) function gpu_commutator_220(L, R, jbas)       ! commutators.f90:399-432
+    type(sq_op) :: L, R
+    type(spd) :: jbas
+    type(c_ptr) :: hl, hr
+    real(c_double) :: x
+    hl = push(L) ; hr = push(R)
+    call check(c_comm220(hl, hr, x), 'imsrg_commutator_220')
+    gpu_commutator_220 = x
+    call drop(hl) ; call drop(hr)
+  end function gpu_commutator_220
+
+  ! one term of [L,R] into a scratch sq_op shaped like RES
+  subroutine term(L, R, RES, mask, T)
+    type(sq_op) :: L, R, RES, T
+    integer(c_int), intent(in) :: mask
+    type(c_ptr) :: hl, hr, ht
+    call duplicate_sq_op(RES, T)
+    T%herm = RES%herm
+    hl = push(L) ; hr = push(R) ; ht = fresh(RES%herm)
+    call check(c_commutator(ctx, hl, hr, ht, mask), 'imsrg_commutator')
+    call pull(ht, T)
+    call drop(hl) ; call drop(hr) ; call drop(ht)
+  end subroutine term
+
+  subroutine add_onebody(RES, T, overwrite)
+    type(sq_op) :: RES, T
+    logical, intent(in) :: overwrite
+    if (overwrite) then
+       RES%fhh = T%fhh ; RES%fph = T%fph ; RES%fpp = T%fpp
+    else
+       RES%fhh = RES%fhh + T%fhh ; RES%fph = RES%fph + T%fph ; RES%fpp = RES%fpp + T%fpp
+    end if
+  end subroutine add_onebody
+
+  subroutine add_twobody(RES, T, overwrite)
+    type(sq_op) :: RES, T
+    logical, intent(in) :: overwrite
+    integer :: q, g
+    do q = 1, RES%nblocks
+       do g = 1, 6
+          if (overwrite) then
+             RES%mat(q)%gam(g)%X = T%mat(q)%gam(g)%X
+          else
+             RES%mat(q)%gam(g)%X = RES%mat(q)%gam(g)%X + T%mat(q)%gam(g)%X
+          end if
+       end do
+    end do
+  end subroutine add_twobody
+
+  subroutine gpu_commutator_111(L, R, RES, jbas)        ! overwrites the one-body part
+    type(sq_op) :: L, R, RES, T
+    type(spd) :: jbas
+    call term(L, R, RES, T111, T)
+    call add_onebody(RES, T, .true.)
+  end subroutine gpu_commutator_111
+
+  subroutine gpu_commutator_121(L, R, RES, jbas)        ! accumulates one-body
+    type(sq_op) :: L, R, RES, T
+    type(spd) :: jbas
+    call term(L, R, RES, T121, T)
+    call add_onebody(RES, T, .false.)
+  end subroutine gpu_commutator_121
+
+  subroutine gpu_commutator_122(L, R, RES, jbas)        ! overwrites every stored two-body element
+    type(sq_op) :: L, R, RES, T
+    type(spd) :: jbas
+    call term(L, R, RES, T122, T)
+    call add_twobody(RES, T, .true.)
+  end subroutine gpu_commutator_122
+
+  ! accumulates two-body; w1/w2 are kept in the argument list for call-site
+  ! compatibility -- the device keeps its own pp/hh intermediates
+  subroutine gpu_commutator_222_pp_hh(L, R, RES, w1, w2, jbas)
+    type(sq_op) :: L, R, RES, w1, w2, T
+    type(spd) :: jbas
+    call term(L, R, RES, T222PPHH, T)
+    call add_twobody(RES, T, .false.)
+  end subroutine gpu_commutator_222_pp_hh
+
+  subroutine gpu_commutator_221(L, R, RES, w1, w2, jbas)  ! accumulates one-body
+    type(sq_op) :: L, R, RES, w1, w2, T
+    type(spd) :: jbas
+    call term(L, R, RES, T221, T)
+    call add_onebody(RES, T, .false.)
+  end subroutine gpu_commutator_221
+
+  ! the device forms the Pandya matrices of both operands itself, so this takes
+  ! the operators rather than their cc_mat (LCC/RCC of the reference)
+  subroutine gpu_commutator_222_ph(L, R, RES, jbas)     ! accumulates two-body
+    type(sq_op) :: L, R, RES, T
+    type(spd) :: jbas
+    call term(L, R, RES, T222PH, T)
+    call add_twobody(RES, T, .false.)
+  end subroutine gpu_commutator_222_ph
+
+  ! calculate_cross_coupled (cross_coupled.f90:1190-1300) into an allocated cc_mat
+  subroutine gpu_calculate_cross_coupled(HS, CCME, jbas)
+    type(sq_op) :: HS
+    type(cc_mat) :: CCME
+    type(spd) :: jbas
+    type(c_ptr) :: h
+    real(8), allocatable :: v(:)
+    integer(c_int64_t) :: n, k
+    integer :: q, m
+    n = c_cc_size(ctx)
+    allocate(v(max(n, 1_c_int64_t)))
+    h = push(HS)
+    call check(c_cross_coupled(h, v, n), 'imsrg_cross_coupled')
+    call drop(h)
+    k = 0
+    do q = 1, CCME%nblocks
+       m = CCME%rlen(q) * CCME%nph(q)
+       if (m == 0) cycle
+       CCME%CCX(q)%X = reshape(v(k+1:k+m), [CCME%rlen(q), CCME%nph(q)])
+       k = k + m
+    end do
+    CCME%herm = HS%herm
+  end subroutine gpu_calculate_cross_coupled
+
+  ! BCH_EXPAND (IMSRG_MAGNUS.f90:149-234): HS = e^G H e^-G
+  subroutine gpu_BCH_EXPAND(HS, G, H, jbas, quads)
+    type(sq_op) :: HS, G, H
+    type(spd) :: jbas
+    character(1) :: quads
+    type(c_ptr) :: hg, hh, hs_
+    integer(c_int) :: nterms
+    if (quads == 'y') error stop 'imsrg_gpu: quadrupole restoration is not provided'
+    hg = push(G) ; hh = push(H) ; hs_ = fresh(1)
+    call check(c_bch(ctx, hs_, hg, hh, nterms), 'imsrg_bch_expand')
+    HS%herm = H%herm
+    call pull(hs_, HS)
+    call drop(hg) ; call drop(hh) ; call drop(hs_)
+  end subroutine gpu_BCH_EXPAND
+
+  ! MAGNUS_EXPAND (IMSRG_MAGNUS.f90:622-693): DG <- dOmega/ds
+  subroutine gpu_MAGNUS_EXPAND(DG, G, AD, jbas)
+    type(sq_op) :: DG, G, AD
+    type(spd) :: jbas
+    type(c_ptr) :: hdg, hg
+    integer(c_int) :: nterms
+    hdg = push(DG) ; hg = push(G)
+    call check(c_magexp(ctx, hdg, hg, nterms), 'imsrg_magnus_expand')
+    call pull(hdg, DG)
+    call drop(hdg) ; call drop(hg)
+  end subroutine gpu_MAGNUS_EXPAND
+
+  ! build_gs_white / build_gs_atan / build_gs_w2 (generators.f90)
+  subroutine gpu_build_generator(kind, H, ETA)
+    integer, intent(in) :: kind
+    type(sq_op) :: H, ETA
+    type(c_ptr) :: hh, he
+    hh = push(H) ; he = fresh(-1)
+    call check(c_generator(ctx, int(kind, c_int), hh, he), 'imsrg_generator')
+    ETA%herm = -1
+    call pull(he, ETA)
+    call drop(hh) ; call drop(he)
+  end subroutine gpu_build_generator
+
+  ! magnus_decouple (IMSRG_MAGNUS.f90:15-128) with H(s), Omega(s) resident on the
+  ! device for the whole flow; prints the reference's step line
+  subroutine gpu_magnus_decouple(HS, G, jbas, kind)
+    type(sq_op) :: HS, G
+    type(spd) :: jbas
+    integer, intent(in) :: kind
+    type(c_ptr) :: hs_, hg, flow
+    real(c_double) :: rec(8)
+    integer(c_int) :: done
+    if (.not. allocated(G%mat)) call duplicate_sq_op(HS, G)
+    G%herm = -1
+    hs_ = push(HS) ; hg = fresh(-1)
+    call check(c_flow_create(ctx, hs_, hg, int(kind, c_int), 0.5d0, flow), 'imsrg_flow_create')
+    call check(c_flow_converged(flow, done), 'imsrg_flow_converged')
+    do while (done == 0)
+       call check(c_flow_step(flow, rec), 'imsrg_flow_step')
+       write(*,'(I6,4(e15.7))') nint(rec(7)), rec(1), rec(2), rec(3), rec(4)
+       call check(c_flow_converged(flow, done), 'imsrg_flow_converged')
+    end do
+    call check(c_flow_destroy(flow), 'imsrg_flow_destroy')
+    call pull(hs_, HS)
+    call pull(hg, G)
+    call drop(hs_) ; call drop(hg)
+  end subroutine gpu_magnus_decouple
+
+end module imsrg_gpu

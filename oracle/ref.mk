@@ -19,7 +19,9 @@ MODS := gzipmod basic_IMSRG cross_coupled commutators isospin_operators \
         EOM_dTZ_commutators brute_force_testing
 OBJS := $(OUT)/anglib.o $(addprefix $(OUT)/,$(addsuffix .o,$(MODS))) $(OUT)/generators.o
 
-all: $(OUT)/ref_harness
+LIBDIR := $(abspath $(HERE)../eom-imsrg_full_amd/imsrg_mi355x/lib)
+
+all: $(OUT)/ref_harness $(if $(wildcard $(LIBDIR)/libimsrg_mi355x.so),$(OUT)/drop_in_test)
 
 $(OUT):
 	@mkdir -p $(OUT)
@@ -65,8 +67,21 @@ $(OUT)/brute_force_testing.o: $(REFDIR)/brute_force_testing.f90 $(OUT)/EOM_scala
 $(OUT)/generators.o: $(REFDIR)/generators.f90 $(OUT)/commutators.o
 	$(FC) $(FFLAGS) -c -o $@ $< -J$(OUT)
 
-$(OUT)/ref_harness: $(HERE)ref_harness.f90 $(OBJS)
-	$(FC) $(FFLAGS) -o $@ $(HERE)ref_harness.f90 $(OBJS) -J$(OUT) $(LIBS)
+$(OUT)/harness_basis.o: $(HERE)harness_basis.f90 $(OUT)/basic_IMSRG.o
+	$(FC) $(FFLAGS) -c -o $@ $< -J$(OUT)
+
+$(OUT)/ref_harness: $(HERE)ref_harness.f90 $(OBJS) $(OUT)/harness_basis.o
+	$(FC) $(FFLAGS) -o $@ $(HERE)ref_harness.f90 $(OBJS) $(OUT)/harness_basis.o -J$(OUT) $(LIBS)
+
+# the product's Fortran drop-in, compiled against the reference modules, and
+# its test program (reference routines vs drop-in on the same sq_op inputs)
+$(OUT)/imsrg_gpu.o: $(HERE)../eom-imsrg_full_amd/fortran/imsrg_gpu.f90 $(OUT)/cross_coupled.o
+	$(FC) $(FFLAGS) -c -o $@ $< -J$(OUT)
+
+$(OUT)/drop_in_test: $(HERE)../tests/fortran/drop_in_test.f90 $(OBJS) $(OUT)/harness_basis.o $(OUT)/imsrg_gpu.o $(LIBDIR)/libimsrg_mi355x.so
+	$(FC) $(FFLAGS) -o $@ $(HERE)../tests/fortran/drop_in_test.f90 $(OBJS) $(OUT)/harness_basis.o \
+	    $(OUT)/imsrg_gpu.o -J$(OUT) $(LIBS) -L$(LIBDIR) -limsrg_mi355x \
+	    -Wl,-rpath,'$$ORIGIN/../../eom-imsrg_full_amd/imsrg_mi355x/lib'
 
 clean:
 	rm -rf $(OUT)
