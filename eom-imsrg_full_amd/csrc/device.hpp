@@ -49,10 +49,11 @@ enum : unsigned {
 void launch_fill(hipStream_t s, double *p, long long n, double v);
 void launch_axpby(hipStream_t s, long long n, double a, const double *A, double b, const double *B,
                   double *C);
-void launch_gemm_f64(hipStream_t s, bool a_kmajor, const GemmProb *probs, const int4 *tiles,
-                     int ntiles, const double *A, const double *B, double *C);
-void launch_gemm_z(hipStream_t s, const GemmProb *probs, const int4 *tiles, int ntiles, const double *A,
-                   const double *B, double *Z, double hL, double hR);
+// mode: 0 FP64 MFMA, 1 split-FP32 (3 FP32-MFMA products), 2 split-BF16 (6 BF16-MFMA products)
+void launch_gemm_f64(hipStream_t s, int mode, const GemmProb *probs, const int4 *tiles, int ntiles,
+                     const double *A, const double *B, double *C);
+void launch_gemm_z(hipStream_t s, int mode, const GemmProb *probs, const int4 *tiles, int ntiles,
+                   const double *A, const double *B, double *Z, double hL, double hR);
 void launch_pandya(hipStream_t s, const DevLayout &L, const double *op, double *cc);
 void launch_122(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES);
 void launch_clear_upper(hipStream_t s, const DevLayout &L, double *RES);

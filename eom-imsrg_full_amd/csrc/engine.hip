@@ -326,7 +326,8 @@ int imsrg_ctx_info(imsrg_ctx c, double *info) {
 
 int imsrg_ctx_set_precision(imsrg_ctx c, int prec) {
   if (!c) return fail(IMSRG_EINVAL, "null context");
-  if (prec != IMSRG_PREC_FP64 && prec != IMSRG_PREC_SPLIT_FP32) return fail(IMSRG_EINVAL, "unknown precision");
+  if (prec != IMSRG_PREC_FP64 && prec != IMSRG_PREC_SPLIT_FP32 && prec != IMSRG_PREC_SPLIT_BF16)
+    return fail(IMSRG_EINVAL, "unknown precision");
   c->prec = prec;
   return 0;
 }
@@ -445,14 +446,14 @@ static int comm_impl(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *R
   double *Wa = c->wpphh, *Wb = c->wpphh + c->lay.vsize;
   if (mask & (T222PPHH | T221)) {
     PROF(c, IMSRG_PROF_GEMM_PPHH,
-         launch_gemm_f64(s, false, c->pp_probs, c->pp_tiles, c->n_pp_tiles, L->d + D.v_off,
+         launch_gemm_f64(s, c->prec, c->pp_probs, c->pp_tiles, c->n_pp_tiles, L->d + D.v_off,
                          R->d + D.v_off, c->wpphh));
   }
   if (mask & T222PH) {
     if (!lcc_ready) PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, L->d, c->cc[0]));
     PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, R->d, c->cc[1]));
     PROF(c, IMSRG_PROF_GEMM_PH,
-         launch_gemm_z(s, c->ph_probs, c->ph_tiles, c->n_ph_tiles, c->cc[0], c->cc[1], c->wph,
+         launch_gemm_z(s, c->prec, c->ph_probs, c->ph_tiles, c->n_ph_tiles, c->cc[0], c->cc[1], c->wph,
                        (double)hL, (double)hR));
   }
   if (mask & T122) PROF(c, IMSRG_PROF_122, launch_122(s, D, L->d, R->d, RES->d));

@@ -150,15 +150,177 @@ __device__ __forceinline__ void gemm_tile_f64(const GemmProb &p, const double *_
   }
 }
 // local (row, col) of accumulator element (ti, tj, v) of this lane.
-// v_mfma_f64_16x16x4 D layout: D[i][j] with j = lane%16, i = lane/16 + 4v.
+// D[i][j] has j = lane%16 and, for v_mfma_f64_16x16x4, i = lane/16 + 4v; for
+// v_mfma_f32_16x16x4f32 and v_mfma_f32_16x16x32_bf16, i = 4*(lane/16) + v.
+template <int MODE>
 __device__ __forceinline__ int acc_row(int ti, int v) {
-  return ((threadIdx.x >> 6) >> 1) * 32 + ti * 16 + ((threadIdx.x & 63) >> 4) + 4 * v;
+  const int g = (threadIdx.x & 63) >> 4;
+  return ((threadIdx.x >> 6) >> 1) * 32 + ti * 16 + (MODE == 0 ? g + 4 * v : 4 * g + v);
 }
 __device__ __forceinline__ int acc_col(int tj) {
   return ((threadIdx.x >> 6) & 1) * 32 + tj * 16 + (threadIdx.x & 15);
 }
 
+// ---- split-precision tiles: the same 64x64 tile and 2x2 quadrant layout;
+// products accumulate in FP32 inside one K slab and in FP64 across slabs.
+typedef float float4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
 template <bool A_KMAJOR>
+__device__ __forceinline__ double load_a(const GemmProb &p, const double *Ap, int m, int k) {
+  if (m >= p.M || k >= p.K) return 0.0;
+  return A_KMAJOR ? Ap[(long long)k * p.lda + m] : Ap[(long long)m * p.lda + k];
+}
+__device__ __forceinline__ double load_b(const GemmProb &p, const double *Bp, int k, int n) {
+  return (n < p.N && k < p.K) ? Bp[(long long)k * p.ldb + n] : 0.0;
+}
+
+// split-FP32: x = hi + lo (FP32 each); hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x4f32
+template <bool A_KMAJOR>
+__device__ void gemm_tile_s32(const GemmProb &p, const double *__restrict__ Ap, const double *__restrict__ Bp,
+                              int m0, int n0, double4_t acc[2][2]) {
+  __shared__ float Ah[16][65], Al[16][65], Bh[16][65], Bl[16][65];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < p.K; k0 += 16) {
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int idx = tid * 4 + e;  // 0..1023 over a 16 x 64 slab
+      const int kk = idx >> 6, mm = idx & 63;
+      const double a = A_KMAJOR ? load_a<true>(p, Ap, m0 + mm, k0 + kk) : 0.0;
+      const double b = load_b(p, Bp, k0 + kk, n0 + mm);
+      if (A_KMAJOR) {
+        const float ah = (float)a;
+        Ah[kk][mm] = ah;
+        Al[kk][mm] = (float)(a - (double)ah);
+      }
+      const float bh = (float)b;
+      Bh[kk][mm] = bh;
+      Bl[kk][mm] = (float)(b - (double)bh);
+    }
+    if (!A_KMAJOR) {
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int idx = tid * 4 + e;  // rows m, k contiguous in memory
+        const int mm = idx >> 4, kk = idx & 15;
+        const double a = load_a<false>(p, Ap, m0 + mm, k0 + kk);
+        const float ah = (float)a;
+        Ah[kk][mm] = ah;
+        Al[kk][mm] = (float)(a - (double)ah);
+      }
+    }
+    __syncthreads();
+    float4_t c[2][2];
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2; j++) c[i][j] = (float4_t){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kq = 0; kq < 4; kq++) {
+      const int k = kq * 4 + (lane >> 4);
+      float ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        ah[t] = Ah[k][wm * 32 + t * 16 + (lane & 15)];
+        al[t] = Al[k][wm * 32 + t * 16 + (lane & 15)];
+        bh[t] = Bh[k][wn * 32 + t * 16 + (lane & 15)];
+        bl[t] = Bl[k][wn * 32 + t * 16 + (lane & 15)];
+      }
+#pragma unroll
+      for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+        for (int tj = 0; tj < 2; tj++) {
+          c[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(al[ti], bh[tj], c[ti][tj], 0, 0, 0);
+          c[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[ti], bl[tj], c[ti][tj], 0, 0, 0);
+          c[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[ti], bh[tj], c[ti][tj], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+      for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[ti][tj][v] += (double)c[ti][tj][v];
+    __syncthreads();
+  }
+}
+
+// split-BF16: x = hi + mid + lo (BF16 each); the six products down to 2^-24
+// relative (hh, hm, mh, hl, lh, mm) on v_mfma_f32_16x16x32_bf16, K slab 32
+template <bool A_KMAJOR>
+__device__ void gemm_tile_bf16(const GemmProb &p, const double *__restrict__ Ap, const double *__restrict__ Bp,
+                               int m0, int n0, double4_t acc[2][2]) {
+  // [slice][row][k]: a lane reads 8 consecutive k of one row as one 16-byte vector
+  __shared__ __attribute__((aligned(16))) __bf16 As[3][64][40];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[3][64][40];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < p.K; k0 += 32) {
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int idx = tid * 8 + e;  // 0..2047 over a 64 x 32 slab
+      int mm, kk;
+      if (A_KMAJOR) { kk = idx >> 6; mm = idx & 63; } else { mm = idx >> 5; kk = idx & 31; }
+      const double a = load_a<A_KMAJOR>(p, Ap, m0 + mm, k0 + kk);
+      const __bf16 h = (__bf16)a;
+      const double r1 = a - (double)(float)h;
+      const __bf16 md = (__bf16)r1;
+      const __bf16 lo = (__bf16)(r1 - (double)(float)md);
+      As[0][mm][kk] = h;
+      As[1][mm][kk] = md;
+      As[2][mm][kk] = lo;
+      const int kb = idx >> 6, nb = idx & 63;
+      const double b = load_b(p, Bp, k0 + kb, n0 + nb);
+      const __bf16 bh = (__bf16)b;
+      const double s1 = b - (double)(float)bh;
+      const __bf16 bm = (__bf16)s1;
+      const __bf16 bo = (__bf16)(s1 - (double)(float)bm);
+      Bs[0][nb][kb] = bh;
+      Bs[1][nb][kb] = bm;
+      Bs[2][nb][kb] = bo;
+    }
+    __syncthreads();
+    const int kofs = 8 * (lane >> 4);
+    bf16x8_t a[3][2], b[3][2];
+#pragma unroll
+    for (int sl = 0; sl < 3; sl++)
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        a[sl][t] = *(const bf16x8_t *)&As[sl][wm * 32 + t * 16 + (lane & 15)][kofs];
+        b[sl][t] = *(const bf16x8_t *)&Bs[sl][wn * 32 + t * 16 + (lane & 15)][kofs];
+      }
+#pragma unroll
+    for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+      for (int tj = 0; tj < 2; tj++) {
+        float4_t c = {0.f, 0.f, 0.f, 0.f};
+        // smallest terms first so they are not lost against the leading one
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][ti], b[1][tj], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][ti], b[0][tj], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][ti], b[2][tj], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][ti], b[0][tj], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][ti], b[1][tj], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][ti], b[0][tj], c, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[ti][tj][v] += (double)c[v];
+      }
+    __syncthreads();
+  }
+}
+
+template <bool A_KMAJOR, int MODE>
+__device__ __forceinline__ void gemm_tile(const GemmProb &p, const double *__restrict__ Ap,
+                                          const double *__restrict__ Bp, int m0, int n0,
+                                          double (*As)[GBM + GPAD], double (*Bs)[GBN + GPAD],
+                                          double4_t acc[2][2]) {
+  if constexpr (MODE == 0) gemm_tile_f64<A_KMAJOR>(p, Ap, Bp, m0, n0, As, Bs, acc);
+  else if constexpr (MODE == 1) gemm_tile_s32<A_KMAJOR>(p, Ap, Bp, m0, n0, acc);
+  else gemm_tile_bf16<A_KMAJOR>(p, Ap, Bp, m0, n0, acc);
+}
+
+template <bool A_KMAJOR, int MODE>
 __global__ __launch_bounds__(256) void k_gemm_f64(const GemmProb *__restrict__ probs,
                                                   const int4 *__restrict__ tiles, int ntiles,
                                                   const double *__restrict__ A,
@@ -172,7 +334,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmProb *__restrict__ p
   __shared__ double As[GBK][GBM + GPAD];
   __shared__ double Bs[GBK][GBN + GPAD];
   double4_t acc[2][2];
-  gemm_tile_f64<A_KMAJOR>(p, A + p.a_off, B + p.b_off, m0, n0, As, Bs, acc);
+  gemm_tile<A_KMAJOR, MODE>(p, A + p.a_off, B + p.b_off, m0, n0, As, Bs, acc);
   double *Cp = C + p.c_off;
 #pragma unroll
   for (int ti = 0; ti < 2; ti++)
@@ -180,7 +342,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmProb *__restrict__ p
     for (int tj = 0; tj < 2; tj++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int row = m0 + acc_row(ti, v), col = n0 + acc_col(tj);
+        const int row = m0 + acc_row<MODE>(ti, v), col = n0 + acc_col(tj);
         if (row < p.M && col < p.N) Cp[(long long)row * p.ldc + col] = acc[ti][tj][v];
       }
 }
@@ -189,6 +351,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmProb *__restrict__ p
 // Z = hL*W - hR*W^T written directly.  One workgroup per tile pair (tm <= tn):
 // it computes W(tm,tn) and W(tn,tm) and exchanges them through LDS, so Z costs
 // no more MFMA work than W and never needs a separate transpose pass.
+template <int MODE>
 __global__ __launch_bounds__(256) void k_gemm_z(const GemmProb *__restrict__ probs,
                                                 const int4 *__restrict__ tiles, int ntiles,
                                                 const double *__restrict__ A,
@@ -205,16 +368,16 @@ __global__ __launch_bounds__(256) void k_gemm_z(const GemmProb *__restrict__ pro
   double4_t c1[2][2], c2[2][2];
   const double *Ap = A + p.a_off, *Bp = B + p.b_off;
   double *Zp = Z + p.c_off;
-  gemm_tile_f64<true>(p, Ap, Bp, m0, n0, As, Bs, c1);  // W(tm, tn)
+  gemm_tile<true, MODE>(p, Ap, Bp, m0, n0, As, Bs, c1);  // W(tm, tn)
   const bool diag = (tl.y == tl.z);
-  if (!diag) gemm_tile_f64<true>(p, Ap, Bp, n0, m0, As, Bs, c2);  // W(tn, tm)
+  if (!diag) gemm_tile<true, MODE>(p, Ap, Bp, n0, m0, As, Bs, c2);  // W(tn, tm)
   // Z(tm,tn)[r][c] = hL*W(tm,tn)[r][c] - hR*W(tn,tm)[c][r]
 #pragma unroll
   for (int ti = 0; ti < 2; ti++)
 #pragma unroll
     for (int tj = 0; tj < 2; tj++)
 #pragma unroll
-      for (int v = 0; v < 4; v++) T[acc_row(ti, v)][acc_col(tj)] = diag ? c1[ti][tj][v] : c2[ti][tj][v];
+      for (int v = 0; v < 4; v++) T[acc_row<MODE>(ti, v)][acc_col(tj)] = diag ? c1[ti][tj][v] : c2[ti][tj][v];
   __syncthreads();
 #pragma unroll
   for (int ti = 0; ti < 2; ti++)
@@ -222,7 +385,7 @@ __global__ __launch_bounds__(256) void k_gemm_z(const GemmProb *__restrict__ pro
     for (int tj = 0; tj < 2; tj++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int r = acc_row(ti, v), c = acc_col(tj);
+        const int r = acc_row<MODE>(ti, v), c = acc_col(tj);
         const int row = m0 + r, col = n0 + c;
         if (row < p.M && col < p.N) Zp[(long long)row * p.ldc + col] = hL * c1[ti][tj][v] - hR * T[c][r];
       }
@@ -233,7 +396,7 @@ __global__ __launch_bounds__(256) void k_gemm_z(const GemmProb *__restrict__ pro
 #pragma unroll
     for (int tj = 0; tj < 2; tj++)
 #pragma unroll
-      for (int v = 0; v < 4; v++) T[acc_row(ti, v)][acc_col(tj)] = c1[ti][tj][v];
+      for (int v = 0; v < 4; v++) T[acc_row<MODE>(ti, v)][acc_col(tj)] = c1[ti][tj][v];
   __syncthreads();
 #pragma unroll
   for (int ti = 0; ti < 2; ti++)
@@ -241,24 +404,31 @@ __global__ __launch_bounds__(256) void k_gemm_z(const GemmProb *__restrict__ pro
     for (int tj = 0; tj < 2; tj++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int r = acc_row(ti, v), c = acc_col(tj);
+        const int r = acc_row<MODE>(ti, v), c = acc_col(tj);
         const int row = n0 + r, col = m0 + c;
         if (row < p.M && col < p.N) Zp[(long long)row * p.ldc + col] = hL * c2[ti][tj][v] - hR * T[c][r];
       }
 }
-void launch_gemm_z(hipStream_t s, const GemmProb *probs, const int4 *tiles, int ntiles, const double *A,
-                   const double *B, double *Z, double hL, double hR) {
+void launch_gemm_z(hipStream_t s, int mode, const GemmProb *probs, const int4 *tiles, int ntiles,
+                   const double *A, const double *B, double *Z, double hL, double hR) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_gemm_z, dim3(ntiles), dim3(256), 0, s, probs, tiles, ntiles, A, B, Z, hL, hR);
+  if (mode == 1)
+    hipLaunchKernelGGL(k_gemm_z<1>, dim3(ntiles), dim3(256), 0, s, probs, tiles, ntiles, A, B, Z, hL, hR);
+  else if (mode == 2)
+    hipLaunchKernelGGL(k_gemm_z<2>, dim3(ntiles), dim3(256), 0, s, probs, tiles, ntiles, A, B, Z, hL, hR);
+  else
+    hipLaunchKernelGGL(k_gemm_z<0>, dim3(ntiles), dim3(256), 0, s, probs, tiles, ntiles, A, B, Z, hL, hR);
 }
 
-void launch_gemm_f64(hipStream_t s, bool a_kmajor, const GemmProb *probs, const int4 *tiles,
-                     int ntiles, const double *A, const double *B, double *C) {
+void launch_gemm_f64(hipStream_t s, int mode, const GemmProb *probs, const int4 *tiles, int ntiles,
+                     const double *A, const double *B, double *C) {
   if (ntiles <= 0) return;
-  if (a_kmajor)
-    hipLaunchKernelGGL(k_gemm_f64<true>, dim3(ntiles), dim3(256), 0, s, probs, tiles, ntiles, A, B, C);
+  if (mode == 1)
+    hipLaunchKernelGGL((k_gemm_f64<false, 1>), dim3(ntiles), dim3(256), 0, s, probs, tiles, ntiles, A, B, C);
+  else if (mode == 2)
+    hipLaunchKernelGGL((k_gemm_f64<false, 2>), dim3(ntiles), dim3(256), 0, s, probs, tiles, ntiles, A, B, C);
   else
-    hipLaunchKernelGGL(k_gemm_f64<false>, dim3(ntiles), dim3(256), 0, s, probs, tiles, ntiles, A, B, C);
+    hipLaunchKernelGGL((k_gemm_f64<false, 0>), dim3(ntiles), dim3(256), 0, s, probs, tiles, ntiles, A, B, C);
 }
 
 // ------------------------------------------------------------------ Pandya transform

@@ -45,9 +45,11 @@ typedef struct imsrg_op_s *imsrg_op;
 #define IMSRG_T222PH (1u << 7)    /* commutator_222_ph   commutators.f90:771-925 */
 #define IMSRG_TALL 0xFFu
 
-/* GEMM arithmetic of the 222 products */
+/* arithmetic of the 222 GEMMs (pp/hh ladder and Pandya products); everything
+ * else (E0, norms, gathers, generators) always runs in FP64 */
 #define IMSRG_PREC_FP64 0
-#define IMSRG_PREC_SPLIT_FP32 1   /* 2-term split FP32 operands, FP32 MFMA, FP64 combine */
+#define IMSRG_PREC_SPLIT_FP32 1   /* x = hi+lo in FP32; hi*hi+hi*lo+lo*hi on FP32 MFMA, FP64 across K slabs */
+#define IMSRG_PREC_SPLIT_BF16 2   /* x = hi+mid+lo in BF16; 6 products on BF16 MFMA, FP32 in-slab, FP64 across */
 
 /* generator kinds (generators.f90) */
 #define IMSRG_GEN_WHITE 0         /* build_gs_white  generators.f90:4-106   */

@@ -1,0 +1,53 @@
+"""Split-precision GEMM variants of the 222 products against the FP64 path.
+Everything outside the two GEMMs stays FP64, so the only error source is the
+FP32 accumulation inside a K slab (about 2^-24 relative)."""
+import numpy as np
+import pytest
+
+from helpers import rel
+from imsrg_mi355x import Basis, random_operator, synthetic_hamiltonian
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def e4():
+    from imsrg_mi355x import engine as E
+    B = Basis.ho(4, 3)
+    return B, E.Context(B), E
+
+
+@pytest.mark.parametrize("prec", [1, 2], ids=["split32", "bf16"])
+def test_split_commutator_close_to_fp64(e4, prec):
+    B, ctx, E = e4
+    L, R = ctx.op(-1, random_operator(B, -1, 1)), ctx.op(1, random_operator(B, 1, 2))
+    ctx.set_precision(0)
+    ref = ctx.commutator(L, R).download()
+    for mask in (E.T222PPHH, E.T222PH, E.T221, E.TALL):
+        ctx.set_precision(0)
+        want = ctx.commutator(L, R, mask=mask).download()
+        ctx.set_precision(prec)
+        got = ctx.commutator(L, R, mask=mask).download()
+        err = rel(got, want)
+        print(f"[precision {prec}] mask {mask}: rel err {err:.2e}", flush=True)
+        assert err < 2e-6
+        assert err > 0 or mask == E.T221  # the split path really ran
+    ctx.set_precision(0)
+    assert np.array_equal(ctx.commutator(L, R).download(), ref)  # FP64 path restored
+
+
+@pytest.mark.parametrize("prec", [1, 2], ids=["split32", "bf16"])
+def test_split_flow_energy(e4, prec):
+    B, ctx, E = e4
+    H0 = ctx.op(1, synthetic_hamiltonian(B, 0))
+    H0.normal_order()
+    v = H0.download()
+    ctx.set_precision(0)
+    a = ctx.magnus_flow(ctx.op(1, v), "white", 0.5, 200)
+    ctx.set_precision(prec)
+    b = ctx.magnus_flow(ctx.op(1, v), "white", 0.5, 200)
+    ctx.set_precision(0)
+    n = min(a["steps"], b["steps"])
+    dE = np.abs(a["e0"][:n] - b["e0"][:n]).max()
+    print(f"[precision {prec}] emax=4 flow: {a['steps']} vs {b['steps']} steps, max |dE0| {dE:.2e} MeV", flush=True)
+    assert dE < 1e-6
