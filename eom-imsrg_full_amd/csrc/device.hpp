@@ -34,9 +34,19 @@ struct DevLayout {
   long long f_off, v_off, op_size, vsize;
 };
 
-struct GemmProb {
-  int M, N, K, lda, ldb, ldc;
-  long long a_off, b_off, c_off;
+// One K segment of a segmented GEMM: sign * A * B with A taken from operand
+// `src` and B from the other one (offsets in doubles from the operand base).
+struct SegDesc {
+  long long a_off, b_off;
+  int K, lda, ldb, src;
+  double sign;
+};
+// One square output block: C (M x M) = sum of nseg segments; the running sum
+// after the first nmid segments may also be stored (w_off into a side buffer).
+struct SegProb {
+  int M, nseg, nmid, pad;
+  long long c_off, w_off;
+  SegDesc seg[4];
 };
 
 // term bits (mirror the C ABI)
@@ -49,22 +59,27 @@ enum : unsigned {
 void launch_fill(hipStream_t s, double *p, long long n, double v);
 void launch_axpby(hipStream_t s, long long n, double a, const double *A, double b, const double *B,
                   double *C);
-// mode: 0 FP64 MFMA, 1 split-FP32 (3 FP32-MFMA products), 2 split-BF16 (6 BF16-MFMA products)
-void launch_gemm_f64(hipStream_t s, int mode, const GemmProb *probs, const int4 *tiles, int ntiles,
-                     const double *A, const double *B, double *C);
-void launch_gemm_z(hipStream_t s, int mode, const GemmProb *probs, const int4 *tiles, int ntiles,
-                   const double *A, const double *B, double *Z, double hL, double hR);
+// mode: 0 FP64 MFMA, 1 split-FP32 (3 FP32-MFMA products), 2 split-BF16 (6 BF16-MFMA products).
+// 222_pp_hh: RES(upper) += L*D*R - R*D*L per channel block; with W, the pp part
+// (W1 - pm*W1^T) is also stored to W (upper triangle) for commutator_221.
+void launch_seg_gemm_pphh(hipStream_t s, int mode, const SegProb *probs, const int4 *tiles, int ntiles,
+                          const double *X0, const double *X1, double *C, double *W);
+// 222_ph: Z = hL*LCC*RCC^T - hR*RCC*LCC^T per Pandya block, both triangles
+void launch_seg_gemm_z(hipStream_t s, int mode, const SegProb *probs, const int4 *tiles, int ntiles,
+                       const double *LCC, const double *RCC, double *Z, double hL, double hR);
 void launch_pandya(hipStream_t s, const DevLayout &L, const double *op, double *cc);
-void launch_122(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES);
-void launch_clear_upper(hipStream_t s, const DevLayout &L, double *RES);
-void launch_pphh_combine(hipStream_t s, const DevLayout &L, const double *Wa, const double *Wb,
-                         double *RES, double pm);
+// 122: M = A_L R - A_R L over whole channel blocks (diagonal symmetrised);
+// launch_sym_mirror later forms RES = M + herm*M^T
+void launch_122(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES,
+                int hres);
+void launch_clear_blocks(hipStream_t s, const DevLayout &L, double *RES);
 void launch_ph_gather(hipStream_t s, const DevLayout &L, const double *Z, double *Tbuf, double *RES,
                       double hL, double hR);
-void launch_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm);
+// upper += herm * (strict lower)^T, then lower = herm * upper (diagonal zeroed for herm = -1)
+void launch_sym_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm);
+// 111 + 121 + 221 one-body outputs; Wpp = pp intermediate of launch_seg_gemm_pphh
 void launch_onebody(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop,
-                    const double *Wa, const double *Wb, double *RES, int hL, int hR, int hres,
-                    unsigned mask);
+                    const double *Wpp, double *RES, int hL, int hR, int hres, unsigned mask);
 // E0 = 110 + 220 into RES[0]; partial buffer of >= 1024 doubles
 void launch_e0(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES,
                int hL, int hR, unsigned mask, double *partials);

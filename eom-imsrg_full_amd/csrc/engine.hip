@@ -35,7 +35,7 @@ struct imsrg_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
   std::vector<void *> dallocs;
-  double *wpphh = nullptr;  // W1 | W2, 2 * vsize
+  double *wpp = nullptr;    // 221 intermediate W1 - pm*W1^T (upper triangles), vsize
   double *cc[2] = {nullptr, nullptr};
   double *wph = nullptr;
   double *tph = nullptr;    // ph J' terms
@@ -44,7 +44,7 @@ struct imsrg_ctx_s {
   double *hscal = nullptr;  // pinned host scalars
   double *mono = nullptr;
   double *hstage = nullptr; // pinned op_size staging
-  GemmProb *pp_probs = nullptr, *ph_probs = nullptr;
+  SegProb *pp_probs = nullptr, *ph_probs = nullptr;
   int4 *pp_tiles = nullptr, *ph_tiles = nullptr;
   int n_pp_tiles = 0, n_ph_tiles = 0;
   int prec = IMSRG_PREC_FP64;
@@ -113,19 +113,37 @@ static int dalloc(imsrg_ctx_s *c, size_t ndouble, double **out) {
   return 0;
 }
 
-// tile schedule, heaviest problems first; with `pairs`, only tiles (tm <= tn)
-// (each workgroup then owns the mirrored tile too)
-static void build_tiles(const std::vector<GemmProb> &probs, std::vector<int4> &tiles, bool pairs = false) {
-  tiles.clear();
+// Upper-triangle 64x64 tile schedule (tm <= tn) of one segmented-GEMM launch.
+// Workgroup i is dispatched to XCD i % 8, so problems are spread over the 8
+// XCDs by flop weight (heaviest first onto the least-loaded XCD): every tile
+// of one channel block then runs on one XCD and shares its L2, and each XCD's
+// list runs heaviest first.  Short lists are padded with empty slots (x = -1).
+static void build_seg_tiles(const std::vector<SegProb> &probs, std::vector<int4> &tiles) {
+  const int NX = 8;
+  std::vector<double> w(probs.size());
   std::vector<int> order(probs.size());
-  for (size_t i = 0; i < probs.size(); i++) order[i] = (int)i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return probs[a].K > probs[b].K; });
-  for (int pi : order) {
-    const GemmProb &p = probs[pi];
-    int tm = (p.M + 63) / 64, tn = (p.N + 63) / 64;
-    for (int i = 0; i < tm; i++)
-      for (int j = pairs ? i : 0; j < tn; j++) tiles.push_back(make_int4(pi, i, j, 0));
+  for (size_t i = 0; i < probs.size(); i++) {
+    double kt = 0;
+    for (int s = 0; s < probs[i].nseg; s++) kt += probs[i].seg[s].K;
+    const double T = (probs[i].M + 63) / 64;
+    w[i] = kt * T * (T + 1) / 2;
+    order[i] = (int)i;
   }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return w[a] > w[b]; });
+  std::vector<std::vector<int4>> bins(NX);
+  std::vector<double> load(NX, 0.0);
+  for (int pi : order) {
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    const int T = (probs[pi].M + 63) / 64;
+    for (int i = 0; i < T; i++)
+      for (int j = i; j < T; j++) bins[x].push_back(make_int4(pi, i, j, 0));
+    load[x] += w[pi];
+  }
+  size_t len = 0;
+  for (auto &b : bins) len = std::max(len, b.size());
+  tiles.assign(len * NX, make_int4(-1, 0, 0, 0));
+  for (int x = 0; x < NX; x++)
+    for (size_t i = 0; i < bins[x].size(); i++) tiles[i * NX + x] = bins[x][i];
 }
 
 static int op_new(imsrg_ctx_s *c, int herm, imsrg_op_s **out) {
@@ -239,7 +257,7 @@ int imsrg_ctx_create(int nsp, const int *n, const int *l, const int *jj, const i
   D.nmirror = (int)mtiles.size();
   UP(mtiles, mirror_tiles);
 #undef UP
-  if (!rc) rc = dalloc(c, 2 * (size_t)Ly.vsize, &c->wpphh);
+  if (!rc) rc = dalloc(c, (size_t)Ly.vsize, &c->wpp);
   if (!rc) rc = dalloc(c, Ly.ccsize, &c->cc[0]);
   if (!rc) rc = dalloc(c, Ly.ccsize, &c->cc[1]);
   if (!rc) rc = dalloc(c, Ly.cwsize, &c->wph);
@@ -251,37 +269,55 @@ int imsrg_ctx_create(int nsp, const int *n, const int *l, const int *jj, const i
     rc = fail(IMSRG_ENOMEM, "hipHostMalloc failed");
   if (!rc && hipHostMalloc((void **)&c->hstage, Ly.op_size * sizeof(double), hipHostMallocDefault) != hipSuccess)
     rc = fail(IMSRG_ENOMEM, "hipHostMalloc staging failed");
-  // GEMM problem tables
+  // GEMM problem tables.  222_pp_hh per channel block (pairs ordered pp|ph|hh):
+  //   RES += L_pp R_pp - R_pp L_pp - L_hh R_hh + R_hh L_hh, midpoint after the pp part.
+  // 222_ph per Pandya block (column-major r x nb): Z = g0*LCC*RCC^T + g1*RCC*LCC^T.
   if (!rc) {
-    std::vector<GemmProb> pp;
+    std::vector<SegProb> pp;
     for (int q = 0; q < Ly.nblocks; q++) {
-      int dim = Ly.bdim[q];
-      if (dim == 0) continue;
-      int np = Ly.bnp[q], nb = Ly.bnb[q], nh = Ly.bnh[q];
-      long long bo = Ly.boff[q];
-      pp.push_back(GemmProb{dim, dim, np, dim, dim, dim, bo, bo, bo});
-      pp.push_back(GemmProb{dim, dim, nh, dim, dim, dim, bo + np + nb, bo + (long long)(np + nb) * dim,
-                            Ly.vsize + bo});
+      const int dim = Ly.bdim[q], np = Ly.bnp[q], nb = Ly.bnb[q], nh = Ly.bnh[q];
+      if (dim == 0 || np + nh == 0) continue;
+      const long long bo = Ly.boff[q];
+      const long long ah = bo + np + nb, bh = bo + (long long)(np + nb) * dim;
+      SegProb P{};
+      P.M = dim;
+      P.c_off = bo;
+      P.w_off = bo;
+      P.nmid = 2;
+      P.nseg = nh > 0 ? 4 : 2;
+      P.seg[0] = SegDesc{bo, bo, np, dim, dim, 0, 1.0};
+      P.seg[1] = SegDesc{bo, bo, np, dim, dim, 1, -1.0};
+      P.seg[2] = SegDesc{ah, bh, nh, dim, dim, 0, -1.0};
+      P.seg[3] = SegDesc{ah, bh, nh, dim, dim, 1, 1.0};
+      pp.push_back(P);
     }
     std::vector<int4> tiles;
-    build_tiles(pp, tiles);
+    build_seg_tiles(pp, tiles);
     c->n_pp_tiles = (int)tiles.size();
-    rc = upload_table(c, pp, (const GemmProb **)&c->pp_probs);
+    rc = upload_table(c, pp, (const SegProb **)&c->pp_probs);
     if (!rc) rc = upload_table(c, tiles, (const int4 **)&c->pp_tiles);
   }
   if (!rc) {
-    std::vector<GemmProb> ph;
+    std::vector<SegProb> ph;
     for (int q = 0; q < Ly.ncc; q++) {
-      int r = Ly.ccrlen[q];
-      if (r == 0) continue;
-      ph.push_back(GemmProb{r, r, Ly.ccnph[q], r, r, r, Ly.ccoff[q], Ly.ccoff[q], Ly.cwoff[q]});
+      const int r = Ly.ccrlen[q], nbq = Ly.ccnph[q];
+      if (r == 0 || nbq == 0) continue;  // Z stays zero
+      SegProb P{};
+      P.M = r;
+      P.c_off = Ly.cwoff[q];
+      P.w_off = -1;
+      P.nmid = P.nseg = 2;
+      P.seg[0] = SegDesc{Ly.ccoff[q], Ly.ccoff[q], nbq, r, r, 0, 1.0};
+      P.seg[1] = SegDesc{Ly.ccoff[q], Ly.ccoff[q], nbq, r, r, 1, 1.0};
+      ph.push_back(P);
     }
     std::vector<int4> tiles;
-    build_tiles(ph, tiles, true);
+    build_seg_tiles(ph, tiles);
     c->n_ph_tiles = (int)tiles.size();
-    rc = upload_table(c, ph, (const GemmProb **)&c->ph_probs);
+    rc = upload_table(c, ph, (const SegProb **)&c->ph_probs);
     if (!rc) rc = upload_table(c, tiles, (const int4 **)&c->ph_tiles);
   }
+  if (!rc) launch_fill(c->stream, c->wpp, Ly.vsize, 0.0);
   if (!rc && Ly.cwsize > 0) launch_fill(c->stream, c->wph, Ly.cwsize, 0.0);
   if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(IMSRG_EDEVICE, "context setup failed");
   if (rc) {
@@ -443,25 +479,23 @@ static int comm_impl(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *R
   const DevLayout &D = c->dl;
   const int hL = L->herm, hR = R->herm, hres = RES->herm;
   if (mask & (T110 | T220)) PROF(c, IMSRG_PROF_E0, launch_e0(s, D, L->d, R->d, RES->d, hL, hR, mask, c->partials));
-  double *Wa = c->wpphh, *Wb = c->wpphh + c->lay.vsize;
-  if (mask & (T222PPHH | T221)) {
-    PROF(c, IMSRG_PROF_GEMM_PPHH,
-         launch_gemm_f64(s, c->prec, c->pp_probs, c->pp_tiles, c->n_pp_tiles, L->d + D.v_off,
-                         R->d + D.v_off, c->wpphh));
-  }
   if (mask & T222PH) {
     if (!lcc_ready) PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, L->d, c->cc[0]));
     PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, R->d, c->cc[1]));
     PROF(c, IMSRG_PROF_GEMM_PH,
-         launch_gemm_z(s, c->prec, c->ph_probs, c->ph_tiles, c->n_ph_tiles, c->cc[0], c->cc[1], c->wph,
-                       (double)hL, (double)hR));
+         launch_seg_gemm_z(s, c->prec, c->ph_probs, c->ph_tiles, c->n_ph_tiles, c->cc[0], c->cc[1], c->wph,
+                           (double)hL, (double)hR));
   }
-  if (mask & T122) PROF(c, IMSRG_PROF_122, launch_122(s, D, L->d, R->d, RES->d));
-  else launch_clear_upper(s, D, RES->d);
-  if (mask & T222PPHH) PROF(c, IMSRG_PROF_COMBINE, launch_pphh_combine(s, D, Wa, Wb, RES->d, (double)(hL * hR)));
+  if (mask & T122) PROF(c, IMSRG_PROF_122, launch_122(s, D, L->d, R->d, RES->d, hres));
+  else launch_clear_blocks(s, D, RES->d);
+  if (mask & (T222PPHH | T221))
+    PROF(c, IMSRG_PROF_GEMM_PPHH,
+         launch_seg_gemm_pphh(s, c->prec, c->pp_probs, c->pp_tiles, c->n_pp_tiles, L->d + D.v_off,
+                              R->d + D.v_off, (mask & T222PPHH) ? RES->d + D.v_off : nullptr,
+                              (mask & T221) ? c->wpp : nullptr));
   if (mask & T222PH) PROF(c, IMSRG_PROF_PH_GATHER, launch_ph_gather(s, D, c->wph, c->tph, RES->d, (double)hL, (double)hR));
-  PROF(c, IMSRG_PROF_MIRROR, launch_mirror(s, D, RES->d, hres));
-  PROF(c, IMSRG_PROF_ONEBODY, launch_onebody(s, D, L->d, R->d, Wa, Wb, RES->d, hL, hR, hres, mask));
+  PROF(c, IMSRG_PROF_MIRROR, launch_sym_mirror(s, D, RES->d, hres));
+  PROF(c, IMSRG_PROF_ONEBODY, launch_onebody(s, D, L->d, R->d, c->wpp, RES->d, hL, hR, hres, mask));
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -88,67 +88,134 @@ void launch_axpby(hipStream_t s, long long n, double a, const double *A, double 
   hipLaunchKernelGGL(k_axpby, dim3(grid), dim3(256), 0, s, n, a, A, b, B, C);
 }
 
-// ------------------------------------------------------------------ grouped FP64 GEMM
-// C[M x N] = A[M x K] * B[K x N], one 64x64 output tile per workgroup, tiles of
-// all problems in one launch (tile list sorted heaviest first on the host).
-// A is row-major (lda) or, with A_KMAJOR, stored K x M row-major (lda).
-// B is K x N row-major (ldb).  4 wavefronts, each a 32x32 quadrant built from
-// 2x2 v_mfma_f64_16x16x4 tiles; K staged through LDS 16 at a time.
-constexpr int GBM = 64, GBN = 64, GBK = 16, GPAD = 1;
+// ------------------------------------------------------------------ segmented grouped GEMM
+// Every output block C (M x M) is a signed sum of K segments,
+//   C = sum_s sign_s * A_s * B_s,
+// where segment s takes A from X[src] and B from X[1-src] (X0, X1 = the two
+// operands of the commutator, or their Pandya matrices).  A_s is M x K_s,
+// row-major (element (m,k) at m*lda + k) or, with A_KMAJOR, K-major (k*lda + m);
+// B_s is K_s x M with element (k,n) at k*ldb + n.  The outputs are
+// herm-symmetric, so only tiles tm <= tn are launched: one 64x64 tile per
+// workgroup, 4 wavefronts with 32x32 quadrants of 2x2 v_mfma_f64_16x16x4
+// tiles; in a diagonal tile the strictly-lower quadrant does no MFMA work.
+// K is staged through double-buffered LDS slabs of 16 (one barrier per slab,
+// the next slab's global loads in flight during the current slab's MFMAs).
+#ifndef IMSRG_GEMM_BK
+#define IMSRG_GEMM_BK 16
+#endif
+constexpr int GT = 64, GK = IMSRG_GEMM_BK, GP = 1, GJ = GK / 4;
+static_assert(GK == 16 || GK == 32, "K slab of 16 or 32");
 
-// one 64x64 tile of C = A*B into acc (per-lane 2x2 MFMA 16x16 quadrants)
-template <bool A_KMAJOR>
-__device__ __forceinline__ void gemm_tile_f64(const GemmProb &p, const double *__restrict__ Ap,
-                                              const double *__restrict__ Bp, int m0, int n0,
-                                              double (*As)[GBM + GPAD], double (*Bs)[GBN + GPAD],
-                                              double4_t acc[2][2]) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+struct GemmSmem {
+  double A[2][GK][GT + GP];
+  double B[2][GK][GT + GP];
+};
+static_assert(sizeof(GemmSmem) >= sizeof(double) * GT * (GT + GP), "transpose buffer must fit");
+
+// `need` bit (2*ti + tj): the 16x16 sub-tile holds outputs that are used
+// (inside the block and not strictly below the diagonal); others do no MFMA.
+__device__ __forceinline__ void slab_mfma(const GemmSmem &S, int buf, double4_t acc[2][2], unsigned need) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  for (int i = 0; i < 2; i++)
-    for (int j = 0; j < 2; j++) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < p.K; k0 += GBK) {
-    if (A_KMAJOR) {
-      const int kk = tid >> 4, mm = (tid & 15) * 4;
-      const int k = k0 + kk;
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int m = m0 + mm + j;
-        As[kk][mm + j] = (m < p.M && k < p.K) ? Ap[(long long)k * p.lda + m] : 0.0;
-      }
-    } else {
-      const int mm = tid >> 2, kk = (tid & 3) * 4;
-      const int m = m0 + mm;
+  for (int kq = 0; kq < GK / 4; kq++) {
+    const int k = kq * 4 + (lane >> 4);
+    const double a0 = S.A[buf][k][wm * 32 + (lane & 15)];
+    const double a1 = S.A[buf][k][wm * 32 + 16 + (lane & 15)];
+    const double b0 = S.B[buf][k][wn * 32 + (lane & 15)];
+    const double b1 = S.B[buf][k][wn * 32 + 16 + (lane & 15)];
+    if (need & 1) acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    if (need & 2) acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    if (need & 4) acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    if (need & 8) acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+  }
+}
+
+// acc += sgn * A * B over one segment, FP64 MFMA.  Each thread owns 4 A and
+// 4 B elements of every slab at fixed (row, k-in-slab) positions, kept as
+// 32-bit offsets from the segment base that advance by one slab per step.
+// Loads are issued unconditionally (an out-of-range element reads offset 0,
+// the segment's first element) and the in-range flags are applied, with the
+// sign, only when the slab is written to LDS after the current slab's MFMAs,
+// so the next slab's loads stay in flight meanwhile.
+template <bool A_KMAJOR>
+__device__ void seg_f64(const SegDesc &sgd, double sgn, int M, const double *__restrict__ A,
+                        const double *__restrict__ B, int m0, int n0, GemmSmem &S, double4_t acc[2][2],
+                        unsigned need) {
+  const int K = sgd.K, lda = sgd.lda, ldb = sgd.ldb;
+  const int nk = (K + GK - 1) / GK;
+  if (nk == 0) return;
+  const int tid = threadIdx.x;
+  int aoff[GJ], boff[GJ], akk[GJ], bkk[GJ];
+  unsigned rowok = 0;  // bits 0..GJ-1: A rows in range, bits 16..: B columns in range
+  if (A_KMAJOR) {
+    const int m = m0 + (tid & 63);
+    const bool mv = m < M;
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int k = k0 + kk + j;
-        As[kk + j][mm] = (m < p.M && k < p.K) ? Ap[(long long)m * p.lda + k] : 0.0;
-      }
+    for (int j = 0; j < GJ; j++) {
+      akk[j] = (tid >> 6) + 4 * j;
+      aoff[j] = akk[j] * lda + (mv ? m : 0);
+      rowok |= (unsigned)mv << j;
     }
-    {
-      const int kk = tid >> 4, nn = (tid & 15) * 4;
-      const int k = k0 + kk;
+  } else {
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int n = n0 + nn + j;
-        Bs[kk][nn + j] = (n < p.N && k < p.K) ? Bp[(long long)k * p.ldb + n] : 0.0;
-      }
+    for (int j = 0; j < GJ; j++) {
+      const int m = m0 + tid / GK + (256 / GK) * j;
+      const bool mv = m < M;
+      akk[j] = tid % GK;
+      aoff[j] = (mv ? m : 0) * lda + akk[j];
+      rowok |= (unsigned)mv << j;
     }
-    __syncthreads();
+  }
+  {
+    const int n = n0 + (tid & 63);
+    const bool nv = n < M;
 #pragma unroll
-    for (int kq = 0; kq < GBK / 4; kq++) {
-      const int k = kq * 4 + (lane >> 4);
-      const double a0 = As[k][wm * 32 + (lane & 15)];
-      const double a1 = As[k][wm * 32 + 16 + (lane & 15)];
-      const double b0 = Bs[k][wn * 32 + (lane & 15)];
-      const double b1 = Bs[k][wn * 32 + 16 + (lane & 15)];
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    for (int j = 0; j < GJ; j++) {
+      bkk[j] = (tid >> 6) + 4 * j;
+      boff[j] = bkk[j] * ldb + (nv ? n : 0);
+      rowok |= (unsigned)nv << (16 + j);
     }
+  }
+  const int ainc = A_KMAJOR ? GK * lda : GK, binc = GK * ldb;
+  double ra[GJ], rb[GJ];
+  unsigned ok = 0;
+  auto load = [&](int k0) {
+    ok = 0;
+#pragma unroll
+    for (int j = 0; j < GJ; j++) {
+      const bool ga = ((rowok >> j) & 1) && (k0 + akk[j] < K);
+      const bool gb = ((rowok >> (16 + j)) & 1) && (k0 + bkk[j] < K);
+      ra[j] = A[ga ? aoff[j] : 0];
+      rb[j] = B[gb ? boff[j] : 0];
+      ok |= ((unsigned)ga << j) | ((unsigned)gb << (16 + j));
+      aoff[j] += ainc;
+      boff[j] += binc;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < GJ; j++) {
+      const double a = ((ok >> j) & 1) ? ra[j] : 0.0;
+      const double b = ((ok >> (16 + j)) & 1) ? sgn * rb[j] : 0.0;
+      if (A_KMAJOR) S.A[buf][(tid >> 6) + 4 * j][tid & 63] = a;
+      else S.A[buf][tid % GK][tid / GK + (256 / GK) * j] = a;
+      S.B[buf][(tid >> 6) + 4 * j][tid & 63] = b;
+    }
+  };
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; kt++) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nk;
+    if (more) load((kt + 1) * GK);
+    if (need) slab_mfma(S, cur, acc, need);
+    if (more) store(cur ^ 1);
     __syncthreads();
   }
 }
+
 // local (row, col) of accumulator element (ti, tj, v) of this lane.
 // D[i][j] has j = lane%16 and, for v_mfma_f64_16x16x4, i = lane/16 + 4v; for
 // v_mfma_f32_16x16x4f32 and v_mfma_f32_16x16x32_bf16, i = 4*(lane/16) + v.
@@ -161,41 +228,40 @@ __device__ __forceinline__ int acc_col(int tj) {
   return ((threadIdx.x >> 6) & 1) * 32 + tj * 16 + (threadIdx.x & 15);
 }
 
-// ---- split-precision tiles: the same 64x64 tile and 2x2 quadrant layout;
+// ---- split-precision segments: the same 64x64 tile and 2x2 quadrant layout;
 // products accumulate in FP32 inside one K slab and in FP64 across slabs.
 typedef float float4_t __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 
 template <bool A_KMAJOR>
-__device__ __forceinline__ double load_a(const GemmProb &p, const double *Ap, int m, int k) {
-  if (m >= p.M || k >= p.K) return 0.0;
-  return A_KMAJOR ? Ap[(long long)k * p.lda + m] : Ap[(long long)m * p.lda + k];
+__device__ __forceinline__ double seg_a(const SegDesc &sg, int M, const double *A, int m, int k) {
+  if (m >= M || k >= sg.K) return 0.0;
+  return A_KMAJOR ? A[(long long)k * sg.lda + m] : A[(long long)m * sg.lda + k];
 }
-__device__ __forceinline__ double load_b(const GemmProb &p, const double *Bp, int k, int n) {
-  return (n < p.N && k < p.K) ? Bp[(long long)k * p.ldb + n] : 0.0;
+__device__ __forceinline__ double seg_b(const SegDesc &sg, int M, const double *B, int k, int n) {
+  return (n < M && k < sg.K) ? B[(long long)k * sg.ldb + n] : 0.0;
 }
 
 // split-FP32: x = hi + lo (FP32 each); hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x4f32
 template <bool A_KMAJOR>
-__device__ void gemm_tile_s32(const GemmProb &p, const double *__restrict__ Ap, const double *__restrict__ Bp,
-                              int m0, int n0, double4_t acc[2][2]) {
+__device__ void seg_s32(const SegDesc &sg, double sgn, int M, const double *__restrict__ A,
+                        const double *__restrict__ B, int m0, int n0, double4_t acc[2][2], unsigned need) {
+  const bool active = need != 0;
   __shared__ float Ah[16][65], Al[16][65], Bh[16][65], Bl[16][65];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  for (int i = 0; i < 2; i++)
-    for (int j = 0; j < 2; j++) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < p.K; k0 += 16) {
+  for (int k0 = 0; k0 < sg.K; k0 += 16) {
 #pragma unroll
     for (int e = 0; e < 4; e++) {
       const int idx = tid * 4 + e;  // 0..1023 over a 16 x 64 slab
       const int kk = idx >> 6, mm = idx & 63;
-      const double a = A_KMAJOR ? load_a<true>(p, Ap, m0 + mm, k0 + kk) : 0.0;
-      const double b = load_b(p, Bp, k0 + kk, n0 + mm);
       if (A_KMAJOR) {
+        const double a = seg_a<true>(sg, M, A, m0 + mm, k0 + kk);
         const float ah = (float)a;
         Ah[kk][mm] = ah;
         Al[kk][mm] = (float)(a - (double)ah);
       }
+      const double b = sgn * seg_b(sg, M, B, k0 + kk, n0 + mm);
       const float bh = (float)b;
       Bh[kk][mm] = bh;
       Bl[kk][mm] = (float)(b - (double)bh);
@@ -205,42 +271,44 @@ __device__ void gemm_tile_s32(const GemmProb &p, const double *__restrict__ Ap, 
       for (int e = 0; e < 4; e++) {
         const int idx = tid * 4 + e;  // rows m, k contiguous in memory
         const int mm = idx >> 4, kk = idx & 15;
-        const double a = load_a<false>(p, Ap, m0 + mm, k0 + kk);
+        const double a = seg_a<false>(sg, M, A, m0 + mm, k0 + kk);
         const float ah = (float)a;
         Ah[kk][mm] = ah;
         Al[kk][mm] = (float)(a - (double)ah);
       }
     }
     __syncthreads();
-    float4_t c[2][2];
-    for (int i = 0; i < 2; i++)
-      for (int j = 0; j < 2; j++) c[i][j] = (float4_t){0.f, 0.f, 0.f, 0.f};
+    if (active) {
+      float4_t c[2][2];
+      for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) c[i][j] = (float4_t){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kq = 0; kq < 4; kq++) {
-      const int k = kq * 4 + (lane >> 4);
-      float ah[2], al[2], bh[2], bl[2];
+      for (int kq = 0; kq < 4; kq++) {
+        const int k = kq * 4 + (lane >> 4);
+        float ah[2], al[2], bh[2], bl[2];
 #pragma unroll
-      for (int t = 0; t < 2; t++) {
-        ah[t] = Ah[k][wm * 32 + t * 16 + (lane & 15)];
-        al[t] = Al[k][wm * 32 + t * 16 + (lane & 15)];
-        bh[t] = Bh[k][wn * 32 + t * 16 + (lane & 15)];
-        bl[t] = Bl[k][wn * 32 + t * 16 + (lane & 15)];
+        for (int t = 0; t < 2; t++) {
+          ah[t] = Ah[k][wm * 32 + t * 16 + (lane & 15)];
+          al[t] = Al[k][wm * 32 + t * 16 + (lane & 15)];
+          bh[t] = Bh[k][wn * 32 + t * 16 + (lane & 15)];
+          bl[t] = Bl[k][wn * 32 + t * 16 + (lane & 15)];
+        }
+#pragma unroll
+        for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+          for (int tj = 0; tj < 2; tj++) {
+            c[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(al[ti], bh[tj], c[ti][tj], 0, 0, 0);
+            c[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[ti], bl[tj], c[ti][tj], 0, 0, 0);
+            c[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[ti], bh[tj], c[ti][tj], 0, 0, 0);
+          }
       }
 #pragma unroll
       for (int ti = 0; ti < 2; ti++)
 #pragma unroll
-        for (int tj = 0; tj < 2; tj++) {
-          c[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(al[ti], bh[tj], c[ti][tj], 0, 0, 0);
-          c[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[ti], bl[tj], c[ti][tj], 0, 0, 0);
-          c[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[ti], bh[tj], c[ti][tj], 0, 0, 0);
-        }
+        for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) acc[ti][tj][v] += (double)c[ti][tj][v];
     }
-#pragma unroll
-    for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-      for (int tj = 0; tj < 2; tj++)
-#pragma unroll
-        for (int v = 0; v < 4; v++) acc[ti][tj][v] += (double)c[ti][tj][v];
     __syncthreads();
   }
 }
@@ -248,22 +316,21 @@ __device__ void gemm_tile_s32(const GemmProb &p, const double *__restrict__ Ap, 
 // split-BF16: x = hi + mid + lo (BF16 each); the six products down to 2^-24
 // relative (hh, hm, mh, hl, lh, mm) on v_mfma_f32_16x16x32_bf16, K slab 32
 template <bool A_KMAJOR>
-__device__ void gemm_tile_bf16(const GemmProb &p, const double *__restrict__ Ap, const double *__restrict__ Bp,
-                               int m0, int n0, double4_t acc[2][2]) {
+__device__ void seg_bf16(const SegDesc &sg, double sgn, int M, const double *__restrict__ A,
+                         const double *__restrict__ B, int m0, int n0, double4_t acc[2][2], unsigned need) {
+  const bool active = need != 0;
   // [slice][row][k]: a lane reads 8 consecutive k of one row as one 16-byte vector
   __shared__ __attribute__((aligned(16))) __bf16 As[3][64][40];
   __shared__ __attribute__((aligned(16))) __bf16 Bs[3][64][40];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  for (int i = 0; i < 2; i++)
-    for (int j = 0; j < 2; j++) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < p.K; k0 += 32) {
+  for (int k0 = 0; k0 < sg.K; k0 += 32) {
 #pragma unroll
     for (int e = 0; e < 8; e++) {
       const int idx = tid * 8 + e;  // 0..2047 over a 64 x 32 slab
       int mm, kk;
       if (A_KMAJOR) { kk = idx >> 6; mm = idx & 63; } else { mm = idx >> 5; kk = idx & 31; }
-      const double a = load_a<A_KMAJOR>(p, Ap, m0 + mm, k0 + kk);
+      const double a = seg_a<A_KMAJOR>(sg, M, A, m0 + mm, k0 + kk);
       const __bf16 h = (__bf16)a;
       const double r1 = a - (double)(float)h;
       const __bf16 md = (__bf16)r1;
@@ -272,7 +339,7 @@ __device__ void gemm_tile_bf16(const GemmProb &p, const double *__restrict__ Ap,
       As[1][mm][kk] = md;
       As[2][mm][kk] = lo;
       const int kb = idx >> 6, nb = idx & 63;
-      const double b = load_b(p, Bp, k0 + kb, n0 + nb);
+      const double b = sgn * seg_b(sg, M, B, k0 + kb, n0 + nb);
       const __bf16 bh = (__bf16)b;
       const double s1 = b - (double)(float)bh;
       const __bf16 bm = (__bf16)s1;
@@ -282,153 +349,164 @@ __device__ void gemm_tile_bf16(const GemmProb &p, const double *__restrict__ Ap,
       Bs[2][nb][kb] = bo;
     }
     __syncthreads();
-    const int kofs = 8 * (lane >> 4);
-    bf16x8_t a[3][2], b[3][2];
+    if (active) {
+      const int kofs = 8 * (lane >> 4);
+      bf16x8_t a[3][2], b[3][2];
 #pragma unroll
-    for (int sl = 0; sl < 3; sl++)
+      for (int sl = 0; sl < 3; sl++)
 #pragma unroll
-      for (int t = 0; t < 2; t++) {
-        a[sl][t] = *(const bf16x8_t *)&As[sl][wm * 32 + t * 16 + (lane & 15)][kofs];
-        b[sl][t] = *(const bf16x8_t *)&Bs[sl][wn * 32 + t * 16 + (lane & 15)][kofs];
-      }
+        for (int t = 0; t < 2; t++) {
+          a[sl][t] = *(const bf16x8_t *)&As[sl][wm * 32 + t * 16 + (lane & 15)][kofs];
+          b[sl][t] = *(const bf16x8_t *)&Bs[sl][wn * 32 + t * 16 + (lane & 15)][kofs];
+        }
 #pragma unroll
-    for (int ti = 0; ti < 2; ti++)
+      for (int ti = 0; ti < 2; ti++)
 #pragma unroll
-      for (int tj = 0; tj < 2; tj++) {
-        float4_t c = {0.f, 0.f, 0.f, 0.f};
-        // smallest terms first so they are not lost against the leading one
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][ti], b[1][tj], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][ti], b[0][tj], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][ti], b[2][tj], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][ti], b[0][tj], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][ti], b[1][tj], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][ti], b[0][tj], c, 0, 0, 0);
+        for (int tj = 0; tj < 2; tj++) {
+          float4_t c = {0.f, 0.f, 0.f, 0.f};
+          // smallest terms first so they are not lost against the leading one
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][ti], b[1][tj], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][ti], b[0][tj], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][ti], b[2][tj], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][ti], b[0][tj], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][ti], b[1][tj], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][ti], b[0][tj], c, 0, 0, 0);
 #pragma unroll
-        for (int v = 0; v < 4; v++) acc[ti][tj][v] += (double)c[v];
-      }
+          for (int v = 0; v < 4; v++) acc[ti][tj][v] += (double)c[v];
+        }
+    }
     __syncthreads();
   }
 }
 
 template <bool A_KMAJOR, int MODE>
-__device__ __forceinline__ void gemm_tile(const GemmProb &p, const double *__restrict__ Ap,
-                                          const double *__restrict__ Bp, int m0, int n0,
-                                          double (*As)[GBM + GPAD], double (*Bs)[GBN + GPAD],
-                                          double4_t acc[2][2]) {
-  if constexpr (MODE == 0) gemm_tile_f64<A_KMAJOR>(p, Ap, Bp, m0, n0, As, Bs, acc);
-  else if constexpr (MODE == 1) gemm_tile_s32<A_KMAJOR>(p, Ap, Bp, m0, n0, acc);
-  else gemm_tile_bf16<A_KMAJOR>(p, Ap, Bp, m0, n0, acc);
+__device__ __forceinline__ void run_segments(const SegProb &p, int s0, int s1, const double *X0,
+                                             const double *X1, double g0, double g1, int m0, int n0,
+                                             GemmSmem &S, double4_t acc[2][2], unsigned need) {
+  for (int s = s0; s < s1; s++) {
+    const SegDesc &sg = p.seg[s];
+    const double *A = (sg.src ? X1 : X0) + sg.a_off;
+    const double *B = (sg.src ? X0 : X1) + sg.b_off;
+    const double sgn = sg.sign * (sg.src ? g1 : g0);
+    if constexpr (MODE == 0) seg_f64<A_KMAJOR>(sg, sgn, p.M, A, B, m0, n0, S, acc, need);
+    else if constexpr (MODE == 1) seg_s32<A_KMAJOR>(sg, sgn, p.M, A, B, m0, n0, acc, need);
+    else seg_bf16<A_KMAJOR>(sg, sgn, p.M, A, B, m0, n0, acc, need);
+  }
 }
 
-template <bool A_KMAJOR, int MODE>
-__global__ __launch_bounds__(256) void k_gemm_f64(const GemmProb *__restrict__ probs,
+// EPI 0 (222_pp_hh): C += acc on the upper triangle; with W, the accumulator
+//   after the first nmid segments is also stored to W (upper triangle).
+// EPI 1 (Pandya product): Z = acc on the whole tile and sigma * acc^T on the
+//   mirrored tile, both written row-contiguously through an LDS transpose.
+template <bool A_KMAJOR, int MODE, int EPI>
+__global__ __launch_bounds__(256) void k_seg_gemm(const SegProb *__restrict__ probs,
                                                   const int4 *__restrict__ tiles, int ntiles,
-                                                  const double *__restrict__ A,
-                                                  const double *__restrict__ B,
-                                                  double *__restrict__ C) {
+                                                  const double *__restrict__ X0,
+                                                  const double *__restrict__ X1, double g0, double g1,
+                                                  double *__restrict__ C, double *__restrict__ W,
+                                                  double sigma) {
   const int t = blockIdx.x;
   if (t >= ntiles) return;
   const int4 tl = tiles[t];
-  const GemmProb p = probs[tl.x];
-  const int m0 = tl.y * GBM, n0 = tl.z * GBN;
-  __shared__ double As[GBK][GBM + GPAD];
-  __shared__ double Bs[GBK][GBN + GPAD];
+  if (tl.x < 0) return;  // padding slot of the XCD-balanced schedule
+  const SegProb &p = probs[tl.x];
+  const int m0 = tl.y * GT, n0 = tl.z * GT, M = p.M;
+  const bool diag = tl.y == tl.z;
+  const int wave = threadIdx.x >> 6;
+  unsigned need = 0;
+#pragma unroll
+  for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+    for (int tj = 0; tj < 2; tj++) {
+      const int r16 = m0 + (wave >> 1) * 32 + ti * 16, c16 = n0 + (wave & 1) * 32 + tj * 16;
+      if (r16 < M && c16 < M && !(diag && r16 > c16)) need |= 1u << (2 * ti + tj);
+    }
+  const bool active = need != 0;
+  __shared__ GemmSmem S;
   double4_t acc[2][2];
-  gemm_tile<A_KMAJOR, MODE>(p, A + p.a_off, B + p.b_off, m0, n0, As, Bs, acc);
-  double *Cp = C + p.c_off;
 #pragma unroll
-  for (int ti = 0; ti < 2; ti++)
+  for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int tj = 0; tj < 2; tj++)
+    for (int j = 0; j < 2; j++) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  if (EPI == 0) {
+    run_segments<A_KMAJOR, MODE>(p, 0, p.nmid, X0, X1, g0, g1, m0, n0, S, acc, need);
+    if (W && active) {
+      double *Wp = W + p.w_off;
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int row = m0 + acc_row<MODE>(ti, v), col = n0 + acc_col(tj);
-        if (row < p.M && col < p.N) Cp[(long long)row * p.ldc + col] = acc[ti][tj][v];
-      }
+      for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+        for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int row = m0 + acc_row<MODE>(ti, v), col = n0 + acc_col(tj);
+            if (row < M && col < M && row <= col) Wp[(long long)row * M + col] = acc[ti][tj][v];
+          }
+    }
+    run_segments<A_KMAJOR, MODE>(p, p.nmid, p.nseg, X0, X1, g0, g1, m0, n0, S, acc, need);
+    if (C && active) {
+      double *Cp = C + p.c_off;
+#pragma unroll
+      for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+        for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int row = m0 + acc_row<MODE>(ti, v), col = n0 + acc_col(tj);
+            if (row < M && col < M && row <= col) {
+              const long long o = (long long)row * M + col;
+              Cp[o] = Cp[o] + acc[ti][tj][v];
+            }
+          }
+    }
+  } else {
+    run_segments<A_KMAJOR, MODE>(p, 0, p.nseg, X0, X1, g0, g1, m0, n0, S, acc, need);
+    // every segment loop ends with a barrier, so S is free for the transpose
+    double(*T)[GT + GP] = reinterpret_cast<double(*)[GT + GP]>(&S);
+    if (active) {
+#pragma unroll
+      for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+        for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) T[acc_row<MODE>(ti, v)][acc_col(tj)] = acc[ti][tj][v];
+    }
+    __syncthreads();
+    double *Zp = C + p.c_off;
+    const int c = threadIdx.x & 63;
+    for (int r = threadIdx.x >> 6; r < GT; r += 4) {
+      // tile (tm, tn); in a diagonal tile the lower part is sigma * upper^T
+      const double z = (diag && r > c) ? sigma * T[c][r] : T[r][c];
+      if (m0 + r < M && n0 + c < M) Zp[(long long)(m0 + r) * M + n0 + c] = z;
+    }
+    if (!diag) {
+      for (int r = threadIdx.x >> 6; r < GT; r += 4)
+        if (n0 + r < M && m0 + c < M) Zp[(long long)(n0 + r) * M + m0 + c] = sigma * T[c][r];
+    }
+  }
 }
 
-// Pandya product in antisymmetrised form: W = LCC * RCC^T and
-// Z = hL*W - hR*W^T written directly.  One workgroup per tile pair (tm <= tn):
-// it computes W(tm,tn) and W(tn,tm) and exchanges them through LDS, so Z costs
-// no more MFMA work than W and never needs a separate transpose pass.
-template <int MODE>
-__global__ __launch_bounds__(256) void k_gemm_z(const GemmProb *__restrict__ probs,
-                                                const int4 *__restrict__ tiles, int ntiles,
-                                                const double *__restrict__ A,
-                                                const double *__restrict__ B,
-                                                double *__restrict__ Z, double hL, double hR) {
-  const int t = blockIdx.x;
-  if (t >= ntiles) return;
-  const int4 tl = tiles[t];
-  const GemmProb p = probs[tl.x];
-  const int m0 = tl.y * GBM, n0 = tl.z * GBN;
-  __shared__ double As[GBK][GBM + GPAD];
-  __shared__ double Bs[GBK][GBN + GPAD];
-  __shared__ double T[GBM][GBN + GPAD];
-  double4_t c1[2][2], c2[2][2];
-  const double *Ap = A + p.a_off, *Bp = B + p.b_off;
-  double *Zp = Z + p.c_off;
-  gemm_tile<true, MODE>(p, Ap, Bp, m0, n0, As, Bs, c1);  // W(tm, tn)
-  const bool diag = (tl.y == tl.z);
-  if (!diag) gemm_tile<true, MODE>(p, Ap, Bp, n0, m0, As, Bs, c2);  // W(tn, tm)
-  // Z(tm,tn)[r][c] = hL*W(tm,tn)[r][c] - hR*W(tn,tm)[c][r]
-#pragma unroll
-  for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-    for (int tj = 0; tj < 2; tj++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) T[acc_row<MODE>(ti, v)][acc_col(tj)] = diag ? c1[ti][tj][v] : c2[ti][tj][v];
-  __syncthreads();
-#pragma unroll
-  for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-    for (int tj = 0; tj < 2; tj++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int r = acc_row<MODE>(ti, v), c = acc_col(tj);
-        const int row = m0 + r, col = n0 + c;
-        if (row < p.M && col < p.N) Zp[(long long)row * p.ldc + col] = hL * c1[ti][tj][v] - hR * T[c][r];
-      }
-  if (diag) return;
-  __syncthreads();
-#pragma unroll
-  for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-    for (int tj = 0; tj < 2; tj++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) T[acc_row<MODE>(ti, v)][acc_col(tj)] = c1[ti][tj][v];
-  __syncthreads();
-#pragma unroll
-  for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-    for (int tj = 0; tj < 2; tj++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int r = acc_row<MODE>(ti, v), c = acc_col(tj);
-        const int row = n0 + r, col = m0 + c;
-        if (row < p.M && col < p.N) Zp[(long long)row * p.ldc + col] = hL * c2[ti][tj][v] - hR * T[c][r];
-      }
-}
-void launch_gemm_z(hipStream_t s, int mode, const GemmProb *probs, const int4 *tiles, int ntiles,
-                   const double *A, const double *B, double *Z, double hL, double hR) {
+void launch_seg_gemm_pphh(hipStream_t s, int mode, const SegProb *probs, const int4 *tiles, int ntiles,
+                          const double *X0, const double *X1, double *C, double *W) {
   if (ntiles <= 0) return;
+  const dim3 g(ntiles), b(256);
   if (mode == 1)
-    hipLaunchKernelGGL(k_gemm_z<1>, dim3(ntiles), dim3(256), 0, s, probs, tiles, ntiles, A, B, Z, hL, hR);
+    hipLaunchKernelGGL((k_seg_gemm<false, 1, 0>), g, b, 0, s, probs, tiles, ntiles, X0, X1, 1.0, 1.0, C, W, 0.0);
   else if (mode == 2)
-    hipLaunchKernelGGL(k_gemm_z<2>, dim3(ntiles), dim3(256), 0, s, probs, tiles, ntiles, A, B, Z, hL, hR);
+    hipLaunchKernelGGL((k_seg_gemm<false, 2, 0>), g, b, 0, s, probs, tiles, ntiles, X0, X1, 1.0, 1.0, C, W, 0.0);
   else
-    hipLaunchKernelGGL(k_gemm_z<0>, dim3(ntiles), dim3(256), 0, s, probs, tiles, ntiles, A, B, Z, hL, hR);
+    hipLaunchKernelGGL((k_seg_gemm<false, 0, 0>), g, b, 0, s, probs, tiles, ntiles, X0, X1, 1.0, 1.0, C, W, 0.0);
 }
-
-void launch_gemm_f64(hipStream_t s, int mode, const GemmProb *probs, const int4 *tiles, int ntiles,
-                     const double *A, const double *B, double *C) {
+void launch_seg_gemm_z(hipStream_t s, int mode, const SegProb *probs, const int4 *tiles, int ntiles,
+                       const double *LCC, const double *RCC, double *Z, double hL, double hR) {
   if (ntiles <= 0) return;
+  const dim3 g(ntiles), b(256);
+  const double g0 = hL, g1 = -hR, sigma = -hL * hR;
   if (mode == 1)
-    hipLaunchKernelGGL((k_gemm_f64<false, 1>), dim3(ntiles), dim3(256), 0, s, probs, tiles, ntiles, A, B, C);
+    hipLaunchKernelGGL((k_seg_gemm<true, 1, 1>), g, b, 0, s, probs, tiles, ntiles, LCC, RCC, g0, g1, Z, nullptr, sigma);
   else if (mode == 2)
-    hipLaunchKernelGGL((k_gemm_f64<false, 2>), dim3(ntiles), dim3(256), 0, s, probs, tiles, ntiles, A, B, C);
+    hipLaunchKernelGGL((k_seg_gemm<true, 2, 1>), g, b, 0, s, probs, tiles, ntiles, LCC, RCC, g0, g1, Z, nullptr, sigma);
   else
-    hipLaunchKernelGGL((k_gemm_f64<false, 0>), dim3(ntiles), dim3(256), 0, s, probs, tiles, ntiles, A, B, C);
+    hipLaunchKernelGGL((k_seg_gemm<true, 0, 1>), g, b, 0, s, probs, tiles, ntiles, LCC, RCC, g0, g1, Z, nullptr, sigma);
 }
 
 // ------------------------------------------------------------------ Pandya transform
@@ -468,72 +546,53 @@ void launch_pandya(hipStream_t s, const DevLayout &L, const double *op, double *
 }
 
 // ------------------------------------------------------------------ 122
-// commutator_122 (commutators.f90:283-396), overwriting every upper element:
-// RES(r,s) = sum_{e in nb(r)} c_e [fL(x,i) R(r_e,s) - fR(x,i) L(r_e,s)]
-//          - sum_{e in nb(s)} c_e [fL(i,x) R(r,s_e) - fR(i,x) L(r,s_e)]
+// commutator_122 (commutators.f90:283-396).  With M = A_L R - A_R L, where
+// A_X is the pair-space action of X's one-body part,
+//   M(r,s) = sum_{e in nb(r)} c_e [fL(x,i) R(r_e,s) - fR(x,i) L(r_e,s)],
+// the right-multiplication half equals -hL*hR*M(s,r) (hermiticity of f and
+// V), so RES(r,s) = M(r,s) + herm(RES)*M(s,r).  This kernel writes M over the
+// whole block (one wavefront per row, coalesced row reads, wave-uniform
+// neighbour entries); the diagonal is written already symmetrised,
+// M(r,r)*(1+herm).  k_sym_mirror adds the transpose at the end.
 __global__ __launch_bounds__(64) void k_122(DevLayout L, const double *__restrict__ Lop,
-                                            const double *__restrict__ Rop, double *__restrict__ RES) {
+                                            const double *__restrict__ Rop, double *__restrict__ RES,
+                                            double hres) {
   const int r = blockIdx.x;
   if (r >= L.npairs) return;
-  const int q = L.pair_q[r], base = L.pbase[q], rl = r - base, dim = L.bdim[q], N = L.nsp;
+  const int q = L.pair_q[r], rl = r - L.pbase[q], dim = L.bdim[q], N = L.nsp;
   const long long bo = L.v_off + L.boff[q];
   const double *LV = Lop + bo, *RV = Rop + bo;
-  double *OV = RES + bo;
+  double *OV = RES + bo + (long long)rl * dim;
   const double *FL = Lop + L.f_off, *FR = Rop + L.f_off;
   const int e0 = L.nb_start[r], e1 = L.nb_start[r + 1];
-  for (int s = rl + (int)threadIdx.x; s < dim; s += 64) {
+  for (int s = threadIdx.x; s < dim; s += 64) {
     double val = 0.0;
     for (int e = e0; e < e1; e++) {
-      const int pe = L.nb_pos[e], x = L.nb_x[e], i = L.nb_i[e];
+      const long long pe = L.nb_pos[e];
+      const int xi = L.nb_x[e] * N + L.nb_i[e];
       const double ce = L.nb_coef[e];
-      val += ce * (FL[x * N + i] * RV[(long long)pe * dim + s] - FR[x * N + i] * LV[(long long)pe * dim + s]);
+      val += ce * (FL[xi] * RV[pe * dim + s] - FR[xi] * LV[pe * dim + s]);
     }
-    const int gs = base + s;
-    const int f0 = L.nb_start[gs], f1 = L.nb_start[gs + 1];
-    for (int e = f0; e < f1; e++) {
-      const int pe = L.nb_pos[e], x = L.nb_x[e], i = L.nb_i[e];
-      const double ce = L.nb_coef[e];
-      val -= ce * (FL[i * N + x] * RV[(long long)rl * dim + pe] - FR[i * N + x] * LV[(long long)rl * dim + pe]);
-    }
-    OV[(long long)rl * dim + s] = val;
+    OV[s] = (s == rl) ? val * (1.0 + hres) : val;
   }
 }
-void launch_122(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES) {
+void launch_122(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES,
+                int hres) {
   if (L.npairs <= 0) return;
-  hipLaunchKernelGGL(k_122, dim3(L.npairs), dim3(64), 0, s, L, Lop, Rop, RES);
+  hipLaunchKernelGGL(k_122, dim3(L.npairs), dim3(64), 0, s, L, Lop, Rop, RES, (double)hres);
 }
 
-__global__ __launch_bounds__(64) void k_clear_upper(DevLayout L, double *RES) {
+// zero every channel block (both triangles): the start state when 122 is off
+__global__ __launch_bounds__(64) void k_clear_rows(DevLayout L, double *RES) {
   const int r = blockIdx.x;
   if (r >= L.npairs) return;
   const int q = L.pair_q[r], rl = r - L.pbase[q], dim = L.bdim[q];
-  double *OV = RES + L.v_off + L.boff[q];
-  for (int s = rl + (int)threadIdx.x; s < dim; s += 64) OV[(long long)rl * dim + s] = 0.0;
+  double *OV = RES + L.v_off + L.boff[q] + (long long)rl * dim;
+  for (int s = threadIdx.x; s < dim; s += 64) OV[s] = 0.0;
 }
-void launch_clear_upper(hipStream_t s, const DevLayout &L, double *RES) {
+void launch_clear_blocks(hipStream_t s, const DevLayout &L, double *RES) {
   if (L.npairs <= 0) return;
-  hipLaunchKernelGGL(k_clear_upper, dim3(L.npairs), dim3(64), 0, s, L, RES);
-}
-
-// ------------------------------------------------------------------ 222_pp_hh combine
-// RES += (W1 - W2) - pm (W1 - W2)^T on the upper triangle (commutators.f90:705-766)
-__global__ __launch_bounds__(64) void k_pphh_combine(DevLayout L, const double *__restrict__ Wa,
-                                                     const double *__restrict__ Wb,
-                                                     double *__restrict__ RES, double pm) {
-  const int r = blockIdx.x;
-  if (r >= L.npairs) return;
-  const int q = L.pair_q[r], rl = r - L.pbase[q], dim = L.bdim[q];
-  const double *A = Wa + L.boff[q], *B = Wb + L.boff[q];
-  double *OV = RES + L.v_off + L.boff[q];
-  for (int s = rl + (int)threadIdx.x; s < dim; s += 64) {
-    const long long rs = (long long)rl * dim + s, sr = (long long)s * dim + rl;
-    OV[rs] = OV[rs] + A[rs] - B[rs] - (A[sr] - B[sr]) * pm;
-  }
-}
-void launch_pphh_combine(hipStream_t s, const DevLayout &L, const double *Wa, const double *Wb,
-                         double *RES, double pm) {
-  if (L.npairs <= 0) return;
-  hipLaunchKernelGGL(k_pphh_combine, dim3(L.npairs), dim3(64), 0, s, L, Wa, Wb, RES, pm);
+  hipLaunchKernelGGL(k_clear_rows, dim3(L.npairs), dim3(64), 0, s, L, RES);
 }
 
 // ------------------------------------------------------------------ 222_ph recoupling gather
@@ -621,44 +680,64 @@ void launch_ph_gather(hipStream_t s, const DevLayout &L, const double *Z, double
   hipLaunchKernelGGL(k_ph_apply, dim3(L.npairs), dim3(64), 0, s, L, Tbuf, RES);
 }
 
-// ------------------------------------------------------------------ mirror
-// lower = herm * upper (diagonal zeroed for herm = -1), 32x32 tiles through LDS
-// so both the upper reads and the lower writes are row-contiguous.
-__global__ __launch_bounds__(256) void k_mirror(DevLayout L, const int4 *__restrict__ tiles, double *RES,
-                                                int herm) {
+// ------------------------------------------------------------------ symmetrise + mirror
+// Final pass over every channel block: the upper triangle U holds M + the
+// accumulated 222 terms, the strict lower triangle still holds M^T's source
+// M(s,r) from k_122, so for r < s
+//   RES(r,s) = U(r,s) + herm*lower(s,r),  RES(s,r) = herm*RES(r,s);
+// the diagonal is kept (zeroed for herm = -1).  This reproduces the
+// reference's "upper triangle then mirror x herm" contract (commutators.f90
+// 388-395, 915-923).  32x32 tile pairs go through LDS so all reads and
+// writes are row-contiguous.
+__global__ __launch_bounds__(256) void k_sym_mirror(DevLayout L, const int4 *__restrict__ tiles, double *RES,
+                                                    int herm) {
   const int4 t = tiles[blockIdx.x];  // x = block, y = tile row, z = tile col (y <= z)
   const int q = t.x, dim = L.bdim[q];
   double *OV = RES + L.v_off + L.boff[q];
-  __shared__ double T[32][33];
+  __shared__ double U[32][33], W[32][33];
   const int r0 = t.y * 32, c0 = t.z * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  const double h = (double)herm;
   for (int k = ty; k < 32; k += 8) {
     const int r = r0 + k, c = c0 + tx;
-    T[k][tx] = (r < dim && c < dim) ? OV[(long long)r * dim + c] : 0.0;
+    U[k][tx] = (r < dim && c < dim) ? OV[(long long)r * dim + c] : 0.0;   // upper tile (r0, c0)
+    const int r2 = c0 + k, c2 = r0 + tx;
+    W[k][tx] = (r2 < dim && c2 < dim) ? OV[(long long)r2 * dim + c2] : 0.0;  // lower tile (c0, r0)
   }
   __syncthreads();
-  // element (c, r) of the lower triangle takes herm * upper (r, c)
+  const bool diag = t.y == t.z;
+  // upper: element (r0+k, c0+tx)
   for (int k = ty; k < 32; k += 8) {
-    const int c = c0 + k, r = r0 + tx;  // write row c, column r
-    if (c < dim && r < dim) {
-      if (r < c) OV[(long long)c * dim + r] = herm * T[tx][k];
-      else if (r == c && herm == -1) OV[(long long)c * dim + r] = 0.0;
+    const int r = r0 + k, c = c0 + tx;
+    if (r < dim && c < dim && (!diag || k <= tx)) {
+      double v = U[k][tx];
+      if (!diag || k < tx) v += h * W[tx][k];
+      else if (herm == -1) v = 0.0;
+      OV[(long long)r * dim + c] = v;
     }
   }
+  // lower: element (c0+k, r0+tx) = herm * upper (r0+tx, c0+k)
+  for (int k = ty; k < 32; k += 8) {
+    const int r = c0 + k, c = r0 + tx;
+    if (r < dim && c < dim && (!diag || tx < k)) OV[(long long)r * dim + c] = h * (U[tx][k] + h * W[k][tx]);
+  }
 }
-void launch_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm) {
+void launch_sym_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm) {
   if (L.nmirror <= 0) return;
-  hipLaunchKernelGGL(k_mirror, dim3(L.nmirror), dim3(256), 0, s, L, L.mirror_tiles, RES, herm);
+  hipLaunchKernelGGL(k_sym_mirror, dim3(L.nmirror), dim3(256), 0, s, L, L.mirror_tiles, RES, herm);
 }
 
 // ------------------------------------------------------------------ one-body (111 + 121 + 221)
 __device__ __forceinline__ bool same_ljt(const DevLayout &L, int a, int b) {
   return L.jj[a] == L.jj[b] && L.l[a] == L.l[b] && L.tz[a] == L.tz[b];
 }
-// element of a full-layout W buffer at pair (c,x),(c,y) in block J with v_elem's
-// ordering phase and identical-orbit sqrt2 applied to both pairs
-__device__ __forceinline__ double w_pair(const DevLayout &L, const double *W, int c, int x, int y,
-                                         int J, double pm) {
+// (Y - pm*Y^T) at pair rows (c,x),(c,y) of block J, the 221 intermediate, with
+// v_elem's ordering phase and identical-orbit sqrt2 applied to both pairs.
+// Hole c: Y = W1 = L(:,pp) R(pp,:), read from the upper-triangle store Wpp of
+// the 222_pp_hh GEMM ((W1 - pm W1^T)^T = -pm (W1 - pm W1^T)).  Particle c:
+// Y = W2 = L(:,hh) R(hh,:), summed here (hh ranges are a few pairs long).
+__device__ __forceinline__ double w221(const DevLayout &L, const double *Wpp, const double *VL,
+                                       const double *VR, int c, int x, int y, int J, double pm) {
   const int jc = L.jj[c];
   const int P = (L.l[c] + L.l[x]) & 1, T2 = L.tz[c] + L.tz[x];
   const int q = 3 * J + 2 * (T2 / 2 + 1) + P;
@@ -679,9 +758,19 @@ __device__ __forceinline__ double w_pair(const DevLayout &L, const double *W, in
     s = dpos(L, J, c, y);
   }
   if ((r | s) < 0) return 0.0;
-  const double *Wq = W + L.boff[q];
+  const long long bo = L.boff[q];
   const int dim = L.bdim[q];
-  return Wq[(long long)r * dim + s] * pre - pm * (Wq[(long long)s * dim + r] * pre);
+  double val;
+  if (L.con[c]) {
+    val = (r <= s) ? Wpp[bo + (long long)r * dim + s] : -pm * Wpp[bo + (long long)s * dim + r];
+  } else {
+    const double *Lb = VL + bo, *Rb = VR + bo;
+    val = 0.0;
+    for (int k = L.bnp[q] + L.bnb[q]; k < dim; k++)
+      val += Lb[(long long)r * dim + k] * Rb[(long long)k * dim + s] -
+             Rb[(long long)r * dim + k] * Lb[(long long)k * dim + s];
+  }
+  return val * pre;
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -693,8 +782,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // one wavefront per one-body output (p,q); lanes split the inner sums
 __global__ __launch_bounds__(64) void k_onebody(DevLayout L, const double *__restrict__ Lop,
                                                 const double *__restrict__ Rop,
-                                                const double *__restrict__ Wa,
-                                                const double *__restrict__ Wb,
+                                                const double *__restrict__ Wpp,
                                                 double *__restrict__ RES, int hL, int hR, int hres,
                                                 unsigned mask) {
   const int t = blockIdx.x;
@@ -731,9 +819,9 @@ __global__ __launch_bounds__(64) void k_onebody(DevLayout L, const double *__res
     const int ji = L.jj[p];
     double sm = 0.0;
     for (int c = lane; c < N; c += 64) {
-      const double *W = L.con[c] ? Wa : Wb;  // holes: pp intermediates, particles: hh
       const int jc = L.jj[c];
-      for (int JT = abs(jc - ji); JT <= jc + ji; JT += 2) sm = sm + w_pair(L, W, c, p, qo, JT, pm) * (JT + 1);
+      for (int JT = abs(jc - ji); JT <= jc + ji; JT += 2)
+        sm = sm + w221(L, Wpp, VL, VR, c, p, qo, JT, pm) * (JT + 1);
     }
     val += wave_sum(sm) / (ji + 1.0);
   }
@@ -745,10 +833,9 @@ __global__ __launch_bounds__(64) void k_onebody(DevLayout L, const double *__res
   }
 }
 void launch_onebody(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop,
-                    const double *Wa, const double *Wb, double *RES, int hL, int hR, int hres,
-                    unsigned mask) {
+                    const double *Wpp, double *RES, int hL, int hR, int hres, unsigned mask) {
   if (L.nob <= 0) return;
-  hipLaunchKernelGGL(k_onebody, dim3(L.nob), dim3(64), 0, s, L, Lop, Rop, Wa, Wb, RES, hL, hR, hres, mask);
+  hipLaunchKernelGGL(k_onebody, dim3(L.nob), dim3(64), 0, s, L, Lop, Rop, Wpp, RES, hL, hR, hres, mask);
 }
 
 // ------------------------------------------------------------------ reductions
