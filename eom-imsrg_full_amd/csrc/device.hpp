@@ -6,6 +6,15 @@
 
 namespace imsrg {
 
+// Work items split into 8 contiguous ranges, one per XCD: workgroup w runs on
+// XCD w % 8 and takes item start[w % 8] + w / 8, so items that share data
+// (rows of one channel block, terms of one quadruple run) share one L2.
+// Launch grids are 8 * len; slots past an XCD's range exit at once.
+struct XcdSplit {
+  int start[9];
+  int len;
+};
+
 // Read-only tables resident in HBM, passed by value to every kernel.
 struct DevLayout {
   int nsp, nhole, npart, nblocks, ncc, jtot_max, nhalf, nJ, npairs, nob, nphp, nccol;
@@ -32,6 +41,7 @@ struct DevLayout {
   const int4 *mirror_tiles;  // (block, tile row, tile col) with row <= col, 32x32 tiles
   int nmirror;
   long long f_off, v_off, op_size, vsize;
+  XcdSplit xs_rows, xs_terms;  // k_122 rows, k_ph_terms chunks of 256 terms
 };
 
 // One K segment of a segmented GEMM: sign * A * B with A taken from operand

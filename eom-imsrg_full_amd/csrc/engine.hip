@@ -113,6 +113,26 @@ static int dalloc(imsrg_ctx_s *c, size_t ndouble, double **out) {
   return 0;
 }
 
+// 8 contiguous item ranges of about equal total weight (one per XCD)
+static XcdSplit make_split(const std::vector<double> &w) {
+  XcdSplit x{};
+  double tot = 0;
+  for (double v : w) tot += v;
+  const int n = (int)w.size();
+  double acc = 0;
+  int i = 0;
+  x.start[0] = 0;
+  for (int k = 1; k < 8; k++) {
+    const double target = tot * k / 8.0;
+    while (i < n && acc + 0.5 * w[i] < target) acc += w[i++];
+    x.start[k] = i;
+  }
+  x.start[8] = n;
+  x.len = 0;
+  for (int k = 0; k < 8; k++) x.len = std::max(x.len, x.start[k + 1] - x.start[k]);
+  return x;
+}
+
 // Upper-triangle 64x64 tile schedule (tm <= tn) of one segmented-GEMM launch.
 // Workgroup i is dispatched to XCD i % 8, so problems are spread over the 8
 // XCDs by flop weight (heaviest first onto the least-loaded XCD): every tile
@@ -247,6 +267,13 @@ int imsrg_ctx_create(int nsp, const int *n, const int *l, const int *jj, const i
   {
     std::vector<long long> ub(Ly.ph_ubase.begin(), Ly.ph_ubase.end());
     UP(ub, ph_ubase);
+  }
+  {
+    std::vector<double> wr(Ly.npairs), wt((Ly.ph_terms.size() + 255) / 256, 1.0);
+    for (int r = 0; r < Ly.npairs; r++)
+      wr[r] = Ly.bdim[Ly.pair_q[r]] * (double)(Ly.nb_start[r + 1] - Ly.nb_start[r] + 1);
+    D.xs_rows = make_split(wr);
+    D.xs_terms = make_split(wt);
   }
   std::vector<int4> mtiles;
   for (int q = 0; q < Ly.nblocks; q++) {

@@ -18,6 +18,11 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 
 #define HIP_KERNEL_CHECK() ((void)0)
 
+__device__ __forceinline__ int xcd_item(const XcdSplit &x) {
+  const int xcd = blockIdx.x & 7, it = x.start[xcd] + (int)(blockIdx.x >> 3);
+  return it < x.start[xcd + 1] ? it : -1;
+}
+
 __device__ __forceinline__ int sgnp(int e) { return (e & 1) ? -1 : 1; }  // (-1)^e
 __device__ __forceinline__ bool dtri(int a, int b, int J) { return J <= a + b && J >= abs(a - b); }
 __device__ __forceinline__ int dpos(const DevLayout &L, int J, int a, int b) {
@@ -557,8 +562,8 @@ void launch_pandya(hipStream_t s, const DevLayout &L, const double *op, double *
 __global__ __launch_bounds__(64) void k_122(DevLayout L, const double *__restrict__ Lop,
                                             const double *__restrict__ Rop, double *__restrict__ RES,
                                             double hres) {
-  const int r = blockIdx.x;
-  if (r >= L.npairs) return;
+  const int r = xcd_item(L.xs_rows);
+  if (r < 0) return;
   const int q = L.pair_q[r], rl = r - L.pbase[q], dim = L.bdim[q], N = L.nsp;
   const long long bo = L.v_off + L.boff[q];
   const double *LV = Lop + bo, *RV = Rop + bo;
@@ -579,7 +584,7 @@ __global__ __launch_bounds__(64) void k_122(DevLayout L, const double *__restric
 void launch_122(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES,
                 int hres) {
   if (L.npairs <= 0) return;
-  hipLaunchKernelGGL(k_122, dim3(L.npairs), dim3(64), 0, s, L, Lop, Rop, RES, (double)hres);
+  hipLaunchKernelGGL(k_122, dim3(8 * L.xs_rows.len), dim3(64), 0, s, L, Lop, Rop, RES, (double)hres);
 }
 
 // zero every channel block (both triangles): the start state when 122 is off
@@ -609,8 +614,9 @@ void launch_clear_blocks(hipStream_t s, const DevLayout &L, double *RES) {
 // sums to every upper element, one wavefront per RES row.
 __global__ __launch_bounds__(256) void k_ph_terms(DevLayout L, const double *__restrict__ Z,
                                                   double *__restrict__ Tbuf, double sigma) {
-  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (t >= L.nphterm) return;
+  const int chunk = xcd_item(L.xs_terms);
+  const long long t = (long long)chunk * 256 + threadIdx.x;
+  if (chunk < 0 || t >= L.nphterm) return;
   const unsigned tc = L.ph_terms[t];
   const unsigned code = L.ph_quads[tc >> 6];
   const int side = (tc >> 5) & 1, m = tc & 31;
@@ -645,13 +651,22 @@ __global__ __launch_bounds__(256) void k_ph_terms(DevLayout L, const double *__r
   Tbuf[t] = v;
 }
 
+// The 6j symbols of one row, {ji jj J; j4 j5 J'} for all (j4, j5, J'), are one
+// contiguous slice of the table; it is staged in LDS once per row.
 __global__ __launch_bounds__(64) void k_ph_apply(DevLayout L, const double *__restrict__ Tbuf,
                                                  double *__restrict__ RES) {
+  extern __shared__ double S6[];  // nhalf * nhalf * nJ
   const int r = blockIdx.x;
   if (r >= L.npairs) return;
   const int q = L.pair_q[r], base = L.pbase[q], rl = r - base, dim = L.bdim[q], J = L.bJ[q];
   const int i = L.pair_a[r], j = L.pair_b[r];
   const int ji = L.jj[i], jj = L.jj[j], li = L.l[i], lj = L.l[j], ti = L.tz[i], tj = L.tz[j];
+  const int h = L.nhalf, nJ = L.nJ, nsl = h * h * nJ;
+  {
+    const double *src = L.sixj + ((((size_t)(ji >> 1) * h + (jj >> 1)) * nJ + (J >> 1)) * h * h) * nJ;
+    for (int e = threadIdx.x; e < nsl; e += 64) S6[e] = src[e];
+  }
+  __syncthreads();
   const double pre = (i == j) ? .70710678118 : 1.0;
   double *OV = RES + L.v_off + L.boff[q] + (long long)rl * dim;
   const unsigned *toff = L.ph_toff + L.ph_ubase[r] - rl;
@@ -663,10 +678,14 @@ __global__ __launch_bounds__(64) void k_ph_apply(DevLayout L, const double *__re
     const int a2 = max(abs(ji - jl), abs(jj - jk)), b2 = min(ji + jl, jj + jk);
     const double *T2 = T1 + (b1 >= a1 ? (b1 - a1) / 2 + 1 : 0);
     double s1 = 0.0, s2 = 0.0;
-    if (abs(tl - tj) == abs(ti - tk) && (ll + lj) % 2 == (li + lk) % 2)
-      for (int JP = a1, m = 0; JP <= b1; JP += 2, m++) s1 += dsixj(L, ji, jj, J, jl, jk, JP) * T1[m];
-    if (abs(tk - tj) == abs(ti - tl) && (lk + lj) % 2 == (li + ll) % 2)
-      for (int JP = a2, m = 0; JP <= b2; JP += 2, m++) s2 += dsixj(L, ji, jj, J, jk, jl, JP) * T2[m];
+    if (abs(tl - tj) == abs(ti - tk) && (ll + lj) % 2 == (li + lk) % 2) {
+      const double *c1 = S6 + ((jl >> 1) * h + (jk >> 1)) * nJ;
+      for (int JP = a1, m = 0; JP <= b1; JP += 2, m++) s1 += c1[JP >> 1] * T1[m];
+    }
+    if (abs(tk - tj) == abs(ti - tl) && (lk + lj) % 2 == (li + ll) % 2) {
+      const double *c2 = S6 + ((jk >> 1) * h + (jl >> 1)) * nJ;
+      for (int JP = a2, m = 0; JP <= b2; JP += 2, m++) s2 += c2[JP >> 1] * T2[m];
+    }
     const double out = -(double)sgnp((jl + jk + J) >> 1) * s1 + s2;
     const double pre2 = (k == l) ? .70710678118 : 1.0;
     OV[s] = OV[s] + out * pre * pre2;
@@ -675,9 +694,9 @@ __global__ __launch_bounds__(64) void k_ph_apply(DevLayout L, const double *__re
 void launch_ph_gather(hipStream_t s, const DevLayout &L, const double *Z, double *Tbuf, double *RES,
                       double hL, double hR) {
   if (L.nphterm <= 0) return;
-  hipLaunchKernelGGL(k_ph_terms, dim3((unsigned)((L.nphterm + 255) / 256)), dim3(256), 0, s, L, Z, Tbuf,
-                     -hL * hR);
-  hipLaunchKernelGGL(k_ph_apply, dim3(L.npairs), dim3(64), 0, s, L, Tbuf, RES);
+  hipLaunchKernelGGL(k_ph_terms, dim3(8 * L.xs_terms.len), dim3(256), 0, s, L, Z, Tbuf, -hL * hR);
+  const size_t lds = sizeof(double) * L.nhalf * L.nhalf * L.nJ;
+  hipLaunchKernelGGL(k_ph_apply, dim3(L.npairs), dim3(64), lds, s, L, Tbuf, RES);
 }
 
 // ------------------------------------------------------------------ symmetrise + mirror
@@ -779,53 +798,56 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// one wavefront per one-body output (p,q); lanes split the inner sums
-__global__ __launch_bounds__(64) void k_onebody(DevLayout L, const double *__restrict__ Lop,
-                                                const double *__restrict__ Rop,
-                                                const double *__restrict__ Wpp,
-                                                double *__restrict__ RES, int hL, int hR, int hres,
-                                                unsigned mask) {
+// one 256-thread workgroup per one-body output (p,q); the (orbit, J) pairs of
+// the 121 and 221 sums are flattened over the threads (MJ = nJ slots per
+// orbit, the most J values two orbits can couple to), then reduced in LDS.
+__global__ __launch_bounds__(256) void k_onebody(DevLayout L, const double *__restrict__ Lop,
+                                                 const double *__restrict__ Rop,
+                                                 const double *__restrict__ Wpp,
+                                                 double *__restrict__ RES, int hL, int hR, int hres,
+                                                 unsigned mask) {
+  __shared__ double red[4];
   const int t = blockIdx.x;
   if (t >= L.nob) return;
-  const int lane = threadIdx.x;
-  const int p = L.ob_p[t], qo = L.ob_q[t], N = L.nsp;
+  const int tid = threadIdx.x;
+  const int p = L.ob_p[t], qo = L.ob_q[t], N = L.nsp, MJ = L.nJ;
   const double *FL = Lop + L.f_off, *FR = Rop + L.f_off;
   const double *VL = Lop + L.v_off, *VR = Rop + L.v_off;
   double val = 0.0;
-  if (mask & T111) {
-    double s = 0.0;
-    for (int k = lane; k < N; k += 64) s += FL[p * N + k] * FR[k * N + qo] - FR[p * N + k] * FL[k * N + qo];
-    val += wave_sum(s);
-  }
+  if (mask & T111)
+    for (int k = tid; k < N; k += 256) val += FL[p * N + k] * FR[k * N + qo] - FR[p * N + k] * FL[k * N + qo];
   const bool same = same_ljt(L, p, qo);
   if (same && (mask & T121)) {
     const int jq = L.jj[qo];
     double sm = 0.0;
-    for (int e = lane; e < L.nphp; e += 64) {
+    for (int it = tid; it < L.nphp * MJ; it += 256) {
+      const int e = it / MJ, m = it - e * MJ;
       const int a = L.php_a[e], i = L.php_i[e], ji = L.jj[i];
-      double smx = 0.0, smy = 0.0, smx2 = 0.0, smy2 = 0.0;
-      for (int JT = abs(ji - jq); JT <= ji + jq; JT += 2) {
-        smx = smx + dv(L, VR, a, p, i, qo, JT) * (JT + 1);
-        smx2 = smx2 + dv(L, VL, a, p, i, qo, JT) * (JT + 1);
-        smy = smy + dv(L, VR, i, p, a, qo, JT) * (JT + 1);
-        smy2 = smy2 + dv(L, VL, i, p, a, qo, JT) * (JT + 1);
-      }
-      sm = sm + FL[a * N + i] * (hL * smx - smy) - FR[a * N + i] * (hR * smx2 - smy2);
+      const int JT = abs(ji - jq) + 2 * m;
+      if (JT > ji + jq) continue;
+      const double smx = dv(L, VR, a, p, i, qo, JT), smx2 = dv(L, VL, a, p, i, qo, JT);
+      const double smy = dv(L, VR, i, p, a, qo, JT), smy2 = dv(L, VL, i, p, a, qo, JT);
+      sm += (JT + 1) * (FL[a * N + i] * (hL * smx - smy) - FR[a * N + i] * (hR * smx2 - smy2));
     }
-    val += wave_sum(sm) / (L.jj[p] + 1.0);
+    val += sm / (L.jj[p] + 1.0);
   }
   if (same && (mask & T221)) {
     const double pm = (double)(hL * hR);
     const int ji = L.jj[p];
     double sm = 0.0;
-    for (int c = lane; c < N; c += 64) {
-      const int jc = L.jj[c];
-      for (int JT = abs(jc - ji); JT <= jc + ji; JT += 2)
-        sm = sm + w221(L, Wpp, VL, VR, c, p, qo, JT, pm) * (JT + 1);
+    for (int it = tid; it < N * MJ; it += 256) {
+      const int c = it / MJ, m = it - c * MJ;
+      const int jc = L.jj[c], JT = abs(jc - ji) + 2 * m;
+      if (JT > jc + ji) continue;
+      sm += w221(L, Wpp, VL, VR, c, p, qo, JT, pm) * (JT + 1);
     }
-    val += wave_sum(sm) / (ji + 1.0);
+    val += sm / (ji + 1.0);
   }
-  if (lane == 0) {
+  val = wave_sum(val);
+  if ((tid & 63) == 0) red[tid >> 6] = val;
+  __syncthreads();
+  if (tid == 0) {
+    val = red[0] + red[1] + red[2] + red[3];
     double *F = RES + L.f_off;
     F[p * N + qo] = val;
     if (p != qo) F[qo * N + p] = hres * val;
@@ -835,7 +857,7 @@ __global__ __launch_bounds__(64) void k_onebody(DevLayout L, const double *__res
 void launch_onebody(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop,
                     const double *Wpp, double *RES, int hL, int hR, int hres, unsigned mask) {
   if (L.nob <= 0) return;
-  hipLaunchKernelGGL(k_onebody, dim3(L.nob), dim3(64), 0, s, L, Lop, Rop, Wpp, RES, hL, hR, hres, mask);
+  hipLaunchKernelGGL(k_onebody, dim3(L.nob), dim3(256), 0, s, L, Lop, Rop, Wpp, RES, hL, hR, hres, mask);
 }
 
 // ------------------------------------------------------------------ reductions
