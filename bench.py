@@ -14,8 +14,11 @@ normal-ordered on the device).
    then exactly K steps of the same flow timed between two stream
    synchronizations.  Inputs are resident in HBM when the timer starts.
 
-The headline value is the steps/s of the fastest split variant whose |dE0|
-stays below 1e-6 MeV (else the FP64 path).  Prints ONE JSON line on stdout.
+The headline value is the steps/s of the fastest variant that meets the
+|dE0| < 1e-6 MeV bar: the FP64 path (which matches the FP64 oracle to 1e-10
+relative in the parity tests) or a split variant whose lockstep |dE0| stays
+below 1e-6 MeV.  Every variant is reported with its speed and error.
+Prints ONE JSON line on stdout.
 """
 import argparse
 import json
@@ -47,28 +50,46 @@ def parse():
     p.add_argument("--precision", default="all", choices=["all", "fp64", "split32", "bf16"])
     p.add_argument("--cpu-baseline", default="auto", choices=["auto", "reference", "port", "none"])
     p.add_argument("--cpu-reps", type=int, default=3, help="commutators in the CPU sample")
+    p.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                   help="print the calibrated HBM traffic table from two rocprofv3 --pmc runs and exit")
     return p.parse_args()
 
 
 def kernel_work(info, basis):
-    """Algorithmic flop (GEMMs) or bytes (gathers) per launch of each kernel class."""
+    """Algorithmic flop (GEMMs) or bytes (gathers) per launch of each kernel class:
+    every array the kernel must touch, read or written once."""
     dims = np.array([b.dim for b in basis.blocks], dtype=np.float64)
-    upper = float((dims * (dims + 1) / 2).sum()) * 8.0  # bytes of the upper triangles
-    full = float((dims * dims).sum()) * 8.0
-    cc = info["ccsize"] * 8.0
-    w = info["cwsize"] * 8.0
+    full = float((dims * dims).sum()) * 8.0      # all channel blocks of one operator
+    upper = info["nupper"] * 8.0                  # their upper triangles
+    cc = info["ccsize"] * 8.0                     # Pandya matrices of one operator
+    z = info["cwsize"] * 8.0                      # Z = hL*W - hR*W^T, all Pandya blocks
+    t = info["nphterm"] * 8.0                     # ph recoupling J' terms
     return {
         "gemm_pphh": ("mfma", info["flop_pphh"]),
         "gemm_ph": ("mfma", info["flop_ph"]),
-        # Pandya: read every channel block once, write the cross-coupled matrix
-        "pandya": ("hbm", full + cc),
-        # 122: read L and R channel blocks once, write the upper triangle of RES
-        "k122": ("hbm", 2 * full + upper),
-        # ph recoupling: read W once, read-modify-write the upper triangle of RES
-        "ph_gather": ("hbm", w + 2 * upper),
-        "pphh_combine": ("hbm", 2 * full + 2 * upper),
-        "mirror": ("hbm", 2 * upper),
+        "pandya": ("hbm", full + cc),             # read one operator's blocks, write its CC matrix
+        "k122": ("hbm", 3 * full),                # read L and R blocks, write M over whole blocks
+        "ph_terms": ("hbm", z + t),               # read Z once, write the terms
+        "ph_apply": ("hbm", t + 2 * upper),       # read the terms, read-modify-write the upper triangles
+        "sym_mirror": ("hbm", 2 * full),          # read and write whole blocks
     }
+
+
+def roofline_of(cls, work, prof, head):
+    bound, per_launch = work[cls]
+    ms, n = prof[cls]
+    avg_s = ms / 1e3 / n
+    if bound == "mfma":
+        achieved = per_launch / avg_s / 1e12
+        peak = {"fp64": PEAK["fp64_mfma_tflops"], "split32": PEAK["fp32_mfma_tflops"] / 3,
+                "bf16": 2500.0 / 6}[head]
+        unit = "TFLOP/s"
+    else:
+        achieved = per_launch / avg_s / 1e9
+        peak = PEAK["hbm_gbs"]
+        unit = "GB/s"
+    return {"kernel": cls, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+            "frac": achieved / peak, "per_launch": per_launch, "avg_launch_ms": avg_s * 1e3, "launches": n}
 
 
 def cpu_baseline(mode, basis, Hn_vec, eta_vec, comm_per_step, reps):
@@ -101,6 +122,57 @@ def cpu_baseline(mode, basis, Hn_vec, eta_vec, comm_per_step, reps):
 
 def basis_tag(basis):
     return int(max(2 * basis.n + basis.l))
+
+
+# kernel class -> kernel symbol of the FP64 path (rocprofv3 names)
+KERNEL_NAME = {
+    "gemm_pphh": "k_seg_gemm<false, 0, 0>", "gemm_ph": "k_seg_gemm<true, 0, 1>", "pandya": "k_pandya(",
+    "k122": "k_122(", "ph_terms": "k_ph_terms(", "ph_apply": "k_ph_apply(", "sym_mirror": "k_sym_mirror(",
+    "onebody": "k_onebody(",
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_emax8_fp64.txt")
+
+
+def pmc_summary(fetch_csv, write_csv, op_bytes):
+    """Per-dispatch HBM traffic of every kernel class from two rocprofv3 --pmc
+    runs (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch).  Each counter is
+    scaled by what it reads for k_axpby, a copy-like kernel of known size that
+    reads two operators and writes one (2*op_bytes in, op_bytes out)."""
+    import csv
+
+    def per_kernel(path):
+        acc = {}
+        for r in csv.DictReader(open(path)):
+            acc.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        return {k: sum(v) / len(v) for k, v in acc.items()}
+
+    fet, wri = per_kernel(fetch_csv), per_kernel(write_csv)
+    ax = [k for k in fet if "k_axpby" in k][0]
+    ffac = 2 * op_bytes / 1024.0 / fet[ax]
+    wfac = op_bytes / 1024.0 / wri[ax]
+    lines = [f"# calibration k_axpby: FETCH_SIZE {fet[ax]:.1f} KiB for {2 * op_bytes} B read -> x{ffac:.3f}; "
+             f"WRITE_SIZE {wri[ax]:.1f} KiB for {op_bytes} B written -> x{wfac:.3f}",
+             "# class fetch_KiB write_KiB hbm_bytes_per_launch(calibrated)"]
+    for cls, name in KERNEL_NAME.items():
+        kf = [k for k in fet if name in k]
+        kw = [k for k in wri if name in k]
+        if not kf or not kw:
+            continue
+        f, w = fet[kf[0]], wri[kw[0]]
+        lines.append(f"{cls} {f:.1f} {w:.1f} {1024.0 * (f * ffac + w * wfac):.0f}")
+    return "\n".join(lines)
+
+
+def load_traffic():
+    """calibrated HBM bytes per launch per kernel class, if measured for this workload"""
+    if not os.path.exists(TRAFFIC_FILE):
+        return {}
+    out = {}
+    for line in open(TRAFFIC_FILE):
+        f = line.split()
+        if len(f) == 4 and not line.startswith("#") and f[0] in KERNEL_NAME:
+            out[f[0]] = float(f[3])
+    return out
 
 
 PRECS = {"fp64": 0, "split32": 1, "bf16": 2}
@@ -160,6 +232,10 @@ def main():
     from imsrg_mi355x import engine as E
 
     basis = Basis.ho(a.emax, a.holes)
+    if a.pmc_summary:
+        ctx = E.Context(basis)
+        print(pmc_summary(a.pmc_summary[0], a.pmc_summary[1], ctx.info()["op_bytes"]))
+        return
     log(f"emax={a.emax} holes/iso={a.holes} orbits={basis.nsp} blocks={len(basis.blocks)} neq={basis.neq}")
     ctx = E.Context(basis)
     info = ctx.info()
@@ -193,28 +269,27 @@ def main():
             v["max_dOmega_F"] = float(max(c["dOmega_F"] for c in checks[prec]))
             v["meets_1e-6_MeV"] = bool(v["max_abs_dE0_MeV"] < 1e-6)
         variants[prec] = v
-    ok = [p for p in variants if p != "fp64" and variants[p].get("meets_1e-6_MeV")]
+    if "fp64" in variants:
+        variants["fp64"]["meets_1e-6_MeV"] = True  # it is the FP64 reference path
+    ok = [p for p in variants if variants[p].get("meets_1e-6_MeV")]
     head = max(ok, key=lambda p: variants[p]["steps_per_s"]) if ok else precs[0]
+    for p in variants:
+        if "fp64" in variants:
+            variants[p]["speedup_vs_fp64"] = variants[p]["steps_per_s"] / variants["fp64"]["steps_per_s"]
     recs, elapsed, prof = runs[head]
     steps_per_s = a.steps / elapsed
     gen_comm = 1 if a.generator == "w2" else 0
     ncomm = sum(r["n_magnus"] + r["n_bch"] + gen_comm for r in recs)
     comm_per_step = ncomm / a.steps
-    # roofline of the dominant kernel class
+    # roofline of the dominant kernel class, and of every class with a work model
     work = kernel_work(info, basis)
     dom = max(work, key=lambda k: prof[k][0])
-    ms, n = prof[dom]
-    bound, per_launch = work[dom]
-    avg_s = ms / 1e3 / n
-    if bound == "mfma":
-        achieved = per_launch / avg_s / 1e12
-        peak = {"fp64": PEAK["fp64_mfma_tflops"], "split32": PEAK["fp32_mfma_tflops"] / 3,
-                "bf16": 2500.0 / 6}[head]
-        unit = "TFLOP/s"
-    else:
-        achieved = per_launch / avg_s / 1e9
-        peak = PEAK["hbm_gbs"]
-        unit = "GB/s"
+    roof = roofline_of(dom, work, prof, head)
+    traffic = load_traffic() if (head == "fp64" and a.emax == 8 and a.holes == 6) else {}
+    roof["traffic"] = traffic.get(dom)
+    by_kernel = {k: roofline_of(k, work, prof, head) for k in work if prof[k][1]}
+    for k in by_kernel:
+        by_kernel[k]["traffic"] = traffic.get(k)
     kernels = {k: {"ms_total": round(v[0], 3), "launches": v[1]} for k, v in prof.items() if v[1]}
     log("kernel classes (ms, launches):", json.dumps(kernels))
     cpu = None
@@ -243,9 +318,11 @@ def main():
             "variants": variants,
             "speedup_vs_fp64": (steps_per_s / variants["fp64"]["steps_per_s"]) if "fp64" in variants else None,
         },
-        "roofline": {"kernel": dom, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
-                     "frac": achieved / peak, "traffic": None,
-                     "per_launch": per_launch, "avg_launch_ms": avg_s * 1e3, "launches": n},
+        "roofline": roof,
+        "roofline_by_kernel": {k: {"achieved": round(v["achieved"], 2), "unit": v["unit"],
+                                   "frac": round(v["frac"], 4), "avg_launch_ms": round(v["avg_launch_ms"], 4),
+                                   "traffic": v["traffic"]}
+                               for k, v in by_kernel.items()},
         "kernels": kernels,
         "cpu_baseline": cpu,
     }

@@ -83,8 +83,9 @@ void launch_pandya(hipStream_t s, const DevLayout &L, const double *op, double *
 void launch_122(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES,
                 int hres);
 void launch_clear_blocks(hipStream_t s, const DevLayout &L, double *RES);
-void launch_ph_gather(hipStream_t s, const DevLayout &L, const double *Z, double *Tbuf, double *RES,
-                      double hL, double hR);
+// ph recoupling: J' terms T(quadruple, J') from Z, then RES(upper) += 6j-weighted sums
+void launch_ph_terms(hipStream_t s, const DevLayout &L, const double *Z, double *Tbuf, double hL, double hR);
+void launch_ph_apply(hipStream_t s, const DevLayout &L, const double *Tbuf, double *RES);
 // upper += herm * (strict lower)^T, then lower = herm * upper (diagonal zeroed for herm = -1)
 void launch_sym_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm);
 // 111 + 121 + 221 one-body outputs; Wpp = pp intermediate of launch_seg_gemm_pphh

@@ -380,9 +380,11 @@ int imsrg_ctx_info(imsrg_ctx c, double *info) {
   int maxdim = 0, maxr = 0;
   for (int d : L.bdim) maxdim = std::max(maxdim, d);
   for (int r : L.ccrlen) maxr = std::max(maxr, r);
+  double nup = 0;
+  for (int d : L.bdim) nup += 0.5 * d * (d + 1.0);
   double v[16] = {(double)L.nsp, (double)L.nhole, (double)L.nblocks, (double)L.ncc, (double)L.npairs,
                   (double)maxdim, (double)maxr, 8.0 * L.op_size, (double)L.vsize, (double)L.ccsize,
-                  (double)L.cwsize, L.flop_pphh, L.flop_ph, 0, 0, 0};
+                  (double)L.cwsize, L.flop_pphh, L.flop_ph, (double)L.ph_terms.size(), nup, 0};
   std::memcpy(info, v, sizeof(v));
   return 0;
 }
@@ -520,7 +522,10 @@ static int comm_impl(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *R
          launch_seg_gemm_pphh(s, c->prec, c->pp_probs, c->pp_tiles, c->n_pp_tiles, L->d + D.v_off,
                               R->d + D.v_off, (mask & T222PPHH) ? RES->d + D.v_off : nullptr,
                               (mask & T221) ? c->wpp : nullptr));
-  if (mask & T222PH) PROF(c, IMSRG_PROF_PH_GATHER, launch_ph_gather(s, D, c->wph, c->tph, RES->d, (double)hL, (double)hR));
+  if (mask & T222PH) {
+    PROF(c, IMSRG_PROF_PH_TERMS, launch_ph_terms(s, D, c->wph, c->tph, (double)hL, (double)hR));
+    PROF(c, IMSRG_PROF_PH_APPLY, launch_ph_apply(s, D, c->tph, RES->d));
+  }
   PROF(c, IMSRG_PROF_MIRROR, launch_sym_mirror(s, D, RES->d, hres));
   PROF(c, IMSRG_PROF_ONEBODY, launch_onebody(s, D, L->d, R->d, c->wpp, RES->d, hL, hR, hres, mask));
   HIPCHK(hipGetLastError());

@@ -691,10 +691,12 @@ __global__ __launch_bounds__(64) void k_ph_apply(DevLayout L, const double *__re
     OV[s] = OV[s] + out * pre * pre2;
   }
 }
-void launch_ph_gather(hipStream_t s, const DevLayout &L, const double *Z, double *Tbuf, double *RES,
-                      double hL, double hR) {
+void launch_ph_terms(hipStream_t s, const DevLayout &L, const double *Z, double *Tbuf, double hL, double hR) {
   if (L.nphterm <= 0) return;
   hipLaunchKernelGGL(k_ph_terms, dim3(8 * L.xs_terms.len), dim3(256), 0, s, L, Z, Tbuf, -hL * hR);
+}
+void launch_ph_apply(hipStream_t s, const DevLayout &L, const double *Tbuf, double *RES) {
+  if (L.nphterm <= 0) return;
   const size_t lds = sizeof(double) * L.nhalf * L.nhalf * L.nJ;
   hipLaunchKernelGGL(k_ph_apply, dim3(L.npairs), dim3(64), lds, s, L, Tbuf, RES);
 }

@@ -81,7 +81,7 @@ _SIG = {
     "imsrg_ctx_profile": (C.c_int, [_vp, C.c_int]),
     "imsrg_ctx_profile_read": (C.c_int, [_vp, _dp, C.POINTER(C.c_longlong), C.c_int, C.c_int]),
 }
-PROF_CLASSES = ["gemm_pphh", "gemm_ph", "pandya", "k122", "pphh_combine", "ph_gather", "mirror",
+PROF_CLASSES = ["gemm_pphh", "gemm_ph", "pandya", "k122", "ph_apply", "ph_terms", "sym_mirror",
                 "onebody", "e0", "norm", "axpby", "generator"]
 for _name, (_res, _args) in _SIG.items():
     _f = getattr(_lib, _name)
@@ -142,7 +142,7 @@ class Context:
         v = np.zeros(16)
         _chk(_lib.imsrg_ctx_info(self.h, _dptr(v)))
         keys = ["nsp", "nhole", "nblocks", "ncc", "npairs", "max_dim", "max_cc_rows", "op_bytes",
-                "vsize", "ccsize", "cwsize", "flop_pphh", "flop_ph"]
+                "vsize", "ccsize", "cwsize", "flop_pphh", "flop_ph", "nphterm", "nupper"]
         return dict(zip(keys, v[:len(keys)]))
 
     def sync(self):

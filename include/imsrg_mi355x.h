@@ -71,7 +71,8 @@ int imsrg_ctx_destroy(imsrg_ctx ctx);
 int64_t imsrg_ctx_neq(imsrg_ctx ctx);
 /* info[0..15]: nsp, nhole, nblocks, ncc, npairs, max block dim, max CC rows,
  * device bytes per operator, vsize, ccsize, cwsize, flop_pphh, flop_ph
- * (as doubles: algorithmic GEMM flop per commutator), 0,0,0 */
+ * (as doubles: algorithmic GEMM flop per commutator), ph recoupling J' terms,
+ * upper-triangle two-body elements, 0 */
 int imsrg_ctx_info(imsrg_ctx ctx, double *info16);
 int imsrg_ctx_set_precision(imsrg_ctx ctx, int prec);
 int imsrg_ctx_synchronize(imsrg_ctx ctx);
@@ -146,14 +147,14 @@ int imsrg_flow_destroy(imsrg_flow flow);
  * While enabled, every engine kernel launch is bracketed by HIP events on the
  * context stream; profile_read synchronizes the stream and returns the summed
  * milliseconds and launch counts per kernel class. */
-#define IMSRG_PROF_GEMM_PPHH 0
-#define IMSRG_PROF_GEMM_PH 1
-#define IMSRG_PROF_PANDYA 2
-#define IMSRG_PROF_122 3
-#define IMSRG_PROF_COMBINE 4
-#define IMSRG_PROF_PH_GATHER 5
-#define IMSRG_PROF_MIRROR 6
-#define IMSRG_PROF_ONEBODY 7
+#define IMSRG_PROF_GEMM_PPHH 0  /* segmented GEMM of 222_pp_hh (+ 221 intermediate) */
+#define IMSRG_PROF_GEMM_PH 1    /* segmented GEMM of the Pandya product Z */
+#define IMSRG_PROF_PANDYA 2     /* Pandya transforms */
+#define IMSRG_PROF_122 3        /* 122 rows */
+#define IMSRG_PROF_PH_APPLY 4   /* ph recoupling: 6j-weighted sums into RES */
+#define IMSRG_PROF_PH_TERMS 5   /* ph recoupling: gather of Z terms */
+#define IMSRG_PROF_MIRROR 6     /* symmetrise + mirror pass */
+#define IMSRG_PROF_ONEBODY 7    /* 111 + 121 + 221 */
 #define IMSRG_PROF_E0 8
 #define IMSRG_PROF_NORM 9
 #define IMSRG_PROF_AXPBY 10
