@@ -40,7 +40,7 @@ struct DevLayout {
   const long long *ph_ubase; // per global pair row r: index of its first upper element
   const int4 *mirror_tiles;  // (block, tile row, tile col) with row <= col, 32x32 tiles
   int nmirror;
-  long long f_off, v_off, op_size, vsize;
+  long long f_off, v_off, op_size, vsize, ccsize, cwsize;  // per-instance strides
   XcdSplit xs_rows, xs_terms;  // k_122 rows, k_ph_terms chunks of 256 terms
 };
 
@@ -66,42 +66,61 @@ enum : unsigned {
 };
 
 // ---- launchers (kernels.hip) ----
-void launch_fill(hipStream_t s, double *p, long long n, double v);
-void launch_axpby(hipStream_t s, long long n, double a, const double *A, double b, const double *B,
-                  double *C);
+// Every launcher covers the instances listed in the device table `act`
+// (nact entries; act == nullptr with nact == 1 means instance 0).  Instance k
+// of an operator starts at k*op_size, of a Pandya matrix at k*ccsize, of Z at
+// k*cwsize, of the ph terms at k*nphterm, of the 221 store at k*vsize, of the
+// reduction partials at k*kPartials and of the scalar slots at k*kScal.
+void launch_fill(hipStream_t s, double *p, long long n, double v, const int *act = nullptr, int nact = 1,
+                 long long stride = 0);
+void launch_axpby(hipStream_t s, long long n, double a, const double *A, double b, const double *B, double *C,
+                  const int *act, int nact, long long stride);
+void launch_copy(hipStream_t s, long long n, const double *A, double *C, const int *act, int nact,
+                 long long stride);
 // mode: 0 FP64 MFMA, 1 split-FP32 (3 FP32-MFMA products), 2 split-BF16 (6 BF16-MFMA products).
 // 222_pp_hh: RES(upper) += L*D*R - R*D*L per channel block; with W, the pp part
 // (W1 - pm*W1^T) is also stored to W (upper triangle) for commutator_221.
 void launch_seg_gemm_pphh(hipStream_t s, int mode, const SegProb *probs, const int4 *tiles, int ntiles,
-                          const double *X0, const double *X1, double *C, double *W);
+                          const double *X0, const double *X1, double *C, double *W, const int *act, int nact,
+                          long long sop, long long sw);
 // 222_ph: Z = hL*LCC*RCC^T - hR*RCC*LCC^T per Pandya block, both triangles
 void launch_seg_gemm_z(hipStream_t s, int mode, const SegProb *probs, const int4 *tiles, int ntiles,
-                       const double *LCC, const double *RCC, double *Z, double hL, double hR);
-void launch_pandya(hipStream_t s, const DevLayout &L, const double *op, double *cc);
+                       const double *LCC, const double *RCC, double *Z, double hL, double hR, const int *act,
+                       int nact, long long scc, long long sz);
+void launch_pandya(hipStream_t s, const DevLayout &L, const double *op, double *cc, const int *act, int nact);
 // 122: M = A_L R - A_R L over whole channel blocks (diagonal symmetrised);
 // launch_sym_mirror later forms RES = M + herm*M^T
 void launch_122(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES,
-                int hres);
-void launch_clear_blocks(hipStream_t s, const DevLayout &L, double *RES);
+                int hres, const int *act, int nact);
+void launch_clear_blocks(hipStream_t s, const DevLayout &L, double *RES, const int *act, int nact);
 // ph recoupling: J' terms T(quadruple, J') from Z, then RES(upper) += 6j-weighted sums
-void launch_ph_terms(hipStream_t s, const DevLayout &L, const double *Z, double *Tbuf, double hL, double hR);
-void launch_ph_apply(hipStream_t s, const DevLayout &L, const double *Tbuf, double *RES);
+void launch_ph_terms(hipStream_t s, const DevLayout &L, const double *Z, double *Tbuf, double hL, double hR,
+                     const int *act, int nact);
+void launch_ph_apply(hipStream_t s, const DevLayout &L, const double *Tbuf, double *RES, const int *act,
+                     int nact);
 // upper += herm * (strict lower)^T, then lower = herm * upper (diagonal zeroed for herm = -1)
-void launch_sym_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm);
+void launch_sym_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm, const int *act, int nact);
 // 111 + 121 + 221 one-body outputs; Wpp = pp intermediate of launch_seg_gemm_pphh
 void launch_onebody(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop,
-                    const double *Wpp, double *RES, int hL, int hR, int hres, unsigned mask);
-// E0 = 110 + 220 into RES[0]; partial buffer of >= 1024 doubles
+                    const double *Wpp, double *RES, int hL, int hR, int hres, unsigned mask, const int *act,
+                    int nact);
+// E0 = 110 + 220 into RES[inst * rstride]
 void launch_e0(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES,
-               int hL, int hR, unsigned mask, double *partials);
-void launch_sumsq(hipStream_t s, const DevLayout &L, const double *op, double *partials, double *out);
-void launch_mbpt2(hipStream_t s, const DevLayout &L, const double *op, double *partials, double *out);
-void launch_monopole(hipStream_t s, const DevLayout &L, const double *H, double *mono);
+               int hL, int hR, unsigned mask, double *partials, const int *act, int nact, long long rstride);
+// Frobenius norm (sqrt of the sum of squares of F and V) into out[inst * kScal]
+void launch_sumsq(hipStream_t s, const DevLayout &L, const double *op, double *partials, double *out,
+                  const int *act, int nact);
+void launch_mbpt2(hipStream_t s, const DevLayout &L, const double *op, double *partials, double *out,
+                  const int *act, int nact);
+void launch_monopole(hipStream_t s, const DevLayout &L, const double *H, double *mono, const int *act,
+                     int nact);
 void launch_eta(hipStream_t s, const DevLayout &L, const double *Hden, const double *Hnum,
-                const double *mono, double *ETA, int kind);
-void launch_normal_order(hipStream_t s, const DevLayout &L, const double *T, double *H,
-                         double *partials);
+                const double *mono, double *ETA, int kind, const int *act, int nact);
+void launch_normal_order(hipStream_t s, const DevLayout &L, const double *T, double *H, const int *act,
+                         int nact);
+void launch_gather_e0(hipStream_t s, const double *op, long long stride, double *out, const int *act, int nact);
 
-constexpr int kPartials = 1024;
+constexpr int kPartials = 1024;  // reduction partials per instance
+constexpr int kScal = 64;        // scalar slots per instance
 
 }  // namespace imsrg

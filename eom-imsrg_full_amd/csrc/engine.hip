@@ -35,19 +35,24 @@ struct imsrg_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
   std::vector<void *> dallocs;
-  double *wpp = nullptr;    // 221 intermediate W1 - pm*W1^T (upper triangles), vsize
+  int nbatch = 1;           // independent instances per operator
+  double *wpp = nullptr;    // 221 intermediate W1 - pm*W1^T (upper triangles), vsize per instance
   double *cc[2] = {nullptr, nullptr};
   double *wph = nullptr;
   double *tph = nullptr;    // ph J' terms
   double *partials = nullptr;
-  double *scal = nullptr;   // device scalars
-  double *hscal = nullptr;  // pinned host scalars
+  double *scal = nullptr;   // device scalars, kScal per instance
+  double *hscal = nullptr;  // pinned host mirror of scal
   double *mono = nullptr;
   double *hstage = nullptr; // pinned op_size staging
   SegProb *pp_probs = nullptr, *ph_probs = nullptr;
   int4 *pp_tiles = nullptr, *ph_tiles = nullptr;
   int n_pp_tiles = 0, n_ph_tiles = 0;
   int prec = IMSRG_PREC_FP64;
+  // active-instance tables: slot 0 = all instances (constant), 1.. = driver work sets
+  static constexpr int kSlots = 4;
+  int *act_d[kSlots] = {nullptr, nullptr, nullptr, nullptr};
+  int *act_h[kSlots] = {nullptr, nullptr, nullptr, nullptr};
   imsrg_op_s *tmp[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // per-kernel-class HIP event timing on this context's stream
   bool prof_on = false;
@@ -166,15 +171,45 @@ static void build_seg_tiles(const std::vector<SegProb> &probs, std::vector<int4>
     for (size_t i = 0; i < bins[x].size(); i++) tiles[i * NX + x] = bins[x][i];
 }
 
+// A set of active instances: device table (what the kernels index with
+// blockIdx.y) and its host copy.
+struct ActSet {
+  const int *d = nullptr;
+  std::vector<int> h;
+  int n() const { return (int)h.size(); }
+};
+static ActSet all_set(imsrg_ctx_s *c) {
+  ActSet a;
+  a.d = c->act_d[0];
+  a.h.resize(c->nbatch);
+  for (int k = 0; k < c->nbatch; k++) a.h[k] = k;
+  return a;
+}
+// Writes `list` into table `slot` (an asynchronous copy on the context
+// stream).  Drivers rewrite a slot only after a stream synchronization (a
+// norm read-back), so the pinned source is never overwritten while a copy
+// from it is pending.
+static ActSet make_set(imsrg_ctx_s *c, int slot, const std::vector<int> &list) {
+  ActSet a;
+  a.d = c->act_d[slot];
+  a.h = list;
+  if (!list.empty()) {
+    std::memcpy(c->act_h[slot], list.data(), list.size() * sizeof(int));
+    (void)hipMemcpyAsync(c->act_d[slot], c->act_h[slot], list.size() * sizeof(int), hipMemcpyHostToDevice,
+                         c->stream);
+  }
+  return a;
+}
+
 static int op_new(imsrg_ctx_s *c, int herm, imsrg_op_s **out) {
   imsrg_op_s *op = new imsrg_op_s;
   op->ctx = c;
   op->herm = herm;
-  if (hipMalloc(&op->d, sizeof(double) * c->lay.op_size) != hipSuccess) {
+  if (hipMalloc(&op->d, sizeof(double) * c->lay.op_size * c->nbatch) != hipSuccess) {
     delete op;
     return fail(IMSRG_ENOMEM, "hipMalloc operator failed");
   }
-  launch_fill(c->stream, op->d, c->lay.op_size, 0.0);
+  launch_fill(c->stream, op->d, c->lay.op_size * c->nbatch, 0.0);
   *out = op;
   return 0;
 }
@@ -185,28 +220,40 @@ static imsrg_op_s *tmp_op(imsrg_ctx_s *c, int slot, int herm) {
   c->tmp[slot]->herm = herm;
   return c->tmp[slot];
 }
-static void op_copy(imsrg_op_s *src, imsrg_op_s *dst) {
+static void op_copy(imsrg_op_s *src, imsrg_op_s *dst, const ActSet &a) {
   imsrg_ctx_s *c = src->ctx;
-  (void)hipMemcpyAsync(dst->d, src->d, sizeof(double) * c->lay.op_size, hipMemcpyDeviceToDevice, c->stream);
+  launch_copy(c->stream, c->lay.op_size, src->d, dst->d, a.d, a.n(), c->lay.op_size);
   dst->herm = src->herm;
 }
-static int read_scalar(imsrg_ctx_s *c, int idx, double *out) {
-  HIPCHK(hipMemcpyAsync(c->hscal + idx, c->scal + idx, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+// scalar slot `idx` of every instance in `a`, in the order of a.h
+static int read_scalars(imsrg_ctx_s *c, int idx, const ActSet &a, std::vector<double> &out) {
+  HIPCHK(hipMemcpyAsync(c->hscal, c->scal, sizeof(double) * kScal * c->nbatch, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  *out = c->hscal[idx];
+  out.resize(a.n());
+  for (int k = 0; k < a.n(); k++) out[k] = c->hscal[(size_t)a.h[k] * kScal + idx];
   return 0;
 }
-static int op_norm(imsrg_op_s *op, double *out) {
+// mat_frob_norm of every instance in `a` (scalar slot 0)
+static int op_norms(imsrg_op_s *op, const ActSet &a, std::vector<double> &out) {
   imsrg_ctx_s *c = op->ctx;
-  PROF(c, IMSRG_PROF_NORM, launch_sumsq(c->stream, c->dl, op->d, c->partials, c->scal));
-  return read_scalar(c, 0, out);
+  PROF(c, IMSRG_PROF_NORM, launch_sumsq(c->stream, c->dl, op->d, c->partials, c->scal, a.d, a.n()));
+  return read_scalars(c, 0, a, out);
+}
+static int single_only(imsrg_ctx_s *c, const char *what) {
+  return c->nbatch == 1 ? 0 : fail(IMSRG_EINVAL, std::string(what) + ": batched context, use the _batch variant");
 }
 
 // ---------------------------------------------------------------- context
 int imsrg_ctx_create(int nsp, const int *n, const int *l, const int *jj, const int *tz, const int *con,
                      int device, imsrg_ctx *out) {
+  return imsrg_ctx_create_batch(nsp, n, l, jj, tz, con, device, 1, out);
+}
+int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, const int *tz, const int *con,
+                           int device, int nbatch, imsrg_ctx *out) {
   if (!out || !n || !l || !jj || !tz || !con) return fail(IMSRG_EINVAL, "null argument");
+  if (nbatch < 1 || nbatch > 4096) return fail(IMSRG_EINVAL, "nbatch must be 1..4096");
   imsrg_ctx_s *c = new imsrg_ctx_s;
+  c->nbatch = nbatch;
   std::string err = c->lay.build(nsp, n, l, jj, tz, con);
   if (!err.empty()) {
     delete c;
@@ -230,6 +277,7 @@ int imsrg_ctx_create(int nsp, const int *n, const int *l, const int *jj, const i
   D.jtot_max = Ly.jtot_max; D.nhalf = Ly.nhalf; D.nJ = Ly.nJ; D.npairs = Ly.npairs;
   D.nob = (int)Ly.ob_p.size();
   D.f_off = Ly.f_off; D.v_off = Ly.v_off; D.op_size = Ly.op_size; D.vsize = Ly.vsize;
+  D.ccsize = Ly.ccsize; D.cwsize = Ly.cwsize;
   std::vector<int> php_a, php_i;
   for (int a : Ly.parts)
     for (int i : Ly.holes)
@@ -284,17 +332,35 @@ int imsrg_ctx_create(int nsp, const int *n, const int *l, const int *jj, const i
   D.nmirror = (int)mtiles.size();
   UP(mtiles, mirror_tiles);
 #undef UP
-  if (!rc) rc = dalloc(c, (size_t)Ly.vsize, &c->wpp);
-  if (!rc) rc = dalloc(c, Ly.ccsize, &c->cc[0]);
-  if (!rc) rc = dalloc(c, Ly.ccsize, &c->cc[1]);
-  if (!rc) rc = dalloc(c, Ly.cwsize, &c->wph);
-  if (!rc) rc = dalloc(c, Ly.ph_terms.size(), &c->tph);
-  if (!rc) rc = dalloc(c, kPartials, &c->partials);
-  if (!rc) rc = dalloc(c, 64, &c->scal);
-  if (!rc) rc = dalloc(c, (size_t)Ly.nsp * Ly.nsp, &c->mono);
-  if (!rc && hipHostMalloc((void **)&c->hscal, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+  const size_t B = (size_t)nbatch;
+  if (!rc) rc = dalloc(c, (size_t)Ly.vsize * B, &c->wpp);
+  if (!rc) rc = dalloc(c, Ly.ccsize * B, &c->cc[0]);
+  if (!rc) rc = dalloc(c, Ly.ccsize * B, &c->cc[1]);
+  if (!rc) rc = dalloc(c, Ly.cwsize * B, &c->wph);
+  if (!rc) rc = dalloc(c, Ly.ph_terms.size() * B, &c->tph);
+  if (!rc) rc = dalloc(c, kPartials * B, &c->partials);
+  if (!rc) rc = dalloc(c, kScal * B, &c->scal);
+  if (!rc) rc = dalloc(c, (size_t)Ly.nsp * Ly.nsp * B, &c->mono);
+  for (int k = 0; k < imsrg_ctx_s::kSlots && !rc; k++) {
+    void *p = nullptr;
+    if (hipMalloc(&p, B * sizeof(int)) != hipSuccess) {
+      rc = fail(IMSRG_ENOMEM, "hipMalloc instance table failed");
+      break;
+    }
+    c->dallocs.push_back(p);
+    c->act_d[k] = (int *)p;
+    if (hipHostMalloc((void **)&c->act_h[k], B * sizeof(int), hipHostMallocDefault) != hipSuccess)
+      rc = fail(IMSRG_ENOMEM, "hipHostMalloc failed");
+  }
+  if (!rc) {
+    for (size_t k = 0; k < B; k++) c->act_h[0][k] = (int)k;
+    if (hipMemcpy(c->act_d[0], c->act_h[0], B * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+      rc = fail(IMSRG_EDEVICE, "instance table upload failed");
+  }
+  if (!rc && hipHostMalloc((void **)&c->hscal, kScal * B * sizeof(double), hipHostMallocDefault) != hipSuccess)
     rc = fail(IMSRG_ENOMEM, "hipHostMalloc failed");
-  if (!rc && hipHostMalloc((void **)&c->hstage, Ly.op_size * sizeof(double), hipHostMallocDefault) != hipSuccess)
+  if (!rc &&
+      hipHostMalloc((void **)&c->hstage, Ly.op_size * B * sizeof(double), hipHostMallocDefault) != hipSuccess)
     rc = fail(IMSRG_ENOMEM, "hipHostMalloc staging failed");
   // GEMM problem tables.  222_pp_hh per channel block (pairs ordered pp|ph|hh):
   //   RES += L_pp R_pp - R_pp L_pp - L_hh R_hh + R_hh L_hh, midpoint after the pp part.
@@ -344,8 +410,8 @@ int imsrg_ctx_create(int nsp, const int *n, const int *l, const int *jj, const i
     rc = upload_table(c, ph, (const SegProb **)&c->ph_probs);
     if (!rc) rc = upload_table(c, tiles, (const int4 **)&c->ph_tiles);
   }
-  if (!rc) launch_fill(c->stream, c->wpp, Ly.vsize, 0.0);
-  if (!rc && Ly.cwsize > 0) launch_fill(c->stream, c->wph, Ly.cwsize, 0.0);
+  if (!rc) launch_fill(c->stream, c->wpp, Ly.vsize * B, 0.0);
+  if (!rc && Ly.cwsize > 0) launch_fill(c->stream, c->wph, Ly.cwsize * B, 0.0);
   if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(IMSRG_EDEVICE, "context setup failed");
   if (rc) {
     imsrg_ctx_destroy(c);
@@ -366,6 +432,8 @@ int imsrg_ctx_destroy(imsrg_ctx c) {
   for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
   for (void *p : c->dallocs) (void)hipFree(p);
   if (c->hscal) (void)hipHostFree(c->hscal);
+  for (int k = 0; k < imsrg_ctx_s::kSlots; k++)
+    if (c->act_h[k]) (void)hipHostFree(c->act_h[k]);
   if (c->hstage) (void)hipHostFree(c->hstage);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -373,6 +441,7 @@ int imsrg_ctx_destroy(imsrg_ctx c) {
 }
 
 int64_t imsrg_ctx_neq(imsrg_ctx c) { return c ? c->lay.neq : -1; }
+int imsrg_ctx_nbatch(imsrg_ctx c) { return c ? c->nbatch : -1; }
 
 int imsrg_ctx_info(imsrg_ctx c, double *info) {
   if (!c || !info) return fail(IMSRG_EINVAL, "null argument");
@@ -426,75 +495,113 @@ int imsrg_op_get_herm(imsrg_op op, int *herm) {
   *herm = op->herm;
   return 0;
 }
-int imsrg_op_upload(imsrg_op op, const double *vec, int64_t neq) {
+// vec holds nbatch vectorize-order vectors back to back
+int imsrg_op_upload(imsrg_op op, const double *vec, int64_t n) {
   if (!op || !vec) return fail(IMSRG_EINVAL, "null argument");
   imsrg_ctx_s *c = op->ctx;
-  if (neq != c->lay.neq) return fail(IMSRG_EINVAL, "neq mismatch");
+  if (n != c->lay.neq * c->nbatch) return fail(IMSRG_EINVAL, "length must be neq * nbatch");
   HIPCHK(hipStreamSynchronize(c->stream));
-  c->lay.unpack_vec(vec, op->herm, c->hstage);
-  HIPCHK(hipMemcpyAsync(op->d, c->hstage, sizeof(double) * c->lay.op_size, hipMemcpyHostToDevice, c->stream));
+  for (int k = 0; k < c->nbatch; k++)
+    c->lay.unpack_vec(vec + (size_t)k * c->lay.neq, op->herm, c->hstage + (size_t)k * c->lay.op_size);
+  HIPCHK(hipMemcpyAsync(op->d, c->hstage, sizeof(double) * c->lay.op_size * c->nbatch, hipMemcpyHostToDevice,
+                        c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }
-int imsrg_op_download(imsrg_op op, double *vec, int64_t neq) {
+int imsrg_op_download(imsrg_op op, double *vec, int64_t n) {
   if (!op || !vec) return fail(IMSRG_EINVAL, "null argument");
   imsrg_ctx_s *c = op->ctx;
-  if (neq != c->lay.neq) return fail(IMSRG_EINVAL, "neq mismatch");
-  HIPCHK(hipMemcpyAsync(c->hstage, op->d, sizeof(double) * c->lay.op_size, hipMemcpyDeviceToHost, c->stream));
+  if (n != c->lay.neq * c->nbatch) return fail(IMSRG_EINVAL, "length must be neq * nbatch");
+  HIPCHK(hipMemcpyAsync(c->hstage, op->d, sizeof(double) * c->lay.op_size * c->nbatch, hipMemcpyDeviceToHost,
+                        c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  c->lay.pack_vec(c->hstage, vec);
+  for (int k = 0; k < c->nbatch; k++)
+    c->lay.pack_vec(c->hstage + (size_t)k * c->lay.op_size, vec + (size_t)k * c->lay.neq);
   return 0;
 }
 int imsrg_op_copy(imsrg_op src, imsrg_op dst) {
   if (!src || !dst || src->ctx != dst->ctx) return fail(IMSRG_EINVAL, "bad operands");
-  op_copy(src, dst);
+  op_copy(src, dst, all_set(src->ctx));
   return 0;
 }
 int imsrg_op_clear(imsrg_op op) {
   if (!op) return fail(IMSRG_EINVAL, "null operand");
-  launch_fill(op->ctx->stream, op->d, op->ctx->lay.op_size, 0.0);
+  launch_fill(op->ctx->stream, op->d, op->ctx->lay.op_size * op->ctx->nbatch, 0.0);
   return 0;
 }
 int imsrg_op_scale(imsrg_op op, double a) {
   if (!op) return fail(IMSRG_EINVAL, "null operand");
-  launch_axpby(op->ctx->stream, op->ctx->lay.op_size, a, op->d, 0.0, op->d, op->d);
+  imsrg_ctx_s *c = op->ctx;
+  const ActSet all = all_set(c);
+  launch_axpby(c->stream, c->lay.op_size, a, op->d, 0.0, op->d, op->d, all.d, all.n(), c->lay.op_size);
   return 0;
 }
 int imsrg_op_add(imsrg_op A, double ax, imsrg_op B, double bx, imsrg_op C) {
   if (!A || !B || !C || A->ctx != B->ctx || A->ctx != C->ctx) return fail(IMSRG_EINVAL, "bad operands");
-  launch_axpby(A->ctx->stream, A->ctx->lay.op_size, ax, A->d, bx, B->d, C->d);
+  imsrg_ctx_s *c = A->ctx;
+  const ActSet all = all_set(c);
+  launch_axpby(c->stream, c->lay.op_size, ax, A->d, bx, B->d, C->d, all.d, all.n(), c->lay.op_size);
   return 0;
+}
+int imsrg_op_e0_batch(imsrg_op op, double *e0) {
+  if (!op || !e0) return fail(IMSRG_EINVAL, "null argument");
+  imsrg_ctx_s *c = op->ctx;
+  const ActSet all = all_set(c);
+  launch_gather_e0(c->stream, op->d, c->lay.op_size, c->scal + 1, all.d, all.n());
+  std::vector<double> v;
+  int rc = read_scalars(c, 1, all, v);
+  if (!rc) std::copy(v.begin(), v.end(), e0);
+  return rc;
 }
 int imsrg_op_get_e0(imsrg_op op, double *e0) {
   if (!op || !e0) return fail(IMSRG_EINVAL, "null argument");
-  imsrg_ctx_s *c = op->ctx;
-  HIPCHK(hipMemcpyAsync(c->hscal + 1, op->d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  *e0 = c->hscal[1];
-  return 0;
+  if (int rc = single_only(op->ctx, "imsrg_op_get_e0")) return rc;
+  return imsrg_op_e0_batch(op, e0);
 }
 int imsrg_op_set_e0(imsrg_op op, double e0) {
   if (!op) return fail(IMSRG_EINVAL, "null operand");
-  launch_fill(op->ctx->stream, op->d, 1, e0);
+  imsrg_ctx_s *c = op->ctx;
+  const ActSet all = all_set(c);
+  launch_fill(c->stream, op->d, 1, e0, all.d, all.n(), c->lay.op_size);
   return 0;
+}
+int imsrg_op_norm_batch(imsrg_op op, double *out) {
+  if (!op || !out) return fail(IMSRG_EINVAL, "null argument");
+  std::vector<double> v;
+  int rc = op_norms(op, all_set(op->ctx), v);
+  if (!rc) std::copy(v.begin(), v.end(), out);
+  return rc;
 }
 int imsrg_op_norm(imsrg_op op, double *out) {
   if (!op || !out) return fail(IMSRG_EINVAL, "null argument");
-  return op_norm(op, out);
+  if (int rc = single_only(op->ctx, "imsrg_op_norm")) return rc;
+  return imsrg_op_norm_batch(op, out);
+}
+static int mbpt2_set(imsrg_op_s *H, const ActSet &a, std::vector<double> &out) {
+  imsrg_ctx_s *c = H->ctx;
+  PROF(c, IMSRG_PROF_NORM, launch_mbpt2(c->stream, c->dl, H->d, c->partials, c->scal + 2, a.d, a.n()));
+  return read_scalars(c, 2, a, out);
+}
+int imsrg_mbpt2_batch(imsrg_op H, double *out) {
+  if (!H || !out) return fail(IMSRG_EINVAL, "null argument");
+  std::vector<double> v;
+  int rc = mbpt2_set(H, all_set(H->ctx), v);
+  if (!rc) std::copy(v.begin(), v.end(), out);
+  return rc;
 }
 int imsrg_mbpt2(imsrg_op H, double *out) {
   if (!H || !out) return fail(IMSRG_EINVAL, "null argument");
-  imsrg_ctx_s *c = H->ctx;
-  PROF(c, IMSRG_PROF_NORM, launch_mbpt2(c->stream, c->dl, H->d, c->partials, c->scal + 2));
-  return read_scalar(c, 2, out);
+  if (int rc = single_only(H->ctx, "imsrg_mbpt2")) return rc;
+  return imsrg_mbpt2_batch(H, out);
 }
 int imsrg_normal_order(imsrg_op H) {
   if (!H) return fail(IMSRG_EINVAL, "null operand");
   imsrg_ctx_s *c = H->ctx;
   imsrg_op_s *T = tmp_op(c, 5, H->herm);
   if (!T) return IMSRG_ENOMEM;
-  op_copy(H, T);
-  launch_normal_order(c->stream, c->dl, T->d, H->d, c->partials);
+  const ActSet all = all_set(c);
+  op_copy(H, T, all);
+  launch_normal_order(c->stream, c->dl, T->d, H->d, all.d, all.n());
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -502,32 +609,37 @@ int imsrg_normal_order(imsrg_op H) {
 // ---------------------------------------------------------------- commutator
 // lcc_ready: cc[0] already holds the Pandya matrices of L (drivers reuse them
 // across the nested commutators of one BCH / Magnus series, where L = Omega).
+// Every launch covers the instances of `a`.
 static int comm_impl(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *RES, unsigned mask,
-                     bool lcc_ready) {
+                     bool lcc_ready, const ActSet &a) {
   hipStream_t s = c->stream;
   const DevLayout &D = c->dl;
   const int hL = L->herm, hR = R->herm, hres = RES->herm;
-  if (mask & (T110 | T220)) PROF(c, IMSRG_PROF_E0, launch_e0(s, D, L->d, R->d, RES->d, hL, hR, mask, c->partials));
+  const int *ad = a.d, na = a.n();
+  const long long OS = D.op_size;
+  if (na == 0) return 0;
+  if (mask & (T110 | T220))
+    PROF(c, IMSRG_PROF_E0, launch_e0(s, D, L->d, R->d, RES->d, hL, hR, mask, c->partials, ad, na, OS));
   if (mask & T222PH) {
-    if (!lcc_ready) PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, L->d, c->cc[0]));
-    PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, R->d, c->cc[1]));
+    if (!lcc_ready) PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, L->d, c->cc[0], ad, na));
+    PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, R->d, c->cc[1], ad, na));
     PROF(c, IMSRG_PROF_GEMM_PH,
          launch_seg_gemm_z(s, c->prec, c->ph_probs, c->ph_tiles, c->n_ph_tiles, c->cc[0], c->cc[1], c->wph,
-                           (double)hL, (double)hR));
+                           (double)hL, (double)hR, ad, na, D.ccsize, D.cwsize));
   }
-  if (mask & T122) PROF(c, IMSRG_PROF_122, launch_122(s, D, L->d, R->d, RES->d, hres));
-  else launch_clear_blocks(s, D, RES->d);
+  if (mask & T122) PROF(c, IMSRG_PROF_122, launch_122(s, D, L->d, R->d, RES->d, hres, ad, na));
+  else launch_clear_blocks(s, D, RES->d, ad, na);
   if (mask & (T222PPHH | T221))
     PROF(c, IMSRG_PROF_GEMM_PPHH,
          launch_seg_gemm_pphh(s, c->prec, c->pp_probs, c->pp_tiles, c->n_pp_tiles, L->d + D.v_off,
                               R->d + D.v_off, (mask & T222PPHH) ? RES->d + D.v_off : nullptr,
-                              (mask & T221) ? c->wpp : nullptr));
+                              (mask & T221) ? c->wpp : nullptr, ad, na, OS, D.vsize));
   if (mask & T222PH) {
-    PROF(c, IMSRG_PROF_PH_TERMS, launch_ph_terms(s, D, c->wph, c->tph, (double)hL, (double)hR));
-    PROF(c, IMSRG_PROF_PH_APPLY, launch_ph_apply(s, D, c->tph, RES->d));
+    PROF(c, IMSRG_PROF_PH_TERMS, launch_ph_terms(s, D, c->wph, c->tph, (double)hL, (double)hR, ad, na));
+    PROF(c, IMSRG_PROF_PH_APPLY, launch_ph_apply(s, D, c->tph, RES->d, ad, na));
   }
-  PROF(c, IMSRG_PROF_MIRROR, launch_sym_mirror(s, D, RES->d, hres));
-  PROF(c, IMSRG_PROF_ONEBODY, launch_onebody(s, D, L->d, R->d, c->wpp, RES->d, hL, hR, hres, mask));
+  PROF(c, IMSRG_PROF_MIRROR, launch_sym_mirror(s, D, RES->d, hres, ad, na));
+  PROF(c, IMSRG_PROF_ONEBODY, launch_onebody(s, D, L->d, R->d, c->wpp, RES->d, hL, hR, hres, mask, ad, na));
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -537,14 +649,19 @@ int imsrg_commutator(imsrg_ctx c, imsrg_op L, imsrg_op R, imsrg_op RES, unsigned
   if (L->ctx != c || R->ctx != c || RES->ctx != c) return fail(IMSRG_EINVAL, "operands of another context");
   if (RES == L || RES == R) return fail(IMSRG_EINVAL, "RES must not alias L or R");
   if (RES->herm != -L->herm * R->herm) return fail(IMSRG_EINVAL, "RES herm must be -herm(L)*herm(R)");
-  return comm_impl(c, L, R, RES, mask & IMSRG_TALL, false);
+  return comm_impl(c, L, R, RES, mask & IMSRG_TALL, false, all_set(c));
 }
 
 static int scalar_term(imsrg_op L, imsrg_op R, unsigned mask, double *out) {
   if (!L || !R || !out || L->ctx != R->ctx) return fail(IMSRG_EINVAL, "bad operands");
   imsrg_ctx_s *c = L->ctx;
-  launch_e0(c->stream, c->dl, L->d, R->d, c->scal + 3, L->herm, R->herm, mask, c->partials);
-  return read_scalar(c, 3, out);
+  if (int rc = single_only(c, "commutator_110/220")) return rc;
+  const ActSet all = all_set(c);
+  launch_e0(c->stream, c->dl, L->d, R->d, c->scal + 3, L->herm, R->herm, mask, c->partials, all.d, all.n(), kScal);
+  std::vector<double> v;
+  int rc = read_scalars(c, 3, all, v);
+  if (!rc) *out = v[0];
+  return rc;
 }
 int imsrg_commutator_110(imsrg_op L, imsrg_op R, double *out) { return scalar_term(L, R, T110, out); }
 int imsrg_commutator_220(imsrg_op L, imsrg_op R, double *out) { return scalar_term(L, R, T220, out); }
@@ -558,8 +675,10 @@ int64_t imsrg_cross_coupled_size(imsrg_ctx c) {
 int imsrg_cross_coupled(imsrg_op op, double *host_out, int64_t n) {
   if (!op || !host_out) return fail(IMSRG_EINVAL, "null argument");
   imsrg_ctx_s *c = op->ctx;
+  if (int rc = single_only(c, "imsrg_cross_coupled")) return rc;
   if (n != imsrg_cross_coupled_size(c)) return fail(IMSRG_EINVAL, "size mismatch");
-  launch_pandya(c->stream, c->dl, op->d, c->cc[0]);
+  const ActSet all = all_set(c);
+  launch_pandya(c->stream, c->dl, op->d, c->cc[0], all.d, all.n());
   int64_t k = 0;
   for (int q = 0; q < c->lay.ncc; q++) {
     int64_t m = (int64_t)c->lay.ccrlen[q] * c->lay.ccnph[q];
@@ -573,27 +692,28 @@ int imsrg_cross_coupled(imsrg_op op, double *host_out, int64_t n) {
 }
 
 // ---------------------------------------------------------------- generators
-static void gen_simple(imsrg_ctx_s *c, imsrg_op_s *Hden, imsrg_op_s *Hnum, imsrg_op_s *ETA, int kind) {
+static void gen_simple(imsrg_ctx_s *c, imsrg_op_s *Hden, imsrg_op_s *Hnum, imsrg_op_s *ETA, int kind,
+                       const ActSet &a) {
   hipStream_t s = c->stream;
   PROF(c, IMSRG_PROF_GENERATOR, {
-    launch_fill(s, ETA->d + c->lay.f_off, c->lay.op_size - c->lay.f_off, 0.0);
-    launch_monopole(s, c->dl, Hden->d, c->mono);
-    launch_eta(s, c->dl, Hden->d, Hnum->d, c->mono, ETA->d, kind);
+    launch_fill(s, ETA->d + c->lay.f_off, c->lay.op_size - c->lay.f_off, 0.0, a.d, a.n(), c->lay.op_size);
+    launch_monopole(s, c->dl, Hden->d, c->mono, a.d, a.n());
+    launch_eta(s, c->dl, Hden->d, Hnum->d, c->mono, ETA->d, kind, a.d, a.n());
   });
   ETA->herm = -1;
 }
-static int gen_impl(imsrg_ctx_s *c, int kind, imsrg_op_s *H, imsrg_op_s *ETA) {
+static int gen_impl(imsrg_ctx_s *c, int kind, imsrg_op_s *H, imsrg_op_s *ETA, const ActSet &a) {
   if (kind == IMSRG_GEN_WHITE || kind == IMSRG_GEN_ATAN) {
-    gen_simple(c, H, H, ETA, kind == IMSRG_GEN_WHITE ? 0 : 1);
+    gen_simple(c, H, H, ETA, kind == IMSRG_GEN_WHITE ? 0 : 1, a);
   } else if (kind == IMSRG_GEN_W2) {
     // build_gs_w2 (generators.f90:109-160)
     imsrg_op_s *E1 = tmp_op(c, 2, -1), *H2 = tmp_op(c, 3, 1), *E2 = tmp_op(c, 4, -1);
     if (!E1 || !H2 || !E2) return IMSRG_ENOMEM;
-    gen_simple(c, H, H, E1, 1);
-    int rc = comm_impl(c, E1, H, H2, IMSRG_TALL, false);
+    gen_simple(c, H, H, E1, 1, a);
+    int rc = comm_impl(c, E1, H, H2, IMSRG_TALL, false, a);
     if (rc) return rc;
-    gen_simple(c, H, H2, E2, 1);
-    launch_axpby(c->stream, c->lay.op_size, 1.5, E1->d, 0.5, E2->d, ETA->d);
+    gen_simple(c, H, H2, E2, 1, a);
+    launch_axpby(c->stream, c->lay.op_size, 1.5, E1->d, 0.5, E2->d, ETA->d, a.d, a.n(), c->lay.op_size);
     ETA->herm = -1;
   } else {
     return fail(IMSRG_EINVAL, "unknown generator kind");
@@ -603,11 +723,11 @@ static int gen_impl(imsrg_ctx_s *c, int kind, imsrg_op_s *H, imsrg_op_s *ETA) {
 }
 int imsrg_generator(imsrg_ctx c, int kind, imsrg_op H, imsrg_op ETA) {
   if (!c || !H || !ETA || H == ETA) return fail(IMSRG_EINVAL, "bad operands");
-  return gen_impl(c, kind, H, ETA);
+  return gen_impl(c, kind, H, ETA, all_set(c));
 }
 int imsrg_generator_atanxx(imsrg_ctx c, imsrg_op Hold, imsrg_op H, imsrg_op ETA) {
   if (!c || !Hold || !H || !ETA || ETA == H || ETA == Hold) return fail(IMSRG_EINVAL, "bad operands");
-  gen_simple(c, Hold, H, ETA, 1);
+  gen_simple(c, Hold, H, ETA, 1, all_set(c));
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -616,88 +736,133 @@ int imsrg_generator_atanxx(imsrg_ctx c, imsrg_op Hold, imsrg_op H, imsrg_op ETA)
 static const double kConv = (double)1e-6f;   // real(8), parameter :: conv = 1e-6
 static const double kTiny = (double)1e-9f;   // fullnorm < 1e-9 (default-real literal)
 
-// BCH_EXPAND (IMSRG_MAGNUS.f90:149-234)
-static int bch_impl(imsrg_ctx_s *c, imsrg_op_s *HS, imsrg_op_s *G, imsrg_op_s *H, int *nterms) {
+// BCH_EXPAND (IMSRG_MAGNUS.f90:149-234) on the instances of `a`.  Every
+// instance keeps its own truncation: after each nested commutator the
+// instances whose ||ad^n||/n! fell below conv leave the working set, so one
+// launch per kernel serves all instances still expanding.  nterms[k] is the
+// number of nested commutators instance a.h[k] used.
+static int bch_impl(imsrg_ctx_s *c, imsrg_op_s *HS, imsrg_op_s *G, imsrg_op_s *H, const ActSet &a,
+                    std::vector<int> &nterms) {
   imsrg_op_s *AD = tmp_op(c, 0, 1), *INT2 = tmp_op(c, 1, 1);
   if (!AD || !INT2) return IMSRG_ENOMEM;
-  op_copy(H, HS);
-  op_copy(H, AD);  // AD_1 = H
+  nterms.assign(a.n(), 0);
+  if (a.n() == 0) return 0;
+  op_copy(H, HS, a);
+  op_copy(H, AD, a);  // AD_1 = H
   AD->herm = 1;
   INT2->herm = 1;
-  launch_pandya(c->stream, c->dl, G->d, c->cc[0]);
+  const long long OS = c->lay.op_size;
+  launch_pandya(c->stream, c->dl, G->d, c->cc[0], a.d, a.n());
+  std::vector<int> pos(c->nbatch, -1);
+  for (int k = 0; k < a.n(); k++) pos[a.h[k]] = k;
+  ActSet cur = a;
   double coef = 1.0;
-  int n = 0, iw;
+  int iw;
   for (iw = 2; iw <= 40; iw++) {
     coef = coef / (iw - 1.0);
-    int rc = comm_impl(c, G, AD, INT2, IMSRG_TALL, true);
+    int rc = comm_impl(c, G, AD, INT2, IMSRG_TALL, true, cur);
     if (rc) return rc;
-    n++;
-    PROF(c, IMSRG_PROF_AXPBY, launch_axpby(c->stream, c->lay.op_size, 1.0, HS->d, coef, INT2->d, HS->d));
-    double nrm;
-    rc = op_norm(INT2, &nrm);
+    PROF(c, IMSRG_PROF_AXPBY, launch_axpby(c->stream, OS, 1.0, HS->d, coef, INT2->d, HS->d, cur.d, cur.n(), OS));
+    std::vector<double> nrm;
+    rc = op_norms(INT2, cur, nrm);
     if (rc) return rc;
-    if (nrm * coef < kConv) break;
+    std::vector<int> next;
+    for (int k = 0; k < cur.n(); k++) {
+      nterms[pos[cur.h[k]]]++;
+      if (!(nrm[k] * coef < kConv)) next.push_back(cur.h[k]);
+    }
+    if (next.empty()) break;
     std::swap(c->tmp[0], c->tmp[1]);
     AD = c->tmp[0];
     INT2 = c->tmp[1];
+    if (next.size() != cur.h.size()) cur = make_set(c, 1, next);
   }
-  if (nterms) *nterms = n;
   if (iw > 40) return fail(IMSRG_ECLIP, "clipping BCH");
   return 0;
 }
 int imsrg_bch_expand(imsrg_ctx c, imsrg_op HS, imsrg_op G, imsrg_op H, int *nterms) {
   if (!c || !HS || !G || !H || HS == G || HS == H) return fail(IMSRG_EINVAL, "bad operands");
-  return bch_impl(c, HS, G, H, nterms);
+  std::vector<int> nt;
+  int rc = bch_impl(c, HS, G, H, all_set(c), nt);
+  if (!rc && nterms) std::copy(nt.begin(), nt.end(), nterms);
+  return rc;
 }
 
-// MAGNUS_EXPAND (IMSRG_MAGNUS.f90:622-693)
-static int magexp_impl(imsrg_ctx_s *c, imsrg_op_s *DG, imsrg_op_s *G, int *nterms) {
+// MAGNUS_EXPAND (IMSRG_MAGNUS.f90:622-693) on the instances of `a`, each with
+// its own early exits; nterms as in bch_impl.
+static int magexp_impl(imsrg_ctx_s *c, imsrg_op_s *DG, imsrg_op_s *G, const ActSet &a, std::vector<int> &nterms) {
   static const double cof[7] = {1.0, -0.5, .0833333333, 0.0, -0.00138888888, 0.0, 3.306878e-5};
-  double advals[7] = {0, 0, 0, 0, 0, 0, 0};
+  nterms.assign(a.n(), 0);
+  if (a.n() == 0) return 0;
   imsrg_op_s *cur = tmp_op(c, 0, DG->herm), *oth = tmp_op(c, 1, DG->herm);
   if (!cur || !oth) return IMSRG_ENOMEM;
-  op_copy(DG, cur);
-  op_copy(DG, oth);
-  int rc = op_norm(cur, &advals[0]);
+  const long long OS = c->lay.op_size;
+  op_copy(DG, cur, a);
+  op_copy(DG, oth, a);
+  std::vector<double> ad0, fullnorm;
+  int rc = op_norms(cur, a, ad0);
+  if (!rc) rc = op_norms(G, a, fullnorm);
   if (rc) return rc;
-  double fullnorm;
-  rc = op_norm(G, &fullnorm);
-  if (rc) return rc;
-  int n = 0;
-  if (nterms) *nterms = 0;
-  if (fullnorm < kTiny) return 0;
-  launch_pandya(c->stream, c->dl, G->d, c->cc[0]);
+  // per instance (position k in a): advals(1..7)
+  std::vector<std::vector<double>> advals(a.n(), std::vector<double>(7, 0.0));
+  std::vector<int> live;  // positions in a
+  for (int k = 0; k < a.n(); k++) {
+    advals[k][0] = ad0[k];
+    if (!(fullnorm[k] < kTiny)) live.push_back(k);
+  }
+  if (live.empty()) return 0;
+  auto to_set = [&](const std::vector<int> &ks, bool same_as_a) {
+    if (same_as_a) return a;
+    std::vector<int> inst;
+    for (int k : ks) inst.push_back(a.h[k]);
+    return make_set(c, 2, inst);
+  };
+  ActSet curset = to_set(live, (int)live.size() == a.n());
+  launch_pandya(c->stream, c->dl, G->d, c->cc[0], curset.d, curset.n());
   int q = 1;
   for (int i = 2; i <= 7; i++) {
-    double adnorm = advals[i - q - 1];
+    const int qprev = q;
     q = (std::fabs(cof[i - 1]) > kConv) ? 1 : 2;
-    if (std::fabs(adnorm / fullnorm) < kConv) break;
-    rc = comm_impl(c, G, cur, oth, IMSRG_TALL & ~(T110 | T220), true);
+    std::vector<int> keep;
+    for (int k : live)
+      if (!(std::fabs(advals[k][i - qprev - 1] / fullnorm[k]) < kConv)) keep.push_back(k);
+    if (keep.empty()) break;
+    if (keep.size() != live.size()) curset = to_set(keep, false);
+    live = keep;
+    rc = comm_impl(c, G, cur, oth, IMSRG_TALL & ~(T110 | T220), true, curset);
     if (rc) return rc;
-    n++;
-    PROF(c, IMSRG_PROF_AXPBY, launch_axpby(c->stream, c->lay.op_size, 1.0, DG->d, cof[i - 1], oth->d, DG->d));
-    double nrm;
-    rc = op_norm(oth, &nrm);
+    PROF(c, IMSRG_PROF_AXPBY,
+         launch_axpby(c->stream, OS, 1.0, DG->d, cof[i - 1], oth->d, DG->d, curset.d, curset.n(), OS));
+    std::vector<double> nrm;
+    rc = op_norms(oth, curset, nrm);
     if (rc) return rc;
-    advals[i - 1] = nrm * std::fabs(cof[i - 1]);
+    for (size_t m = 0; m < live.size(); m++) {
+      advals[live[m]][i - 1] = nrm[m] * std::fabs(cof[i - 1]);
+      nterms[live[m]]++;
+    }
     std::swap(cur, oth);
   }
-  if (nterms) *nterms = n;
   return 0;
 }
 int imsrg_magnus_expand(imsrg_ctx c, imsrg_op DG, imsrg_op G, int *nterms) {
   if (!c || !DG || !G || DG == G) return fail(IMSRG_EINVAL, "bad operands");
-  return magexp_impl(c, DG, G, nterms);
+  std::vector<int> nt;
+  int rc = magexp_impl(c, DG, G, all_set(c), nt);
+  if (!rc && nterms) std::copy(nt.begin(), nt.end(), nterms);
+  return rc;
 }
 
 // magnus_decouple (IMSRG_MAGNUS.f90:15-128) without checkpointing, split into
-// set-up (:31-96) and one pass of the step loop (:98-126)
+// set-up (:31-96) and one pass of the step loop (:98-126).  With a batched
+// context every instance flows on its own: an instance whose |MBPT2| has
+// fallen to conv stops stepping while the others go on.
 struct imsrg_flow_s {
   imsrg_ctx_s *ctx = nullptr;
   imsrg_op_s *HS = nullptr, *G = nullptr, *H0 = nullptr, *DG = nullptr;
   int gen_kind = 0;
-  double ds = 0.5, s = 0.0, crit = 0.0, emp2 = 0.0;
-  int steps = 0;
+  double ds = 0.5;
+  std::vector<double> s, crit, emp2;
+  std::vector<int> steps;
 };
 
 int imsrg_flow_create(imsrg_ctx c, imsrg_op HS, imsrg_op G, int gen_kind, double ds, imsrg_flow *out) {
@@ -705,17 +870,20 @@ int imsrg_flow_create(imsrg_ctx c, imsrg_op HS, imsrg_op G, int gen_kind, double
   if (HS->ctx != c || G->ctx != c) return fail(IMSRG_EINVAL, "operands of another context");
   imsrg_flow_s *f = new imsrg_flow_s;
   f->ctx = c; f->HS = HS; f->G = G; f->gen_kind = gen_kind; f->ds = ds;
+  const int B = c->nbatch;
+  f->s.assign(B, 0.0); f->crit.assign(B, 0.0); f->emp2.assign(B, 0.0); f->steps.assign(B, 0);
+  const ActSet all = all_set(c);
   int rc = op_new(c, 1, &f->H0);
   if (!rc) rc = op_new(c, -1, &f->DG);
   if (!rc) {
-    launch_fill(c->stream, G->d, c->lay.op_size, 0.0);
+    launch_fill(c->stream, G->d, c->lay.op_size * B, 0.0);
     G->herm = -1;
-    rc = gen_impl(c, gen_kind, HS, f->DG);
+    rc = gen_impl(c, gen_kind, HS, f->DG, all);
   }
   if (!rc) {
-    op_copy(HS, f->H0);
-    rc = imsrg_mbpt2(HS, &f->emp2);
-    f->crit = std::fabs(f->emp2);
+    op_copy(HS, f->H0, all);
+    rc = mbpt2_set(HS, all, f->emp2);
+    for (int k = 0; k < B; k++) f->crit[k] = std::fabs(f->emp2[k]);
   }
   if (rc) {
     imsrg_flow_destroy(f);
@@ -725,36 +893,63 @@ int imsrg_flow_create(imsrg_ctx c, imsrg_op HS, imsrg_op G, int gen_kind, double
   return 0;
 }
 
-int imsrg_flow_step(imsrg_flow f, double *rec8) {
+// rec: 8 doubles per instance (s, E0, E0+MBPT2, crit, Magnus terms, BCH
+// terms, steps taken, 1 if the instance stepped in this call else 0)
+int imsrg_flow_step(imsrg_flow f, double *rec) {
   if (!f) return fail(IMSRG_EINVAL, "null flow");
   imsrg_ctx_s *c = f->ctx;
-  int nm = 0, nb = 0;
-  int rc = magexp_impl(c, f->DG, f->G, &nm);
-  if (rc) return rc;
-  PROF(c, IMSRG_PROF_AXPBY,
-       launch_axpby(c->stream, c->lay.op_size, 1.0, f->G->d, f->ds, f->DG->d, f->G->d));  // euler_step :756-766
-  f->s = f->s + f->ds;
-  rc = bch_impl(c, f->HS, f->G, f->H0, &nb);
-  if (rc) return rc;
-  rc = gen_impl(c, f->gen_kind, f->HS, f->DG);
-  if (rc) return rc;
-  rc = imsrg_mbpt2(f->HS, &f->emp2);
-  if (rc) return rc;
-  f->crit = std::fabs(f->emp2);
-  double e0 = 0.0;
-  rc = imsrg_op_get_e0(f->HS, &e0);
-  if (rc) return rc;
-  f->steps++;
-  if (rec8) {
-    const double r[8] = {f->s, e0, e0 + f->emp2, f->crit, (double)nm, (double)nb, (double)f->steps, 0.0};
-    std::memcpy(rec8, r, sizeof(r));
+  const int B = c->nbatch;
+  std::vector<int> live;
+  for (int k = 0; k < B; k++)
+    if (f->crit[k] > kConv) live.push_back(k);
+  std::vector<int> nm(B, 0), nb(B, 0);
+  std::vector<double> e0(B, 0.0);
+  int rc = 0;
+  if (!live.empty()) {
+    const ActSet a = ((int)live.size() == B) ? all_set(c) : make_set(c, 3, live);
+    std::vector<int> tm, tb;
+    rc = magexp_impl(c, f->DG, f->G, a, tm);
+    if (rc) return rc;
+    PROF(c, IMSRG_PROF_AXPBY,
+         launch_axpby(c->stream, c->lay.op_size, 1.0, f->G->d, f->ds, f->DG->d, f->G->d, a.d, a.n(),
+                      c->lay.op_size));  // euler_step :756-766
+    rc = bch_impl(c, f->HS, f->G, f->H0, a, tb);
+    if (rc) return rc;
+    rc = gen_impl(c, f->gen_kind, f->HS, f->DG, a);
+    if (rc) return rc;
+    std::vector<double> mp;
+    rc = mbpt2_set(f->HS, a, mp);
+    if (rc) return rc;
+    for (int m = 0; m < a.n(); m++) {
+      const int k = a.h[m];
+      f->s[k] += f->ds;
+      f->emp2[k] = mp[m];
+      f->crit[k] = std::fabs(mp[m]);
+      f->steps[k]++;
+      nm[k] = tm[m];
+      nb[k] = tb[m];
+    }
+  }
+  if (rec) {
+    rc = imsrg_op_e0_batch(f->HS, e0.data());
+    if (rc) return rc;
+    std::vector<char> stepped(B, 0);
+    for (int k : live) stepped[k] = 1;
+    for (int k = 0; k < B; k++) {
+      const double r[8] = {f->s[k], e0[k], e0[k] + f->emp2[k], f->crit[k], (double)nm[k], (double)nb[k],
+                           (double)f->steps[k], stepped[k] ? 1.0 : 0.0};
+      std::memcpy(rec + 8 * (size_t)k, r, sizeof(r));
+    }
   }
   return 0;
 }
 
 int imsrg_flow_converged(imsrg_flow f, int *yes) {
   if (!f || !yes) return fail(IMSRG_EINVAL, "null argument");
-  *yes = !(f->crit > kConv);  // do while (crit > 1e-6)
+  int all = 1;
+  for (double cr : f->crit)
+    if (cr > kConv) all = 0;  // do while (crit > 1e-6)
+  *yes = all;
   return 0;
 }
 
@@ -769,6 +964,8 @@ int imsrg_flow_destroy(imsrg_flow f) {
 int imsrg_magnus_flow(imsrg_ctx c, imsrg_op HS, imsrg_op G, int gen_kind, double ds, int max_steps,
                       double *s_out, double *e0_out, double *e0mbpt2_out, double *crit_out,
                       int *nmag_out, int *nbch_out, int *steps_out) {
+  if (!c) return fail(IMSRG_EINVAL, "null context");
+  if (int rc = single_only(c, "imsrg_magnus_flow")) return rc;
   imsrg_flow f = nullptr;
   int rc = imsrg_flow_create(c, HS, G, gen_kind, ds, &f);
   if (rc) return rc;

@@ -18,6 +18,11 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 
 #define HIP_KERNEL_CHECK() ((void)0)
 
+// Instance of a batched launch: blockIdx.y indexes the active-instance table
+// (null table: one instance, number 0).  Instance k of an operator, Pandya
+// matrix or workspace starts k strides after the buffer base.
+__device__ __forceinline__ long long inst_id(const int *act) { return act ? act[blockIdx.y] : 0; }
+
 __device__ __forceinline__ int xcd_item(const XcdSplit &x) {
   const int xcd = blockIdx.x & 7, it = x.start[xcd] + (int)(blockIdx.x >> 3);
   return it < x.start[xcd + 1] ? it : -1;
@@ -67,30 +72,53 @@ __device__ double dv(const DevLayout &L, const double *V, int a, int b, int c, i
 }
 
 // ------------------------------------------------------------------ elementwise
-__global__ void k_fill(double *p, long long n, double v) {
+__global__ void k_fill(double *p, long long n, double v, const int *act, long long stride) {
+  p += inst_id(act) * stride;
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long st = (long long)gridDim.x * blockDim.x;
   for (; i < n; i += st) p[i] = v;
 }
-void launch_fill(hipStream_t s, double *p, long long n, double v) {
-  if (n <= 0) return;
+void launch_fill(hipStream_t s, double *p, long long n, double v, const int *act, int nact, long long stride) {
+  if (n <= 0 || nact <= 0) return;
   long long nb = (n + 255) / 256;
   int grid = (int)(nb < 8192 ? nb : 8192);
-  hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, s, p, n, v);
+  hipLaunchKernelGGL(k_fill, dim3(grid, nact), dim3(256), 0, s, p, n, v, act, stride);
 }
 
 // add_sq_op (basic_IMSRG.f90:3450-3478): C = A*ax + B*bx elementwise
-__global__ void k_axpby(long long n, double a, const double *A, double b, const double *B, double *C) {
+__global__ void k_axpby(long long n, double a, const double *A, double b, const double *B, double *C,
+                        const int *act, long long stride) {
+  const long long o = inst_id(act) * stride;
+  A += o;
+  B += o;
+  C += o;
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long st = (long long)gridDim.x * blockDim.x;
   for (; i < n; i += st) C[i] = A[i] * a + B[i] * b;
 }
 void launch_axpby(hipStream_t s, long long n, double a, const double *A, double b, const double *B,
-                  double *C) {
-  if (n <= 0) return;
+                  double *C, const int *act, int nact, long long stride) {
+  if (n <= 0 || nact <= 0) return;
   long long nb = (n + 255) / 256;
   int grid = (int)(nb < 8192 ? nb : 8192);
-  hipLaunchKernelGGL(k_axpby, dim3(grid), dim3(256), 0, s, n, a, A, b, B, C);
+  hipLaunchKernelGGL(k_axpby, dim3(grid, nact), dim3(256), 0, s, n, a, A, b, B, C, act, stride);
+}
+
+// copy_sq_op (basic_IMSRG.f90:2942-2969) on the active instances
+__global__ void k_copy(long long n, const double *A, double *C, const int *act, long long stride) {
+  const long long o = inst_id(act) * stride;
+  A += o;
+  C += o;
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long st = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += st) C[i] = A[i];
+}
+void launch_copy(hipStream_t s, long long n, const double *A, double *C, const int *act, int nact,
+                 long long stride) {
+  if (n <= 0 || nact <= 0) return;
+  long long nb = (n + 255) / 256;
+  int grid = (int)(nb < 8192 ? nb : 8192);
+  hipLaunchKernelGGL(k_copy, dim3(grid, nact), dim3(256), 0, s, n, A, C, act, stride);
 }
 
 // ------------------------------------------------------------------ segmented grouped GEMM
@@ -409,9 +437,17 @@ __global__ __launch_bounds__(256) void k_seg_gemm(const SegProb *__restrict__ pr
                                                   const double *__restrict__ X0,
                                                   const double *__restrict__ X1, double g0, double g1,
                                                   double *__restrict__ C, double *__restrict__ W,
-                                                  double sigma) {
+                                                  double sigma, const int *act, long long sx,
+                                                  long long sc, long long sw) {
   const int t = blockIdx.x;
   if (t >= ntiles) return;
+  {
+    const long long inst = inst_id(act);
+    X0 += inst * sx;
+    X1 += inst * sx;
+    if (C) C += inst * sc;
+    if (W) W += inst * sw;
+  }
   const int4 tl = tiles[t];
   if (tl.x < 0) return;  // padding slot of the XCD-balanced schedule
   const SegProb &p = probs[tl.x];
@@ -491,36 +527,49 @@ __global__ __launch_bounds__(256) void k_seg_gemm(const SegProb *__restrict__ pr
 }
 
 void launch_seg_gemm_pphh(hipStream_t s, int mode, const SegProb *probs, const int4 *tiles, int ntiles,
-                          const double *X0, const double *X1, double *C, double *W) {
-  if (ntiles <= 0) return;
-  const dim3 g(ntiles), b(256);
+                          const double *X0, const double *X1, double *C, double *W, const int *act, int nact,
+                          long long sop, long long sw) {
+  if (ntiles <= 0 || nact <= 0) return;
+  const dim3 g(ntiles, nact), b(256);
   if (mode == 1)
-    hipLaunchKernelGGL((k_seg_gemm<false, 1, 0>), g, b, 0, s, probs, tiles, ntiles, X0, X1, 1.0, 1.0, C, W, 0.0);
+    hipLaunchKernelGGL((k_seg_gemm<false, 1, 0>), g, b, 0, s, probs, tiles, ntiles, X0, X1, 1.0, 1.0, C, W, 0.0,
+                       act, sop, sop, sw);
   else if (mode == 2)
-    hipLaunchKernelGGL((k_seg_gemm<false, 2, 0>), g, b, 0, s, probs, tiles, ntiles, X0, X1, 1.0, 1.0, C, W, 0.0);
+    hipLaunchKernelGGL((k_seg_gemm<false, 2, 0>), g, b, 0, s, probs, tiles, ntiles, X0, X1, 1.0, 1.0, C, W, 0.0,
+                       act, sop, sop, sw);
   else
-    hipLaunchKernelGGL((k_seg_gemm<false, 0, 0>), g, b, 0, s, probs, tiles, ntiles, X0, X1, 1.0, 1.0, C, W, 0.0);
+    hipLaunchKernelGGL((k_seg_gemm<false, 0, 0>), g, b, 0, s, probs, tiles, ntiles, X0, X1, 1.0, 1.0, C, W, 0.0,
+                       act, sop, sop, sw);
 }
 void launch_seg_gemm_z(hipStream_t s, int mode, const SegProb *probs, const int4 *tiles, int ntiles,
-                       const double *LCC, const double *RCC, double *Z, double hL, double hR) {
-  if (ntiles <= 0) return;
-  const dim3 g(ntiles), b(256);
+                       const double *LCC, const double *RCC, double *Z, double hL, double hR, const int *act,
+                       int nact, long long scc, long long sz) {
+  if (ntiles <= 0 || nact <= 0) return;
+  const dim3 g(ntiles, nact), b(256);
   const double g0 = hL, g1 = -hR, sigma = -hL * hR;
   if (mode == 1)
-    hipLaunchKernelGGL((k_seg_gemm<true, 1, 1>), g, b, 0, s, probs, tiles, ntiles, LCC, RCC, g0, g1, Z, nullptr, sigma);
+    hipLaunchKernelGGL((k_seg_gemm<true, 1, 1>), g, b, 0, s, probs, tiles, ntiles, LCC, RCC, g0, g1, Z, nullptr,
+                       sigma, act, scc, sz, 0LL);
   else if (mode == 2)
-    hipLaunchKernelGGL((k_seg_gemm<true, 2, 1>), g, b, 0, s, probs, tiles, ntiles, LCC, RCC, g0, g1, Z, nullptr, sigma);
+    hipLaunchKernelGGL((k_seg_gemm<true, 2, 1>), g, b, 0, s, probs, tiles, ntiles, LCC, RCC, g0, g1, Z, nullptr,
+                       sigma, act, scc, sz, 0LL);
   else
-    hipLaunchKernelGGL((k_seg_gemm<true, 0, 1>), g, b, 0, s, probs, tiles, ntiles, LCC, RCC, g0, g1, Z, nullptr, sigma);
+    hipLaunchKernelGGL((k_seg_gemm<true, 0, 1>), g, b, 0, s, probs, tiles, ntiles, LCC, RCC, g0, g1, Z, nullptr,
+                       sigma, act, scc, sz, 0LL);
 }
 
 // ------------------------------------------------------------------ Pandya transform
 // calculate_cross_coupled (cross_coupled.f90:1190-1300): one workgroup per
 // Pandya column (p,h), threads over rows (a,b); output column-major per block.
 __global__ __launch_bounds__(256) void k_pandya(DevLayout L, const double *__restrict__ op,
-                                                double *__restrict__ cc) {
+                                                double *__restrict__ cc, const int *act) {
   const int col = blockIdx.x;
   if (col >= L.nccol) return;
+  {
+    const long long inst = inst_id(act);
+    op += inst * L.op_size;
+    cc += inst * L.ccsize;
+  }
   const int q1 = L.cccol_q[col];
   const int c = col - L.cccbase[q1];
   const int p = L.cccol_p[col], h = L.cccol_h[col];
@@ -545,9 +594,9 @@ __global__ __launch_bounds__(256) void k_pandya(DevLayout L, const double *__res
     out[r] = val;
   }
 }
-void launch_pandya(hipStream_t s, const DevLayout &L, const double *op, double *cc) {
-  if (L.nccol <= 0) return;
-  hipLaunchKernelGGL(k_pandya, dim3(L.nccol), dim3(256), 0, s, L, op, cc);
+void launch_pandya(hipStream_t s, const DevLayout &L, const double *op, double *cc, const int *act, int nact) {
+  if (L.nccol <= 0 || nact <= 0) return;
+  hipLaunchKernelGGL(k_pandya, dim3(L.nccol, nact), dim3(256), 0, s, L, op, cc, act);
 }
 
 // ------------------------------------------------------------------ 122
@@ -561,9 +610,15 @@ void launch_pandya(hipStream_t s, const DevLayout &L, const double *op, double *
 // M(r,r)*(1+herm).  k_sym_mirror adds the transpose at the end.
 __global__ __launch_bounds__(64) void k_122(DevLayout L, const double *__restrict__ Lop,
                                             const double *__restrict__ Rop, double *__restrict__ RES,
-                                            double hres) {
+                                            double hres, const int *act) {
   const int r = xcd_item(L.xs_rows);
   if (r < 0) return;
+  {
+    const long long o = inst_id(act) * L.op_size;
+    Lop += o;
+    Rop += o;
+    RES += o;
+  }
   const int q = L.pair_q[r], rl = r - L.pbase[q], dim = L.bdim[q], N = L.nsp;
   const long long bo = L.v_off + L.boff[q];
   const double *LV = Lop + bo, *RV = Rop + bo;
@@ -582,22 +637,23 @@ __global__ __launch_bounds__(64) void k_122(DevLayout L, const double *__restric
   }
 }
 void launch_122(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES,
-                int hres) {
-  if (L.npairs <= 0) return;
-  hipLaunchKernelGGL(k_122, dim3(8 * L.xs_rows.len), dim3(64), 0, s, L, Lop, Rop, RES, (double)hres);
+                int hres, const int *act, int nact) {
+  if (L.npairs <= 0 || nact <= 0) return;
+  hipLaunchKernelGGL(k_122, dim3(8 * L.xs_rows.len, nact), dim3(64), 0, s, L, Lop, Rop, RES, (double)hres, act);
 }
 
 // zero every channel block (both triangles): the start state when 122 is off
-__global__ __launch_bounds__(64) void k_clear_rows(DevLayout L, double *RES) {
+__global__ __launch_bounds__(64) void k_clear_rows(DevLayout L, double *RES, const int *act) {
   const int r = blockIdx.x;
   if (r >= L.npairs) return;
+  RES += inst_id(act) * L.op_size;
   const int q = L.pair_q[r], rl = r - L.pbase[q], dim = L.bdim[q];
   double *OV = RES + L.v_off + L.boff[q] + (long long)rl * dim;
   for (int s = threadIdx.x; s < dim; s += 64) OV[s] = 0.0;
 }
-void launch_clear_blocks(hipStream_t s, const DevLayout &L, double *RES) {
-  if (L.npairs <= 0) return;
-  hipLaunchKernelGGL(k_clear_rows, dim3(L.npairs), dim3(64), 0, s, L, RES);
+void launch_clear_blocks(hipStream_t s, const DevLayout &L, double *RES, const int *act, int nact) {
+  if (L.npairs <= 0 || nact <= 0) return;
+  hipLaunchKernelGGL(k_clear_rows, dim3(L.npairs, nact), dim3(64), 0, s, L, RES, act);
 }
 
 // ------------------------------------------------------------------ 222_ph recoupling gather
@@ -613,10 +669,15 @@ void launch_clear_blocks(hipStream_t s, const DevLayout &L, double *RES) {
 // (i,k,j,l) so that runs of threads share Z rows); phase B adds the 6j-weighted
 // sums to every upper element, one wavefront per RES row.
 __global__ __launch_bounds__(256) void k_ph_terms(DevLayout L, const double *__restrict__ Z,
-                                                  double *__restrict__ Tbuf, double sigma) {
+                                                  double *__restrict__ Tbuf, double sigma, const int *act) {
   const int chunk = xcd_item(L.xs_terms);
   const long long t = (long long)chunk * 256 + threadIdx.x;
   if (chunk < 0 || t >= L.nphterm) return;
+  {
+    const long long inst = inst_id(act);
+    Z += inst * L.cwsize;
+    Tbuf += inst * L.nphterm;
+  }
   const unsigned tc = L.ph_terms[t];
   const unsigned code = L.ph_quads[tc >> 6];
   const int side = (tc >> 5) & 1, m = tc & 31;
@@ -654,10 +715,15 @@ __global__ __launch_bounds__(256) void k_ph_terms(DevLayout L, const double *__r
 // The 6j symbols of one row, {ji jj J; j4 j5 J'} for all (j4, j5, J'), are one
 // contiguous slice of the table; it is staged in LDS once per row.
 __global__ __launch_bounds__(64) void k_ph_apply(DevLayout L, const double *__restrict__ Tbuf,
-                                                 double *__restrict__ RES) {
+                                                 double *__restrict__ RES, const int *act) {
   extern __shared__ double S6[];  // nhalf * nhalf * nJ
   const int r = blockIdx.x;
   if (r >= L.npairs) return;
+  {
+    const long long inst = inst_id(act);
+    Tbuf += inst * L.nphterm;
+    RES += inst * L.op_size;
+  }
   const int q = L.pair_q[r], base = L.pbase[q], rl = r - base, dim = L.bdim[q], J = L.bJ[q];
   const int i = L.pair_a[r], j = L.pair_b[r];
   const int ji = L.jj[i], jj = L.jj[j], li = L.l[i], lj = L.l[j], ti = L.tz[i], tj = L.tz[j];
@@ -691,14 +757,16 @@ __global__ __launch_bounds__(64) void k_ph_apply(DevLayout L, const double *__re
     OV[s] = OV[s] + out * pre * pre2;
   }
 }
-void launch_ph_terms(hipStream_t s, const DevLayout &L, const double *Z, double *Tbuf, double hL, double hR) {
-  if (L.nphterm <= 0) return;
-  hipLaunchKernelGGL(k_ph_terms, dim3(8 * L.xs_terms.len), dim3(256), 0, s, L, Z, Tbuf, -hL * hR);
+void launch_ph_terms(hipStream_t s, const DevLayout &L, const double *Z, double *Tbuf, double hL, double hR,
+                     const int *act, int nact) {
+  if (L.nphterm <= 0 || nact <= 0) return;
+  hipLaunchKernelGGL(k_ph_terms, dim3(8 * L.xs_terms.len, nact), dim3(256), 0, s, L, Z, Tbuf, -hL * hR, act);
 }
-void launch_ph_apply(hipStream_t s, const DevLayout &L, const double *Tbuf, double *RES) {
-  if (L.nphterm <= 0) return;
+void launch_ph_apply(hipStream_t s, const DevLayout &L, const double *Tbuf, double *RES, const int *act,
+                     int nact) {
+  if (L.nphterm <= 0 || nact <= 0) return;
   const size_t lds = sizeof(double) * L.nhalf * L.nhalf * L.nJ;
-  hipLaunchKernelGGL(k_ph_apply, dim3(L.npairs), dim3(64), lds, s, L, Tbuf, RES);
+  hipLaunchKernelGGL(k_ph_apply, dim3(L.npairs, nact), dim3(64), lds, s, L, Tbuf, RES, act);
 }
 
 // ------------------------------------------------------------------ symmetrise + mirror
@@ -711,7 +779,8 @@ void launch_ph_apply(hipStream_t s, const DevLayout &L, const double *Tbuf, doub
 // 388-395, 915-923).  32x32 tile pairs go through LDS so all reads and
 // writes are row-contiguous.
 __global__ __launch_bounds__(256) void k_sym_mirror(DevLayout L, const int4 *__restrict__ tiles, double *RES,
-                                                    int herm) {
+                                                    int herm, const int *act) {
+  RES += inst_id(act) * L.op_size;
   const int4 t = tiles[blockIdx.x];  // x = block, y = tile row, z = tile col (y <= z)
   const int q = t.x, dim = L.bdim[q];
   double *OV = RES + L.v_off + L.boff[q];
@@ -743,9 +812,9 @@ __global__ __launch_bounds__(256) void k_sym_mirror(DevLayout L, const int4 *__r
     if (r < dim && c < dim && (!diag || tx < k)) OV[(long long)r * dim + c] = h * (U[tx][k] + h * W[k][tx]);
   }
 }
-void launch_sym_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm) {
-  if (L.nmirror <= 0) return;
-  hipLaunchKernelGGL(k_sym_mirror, dim3(L.nmirror), dim3(256), 0, s, L, L.mirror_tiles, RES, herm);
+void launch_sym_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm, const int *act, int nact) {
+  if (L.nmirror <= 0 || nact <= 0) return;
+  hipLaunchKernelGGL(k_sym_mirror, dim3(L.nmirror, nact), dim3(256), 0, s, L, L.mirror_tiles, RES, herm, act);
 }
 
 // ------------------------------------------------------------------ one-body (111 + 121 + 221)
@@ -807,10 +876,17 @@ __global__ __launch_bounds__(256) void k_onebody(DevLayout L, const double *__re
                                                  const double *__restrict__ Rop,
                                                  const double *__restrict__ Wpp,
                                                  double *__restrict__ RES, int hL, int hR, int hres,
-                                                 unsigned mask) {
+                                                 unsigned mask, const int *act) {
   __shared__ double red[4];
   const int t = blockIdx.x;
   if (t >= L.nob) return;
+  {
+    const long long inst = inst_id(act);
+    Lop += inst * L.op_size;
+    Rop += inst * L.op_size;
+    RES += inst * L.op_size;
+    Wpp += inst * L.vsize;
+  }
   const int tid = threadIdx.x;
   const int p = L.ob_p[t], qo = L.ob_q[t], N = L.nsp, MJ = L.nJ;
   const double *FL = Lop + L.f_off, *FR = Rop + L.f_off;
@@ -857,9 +933,11 @@ __global__ __launch_bounds__(256) void k_onebody(DevLayout L, const double *__re
   }
 }
 void launch_onebody(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop,
-                    const double *Wpp, double *RES, int hL, int hR, int hres, unsigned mask) {
-  if (L.nob <= 0) return;
-  hipLaunchKernelGGL(k_onebody, dim3(L.nob), dim3(256), 0, s, L, Lop, Rop, Wpp, RES, hL, hR, hres, mask);
+                    const double *Wpp, double *RES, int hL, int hR, int hres, unsigned mask, const int *act,
+                    int nact) {
+  if (L.nob <= 0 || nact <= 0) return;
+  hipLaunchKernelGGL(k_onebody, dim3(L.nob, nact), dim3(256), 0, s, L, Lop, Rop, Wpp, RES, hL, hR, hres, mask,
+                     act);
 }
 
 // ------------------------------------------------------------------ reductions
@@ -878,8 +956,14 @@ __device__ double block_sum256(double v, double *sh) {
 
 // 220 per block (commutators.f90:399-432) into partials[q]; 110 into partials[nblocks]
 __global__ __launch_bounds__(256) void k_e0_partial(DevLayout L, const double *Lop, const double *Rop,
-                                                     int hL, int hR, double *partials) {
+                                                     int hL, int hR, double *partials, const int *act) {
   __shared__ double sh[256];
+  {
+    const long long inst = inst_id(act);
+    Lop += inst * L.op_size;
+    Rop += inst * L.op_size;
+    partials += inst * kPartials;
+  }
   const int b = blockIdx.x;
   if (b < L.nblocks) {
     const int np = L.bnp[b], nb = L.bnb[b], nh = L.bnh[b], dim = L.bdim[b];
@@ -907,8 +991,13 @@ __global__ __launch_bounds__(256) void k_e0_partial(DevLayout L, const double *L
   }
 }
 __global__ __launch_bounds__(256) void k_e0_final(int nblocks, const double *partials, double *RES,
-                                                   int use110, int use220) {
+                                                   int use110, int use220, const int *act, long long rstride) {
   __shared__ double sh[256];
+  {
+    const long long inst = inst_id(act);
+    partials += inst * kPartials;
+    RES += inst * rstride;
+  }
   double s = 0.0;
   for (int q = threadIdx.x; q < nblocks; q += 256) s += partials[q];
   s = block_sum256(s, sh);
@@ -920,38 +1009,61 @@ __global__ __launch_bounds__(256) void k_e0_final(int nblocks, const double *par
   }
 }
 void launch_e0(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES,
-               int hL, int hR, unsigned mask, double *partials) {
-  hipLaunchKernelGGL(k_e0_partial, dim3(L.nblocks + 1), dim3(256), 0, s, L, Lop, Rop, hL, hR, partials);
+               int hL, int hR, unsigned mask, double *partials, const int *act, int nact, long long rstride) {
+  if (nact <= 0) return;
+  hipLaunchKernelGGL(k_e0_partial, dim3(L.nblocks + 1, nact), dim3(256), 0, s, L, Lop, Rop, hL, hR, partials,
+                     act);
   const int use220 = (mask & T220) && (hL * hR == -1);
-  hipLaunchKernelGGL(k_e0_final, dim3(1), dim3(256), 0, s, L.nblocks, partials, RES,
-                     (mask & T110) ? 1 : 0, use220 ? 1 : 0);
+  hipLaunchKernelGGL(k_e0_final, dim3(1, nact), dim3(256), 0, s, L.nblocks, partials, RES,
+                     (mask & T110) ? 1 : 0, use220 ? 1 : 0, act, rstride);
 }
 
 // sum of squares of F and V = mat_frob_norm^2 (basic_IMSRG.f90:3921-3952)
-__global__ __launch_bounds__(256) void k_sumsq_partial(const double *x, long long n, double *partials) {
+__global__ __launch_bounds__(256) void k_sumsq_partial(const double *x, long long n, double *partials,
+                                                        const int *act, long long stride) {
   __shared__ double sh[256];
+  {
+    const long long inst = inst_id(act);
+    x += inst * stride;
+    partials += inst * kPartials;
+  }
   double s = 0.0;
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
     s += x[i] * x[i];
   s = block_sum256(s, sh);
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
-__global__ __launch_bounds__(256) void k_final_sum(const double *partials, int n, double *out, int do_sqrt) {
+__global__ __launch_bounds__(256) void k_final_sum(const double *partials, int n, double *out, int do_sqrt,
+                                                    const int *act) {
   __shared__ double sh[256];
+  {
+    const long long inst = inst_id(act);
+    partials += inst * kPartials;
+    out += inst * kScal;
+  }
   double s = 0.0;
   for (int i = threadIdx.x; i < n; i += 256) s += partials[i];
   s = block_sum256(s, sh);
   if (threadIdx.x == 0) out[0] = do_sqrt ? sqrt(s) : s;
 }
-void launch_sumsq(hipStream_t s, const DevLayout &L, const double *op, double *partials, double *out) {
+void launch_sumsq(hipStream_t s, const DevLayout &L, const double *op, double *partials, double *out,
+                  const int *act, int nact) {
+  if (nact <= 0) return;
   const long long n = L.op_size - L.f_off;
-  hipLaunchKernelGGL(k_sumsq_partial, dim3(kPartials), dim3(256), 0, s, op + L.f_off, n, partials);
-  hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, s, partials, kPartials, out, 1);
+  hipLaunchKernelGGL(k_sumsq_partial, dim3(kPartials, nact), dim3(256), 0, s, op + L.f_off, n, partials, act,
+                     L.op_size);
+  hipLaunchKernelGGL(k_final_sum, dim3(1, nact), dim3(256), 0, s, partials, kPartials, out, 1, act);
 }
 
 // mbpt2 (basic_IMSRG.f90:4030-4064): one workgroup per block
-__global__ __launch_bounds__(256) void k_mbpt2_partial(DevLayout L, const double *H, double *partials) {
+__global__ __launch_bounds__(256) void k_mbpt2_partial(DevLayout L, const double *H, double *partials,
+                                                        const int *act) {
   __shared__ double sh[256];
+  {
+    const long long inst = inst_id(act);
+    H += inst * L.op_size;
+    partials += inst * kPartials;
+  }
   const int q = blockIdx.x;
   const int np = L.bnp[q], nb = L.bnb[q], nh = L.bnh[q], dim = L.bdim[q], base = L.pbase[q], N = L.nsp;
   const double *V = H + L.v_off + L.boff[q];
@@ -968,14 +1080,21 @@ __global__ __launch_bounds__(256) void k_mbpt2_partial(DevLayout L, const double
   s = block_sum256(s, sh);
   if (threadIdx.x == 0) partials[q] = (L.bJ[q] + 1.0) * s;
 }
-void launch_mbpt2(hipStream_t s, const DevLayout &L, const double *op, double *partials, double *out) {
-  hipLaunchKernelGGL(k_mbpt2_partial, dim3(L.nblocks), dim3(256), 0, s, L, op, partials);
-  hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, s, partials, L.nblocks, out, 0);
+void launch_mbpt2(hipStream_t s, const DevLayout &L, const double *op, double *partials, double *out,
+                  const int *act, int nact) {
+  if (nact <= 0) return;
+  hipLaunchKernelGGL(k_mbpt2_partial, dim3(L.nblocks, nact), dim3(256), 0, s, L, op, partials, act);
+  hipLaunchKernelGGL(k_final_sum, dim3(1, nact), dim3(256), 0, s, partials, L.nblocks, out, 0, act);
 }
 
 // ------------------------------------------------------------------ generators
 // twobody_monopole (basic_IMSRG.f90:2031-2046) for every orbit pair
-__global__ void k_monopole(DevLayout L, const double *H, double *mono) {
+__global__ void k_monopole(DevLayout L, const double *H, double *mono, const int *act) {
+  {
+    const long long inst = inst_id(act);
+    H += inst * L.op_size;
+    mono += inst * L.nsp * L.nsp;
+  }
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int N = L.nsp;
   if (t >= N * N) return;
@@ -985,14 +1104,23 @@ __global__ void k_monopole(DevLayout L, const double *H, double *mono) {
   for (int JT = abs(ja - jb); JT <= ja + jb; JT += 2) sm = sm + (JT + 1) * dv(L, V, a, b, a, b, JT);
   mono[t] = sm / (ja + 1.0) / (jb + 1.0);
 }
-void launch_monopole(hipStream_t s, const DevLayout &L, const double *H, double *mono) {
+void launch_monopole(hipStream_t s, const DevLayout &L, const double *H, double *mono, const int *act,
+                     int nact) {
+  if (nact <= 0) return;
   const int n = L.nsp * L.nsp;
-  hipLaunchKernelGGL(k_monopole, dim3((n + 255) / 256), dim3(256), 0, s, L, H, mono);
+  hipLaunchKernelGGL(k_monopole, dim3((n + 255) / 256, nact), dim3(256), 0, s, L, H, mono, act);
 }
 
 // build_gs_white / build_gs_atan / build_gs_atanxx (generators.f90:4-374):
 // ETA must be zero on entry; kind 0 = H/Eden, 1 = atan(2H/Eden)/2
-__global__ void k_eta_onebody(DevLayout L, const double *Hden, const double *Hnum, double *ETA, int kind) {
+__global__ void k_eta_onebody(DevLayout L, const double *Hden, const double *Hnum, double *ETA, int kind,
+                              const int *act) {
+  {
+    const long long o = inst_id(act) * L.op_size;
+    Hden += o;
+    Hnum += o;
+    ETA += o;
+  }
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= L.nphp) return;
   const int a = L.php_a[e], i = L.php_i[e], N = L.nsp, ji = L.jj[i];
@@ -1007,7 +1135,14 @@ __global__ void k_eta_onebody(DevLayout L, const double *Hden, const double *Hnu
   ETA[L.f_off + i * N + a] = -eta;
 }
 __global__ __launch_bounds__(256) void k_eta_twobody(DevLayout L, const double *Hden, const double *Hnum,
-                                                      const double *mono, double *ETA, int kind) {
+                                                      const double *mono, double *ETA, int kind, const int *act) {
+  {
+    const long long inst = inst_id(act);
+    Hden += inst * L.op_size;
+    Hnum += inst * L.op_size;
+    ETA += inst * L.op_size;
+    mono += inst * L.nsp * L.nsp;
+  }
   const int q = blockIdx.x;
   const int np = L.bnp[q], nb = L.bnb[q], nh = L.bnh[q], dim = L.bdim[q], base = L.pbase[q], N = L.nsp;
   const double *Fd = Hden + L.f_off;
@@ -1033,16 +1168,24 @@ __global__ __launch_bounds__(256) void k_eta_twobody(DevLayout L, const double *
   }
 }
 void launch_eta(hipStream_t s, const DevLayout &L, const double *Hden, const double *Hnum,
-                const double *mono, double *ETA, int kind) {
+                const double *mono, double *ETA, int kind, const int *act, int nact) {
+  if (nact <= 0) return;
   if (L.nphp > 0)
-    hipLaunchKernelGGL(k_eta_onebody, dim3((L.nphp + 63) / 64), dim3(64), 0, s, L, Hden, Hnum, ETA, kind);
-  hipLaunchKernelGGL(k_eta_twobody, dim3(L.nblocks), dim3(256), 0, s, L, Hden, Hnum, mono, ETA, kind);
+    hipLaunchKernelGGL(k_eta_onebody, dim3((L.nphp + 63) / 64, nact), dim3(64), 0, s, L, Hden, Hnum, ETA, kind,
+                       act);
+  hipLaunchKernelGGL(k_eta_twobody, dim3(L.nblocks, nact), dim3(256), 0, s, L, Hden, Hnum, mono, ETA, kind,
+                     act);
 }
 
 // ------------------------------------------------------------------ normal ordering
 // normal_order (basic_IMSRG.f90:2375-2506); T is an unmodified copy of H
-__global__ __launch_bounds__(256) void k_no_e0(DevLayout L, const double *T, double *H) {
+__global__ __launch_bounds__(256) void k_no_e0(DevLayout L, const double *T, double *H, const int *act) {
   __shared__ double sh[256];
+  {
+    const long long o = inst_id(act) * L.op_size;
+    T += o;
+    H += o;
+  }
   const int N = L.nsp;
   const double *F = T + L.f_off, *V = T + L.v_off;
   double s1 = 0.0;
@@ -1058,7 +1201,12 @@ __global__ __launch_bounds__(256) void k_no_e0(DevLayout L, const double *T, dou
   s2 = block_sum256(s2, sh);
   if (threadIdx.x == 0) H[0] = (s1 * 2.0 + s2) * 0.5;
 }
-__global__ void k_no_f(DevLayout L, const double *T, double *H, int herm) {
+__global__ void k_no_f(DevLayout L, const double *T, double *H, int herm, const int *act) {
+  {
+    const long long o = inst_id(act) * L.op_size;
+    T += o;
+    H += o;
+  }
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= L.nob) return;
   const int a = L.ob_p[t], b = L.ob_q[t], N = L.nsp;
@@ -1074,11 +1222,22 @@ __global__ void k_no_f(DevLayout L, const double *T, double *H, int herm) {
   H[L.f_off + a * N + b] = val;
   if (a != b) H[L.f_off + b * N + a] = herm * val;
 }
-void launch_normal_order(hipStream_t s, const DevLayout &L, const double *T, double *H, double *partials) {
-  (void)partials;
-  hipLaunchKernelGGL(k_no_e0, dim3(1), dim3(256), 0, s, L, T, H);
-  if (L.nob > 0) hipLaunchKernelGGL(k_no_f, dim3((L.nob + 63) / 64), dim3(64), 0, s, L, T, H, 1);
+void launch_normal_order(hipStream_t s, const DevLayout &L, const double *T, double *H, const int *act,
+                         int nact) {
+  if (nact <= 0) return;
+  hipLaunchKernelGGL(k_no_e0, dim3(1, nact), dim3(256), 0, s, L, T, H, act);
+  if (L.nob > 0) hipLaunchKernelGGL(k_no_f, dim3((L.nob + 63) / 64, nact), dim3(64), 0, s, L, T, H, 1, act);
 }
 
-// split-FP32 GEMM lives in gemm_split.hip
+// E0 of every active instance of an operator into out[inst * kScal]
+__global__ void k_gather_e0(const double *op, long long stride, double *out, const int *act, int nact) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nact) return;
+  const long long inst = act ? act[k] : 0;
+  out[inst * kScal] = op[inst * stride];
+}
+void launch_gather_e0(hipStream_t s, const double *op, long long stride, double *out, const int *act, int nact) {
+  if (nact <= 0) return;
+  hipLaunchKernelGGL(k_gather_e0, dim3((nact + 63) / 64), dim3(64), 0, s, op, stride, out, act, nact);
+}
 }  // namespace imsrg

@@ -42,6 +42,8 @@ _vp, _ip, _dp, _i64 = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c
 _SIG = {
     "imsrg_last_error": (C.c_char_p, []),
     "imsrg_ctx_create": (C.c_int, [C.c_int, _ip, _ip, _ip, _ip, _ip, C.c_int, C.POINTER(_vp)]),
+    "imsrg_ctx_create_batch": (C.c_int, [C.c_int, _ip, _ip, _ip, _ip, _ip, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "imsrg_ctx_nbatch": (C.c_int, [_vp]),
     "imsrg_ctx_destroy": (C.c_int, [_vp]),
     "imsrg_ctx_neq": (_i64, [_vp]),
     "imsrg_ctx_info": (C.c_int, [_vp, _dp]),
@@ -62,6 +64,9 @@ _SIG = {
     "imsrg_op_set_e0": (C.c_int, [_vp, C.c_double]),
     "imsrg_op_norm": (C.c_int, [_vp, _dp]),
     "imsrg_mbpt2": (C.c_int, [_vp, _dp]),
+    "imsrg_op_e0_batch": (C.c_int, [_vp, _dp]),
+    "imsrg_op_norm_batch": (C.c_int, [_vp, _dp]),
+    "imsrg_mbpt2_batch": (C.c_int, [_vp, _dp]),
     "imsrg_normal_order": (C.c_int, [_vp]),
     "imsrg_commutator": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint]),
     "imsrg_commutator_110": (C.c_int, [_vp, _vp, _dp]),
@@ -109,15 +114,19 @@ def _iptr(a):
 
 
 class Context:
-    """A device context bound to one basis (imsrg_ctx_create)."""
+    """A device context bound to one basis (imsrg_ctx_create / _batch).
 
-    def __init__(self, basis: Basis, device: int = 0, precision: int = PREC_FP64):
+    With nbatch > 1 every operator holds nbatch independent instances; vectors
+    are then arrays of shape (nbatch, neq) and scalars arrays of nbatch."""
+
+    def __init__(self, basis: Basis, device: int = 0, precision: int = PREC_FP64, nbatch: int = 1):
         self.basis = basis
         t = np.ascontiguousarray(basis.orbit_table())
         cols = [np.ascontiguousarray(t[:, k]) for k in range(5)]
         h = _vp()
-        _chk(_lib.imsrg_ctx_create(len(t), *[_iptr(c) for c in cols], device, C.byref(h)))
+        _chk(_lib.imsrg_ctx_create_batch(len(t), *[_iptr(c) for c in cols], device, int(nbatch), C.byref(h)))
         self.h = h
+        self.nbatch = int(_lib.imsrg_ctx_nbatch(h))
         self.neq = int(_lib.imsrg_ctx_neq(h))
         if self.neq != basis.neq:
             raise ImsrgError(f"neq mismatch {self.neq} vs {basis.neq}")
@@ -175,16 +184,20 @@ class Context:
         _chk(_lib.imsrg_generator_atanxx(self.h, Hold.h, H.h, ETA.h))
         return ETA
 
+    def _per_inst(self, a):
+        return int(a[0]) if self.nbatch == 1 else a.copy()
+
     def bch(self, G, H, HS=None):
+        """HS = e^G H e^-G; also the nested-commutator count (array per instance if batched)"""
         HS = HS or Operator(self, 1)
-        n = C.c_int(0)
-        _chk(_lib.imsrg_bch_expand(self.h, HS.h, G.h, H.h, C.byref(n)))
-        return HS, n.value
+        n = np.zeros(self.nbatch, np.int32)
+        _chk(_lib.imsrg_bch_expand(self.h, HS.h, G.h, H.h, _iptr(n)))
+        return HS, self._per_inst(n)
 
     def magnus_expand(self, DG, G):
-        n = C.c_int(0)
-        _chk(_lib.imsrg_magnus_expand(self.h, DG.h, G.h, C.byref(n)))
-        return DG, n.value
+        n = np.zeros(self.nbatch, np.int32)
+        _chk(_lib.imsrg_magnus_expand(self.h, DG.h, G.h, _iptr(n)))
+        return DG, self._per_inst(n)
 
     def magnus_flow(self, HS, kind="white", ds=0.5, max_steps=200, G=None):
         G = G or Operator(self, -1)
@@ -246,14 +259,21 @@ class Operator:
         _chk(_lib.imsrg_op_set_herm(self.h, int(h)))
 
     def upload(self, vec):
+        """vectorize-order vector, or (nbatch, neq) array for a batched context"""
         v = np.ascontiguousarray(vec, dtype=np.float64)
         _chk(_lib.imsrg_op_upload(self.h, _dptr(v), v.size))
         return self
 
     def download(self):
-        v = np.zeros(self.ctx.neq)
+        B = self.ctx.nbatch
+        v = np.zeros(self.ctx.neq * B)
         _chk(_lib.imsrg_op_download(self.h, _dptr(v), v.size))
-        return v
+        return v if B == 1 else v.reshape(B, self.ctx.neq)
+
+    def _scalars(self, fn):
+        v = np.zeros(self.ctx.nbatch)
+        _chk(fn(self.h, _dptr(v)))
+        return float(v[0]) if self.ctx.nbatch == 1 else v
 
     def copy_from(self, other):
         _chk(_lib.imsrg_op_copy(other.h, self.h))
@@ -272,23 +292,17 @@ class Operator:
 
     @property
     def e0(self):
-        v = C.c_double(0)
-        _chk(_lib.imsrg_op_get_e0(self.h, C.byref(v)))
-        return v.value
+        return self._scalars(_lib.imsrg_op_e0_batch)
 
     @e0.setter
     def e0(self, x):
         _chk(_lib.imsrg_op_set_e0(self.h, float(x)))
 
     def norm(self):
-        v = C.c_double(0)
-        _chk(_lib.imsrg_op_norm(self.h, C.byref(v)))
-        return v.value
+        return self._scalars(_lib.imsrg_op_norm_batch)
 
     def mbpt2(self):
-        v = C.c_double(0)
-        _chk(_lib.imsrg_mbpt2(self.h, C.byref(v)))
-        return v.value
+        return self._scalars(_lib.imsrg_mbpt2_batch)
 
     def normal_order(self):
         _chk(_lib.imsrg_normal_order(self.h))
@@ -319,10 +333,13 @@ class Flow:
         self.h = h
 
     def step(self):
-        r = np.zeros(8)
+        """one Magnus step; a record dict (a list of them, one per instance, if batched)"""
+        B = self.ctx.nbatch
+        r = np.zeros(8 * B)
         _chk(_lib.imsrg_flow_step(self.h, _dptr(r)))
-        return dict(s=r[0], e0=r[1], e0_mbpt2=r[2], crit=r[3], n_magnus=int(r[4]), n_bch=int(r[5]),
-                    step=int(r[6]))
+        recs = [dict(s=x[0], e0=x[1], e0_mbpt2=x[2], crit=x[3], n_magnus=int(x[4]), n_bch=int(x[5]),
+                     step=int(x[6]), stepped=bool(x[7])) for x in r.reshape(B, 8)]
+        return recs[0] if B == 1 else recs
 
     @property
     def converged(self):

@@ -49,6 +49,23 @@ def synthetic_hamiltonian(basis: Basis, seed: int = 0, hw: float = 20.0,
     return basis.vec_from_full(0.0, f, mats)
 
 
+def batch_parameters(seed: int):
+    """(hw, sigma2) of batch member `seed`: hw uniform in [16, 24] MeV and the
+    two-body width uniform in [0.5, 1.5] MeV, drawn from PCG64(1000 + seed)."""
+    rng = np.random.Generator(np.random.PCG64(1000 + seed))
+    return float(rng.uniform(16.0, 24.0)), float(rng.uniform(0.5, 1.5))
+
+
+def batch_hamiltonians(basis: Basis, seeds):
+    """Bare Hamiltonians of a batch of independent instances, shape
+    (len(seeds), neq): member k has seed seeds[k] and batch_parameters(seed)."""
+    out = []
+    for sd in seeds:
+        hw, s2 = batch_parameters(sd)
+        out.append(synthetic_hamiltonian(basis, seed=sd, hw=hw, sigma1=0.5, sigma2=s2))
+    return np.stack(out)
+
+
 def random_operator(basis: Basis, herm: int, seed: int):
     """Random operator with hermiticity ``herm`` (+1/-1) as a vectorize stream."""
     rng = np.random.Generator(np.random.PCG64(seed))

@@ -66,6 +66,15 @@ const char *imsrg_last_error(void);
  * the device.  device < 0 keeps the current device. */
 int imsrg_ctx_create(int nsp, const int *n, const int *l, const int *jj, const int *tz,
                      const int *con, int device, imsrg_ctx *out);
+/* Batched context (no counterpart in the reference, which flows one
+ * Hamiltonian per process): every operator of the context holds nbatch
+ * independent instances on the same basis, and every call acts on all of
+ * them with one launch per kernel (the instance is the second grid
+ * dimension).  nbatch = 1 is imsrg_ctx_create.  Scalar getters of a batched
+ * context are the _batch variants below. */
+int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, const int *tz,
+                           const int *con, int device, int nbatch, imsrg_ctx *out);
+int imsrg_ctx_nbatch(imsrg_ctx ctx);
 int imsrg_ctx_destroy(imsrg_ctx ctx);
 /* length of the vectorize stream (IMSRG_ODE.f90:36-44) */
 int64_t imsrg_ctx_neq(imsrg_ctx ctx);
@@ -84,17 +93,22 @@ int imsrg_op_create(imsrg_ctx ctx, int herm, imsrg_op *out);     /* duplicate_sq
 int imsrg_op_destroy(imsrg_op op);
 int imsrg_op_set_herm(imsrg_op op, int herm);
 int imsrg_op_get_herm(imsrg_op op, int *herm);
-int imsrg_op_upload(imsrg_op op, const double *vec, int64_t neq);   /* repackage :3857-3918 */
-int imsrg_op_download(imsrg_op op, double *vec, int64_t neq);       /* vectorize :3794-3854 */
+/* vec: nbatch vectorize-order vectors back to back, n = neq * nbatch */
+int imsrg_op_upload(imsrg_op op, const double *vec, int64_t n);     /* repackage :3857-3918 */
+int imsrg_op_download(imsrg_op op, double *vec, int64_t n);         /* vectorize :3794-3854 */
 int imsrg_op_copy(imsrg_op src, imsrg_op dst);                      /* copy_sq_op :2942-2969 */
 int imsrg_op_clear(imsrg_op op);                                    /* clear_sq_op :3537-3562 */
 int imsrg_op_scale(imsrg_op op, double a);                          /* scale_sq_op :3510-3534 */
 /* C = ax*A + bx*B (C may alias A or B)                                add_sq_op :3450-3478 */
 int imsrg_op_add(imsrg_op A, double ax, imsrg_op B, double bx, imsrg_op C);
 int imsrg_op_get_e0(imsrg_op op, double *e0);
-int imsrg_op_set_e0(imsrg_op op, double e0);
+int imsrg_op_set_e0(imsrg_op op, double e0);                        /* every instance */
 int imsrg_op_norm(imsrg_op op, double *out);                        /* mat_frob_norm :3921-3952 */
 int imsrg_mbpt2(imsrg_op H, double *out);                           /* mbpt2 :4030-4064 */
+/* the same scalars for every instance of a batched context (out[nbatch]) */
+int imsrg_op_e0_batch(imsrg_op op, double *e0);
+int imsrg_op_norm_batch(imsrg_op op, double *out);
+int imsrg_mbpt2_batch(imsrg_op H, double *out);
 int imsrg_normal_order(imsrg_op H);                                 /* normal_order :2375-2506 */
 
 /* ---- commutator engine --------------------------------------------------
@@ -119,11 +133,12 @@ int imsrg_generator_atanxx(imsrg_ctx ctx, imsrg_op Hold, imsrg_op H, imsrg_op ET
 
 /* ---- Magnus driver (IMSRG_MAGNUS.f90) -------------------------------------- */
 /* HS = e^G H e^-G by the nested-commutator series (BCH_EXPAND :149-234);
- * *nterms = commutators evaluated.  Returns IMSRG_ECLIP after 39 terms. */
+ * nterms[k] = commutators evaluated for instance k (nbatch entries, each
+ * instance truncates on its own).  Returns IMSRG_ECLIP after 39 terms. */
 int imsrg_bch_expand(imsrg_ctx ctx, imsrg_op HS, imsrg_op G, imsrg_op H, int *nterms);
-/* DG <- dOmega/ds series (MAGNUS_EXPAND :622-693) */
+/* DG <- dOmega/ds series (MAGNUS_EXPAND :622-693); nterms as above */
 int imsrg_magnus_expand(imsrg_ctx ctx, imsrg_op DG, imsrg_op G, int *nterms);
-/* full flow (magnus_decouple :15-128, no checkpointing): HS in = normal-ordered
+/* full flow (magnus_decouple :15-128, no checkpointing; nbatch = 1): HS in = normal-ordered
  * H(0), out = decoupled H(s); G out = Omega(s).  Per-step records (arrays of
  * length max_steps, any may be NULL): s, E0, E0+MBPT2, |MBPT2|, #Magnus and
  * #BCH commutators.  *steps = steps taken. */
@@ -133,8 +148,11 @@ int imsrg_magnus_flow(imsrg_ctx ctx, imsrg_op HS, imsrg_op G, int gen_kind, doub
 
 /* The same flow one step at a time: set-up of magnus_decouple (:31-96; Omega
  * reset to 0, DG = generator(HS), H(0) kept, crit = |MBPT2(HS)|), then one
- * pass of its step loop (:98-126) per imsrg_flow_step.  rec8 receives
- * s, E0, E0+MBPT2, |MBPT2|, #Magnus, #BCH commutators, step index, 0.
+ * pass of its step loop (:98-126) per imsrg_flow_step.  rec8 receives, per
+ * instance (8 * nbatch doubles), s, E0, E0+MBPT2, |MBPT2|, #Magnus, #BCH
+ * commutators, steps taken, 1 if the instance stepped in this call.  In a
+ * batched context an instance whose |MBPT2| reached float32(1e-6) stops
+ * stepping while the others go on; imsrg_flow_converged reports all done.
  * HS and G stay owned by the caller and must outlive the flow. */
 typedef struct imsrg_flow_s *imsrg_flow;
 int imsrg_flow_create(imsrg_ctx ctx, imsrg_op HS, imsrg_op G, int gen_kind, double ds, imsrg_flow *out);
