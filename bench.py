@@ -50,6 +50,9 @@ def parse():
     p.add_argument("--precision", default="all", choices=["all", "fp64", "split32", "bf16"])
     p.add_argument("--cpu-baseline", default="auto", choices=["auto", "reference", "port", "none"])
     p.add_argument("--cpu-reps", type=int, default=3, help="commutators in the CPU sample")
+    p.add_argument("--batch", type=int, default=16, help="instances of the batched measurement (0: skip)")
+    p.add_argument("--batch-emax", type=int, default=6)
+    p.add_argument("--batch-holes", type=int, default=3)
     p.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                    help="print the calibrated HBM traffic table from two rocprofv3 --pmc runs and exit")
     return p.parse_args()
@@ -222,13 +225,70 @@ def verify(ctx, E, H0vec, a, prec):
     return out
 
 
+def batch_bench(a, E, Basis, batch_hamiltonians):
+    """Config-3 style measurement: `a.batch` independent Hamiltonians (seeds
+    0.., hw in [16,24] MeV, sigma2 in [0.5,1.5] MeV) on one batched context,
+    every kernel launched once for all active instances, against the same
+    instances flowed one after another on a single-instance context.  Both
+    run W untimed + K timed Magnus steps per instance."""
+    basis = Basis.ho(a.batch_emax, a.batch_holes)
+    H = batch_hamiltonians(basis, range(a.batch))
+    out = {"workload": f"{a.batch} Hamiltonians emax={a.batch_emax}, {a.batch_holes} holes/isospin, "
+                       f"seeds 0-{a.batch - 1}, hw U[16,24], sigma2 U[0.5,1.5], {a.generator} generator, ds=0.5",
+           "orbits": basis.nsp, "neq": basis.neq}
+    bctx = E.Context(basis, nbatch=a.batch)
+    sctx = E.Context(basis)
+    for prec in ("fp64", "split32"):
+        bctx.set_precision(PRECS[prec])
+        sctx.set_precision(PRECS[prec])
+        bH = bctx.op(1, H).normal_order()
+        f = bctx.flow(bH, a.generator, 0.5)
+        for _ in range(a.warmup):
+            f.step()
+        bctx.sync()
+        t0 = time.perf_counter()
+        nsteps = 0
+        for _ in range(a.steps):
+            nsteps += sum(r["stepped"] for r in f.step())
+        bctx.sync()
+        tb = time.perf_counter() - t0
+        e0b = np.array(bH.e0)
+        f.close()
+        ts, nseq, e0s = 0.0, 0, []
+        for k in range(a.batch):
+            sH = sctx.op(1, H[k]).normal_order()
+            sf = sctx.flow(sH, a.generator, 0.5)
+            for _ in range(a.warmup):
+                sf.step()
+            sctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(a.steps):
+                if not sf.converged:
+                    sf.step()
+                    nseq += 1
+            sctx.sync()
+            ts += time.perf_counter() - t0
+            e0s.append(sH.e0)
+            sf.close()
+        same = bool(np.array_equal(e0b, np.array(e0s)))
+        out[prec] = {"instance_steps_per_s": nsteps / tb, "sequential_instance_steps_per_s": nseq / ts,
+                     "speedup_vs_sequential": (nsteps / tb) / (nseq / ts), "ms_per_batched_step": 1e3 * tb / a.steps,
+                     "E0_identical_to_sequential": same}
+        log(f"batch {prec}: {nsteps / tb:.2f} instance-steps/s batched vs {nseq / ts:.2f} sequential "
+            f"(x{(nsteps / tb) / (nseq / ts):.2f}); E0 identical: {same}")
+    out["split32_speedup_vs_fp64"] = out["split32"]["instance_steps_per_s"] / out["fp64"]["instance_steps_per_s"]
+    bctx.close()
+    sctx.close()
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); multi-GPU sharding of "
               "this path is design-only", file=sys.stderr)
         sys.exit(2)
-    from imsrg_mi355x import Basis, synthetic_hamiltonian
+    from imsrg_mi355x import Basis, synthetic_hamiltonian, batch_hamiltonians
     from imsrg_mi355x import engine as E
 
     basis = Basis.ho(a.emax, a.holes)
@@ -292,6 +352,10 @@ def main():
         by_kernel[k]["traffic"] = traffic.get(k)
     kernels = {k: {"ms_total": round(v[0], 3), "launches": v[1]} for k, v in prof.items() if v[1]}
     log("kernel classes (ms, launches):", json.dumps(kernels))
+    batch = None
+    if a.batch > 0:
+        log(f"batched flows: {a.batch} instances at emax={a.batch_emax}")
+        batch = batch_bench(a, E, Basis, batch_hamiltonians)
     cpu = None
     if a.cpu_baseline != "none":
         log("timing the CPU baseline sample")
@@ -317,6 +381,7 @@ def main():
             "precision": head, "first_timed_s": recs[0]["s"], "E0_last": recs[-1]["e0"],
             "variants": variants,
             "speedup_vs_fp64": (steps_per_s / variants["fp64"]["steps_per_s"]) if "fp64" in variants else None,
+            "batch": batch,
         },
         "roofline": roof,
         "roofline_by_kernel": {k: {"achieved": round(v["achieved"], 2), "unit": v["unit"],

@@ -44,6 +44,12 @@ struct DevLayout {
   XcdSplit xs_rows, xs_terms;  // k_122 rows, k_ph_terms chunks of 256 terms
 };
 
+// A plain GEMM problem C = A * B (A M x K, B K x N, C M x N, all row-major)
+struct GemmProb {
+  int M, N, K, lda, ldb, ldc;
+  long long a_off, b_off, c_off;
+};
+
 // One K segment of a segmented GEMM: sign * A * B with A taken from operand
 // `src` and B from the other one (offsets in doubles from the operand base).
 struct SegDesc {
@@ -119,6 +125,28 @@ void launch_eta(hipStream_t s, const DevLayout &L, const double *Hden, const dou
 void launch_normal_order(hipStream_t s, const DevLayout &L, const double *T, double *H, const int *act,
                          int nact);
 void launch_gather_e0(hipStream_t s, const double *op, long long stride, double *out, const int *act, int nact);
+
+// ---- adjoint commutator (adjoint.hip): Ybar += d<C,[P,Y]>/dY term by term
+void launch_adj_e0(hipStream_t s, const DevLayout &L, const double *P, const double *C, double *Y, int hP, int hY,
+                   unsigned mask);
+void launch_adj_onebody(hipStream_t s, const DevLayout &L, const double *P, const double *C, double *Y, int hP,
+                        int hY, int hC, unsigned mask);
+void launch_adj_122(hipStream_t s, const DevLayout &L, const double *P, const double *C, double *Y, int hC);
+void launch_adj_pphh(hipStream_t s, const DevLayout &L, const double *Q, double *Y, int hP, int hC);
+void launch_adj_ph_apply(hipStream_t s, const DevLayout &L, const double *C, double *Tbar, int hC);
+void launch_adj_ph_terms(hipStream_t s, const DevLayout &L, const double *Tbar, double *Zbar, double sigma);
+void launch_adj_ph_sbt(hipStream_t s, const DevLayout &L, const int *ccrlen_host, double *Z, int hP, int hY);
+void launch_adj_pandya(hipStream_t s, const DevLayout &L, const double *ccb, double *Y);
+// X <- (X + h X^T)/2 on the one-body part and every channel block; E0 <- 0 unless keep_e0
+void launch_project(hipStream_t s, const DevLayout &L, double *X, int h, int keep_e0);
+void launch_gemm_plain(hipStream_t s, const GemmProb *probs, const int4 *tiles, int ntiles, const double *A,
+                       const double *B, double *C);
+// transpose of the White (kind 0) / atan (kind 1) generator at H (mono = its
+// monopoles): Hb += d<Eb, eta(H)>/dH; mbar: nsp*nsp scratch
+void launch_adj_generator(hipStream_t s, const DevLayout &L, const double *H, const double *mono,
+                          const double *Eb, double *Hb, double *mbar, int kind);
+// transpose of normal ordering: Hb (a copy of the cotangent Cb on entry) becomes d<Cb, NO(H)>/dH
+void launch_adj_normal_order(hipStream_t s, const DevLayout &L, const double *Cb, double *Hb, int herm);
 
 constexpr int kPartials = 1024;  // reduction partials per instance
 constexpr int kScal = 64;        // scalar slots per instance

@@ -44,8 +44,12 @@ struct imsrg_ctx_s {
   double *scal = nullptr;   // device scalars, kScal per instance
   double *hscal = nullptr;  // pinned host mirror of scal
   double *mono = nullptr;
+  double *mbar = nullptr;   // generator-transpose scratch (nsp * nsp)
   double *hstage = nullptr; // pinned op_size staging
   SegProb *pp_probs = nullptr, *ph_probs = nullptr;
+  GemmProb *adj_pp_probs = nullptr, *adj_ph_probs = nullptr;  // adjoint products
+  int4 *adj_pp_tiles = nullptr, *adj_ph_tiles = nullptr;
+  int n_adj_pp_tiles = 0, n_adj_ph_tiles = 0;
   int4 *pp_tiles = nullptr, *ph_tiles = nullptr;
   int n_pp_tiles = 0, n_ph_tiles = 0;
   int prec = IMSRG_PREC_FP64;
@@ -341,6 +345,7 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
   if (!rc) rc = dalloc(c, kPartials * B, &c->partials);
   if (!rc) rc = dalloc(c, kScal * B, &c->scal);
   if (!rc) rc = dalloc(c, (size_t)Ly.nsp * Ly.nsp * B, &c->mono);
+  if (!rc) rc = dalloc(c, (size_t)Ly.nsp * Ly.nsp, &c->mbar);
   for (int k = 0; k < imsrg_ctx_s::kSlots && !rc; k++) {
     void *p = nullptr;
     if (hipMalloc(&p, B * sizeof(int)) != hipSuccess) {
@@ -369,7 +374,7 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
     std::vector<SegProb> pp;
     for (int q = 0; q < Ly.nblocks; q++) {
       const int dim = Ly.bdim[q], np = Ly.bnp[q], nb = Ly.bnb[q], nh = Ly.bnh[q];
-      if (dim == 0 || np + nh == 0) continue;
+      if (dim == 0) continue;  // blocks without pp/hh pairs still write their zeros
       const long long bo = Ly.boff[q];
       const long long ah = bo + np + nb, bh = bo + (long long)(np + nb) * dim;
       SegProb P{};
@@ -394,7 +399,7 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
     std::vector<SegProb> ph;
     for (int q = 0; q < Ly.ncc; q++) {
       const int r = Ly.ccrlen[q], nbq = Ly.ccnph[q];
-      if (r == 0 || nbq == 0) continue;  // Z stays zero
+      if (r == 0) continue;  // blocks without ph columns still write Z = 0 on every call
       SegProb P{};
       P.M = r;
       P.c_off = Ly.cwoff[q];
@@ -409,6 +414,33 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
     c->n_ph_tiles = (int)tiles.size();
     rc = upload_table(c, ph, (const SegProb **)&c->ph_probs);
     if (!rc) rc = upload_table(c, tiles, (const int4 **)&c->ph_tiles);
+  }
+  // adjoint products: Q = P * Cbar per channel block; Ybar_CC^T = P_CC^T * Sbar^T per Pandya block
+  if (!rc) {
+    std::vector<GemmProb> pp, ph;
+    std::vector<int4> tp, th;
+    for (int q = 0; q < Ly.nblocks; q++) {
+      const int dim = Ly.bdim[q];
+      if (dim == 0) continue;
+      const int id = (int)pp.size();
+      pp.push_back(GemmProb{dim, dim, dim, dim, dim, dim, Ly.boff[q], Ly.boff[q], Ly.boff[q]});
+      for (int i = 0; i < (dim + 63) / 64; i++)
+        for (int j = 0; j < (dim + 63) / 64; j++) tp.push_back(make_int4(id, i, j, 0));
+    }
+    for (int q = 0; q < Ly.ncc; q++) {
+      const int r = Ly.ccrlen[q], nbq = Ly.ccnph[q];
+      if (r == 0 || nbq == 0) continue;
+      const int id = (int)ph.size();
+      ph.push_back(GemmProb{nbq, r, r, r, r, r, Ly.ccoff[q], Ly.cwoff[q], Ly.ccoff[q]});
+      for (int i = 0; i < (nbq + 63) / 64; i++)
+        for (int j = 0; j < (r + 63) / 64; j++) th.push_back(make_int4(id, i, j, 0));
+    }
+    c->n_adj_pp_tiles = (int)tp.size();
+    c->n_adj_ph_tiles = (int)th.size();
+    rc = upload_table(c, pp, (const GemmProb **)&c->adj_pp_probs);
+    if (!rc) rc = upload_table(c, tp, (const int4 **)&c->adj_pp_tiles);
+    if (!rc) rc = upload_table(c, ph, (const GemmProb **)&c->adj_ph_probs);
+    if (!rc) rc = upload_table(c, th, (const int4 **)&c->adj_ph_tiles);
   }
   if (!rc) launch_fill(c->stream, c->wpp, Ly.vsize * B, 0.0);
   if (!rc && Ly.cwsize > 0) launch_fill(c->stream, c->wph, Ly.cwsize * B, 0.0);
@@ -689,6 +721,55 @@ int imsrg_cross_coupled(imsrg_op op, double *host_out, int64_t n) {
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
+}
+
+// ---------------------------------------------------------------- adjoint commutator
+// Ybar = d <Cbar, [P, Y]> / dY for Y of hermiticity hY, restricted to the terms
+// in mask, projected onto hermiticity hY (E0 of Ybar is 0: the commutator does
+// not depend on E0 of its operands).  <,> is the Frobenius product over the
+// full device layout.  Single-instance contexts only.
+static int adjoint_impl(imsrg_ctx_s *c, imsrg_op_s *P, imsrg_op_s *Cin, imsrg_op_s *Ybar, int hY, unsigned mask) {
+  hipStream_t s = c->stream;
+  const DevLayout &D = c->dl;
+  const Layout &Ly = c->lay;
+  const int hP = P->herm, hC = -hP * hY;
+  if (Cin->herm != hC) return fail(IMSRG_EINVAL, "cotangent herm must be -herm(P)*hY");
+  const ActSet all = all_set(c);
+  imsrg_op_s *Ct = tmp_op(c, 5, hC);
+  if (!Ct) return IMSRG_ENOMEM;
+  op_copy(Cin, Ct, all);
+  launch_project(s, D, Ct->d, hC, 1);
+  launch_fill(s, Ybar->d, Ly.op_size, 0.0);
+  if (mask & (T110 | T220)) launch_adj_e0(s, D, P->d, Ct->d, Ybar->d, hP, hY, mask);
+  launch_adj_onebody(s, D, P->d, Ct->d, Ybar->d, hP, hY, hC, mask);
+  if (mask & T122) launch_adj_122(s, D, P->d, Ct->d, Ybar->d, hC);
+  if (mask & T222PPHH) {
+    launch_gemm_plain(s, c->adj_pp_probs, c->adj_pp_tiles, c->n_adj_pp_tiles, P->d + D.v_off, Ct->d + D.v_off,
+                      c->wpp);
+    launch_adj_pphh(s, D, c->wpp, Ybar->d, hP, hC);
+  }
+  if (mask & T222PH) {
+    launch_pandya(s, D, P->d, c->cc[0], all.d, all.n());
+    launch_fill(s, c->tph, (long long)Ly.ph_terms.size(), 0.0);
+    launch_adj_ph_apply(s, D, Ct->d, c->tph, hC);
+    launch_fill(s, c->wph, Ly.cwsize, 0.0);
+    launch_adj_ph_terms(s, D, c->tph, c->wph, (double)(-hP * hY));
+    launch_adj_ph_sbt(s, D, Ly.ccrlen.data(), c->wph, hP, hY);
+    launch_gemm_plain(s, c->adj_ph_probs, c->adj_ph_tiles, c->n_adj_ph_tiles, c->cc[0], c->wph, c->cc[1]);
+    launch_adj_pandya(s, D, c->cc[1], Ybar->d);
+  }
+  launch_project(s, D, Ybar->d, hY, 0);
+  Ybar->herm = hY;
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+int imsrg_commutator_adjoint(imsrg_ctx c, imsrg_op P, imsrg_op Cbar, imsrg_op Ybar, int hY, unsigned mask) {
+  if (!c || !P || !Cbar || !Ybar) return fail(IMSRG_EINVAL, "null argument");
+  if (P->ctx != c || Cbar->ctx != c || Ybar->ctx != c) return fail(IMSRG_EINVAL, "operands of another context");
+  if (Ybar == P || Ybar == Cbar) return fail(IMSRG_EINVAL, "Ybar must not alias P or Cbar");
+  if (hY != 1 && hY != -1) return fail(IMSRG_EINVAL, "hY must be +1 or -1");
+  if (int rc = single_only(c, "imsrg_commutator_adjoint")) return rc;
+  return adjoint_impl(c, P, Cbar, Ybar, hY, mask & IMSRG_TALL);
 }
 
 // ---------------------------------------------------------------- generators
@@ -987,6 +1068,219 @@ int imsrg_magnus_flow(imsrg_ctx c, imsrg_op HS, imsrg_op G, int gen_kind, double
   if (steps_out) *steps_out = steps;
   imsrg_flow_destroy(f);
   return rc;
+}
+
+
+// ---------------------------------------------------------------- gradient through the Magnus flow
+// L = E0 of H(s_N) after nsteps steps of magnus_decouple (IMSRG_MAGNUS.f90:98-126)
+// from the normal-ordered H0, differentiated by the discrete adjoint: every
+// nested commutator of every BCH and Magnus series is transposed with
+// adjoint_impl, the Euler step and the generator by hand; the series keep the
+// term counts the forward flow chose.  Omega(s) is stored at ncheck
+// checkpoints and each segment is recomputed from its checkpoint.
+namespace {
+struct OpSet {
+  imsrg_ctx_s *c;
+  std::vector<imsrg_op_s *> ops;
+  explicit OpSet(imsrg_ctx_s *cc) : c(cc) {}
+  ~OpSet() {
+    for (auto *o : ops) imsrg_op_destroy(o);
+  }
+  imsrg_op_s *get(int herm) {
+    imsrg_op_s *o = nullptr;
+    if (op_new(c, herm, &o) != 0) return nullptr;
+    ops.push_back(o);
+    return o;
+  }
+};
+}  // namespace
+
+static void axpby_all(imsrg_ctx_s *c, double a, imsrg_op_s *A, double b, imsrg_op_s *B, imsrg_op_s *C) {
+  launch_axpby(c->stream, c->lay.op_size, a, A->d, b, B->d, C->d, nullptr, 1, 0);
+}
+
+// A[0] = X0 and A[k] = [Om, A[k-1]], k = 1..K (the nested commutators of one series)
+static int nested_chain(imsrg_ctx_s *c, imsrg_op_s *Om, imsrg_op_s *X0, int K, unsigned mask,
+                        std::vector<imsrg_op_s *> &A) {
+  const ActSet all = all_set(c);
+  op_copy(X0, A[0], all);
+  launch_pandya(c->stream, c->dl, Om->d, c->cc[0], all.d, all.n());
+  for (int k = 1; k <= K; k++) {
+    A[k]->herm = -Om->herm * A[k - 1]->herm;
+    int rc = comm_impl(c, Om, A[k - 1], A[k], mask, true, all);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+// Backward through S = sum_k w[k] A[k], A[k] = [Om, A[k-1]]: given Sbar, adds
+// dL/dA[0] to X0bar and dL/dOm to Ombar.
+static int nested_backward(imsrg_ctx_s *c, imsrg_op_s *Om, const std::vector<imsrg_op_s *> &A, int K,
+                           const std::vector<double> &w, unsigned mask, imsrg_op_s *Sbar, imsrg_op_s *X0bar,
+                           imsrg_op_s *Ombar, OpSet &tmp) {
+  const int hA = A[0]->herm;
+  imsrg_op_s *Ab = tmp.get(hA), *T1 = tmp.get(hA), *T2 = tmp.get(Om->herm);
+  if (!Ab || !T1 || !T2) return IMSRG_ENOMEM;
+  axpby_all(c, w[K], Sbar, 0.0, Sbar, Ab);
+  Ab->herm = hA;
+  for (int k = K; k >= 1; k--) {
+    int rc = adjoint_impl(c, Om, Ab, T1, hA, mask);  // d/dA[k-1]
+    if (!rc) rc = adjoint_impl(c, A[k - 1], Ab, T2, Om->herm, mask);  // d/dOm (with a minus sign)
+    if (rc) return rc;
+    axpby_all(c, 1.0, Ombar, -1.0, T2, Ombar);
+    axpby_all(c, w[k - 1], Sbar, 1.0, T1, Ab);
+  }
+  axpby_all(c, 1.0, X0bar, 1.0, Ab, X0bar);
+  return 0;
+}
+
+// Hbar += d<Eb, generator(H)>/dH (White or atan)
+static int generator_backward(imsrg_ctx_s *c, int kind, imsrg_op_s *H, imsrg_op_s *Eb, imsrg_op_s *Hbar,
+                              OpSet &tmp) {
+  imsrg_op_s *G = tmp.get(1);
+  if (!G) return IMSRG_ENOMEM;
+  const ActSet all = all_set(c);
+  launch_fill(c->stream, G->d, c->lay.op_size, 0.0);
+  launch_monopole(c->stream, c->dl, H->d, c->mono, all.d, all.n());
+  launch_adj_generator(c->stream, c->dl, H->d, c->mono, Eb->d, G->d, c->mbar, kind == IMSRG_GEN_WHITE ? 0 : 1);
+  launch_project(c->stream, c->dl, G->d, 1, 0);
+  axpby_all(c, 1.0, Hbar, 1.0, G, Hbar);
+  return 0;
+}
+
+static int flow_grad_impl(imsrg_ctx_s *c, imsrg_op_s *H0, int kind, double ds, int nsteps, int ncheck,
+                          double *e0_out, imsrg_op_s *H0b) {
+  static const double cof[7] = {1.0, -0.5, .0833333333, 0.0, -0.00138888888, 0.0, 3.306878e-5};
+  const ActSet all = all_set(c);
+  const unsigned maskB = IMSRG_TALL, maskM = IMSRG_TALL & ~(T110 | T220);
+  OpSet keep(c);
+  const int K = std::max(1, (nsteps + ncheck - 1) / ncheck);
+  const int nseg = (nsteps + K - 1) / K;
+  imsrg_op_s *Om = keep.get(-1), *eta = keep.get(-1), *DG = keep.get(-1), *HS = keep.get(1);
+  if (!Om || !eta || !DG || !HS) return IMSRG_ENOMEM;
+  std::vector<imsrg_op_s *> ck(nseg, nullptr);
+  std::vector<int> nm(nsteps, 0), nb(nsteps, 0);
+  // forward with checkpoints (the same sequence as imsrg_flow_step)
+  int rc = gen_impl(c, kind, H0, eta, all);
+  for (int n = 0; n < nsteps && !rc; n++) {
+    if (n % K == 0) {
+      ck[n / K] = keep.get(-1);
+      if (!ck[n / K]) return IMSRG_ENOMEM;
+      op_copy(Om, ck[n / K], all);
+    }
+    std::vector<int> t;
+    op_copy(eta, DG, all);
+    rc = magexp_impl(c, DG, Om, all, t);
+    if (rc) break;
+    nm[n] = t[0];
+    axpby_all(c, 1.0, Om, ds, DG, Om);
+    rc = bch_impl(c, HS, Om, H0, all, t);
+    if (rc) break;
+    nb[n] = t[0];
+    rc = gen_impl(c, kind, HS, eta, all);
+  }
+  if (rc) return rc;
+  if (e0_out) {
+    rc = imsrg_op_e0_batch(HS, e0_out);
+    if (rc) return rc;
+  }
+  // backward
+  imsrg_op_s *Hb = keep.get(1), *Omb = keep.get(-1), *DGb = keep.get(-1), *etab = keep.get(-1);
+  if (!Hb || !Omb || !DGb || !etab) return IMSRG_ENOMEM;
+  std::vector<imsrg_op_s *> chain(41);
+  for (auto &a : chain)
+    if (!(a = keep.get(1))) return IMSRG_ENOMEM;
+  launch_fill(c->stream, H0b->d, c->lay.op_size, 0.0);
+  H0b->herm = 1;
+  launch_fill(c->stream, Hb->d, c->lay.op_size, 0.0);
+  launch_fill(c->stream, Hb->d, 1, 1.0);  // dL/dE0(H_N) = 1
+  launch_fill(c->stream, Omb->d, c->lay.op_size, 0.0);
+  std::vector<double> wB(41), wM(7);
+  for (int k = 0; k < 7; k++) wM[k] = cof[k];
+  wB[0] = 1.0;
+  for (int k = 1; k < 41; k++) wB[k] = wB[k - 1] / k;
+  for (int seg = nseg - 1; seg >= 0; seg--) {
+    const int n0 = seg * K, n1 = std::min(n0 + K, nsteps);
+    OpSet segops(c);
+    std::vector<imsrg_op_s *> Oms(n1 - n0 + 1), etas(n1 - n0), Hs(n1 - n0);
+    for (auto &o : Oms)
+      if (!(o = segops.get(-1))) return IMSRG_ENOMEM;
+    for (int i = 0; i < n1 - n0; i++) {
+      etas[i] = segops.get(-1);
+      Hs[i] = segops.get(1);
+      if (!etas[i] || !Hs[i]) return IMSRG_ENOMEM;
+    }
+    // recompute Omega, H and eta over the segment
+    op_copy(ck[seg], Oms[0], all);
+    std::vector<int> t;
+    if (n0 == 0) op_copy(H0, Hs[0], all);
+    else if ((rc = bch_impl(c, Hs[0], Oms[0], H0, all, t))) return rc;
+    for (int i = 0; i < n1 - n0; i++) {
+      if ((rc = gen_impl(c, kind, Hs[i], etas[i], all))) return rc;
+      op_copy(etas[i], DG, all);
+      if ((rc = magexp_impl(c, DG, Oms[i], all, t))) return rc;
+      axpby_all(c, 1.0, Oms[i], ds, DG, Oms[i + 1]);
+      if (i + 1 < n1 - n0 && (rc = bch_impl(c, Hs[i + 1], Oms[i + 1], H0, all, t))) return rc;
+    }
+    for (int n = n1; n > n0; n--) {
+      const int i = n - n0;  // Oms[i] = Omega_n
+      OpSet tmp(c);
+      // H_n = BCH(Omega_n, H0)
+      for (int k = 0; k <= nb[n - 1]; k++) chain[k]->herm = 1;
+      if ((rc = nested_chain(c, Oms[i], H0, nb[n - 1], maskB, chain))) return rc;
+      if ((rc = nested_backward(c, Oms[i], chain, nb[n - 1], wB, maskB, Hb, H0b, Omb, tmp))) return rc;
+      // Omega_n = Omega_{n-1} + ds dOmega_{n-1}
+      axpby_all(c, ds, Omb, 0.0, Omb, DGb);
+      // dOmega_{n-1} = MAGNUS_EXPAND(eta_{n-1}, Omega_{n-1})
+      launch_fill(c->stream, etab->d, c->lay.op_size, 0.0);
+      for (int k = 0; k <= nm[n - 1]; k++) chain[k]->herm = -1;
+      if ((rc = nested_chain(c, Oms[i - 1], etas[i - 1], nm[n - 1], maskM, chain))) return rc;
+      if ((rc = nested_backward(c, Oms[i - 1], chain, nm[n - 1], wM, maskM, DGb, etab, Omb, tmp))) return rc;
+      // eta_{n-1} = generator(H_{n-1}); H_{n-1}'s cotangent starts here
+      launch_fill(c->stream, Hb->d, c->lay.op_size, 0.0);
+      if ((rc = generator_backward(c, kind, Hs[i - 1], etab, (n - 1 == 0) ? H0b : Hb, tmp))) return rc;
+      if ((rc = (int)hipStreamSynchronize(c->stream) == hipSuccess ? 0 : fail(IMSRG_EDEVICE, "gradient step")))
+        return rc;
+    }
+  }
+  launch_project(c->stream, c->dl, H0b->d, 1, 1);  // dE0(s_N)/dE0(0) = 1 stays
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int imsrg_flow_gradient(imsrg_ctx c, imsrg_op H0, int gen_kind, double ds, int nsteps, int ncheck, double *e0,
+                        imsrg_op H0bar) {
+  if (!c || !H0 || !H0bar || H0 == H0bar) return fail(IMSRG_EINVAL, "bad operands");
+  if (int rc = single_only(c, "imsrg_flow_gradient")) return rc;
+  if (gen_kind != IMSRG_GEN_WHITE && gen_kind != IMSRG_GEN_ATAN)
+    return fail(IMSRG_EINVAL, "gradient: White or atan generator");
+  if (nsteps < 1 || ncheck < 1) return fail(IMSRG_EINVAL, "nsteps and ncheck must be >= 1");
+  if (H0->herm != 1) return fail(IMSRG_EINVAL, "H0 must be hermitian");
+  return flow_grad_impl(c, H0, gen_kind, ds, nsteps, ncheck, e0, H0bar);
+}
+
+int imsrg_normal_order_adjoint(imsrg_op Cbar, imsrg_op Hbar) {
+  if (!Cbar || !Hbar || Cbar == Hbar || Cbar->ctx != Hbar->ctx) return fail(IMSRG_EINVAL, "bad operands");
+  imsrg_ctx_s *c = Cbar->ctx;
+  if (int rc = single_only(c, "imsrg_normal_order_adjoint")) return rc;
+  op_copy(Cbar, Hbar, all_set(c));
+  launch_adj_normal_order(c->stream, c->dl, Cbar->d, Hbar->d, Cbar->herm);
+  launch_project(c->stream, c->dl, Hbar->d, Cbar->herm, 0);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// gradient with respect to the vectorize stream: transpose of repackage
+int imsrg_op_download_cotangent(imsrg_op op, double *vec, int64_t n) {
+  if (!op || !vec) return fail(IMSRG_EINVAL, "null argument");
+  imsrg_ctx_s *c = op->ctx;
+  if (n != c->lay.neq * c->nbatch) return fail(IMSRG_EINVAL, "length must be neq * nbatch");
+  HIPCHK(hipMemcpyAsync(c->hstage, op->d, sizeof(double) * c->lay.op_size * c->nbatch, hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < c->nbatch; k++)
+    c->lay.unpack_vec_transpose(c->hstage + (size_t)k * c->lay.op_size, op->herm, vec + (size_t)k * c->lay.neq);
+  return 0;
 }
 
 // ---------------------------------------------------------------- profiling

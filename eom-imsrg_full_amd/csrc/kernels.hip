@@ -11,65 +11,9 @@
 #include <cmath>
 
 #include "device.hpp"
+#include "devfun.hpp"
 
 namespace imsrg {
-
-typedef double double4_t __attribute__((ext_vector_type(4)));
-
-#define HIP_KERNEL_CHECK() ((void)0)
-
-// Instance of a batched launch: blockIdx.y indexes the active-instance table
-// (null table: one instance, number 0).  Instance k of an operator, Pandya
-// matrix or workspace starts k strides after the buffer base.
-__device__ __forceinline__ long long inst_id(const int *act) { return act ? act[blockIdx.y] : 0; }
-
-__device__ __forceinline__ int xcd_item(const XcdSplit &x) {
-  const int xcd = blockIdx.x & 7, it = x.start[xcd] + (int)(blockIdx.x >> 3);
-  return it < x.start[xcd + 1] ? it : -1;
-}
-
-__device__ __forceinline__ int sgnp(int e) { return (e & 1) ? -1 : 1; }  // (-1)^e
-__device__ __forceinline__ bool dtri(int a, int b, int J) { return J <= a + b && J >= abs(a - b); }
-__device__ __forceinline__ int dpos(const DevLayout &L, int J, int a, int b) {
-  return L.pidx[(size_t)(J >> 1) * L.nsp * L.nsp + (size_t)a * L.nsp + b];
-}
-__device__ __forceinline__ double dsixj(const DevLayout &L, int j1, int j2, int j3, int j4, int j5,
-                                        int j6) {
-  // table holds 0 for any failed triangle, matching sixj (basic_IMSRG.f90:2509-2555)
-  const int h = L.nhalf, nJ = L.nJ;
-  size_t idx = (((((size_t)(j1 >> 1) * h + (j2 >> 1)) * nJ + (j3 >> 1)) * h + (j4 >> 1)) * h +
-                (j5 >> 1)) * nJ + (j6 >> 1);
-  return L.sixj[idx];
-}
-
-// v_elem (basic_IMSRG.f90:1270-1385) on the full layout; V = operator + v_off
-__device__ double dv(const DevLayout &L, const double *V, int a, int b, int c, int d, int J) {
-  const int ja = L.jj[a], jb = L.jj[b], jc = L.jj[c], jd = L.jj[d];
-  if (!(dtri(ja, jb, J) && dtri(jc, jd, J))) return 0.0;
-  const int P = (L.l[a] + L.l[b]) & 1;
-  if (((L.l[c] + L.l[d]) & 1) != P) return 0.0;
-  const int T2 = L.tz[a] + L.tz[b];
-  if (L.tz[c] + L.tz[d] != T2) return 0.0;
-  const int q = 3 * J + 2 * (T2 / 2 + 1) + P;
-  double pre = 1.0;
-  int r, s;
-  if (a > b) {
-    r = dpos(L, J, b, a);
-    pre = sgnp(1 + ((ja + jb - J) >> 1));
-  } else {
-    if (a == b) pre = pre * M_SQRT2;
-    r = dpos(L, J, a, b);
-  }
-  if (c > d) {
-    s = dpos(L, J, d, c);
-    pre = pre * sgnp(1 + ((jc + jd - J) >> 1));
-  } else {
-    if (c == d) pre = pre * M_SQRT2;
-    s = dpos(L, J, c, d);
-  }
-  if ((r | s) < 0) return 0.0;  // unreachable for valid tables; never read out of bounds
-  return V[L.boff[q] + (long long)r * L.bdim[q] + s] * pre;
-}
 
 // ------------------------------------------------------------------ elementwise
 __global__ void k_fill(double *p, long long n, double v, const int *act, long long stride) {
@@ -818,9 +762,6 @@ void launch_sym_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm,
 }
 
 // ------------------------------------------------------------------ one-body (111 + 121 + 221)
-__device__ __forceinline__ bool same_ljt(const DevLayout &L, int a, int b) {
-  return L.jj[a] == L.jj[b] && L.l[a] == L.l[b] && L.tz[a] == L.tz[b];
-}
 // (Y - pm*Y^T) at pair rows (c,x),(c,y) of block J, the 221 intermediate, with
 // v_elem's ordering phase and identical-orbit sqrt2 applied to both pairs.
 // Hole c: Y = W1 = L(:,pp) R(pp,:), read from the upper-triangle store Wpp of
@@ -863,11 +804,6 @@ __device__ __forceinline__ double w221(const DevLayout &L, const double *Wpp, co
   return val * pre;
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
 
 // one 256-thread workgroup per one-body output (p,q); the (orbit, J) pairs of
 // the 121 and 221 sums are flattened over the threads (MJ = nJ slots per
@@ -941,18 +877,6 @@ void launch_onebody(hipStream_t s, const DevLayout &L, const double *Lop, const 
 }
 
 // ------------------------------------------------------------------ reductions
-__device__ double block_sum256(double v, double *sh) {
-  const int tid = threadIdx.x;
-  sh[tid] = v;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (tid < w) sh[tid] += sh[tid + w];
-    __syncthreads();
-  }
-  const double r = sh[0];
-  __syncthreads();
-  return r;
-}
 
 // 220 per block (commutators.f90:399-432) into partials[q]; 110 into partials[nblocks]
 __global__ __launch_bounds__(256) void k_e0_partial(DevLayout L, const double *Lop, const double *Rop,

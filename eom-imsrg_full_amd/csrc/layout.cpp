@@ -360,6 +360,39 @@ void Layout::unpack_vec(const double *v, int herm, double *buf) const {
   }
 }
 
+void Layout::unpack_vec_transpose(const double *buf, int herm, double *v) const {
+  size_t k = 0;
+  v[k++] = buf[0];
+  const double *F = buf + f_off;
+  for (int i = 0; i < nhole; i++)
+    for (int j = 0; j < nhole; j++) v[k++] = F[(size_t)holes[j] * nsp + holes[i]];
+  for (int i = 0; i < npart; i++)
+    for (int j = 0; j < npart; j++) v[k++] = F[(size_t)parts[j] * nsp + parts[i]];
+  for (int i = 0; i < nhole; i++)
+    for (int j = 0; j < npart; j++)
+      v[k++] = F[(size_t)parts[j] * nsp + holes[i]] + herm * F[(size_t)holes[i] * nsp + parts[j]];
+  const double *V = buf + v_off;
+  for (int q = 0; q < nblocks; q++) {
+    int dim = bdim[q];
+    const int cls_off[4] = {0, bnp[q], bnp[q] + bnb[q], dim};
+    const double *M = V + boff[q];
+    for (int g = 0; g < 6; g++) {
+      int r0 = cls_off[SEA1[g] - 1], nr = cls_off[SEA1[g]] - r0;
+      int c0 = cls_off[SEA2[g] - 1], nc = cls_off[SEA2[g]] - c0;
+      for (int i = 0; i < nc; i++)
+        for (int j = SQS[g] ? i : 0; j < nr; j++) {
+          const double a = M[(size_t)(r0 + j) * dim + (c0 + i)];
+          if (SQS[g]) {
+            // X(j,i) = x then X(i,j) = herm*x; on the diagonal the second write wins
+            v[k++] = (j == i) ? herm * a : a + herm * M[(size_t)(r0 + i) * dim + (c0 + j)];
+          } else {
+            v[k++] = a + herm * M[(size_t)(c0 + i) * dim + (r0 + j)];
+          }
+        }
+    }
+  }
+}
+
 // vectorize (basic_IMSRG.f90:3794-3854) from the full layout
 void Layout::pack_vec(const double *buf, double *v) const {
   size_t k = 0;

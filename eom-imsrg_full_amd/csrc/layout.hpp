@@ -86,6 +86,10 @@ struct Layout {
                     const int *con_);
 
   // host conversions between the reference vectorize stream and the full layout
+  // gradient with respect to the vectorize stream of a full-layout cotangent:
+  // the transpose of unpack_vec (every full element unpack_vec writes adds
+  // its cotangent, times the factor it was written with, to its source)
+  void unpack_vec_transpose(const double *cot, int herm, double *v) const;
   void unpack_vec(const double *vec, int herm, double *opbuf) const;   // repackage
   void pack_vec(const double *opbuf, double *vec) const;               // vectorize
 

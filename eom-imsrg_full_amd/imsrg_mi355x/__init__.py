@@ -8,4 +8,4 @@ Host-side Python view of the C-ABI library ``libimsrg_mi355x.so``:
   raises if it is missing -- there is no CPU fallback).
 """
 from .basis import Basis, hk_orbits, closed_shell_con, gemm_flops_222, pandya_blocks  # noqa: F401
-from .synthetic import synthetic_hamiltonian, random_operator, config_basis, batch_hamiltonians, batch_parameters  # noqa: F401
+from .synthetic import synthetic_hamiltonian, random_operator, config_basis, batch_hamiltonians, batch_parameters, gaussian_direction  # noqa: F401

@@ -69,6 +69,10 @@ _SIG = {
     "imsrg_mbpt2_batch": (C.c_int, [_vp, _dp]),
     "imsrg_normal_order": (C.c_int, [_vp]),
     "imsrg_commutator": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint]),
+    "imsrg_commutator_adjoint": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_uint]),
+    "imsrg_flow_gradient": (C.c_int, [_vp, _vp, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _vp]),
+    "imsrg_normal_order_adjoint": (C.c_int, [_vp, _vp]),
+    "imsrg_op_download_cotangent": (C.c_int, [_vp, _dp, _i64]),
     "imsrg_commutator_110": (C.c_int, [_vp, _vp, _dp]),
     "imsrg_commutator_220": (C.c_int, [_vp, _vp, _dp]),
     "imsrg_cross_coupled": (C.c_int, [_vp, _dp, _i64]),
@@ -173,6 +177,22 @@ class Context:
             RES = Operator(self, -L.herm * R.herm)
         _chk(_lib.imsrg_commutator(self.h, L.h, R.h, RES.h, mask))
         return RES
+
+    def commutator_adjoint(self, P, Cbar, hY, mask=TALL, Ybar=None):
+        """Ybar = d<Cbar, [P, Y]>/dY for Y of hermiticity hY (Frobenius product
+        over the full layout), projected onto hY."""
+        Ybar = Ybar or Operator(self, hY)
+        _chk(_lib.imsrg_commutator_adjoint(self.h, P.h, Cbar.h, Ybar.h, int(hY), mask))
+        return Ybar
+
+    def flow_gradient(self, H0, kind="white", ds=0.5, nsteps=8, ncheck=8):
+        """E0 after nsteps Magnus steps from the normal-ordered H0, and dE0/dH0
+        as an operator (full-layout Frobenius gradient, hermitian projection)."""
+        Hb = Operator(self, 1)
+        e0 = np.zeros(1)
+        _chk(_lib.imsrg_flow_gradient(self.h, H0.h, GEN[kind], float(ds), int(nsteps), int(ncheck), _dptr(e0),
+                                      Hb.h))
+        return float(e0[0]), Hb
 
     def generator(self, kind, H, ETA=None):
         ETA = ETA or Operator(self, -1)
@@ -307,6 +327,19 @@ class Operator:
     def normal_order(self):
         _chk(_lib.imsrg_normal_order(self.h))
         return self
+
+    def normal_order_adjoint(self):
+        """transpose of normal ordering applied to this cotangent (a new operator)"""
+        out = Operator(self.ctx, self.herm)
+        _chk(_lib.imsrg_normal_order_adjoint(self.h, out.h))
+        return out
+
+    def download_cotangent(self):
+        """this full-layout cotangent as a gradient with respect to the vectorize stream"""
+        B = self.ctx.nbatch
+        v = np.zeros(self.ctx.neq * B)
+        _chk(_lib.imsrg_op_download_cotangent(self.h, _dptr(v), v.size))
+        return v if B == 1 else v.reshape(B, self.ctx.neq)
 
 
 def commutator_110(L, R):

@@ -66,6 +66,27 @@ def batch_hamiltonians(basis: Basis, seeds):
     return np.stack(out)
 
 
+def gaussian_direction(basis: Basis, seed: int, sigma: float = 1.0):
+    """A random Hermitian perturbation as a vectorize stream: symmetric
+    Gaussian one-body part within (l, j, tz) (diagonal included, i.e. the
+    single-particle energies), symmetric Gaussian two-body blocks with the
+    Pauli zeros, E0 = 0 (normal ordering overwrites it)."""
+    rng = np.random.Generator(np.random.PCG64(5000 + seed))
+    N = basis.nsp
+    f = np.zeros((N, N))
+    same = basis.same_ljt()
+    off = _sym_gauss(rng, N, sigma)
+    f[same] = off[same]
+    mats = []
+    for blk in basis.blocks:
+        M = _sym_gauss(rng, blk.dim, sigma)
+        ok = basis.pauli_mask(blk)
+        M[~ok, :] = 0.0
+        M[:, ~ok] = 0.0
+        mats.append(M)
+    return basis.vec_from_full(0.0, f, mats)
+
+
 def random_operator(basis: Basis, herm: int, seed: int):
     """Random operator with hermiticity ``herm`` (+1/-1) as a vectorize stream."""
     rng = np.random.Generator(np.random.PCG64(seed))

@@ -126,6 +126,30 @@ int imsrg_commutator_220(imsrg_op L, imsrg_op R, double *out);
 int imsrg_cross_coupled(imsrg_op op, double *host_out, int64_t n);
 int64_t imsrg_cross_coupled_size(imsrg_ctx ctx);
 
+/* Adjoint commutator (no counterpart in the reference; the gradient path):
+ * Ybar = d <Cbar, [P, Y]> / dY for Y of hermiticity hY, for the terms in
+ * mask, where <,> is the Frobenius product over E0, the full one-body matrix
+ * and every full (J,P,Tz) channel matrix.  Cbar (herm -herm(P)*hY) is used
+ * after projection onto its hermiticity; Ybar comes out projected onto hY,
+ * E0 = 0.  Every term is the exact transpose of its forward kernel.  By
+ * antisymmetry d<Cbar,[X,Y]>/dX = -(the same call with P = Y).  nbatch = 1. */
+int imsrg_commutator_adjoint(imsrg_ctx ctx, imsrg_op P, imsrg_op Cbar, imsrg_op Ybar, int hY, unsigned mask);
+
+/* Gradient of E0(s_N) after nsteps Magnus steps (magnus_decouple's step loop,
+ * IMSRG_MAGNUS.f90:98-126, White or atan generator) with respect to the
+ * normal-ordered H0, by the discrete adjoint of the flow: every nested
+ * commutator of the BCH and Magnus series is transposed (term counts held at
+ * the forward flow's), Omega(s) is kept at ncheck checkpoints and each segment
+ * is recomputed.  *e0 = E0(s_N); H0bar = dE0/dH0 (Frobenius over the full
+ * layout, hermitian projection).  nbatch = 1.  No counterpart in the reference. */
+int imsrg_flow_gradient(imsrg_ctx ctx, imsrg_op H0, int gen_kind, double ds, int nsteps, int ncheck,
+                        double *e0, imsrg_op H0bar);
+/* transpose of imsrg_normal_order: Hbar = d<Cbar, NO(H)>/dH (E0 of Hbar = 0) */
+int imsrg_normal_order_adjoint(imsrg_op Cbar, imsrg_op Hbar);
+/* a cotangent as the gradient with respect to the vectorize stream (the
+ * transpose of repackage, basic_IMSRG.f90:3857-3918), n = neq * nbatch */
+int imsrg_op_download_cotangent(imsrg_op op, double *vec, int64_t n);
+
 /* ---- generators (external build_generator callbacks) ---------------------- */
 int imsrg_generator(imsrg_ctx ctx, int kind, imsrg_op H, imsrg_op ETA);
 /* build_gs_atanxx (generators.f90:271-374): denominators from Hold, numerators from H */
