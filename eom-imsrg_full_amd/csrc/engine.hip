@@ -6,7 +6,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/imsrg_mi355x.h"
@@ -60,6 +62,13 @@ struct imsrg_ctx_s {
   imsrg_op_s *tmp[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // per-kernel-class HIP event timing on this context's stream
   bool prof_on = false;
+  // hipGraph replay of whole commutators: one executable graph per distinct
+  // (operands, herms, mask, Pandya reuse, instance table, precision)
+  bool graphs = false;
+  typedef std::tuple<const void *, const void *, const void *, int, int, int, unsigned, int, const void *, int, int>
+      GraphKey;
+  std::map<GraphKey, hipGraphExec_t> gexec;
+  long long graph_launches = 0, graph_builds = 0;
   std::vector<hipEvent_t> evpool;
   size_t evused = 0;
   struct Rec { int cls; hipEvent_t a, b; };
@@ -461,6 +470,7 @@ int imsrg_ctx_destroy(imsrg_ctx c) {
       (void)hipFree(t->d);
       delete t;
     }
+  for (auto &kv : c->gexec) (void)hipGraphExecDestroy(kv.second);
   for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
   for (void *p : c->dallocs) (void)hipFree(p);
   if (c->hscal) (void)hipHostFree(c->hscal);
@@ -497,6 +507,18 @@ int imsrg_ctx_set_precision(imsrg_ctx c, int prec) {
   c->prec = prec;
   return 0;
 }
+int imsrg_ctx_set_graphs(imsrg_ctx c, int enable) {
+  if (!c) return fail(IMSRG_EINVAL, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->graphs = enable != 0;
+  return 0;
+}
+int imsrg_ctx_graph_stats(imsrg_ctx c, long long *builds, long long *launches) {
+  if (!c || !builds || !launches) return fail(IMSRG_EINVAL, "null argument");
+  *builds = c->graph_builds;
+  *launches = c->graph_launches;
+  return 0;
+}
 int imsrg_ctx_synchronize(imsrg_ctx c) {
   if (!c) return fail(IMSRG_EINVAL, "null context");
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -510,9 +532,20 @@ int imsrg_op_create(imsrg_ctx c, int herm, imsrg_op *out) {
   if (herm != 1 && herm != -1) return fail(IMSRG_EINVAL, "herm must be +1 or -1");
   return op_new(c, herm, out);
 }
+static void drop_graphs_of(imsrg_ctx_s *c, const void *d) {
+  for (auto it = c->gexec.begin(); it != c->gexec.end();) {
+    if (std::get<0>(it->first) == d || std::get<1>(it->first) == d || std::get<2>(it->first) == d) {
+      (void)hipGraphExecDestroy(it->second);
+      it = c->gexec.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
 int imsrg_op_destroy(imsrg_op op) {
   if (!op) return 0;
   (void)hipStreamSynchronize(op->ctx->stream);
+  drop_graphs_of(op->ctx, op->d);
   (void)hipFree(op->d);
   delete op;
   return 0;
@@ -642,8 +675,42 @@ int imsrg_normal_order(imsrg_op H) {
 // lcc_ready: cc[0] already holds the Pandya matrices of L (drivers reuse them
 // across the nested commutators of one BCH / Magnus series, where L = Omega).
 // Every launch covers the instances of `a`.
+static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *RES, unsigned mask,
+                         bool lcc_ready, const ActSet &a);
+// One commutator: the launch sequence of comm_launches, replayed from a cached
+// hipGraph when graph mode is on (and profiling off).  The graph holds kernel
+// nodes only; operands, workspaces and the instance table are fixed device
+// addresses, so a replay is the same work as the eager sequence.
 static int comm_impl(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *RES, unsigned mask,
                      bool lcc_ready, const ActSet &a) {
+  if (!c->graphs || c->prof_on || a.n() == 0) return comm_launches(c, L, R, RES, mask, lcc_ready, a);
+  const imsrg_ctx_s::GraphKey key(L->d, R->d, RES->d, L->herm, R->herm, RES->herm, mask, lcc_ready ? 1 : 0, a.d,
+                                  a.n(), c->prec);
+  auto it = c->gexec.find(key);
+  if (it == c->gexec.end()) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = comm_launches(c, L, R, RES, mask, lcc_ready, a);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(IMSRG_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(IMSRG_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+    it = c->gexec.emplace(key, x).first;
+    c->graph_builds++;
+  }
+  HIPCHK(hipGraphLaunch(it->second, c->stream));
+  c->graph_launches++;
+  return 0;
+}
+
+static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *RES, unsigned mask,
+                         bool lcc_ready, const ActSet &a) {
   hipStream_t s = c->stream;
   const DevLayout &D = c->dl;
   const int hL = L->herm, hR = R->herm, hres = RES->herm;

@@ -49,6 +49,8 @@ _SIG = {
     "imsrg_ctx_info": (C.c_int, [_vp, _dp]),
     "imsrg_ctx_set_precision": (C.c_int, [_vp, C.c_int]),
     "imsrg_ctx_synchronize": (C.c_int, [_vp]),
+    "imsrg_ctx_set_graphs": (C.c_int, [_vp, C.c_int]),
+    "imsrg_ctx_graph_stats": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "imsrg_ctx_stream": (_vp, [_vp]),
     "imsrg_op_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
     "imsrg_op_destroy": (C.c_int, [_vp]),
@@ -160,6 +162,15 @@ class Context:
 
     def sync(self):
         _chk(_lib.imsrg_ctx_synchronize(self.h))
+
+    def set_graphs(self, enable=True):
+        """replay every commutator from a cached hipGraph"""
+        _chk(_lib.imsrg_ctx_set_graphs(self.h, 1 if enable else 0))
+
+    def graph_stats(self):
+        b, n = C.c_longlong(0), C.c_longlong(0)
+        _chk(_lib.imsrg_ctx_graph_stats(self.h, C.byref(b), C.byref(n)))
+        return b.value, n.value
 
     @property
     def stream(self):

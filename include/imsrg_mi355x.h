@@ -85,6 +85,12 @@ int64_t imsrg_ctx_neq(imsrg_ctx ctx);
 int imsrg_ctx_info(imsrg_ctx ctx, double *info16);
 int imsrg_ctx_set_precision(imsrg_ctx ctx, int prec);
 int imsrg_ctx_synchronize(imsrg_ctx ctx);
+/* Graph mode (no counterpart in the reference): every commutator the context
+ * evaluates -- directly or inside BCH / Magnus series and flows -- is captured
+ * once per distinct (operands, hermiticities, term mask, instance set,
+ * precision) into a hipGraph and replayed.  Off while profiling is on. */
+int imsrg_ctx_set_graphs(imsrg_ctx ctx, int enable);
+int imsrg_ctx_graph_stats(imsrg_ctx ctx, long long *builds, long long *launches);
 /* the hipStream_t the context launches on (as void*) */
 void *imsrg_ctx_stream(imsrg_ctx ctx);
 
