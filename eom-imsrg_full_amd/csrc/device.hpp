@@ -83,7 +83,8 @@ void launch_axpby(hipStream_t s, long long n, double a, const double *A, double 
                   const int *act, int nact, long long stride);
 void launch_copy(hipStream_t s, long long n, const double *A, double *C, const int *act, int nact,
                  long long stride);
-// mode: 0 FP64 MFMA, 1 split-FP32 (3 FP32-MFMA products), 2 split-BF16 (6 BF16-MFMA products).
+// mode: 0 FP64 MFMA (64 x 64 tiles), 1 split-FP32 (3 FP32-MFMA products), 2 split-BF16
+// (6 BF16-MFMA products), 3 FP64 MFMA on 128 x 64 tiles (tile list built for that shape).
 // 222_pp_hh: RES(upper) += L*D*R - R*D*L per channel block; with W, the pp part
 // (W1 - pm*W1^T) is also stored to W (upper triangle) for commutator_221.
 void launch_seg_gemm_pphh(hipStream_t s, int mode, const SegProb *probs, const int4 *tiles, int ntiles,

@@ -52,8 +52,14 @@ struct imsrg_ctx_s {
   GemmProb *adj_pp_probs = nullptr, *adj_ph_probs = nullptr;  // adjoint products
   int4 *adj_pp_tiles = nullptr, *adj_ph_tiles = nullptr;
   int n_adj_pp_tiles = 0, n_adj_ph_tiles = 0;
-  int4 *pp_tiles = nullptr, *ph_tiles = nullptr;
+  int4 *pp_tiles = nullptr, *ph_tiles = nullptr;        // 64 x 64 tiles
   int n_pp_tiles = 0, n_ph_tiles = 0;
+  int4 *pp_tiles_w = nullptr, *ph_tiles_w = nullptr;    // 128 x 64 tiles (FP64 path)
+  int n_pp_tiles_w = 0, n_ph_tiles_w = 0;
+  // FP64 tile shape per GEMM, chosen by timing both on the first FP64
+  // commutator of the context: -1 not yet tuned, 0 64x64, 1 128x64
+  int shape_pp = -1, shape_ph = -1;
+  float tune_ms[4] = {0, 0, 0, 0};                      // pp 64, pp 128x64, ph 64, ph 128x64
   int prec = IMSRG_PREC_FP64;
   // active-instance tables: slot 0 = all instances (constant), 1.. = driver work sets
   static constexpr int kSlots = 4;
@@ -156,15 +162,21 @@ static XcdSplit make_split(const std::vector<double> &w) {
 // XCDs by flop weight (heaviest first onto the least-loaded XCD): every tile
 // of one channel block then runs on one XCD and shares its L2, and each XCD's
 // list runs heaviest first.  Short lists are padded with empty slots (x = -1).
-static void build_seg_tiles(const std::vector<SegProb> &probs, std::vector<int4> &tiles) {
+static void upper_tiles(int M, int TM, int TN, int pi, std::vector<int4> &out) {
+  for (int i = 0; i * TM < M; i++)
+    for (int j = 0; j * TN < M; j++)
+      if (i * TM <= j * TN + TN - 1) out.push_back(make_int4(pi, i, j, 0));
+}
+static void build_seg_tiles(const std::vector<SegProb> &probs, std::vector<int4> &tiles, int TM = 64, int TN = 64) {
   const int NX = 8;
   std::vector<double> w(probs.size());
   std::vector<int> order(probs.size());
   for (size_t i = 0; i < probs.size(); i++) {
     double kt = 0;
     for (int s = 0; s < probs[i].nseg; s++) kt += probs[i].seg[s].K;
-    const double T = (probs[i].M + 63) / 64;
-    w[i] = kt * T * (T + 1) / 2;
+    std::vector<int4> t;
+    upper_tiles(probs[i].M, TM, TN, 0, t);
+    w[i] = kt * (double)t.size() * TM * TN;
     order[i] = (int)i;
   }
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return w[a] > w[b]; });
@@ -172,9 +184,7 @@ static void build_seg_tiles(const std::vector<SegProb> &probs, std::vector<int4>
   std::vector<double> load(NX, 0.0);
   for (int pi : order) {
     const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-    const int T = (probs[pi].M + 63) / 64;
-    for (int i = 0; i < T; i++)
-      for (int j = i; j < T; j++) bins[x].push_back(make_int4(pi, i, j, 0));
+    upper_tiles(probs[pi].M, TM, TN, pi, bins[x]);
     load[x] += w[pi];
   }
   size_t len = 0;
@@ -398,11 +408,14 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
       P.seg[3] = SegDesc{ah, bh, nh, dim, dim, 1, 1.0};
       pp.push_back(P);
     }
-    std::vector<int4> tiles;
+    std::vector<int4> tiles, tw;
     build_seg_tiles(pp, tiles);
+    build_seg_tiles(pp, tw, 128, 64);
     c->n_pp_tiles = (int)tiles.size();
+    c->n_pp_tiles_w = (int)tw.size();
     rc = upload_table(c, pp, (const SegProb **)&c->pp_probs);
     if (!rc) rc = upload_table(c, tiles, (const int4 **)&c->pp_tiles);
+    if (!rc) rc = upload_table(c, tw, (const int4 **)&c->pp_tiles_w);
   }
   if (!rc) {
     std::vector<SegProb> ph;
@@ -418,11 +431,14 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
       P.seg[1] = SegDesc{Ly.ccoff[q], Ly.ccoff[q], nbq, r, r, 1, 1.0};
       ph.push_back(P);
     }
-    std::vector<int4> tiles;
+    std::vector<int4> tiles, tw;
     build_seg_tiles(ph, tiles);
+    build_seg_tiles(ph, tw, 128, 64);
     c->n_ph_tiles = (int)tiles.size();
+    c->n_ph_tiles_w = (int)tw.size();
     rc = upload_table(c, ph, (const SegProb **)&c->ph_probs);
     if (!rc) rc = upload_table(c, tiles, (const int4 **)&c->ph_tiles);
+    if (!rc) rc = upload_table(c, tw, (const int4 **)&c->ph_tiles_w);
   }
   // adjoint products: Q = P * Cbar per channel block; Ybar_CC^T = P_CC^T * Sbar^T per Pandya block
   if (!rc) {
@@ -495,7 +511,8 @@ int imsrg_ctx_info(imsrg_ctx c, double *info) {
   for (int d : L.bdim) nup += 0.5 * d * (d + 1.0);
   double v[16] = {(double)L.nsp, (double)L.nhole, (double)L.nblocks, (double)L.ncc, (double)L.npairs,
                   (double)maxdim, (double)maxr, 8.0 * L.op_size, (double)L.vsize, (double)L.ccsize,
-                  (double)L.cwsize, L.flop_pphh, L.flop_ph, (double)L.ph_terms.size(), nup, 0};
+                  (double)L.cwsize, L.flop_pphh, L.flop_ph, (double)L.ph_terms.size(), nup,
+                  (double)(c->shape_pp + 1) * 4 + (double)(c->shape_ph + 1)};
   std::memcpy(info, v, sizeof(v));
   return 0;
 }
@@ -505,6 +522,21 @@ int imsrg_ctx_set_precision(imsrg_ctx c, int prec) {
   if (prec != IMSRG_PREC_FP64 && prec != IMSRG_PREC_SPLIT_FP32 && prec != IMSRG_PREC_SPLIT_BF16)
     return fail(IMSRG_EINVAL, "unknown precision");
   c->prec = prec;
+  return 0;
+}
+int imsrg_ctx_tuning(imsrg_ctx c, int *shape_pp, int *shape_ph, float *ms4) {
+  if (!c || !shape_pp || !shape_ph) return fail(IMSRG_EINVAL, "null argument");
+  *shape_pp = c->shape_pp;
+  *shape_ph = c->shape_ph;
+  if (ms4) std::memcpy(ms4, c->tune_ms, sizeof(c->tune_ms));
+  return 0;
+}
+int imsrg_ctx_set_tile_shapes(imsrg_ctx c, int shape_pp, int shape_ph) {
+  if (!c || shape_pp < -1 || shape_pp > 1 || shape_ph < -1 || shape_ph > 1)
+    return fail(IMSRG_EINVAL, "shapes: -1 (tune), 0 (64x64) or 1 (128x64)");
+  if (shape_pp < 0 || shape_ph < 0) shape_pp = shape_ph = -1;
+  c->shape_pp = shape_pp;
+  c->shape_ph = shape_ph;
   return 0;
 }
 int imsrg_ctx_set_graphs(imsrg_ctx c, int enable) {
@@ -677,12 +709,67 @@ int imsrg_normal_order(imsrg_op H) {
 // Every launch covers the instances of `a`.
 static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *RES, unsigned mask,
                          bool lcc_ready, const ActSet &a);
+
+// Tile-shape autotuner: on the first FP64 commutator of a context, each 222
+// GEMM is timed (HIP events, best of 3) in both FP64 tile shapes on the
+// context's real channel/Pandya blocks and the faster one is kept.  The
+// pp/hh timing writes nothing (no C, no W); the Pandya timing writes the Z
+// workspace, which every commutator recomputes before reading.
+static int autotune(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, const ActSet &a) {
+  hipStream_t s = c->stream;
+  const DevLayout &D = c->dl;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  auto timed = [&](auto &&launch) {
+    float best = 1e30f;
+    for (int r = 0; r < 4; r++) {
+      (void)hipEventRecord(e0, s);
+      launch();
+      (void)hipEventRecord(e1, s);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (r > 0) best = std::min(best, ms);  // first run warms up
+    }
+    return best;
+  };
+  const double *X0 = L->d + D.v_off, *X1 = R->d + D.v_off;
+  c->tune_ms[0] = timed([&] {
+    launch_seg_gemm_pphh(s, 0, c->pp_probs, c->pp_tiles, c->n_pp_tiles, X0, X1, nullptr, nullptr, a.d, a.n(), D.op_size,
+                         D.vsize);
+  });
+  c->tune_ms[1] = timed([&] {
+    launch_seg_gemm_pphh(s, 3, c->pp_probs, c->pp_tiles_w, c->n_pp_tiles_w, X0, X1, nullptr, nullptr, a.d, a.n(),
+                         D.op_size, D.vsize);
+  });
+  launch_pandya(s, D, L->d, c->cc[0], a.d, a.n());
+  launch_pandya(s, D, R->d, c->cc[1], a.d, a.n());
+  c->tune_ms[2] = timed([&] {
+    launch_seg_gemm_z(s, 0, c->ph_probs, c->ph_tiles, c->n_ph_tiles, c->cc[0], c->cc[1], c->wph, (double)L->herm,
+                      (double)R->herm, a.d, a.n(), D.ccsize, D.cwsize);
+  });
+  c->tune_ms[3] = timed([&] {
+    launch_seg_gemm_z(s, 3, c->ph_probs, c->ph_tiles_w, c->n_ph_tiles_w, c->cc[0], c->cc[1], c->wph, (double)L->herm,
+                      (double)R->herm, a.d, a.n(), D.ccsize, D.cwsize);
+  });
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->shape_pp = c->tune_ms[1] < c->tune_ms[0] ? 1 : 0;
+  c->shape_ph = c->tune_ms[3] < c->tune_ms[2] ? 1 : 0;
+  HIPCHK(hipGetLastError());
+  return 0;
+}
 // One commutator: the launch sequence of comm_launches, replayed from a cached
 // hipGraph when graph mode is on (and profiling off).  The graph holds kernel
 // nodes only; operands, workspaces and the instance table are fixed device
 // addresses, so a replay is the same work as the eager sequence.
 static int comm_impl(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *RES, unsigned mask,
                      bool lcc_ready, const ActSet &a) {
+  if (c->prec == IMSRG_PREC_FP64 && c->shape_pp < 0 && a.n() > 0 && (mask & (T222PPHH | T221 | T222PH))) {
+    int rc = autotune(c, L, R, a);
+    if (rc) return rc;
+  }
   if (!c->graphs || c->prof_on || a.n() == 0) return comm_launches(c, L, R, RES, mask, lcc_ready, a);
   const imsrg_ctx_s::GraphKey key(L->d, R->d, RES->d, L->herm, R->herm, RES->herm, mask, lcc_ready ? 1 : 0, a.d,
                                   a.n(), c->prec);
@@ -722,17 +809,22 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
   if (mask & T222PH) {
     if (!lcc_ready) PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, L->d, c->cc[0], ad, na));
     PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, R->d, c->cc[1], ad, na));
+    const bool w = c->prec == IMSRG_PREC_FP64 && c->shape_ph == 1;
     PROF(c, IMSRG_PROF_GEMM_PH,
-         launch_seg_gemm_z(s, c->prec, c->ph_probs, c->ph_tiles, c->n_ph_tiles, c->cc[0], c->cc[1], c->wph,
-                           (double)hL, (double)hR, ad, na, D.ccsize, D.cwsize));
+         launch_seg_gemm_z(s, w ? 3 : c->prec, c->ph_probs, w ? c->ph_tiles_w : c->ph_tiles,
+                           w ? c->n_ph_tiles_w : c->n_ph_tiles, c->cc[0], c->cc[1], c->wph, (double)hL, (double)hR,
+                           ad, na, D.ccsize, D.cwsize));
   }
   if (mask & T122) PROF(c, IMSRG_PROF_122, launch_122(s, D, L->d, R->d, RES->d, hres, ad, na));
   else launch_clear_blocks(s, D, RES->d, ad, na);
-  if (mask & (T222PPHH | T221))
+  if (mask & (T222PPHH | T221)) {
+    const bool w = c->prec == IMSRG_PREC_FP64 && c->shape_pp == 1;
     PROF(c, IMSRG_PROF_GEMM_PPHH,
-         launch_seg_gemm_pphh(s, c->prec, c->pp_probs, c->pp_tiles, c->n_pp_tiles, L->d + D.v_off,
-                              R->d + D.v_off, (mask & T222PPHH) ? RES->d + D.v_off : nullptr,
-                              (mask & T221) ? c->wpp : nullptr, ad, na, OS, D.vsize));
+         launch_seg_gemm_pphh(s, w ? 3 : c->prec, c->pp_probs, w ? c->pp_tiles_w : c->pp_tiles,
+                              w ? c->n_pp_tiles_w : c->n_pp_tiles, L->d + D.v_off, R->d + D.v_off,
+                              (mask & T222PPHH) ? RES->d + D.v_off : nullptr, (mask & T221) ? c->wpp : nullptr, ad,
+                              na, OS, D.vsize));
+  }
   if (mask & T222PH) {
     PROF(c, IMSRG_PROF_PH_TERMS, launch_ph_terms(s, D, c->wph, c->tph, (double)hL, (double)hR, ad, na));
     PROF(c, IMSRG_PROF_PH_APPLY, launch_ph_apply(s, D, c->tph, RES->d, ad, na));

@@ -80,6 +80,9 @@ void launch_copy(hipStream_t s, long long n, const double *A, double *C, const i
 #ifndef IMSRG_GEMM_BK
 #define IMSRG_GEMM_BK 16
 #endif
+#ifndef IMSRG_GEMM_PF
+#define IMSRG_GEMM_PF 2  // K slabs of global loads in flight ahead of the MFMAs (1 or 2)
+#endif
 constexpr int GT = 64, GK = IMSRG_GEMM_BK, GP = 1, GJ = GK / 4;
 static_assert(GK == 16 || GK == 32, "K slab of 16 or 32");
 
@@ -155,9 +158,11 @@ __device__ void seg_f64(const SegDesc &sgd, double sgn, int M, const double *__r
     }
   }
   const int ainc = A_KMAJOR ? GK * lda : GK, binc = GK * ldb;
-  double ra[GJ], rb[GJ];
-  unsigned ok = 0;
-  auto load = [&](int k0) {
+  // two register sets: the loads of slab kt+2 are issued while slab kt is
+  // multiplied, so each load has two slabs of MFMA work to arrive in
+  double ra0[GJ], rb0[GJ], ra1[GJ], rb1[GJ];
+  unsigned ok0 = 0, ok1 = 0;
+  auto load = [&](int k0, double (&ra)[GJ], double (&rb)[GJ], unsigned &ok) {
     ok = 0;
 #pragma unroll
     for (int j = 0; j < GJ; j++) {
@@ -170,7 +175,7 @@ __device__ void seg_f64(const SegDesc &sgd, double sgn, int M, const double *__r
       boff[j] += binc;
     }
   };
-  auto store = [&](int buf) {
+  auto store = [&](int buf, const double (&ra)[GJ], const double (&rb)[GJ], unsigned ok) {
 #pragma unroll
     for (int j = 0; j < GJ; j++) {
       const double a = ((ok >> j) & 1) ? ra[j] : 0.0;
@@ -180,17 +185,40 @@ __device__ void seg_f64(const SegDesc &sgd, double sgn, int M, const double *__r
       S.B[buf][(tid >> 6) + 4 * j][tid & 63] = b;
     }
   };
-  load(0);
-  store(0);
+#if IMSRG_GEMM_PF == 2
+  load(0, ra0, rb0, ok0);
+  store(0, ra0, rb0, ok0);
+  if (nk > 1) load(GK, ra1, rb1, ok1);
+  __syncthreads();
+  for (int kt = 0; kt < nk; kt += 2) {
+    // slab kt in LDS 0, slab kt+1 in registers 1, registers 0 free
+    if (kt + 2 < nk) load((kt + 2) * GK, ra0, rb0, ok0);
+    if (need) slab_mfma(S, 0, acc, need);
+    if (kt + 1 < nk) store(1, ra1, rb1, ok1);
+    __syncthreads();
+    if (kt + 1 >= nk) break;
+    // slab kt+1 in LDS 1, slab kt+2 in registers 0, registers 1 free
+    if (kt + 3 < nk) load((kt + 3) * GK, ra1, rb1, ok1);
+    if (need) slab_mfma(S, 1, acc, need);
+    if (kt + 2 < nk) store(0, ra0, rb0, ok0);
+    __syncthreads();
+  }
+#else
+  (void)ra1;
+  (void)rb1;
+  (void)ok1;
+  load(0, ra0, rb0, ok0);
+  store(0, ra0, rb0, ok0);
   __syncthreads();
   for (int kt = 0; kt < nk; kt++) {
     const int cur = kt & 1;
     const bool more = kt + 1 < nk;
-    if (more) load((kt + 1) * GK);
+    if (more) load((kt + 1) * GK, ra0, rb0, ok0);
     if (need) slab_mfma(S, cur, acc, need);
-    if (more) store(cur ^ 1);
+    if (more) store(cur ^ 1, ra0, rb0, ok0);
     __syncthreads();
   }
+#endif
 }
 
 // local (row, col) of accumulator element (ti, tj, v) of this lane.
@@ -470,11 +498,233 @@ __global__ __launch_bounds__(256) void k_seg_gemm(const SegProb *__restrict__ pr
   }
 }
 
+// ---- FP64 path on 128 x 64 tiles: 4 wavefronts, each 32 rows x 64 columns
+// (2 x 4 v_mfma_f64_16x16x4 tiles), so every k-step reads 6 fragments for 8
+// MFMAs and a K slab moves 10.7 flop per byte.  Tile (I, J) covers rows
+// [128I, 128I+128) and columns [64J, 64J+64); tiles with no element on or
+// above the diagonal are not launched.  Writes: direct for row <= col, and
+// (EPI 1) mirrored sigma * value for row < col, which covers every element of
+// Z exactly once whatever the tile shape.
+constexpr int WM = 128, WN = 64;
+struct GemmSmemW {
+  double A[2][GK][WM + GP];
+  double B[2][GK][WN + GP];
+};
+static_assert(sizeof(GemmSmemW) >= sizeof(double) * 64 * (64 + GP), "transpose buffer must fit");
+
+__device__ __forceinline__ void slab_mfma_w(const GemmSmemW &S, int buf, double4_t acc[2][4], unsigned need) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int kq = 0; kq < GK / 4; kq++) {
+    const int k = kq * 4 + (lane >> 4);
+    double a[2], b[4];
+#pragma unroll
+    for (int t = 0; t < 2; t++) a[t] = S.A[buf][k][wave * 32 + t * 16 + (lane & 15)];
+#pragma unroll
+    for (int t = 0; t < 4; t++) b[t] = S.B[buf][k][t * 16 + (lane & 15)];
+#pragma unroll
+    for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+      for (int tj = 0; tj < 4; tj++)
+        if (need & (1u << (4 * ti + tj)))
+          acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
+  }
+}
+
+template <bool A_KMAJOR>
+__device__ void seg_f64_w(const SegDesc &sgd, double sgn, int M, const double *__restrict__ A,
+                          const double *__restrict__ B, int m0, int n0, GemmSmemW &S, double4_t acc[2][4],
+                          unsigned need) {
+  constexpr int JA = WM * GK / 256, JB = WN * GK / 256;  // elements per thread and slab
+  const int K = sgd.K, lda = sgd.lda, ldb = sgd.ldb;
+  const int nk = (K + GK - 1) / GK;
+  if (nk == 0) return;
+  const int tid = threadIdx.x;
+  int aoff[JA], akk[JA], boff[JB], bkk[JB];
+  unsigned rowok = 0;  // bits 0..JA-1: A rows in range, bits 16..: B columns in range
+  if (A_KMAJOR) {
+    const int m = m0 + (tid & 127);
+    const bool mv = m < M;
+#pragma unroll
+    for (int j = 0; j < JA; j++) {
+      akk[j] = (tid >> 7) + 2 * j;
+      aoff[j] = akk[j] * lda + (mv ? m : 0);
+      rowok |= (unsigned)mv << j;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < JA; j++) {
+      const int m = m0 + tid / GK + (256 / GK) * j;
+      const bool mv = m < M;
+      akk[j] = tid % GK;
+      aoff[j] = (mv ? m : 0) * lda + akk[j];
+      rowok |= (unsigned)mv << j;
+    }
+  }
+  {
+    const int n = n0 + (tid & 63);
+    const bool nv = n < M;
+#pragma unroll
+    for (int j = 0; j < JB; j++) {
+      bkk[j] = (tid >> 6) + 4 * j;
+      boff[j] = bkk[j] * ldb + (nv ? n : 0);
+      rowok |= (unsigned)nv << (16 + j);
+    }
+  }
+  const int ainc = A_KMAJOR ? GK * lda : GK, binc = GK * ldb;
+  double ra[JA], rb[JB];
+  unsigned ok = 0;
+  auto load = [&](int k0) {
+    ok = 0;
+#pragma unroll
+    for (int j = 0; j < JA; j++) {
+      const bool g = ((rowok >> j) & 1) && (k0 + akk[j] < K);
+      ra[j] = A[g ? aoff[j] : 0];
+      ok |= (unsigned)g << j;
+      aoff[j] += ainc;
+    }
+#pragma unroll
+    for (int j = 0; j < JB; j++) {
+      const bool g = ((rowok >> (16 + j)) & 1) && (k0 + bkk[j] < K);
+      rb[j] = B[g ? boff[j] : 0];
+      ok |= (unsigned)g << (16 + j);
+      boff[j] += binc;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < JA; j++) {
+      const double a = ((ok >> j) & 1) ? ra[j] : 0.0;
+      if (A_KMAJOR) S.A[buf][(tid >> 7) + 2 * j][tid & 127] = a;
+      else S.A[buf][tid % GK][tid / GK + (256 / GK) * j] = a;
+    }
+#pragma unroll
+    for (int j = 0; j < JB; j++) S.B[buf][(tid >> 6) + 4 * j][tid & 63] = ((ok >> (16 + j)) & 1) ? sgn * rb[j] : 0.0;
+  };
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; kt++) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nk;
+    if (more) load((kt + 1) * GK);
+    if (need) slab_mfma_w(S, cur, acc, need);
+    if (more) store(cur ^ 1);
+    __syncthreads();
+  }
+}
+
+template <bool A_KMAJOR, int EPI>
+__global__ __launch_bounds__(256) void k_seg_gemm_w(const SegProb *__restrict__ probs, const int4 *__restrict__ tiles,
+                                                    int ntiles, const double *__restrict__ X0,
+                                                    const double *__restrict__ X1, double g0, double g1,
+                                                    double *__restrict__ C, double *__restrict__ W, double sigma,
+                                                    const int *act, long long sx, long long sc, long long sw) {
+  const int t = blockIdx.x;
+  if (t >= ntiles) return;
+  {
+    const long long inst = inst_id(act);
+    X0 += inst * sx;
+    X1 += inst * sx;
+    if (C) C += inst * sc;
+    if (W) W += inst * sw;
+  }
+  const int4 tl = tiles[t];
+  if (tl.x < 0) return;
+  const SegProb &p = probs[tl.x];
+  const int m0 = tl.y * WM, n0 = tl.z * WN, M = p.M;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
+  unsigned need = 0;
+#pragma unroll
+  for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+    for (int tj = 0; tj < 4; tj++) {
+      const int r16 = m0 + wave * 32 + ti * 16, c16 = n0 + tj * 16;
+      if (r16 < M && c16 < M && r16 <= c16) need |= 1u << (4 * ti + tj);
+    }
+  __shared__ GemmSmemW S;
+  double4_t acc[2][4];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  auto run = [&](int s0, int s1) {
+    for (int s = s0; s < s1; s++) {
+      const SegDesc &sg = p.seg[s];
+      seg_f64_w<A_KMAJOR>(sg, sg.sign * (sg.src ? g1 : g0), M, (sg.src ? X1 : X0) + sg.a_off,
+                          (sg.src ? X0 : X1) + sg.b_off, m0, n0, S, acc, need);
+    }
+  };
+  // accumulator element (ti, tj, v): row lane/16 + 4v of 16x16 tile (ti, tj)
+  if (EPI == 0) {
+    run(0, p.nmid);
+    if (W && need) {
+      double *Wp = W + p.w_off;
+#pragma unroll
+      for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+        for (int tj = 0; tj < 4; tj++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int row = m0 + wave * 32 + ti * 16 + g + 4 * v, col = n0 + tj * 16 + (lane & 15);
+            if (row < M && col < M && row <= col) Wp[(long long)row * M + col] = acc[ti][tj][v];
+          }
+    }
+    run(p.nmid, p.nseg);
+    if (C && need) {
+      double *Cp = C + p.c_off;
+#pragma unroll
+      for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+        for (int tj = 0; tj < 4; tj++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int row = m0 + wave * 32 + ti * 16 + g + 4 * v, col = n0 + tj * 16 + (lane & 15);
+            if (row < M && col < M && row <= col) {
+              const long long o = (long long)row * M + col;
+              Cp[o] = Cp[o] + acc[ti][tj][v];
+            }
+          }
+    }
+  } else {
+    run(0, p.nseg);
+    double *Zp = C + p.c_off;
+    double(*T)[64 + GP] = reinterpret_cast<double(*)[64 + GP]>(&S);
+    const int c = threadIdx.x & 63;
+    for (int h = 0; h < 2; h++) {  // rows 64h .. 64h+63 of the tile go through LDS
+      if ((wave >> 1) == h) {
+#pragma unroll
+        for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+          for (int tj = 0; tj < 4; tj++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) T[(wave & 1) * 32 + ti * 16 + g + 4 * v][tj * 16 + (lane & 15)] = acc[ti][tj][v];
+      }
+      __syncthreads();
+      const int rb = m0 + 64 * h;
+      for (int r = threadIdx.x >> 6; r < 64; r += 4) {  // direct: element (rb + r, n0 + c)
+        const int row = rb + r, col = n0 + c;
+        if (row < M && col < M && row <= col) Zp[(long long)row * M + col] = T[r][c];
+      }
+      for (int r = threadIdx.x >> 6; r < 64; r += 4) {  // mirrored: element (n0 + r, rb + c) = sigma * (rb + c, n0 + r)
+        const int row = n0 + r, col = rb + c;
+        if (row < M && col < M && col < row) Zp[(long long)row * M + col] = sigma * T[c][r];
+      }
+      __syncthreads();
+    }
+  }
+}
+
 void launch_seg_gemm_pphh(hipStream_t s, int mode, const SegProb *probs, const int4 *tiles, int ntiles,
                           const double *X0, const double *X1, double *C, double *W, const int *act, int nact,
                           long long sop, long long sw) {
   if (ntiles <= 0 || nact <= 0) return;
   const dim3 g(ntiles, nact), b(256);
+  if (mode == 3) {  // FP64 on 128 x 64 tiles (tiles built for that shape)
+    hipLaunchKernelGGL((k_seg_gemm_w<false, 0>), g, b, 0, s, probs, tiles, ntiles, X0, X1, 1.0, 1.0, C, W, 0.0, act,
+                       sop, sop, sw);
+    return;
+  }
   if (mode == 1)
     hipLaunchKernelGGL((k_seg_gemm<false, 1, 0>), g, b, 0, s, probs, tiles, ntiles, X0, X1, 1.0, 1.0, C, W, 0.0,
                        act, sop, sop, sw);
@@ -491,6 +741,11 @@ void launch_seg_gemm_z(hipStream_t s, int mode, const SegProb *probs, const int4
   if (ntiles <= 0 || nact <= 0) return;
   const dim3 g(ntiles, nact), b(256);
   const double g0 = hL, g1 = -hR, sigma = -hL * hR;
+  if (mode == 3) {  // FP64 on 128 x 64 tiles
+    hipLaunchKernelGGL((k_seg_gemm_w<true, 1>), g, b, 0, s, probs, tiles, ntiles, LCC, RCC, g0, g1, Z, nullptr, sigma,
+                       act, scc, sz, 0LL);
+    return;
+  }
   if (mode == 1)
     hipLaunchKernelGGL((k_seg_gemm<true, 1, 1>), g, b, 0, s, probs, tiles, ntiles, LCC, RCC, g0, g1, Z, nullptr,
                        sigma, act, scc, sz, 0LL);

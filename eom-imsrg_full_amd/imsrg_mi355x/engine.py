@@ -50,6 +50,8 @@ _SIG = {
     "imsrg_ctx_set_precision": (C.c_int, [_vp, C.c_int]),
     "imsrg_ctx_synchronize": (C.c_int, [_vp]),
     "imsrg_ctx_set_graphs": (C.c_int, [_vp, C.c_int]),
+    "imsrg_ctx_tuning": (C.c_int, [_vp, _ip, _ip, C.POINTER(C.c_float)]),
+    "imsrg_ctx_set_tile_shapes": (C.c_int, [_vp, C.c_int, C.c_int]),
     "imsrg_ctx_graph_stats": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "imsrg_ctx_stream": (_vp, [_vp]),
     "imsrg_op_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
@@ -166,6 +168,18 @@ class Context:
     def set_graphs(self, enable=True):
         """replay every commutator from a cached hipGraph"""
         _chk(_lib.imsrg_ctx_set_graphs(self.h, 1 if enable else 0))
+
+    def tuning(self):
+        """FP64 tile shapes chosen by the autotuner and its four timings (ms)"""
+        a, b = C.c_int(0), C.c_int(0)
+        ms = (C.c_float * 4)()
+        _chk(_lib.imsrg_ctx_tuning(self.h, C.byref(a), C.byref(b), ms))
+        names = {-1: "untuned", 0: "64x64", 1: "128x64"}
+        return dict(pphh=names[a.value], ph=names[b.value], ms_pphh_64=ms[0], ms_pphh_128x64=ms[1],
+                    ms_ph_64=ms[2], ms_ph_128x64=ms[3])
+
+    def set_tile_shapes(self, pphh, ph):
+        _chk(_lib.imsrg_ctx_set_tile_shapes(self.h, int(pphh), int(ph)))
 
     def graph_stats(self):
         b, n = C.c_longlong(0), C.c_longlong(0)

@@ -81,7 +81,8 @@ int64_t imsrg_ctx_neq(imsrg_ctx ctx);
 /* info[0..15]: nsp, nhole, nblocks, ncc, npairs, max block dim, max CC rows,
  * device bytes per operator, vsize, ccsize, cwsize, flop_pphh, flop_ph
  * (as doubles: algorithmic GEMM flop per commutator), ph recoupling J' terms,
- * upper-triangle two-body elements, 0 */
+ * upper-triangle two-body elements, FP64 tile shapes chosen by the autotuner
+ * (4*(pp+1) + (ph+1), shapes -1 untuned, 0 64x64, 1 128x64) */
 int imsrg_ctx_info(imsrg_ctx ctx, double *info16);
 int imsrg_ctx_set_precision(imsrg_ctx ctx, int prec);
 int imsrg_ctx_synchronize(imsrg_ctx ctx);
@@ -90,6 +91,12 @@ int imsrg_ctx_synchronize(imsrg_ctx ctx);
  * once per distinct (operands, hermiticities, term mask, instance set,
  * precision) into a hipGraph and replayed.  Off while profiling is on. */
 int imsrg_ctx_set_graphs(imsrg_ctx ctx, int enable);
+/* FP64 tile shapes the autotuner chose on the context's first FP64 commutator
+ * (-1 not yet, 0 64x64, 1 128x64) and the four timings (ms: pp/hh 64x64,
+ * pp/hh 128x64, Pandya product 64x64, 128x64) */
+int imsrg_ctx_tuning(imsrg_ctx ctx, int *shape_pp, int *shape_ph, float *ms4);
+/* fix the FP64 tile shapes (0 or 1 each), or -1 to tune again on the next commutator */
+int imsrg_ctx_set_tile_shapes(imsrg_ctx ctx, int shape_pp, int shape_ph);
 int imsrg_ctx_graph_stats(imsrg_ctx ctx, long long *builds, long long *launches);
 /* the hipStream_t the context launches on (as void*) */
 void *imsrg_ctx_stream(imsrg_ctx ctx);

@@ -117,3 +117,16 @@ def test_magnus_flow(env):
     assert rel(got["e0"], want["e0"]) <= 1e-10
     assert_close(got["HS"].download(), want["HS"].vec(), what="final H(s)")
     assert_close(got["G"].download(), want["G"].vec(), what="final Omega(s)")
+
+
+@pytest.mark.parametrize("shapes", [(0, 0), (1, 1), (0, 1), (1, 0)], ids=["64/64", "128x64/128x64", "64/128x64", "128x64/64"])
+def test_fp64_tile_shapes(env, shapes):
+    """both FP64 GEMM tile shapes the autotuner chooses between give the oracle's commutator"""
+    B, ctx, ob, E = env
+    L = random_operator(B, -1, 3)
+    R = random_operator(B, 1, 4)
+    want = O.commutator(ob.op(L, -1), ob.op(R, 1)).vec()
+    ctx.set_tile_shapes(*shapes)
+    got = ctx.commutator(ctx.op(-1, L), ctx.op(1, R)).download()
+    ctx.set_tile_shapes(-1, -1)
+    assert_close(got, want, what=f"tile shapes {shapes}")
