@@ -274,8 +274,8 @@ __global__ __launch_bounds__(256) void k_adj_pandya(DevLayout L, const double *_
   const double sq = sqrt(JC + 1.0);
   for (int r = threadIdx.x; r < rlen; r += blockDim.x) {
     const double g = in[r];
-    if (g == 0.0) continue;
     const int a = L.ccrow_a[L.ccrbase[q1] + r], b = L.ccrow_b[L.ccrbase[q1] + r];
+    if (g == 0.0 || a < 0) continue;
     const int ja = L.jj[a], jb = L.jj[b];
     if (!((((L.l[a] + lh) & 1) == ((L.l[b] + lp) & 1)) && (L.tz[a] + th == L.tz[b] + tp))) continue;
     const double g0 = g * sgnp((jh + jb + JC) >> 1) * sq;

@@ -47,7 +47,7 @@ struct imsrg_ctx_s {
   double *hscal = nullptr;  // pinned host mirror of scal
   double *mono = nullptr;
   double *mbar = nullptr;   // generator-transpose scratch (nsp * nsp)
-  double *hstage = nullptr; // pinned op_size staging
+  double *hstage = nullptr; // pinned staging: max(op_size * nbatch, ccsize) doubles
   SegProb *pp_probs = nullptr, *ph_probs = nullptr;
   GemmProb *adj_pp_probs = nullptr, *adj_ph_probs = nullptr;  // adjoint products
   int4 *adj_pp_tiles = nullptr, *adj_ph_tiles = nullptr;
@@ -384,7 +384,8 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
   if (!rc && hipHostMalloc((void **)&c->hscal, kScal * B * sizeof(double), hipHostMallocDefault) != hipSuccess)
     rc = fail(IMSRG_ENOMEM, "hipHostMalloc failed");
   if (!rc &&
-      hipHostMalloc((void **)&c->hstage, Ly.op_size * B * sizeof(double), hipHostMallocDefault) != hipSuccess)
+      hipHostMalloc((void **)&c->hstage, std::max<size_t>(Ly.op_size * B, Ly.ccsize) * sizeof(double),
+                    hipHostMallocDefault) != hipSuccess)
     rc = fail(IMSRG_ENOMEM, "hipHostMalloc staging failed");
   // GEMM problem tables.  222_pp_hh per channel block (pairs ordered pp|ph|hh):
   //   RES += L_pp R_pp - R_pp L_pp - L_hh R_hh + R_hh L_hh, midpoint after the pp part.
@@ -860,7 +861,7 @@ int imsrg_commutator_220(imsrg_op L, imsrg_op R, double *out) { return scalar_te
 int64_t imsrg_cross_coupled_size(imsrg_ctx c) {
   if (!c) return -1;
   int64_t n = 0;
-  for (int q = 0; q < c->lay.ncc; q++) n += (int64_t)c->lay.ccrlen[q] * c->lay.ccnph[q];
+  for (int q = 0; q < c->lay.ncc; q++) n += (int64_t)c->lay.ccrlen_ref[q] * c->lay.ccnph[q];
   return n;
 }
 int imsrg_cross_coupled(imsrg_op op, double *host_out, int64_t n) {
@@ -870,15 +871,18 @@ int imsrg_cross_coupled(imsrg_op op, double *host_out, int64_t n) {
   if (n != imsrg_cross_coupled_size(c)) return fail(IMSRG_EINVAL, "size mismatch");
   const ActSet all = all_set(c);
   launch_pandya(c->stream, c->dl, op->d, c->cc[0], all.d, all.n());
-  int64_t k = 0;
-  for (int q = 0; q < c->lay.ncc; q++) {
-    int64_t m = (int64_t)c->lay.ccrlen[q] * c->lay.ccnph[q];
-    if (m == 0) continue;
-    HIPCHK(hipMemcpyAsync(host_out + k, c->cc[0] + c->lay.ccoff[q], m * sizeof(double),
-                          hipMemcpyDeviceToHost, c->stream));
-    k += m;
-  }
+  // device rows are paired (a,b)/(b,a); the host copy is in the reference's row order
+  const Layout &Ly = c->lay;
+  HIPCHK(hipMemcpyAsync(c->hstage, c->cc[0], sizeof(double) * Ly.ccsize, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  int64_t k = 0;
+  for (int q = 0; q < Ly.ncc; q++) {
+    const int rr = Ly.ccrlen_ref[q], rd = Ly.ccrlen[q], nb = Ly.ccnph[q];
+    const double *blk = c->hstage + Ly.ccoff[q];
+    const int *ref = Ly.ccref.data() + Ly.ccrefbase[q];
+    for (int col = 0; col < nb; col++)
+      for (int r = 0; r < rr; r++) host_out[k++] = blk[(int64_t)col * rd + ref[r]];
+  }
   return 0;
 }
 

@@ -233,6 +233,14 @@ __device__ __forceinline__ int acc_col(int tj) {
   return ((threadIdx.x >> 6) & 1) * 32 + tj * 16 + (threadIdx.x & 15);
 }
 
+// Z of one Pandya block (M x M, M even) is stored in 2x2 blocks: element
+// (R, C) at ((R/2)*(M/2) + C/2)*4 + (R%2)*2 + C%2.  Rows 2u, 2u+1 are the
+// pairs (a,b), (b,a), so the two Z elements of one ph J' term share a block.
+// Only blocks u <= v are stored (Z(R,C) = sigma Z(C,R) gives the rest).
+__device__ __forceinline__ long long zidx(int R, int C, int M) {
+  return ((long long)(R >> 1) * (M >> 1) + (C >> 1)) * 4 + (R & 1) * 2 + (C & 1);
+}
+
 // ---- split-precision segments: the same 64x64 tile and 2x2 quadrant layout;
 // products accumulate in FP32 inside one K slab and in FP64 across slabs.
 typedef float float4_t __attribute__((ext_vector_type(4)));
@@ -473,27 +481,23 @@ __global__ __launch_bounds__(256) void k_seg_gemm(const SegProb *__restrict__ pr
     }
   } else {
     run_segments<A_KMAJOR, MODE>(p, 0, p.nseg, X0, X1, g0, g1, m0, n0, S, acc, need);
-    // every segment loop ends with a barrier, so S is free for the transpose
-    double(*T)[GT + GP] = reinterpret_cast<double(*)[GT + GP]>(&S);
-    if (active) {
+    // Z in 2x2 blocks, upper blocks only: every computed element on or above
+    // the diagonal directly, and the strictly lower element of a diagonal block
+    // as sigma * its mirror
+    if (need) {
+      double *Zp = C + p.c_off;
 #pragma unroll
       for (int ti = 0; ti < 2; ti++)
 #pragma unroll
         for (int tj = 0; tj < 2; tj++)
 #pragma unroll
-          for (int v = 0; v < 4; v++) T[acc_row<MODE>(ti, v)][acc_col(tj)] = acc[ti][tj][v];
-    }
-    __syncthreads();
-    double *Zp = C + p.c_off;
-    const int c = threadIdx.x & 63;
-    for (int r = threadIdx.x >> 6; r < GT; r += 4) {
-      // tile (tm, tn); in a diagonal tile the lower part is sigma * upper^T
-      const double z = (diag && r > c) ? sigma * T[c][r] : T[r][c];
-      if (m0 + r < M && n0 + c < M) Zp[(long long)(m0 + r) * M + n0 + c] = z;
-    }
-    if (!diag) {
-      for (int r = threadIdx.x >> 6; r < GT; r += 4)
-        if (n0 + r < M && m0 + c < M) Zp[(long long)(n0 + r) * M + m0 + c] = sigma * T[c][r];
+          for (int v = 0; v < 4; v++) {
+            const int row = m0 + acc_row<MODE>(ti, v), col = n0 + acc_col(tj);
+            if (row < M && col < M && row <= col) {
+              Zp[zidx(row, col, M)] = acc[ti][tj][v];
+              if (row < col && (row >> 1) == (col >> 1)) Zp[zidx(col, row, M)] = sigma * acc[ti][tj][v];
+            }
+          }
     }
   }
 }
@@ -688,29 +692,20 @@ __global__ __launch_bounds__(256) void k_seg_gemm_w(const SegProb *__restrict__ 
     }
   } else {
     run(0, p.nseg);
-    double *Zp = C + p.c_off;
-    double(*T)[64 + GP] = reinterpret_cast<double(*)[64 + GP]>(&S);
-    const int c = threadIdx.x & 63;
-    for (int h = 0; h < 2; h++) {  // rows 64h .. 64h+63 of the tile go through LDS
-      if ((wave >> 1) == h) {
+    if (need) {  // Z in 2x2 blocks, upper blocks only (see zidx)
+      double *Zp = C + p.c_off;
 #pragma unroll
-        for (int ti = 0; ti < 2; ti++)
+      for (int ti = 0; ti < 2; ti++)
 #pragma unroll
-          for (int tj = 0; tj < 4; tj++)
+        for (int tj = 0; tj < 4; tj++)
 #pragma unroll
-            for (int v = 0; v < 4; v++) T[(wave & 1) * 32 + ti * 16 + g + 4 * v][tj * 16 + (lane & 15)] = acc[ti][tj][v];
-      }
-      __syncthreads();
-      const int rb = m0 + 64 * h;
-      for (int r = threadIdx.x >> 6; r < 64; r += 4) {  // direct: element (rb + r, n0 + c)
-        const int row = rb + r, col = n0 + c;
-        if (row < M && col < M && row <= col) Zp[(long long)row * M + col] = T[r][c];
-      }
-      for (int r = threadIdx.x >> 6; r < 64; r += 4) {  // mirrored: element (n0 + r, rb + c) = sigma * (rb + c, n0 + r)
-        const int row = n0 + r, col = rb + c;
-        if (row < M && col < M && col < row) Zp[(long long)row * M + col] = sigma * T[c][r];
-      }
-      __syncthreads();
+          for (int v = 0; v < 4; v++) {
+            const int row = m0 + wave * 32 + ti * 16 + g + 4 * v, col = n0 + tj * 16 + (lane & 15);
+            if (row < M && col < M && row <= col) {
+              Zp[zidx(row, col, M)] = acc[ti][tj][v];
+              if (row < col && (row >> 1) == (col >> 1)) Zp[zidx(col, row, M)] = sigma * acc[ti][tj][v];
+            }
+          }
     }
   }
 }
@@ -779,6 +774,10 @@ __global__ __launch_bounds__(256) void k_pandya(DevLayout L, const double *__res
   const double sq = sqrt(JC + 1.0);
   for (int r = threadIdx.x; r < rlen; r += blockDim.x) {
     const int a = L.ccrow_a[L.ccrbase[q1] + r], b = L.ccrow_b[L.ccrbase[q1] + r];
+    if (a < 0) {  // zero partner row of an (a,a) row
+      out[r] = 0.0;
+      continue;
+    }
     const int ja = L.jj[a], jb = L.jj[b];
     double val = 0.0;
     if ((((L.l[a] + lh) & 1) == ((L.l[b] + lp) & 1)) && (L.tz[a] + th == L.tz[b] + tp)) {
@@ -867,6 +866,23 @@ void launch_clear_blocks(hipStream_t s, const DevLayout &L, double *RES, const i
 // every T once (one thread per J' term, terms ordered by quadruple sorted by
 // (i,k,j,l) so that runs of threads share Z rows); phase B adds the 6j-weighted
 // sums to every upper element, one wavefront per RES row.
+// Z(R1,C1) and Z(R2,C2) where (R1,R2) and (C1,C2) are the two orders of one
+// orbit pair each, so both lie in the 2x2 block (R1/2, C1/2): one 32-byte load
+// from the stored upper block (or its mirror, times sigma)
+__device__ __forceinline__ void zpair(const double *__restrict__ Zq, int M, int R1, int C1, int R2, int C2,
+                                      double sigma, double &z1, double &z2) {
+  const int u = R1 >> 1, w = C1 >> 1;
+  if (u <= w) {
+    const double4_t b = *reinterpret_cast<const double4_t *>(Zq + ((long long)u * (M >> 1) + w) * 4);
+    z1 = b[(R1 & 1) * 2 + (C1 & 1)];
+    z2 = b[(R2 & 1) * 2 + (C2 & 1)];
+  } else {
+    const double4_t b = *reinterpret_cast<const double4_t *>(Zq + ((long long)w * (M >> 1) + u) * 4);
+    z1 = sigma * b[(C1 & 1) * 2 + (R1 & 1)];
+    z2 = sigma * b[(C2 & 1) * 2 + (R2 & 1)];
+  }
+}
+
 __global__ __launch_bounds__(256) void k_ph_terms(DevLayout L, const double *__restrict__ Z,
                                                   double *__restrict__ Tbuf, double sigma, const int *act) {
   const int chunk = xcd_item(L.xs_terms);
@@ -892,9 +908,9 @@ __global__ __launch_bounds__(256) void k_ph_terms(DevLayout L, const double *__r
     const int *rowt = L.ccrow + (size_t)qx * N * N;
     const int gik = rowt[k * N + i], rjl = rowt[j * N + l], rik = rowt[i * N + k], gjl = rowt[l * N + j];
     if ((gik | rjl | rik | gjl) >= 0) {
-      const long long rl = L.ccrlen[qx];
-      const double *Zq = Z + L.cwoff[qx];
-      v = Zq[gik * rl + rjl] + ph * sigma * Zq[rik * rl + gjl];
+      double z1, z2;
+      zpair(Z + L.cwoff[qx], L.ccrlen[qx], gik, rjl, rik, gjl, sigma, z1, z2);
+      v = z1 + ph * sigma * z2;
     }
   } else {
     const int Tz = abs(L.tz[i] - L.tz[l]) / 2, P = (L.l[i] + L.l[l]) % 2;
@@ -903,9 +919,9 @@ __global__ __launch_bounds__(256) void k_ph_terms(DevLayout L, const double *__r
     const int *rowt = L.ccrow + (size_t)qx * N * N;
     const int rjk = rowt[j * N + k], gil = rowt[l * N + i], gjk = rowt[k * N + j], ril = rowt[i * N + l];
     if ((rjk | gil | gjk | ril) >= 0) {
-      const long long rl = L.ccrlen[qx];
-      const double *Zq = Z + L.cwoff[qx];
-      v = sigma * Zq[rjk * rl + gil] + ph * Zq[gjk * rl + ril];
+      double z1, z2;
+      zpair(Z + L.cwoff[qx], L.ccrlen[qx], rjk, gil, gjk, ril, sigma, z1, z2);
+      v = sigma * z1 + ph * z2;
     }
   }
   Tbuf[t] = v;

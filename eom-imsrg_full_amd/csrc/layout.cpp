@@ -185,8 +185,8 @@ std::string Layout::build(int nsp_, const int *n_, const int *l_, const int *jj_
   ccJ.assign(ncc, 0); ccrlen.assign(ncc, 0); ccnph.assign(ncc, 0); ccTz.assign(ncc, 0); ccP.assign(ncc, 0);
   ccoff.assign(ncc, 0); cwoff.assign(ncc, 0);
   ccrow.assign((size_t)ncc * nn2, -1);
-  ccrbase.assign(ncc + 1, 0); cccbase.assign(ncc + 1, 0);
-  ccrow_a.clear(); ccrow_b.clear(); cccol_p.clear(); cccol_h.clear();
+  ccrbase.assign(ncc + 1, 0); cccbase.assign(ncc + 1, 0); ccrefbase.assign(ncc + 1, 0); ccrlen_ref.assign(ncc, 0);
+  ccrow_a.clear(); ccrow_b.clear(); cccol_p.clear(); cccol_h.clear(); ccref.clear();
   int64_t coff = 0, woff = 0;
   for (int q1 = 0; q1 < ncc; q1++) {
     int JC = (q1 % (JTM + 1)) * 2;
@@ -195,21 +195,40 @@ std::string Layout::build(int nsp_, const int *n_, const int *l_, const int *jj_
     ccJ[q1] = JC; ccTz[q1] = TZ; ccP[q1] = PAR;
     ccrbase[q1] = (int)ccrow_a.size();
     cccbase[q1] = (int)cccol_p.size();
+    ccrefbase[q1] = (int)ccref.size();
+    auto valid = [&](int i, int j) {
+      return tri(jj[i], jj[j], JC) && (l[i] + l[j]) % 2 == PAR && std::abs(tz[i] - tz[j]) / 2 == TZ;
+    };
     int r = 0, nb = 0;
+    for (int a = 0; a < nsp; a++)
+      for (int b = a; b < nsp; b++) {
+        if (!valid(a, b)) continue;
+        ccrow[((size_t)q1 * nsp + a) * nsp + b] = r;
+        ccrow_a.push_back(a);
+        ccrow_b.push_back(b);
+        if (a != b) {
+          ccrow[((size_t)q1 * nsp + b) * nsp + a] = r + 1;
+          ccrow_a.push_back(b);
+          ccrow_b.push_back(a);
+        } else {
+          ccrow_a.push_back(-1);
+          ccrow_b.push_back(-1);
+        }
+        r += 2;
+      }
+    int rref = 0;
     for (int i = 0; i < nsp; i++)
       for (int j = 0; j < nsp; j++) {
-        if (!tri(jj[i], jj[j], JC)) continue;
-        if ((l[i] + l[j]) % 2 != PAR) continue;
-        if (std::abs(tz[i] - tz[j]) / 2 != TZ) continue;
-        ccrow[((size_t)q1 * nsp + i) * nsp + j] = r++;
-        ccrow_a.push_back(i);
-        ccrow_b.push_back(j);
+        if (!valid(i, j)) continue;
+        ccref.push_back(ccrow[((size_t)q1 * nsp + i) * nsp + j]);
+        rref++;
         if (con[i] == 0 && con[j] == 1) {
           nb++;
           cccol_p.push_back(i);
           cccol_h.push_back(j);
         }
       }
+    ccrlen_ref[q1] = rref;
     ccrlen[q1] = r;
     ccnph[q1] = nb;
     ccoff[q1] = coff;
@@ -219,6 +238,7 @@ std::string Layout::build(int nsp_, const int *n_, const int *l_, const int *jj_
   }
   ccrbase[ncc] = (int)ccrow_a.size();
   cccbase[ncc] = (int)cccol_p.size();
+  ccrefbase[ncc] = (int)ccref.size();
   ccsize = coff;
   cwsize = woff;
   // quadruples of the ph recoupling kernel: every (i<=j, k<=l) whose pairs
@@ -316,7 +336,7 @@ std::string Layout::build(int nsp_, const int *n_, const int *l_, const int *jj_
   flop_pphh = 0;
   for (int qq = 0; qq < nblocks; qq++) flop_pphh += 2.0 * bdim[qq] * bdim[qq] * (bnp[qq] + bnh[qq]);
   flop_ph = 0;
-  for (int q1 = 0; q1 < ncc; q1++) flop_ph += 2.0 * ccrlen[q1] * ccrlen[q1] * ccnph[q1];
+  for (int q1 = 0; q1 < ncc; q1++) flop_ph += 2.0 * ccrlen_ref[q1] * ccrlen_ref[q1] * ccnph[q1];
   return "";
 }
 
