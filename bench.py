@@ -55,6 +55,8 @@ def parse():
     p.add_argument("--batch-holes", type=int, default=3)
     p.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                    help="print the calibrated HBM traffic table from two rocprofv3 --pmc runs and exit")
+    p.add_argument("--trace-summary", metavar="KERNEL_TRACE_CSV",
+                   help="print the per-kernel table of a rocprofv3 --kernel-trace run and exit")
     return p.parse_args()
 
 
@@ -164,6 +166,41 @@ def pmc_summary(fetch_csv, write_csv, op_bytes):
         f, w = fet[kf[0]], wri[kw[0]]
         lines.append(f"{cls} {f:.1f} {w:.1f} {1024.0 * (f * ffac + w * wfac):.0f}")
     return "\n".join(lines)
+
+
+def trace_summary(trace_csv):
+    """Markdown table of a rocprofv3 --kernel-trace CSV.  The default run also
+    holds the batched emax=6 flows (grid y = active instances) and their
+    sequential comparison; a kernel's emax=8 dispatches are the single-instance
+    ones (grid y = 1) with its largest grid x, since every size-dependent grid
+    grows with the basis.  Kernels with a size-independent grid are listed with
+    all their dispatches only."""
+    import csv
+    groups = {}
+    for r in csv.DictReader(open(trace_csv)):
+        name = r["Kernel_Name"].split("(")[0].replace("imsrg::", "").replace("void ", "")
+        key = (int(r["Grid_Size_X"]), int(r["Grid_Size_Y"]), int(r["Grid_Size_Z"]))
+        dur = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+        groups.setdefault(name, {}).setdefault(key, []).append(dur)
+    rows = []
+    for name, g in groups.items():
+        alld = [d for v in g.values() for d in v]
+        single = [k for k in g if k[1] == 1]
+        big = max(single) if single else None
+        sized = big is not None and len({k[0] for k in single}) > 1
+        d8 = g[big] if sized else []
+        rows.append((sum(alld), name, len(alld), sum(alld) / len(alld) / 1e3, len(d8),
+                     (sum(d8) / len(d8) / 1e3) if d8 else None, big[0] if sized else None))
+    rows.sort(reverse=True)
+    tot = sum(r[0] for r in rows)
+    out = ["| kernel | calls | total ms | avg us | % | emax=8 calls | emax=8 avg us | emax=8 grid x |",
+           "|---|---|---|---|---|---|---|---|"]
+    for t, name, n, avg, n8, avg8, gx in rows:
+        if t / tot < 1e-3:
+            continue
+        e8 = f"{n8} | {avg8:.1f} | {gx}" if avg8 is not None else "- | - | -"
+        out.append(f"| `{name}` | {n} | {t / 1e6:.3f} | {avg:.1f} | {100 * t / tot:.2f} | {e8} |")
+    return "\n".join(out)
 
 
 def load_traffic():
@@ -292,6 +329,9 @@ def main():
     from imsrg_mi355x import engine as E
 
     basis = Basis.ho(a.emax, a.holes)
+    if a.trace_summary:
+        print(trace_summary(a.trace_summary))
+        return
     if a.pmc_summary:
         ctx = E.Context(basis)
         print(pmc_summary(a.pmc_summary[0], a.pmc_summary[1], ctx.info()["op_bytes"]))

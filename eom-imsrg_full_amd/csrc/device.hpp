@@ -32,16 +32,22 @@ struct DevLayout {
   const long long *ccoff, *cwoff;
   const int *ccrow, *ccrbase, *ccrow_a, *ccrow_b, *cccbase, *cccol_p, *cccol_h, *cccol_q;
   const double *sixj;
+  const double *ap6_val;       // packed non-zero 6j runs of the ph apply, per row key (ji, jj, J)
+  const int *ap6_base, *ap6_off;
+  int ap6_max;
   const unsigned *ph_quads;  // (i<<24|j<<16|k<<8|l) of every upper (ij,kl), sorted by (i,k,j,l)
   long long nquad;
   const unsigned *ph_terms;  // J' terms: quad<<6 | side<<5 | m, in quadruple order
   long long nphterm;
+  const unsigned *ph_zblk;     // per J' term: 2x2 block of Z holding its two elements (units of 4 doubles)
+  const unsigned char *ph_zsel;  // per J' term: element positions, mirror, phase, side, valid
   const unsigned *ph_toff;   // per upper element (row-major per block): first T1 term of its quadruple
   const long long *ph_ubase; // per global pair row r: index of its first upper element
   const int4 *mirror_tiles;  // (block, tile row, tile col) with row <= col, 32x32 tiles
   int nmirror;
   long long f_off, v_off, op_size, vsize, ccsize, cwsize;  // per-instance strides
-  XcdSplit xs_rows, xs_terms;  // k_122 rows, k_ph_terms chunks of 256 terms
+  const int *apply_rows;      // k_ph_apply row order: pair rows sorted by (i, j, J)
+  XcdSplit xs_rows, xs_terms, xs_apply;  // k_122 rows, k_ph_terms chunks of 256 terms, apply_rows
 };
 
 // A plain GEMM problem C = A * B (A M x K, B K x N, C M x N, all row-major)

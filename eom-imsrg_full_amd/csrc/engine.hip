@@ -330,11 +330,14 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
   UP(Ly.ccrow, ccrow); UP(Ly.ccrbase, ccrbase); UP(Ly.ccrow_a, ccrow_a); UP(Ly.ccrow_b, ccrow_b);
   UP(Ly.cccbase, cccbase); UP(Ly.cccol_p, cccol_p); UP(Ly.cccol_h, cccol_h); UP(cccol_q, cccol_q);
   UP(Ly.sixj, sixj);
+  UP(Ly.ap6_val, ap6_val); UP(Ly.ap6_base, ap6_base); UP(Ly.ap6_off, ap6_off);
+  D.ap6_max = Ly.ap6_max;
   UP(Ly.ph_quads, ph_quads);
   D.nquad = (long long)Ly.ph_quads.size();
   UP(Ly.ph_terms, ph_terms);
   D.nphterm = (long long)Ly.ph_terms.size();
   UP(Ly.ph_toff, ph_toff);
+  UP(Ly.ph_zblk, ph_zblk); UP(Ly.ph_zsel, ph_zsel);
   {
     std::vector<long long> ub(Ly.ph_ubase.begin(), Ly.ph_ubase.end());
     UP(ub, ph_ubase);
@@ -345,6 +348,20 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
       wr[r] = Ly.bdim[Ly.pair_q[r]] * (double)(Ly.nb_start[r + 1] - Ly.nb_start[r] + 1);
     D.xs_rows = make_split(wr);
     D.xs_terms = make_split(wt);
+    std::vector<int> arow(Ly.npairs);
+    for (int r = 0; r < Ly.npairs; r++) arow[r] = r;
+    std::sort(arow.begin(), arow.end(), [&](int x, int y) {
+      if (Ly.pair_a[x] != Ly.pair_a[y]) return Ly.pair_a[x] < Ly.pair_a[y];
+      if (Ly.pair_b[x] != Ly.pair_b[y]) return Ly.pair_b[x] < Ly.pair_b[y];
+      return Ly.bJ[Ly.pair_q[x]] < Ly.bJ[Ly.pair_q[y]];
+    });
+    std::vector<double> wa(Ly.npairs);
+    for (int k = 0; k < Ly.npairs; k++) {
+      const int r = arow[k], q = Ly.pair_q[r];
+      wa[k] = 64.0 + (Ly.bdim[q] - (r - Ly.pbase[q]));
+    }
+    D.xs_apply = make_split(wa);
+    UP(arow, apply_rows);
   }
   std::vector<int4> mtiles;
   for (int q = 0; q < Ly.nblocks; q++) {

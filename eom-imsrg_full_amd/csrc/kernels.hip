@@ -772,6 +772,9 @@ __global__ __launch_bounds__(256) void k_pandya(DevLayout L, const double *__res
   const double *V = op + L.v_off;
   double *out = cc + L.ccoff[q1] + (long long)c * rlen;
   const double sq = sqrt(JC + 1.0);
+  const int N = L.nsp, h6 = L.nhalf, nJ = L.nJ;
+  const size_t NN = (size_t)N * N;
+  const double *six0 = L.sixj + (((size_t)(jp >> 1) * h6 + (jh >> 1)) * nJ + (JC >> 1)) * h6 * h6 * nJ;
   for (int r = threadIdx.x; r < rlen; r += blockDim.x) {
     const int a = L.ccrow_a[L.ccrbase[q1] + r], b = L.ccrow_b[L.ccrbase[q1] + r];
     if (a < 0) {  // zero partner row of an (a,a) row
@@ -783,10 +786,35 @@ __global__ __launch_bounds__(256) void k_pandya(DevLayout L, const double *__res
     if ((((L.l[a] + lh) & 1) == ((L.l[b] + lp) & 1)) && (L.tz[a] + th == L.tz[b] + tp)) {
       const int Jmin = max(abs(jp - jb), abs(ja - jh));
       const int Jmax = min(jp + jb, ja + jh);
+      // v_elem(h,a,p,b;J'): bra pair (h,a), ket pair (p,b), block (J', T2, P)
+      const int qb = 2 * ((th + L.tz[a]) / 2 + 1) + ((lh + L.l[a]) & 1);
+      const bool sw1 = h > a, sw2 = p > b;
+      const int *pr = L.pidx + (sw1 ? a * N + h : h * N + a);
+      const int *ps = L.pidx + (sw2 ? b * N + p : p * N + b);
+      const double *six = six0 + ((ja >> 1) * h6 + (jb >> 1)) * nJ;
       double sm = 0.0;
-      for (int JT = Jmin; JT <= Jmax; JT += 2)
-        sm = sm + (double)(sgnp(JT >> 1) * (JT + 1)) * dsixj(L, jp, jh, JC, ja, jb, JT) *
-                      dv(L, V, h, a, p, b, JT);
+      for (int J0 = Jmin; J0 <= Jmax; J0 += 8) {
+        double vv[4], cf[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int JT = J0 + 2 * u;
+          vv[u] = 0.0;
+          cf[u] = 0.0;
+          if (JT <= Jmax) {
+            const size_t jo = (size_t)(JT >> 1) * NN;
+            const int rr = pr[jo], ss = ps[jo], q = 3 * JT + qb;
+            double pre = 1.0;
+            if (sw1) pre = sgnp(1 + ((jh + ja - JT) >> 1));
+            else if (h == a) pre = pre * M_SQRT2;
+            if (sw2) pre = pre * sgnp(1 + ((jp + jb - JT) >> 1));
+            else if (p == b) pre = pre * M_SQRT2;
+            if ((rr | ss) >= 0) vv[u] = V[L.boff[q] + (long long)rr * L.bdim[q] + ss] * pre;
+            cf[u] = (double)(sgnp(JT >> 1) * (JT + 1)) * six[JT >> 1];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) sm = sm + cf[u] * vv[u];
+      }
       val = sm * sgnp((jh + jb + JC) >> 1) * sq;
     }
     out[r] = val;
@@ -866,23 +894,12 @@ void launch_clear_blocks(hipStream_t s, const DevLayout &L, double *RES, const i
 // every T once (one thread per J' term, terms ordered by quadruple sorted by
 // (i,k,j,l) so that runs of threads share Z rows); phase B adds the 6j-weighted
 // sums to every upper element, one wavefront per RES row.
-// Z(R1,C1) and Z(R2,C2) where (R1,R2) and (C1,C2) are the two orders of one
-// orbit pair each, so both lie in the 2x2 block (R1/2, C1/2): one 32-byte load
-// from the stored upper block (or its mirror, times sigma)
-__device__ __forceinline__ void zpair(const double *__restrict__ Zq, int M, int R1, int C1, int R2, int C2,
-                                      double sigma, double &z1, double &z2) {
-  const int u = R1 >> 1, w = C1 >> 1;
-  if (u <= w) {
-    const double4_t b = *reinterpret_cast<const double4_t *>(Zq + ((long long)u * (M >> 1) + w) * 4);
-    z1 = b[(R1 & 1) * 2 + (C1 & 1)];
-    z2 = b[(R2 & 1) * 2 + (C2 & 1)];
-  } else {
-    const double4_t b = *reinterpret_cast<const double4_t *>(Zq + ((long long)w * (M >> 1) + u) * 4);
-    z1 = sigma * b[(C1 & 1) * 2 + (R1 & 1)];
-    z2 = sigma * b[(C2 & 1) * 2 + (R2 & 1)];
-  }
-}
-
+// Z is stored in 2x2 blocks, upper blocks only: Z(R1,C1) and Z(R2,C2), where
+// (R1,R2) and (C1,C2) are the two orders of one orbit pair each, lie in one
+// block (R1/2, C1/2), or in its mirror (C1/2, R1/2) times sigma.
+// One thread per J' term: T = z1 + ph*sigma*z2 (side 0) or sigma*z1 + ph*z2
+// (side 1), both Z elements read with one 32-byte load of the block the
+// host-built address table names.
 __global__ __launch_bounds__(256) void k_ph_terms(DevLayout L, const double *__restrict__ Z,
                                                   double *__restrict__ Tbuf, double sigma, const int *act) {
   const int chunk = xcd_item(L.xs_terms);
@@ -893,47 +910,34 @@ __global__ __launch_bounds__(256) void k_ph_terms(DevLayout L, const double *__r
     Z += inst * L.cwsize;
     Tbuf += inst * L.nphterm;
   }
-  const unsigned tc = L.ph_terms[t];
-  const unsigned code = L.ph_quads[tc >> 6];
-  const int side = (tc >> 5) & 1, m = tc & 31;
-  const int i = code >> 24, j = (code >> 16) & 255, k = (code >> 8) & 255, l = code & 255;
-  const int N = L.nsp, JTM1 = L.jtot_max + 1;
-  const int ji = L.jj[i], jj = L.jj[j], jk = L.jj[k], jl = L.jj[l];
-  const double ph = sgnp((ji + jj + jk + jl) >> 1);
+  const unsigned sel = L.ph_zsel[t];
   double v = 0.0;
-  if (side == 0) {
-    const int Tz = abs(L.tz[i] - L.tz[k]) / 2, P = (L.l[i] + L.l[k]) % 2;
-    const int JP = max(abs(jj - jl), abs(ji - jk)) + 2 * m;
-    const int qx = (JP >> 1) + Tz * JTM1 + 2 * P * JTM1;
-    const int *rowt = L.ccrow + (size_t)qx * N * N;
-    const int gik = rowt[k * N + i], rjl = rowt[j * N + l], rik = rowt[i * N + k], gjl = rowt[l * N + j];
-    if ((gik | rjl | rik | gjl) >= 0) {
-      double z1, z2;
-      zpair(Z + L.cwoff[qx], L.ccrlen[qx], gik, rjl, rik, gjl, sigma, z1, z2);
-      v = z1 + ph * sigma * z2;
+  if (sel & 128) {
+    const double4_t b = reinterpret_cast<const double4_t *>(Z)[L.ph_zblk[t]];
+    double z1 = b[sel & 3], z2 = b[(sel >> 2) & 3];
+    if (sel & 16) {
+      z1 = sigma * z1;
+      z2 = sigma * z2;
     }
-  } else {
-    const int Tz = abs(L.tz[i] - L.tz[l]) / 2, P = (L.l[i] + L.l[l]) % 2;
-    const int JP = max(abs(ji - jl), abs(jj - jk)) + 2 * m;
-    const int qx = (JP >> 1) + Tz * JTM1 + 2 * P * JTM1;
-    const int *rowt = L.ccrow + (size_t)qx * N * N;
-    const int rjk = rowt[j * N + k], gil = rowt[l * N + i], gjk = rowt[k * N + j], ril = rowt[i * N + l];
-    if ((rjk | gil | gjk | ril) >= 0) {
-      double z1, z2;
-      zpair(Z + L.cwoff[qx], L.ccrlen[qx], rjk, gil, gjk, ril, sigma, z1, z2);
-      v = sigma * z1 + ph * z2;
-    }
+    const double ph = (sel & 32) ? -1.0 : 1.0;
+    v = (sel & 64) ? sigma * z1 + ph * z2 : z1 + ph * sigma * z2;
   }
   Tbuf[t] = v;
 }
 
-// The 6j symbols of one row, {ji jj J; j4 j5 J'} for all (j4, j5, J'), are one
-// contiguous slice of the table; it is staged in LDS once per row.
+// The 6j symbols a row can use, {ji jj J; x y J'} with triangle (x, y, J) and
+// J' in the triangle ranges of the two J' sums, are one packed list per
+// (ji, jj, J), staged in LDS once per row (zero-padded to the longest list).
+// Rows run in (i, j, J) order, split into 8 contiguous ranges (one per XCD):
+// the rows of one orbit pair read the same quadruples' J' terms, so the
+// repeats hit L2.  Each output's two J' ranges are contiguous in Tbuf and are
+// loaded 8 at a time, so a thread keeps several loads in flight.
 __global__ __launch_bounds__(64) void k_ph_apply(DevLayout L, const double *__restrict__ Tbuf,
                                                  double *__restrict__ RES, const int *act) {
-  extern __shared__ double S6[];  // nhalf * nhalf * nJ
-  const int r = blockIdx.x;
-  if (r >= L.npairs) return;
+  extern __shared__ double S6[];  // ap6_max doubles, then nhalf * nhalf run offsets
+  const int it = xcd_item(L.xs_apply);
+  if (it < 0) return;
+  const int r = L.apply_rows[it];
   {
     const long long inst = inst_id(act);
     Tbuf += inst * L.nphterm;
@@ -942,10 +946,13 @@ __global__ __launch_bounds__(64) void k_ph_apply(DevLayout L, const double *__re
   const int q = L.pair_q[r], base = L.pbase[q], rl = r - base, dim = L.bdim[q], J = L.bJ[q];
   const int i = L.pair_a[r], j = L.pair_b[r];
   const int ji = L.jj[i], jj = L.jj[j], li = L.l[i], lj = L.l[j], ti = L.tz[i], tj = L.tz[j];
-  const int h = L.nhalf, nJ = L.nJ, nsl = h * h * nJ;
+  const int h = L.nhalf, hh = h * h;
+  int *OFF = reinterpret_cast<int *>(S6 + L.ap6_max);
   {
-    const double *src = L.sixj + ((((size_t)(ji >> 1) * h + (jj >> 1)) * nJ + (J >> 1)) * h * h) * nJ;
-    for (int e = threadIdx.x; e < nsl; e += 64) S6[e] = src[e];
+    const int key = ((ji >> 1) * h + (jj >> 1)) * L.nJ + (J >> 1);
+    const int b0 = L.ap6_base[key], len = L.ap6_base[key + 1] - b0;
+    for (int e = threadIdx.x; e < L.ap6_max; e += 64) S6[e] = e < len ? L.ap6_val[b0 + e] : 0.0;
+    for (int e = threadIdx.x; e < hh; e += 64) OFF[e] = L.ap6_off[(size_t)key * hh + e];
   }
   __syncthreads();
   const double pre = (i == j) ? .70710678118 : 1.0;
@@ -954,22 +961,30 @@ __global__ __launch_bounds__(64) void k_ph_apply(DevLayout L, const double *__re
   for (int s = rl + (int)threadIdx.x; s < dim; s += 64) {
     const int k = L.pair_a[base + s], l = L.pair_b[base + s];
     const int jk = L.jj[k], jl = L.jj[l], lk = L.l[k], ll = L.l[l], tk = L.tz[k], tl = L.tz[l];
-    const double *T1 = Tbuf + toff[s];
+    const double *T = Tbuf + toff[s];
     const int a1 = max(abs(jj - jl), abs(ji - jk)), b1 = min(jj + jl, ji + jk);
     const int a2 = max(abs(ji - jl), abs(jj - jk)), b2 = min(ji + jl, jj + jk);
-    const double *T2 = T1 + (b1 >= a1 ? (b1 - a1) / 2 + 1 : 0);
-    double s1 = 0.0, s2 = 0.0;
-    if (abs(tl - tj) == abs(ti - tk) && (ll + lj) % 2 == (li + lk) % 2) {
-      const double *c1 = S6 + ((jl >> 1) * h + (jk >> 1)) * nJ;
-      for (int JP = a1, m = 0; JP <= b1; JP += 2, m++) s1 += c1[JP >> 1] * T1[m];
+    const int n1 = b1 >= a1 ? (b1 - a1) / 2 + 1 : 0, n2 = b2 >= a2 ? (b2 - a2) / 2 + 1 : 0, n = n1 + n2;
+    // term m < n1: -(-1)^((jl+jk+J)/2) {ji jj J; jl jk J'} T1; else {ji jj J; jk jl J'} T2
+    int o1 = OFF[(jl >> 1) * h + (jk >> 1)], o2 = OFF[(jk >> 1) * h + (jl >> 1)];
+    const bool v1 = o1 >= 0 && abs(tl - tj) == abs(ti - tk) && (ll + lj) % 2 == (li + lk) % 2;
+    const bool v2 = o2 >= 0 && abs(tk - tj) == abs(ti - tl) && (lk + lj) % 2 == (li + ll) % 2;
+    const double g1 = v1 ? -(double)sgnp((jl + jk + J) >> 1) : 0.0, g2 = v2 ? 1.0 : 0.0;
+    o1 = v1 ? o1 : 0;
+    o2 = (v2 ? o2 : 0) - n1;
+    double acc = 0.0;
+    for (int m0 = 0; m0 < n; m0 += 8) {
+      double t[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) t[u] = (m0 + u < n) ? T[m0 + u] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int m = m0 + u;
+        if (m < n) acc += (m < n1 ? g1 * S6[o1 + m] : g2 * S6[o2 + m]) * t[u];
+      }
     }
-    if (abs(tk - tj) == abs(ti - tl) && (lk + lj) % 2 == (li + ll) % 2) {
-      const double *c2 = S6 + ((jk >> 1) * h + (jl >> 1)) * nJ;
-      for (int JP = a2, m = 0; JP <= b2; JP += 2, m++) s2 += c2[JP >> 1] * T2[m];
-    }
-    const double out = -(double)sgnp((jl + jk + J) >> 1) * s1 + s2;
     const double pre2 = (k == l) ? .70710678118 : 1.0;
-    OV[s] = OV[s] + out * pre * pre2;
+    OV[s] = OV[s] + acc * pre * pre2;
   }
 }
 void launch_ph_terms(hipStream_t s, const DevLayout &L, const double *Z, double *Tbuf, double hL, double hR,
@@ -980,8 +995,8 @@ void launch_ph_terms(hipStream_t s, const DevLayout &L, const double *Z, double 
 void launch_ph_apply(hipStream_t s, const DevLayout &L, const double *Tbuf, double *RES, const int *act,
                      int nact) {
   if (L.nphterm <= 0 || nact <= 0) return;
-  const size_t lds = sizeof(double) * L.nhalf * L.nhalf * L.nJ;
-  hipLaunchKernelGGL(k_ph_apply, dim3(L.npairs, nact), dim3(64), lds, s, L, Tbuf, RES, act);
+  const size_t lds = sizeof(double) * L.ap6_max + sizeof(int) * L.nhalf * L.nhalf;
+  hipLaunchKernelGGL(k_ph_apply, dim3(8 * L.xs_apply.len, nact), dim3(64), lds, s, L, Tbuf, RES, act);
 }
 
 // ------------------------------------------------------------------ symmetrise + mirror

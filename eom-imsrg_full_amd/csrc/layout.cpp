@@ -304,6 +304,51 @@ std::string Layout::build(int nsp_, const int *n_, const int *l_, const int *jj_
       }
     }
     ph_ubase[npairs] = (int64_t)ph_toff.size();
+    // Z address of every J' term: the 2x2 block of Z (index in units of 4
+    // doubles from the Z base) holding both Z elements the term combines, and
+    // a selector: bits 0-1 / 2-3 their positions in the stored block, 4 the
+    // block is the mirror (x sigma), 5 the phase (-1)^((ji+jj+jk+jl)/2) is
+    // -1, 6 side (T2), 7 valid (all four Pandya rows exist)
+    ph_zblk.assign(ph_terms.size(), 0);
+    ph_zsel.assign(ph_terms.size(), 0);
+    const int JT1 = JTM + 1;
+    for (size_t t = 0; t < ph_terms.size(); t++) {
+      const uint32_t tc = ph_terms[t], c = ph_quads[tc >> 6];
+      const int side = (tc >> 5) & 1, m = tc & 31;
+      const int oi = c >> 24, oj = (c >> 16) & 255, ok = (c >> 8) & 255, ol = c & 255;
+      int qx, R1, C1, R2, C2;
+      auto row = [&](int q, int a, int b) { return ccrow[((size_t)q * nsp + a) * nsp + b]; };
+      if (side == 0) {
+        const int TZ = std::abs(tz[oi] - tz[ok]) / 2, PAR = (l[oi] + l[ok]) % 2;
+        const int JP = std::max(std::abs(jj[oj] - jj[ol]), std::abs(jj[oi] - jj[ok])) + 2 * m;
+        qx = (JP >> 1) + TZ * JT1 + 2 * PAR * JT1;
+        R1 = row(qx, ok, oi); C1 = row(qx, oj, ol); R2 = row(qx, oi, ok); C2 = row(qx, ol, oj);
+      } else {
+        const int TZ = std::abs(tz[oi] - tz[ol]) / 2, PAR = (l[oi] + l[ol]) % 2;
+        const int JP = std::max(std::abs(jj[oi] - jj[ol]), std::abs(jj[oj] - jj[ok])) + 2 * m;
+        qx = (JP >> 1) + TZ * JT1 + 2 * PAR * JT1;
+        R1 = row(qx, oj, ok); C1 = row(qx, ol, oi); R2 = row(qx, ok, oj); C2 = row(qx, oi, ol);
+      }
+      uint8_t sel = (uint8_t)(side << 6);
+      if (((jj[oi] + jj[oj] + jj[ok] + jj[ol]) >> 1) & 1) sel |= 32;
+      if ((R1 | C1 | R2 | C2) >= 0) {
+        const int u = R1 >> 1, w = C1 >> 1, h2 = ccrlen[qx] >> 1;
+        int e1, e2;
+        if (u <= w) {
+          e1 = (R1 & 1) * 2 + (C1 & 1);
+          e2 = (R2 & 1) * 2 + (C2 & 1);
+        } else {
+          e1 = (C1 & 1) * 2 + (R1 & 1);
+          e2 = (C2 & 1) * 2 + (R2 & 1);
+          sel |= 16;
+        }
+        const int64_t blk = cwoff[qx] / 4 + (int64_t)std::min(u, w) * h2 + std::max(u, w);
+        if (blk > 0xFFFFFFFFll) return "Z too large for 32-bit block indices";
+        ph_zblk[t] = (uint32_t)blk;
+        sel |= (uint8_t)(128 | e1 | (e2 << 2));
+      }
+      ph_zsel[t] = sel;
+    }
   }
   // 6j table
   nhalf = (JTM + 1) / 2;
@@ -320,6 +365,38 @@ std::string Layout::build(int nsp_, const int *n_, const int *l_, const int *jj_
                 sixj[((((i1 * h + i2) * J + a3) * h + i4) * h + i5) * J + a6] =
                     racah6j((int)(2 * i1 + 1), (int)(2 * i2 + 1), (int)(2 * a3), (int)(2 * i4 + 1),
                             (int)(2 * i5 + 1), (int)(2 * a6));
+  }
+  // Packed 6j rows of the ph recoupling apply: for a row key (ji, jj, J) the
+  // symbols {ji jj J; x y J'} that can be non-zero, i.e. triangle (x, y, J)
+  // and J' from max(|jj-x|, |ji-y|) to min(jj+x, ji+y); ap6_off[key*h*h +
+  // (x/2)*h + y/2] is the position of the (x, y) run inside the key's list, -1
+  // when the run is empty.
+  {
+    const int h = nhalf, nk = nhalf * nhalf * nJ;
+    ap6_base.assign(nk + 1, 0);
+    ap6_off.assign((size_t)nk * h * h, -1);
+    ap6_val.clear();
+    ap6_max = 0;
+    for (int a = 0; a < h; a++)
+      for (int b = 0; b < h; b++)
+        for (int c = 0; c < nJ; c++) {
+          const int key = (a * h + b) * nJ + c, ji = 2 * a + 1, jjv = 2 * b + 1, J = 2 * c;
+          ap6_base[key] = (int)ap6_val.size();
+          if (tri(ji, jjv, J)) {
+            for (int x = 0; x < h; x++)
+              for (int y = 0; y < h; y++) {
+                const int X = 2 * x + 1, Y = 2 * y + 1;
+                const int lo = std::max(std::abs(jjv - X), std::abs(ji - Y)), hi = std::min(jjv + X, ji + Y);
+                if (!tri(X, Y, J) || hi < lo) continue;
+                ap6_off[(size_t)key * h * h + x * h + y] = (int)ap6_val.size() - ap6_base[key];
+                for (int JP = lo; JP <= hi; JP += 2)
+                  ap6_val.push_back(sixj[(((((size_t)a * h + b) * nJ + c) * h + x) * h + y) * nJ + (JP >> 1)]);
+              }
+          }
+          ap6_max = std::max(ap6_max, (int)ap6_val.size() - ap6_base[key]);
+        }
+    ap6_base[nk] = (int)ap6_val.size();
+    ap6_max = std::max(ap6_max, 2 * nJ);  // room for an empty run read from position 0
   }
   // neq: the length of the vectorize stream (IMSRG_ODE.f90:36-44)
   {

@@ -72,12 +72,17 @@ struct Layout {
   std::vector<uint32_t> ph_terms;   // quad<<6 | side<<5 | m (side 0: T1, 1: T2)
   std::vector<uint32_t> ph_toff;    // per upper element: offset of its quadruple's T1 terms
   std::vector<int64_t> ph_ubase;    // per global pair row: first upper element
+  std::vector<uint32_t> ph_zblk;    // per J' term: 2x2 Z block (units of 4 doubles)
+  std::vector<uint8_t> ph_zsel;     // per J' term: element selector / flags
 
   // dense 6j table {j1 j2 J3; j4 j5 J6}: j1,j2,j4,j5 half-integer, J3,J6 integer,
   // index order [j1][j2][J3][j4][j5][J6] so that the (j4,j5,J6) a wavefront
   // varies over sit in one small contiguous slab
   int nhalf = 0, nJ = 0;
   std::vector<double> sixj;
+  std::vector<double> ap6_val;      // packed non-zero 6j runs per ph-apply row key (ji, jj, J)
+  std::vector<int> ap6_base, ap6_off;
+  int ap6_max = 0;
 
   int64_t neq = 0;              // length of the reference vectorize stream
   int64_t f_off = 32;           // F offset inside the operator buffer
