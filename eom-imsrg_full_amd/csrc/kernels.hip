@@ -94,6 +94,11 @@ static_assert(sizeof(GemmSmem) >= sizeof(double) * GT * (GT + GP), "transpose bu
 
 // `need` bit (2*ti + tj): the 16x16 sub-tile holds outputs that are used
 // (inside the block and not strictly below the diagonal); others do no MFMA.
+// `need` is wave-uniform (an SGPR).  Off-diagonal tiles inside their block
+// run the FULL instantiation (straight-line MFMAs); the kernel picks it once
+// per workgroup, so neither path carries the other's registers through the
+// K loop.
+template <bool FULL>
 __device__ __forceinline__ void slab_mfma(const GemmSmem &S, int buf, double4_t acc[2][2], unsigned need) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -104,10 +109,10 @@ __device__ __forceinline__ void slab_mfma(const GemmSmem &S, int buf, double4_t 
     const double a1 = S.A[buf][k][wm * 32 + 16 + (lane & 15)];
     const double b0 = S.B[buf][k][wn * 32 + (lane & 15)];
     const double b1 = S.B[buf][k][wn * 32 + 16 + (lane & 15)];
-    if (need & 1) acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-    if (need & 2) acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-    if (need & 4) acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-    if (need & 8) acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    if (FULL || (need & 1)) acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    if (FULL || (need & 2)) acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    if (FULL || (need & 4)) acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    if (FULL || (need & 8)) acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
   }
 }
 
@@ -118,7 +123,7 @@ __device__ __forceinline__ void slab_mfma(const GemmSmem &S, int buf, double4_t 
 // the segment's first element) and the in-range flags are applied, with the
 // sign, only when the slab is written to LDS after the current slab's MFMAs,
 // so the next slab's loads stay in flight meanwhile.
-template <bool A_KMAJOR>
+template <bool A_KMAJOR, bool FULL>
 __device__ void seg_f64(const SegDesc &sgd, double sgn, int M, const double *__restrict__ A,
                         const double *__restrict__ B, int m0, int n0, GemmSmem &S, double4_t acc[2][2],
                         unsigned need) {
@@ -162,7 +167,20 @@ __device__ void seg_f64(const SegDesc &sgd, double sgn, int M, const double *__r
   // multiplied, so each load has two slabs of MFMA work to arrive in
   double ra0[GJ], rb0[GJ], ra1[GJ], rb1[GJ];
   unsigned ok0 = 0, ok1 = 0;
+  // FULL tiles have every row and column in range, so a slab that lies
+  // wholly inside K (all but a partial last one) is loaded and stored
+  // without checks; the test is on scalars, one branch per slab.
   auto load = [&](int k0, double (&ra)[GJ], double (&rb)[GJ], unsigned &ok) {
+    if (FULL && k0 + GK <= K) {
+#pragma unroll
+      for (int j = 0; j < GJ; j++) {
+        ra[j] = A[aoff[j]];
+        rb[j] = B[boff[j]];
+        aoff[j] += ainc;
+        boff[j] += binc;
+      }
+      return;
+    }
     ok = 0;
 #pragma unroll
     for (int j = 0; j < GJ; j++) {
@@ -175,7 +193,16 @@ __device__ void seg_f64(const SegDesc &sgd, double sgn, int M, const double *__r
       boff[j] += binc;
     }
   };
-  auto store = [&](int buf, const double (&ra)[GJ], const double (&rb)[GJ], unsigned ok) {
+  auto store = [&](int buf, const double (&ra)[GJ], const double (&rb)[GJ], unsigned ok, int k0) {
+    if (FULL && k0 + GK <= K) {
+#pragma unroll
+      for (int j = 0; j < GJ; j++) {
+        if (A_KMAJOR) S.A[buf][(tid >> 6) + 4 * j][tid & 63] = ra[j];
+        else S.A[buf][tid % GK][tid / GK + (256 / GK) * j] = ra[j];
+        S.B[buf][(tid >> 6) + 4 * j][tid & 63] = sgn * rb[j];
+      }
+      return;
+    }
 #pragma unroll
     for (int j = 0; j < GJ; j++) {
       const double a = ((ok >> j) & 1) ? ra[j] : 0.0;
@@ -187,20 +214,20 @@ __device__ void seg_f64(const SegDesc &sgd, double sgn, int M, const double *__r
   };
 #if IMSRG_GEMM_PF == 2
   load(0, ra0, rb0, ok0);
-  store(0, ra0, rb0, ok0);
+  store(0, ra0, rb0, ok0, 0);
   if (nk > 1) load(GK, ra1, rb1, ok1);
   __syncthreads();
   for (int kt = 0; kt < nk; kt += 2) {
     // slab kt in LDS 0, slab kt+1 in registers 1, registers 0 free
     if (kt + 2 < nk) load((kt + 2) * GK, ra0, rb0, ok0);
-    if (need) slab_mfma(S, 0, acc, need);
-    if (kt + 1 < nk) store(1, ra1, rb1, ok1);
+    if (need) slab_mfma<FULL>(S, 0, acc, need);
+    if (kt + 1 < nk) store(1, ra1, rb1, ok1, (kt + 1) * GK);
     __syncthreads();
     if (kt + 1 >= nk) break;
     // slab kt+1 in LDS 1, slab kt+2 in registers 0, registers 1 free
     if (kt + 3 < nk) load((kt + 3) * GK, ra1, rb1, ok1);
-    if (need) slab_mfma(S, 1, acc, need);
-    if (kt + 2 < nk) store(0, ra0, rb0, ok0);
+    if (need) slab_mfma<FULL>(S, 1, acc, need);
+    if (kt + 2 < nk) store(0, ra0, rb0, ok0, (kt + 2) * GK);
     __syncthreads();
   }
 #else
@@ -208,14 +235,14 @@ __device__ void seg_f64(const SegDesc &sgd, double sgn, int M, const double *__r
   (void)rb1;
   (void)ok1;
   load(0, ra0, rb0, ok0);
-  store(0, ra0, rb0, ok0);
+  store(0, ra0, rb0, ok0, 0);
   __syncthreads();
   for (int kt = 0; kt < nk; kt++) {
     const int cur = kt & 1;
     const bool more = kt + 1 < nk;
     if (more) load((kt + 1) * GK, ra0, rb0, ok0);
-    if (need) slab_mfma(S, cur, acc, need);
-    if (more) store(cur ^ 1, ra0, rb0, ok0);
+    if (need) slab_mfma<FULL>(S, cur, acc, need);
+    if (more) store(cur ^ 1, ra0, rb0, ok0, (kt + 1) * GK);
     __syncthreads();
   }
 #endif
@@ -392,7 +419,7 @@ __device__ void seg_bf16(const SegDesc &sg, double sgn, int M, const double *__r
   }
 }
 
-template <bool A_KMAJOR, int MODE>
+template <bool A_KMAJOR, int MODE, bool FULL>
 __device__ __forceinline__ void run_segments(const SegProb &p, int s0, int s1, const double *X0,
                                              const double *X1, double g0, double g1, int m0, int n0,
                                              GemmSmem &S, double4_t acc[2][2], unsigned need) {
@@ -401,7 +428,7 @@ __device__ __forceinline__ void run_segments(const SegProb &p, int s0, int s1, c
     const double *A = (sg.src ? X1 : X0) + sg.a_off;
     const double *B = (sg.src ? X0 : X1) + sg.b_off;
     const double sgn = sg.sign * (sg.src ? g1 : g0);
-    if constexpr (MODE == 0) seg_f64<A_KMAJOR>(sg, sgn, p.M, A, B, m0, n0, S, acc, need);
+    if constexpr (MODE == 0) seg_f64<A_KMAJOR, FULL>(sg, sgn, p.M, A, B, m0, n0, S, acc, need);
     else if constexpr (MODE == 1) seg_s32<A_KMAJOR>(sg, sgn, p.M, A, B, m0, n0, acc, need);
     else seg_bf16<A_KMAJOR>(sg, sgn, p.M, A, B, m0, n0, acc, need);
   }
@@ -411,8 +438,72 @@ __device__ __forceinline__ void run_segments(const SegProb &p, int s0, int s1, c
 //   after the first nmid segments is also stored to W (upper triangle).
 // EPI 1 (Pandya product): Z = acc on the whole tile and sigma * acc^T on the
 //   mirrored tile, both written row-contiguously through an LDS transpose.
+template <bool A_KMAJOR, int MODE, int EPI, bool FULL>
+__device__ __forceinline__ void seg_tile(const SegProb &p, int m0, int n0, const double *X0, const double *X1,
+                                         double g0, double g1, double *C, double *W, double sigma,
+                                         unsigned need, GemmSmem &S) {
+  const int M = p.M;
+  const bool active = need != 0;
+  double4_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  if (EPI == 0) {
+    run_segments<A_KMAJOR, MODE, FULL>(p, 0, p.nmid, X0, X1, g0, g1, m0, n0, S, acc, need);
+    if (W && active) {
+      double *Wp = W + p.w_off;
+#pragma unroll
+      for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+        for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int row = m0 + acc_row<MODE>(ti, v), col = n0 + acc_col(tj);
+            if (row < M && col < M && row <= col) Wp[(long long)row * M + col] = acc[ti][tj][v];
+          }
+    }
+    run_segments<A_KMAJOR, MODE, FULL>(p, p.nmid, p.nseg, X0, X1, g0, g1, m0, n0, S, acc, need);
+    if (C && active) {
+      double *Cp = C + p.c_off;
+#pragma unroll
+      for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+        for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int row = m0 + acc_row<MODE>(ti, v), col = n0 + acc_col(tj);
+            if (row < M && col < M && row <= col) {
+              const long long o = (long long)row * M + col;
+              Cp[o] = Cp[o] + acc[ti][tj][v];
+            }
+          }
+    }
+  } else {
+    run_segments<A_KMAJOR, MODE, FULL>(p, 0, p.nseg, X0, X1, g0, g1, m0, n0, S, acc, need);
+    // Z in 2x2 blocks, upper blocks only: every computed element on or above
+    // the diagonal directly, and the strictly lower element of a diagonal block
+    // as sigma * its mirror
+    if (need) {
+      double *Zp = C + p.c_off;
+#pragma unroll
+      for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+        for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int row = m0 + acc_row<MODE>(ti, v), col = n0 + acc_col(tj);
+            if (row < M && col < M && row <= col) {
+              Zp[zidx(row, col, M)] = acc[ti][tj][v];
+              if (row < col && (row >> 1) == (col >> 1)) Zp[zidx(col, row, M)] = sigma * acc[ti][tj][v];
+            }
+          }
+    }
+  }
+}
+
 template <bool A_KMAJOR, int MODE, int EPI>
-__global__ __launch_bounds__(256) void k_seg_gemm(const SegProb *__restrict__ probs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == 0 ? 4 : 1))) void k_seg_gemm(const SegProb *__restrict__ probs,
                                                   const int4 *__restrict__ tiles, int ntiles,
                                                   const double *__restrict__ X0,
                                                   const double *__restrict__ X1, double g0, double g1,
@@ -442,64 +533,13 @@ __global__ __launch_bounds__(256) void k_seg_gemm(const SegProb *__restrict__ pr
       const int r16 = m0 + (wave >> 1) * 32 + ti * 16, c16 = n0 + (wave & 1) * 32 + tj * 16;
       if (r16 < M && c16 < M && !(diag && r16 > c16)) need |= 1u << (2 * ti + tj);
     }
-  const bool active = need != 0;
+  need = __builtin_amdgcn_readfirstlane(need);  // uniform per wavefront
   __shared__ GemmSmem S;
-  double4_t acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
-  if (EPI == 0) {
-    run_segments<A_KMAJOR, MODE>(p, 0, p.nmid, X0, X1, g0, g1, m0, n0, S, acc, need);
-    if (W && active) {
-      double *Wp = W + p.w_off;
-#pragma unroll
-      for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-        for (int tj = 0; tj < 2; tj++)
-#pragma unroll
-          for (int v = 0; v < 4; v++) {
-            const int row = m0 + acc_row<MODE>(ti, v), col = n0 + acc_col(tj);
-            if (row < M && col < M && row <= col) Wp[(long long)row * M + col] = acc[ti][tj][v];
-          }
-    }
-    run_segments<A_KMAJOR, MODE>(p, p.nmid, p.nseg, X0, X1, g0, g1, m0, n0, S, acc, need);
-    if (C && active) {
-      double *Cp = C + p.c_off;
-#pragma unroll
-      for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-        for (int tj = 0; tj < 2; tj++)
-#pragma unroll
-          for (int v = 0; v < 4; v++) {
-            const int row = m0 + acc_row<MODE>(ti, v), col = n0 + acc_col(tj);
-            if (row < M && col < M && row <= col) {
-              const long long o = (long long)row * M + col;
-              Cp[o] = Cp[o] + acc[ti][tj][v];
-            }
-          }
-    }
-  } else {
-    run_segments<A_KMAJOR, MODE>(p, 0, p.nseg, X0, X1, g0, g1, m0, n0, S, acc, need);
-    // Z in 2x2 blocks, upper blocks only: every computed element on or above
-    // the diagonal directly, and the strictly lower element of a diagonal block
-    // as sigma * its mirror
-    if (need) {
-      double *Zp = C + p.c_off;
-#pragma unroll
-      for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-        for (int tj = 0; tj < 2; tj++)
-#pragma unroll
-          for (int v = 0; v < 4; v++) {
-            const int row = m0 + acc_row<MODE>(ti, v), col = n0 + acc_col(tj);
-            if (row < M && col < M && row <= col) {
-              Zp[zidx(row, col, M)] = acc[ti][tj][v];
-              if (row < col && (row >> 1) == (col >> 1)) Zp[zidx(col, row, M)] = sigma * acc[ti][tj][v];
-            }
-          }
-    }
-  }
+  // workgroup-uniform choice: an off-diagonal tile inside the block needs every sub-tile
+  if (MODE == 0 && !diag && m0 + GT <= M && n0 + GT <= M)
+    seg_tile<A_KMAJOR, MODE, EPI, true>(p, m0, n0, X0, X1, g0, g1, C, W, sigma, need, S);
+  else
+    seg_tile<A_KMAJOR, MODE, EPI, false>(p, m0, n0, X0, X1, g0, g1, C, W, sigma, need, S);
 }
 
 // ---- FP64 path on 128 x 64 tiles: 4 wavefronts, each 32 rows x 64 columns
@@ -516,6 +556,7 @@ struct GemmSmemW {
 };
 static_assert(sizeof(GemmSmemW) >= sizeof(double) * 64 * (64 + GP), "transpose buffer must fit");
 
+template <bool FULL>
 __device__ __forceinline__ void slab_mfma_w(const GemmSmemW &S, int buf, double4_t acc[2][4], unsigned need) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -530,12 +571,12 @@ __device__ __forceinline__ void slab_mfma_w(const GemmSmemW &S, int buf, double4
     for (int ti = 0; ti < 2; ti++)
 #pragma unroll
       for (int tj = 0; tj < 4; tj++)
-        if (need & (1u << (4 * ti + tj)))
+        if (FULL || (need & (1u << (4 * ti + tj))))
           acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
   }
 }
 
-template <bool A_KMAJOR>
+template <bool A_KMAJOR, bool FULL>
 __device__ void seg_f64_w(const SegDesc &sgd, double sgn, int M, const double *__restrict__ A,
                           const double *__restrict__ B, int m0, int n0, GemmSmemW &S, double4_t acc[2][4],
                           unsigned need) {
@@ -612,41 +653,18 @@ __device__ void seg_f64_w(const SegDesc &sgd, double sgn, int M, const double *_
     const int cur = kt & 1;
     const bool more = kt + 1 < nk;
     if (more) load((kt + 1) * GK);
-    if (need) slab_mfma_w(S, cur, acc, need);
+    if (need) slab_mfma_w<FULL>(S, cur, acc, need);
     if (more) store(cur ^ 1);
     __syncthreads();
   }
 }
 
-template <bool A_KMAJOR, int EPI>
-__global__ __launch_bounds__(256) void k_seg_gemm_w(const SegProb *__restrict__ probs, const int4 *__restrict__ tiles,
-                                                    int ntiles, const double *__restrict__ X0,
-                                                    const double *__restrict__ X1, double g0, double g1,
-                                                    double *__restrict__ C, double *__restrict__ W, double sigma,
-                                                    const int *act, long long sx, long long sc, long long sw) {
-  const int t = blockIdx.x;
-  if (t >= ntiles) return;
-  {
-    const long long inst = inst_id(act);
-    X0 += inst * sx;
-    X1 += inst * sx;
-    if (C) C += inst * sc;
-    if (W) W += inst * sw;
-  }
-  const int4 tl = tiles[t];
-  if (tl.x < 0) return;
-  const SegProb &p = probs[tl.x];
-  const int m0 = tl.y * WM, n0 = tl.z * WN, M = p.M;
+template <bool A_KMAJOR, int EPI, bool FULL>
+__device__ __forceinline__ void seg_tile_w(const SegProb &p, int m0, int n0, const double *X0, const double *X1,
+                                           double g0, double g1, double *C, double *W, double sigma,
+                                           unsigned need, GemmSmemW &S) {
+  const int M = p.M;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
-  unsigned need = 0;
-#pragma unroll
-  for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-    for (int tj = 0; tj < 4; tj++) {
-      const int r16 = m0 + wave * 32 + ti * 16, c16 = n0 + tj * 16;
-      if (r16 < M && c16 < M && r16 <= c16) need |= 1u << (4 * ti + tj);
-    }
-  __shared__ GemmSmemW S;
   double4_t acc[2][4];
 #pragma unroll
   for (int i = 0; i < 2; i++)
@@ -655,7 +673,7 @@ __global__ __launch_bounds__(256) void k_seg_gemm_w(const SegProb *__restrict__ 
   auto run = [&](int s0, int s1) {
     for (int s = s0; s < s1; s++) {
       const SegDesc &sg = p.seg[s];
-      seg_f64_w<A_KMAJOR>(sg, sg.sign * (sg.src ? g1 : g0), M, (sg.src ? X1 : X0) + sg.a_off,
+      seg_f64_w<A_KMAJOR, FULL>(sg, sg.sign * (sg.src ? g1 : g0), M, (sg.src ? X1 : X0) + sg.a_off,
                           (sg.src ? X0 : X1) + sg.b_off, m0, n0, S, acc, need);
     }
   };
@@ -708,6 +726,42 @@ __global__ __launch_bounds__(256) void k_seg_gemm_w(const SegProb *__restrict__ 
           }
     }
   }
+}
+template <bool A_KMAJOR, int EPI>
+__global__ __launch_bounds__(256) void k_seg_gemm_w(const SegProb *__restrict__ probs, const int4 *__restrict__ tiles,
+                                                    int ntiles, const double *__restrict__ X0,
+                                                    const double *__restrict__ X1, double g0, double g1,
+                                                    double *__restrict__ C, double *__restrict__ W, double sigma,
+                                                    const int *act, long long sx, long long sc, long long sw) {
+  const int t = blockIdx.x;
+  if (t >= ntiles) return;
+  {
+    const long long inst = inst_id(act);
+    X0 += inst * sx;
+    X1 += inst * sx;
+    if (C) C += inst * sc;
+    if (W) W += inst * sw;
+  }
+  const int4 tl = tiles[t];
+  if (tl.x < 0) return;
+  const SegProb &p = probs[tl.x];
+  const int m0 = tl.y * WM, n0 = tl.z * WN, M = p.M;
+  const int wave = threadIdx.x >> 6;
+  unsigned need = 0;
+#pragma unroll
+  for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+    for (int tj = 0; tj < 4; tj++) {
+      const int r16 = m0 + wave * 32 + ti * 16, c16 = n0 + tj * 16;
+      if (r16 < M && c16 < M && r16 <= c16) need |= 1u << (4 * ti + tj);
+    }
+  need = __builtin_amdgcn_readfirstlane(need);  // uniform per wavefront
+  __shared__ GemmSmemW S;
+  // workgroup-uniform choice: every 16x16 sub-tile inside the block and on or above the diagonal
+  if (m0 + WM <= M && n0 + WN <= M && m0 + WM - 16 <= n0)
+    seg_tile_w<A_KMAJOR, EPI, true>(p, m0, n0, X0, X1, g0, g1, C, W, sigma, need, S);
+  else
+    seg_tile_w<A_KMAJOR, EPI, false>(p, m0, n0, X0, X1, g0, g1, C, W, sigma, need, S);
 }
 
 void launch_seg_gemm_pphh(hipStream_t s, int mode, const SegProb *probs, const int4 *tiles, int ntiles,
