@@ -28,6 +28,8 @@ struct DevLayout {
   const double *nb_coef;
   const int *ob_p, *ob_q;
   const int *php_a, *php_i;  // same-(l,j,tz) (particle, hole) pairs
+  int nsame, ob_nchunk;      // one-body outputs with 121/221 terms; 256-term chunks per output
+  const int *same_ob, *ob_same;  // their one-body indices; per one-body output its same index or -1
   const int *ccJ, *ccrlen, *ccnph;
   const long long *ccoff, *cwoff;
   const int *ccrow, *ccrbase, *ccrow_a, *ccrow_b, *cccbase, *cccol_p, *cccol_h, *cccol_q;
@@ -113,10 +115,11 @@ void launch_ph_apply(hipStream_t s, const DevLayout &L, const double *Tbuf, doub
                      int nact);
 // upper += herm * (strict lower)^T, then lower = herm * upper (diagonal zeroed for herm = -1)
 void launch_sym_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm, const int *act, int nact);
-// 111 + 121 + 221 one-body outputs; Wpp = pp intermediate of launch_seg_gemm_pphh
+// 111 + 121 + 221 one-body outputs; Wpp = pp intermediate of launch_seg_gemm_pphh;
+// part: nsame * ob_nchunk doubles per instance of scratch for the 121/221 partial sums
 void launch_onebody(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop,
-                    const double *Wpp, double *RES, int hL, int hR, int hres, unsigned mask, const int *act,
-                    int nact);
+                    const double *Wpp, double *part, double *RES, int hL, int hR, int hres, unsigned mask,
+                    const int *act, int nact);
 // E0 = 110 + 220 into RES[inst * rstride]
 void launch_e0(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES,
                int hL, int hR, unsigned mask, double *partials, const int *act, int nact, long long rstride);

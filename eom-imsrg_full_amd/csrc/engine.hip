@@ -43,6 +43,7 @@ struct imsrg_ctx_s {
   double *wph = nullptr;
   double *tph = nullptr;    // ph J' terms
   double *partials = nullptr;
+  double *obpart = nullptr;  // one-body 121/221 partial sums, nsame * ob_nchunk per instance
   double *scal = nullptr;   // device scalars, kScal per instance
   double *hscal = nullptr;  // pinned host mirror of scal
   double *mono = nullptr;
@@ -309,6 +310,16 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
         php_i.push_back(i);
       }
   D.nphp = (int)php_a.size();
+  std::vector<int> same_ob, ob_same(Ly.ob_p.size(), -1);
+  for (size_t t = 0; t < Ly.ob_p.size(); t++) {
+    const int p = Ly.ob_p[t], q = Ly.ob_q[t];
+    if (Ly.jj[p] == Ly.jj[q] && Ly.l[p] == Ly.l[q] && Ly.tz[p] == Ly.tz[q]) {
+      ob_same[t] = (int)same_ob.size();
+      same_ob.push_back((int)t);
+    }
+  }
+  D.nsame = (int)same_ob.size();
+  D.ob_nchunk = (D.nphp * Ly.nJ + Ly.nsp * Ly.nJ + 255) / 256;
   std::vector<int> cccol_q(Ly.cccol_p.size());
   for (int q = 0; q < Ly.ncc; q++)
     for (int k = Ly.cccbase[q]; k < Ly.cccbase[q + 1]; k++) cccol_q[k] = q;
@@ -326,6 +337,7 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
   UP(Ly.nb_start, nb_start); UP(Ly.nb_pos, nb_pos); UP(Ly.nb_x, nb_x); UP(Ly.nb_i, nb_i);
   UP(Ly.nb_coef, nb_coef);
   UP(Ly.ob_p, ob_p); UP(Ly.ob_q, ob_q); UP(php_a, php_a); UP(php_i, php_i);
+  UP(same_ob, same_ob); UP(ob_same, ob_same);
   UP(Ly.ccJ, ccJ); UP(Ly.ccrlen, ccrlen); UP(Ly.ccnph, ccnph); UP(ccoff, ccoff); UP(cwoff, cwoff);
   UP(Ly.ccrow, ccrow); UP(Ly.ccrbase, ccrbase); UP(Ly.ccrow_a, ccrow_a); UP(Ly.ccrow_b, ccrow_b);
   UP(Ly.cccbase, cccbase); UP(Ly.cccol_p, cccol_p); UP(Ly.cccol_h, cccol_h); UP(cccol_q, cccol_q);
@@ -379,6 +391,7 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
   if (!rc) rc = dalloc(c, Ly.cwsize * B, &c->wph);
   if (!rc) rc = dalloc(c, Ly.ph_terms.size() * B, &c->tph);
   if (!rc) rc = dalloc(c, kPartials * B, &c->partials);
+  if (!rc) rc = dalloc(c, (size_t)std::max(D.nsame, 1) * D.ob_nchunk * B, &c->obpart);
   if (!rc) rc = dalloc(c, kScal * B, &c->scal);
   if (!rc) rc = dalloc(c, (size_t)Ly.nsp * Ly.nsp * B, &c->mono);
   if (!rc) rc = dalloc(c, (size_t)Ly.nsp * Ly.nsp, &c->mbar);
@@ -848,7 +861,7 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
     PROF(c, IMSRG_PROF_PH_APPLY, launch_ph_apply(s, D, c->tph, RES->d, ad, na));
   }
   PROF(c, IMSRG_PROF_MIRROR, launch_sym_mirror(s, D, RES->d, hres, ad, na));
-  PROF(c, IMSRG_PROF_ONEBODY, launch_onebody(s, D, L->d, R->d, c->wpp, RES->d, hL, hR, hres, mask, ad, na));
+  PROF(c, IMSRG_PROF_ONEBODY, launch_onebody(s, D, L->d, R->d, c->wpp, c->obpart, RES->d, hL, hR, hres, mask, ad, na));
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -1145,15 +1145,58 @@ __device__ __forceinline__ double w221(const DevLayout &L, const double *Wpp, co
 }
 
 
-// one 256-thread workgroup per one-body output (p,q); the (orbit, J) pairs of
-// the 121 and 221 sums are flattened over the threads (MJ = nJ slots per
-// orbit, the most J values two orbits can couple to), then reduced in LDS.
-__global__ __launch_bounds__(256) void k_onebody(DevLayout L, const double *__restrict__ Lop,
-                                                 const double *__restrict__ Rop,
-                                                 const double *__restrict__ Wpp,
-                                                 double *__restrict__ RES, int hL, int hR, int hres,
-                                                 unsigned mask, const int *act) {
+// 121 + 221 sums of the one-body outputs (p,q) whose orbits share (l,j,tz)
+// (the others get 111 only).  The (orbit, J) terms of one such output (first
+// the nphp*nJ terms of 121, then the nsp*nJ terms of 221; MJ = nJ slots per
+// orbit, the most J values two orbits can couple to) are cut into chunks of
+// 256, one workgroup per chunk, each writing its (2J+1)-weighted partial sum;
+// k_ob_final adds the partials in chunk order, so results do not depend on
+// scheduling.
+__global__ __launch_bounds__(256) void k_ob_partial(DevLayout L, const double *__restrict__ Lop,
+                                                    const double *__restrict__ Rop,
+                                                    const double *__restrict__ Wpp, double *__restrict__ part,
+                                                    int hL, int hR, unsigned mask, const int *act) {
   __shared__ double red[4];
+  const int sp = blockIdx.x / L.ob_nchunk, ch = blockIdx.x - sp * L.ob_nchunk;
+  if (sp >= L.nsame) return;
+  {
+    const long long inst = inst_id(act);
+    Lop += inst * L.op_size;
+    Rop += inst * L.op_size;
+    Wpp += inst * L.vsize;
+    part += inst * (long long)L.nsame * L.ob_nchunk;
+  }
+  const int tid = threadIdx.x, t = L.same_ob[sp];
+  const int p = L.ob_p[t], qo = L.ob_q[t], N = L.nsp, MJ = L.nJ;
+  const double *FL = Lop + L.f_off, *FR = Rop + L.f_off;
+  const double *VL = Lop + L.v_off, *VR = Rop + L.v_off;
+  const int n121 = (mask & T121) ? L.nphp * MJ : 0, n221 = (mask & T221) ? N * MJ : 0;
+  const int it = ch * 256 + tid;
+  double v = 0.0;
+  if (it < n121) {
+    const int e = it / MJ, m = it - e * MJ;
+    const int a = L.php_a[e], i = L.php_i[e], ji = L.jj[i], jq = L.jj[qo];
+    const int JT = abs(ji - jq) + 2 * m;
+    if (JT <= ji + jq) {
+      const double smx = dv(L, VR, a, p, i, qo, JT), smx2 = dv(L, VL, a, p, i, qo, JT);
+      const double smy = dv(L, VR, i, p, a, qo, JT), smy2 = dv(L, VL, i, p, a, qo, JT);
+      v = (JT + 1) * (FL[a * N + i] * (hL * smx - smy) - FR[a * N + i] * (hR * smx2 - smy2));
+    }
+  } else if (it < n121 + n221) {
+    const int c = (it - n121) / MJ, m = (it - n121) - c * MJ;
+    const int jc = L.jj[c], ji = L.jj[p], JT = abs(jc - ji) + 2 * m;
+    if (JT <= jc + ji) v = w221(L, Wpp, VL, VR, c, p, qo, JT, (double)(hL * hR)) * (JT + 1);
+  }
+  v = wave_sum(v);
+  if ((tid & 63) == 0) red[tid >> 6] = v;
+  __syncthreads();
+  if (tid == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+// one wavefront per one-body output (p,q): 111 (full F products), plus the
+// 121 + 221 partials / (2j_p + 1); writes F(p,q) and F(q,p) = herm F(p,q)
+__global__ __launch_bounds__(64) void k_ob_final(DevLayout L, const double *__restrict__ Lop,
+                                                 const double *__restrict__ Rop, const double *__restrict__ part,
+                                                 double *__restrict__ RES, int hres, unsigned mask, const int *act) {
   const int t = blockIdx.x;
   if (t >= L.nob) return;
   {
@@ -1161,47 +1204,21 @@ __global__ __launch_bounds__(256) void k_onebody(DevLayout L, const double *__re
     Lop += inst * L.op_size;
     Rop += inst * L.op_size;
     RES += inst * L.op_size;
-    Wpp += inst * L.vsize;
+    part += inst * (long long)L.nsame * L.ob_nchunk;
   }
-  const int tid = threadIdx.x;
-  const int p = L.ob_p[t], qo = L.ob_q[t], N = L.nsp, MJ = L.nJ;
+  const int lane = threadIdx.x, p = L.ob_p[t], qo = L.ob_q[t], N = L.nsp;
   const double *FL = Lop + L.f_off, *FR = Rop + L.f_off;
-  const double *VL = Lop + L.v_off, *VR = Rop + L.v_off;
   double val = 0.0;
   if (mask & T111)
-    for (int k = tid; k < N; k += 256) val += FL[p * N + k] * FR[k * N + qo] - FR[p * N + k] * FL[k * N + qo];
-  const bool same = same_ljt(L, p, qo);
-  if (same && (mask & T121)) {
-    const int jq = L.jj[qo];
-    double sm = 0.0;
-    for (int it = tid; it < L.nphp * MJ; it += 256) {
-      const int e = it / MJ, m = it - e * MJ;
-      const int a = L.php_a[e], i = L.php_i[e], ji = L.jj[i];
-      const int JT = abs(ji - jq) + 2 * m;
-      if (JT > ji + jq) continue;
-      const double smx = dv(L, VR, a, p, i, qo, JT), smx2 = dv(L, VL, a, p, i, qo, JT);
-      const double smy = dv(L, VR, i, p, a, qo, JT), smy2 = dv(L, VL, i, p, a, qo, JT);
-      sm += (JT + 1) * (FL[a * N + i] * (hL * smx - smy) - FR[a * N + i] * (hR * smx2 - smy2));
-    }
-    val += sm / (L.jj[p] + 1.0);
-  }
-  if (same && (mask & T221)) {
-    const double pm = (double)(hL * hR);
-    const int ji = L.jj[p];
-    double sm = 0.0;
-    for (int it = tid; it < N * MJ; it += 256) {
-      const int c = it / MJ, m = it - c * MJ;
-      const int jc = L.jj[c], JT = abs(jc - ji) + 2 * m;
-      if (JT > jc + ji) continue;
-      sm += w221(L, Wpp, VL, VR, c, p, qo, JT, pm) * (JT + 1);
-    }
-    val += sm / (ji + 1.0);
-  }
+    for (int k = lane; k < N; k += 64) val += FL[p * N + k] * FR[k * N + qo] - FR[p * N + k] * FL[k * N + qo];
   val = wave_sum(val);
-  if ((tid & 63) == 0) red[tid >> 6] = val;
-  __syncthreads();
-  if (tid == 0) {
-    val = red[0] + red[1] + red[2] + red[3];
+  if (lane == 0) {
+    const int sp = L.ob_same[t];
+    if (sp >= 0 && (mask & (T121 | T221))) {
+      double sm = 0.0;
+      for (int c = 0; c < L.ob_nchunk; c++) sm += part[(long long)sp * L.ob_nchunk + c];
+      val += sm / (L.jj[p] + 1.0);
+    }
     double *F = RES + L.f_off;
     F[p * N + qo] = val;
     if (p != qo) F[qo * N + p] = hres * val;
@@ -1209,11 +1226,13 @@ __global__ __launch_bounds__(256) void k_onebody(DevLayout L, const double *__re
   }
 }
 void launch_onebody(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop,
-                    const double *Wpp, double *RES, int hL, int hR, int hres, unsigned mask, const int *act,
-                    int nact) {
+                    const double *Wpp, double *part, double *RES, int hL, int hR, int hres, unsigned mask,
+                    const int *act, int nact) {
   if (L.nob <= 0 || nact <= 0) return;
-  hipLaunchKernelGGL(k_onebody, dim3(L.nob, nact), dim3(256), 0, s, L, Lop, Rop, Wpp, RES, hL, hR, hres, mask,
-                     act);
+  if (L.nsame > 0 && (mask & (T121 | T221)))
+    hipLaunchKernelGGL(k_ob_partial, dim3(L.nsame * L.ob_nchunk, nact), dim3(256), 0, s, L, Lop, Rop, Wpp, part,
+                       hL, hR, mask, act);
+  hipLaunchKernelGGL(k_ob_final, dim3(L.nob, nact), dim3(64), 0, s, L, Lop, Rop, part, RES, hres, mask, act);
 }
 
 // ------------------------------------------------------------------ reductions
