@@ -103,6 +103,7 @@ void launch_seg_gemm_z(hipStream_t s, int mode, const SegProb *probs, const int4
                        const double *LCC, const double *RCC, double *Z, double hL, double hR, const int *act,
                        int nact, long long scc, long long sz);
 void launch_pandya(hipStream_t s, const DevLayout &L, const double *op, double *cc, const int *act, int nact);
+size_t pandya_lds_bytes(const DevLayout &L);  // per-workgroup LDS of the Pandya kernel
 // 122: M = A_L R - A_R L over whole channel blocks (diagonal symmetrised);
 // launch_sym_mirror later forms RES = M + herm*M^T
 void launch_122(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES,

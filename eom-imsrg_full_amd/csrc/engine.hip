@@ -310,6 +310,12 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
         php_i.push_back(i);
       }
   D.nphp = (int)php_a.size();
+  D.nsp = Ly.nsp; D.nJ = Ly.nJ; D.nhalf = Ly.nhalf; D.nblocks = Ly.nblocks;
+  if (pandya_lds_bytes(D) > 65536) {
+    imsrg_ctx_destroy(c);
+    return fail(IMSRG_EINVAL, "basis too large: the Pandya kernel stages " + std::to_string(pandya_lds_bytes(D)) +
+                                  " bytes of tables per workgroup (limit 65536)");
+  }
   std::vector<int> same_ob, ob_same(Ly.ob_p.size(), -1);
   for (size_t t = 0; t < Ly.ob_p.size(); t++) {
     const int p = Ly.ob_p[t], q = Ly.ob_q[t];

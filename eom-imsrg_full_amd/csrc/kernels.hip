@@ -809,8 +809,13 @@ void launch_seg_gemm_z(hipStream_t s, int mode, const SegProb *probs, const int4
 // ------------------------------------------------------------------ Pandya transform
 // calculate_cross_coupled (cross_coupled.f90:1190-1300): one workgroup per
 // Pandya column (p,h), threads over rows (a,b); output column-major per block.
+// Everything a term needs except the operator element itself is staged in
+// LDS once per column: the 6j slice {jp jh JC; ja jb J'}, the pair positions
+// of (h,a) and (p,b) in every J' block, block offsets and sizes, and the
+// orbit quantum numbers, so each J' term does a single global load.
 __global__ __launch_bounds__(256) void k_pandya(DevLayout L, const double *__restrict__ op,
                                                 double *__restrict__ cc, const int *act) {
+  extern __shared__ double pan_lds[];
   const int col = blockIdx.x;
   if (col >= L.nccol) return;
   {
@@ -826,26 +831,47 @@ __global__ __launch_bounds__(256) void k_pandya(DevLayout L, const double *__res
   const double *V = op + L.v_off;
   double *out = cc + L.ccoff[q1] + (long long)c * rlen;
   const double sq = sqrt(JC + 1.0);
-  const int N = L.nsp, h6 = L.nhalf, nJ = L.nJ;
+  const int N = L.nsp, h6 = L.nhalf, nJ = L.nJ, nsl = h6 * h6 * nJ, NB = L.nblocks;
   const size_t NN = (size_t)N * N;
-  const double *six0 = L.sixj + (((size_t)(jp >> 1) * h6 + (jh >> 1)) * nJ + (JC >> 1)) * h6 * h6 * nJ;
+  double *S6 = pan_lds;
+  long long *BO = reinterpret_cast<long long *>(S6 + nsl);
+  int *PR = reinterpret_cast<int *>(BO + NB);  // [J'/2][a]: position of pair (h,a) in block J'
+  int *PS = PR + nJ * N;                       // [J'/2][b]: position of pair (p,b)
+  int *BD = PS + nJ * N;
+  int *OQ = BD + NB;                           // jj | l << 8 | (2tz + 1) << 16
+  {
+    const double *six0 = L.sixj + (((size_t)(jp >> 1) * h6 + (jh >> 1)) * nJ + (JC >> 1)) * (size_t)nsl;
+    for (int e = threadIdx.x; e < nsl; e += 256) S6[e] = six0[e];
+    for (int e = threadIdx.x; e < NB; e += 256) {
+      BO[e] = L.boff[e];
+      BD[e] = L.bdim[e];
+    }
+    for (int e = threadIdx.x; e < nJ * N; e += 256) {
+      const int J2 = e / N, x = e - J2 * N;
+      const int *pj = L.pidx + (size_t)J2 * NN;
+      PR[e] = h > x ? pj[x * N + h] : pj[h * N + x];
+      PS[e] = p > x ? pj[x * N + p] : pj[p * N + x];
+    }
+    for (int e = threadIdx.x; e < N; e += 256) OQ[e] = L.jj[e] | (L.l[e] << 8) | ((L.tz[e] + 1) << 16);
+  }
+  __syncthreads();
   for (int r = threadIdx.x; r < rlen; r += blockDim.x) {
     const int a = L.ccrow_a[L.ccrbase[q1] + r], b = L.ccrow_b[L.ccrbase[q1] + r];
     if (a < 0) {  // zero partner row of an (a,a) row
       out[r] = 0.0;
       continue;
     }
-    const int ja = L.jj[a], jb = L.jj[b];
+    const int qa = OQ[a], qbb = OQ[b];
+    const int ja = qa & 255, jb = qbb & 255, la = (qa >> 8) & 255, lb = (qbb >> 8) & 255;
+    const int ta = (qa >> 16) - 1, tb = (qbb >> 16) - 1;
     double val = 0.0;
-    if ((((L.l[a] + lh) & 1) == ((L.l[b] + lp) & 1)) && (L.tz[a] + th == L.tz[b] + tp)) {
+    if ((((la + lh) & 1) == ((lb + lp) & 1)) && (ta + th == tb + tp)) {
       const int Jmin = max(abs(jp - jb), abs(ja - jh));
       const int Jmax = min(jp + jb, ja + jh);
       // v_elem(h,a,p,b;J'): bra pair (h,a), ket pair (p,b), block (J', T2, P)
-      const int qb = 2 * ((th + L.tz[a]) / 2 + 1) + ((lh + L.l[a]) & 1);
+      const int qb = 2 * ((th + ta) / 2 + 1) + ((lh + la) & 1);
       const bool sw1 = h > a, sw2 = p > b;
-      const int *pr = L.pidx + (sw1 ? a * N + h : h * N + a);
-      const int *ps = L.pidx + (sw2 ? b * N + p : p * N + b);
-      const double *six = six0 + ((ja >> 1) * h6 + (jb >> 1)) * nJ;
+      const double *six = S6 + ((ja >> 1) * h6 + (jb >> 1)) * nJ;
       double sm = 0.0;
       for (int J0 = Jmin; J0 <= Jmax; J0 += 8) {
         double vv[4], cf[4];
@@ -855,14 +881,13 @@ __global__ __launch_bounds__(256) void k_pandya(DevLayout L, const double *__res
           vv[u] = 0.0;
           cf[u] = 0.0;
           if (JT <= Jmax) {
-            const size_t jo = (size_t)(JT >> 1) * NN;
-            const int rr = pr[jo], ss = ps[jo], q = 3 * JT + qb;
+            const int rr = PR[(JT >> 1) * N + a], ss = PS[(JT >> 1) * N + b], q = 3 * JT + qb;
             double pre = 1.0;
             if (sw1) pre = sgnp(1 + ((jh + ja - JT) >> 1));
             else if (h == a) pre = pre * M_SQRT2;
             if (sw2) pre = pre * sgnp(1 + ((jp + jb - JT) >> 1));
             else if (p == b) pre = pre * M_SQRT2;
-            if ((rr | ss) >= 0) vv[u] = V[L.boff[q] + (long long)rr * L.bdim[q] + ss] * pre;
+            if ((rr | ss) >= 0) vv[u] = V[BO[q] + (long long)rr * BD[q] + ss] * pre;
             cf[u] = (double)(sgnp(JT >> 1) * (JT + 1)) * six[JT >> 1];
           }
         }
@@ -874,9 +899,13 @@ __global__ __launch_bounds__(256) void k_pandya(DevLayout L, const double *__res
     out[r] = val;
   }
 }
+size_t pandya_lds_bytes(const DevLayout &L) {
+  return sizeof(double) * L.nhalf * L.nhalf * L.nJ + sizeof(long long) * L.nblocks +
+         sizeof(int) * (2 * L.nJ * L.nsp + L.nblocks + L.nsp);
+}
 void launch_pandya(hipStream_t s, const DevLayout &L, const double *op, double *cc, const int *act, int nact) {
   if (L.nccol <= 0 || nact <= 0) return;
-  hipLaunchKernelGGL(k_pandya, dim3(L.nccol, nact), dim3(256), 0, s, L, op, cc, act);
+  hipLaunchKernelGGL(k_pandya, dim3(L.nccol, nact), dim3(256), pandya_lds_bytes(L), s, L, op, cc, act);
 }
 
 // ------------------------------------------------------------------ 122
@@ -905,15 +934,30 @@ __global__ __launch_bounds__(64) void k_122(DevLayout L, const double *__restric
   double *OV = RES + bo + (long long)rl * dim;
   const double *FL = Lop + L.f_off, *FR = Rop + L.f_off;
   const int e0 = L.nb_start[r], e1 = L.nb_start[r + 1];
-  for (int s = threadIdx.x; s < dim; s += 64) {
-    double val = 0.0;
+  // four outputs per thread (s0 + 64u + lane): each neighbour entry issues 8
+  // independent row loads and its scalars are read once for all four
+  for (int s0 = 0; s0 < dim; s0 += 256) {
+    double val[4] = {0.0, 0.0, 0.0, 0.0};
     for (int e = e0; e < e1; e++) {
       const long long pe = L.nb_pos[e];
       const int xi = L.nb_x[e] * N + L.nb_i[e];
-      const double ce = L.nb_coef[e];
-      val += ce * (FL[xi] * RV[pe * dim + s] - FR[xi] * LV[pe * dim + s]);
+      const double ce = L.nb_coef[e], fl = FL[xi], fr = FR[xi];
+      const double *rrow = RV + pe * dim, *lrow = LV + pe * dim;
+      double rv[4], lv[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int s = s0 + 64 * u + (int)threadIdx.x;
+        rv[u] = s < dim ? rrow[s] : 0.0;
+        lv[u] = s < dim ? lrow[s] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) val[u] += ce * (fl * rv[u] - fr * lv[u]);
     }
-    OV[s] = (s == rl) ? val * (1.0 + hres) : val;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int s = s0 + 64 * u + (int)threadIdx.x;
+      if (s < dim) OV[s] = (s == rl) ? val[u] * (1.0 + hres) : val[u];
+    }
   }
 }
 void launch_122(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES,
