@@ -133,7 +133,7 @@ def basis_tag(basis):
 KERNEL_NAME = {
     "gemm_pphh": "k_seg_gemm<false, 0, 0>", "gemm_ph": "k_seg_gemm<true, 0, 1>", "pandya": "k_pandya(",
     "k122": "k_122(", "ph_terms": "k_ph_terms(", "ph_apply": "k_ph_apply(", "sym_mirror": "k_sym_mirror(",
-    "onebody": "k_onebody(",
+    "onebody": "k_ob_partial(",
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_emax8_fp64.txt")
 
