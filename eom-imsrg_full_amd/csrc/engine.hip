@@ -69,6 +69,7 @@ struct imsrg_ctx_s {
   imsrg_op_s *tmp[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // per-kernel-class HIP event timing on this context's stream
   bool prof_on = false;
+  unsigned prof_mask = ~0u;  // kernel classes bracketed while profiling
   // hipGraph replay of whole commutators: one executable graph per distinct
   // (operands, herms, mask, Pandya reuse, instance table, precision)
   bool graphs = false;
@@ -98,7 +99,7 @@ static hipEvent_t prof_event(imsrg_ctx_s *c) {
 #define PROF(c, cls, stmt)                                                   \
   do {                                                                       \
     hipEvent_t ea_ = nullptr, eb_ = nullptr;                                 \
-    if ((c)->prof_on) {                                                      \
+    if ((c)->prof_on && (((c)->prof_mask >> (cls)) & 1u)) {                 \
       ea_ = prof_event(c);                                                   \
       eb_ = prof_event(c);                                                   \
       if (ea_ && eb_) (void)hipEventRecord(ea_, (c)->stream);                \
@@ -1486,6 +1487,14 @@ int imsrg_op_download_cotangent(imsrg_op op, double *vec, int64_t n) {
 int imsrg_ctx_profile(imsrg_ctx c, int enable) {
   if (!c) return fail(IMSRG_EINVAL, "null context");
   c->prof_on = enable != 0;
+  c->prof_mask = ~0u;
+  return 0;
+}
+
+int imsrg_ctx_profile_classes(imsrg_ctx c, unsigned mask) {
+  if (!c) return fail(IMSRG_EINVAL, "null context");
+  c->prof_on = mask != 0;
+  c->prof_mask = mask;
   return 0;
 }
 

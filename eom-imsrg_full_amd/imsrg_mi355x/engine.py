@@ -92,6 +92,7 @@ _SIG = {
     "imsrg_flow_converged": (C.c_int, [_vp, _ip]),
     "imsrg_flow_destroy": (C.c_int, [_vp]),
     "imsrg_ctx_profile": (C.c_int, [_vp, C.c_int]),
+    "imsrg_ctx_profile_classes": (C.c_int, [_vp, C.c_uint]),
     "imsrg_ctx_profile_read": (C.c_int, [_vp, _dp, C.POINTER(C.c_longlong), C.c_int, C.c_int]),
 }
 PROF_CLASSES = ["gemm_pphh", "gemm_ph", "pandya", "k122", "ph_apply", "ph_terms", "sym_mirror",
@@ -259,8 +260,15 @@ class Context:
     def flow(self, HS, kind="white", ds=0.5, G=None):
         return Flow(self, HS, kind, ds, G)
 
-    def profile(self, enable=True):
-        _chk(_lib.imsrg_ctx_profile(self.h, 1 if enable else 0))
+    def profile(self, enable=True, classes=None):
+        """time every kernel class with HIP events, or only those named in classes"""
+        if enable and classes is not None:
+            mask = 0
+            for name in classes:
+                mask |= 1 << PROF_CLASSES.index(name)
+            _chk(_lib.imsrg_ctx_profile_classes(self.h, mask))
+        else:
+            _chk(_lib.imsrg_ctx_profile(self.h, 1 if enable else 0))
 
     def profile_read(self, reset=True):
         """{kernel class: (total ms, launches)} since the last reset."""

@@ -216,6 +216,9 @@ int imsrg_flow_destroy(imsrg_flow flow);
 #define IMSRG_PROF_GENERATOR 11
 #define IMSRG_PROF_NCLASS 16
 int imsrg_ctx_profile(imsrg_ctx ctx, int enable);
+/* bracket only the classes in mask (bit k = class k; 0 turns profiling off):
+ * fewer events on the stream while a run is timed */
+int imsrg_ctx_profile_classes(imsrg_ctx ctx, unsigned mask);
 int imsrg_ctx_profile_read(imsrg_ctx ctx, double *ms, long long *launches, int nclass, int reset);
 
 #ifdef __cplusplus
