@@ -1,11 +1,13 @@
 // kernels.hip -- CDNA4 (gfx950) kernels of the IMSRG(2) commutator engine.
 //
 // Operators are device buffers [E0 | F (nsp x nsp) | V (full channel blocks)]
-// described in layout.hpp.  Every two-body kernel walks the upper triangle of
-// each channel block with one wavefront per row (row = one pair); the lower
-// triangle is written only by k_mirror as herm * upper, which reproduces the
+// described in layout.hpp.  The two-body terms of a commutator accumulate into
+// the upper triangle of each channel block (k_122 also leaves the source of
+// its transposed half in the lower triangle); the last pass, k_sym_mirror,
+// adds that half and writes lower = herm * upper, which reproduces the
 // reference's "upper-triangle then mirror" contract for square sub-blocks and
-// its implied herm * transpose for the non-square ones.
+// its implied herm * transpose for the non-square ones.  Launches take a table
+// of active instances (batched contexts) indexed with blockIdx.y.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
