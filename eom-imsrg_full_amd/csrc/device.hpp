@@ -116,11 +116,13 @@ void launch_ph_apply(hipStream_t s, const DevLayout &L, const double *Tbuf, doub
                      int nact);
 // upper += herm * (strict lower)^T, then lower = herm * upper (diagonal zeroed for herm = -1)
 void launch_sym_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm, const int *act, int nact);
-// 111 + 121 + 221 one-body outputs; Wpp = pp intermediate of launch_seg_gemm_pphh;
-// part: nsame * ob_nchunk doubles per instance of scratch for the 121/221 partial sums
-void launch_onebody(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop,
-                    const double *Wpp, double *part, double *RES, int hL, int hR, int hres, unsigned mask,
-                    const int *act, int nact);
+// 111 + 121 + 221 one-body outputs in two launches: the 121/221 partial sums
+// (Wpp = pp intermediate of launch_seg_gemm_pphh; part: nsame * ob_nchunk
+// doubles per instance), then 111 plus the partials into F (and its mirror)
+void launch_ob_partial(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop,
+                       const double *Wpp, double *part, int hL, int hR, unsigned mask, const int *act, int nact);
+void launch_ob_final(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, const double *part,
+                     double *RES, int hres, unsigned mask, const int *act, int nact);
 // E0 = 110 + 220 into RES[inst * rstride]
 void launch_e0(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES,
                int hL, int hR, unsigned mask, double *partials, const int *act, int nact, long long rstride);

@@ -36,6 +36,12 @@ struct imsrg_ctx_s {
   DevLayout dl{};
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream of a commutator: the pp/hh chain (122, pp/hh GEMM, one-body
+  // partials) runs there while the ph chain runs on `stream`; fork/join events
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool dual = true;        // use stream2 (imsrg_ctx_set_concurrency)
+  bool capturing = false;  // graph capture records one stream only
   std::vector<void *> dallocs;
   int nbatch = 1;           // independent instances per operator
   double *wpp = nullptr;    // 221 intermediate W1 - pm*W1^T (upper triangles), vsize per instance
@@ -96,20 +102,21 @@ static hipEvent_t prof_event(imsrg_ctx_s *c) {
   }
   return c->evpool[c->evused++];
 }
-#define PROF(c, cls, stmt)                                                   \
+#define PROFS(c, cls, strm, stmt)                                            \
   do {                                                                       \
     hipEvent_t ea_ = nullptr, eb_ = nullptr;                                 \
     if ((c)->prof_on && (((c)->prof_mask >> (cls)) & 1u)) {                 \
       ea_ = prof_event(c);                                                   \
       eb_ = prof_event(c);                                                   \
-      if (ea_ && eb_) (void)hipEventRecord(ea_, (c)->stream);                \
+      if (ea_ && eb_) (void)hipEventRecord(ea_, (strm));                     \
     }                                                                        \
     stmt;                                                                    \
     if (ea_ && eb_) {                                                        \
-      (void)hipEventRecord(eb_, (c)->stream);                                \
+      (void)hipEventRecord(eb_, (strm));                                     \
       (c)->pend.push_back({(cls), ea_, eb_});                                \
     }                                                                        \
   } while (0)
+#define PROF(c, cls, stmt) PROFS(c, cls, (c)->stream, stmt)
 
 struct imsrg_op_s {
   imsrg_ctx_s *ctx = nullptr;
@@ -295,6 +302,12 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return fail(IMSRG_EDEVICE, "hipStreamCreate failed");
+  }
+  if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    imsrg_ctx_destroy(c);
+    return fail(IMSRG_EDEVICE, "second stream / events failed");
   }
   const Layout &Ly = c->lay;
   DevLayout &D = c->dl;
@@ -519,6 +532,7 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
 int imsrg_ctx_destroy(imsrg_ctx c) {
   if (!c) return 0;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   for (auto *t : c->tmp)
     if (t) {
       (void)hipFree(t->d);
@@ -531,6 +545,9 @@ int imsrg_ctx_destroy(imsrg_ctx c) {
   for (int k = 0; k < imsrg_ctx_s::kSlots; k++)
     if (c->act_h[k]) (void)hipHostFree(c->act_h[k]);
   if (c->hstage) (void)hipHostFree(c->hstage);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -815,7 +832,9 @@ static int comm_impl(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *R
   if (it == c->gexec.end()) {
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
     int rc = comm_launches(c, L, R, RES, mask, lcc_ready, a);
+    c->capturing = false;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc) {
       if (g) (void)hipGraphDestroy(g);
@@ -842,6 +861,28 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
   const int *ad = a.d, na = a.n();
   const long long OS = D.op_size;
   if (na == 0) return 0;
+  // The pp/hh chain (122 or clear, pp/hh GEMM, one-body partials) and the ph
+  // chain (E0, Pandya, Z GEMM, J' terms) touch disjoint outputs and scratch,
+  // so they run on two streams; both end before the ph apply accumulates
+  // into the blocks the pp/hh chain wrote.  A graph capture records one stream.
+  const bool dual = c->dual && !c->capturing;
+  hipStream_t s2 = dual ? c->stream2 : s;
+  if (dual) {
+    HIPCHK(hipEventRecord(c->ev_fork, s));
+    HIPCHK(hipStreamWaitEvent(s2, c->ev_fork, 0));
+  }
+  if (mask & T122) PROFS(c, IMSRG_PROF_122, s2, launch_122(s2, D, L->d, R->d, RES->d, hres, ad, na));
+  else launch_clear_blocks(s2, D, RES->d, ad, na);
+  if (mask & (T222PPHH | T221)) {
+    const bool w = c->prec == IMSRG_PREC_FP64 && c->shape_pp == 1;
+    PROFS(c, IMSRG_PROF_GEMM_PPHH, s2,
+          launch_seg_gemm_pphh(s2, w ? 3 : c->prec, c->pp_probs, w ? c->pp_tiles_w : c->pp_tiles,
+                               w ? c->n_pp_tiles_w : c->n_pp_tiles, L->d + D.v_off, R->d + D.v_off,
+                               (mask & T222PPHH) ? RES->d + D.v_off : nullptr, (mask & T221) ? c->wpp : nullptr,
+                               ad, na, OS, D.vsize));
+  }
+  PROFS(c, IMSRG_PROF_ONEBODY, s2,
+        launch_ob_partial(s2, D, L->d, R->d, c->wpp, c->obpart, hL, hR, mask, ad, na));
   if (mask & (T110 | T220))
     PROF(c, IMSRG_PROF_E0, launch_e0(s, D, L->d, R->d, RES->d, hL, hR, mask, c->partials, ad, na, OS));
   if (mask & T222PH) {
@@ -852,23 +893,15 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
          launch_seg_gemm_z(s, w ? 3 : c->prec, c->ph_probs, w ? c->ph_tiles_w : c->ph_tiles,
                            w ? c->n_ph_tiles_w : c->n_ph_tiles, c->cc[0], c->cc[1], c->wph, (double)hL, (double)hR,
                            ad, na, D.ccsize, D.cwsize));
-  }
-  if (mask & T122) PROF(c, IMSRG_PROF_122, launch_122(s, D, L->d, R->d, RES->d, hres, ad, na));
-  else launch_clear_blocks(s, D, RES->d, ad, na);
-  if (mask & (T222PPHH | T221)) {
-    const bool w = c->prec == IMSRG_PREC_FP64 && c->shape_pp == 1;
-    PROF(c, IMSRG_PROF_GEMM_PPHH,
-         launch_seg_gemm_pphh(s, w ? 3 : c->prec, c->pp_probs, w ? c->pp_tiles_w : c->pp_tiles,
-                              w ? c->n_pp_tiles_w : c->n_pp_tiles, L->d + D.v_off, R->d + D.v_off,
-                              (mask & T222PPHH) ? RES->d + D.v_off : nullptr, (mask & T221) ? c->wpp : nullptr, ad,
-                              na, OS, D.vsize));
-  }
-  if (mask & T222PH) {
     PROF(c, IMSRG_PROF_PH_TERMS, launch_ph_terms(s, D, c->wph, c->tph, (double)hL, (double)hR, ad, na));
-    PROF(c, IMSRG_PROF_PH_APPLY, launch_ph_apply(s, D, c->tph, RES->d, ad, na));
   }
+  if (dual) {
+    HIPCHK(hipEventRecord(c->ev_join, s2));
+    HIPCHK(hipStreamWaitEvent(s, c->ev_join, 0));
+  }
+  if (mask & T222PH) PROF(c, IMSRG_PROF_PH_APPLY, launch_ph_apply(s, D, c->tph, RES->d, ad, na));
   PROF(c, IMSRG_PROF_MIRROR, launch_sym_mirror(s, D, RES->d, hres, ad, na));
-  PROF(c, IMSRG_PROF_ONEBODY, launch_onebody(s, D, L->d, R->d, c->wpp, c->obpart, RES->d, hL, hR, hres, mask, ad, na));
+  PROF(c, IMSRG_PROF_ONEBODY, launch_ob_final(s, D, L->d, R->d, c->obpart, RES->d, hres, mask, ad, na));
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1488,6 +1521,14 @@ int imsrg_ctx_profile(imsrg_ctx c, int enable) {
   if (!c) return fail(IMSRG_EINVAL, "null context");
   c->prof_on = enable != 0;
   c->prof_mask = ~0u;
+  return 0;
+}
+
+int imsrg_ctx_set_concurrency(imsrg_ctx c, int enable) {
+  if (!c) return fail(IMSRG_EINVAL, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream2));
+  c->dual = enable != 0;
   return 0;
 }
 

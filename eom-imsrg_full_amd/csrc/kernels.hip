@@ -1271,13 +1271,15 @@ __global__ __launch_bounds__(64) void k_ob_final(DevLayout L, const double *__re
     else if (hres == -1) F[p * N + p] = 0.0;
   }
 }
-void launch_onebody(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop,
-                    const double *Wpp, double *part, double *RES, int hL, int hR, int hres, unsigned mask,
-                    const int *act, int nact) {
+void launch_ob_partial(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop,
+                       const double *Wpp, double *part, int hL, int hR, unsigned mask, const int *act, int nact) {
+  if (L.nob <= 0 || nact <= 0 || L.nsame <= 0 || !(mask & (T121 | T221))) return;
+  hipLaunchKernelGGL(k_ob_partial, dim3(L.nsame * L.ob_nchunk, nact), dim3(256), 0, s, L, Lop, Rop, Wpp, part,
+                     hL, hR, mask, act);
+}
+void launch_ob_final(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, const double *part,
+                     double *RES, int hres, unsigned mask, const int *act, int nact) {
   if (L.nob <= 0 || nact <= 0) return;
-  if (L.nsame > 0 && (mask & (T121 | T221)))
-    hipLaunchKernelGGL(k_ob_partial, dim3(L.nsame * L.ob_nchunk, nact), dim3(256), 0, s, L, Lop, Rop, Wpp, part,
-                       hL, hR, mask, act);
   hipLaunchKernelGGL(k_ob_final, dim3(L.nob, nact), dim3(64), 0, s, L, Lop, Rop, part, RES, hres, mask, act);
 }
 

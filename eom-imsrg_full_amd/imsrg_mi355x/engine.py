@@ -93,6 +93,7 @@ _SIG = {
     "imsrg_flow_destroy": (C.c_int, [_vp]),
     "imsrg_ctx_profile": (C.c_int, [_vp, C.c_int]),
     "imsrg_ctx_profile_classes": (C.c_int, [_vp, C.c_uint]),
+    "imsrg_ctx_set_concurrency": (C.c_int, [_vp, C.c_int]),
     "imsrg_ctx_profile_read": (C.c_int, [_vp, _dp, C.POINTER(C.c_longlong), C.c_int, C.c_int]),
 }
 PROF_CLASSES = ["gemm_pphh", "gemm_ph", "pandya", "k122", "ph_apply", "ph_terms", "sym_mirror",
@@ -169,6 +170,10 @@ class Context:
     def set_graphs(self, enable=True):
         """replay every commutator from a cached hipGraph"""
         _chk(_lib.imsrg_ctx_set_graphs(self.h, 1 if enable else 0))
+
+    def set_concurrency(self, enable=True):
+        """run the pp/hh and ph chains of each commutator on two streams (default on)"""
+        _chk(_lib.imsrg_ctx_set_concurrency(self.h, 1 if enable else 0))
 
     def tuning(self):
         """FP64 tile shapes chosen by the autotuner and its four timings (ms)"""

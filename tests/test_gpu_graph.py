@@ -79,3 +79,36 @@ def test_graph_batched_flow_matches_eager(e4):
         out.append(([[x["e0"] for x in r] for r in recs], HS.download()))
     assert out[0][0] == out[1][0]
     assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_two_stream_commutators_match_one_stream():
+    """the pp/hh and ph chains on two streams (the default) give the
+    single-stream results bit for bit: single commutators of every hermiticity
+    pair at emax=6, a BCH series and a batched one at emax=4"""
+    from imsrg_mi355x import engine as E
+    B = Basis.ho(6, 3)
+    ctx = E.Context(B)
+    for hl, hr in ((-1, 1), (1, 1), (-1, -1)):
+        L, R = ctx.op(hl, random_operator(B, hl, 81)), ctx.op(hr, random_operator(B, hr, 82))
+        outs = []
+        for dual in (False, True, False, True):
+            ctx.set_concurrency(dual)
+            o = ctx.commutator(L, R)
+            outs.append(o.download())
+        print(f"[streams] emax=6 herm ({hl},{hr}): max |diff| {np.abs(outs[1] - outs[0]).max():.1e}", flush=True)
+        for o in outs[1:]:
+            assert np.array_equal(o, outs[0])
+    B4 = Basis.ho(4, 3)
+    for nb in (1, 3):
+        ctxb = E.Context(B4, nbatch=nb)
+        Hv = batch_hamiltonians(B4, range(nb)) if nb > 1 else synthetic_hamiltonian(B4, seed=5)
+        res = []
+        for dual in (False, True):
+            ctxb.set_concurrency(dual)
+            H = ctxb.op(1, Hv).normal_order()
+            G = ctxb.generator("white", H)
+            G.scale(0.3)
+            HS, nt = ctxb.bch(G, H)
+            res.append((HS.download(), np.atleast_1d(nt)))
+        print(f"[streams] emax=4 nbatch={nb} BCH terms {res[0][1]}", flush=True)
+        assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
