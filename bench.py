@@ -50,6 +50,8 @@ def parse():
     p.add_argument("--precision", default="all", choices=["all", "fp64", "split32", "bf16"])
     p.add_argument("--cpu-baseline", default="auto", choices=["auto", "reference", "port", "none"])
     p.add_argument("--cpu-reps", type=int, default=3, help="commutators in the CPU sample")
+    p.add_argument("--no-concurrency", action="store_true",
+                   help="run each commutator on one stream (default: pp/hh and ph chains on two)")
     p.add_argument("--batch", type=int, default=16, help="instances of the batched measurement (0: skip)")
     p.add_argument("--batch-emax", type=int, default=6)
     p.add_argument("--batch-holes", type=int, default=3)
@@ -224,7 +226,9 @@ def timed_flow(ctx, E, H0vec, a, prec, classes):
     steps with only the dominant class of `classes` bracketed (the largest in
     the warmup; all of `classes` without warmup), so the events add as little
     as possible to the measured time; then one more step, untimed, with every
-    class bracketed for the per-class breakdown."""
+    class bracketed and each commutator on one stream, for the per-class
+    breakdown of kernels running alone (in the timed steps the pp/hh and ph
+    chains of a commutator overlap, which lengthens each kernel's span)."""
     ctx.set_precision(PRECS[prec])
     HS = ctx.op(1, H0vec)
     flow = ctx.flow(HS, a.generator, 0.5)
@@ -243,10 +247,12 @@ def timed_flow(ctx, E, H0vec, a, prec, classes):
     ctx.sync()
     t1 = time.perf_counter()
     prof = ctx.profile_read(reset=True)
+    ctx.set_concurrency(False)  # the breakdown step runs each kernel alone
     ctx.profile(True)
     flow.step()
     breakdown = ctx.profile_read(reset=True)
     ctx.profile(False)
+    ctx.set_concurrency(not a.no_concurrency)
     flow.close()
     return recs, t1 - t0, prof, breakdown
 
@@ -349,6 +355,8 @@ def main():
         return
     log(f"emax={a.emax} holes/iso={a.holes} orbits={basis.nsp} blocks={len(basis.blocks)} neq={basis.neq}")
     ctx = E.Context(basis)
+    if a.no_concurrency:
+        ctx.set_concurrency(False)
     info = ctx.info()
     H = ctx.op(1, synthetic_hamiltonian(basis, seed=0, hw=20.0, sigma1=0.5, sigma2=1.0))
     H.normal_order()
@@ -399,6 +407,11 @@ def main():
     roof = roofline_of(dom, work, prof, head)
     traffic = load_traffic() if (head == "fp64" and a.emax == 8 and a.holes == 6) else {}
     roof["traffic"] = traffic.get(dom)
+    if breakdown[dom][1]:
+        iso = roofline_of(dom, work, breakdown, head)
+        roof["isolated"] = {"achieved": iso["achieved"], "frac": iso["frac"], "avg_launch_ms": iso["avg_launch_ms"],
+                            "note": "one extra step with each commutator on one stream (kernel running alone)"}
+    roof["concurrent_streams"] = 1 if a.no_concurrency else 2
     by_kernel = {k: roofline_of(k, work, breakdown, head) for k in work if breakdown[k][1]}
     for k in by_kernel:
         by_kernel[k]["traffic"] = traffic.get(k)
