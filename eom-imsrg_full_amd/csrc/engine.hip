@@ -861,8 +861,8 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
   const int *ad = a.d, na = a.n();
   const long long OS = D.op_size;
   if (na == 0) return 0;
-  // The pp/hh chain (122 or clear, pp/hh GEMM, one-body partials) and the ph
-  // chain (E0, Pandya, Z GEMM, J' terms) touch disjoint outputs and scratch,
+  // The pp/hh chain (E0, 122 or clear, pp/hh GEMM, one-body partials) and the
+  // ph chain (Pandya, Z GEMM, J' terms) touch disjoint outputs and scratch,
   // so they run on two streams; both end before the ph apply accumulates
   // into the blocks the pp/hh chain wrote.  A graph capture records one stream.
   const bool dual = c->dual && !c->capturing;
@@ -871,6 +871,8 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
     HIPCHK(hipEventRecord(c->ev_fork, s));
     HIPCHK(hipStreamWaitEvent(s2, c->ev_fork, 0));
   }
+  if (mask & (T110 | T220))
+    PROFS(c, IMSRG_PROF_E0, s2, launch_e0(s2, D, L->d, R->d, RES->d, hL, hR, mask, c->partials, ad, na, OS));
   if (mask & T122) PROFS(c, IMSRG_PROF_122, s2, launch_122(s2, D, L->d, R->d, RES->d, hres, ad, na));
   else launch_clear_blocks(s2, D, RES->d, ad, na);
   if (mask & (T222PPHH | T221)) {
@@ -883,8 +885,6 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
   }
   PROFS(c, IMSRG_PROF_ONEBODY, s2,
         launch_ob_partial(s2, D, L->d, R->d, c->wpp, c->obpart, hL, hR, mask, ad, na));
-  if (mask & (T110 | T220))
-    PROF(c, IMSRG_PROF_E0, launch_e0(s, D, L->d, R->d, RES->d, hL, hR, mask, c->partials, ad, na, OS));
   if (mask & T222PH) {
     if (!lcc_ready) PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, L->d, c->cc[0], ad, na));
     PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, R->d, c->cc[1], ad, na));
@@ -941,7 +941,7 @@ int imsrg_cross_coupled(imsrg_op op, double *host_out, int64_t n) {
   if (n != imsrg_cross_coupled_size(c)) return fail(IMSRG_EINVAL, "size mismatch");
   const ActSet all = all_set(c);
   launch_pandya(c->stream, c->dl, op->d, c->cc[0], all.d, all.n());
-  // device rows are paired (a,b)/(b,a); the host copy is in the reference's row order
+  // device rows are ordered (a,b)/(b,a) pairs, then (a,a); the host copy is in the reference's row order
   const Layout &Ly = c->lay;
   HIPCHK(hipMemcpyAsync(c->hstage, c->cc[0], sizeof(double) * Ly.ccsize, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));

@@ -264,7 +264,8 @@ __device__ __forceinline__ int acc_col(int tj) {
 
 // Z of one Pandya block (M x M, M even) is stored in 2x2 blocks: element
 // (R, C) at ((R/2)*(M/2) + C/2)*4 + (R%2)*2 + C%2.  Rows 2u, 2u+1 are the
-// pairs (a,b), (b,a), so the two Z elements of one ph J' term share a block.
+// pairs (a,b), (b,a) (or two (a,a) pairs), so the two Z elements of one ph J'
+// term share a block.
 // Only blocks u <= v are stored (Z(R,C) = sigma Z(C,R) gives the rest).
 __device__ __forceinline__ long long zidx(int R, int C, int M) {
   return ((long long)(R >> 1) * (M >> 1) + (C >> 1)) * 4 + (R & 1) * 2 + (C & 1);
@@ -859,7 +860,7 @@ __global__ __launch_bounds__(256) void k_pandya(DevLayout L, const double *__res
   __syncthreads();
   for (int r = threadIdx.x; r < rlen; r += blockDim.x) {
     const int a = L.ccrow_a[L.ccrbase[q1] + r], b = L.ccrow_b[L.ccrbase[q1] + r];
-    if (a < 0) {  // zero partner row of an (a,a) row
+    if (a < 0) {  // zero row that makes the row count even
       out[r] = 0.0;
       continue;
     }

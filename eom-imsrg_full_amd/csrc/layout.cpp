@@ -199,23 +199,37 @@ std::string Layout::build(int nsp_, const int *n_, const int *l_, const int *jj_
     auto valid = [&](int i, int j) {
       return tri(jj[i], jj[j], JC) && (l[i] + l[j]) % 2 == PAR && std::abs(tz[i] - tz[j]) / 2 == TZ;
     };
+    // rows: the ordered pairs (a,b), (b,a) of every a < b at 2u, 2u+1, then the
+    // (a,a) rows two to a row pair, then one zero row if the count is odd.  A
+    // ph J' term combines Z(R1,C1) and Z(R2,C2) with R2 the reverse of R1's pair
+    // (the same row when it is an (a,a) pair), so both lie in one 2x2 block.
     int r = 0, nb = 0;
+    std::vector<int> same;
     for (int a = 0; a < nsp; a++)
       for (int b = a; b < nsp; b++) {
         if (!valid(a, b)) continue;
+        if (a == b) {
+          same.push_back(a);
+          continue;
+        }
         ccrow[((size_t)q1 * nsp + a) * nsp + b] = r;
+        ccrow[((size_t)q1 * nsp + b) * nsp + a] = r + 1;
         ccrow_a.push_back(a);
         ccrow_b.push_back(b);
-        if (a != b) {
-          ccrow[((size_t)q1 * nsp + b) * nsp + a] = r + 1;
-          ccrow_a.push_back(b);
-          ccrow_b.push_back(a);
-        } else {
-          ccrow_a.push_back(-1);
-          ccrow_b.push_back(-1);
-        }
+        ccrow_a.push_back(b);
+        ccrow_b.push_back(a);
         r += 2;
       }
+    for (int a : same) {
+      ccrow[((size_t)q1 * nsp + a) * nsp + a] = r++;
+      ccrow_a.push_back(a);
+      ccrow_b.push_back(a);
+    }
+    if (r & 1) {
+      ccrow_a.push_back(-1);
+      ccrow_b.push_back(-1);
+      r++;
+    }
     int rref = 0;
     for (int i = 0; i < nsp; i++)
       for (int j = 0; j < nsp; j++) {

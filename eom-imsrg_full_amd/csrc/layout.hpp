@@ -57,9 +57,9 @@ struct Layout {
   int64_t ccsize = 0, cwsize = 0;
   std::vector<int> ccrow;       // [(q*nsp + a)*nsp + b] -> row, -1
   std::vector<int> ccrbase;     // first global row of block q
-  std::vector<int> ccrow_a, ccrow_b;  // per global row (-1 for the zero partner of an (a,a) row)
+  std::vector<int> ccrow_a, ccrow_b;  // per global row (-1 for the zero row that makes the count even)
   // Rows of a Pandya block come in pairs: (a,b) at 2u and (b,a) at 2u+1 (a < b),
-  // (a,a) at 2u with a zero row at 2u+1, so ccrlen is even.  The reference's
+  // then the (a,a) rows, then a zero row if needed, so ccrlen is even.  The reference's
   // row order (all valid ordered pairs, first orbit major) is kept for copies
   // to the host: ccref[ccrefbase[q] + k] = device row of reference row k.
   std::vector<int> ccref, ccrefbase, ccrlen_ref;

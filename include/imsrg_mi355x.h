@@ -99,10 +99,10 @@ int imsrg_ctx_tuning(imsrg_ctx ctx, int *shape_pp, int *shape_ph, float *ms4);
 int imsrg_ctx_set_tile_shapes(imsrg_ctx ctx, int shape_pp, int shape_ph);
 int imsrg_ctx_graph_stats(imsrg_ctx ctx, long long *builds, long long *launches);
 /* Commutator concurrency (no counterpart in the reference): with enable = 1
- * (the default) the pp/hh chain of a commutator (122, pp/hh GEMM, one-body
- * partial sums) runs on a second stream of the context, concurrently with the
- * ph chain (E0, Pandya transforms, Z GEMM, J' terms); both join before the ph
- * apply.  Results are identical either way.  Graph capture uses one stream. */
+ * (the default) the pp/hh chain of a commutator (E0, 122, pp/hh GEMM,
+ * one-body partial sums) runs on a second stream of the context, concurrently
+ * with the ph chain (Pandya transforms, Z GEMM, J' terms); both join before
+ * the ph apply.  Results are identical either way.  Graph capture uses one stream. */
 int imsrg_ctx_set_concurrency(imsrg_ctx ctx, int enable);
 /* the hipStream_t the context launches on (as void*) */
 void *imsrg_ctx_stream(imsrg_ctx ctx);
