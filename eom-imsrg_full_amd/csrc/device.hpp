@@ -35,7 +35,7 @@ struct DevLayout {
   const int *ccrow, *ccrbase, *ccrow_a, *ccrow_b, *cccbase, *cccol_p, *cccol_h, *cccol_q;
   const double *sixj;
   const double *ap6_val;       // packed non-zero 6j runs of the ph apply, per row key (ji, jj, J)
-  const int *ap6_base, *ap6_off;
+  const int *ap6_base;
   int ap6_max;
   const unsigned *ph_quads;  // (i<<24|j<<16|k<<8|l) of every upper (ij,kl), sorted by (i,k,j,l)
   long long nquad;
@@ -44,6 +44,7 @@ struct DevLayout {
   const unsigned *ph_zblk;     // per J' term: 2x2 block of Z holding its two elements (units of 4 doubles)
   const unsigned char *ph_zsel;  // per J' term: element positions, mirror, phase, side, valid
   const unsigned *ph_toff;   // per upper element (row-major per block): first T1 term of its quadruple
+  const unsigned long long *ph_desc;  // per upper element: T offset, run lengths, 6j runs, flags (layout.cpp)
   const long long *ph_ubase; // per global pair row r: index of its first upper element
   const int4 *mirror_tiles;  // (block, tile row, tile col) with row <= col, 32x32 tiles
   int nmirror;

@@ -362,7 +362,7 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
   UP(Ly.ccrow, ccrow); UP(Ly.ccrbase, ccrbase); UP(Ly.ccrow_a, ccrow_a); UP(Ly.ccrow_b, ccrow_b);
   UP(Ly.cccbase, cccbase); UP(Ly.cccol_p, cccol_p); UP(Ly.cccol_h, cccol_h); UP(cccol_q, cccol_q);
   UP(Ly.sixj, sixj);
-  UP(Ly.ap6_val, ap6_val); UP(Ly.ap6_base, ap6_base); UP(Ly.ap6_off, ap6_off);
+  UP(Ly.ap6_val, ap6_val); UP(Ly.ap6_base, ap6_base);
   D.ap6_max = Ly.ap6_max;
   UP(Ly.ph_quads, ph_quads);
   D.nquad = (long long)Ly.ph_quads.size();
@@ -370,6 +370,10 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
   D.nphterm = (long long)Ly.ph_terms.size();
   UP(Ly.ph_toff, ph_toff);
   UP(Ly.ph_zblk, ph_zblk); UP(Ly.ph_zsel, ph_zsel);
+  {
+    std::vector<unsigned long long> pd(Ly.ph_desc.begin(), Ly.ph_desc.end());
+    UP(pd, ph_desc);
+  }
   {
     std::vector<long long> ub(Ly.ph_ubase.begin(), Ly.ph_ubase.end());
     UP(ub, ph_ubase);

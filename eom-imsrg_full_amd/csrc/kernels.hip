@@ -1031,11 +1031,24 @@ __global__ __launch_bounds__(256) void k_ph_terms(DevLayout L, const double *__r
 // (ji, jj, J), staged in LDS once per row (zero-padded to the longest list).
 // Rows run in (i, j, J) order, split into 8 contiguous ranges (one per XCD):
 // the rows of one orbit pair read the same quadruples' J' terms, so the
-// repeats hit L2.  Each output's two J' ranges are contiguous in Tbuf and are
-// loaded 8 at a time, so a thread keeps several loads in flight.
+// repeats hit L2.  What each output needs beyond its row (where its J' terms
+// start, how many, where its 6j runs start in the list, signs) is one 64-bit
+// host-built descriptor.  Each output's two J' ranges are one contiguous run
+// in Tbuf, loaded 8 at a time so a thread keeps several loads in flight.
+__device__ __forceinline__ void ph_decode(unsigned long long d, int &toff, int &n1, int &n, int &o1, int &o2,
+                                          double &g1, double &g2, double &pre2) {
+  toff = (int)(d & 0x7FFFFFFull);
+  n1 = (int)((d >> 27) & 31);
+  n = n1 + (int)((d >> 32) & 31);
+  o1 = (int)((d >> 37) & 1023);
+  o2 = (int)((d >> 47) & 1023) - n1;
+  g1 = ((d >> 57) & 1) ? (((d >> 58) & 1) ? -1.0 : 1.0) : 0.0;
+  g2 = ((d >> 59) & 1) ? 1.0 : 0.0;
+  pre2 = ((d >> 60) & 1) ? .70710678118 : 1.0;
+}
 __global__ __launch_bounds__(64) void k_ph_apply(DevLayout L, const double *__restrict__ Tbuf,
                                                  double *__restrict__ RES, const int *act) {
-  extern __shared__ double S6[];  // ap6_max doubles, then nhalf * nhalf run offsets
+  extern __shared__ double S6[];  // ap6_max doubles
   const int it = xcd_item(L.xs_apply);
   if (it < 0) return;
   const int r = L.apply_rows[it];
@@ -1044,35 +1057,22 @@ __global__ __launch_bounds__(64) void k_ph_apply(DevLayout L, const double *__re
     Tbuf += inst * L.nphterm;
     RES += inst * L.op_size;
   }
-  const int q = L.pair_q[r], base = L.pbase[q], rl = r - base, dim = L.bdim[q], J = L.bJ[q];
+  const int q = L.pair_q[r], rl = r - L.pbase[q], dim = L.bdim[q], J = L.bJ[q];
   const int i = L.pair_a[r], j = L.pair_b[r];
-  const int ji = L.jj[i], jj = L.jj[j], li = L.l[i], lj = L.l[j], ti = L.tz[i], tj = L.tz[j];
-  const int h = L.nhalf, hh = h * h;
-  int *OFF = reinterpret_cast<int *>(S6 + L.ap6_max);
   {
-    const int key = ((ji >> 1) * h + (jj >> 1)) * L.nJ + (J >> 1);
+    const int key = ((L.jj[i] >> 1) * L.nhalf + (L.jj[j] >> 1)) * L.nJ + (J >> 1);
     const int b0 = L.ap6_base[key], len = L.ap6_base[key + 1] - b0;
     for (int e = threadIdx.x; e < L.ap6_max; e += 64) S6[e] = e < len ? L.ap6_val[b0 + e] : 0.0;
-    for (int e = threadIdx.x; e < hh; e += 64) OFF[e] = L.ap6_off[(size_t)key * hh + e];
   }
   __syncthreads();
   const double pre = (i == j) ? .70710678118 : 1.0;
   double *OV = RES + L.v_off + L.boff[q] + (long long)rl * dim;
-  const unsigned *toff = L.ph_toff + L.ph_ubase[r] - rl;
+  const unsigned long long *desc = L.ph_desc + L.ph_ubase[r] - rl;
   for (int s = rl + (int)threadIdx.x; s < dim; s += 64) {
-    const int k = L.pair_a[base + s], l = L.pair_b[base + s];
-    const int jk = L.jj[k], jl = L.jj[l], lk = L.l[k], ll = L.l[l], tk = L.tz[k], tl = L.tz[l];
-    const double *T = Tbuf + toff[s];
-    const int a1 = max(abs(jj - jl), abs(ji - jk)), b1 = min(jj + jl, ji + jk);
-    const int a2 = max(abs(ji - jl), abs(jj - jk)), b2 = min(ji + jl, jj + jk);
-    const int n1 = b1 >= a1 ? (b1 - a1) / 2 + 1 : 0, n2 = b2 >= a2 ? (b2 - a2) / 2 + 1 : 0, n = n1 + n2;
-    // term m < n1: -(-1)^((jl+jk+J)/2) {ji jj J; jl jk J'} T1; else {ji jj J; jk jl J'} T2
-    int o1 = OFF[(jl >> 1) * h + (jk >> 1)], o2 = OFF[(jk >> 1) * h + (jl >> 1)];
-    const bool v1 = o1 >= 0 && abs(tl - tj) == abs(ti - tk) && (ll + lj) % 2 == (li + lk) % 2;
-    const bool v2 = o2 >= 0 && abs(tk - tj) == abs(ti - tl) && (lk + lj) % 2 == (li + ll) % 2;
-    const double g1 = v1 ? -(double)sgnp((jl + jk + J) >> 1) : 0.0, g2 = v2 ? 1.0 : 0.0;
-    o1 = v1 ? o1 : 0;
-    o2 = (v2 ? o2 : 0) - n1;
+    int toff, n1, n, o1, o2;
+    double g1, g2, pre2;
+    ph_decode(desc[s], toff, n1, n, o1, o2, g1, g2, pre2);
+    const double *T = Tbuf + toff;
     double acc = 0.0;
     for (int m0 = 0; m0 < n; m0 += 8) {
       double t[8];
@@ -1084,7 +1084,6 @@ __global__ __launch_bounds__(64) void k_ph_apply(DevLayout L, const double *__re
         if (m < n) acc += (m < n1 ? g1 * S6[o1 + m] : g2 * S6[o2 + m]) * t[u];
       }
     }
-    const double pre2 = (k == l) ? .70710678118 : 1.0;
     OV[s] = OV[s] + acc * pre * pre2;
   }
 }
@@ -1096,7 +1095,7 @@ void launch_ph_terms(hipStream_t s, const DevLayout &L, const double *Z, double 
 void launch_ph_apply(hipStream_t s, const DevLayout &L, const double *Tbuf, double *RES, const int *act,
                      int nact) {
   if (L.nphterm <= 0 || nact <= 0) return;
-  const size_t lds = sizeof(double) * L.ap6_max + sizeof(int) * L.nhalf * L.nhalf;
+  const size_t lds = sizeof(double) * L.ap6_max;
   hipLaunchKernelGGL(k_ph_apply, dim3(8 * L.xs_apply.len, nact), dim3(64), lds, s, L, Tbuf, RES, act);
 }
 

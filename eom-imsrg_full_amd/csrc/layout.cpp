@@ -412,6 +412,46 @@ std::string Layout::build(int nsp_, const int *n_, const int *l_, const int *jj_
     ap6_base[nk] = (int)ap6_val.size();
     ap6_max = std::max(ap6_max, 2 * nJ);  // room for an empty run read from position 0
   }
+  // Per upper element (ij,kl;J) of the ph apply, everything its output needs
+  // beyond the row: bits 0-26 first T1 term of its quadruple, 27-31 n1 (T1
+  // terms), 32-36 n2 (T2 terms), 37-46 / 47-56 start of the {ji jj J; jl jk J'}
+  // / {ji jj J; jk jl J'} runs in the row key's packed 6j list, 57 T1 sum used,
+  // 58 its sign is -1, 59 T2 sum used, 60 k == l (1/sqrt 2).
+  {
+    const int h = nhalf;
+    ph_desc.assign(ph_toff.size(), 0);
+    for (int qq = 0; qq < nblocks; qq++) {
+      const int dim = bdim[qq], base = pbase[qq], J = bJ[qq];
+      for (int r = 0; r < dim; r++) {
+        const int i = pair_a[base + r], j = pair_b[base + r];
+        const int ji = jj[i], jjv = jj[j];
+        const int key = ((ji >> 1) * h + (jjv >> 1)) * nJ + (J >> 1);
+        for (int s2 = r; s2 < dim; s2++) {
+          const int k = pair_a[base + s2], l2 = pair_b[base + s2];
+          const int jk = jj[k], jl = jj[l2];
+          const int a1 = std::max(std::abs(jjv - jl), std::abs(ji - jk)), b1 = std::min(jjv + jl, ji + jk);
+          const int a2 = std::max(std::abs(ji - jl), std::abs(jjv - jk)), b2 = std::min(ji + jl, jjv + jk);
+          const uint64_t n1 = b1 >= a1 ? (b1 - a1) / 2 + 1 : 0, n2 = b2 >= a2 ? (b2 - a2) / 2 + 1 : 0;
+          int o1 = ap6_off[(size_t)key * h * h + (jl >> 1) * h + (jk >> 1)];
+          int o2 = ap6_off[(size_t)key * h * h + (jk >> 1) * h + (jl >> 1)];
+          const bool v1 = o1 >= 0 && std::abs(tz[l2] - tz[j]) == std::abs(tz[i] - tz[k]) &&
+                          (l[l2] + l[j]) % 2 == (l[i] + l[k]) % 2;
+          const bool v2 = o2 >= 0 && std::abs(tz[k] - tz[j]) == std::abs(tz[i] - tz[l2]) &&
+                          (l[k] + l[j]) % 2 == (l[i] + l[l2]) % 2;
+          o1 = v1 ? o1 : 0;
+          o2 = v2 ? o2 : 0;
+          const size_t u = ph_ubase[base + r] + (s2 - r);
+          const uint64_t toff = ph_toff[u];
+          if (toff >= (1ull << 27) || n1 > 31 || n2 > 31 || o1 > 1023 || o2 > 1023)
+            return "basis too large for the ph apply descriptors";
+          const bool neg1 = ((((jl + jk + J) >> 1) & 1) == 0);  // -(-1)^((jl+jk+J)/2) = -1
+          ph_desc[u] = toff | (n1 << 27) | (n2 << 32) | ((uint64_t)o1 << 37) | ((uint64_t)o2 << 47) |
+                       ((uint64_t)v1 << 57) | ((uint64_t)neg1 << 58) | ((uint64_t)v2 << 59) |
+                       ((uint64_t)(k == l2) << 60);
+        }
+      }
+    }
+  }
   // neq: the length of the vectorize stream (IMSRG_ODE.f90:36-44)
   {
     int64_t A = nhole, Pp = npart;
