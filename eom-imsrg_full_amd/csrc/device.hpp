@@ -115,15 +115,23 @@ void launch_ph_terms(hipStream_t s, const DevLayout &L, const double *Z, double 
                      const int *act, int nact);
 void launch_ph_apply(hipStream_t s, const DevLayout &L, const double *Tbuf, double *RES, const int *act,
                      int nact);
-// upper += herm * (strict lower)^T, then lower = herm * upper (diagonal zeroed for herm = -1)
-void launch_sym_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm, const int *act, int nact);
+// upper += herm * (strict lower)^T, then lower = herm * upper (diagonal zeroed for herm = -1).
+// Series epilogue (acc non-null): acc += coef * RES over every channel block;
+// npart non-null: one sum of squares of the final RES per tile pair (nmirror +
+// nob partials per instance, the rest written by launch_ob_final)
+void launch_sym_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm, const int *act, int nact,
+                       double *acc = nullptr, double coef = 0.0, double *npart = nullptr);
 // 111 + 121 + 221 one-body outputs in two launches: the 121/221 partial sums
 // (Wpp = pp intermediate of launch_seg_gemm_pphh; part: nsame * ob_nchunk
 // doubles per instance), then 111 plus the partials into F (and its mirror)
 void launch_ob_partial(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop,
                        const double *Wpp, double *part, int hL, int hR, unsigned mask, const int *act, int nact);
 void launch_ob_final(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, const double *part,
-                     double *RES, int hres, unsigned mask, const int *act, int nact);
+                     double *RES, int hres, unsigned mask, const int *act, int nact, double *acc = nullptr,
+                     double coef = 0.0, double *npart = nullptr);
+// sqrt of the sum of the nmirror + nob partials into out[inst * kScal]
+void launch_norm_from_parts(hipStream_t s, const DevLayout &L, const double *npart, double *out, const int *act,
+                            int nact);
 // E0 = 110 + 220 into RES[inst * rstride]
 void launch_e0(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES,
                int hL, int hR, unsigned mask, double *partials, const int *act, int nact, long long rstride);

@@ -50,6 +50,7 @@ struct imsrg_ctx_s {
   double *tph = nullptr;    // ph J' terms
   double *partials = nullptr;
   double *obpart = nullptr;  // one-body 121/221 partial sums, nsame * ob_nchunk per instance
+  double *npart = nullptr;   // sums of squares of a commutator result, nmirror + nob per instance
   double *scal = nullptr;   // device scalars, kScal per instance
   double *hscal = nullptr;  // pinned host mirror of scal
   double *mono = nullptr;
@@ -79,7 +80,8 @@ struct imsrg_ctx_s {
   // hipGraph replay of whole commutators: one executable graph per distinct
   // (operands, herms, mask, Pandya reuse, instance table, precision)
   bool graphs = false;
-  typedef std::tuple<const void *, const void *, const void *, int, int, int, unsigned, int, const void *, int, int>
+  typedef std::tuple<const void *, const void *, const void *, int, int, int, unsigned, int, const void *, int, int,
+                     const void *, double, int>
       GraphKey;
   std::map<GraphKey, hipGraphExec_t> gexec;
   long long graph_launches = 0, graph_builds = 0;
@@ -416,6 +418,7 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
   if (!rc) rc = dalloc(c, Ly.ph_terms.size() * B, &c->tph);
   if (!rc) rc = dalloc(c, kPartials * B, &c->partials);
   if (!rc) rc = dalloc(c, (size_t)std::max(D.nsame, 1) * D.ob_nchunk * B, &c->obpart);
+  if (!rc) rc = dalloc(c, (size_t)(D.nmirror + D.nob) * B, &c->npart);
   if (!rc) rc = dalloc(c, kScal * B, &c->scal);
   if (!rc) rc = dalloc(c, (size_t)Ly.nsp * Ly.nsp * B, &c->mono);
   if (!rc) rc = dalloc(c, (size_t)Ly.nsp * Ly.nsp, &c->mbar);
@@ -763,11 +766,19 @@ int imsrg_normal_order(imsrg_op H) {
 }
 
 // ---------------------------------------------------------------- commutator
+// Series epilogue of a nested commutator in BCH_EXPAND / MAGNUS_EXPAND:
+// acc += coef * RES (add_sq_op, basic_IMSRG.f90:3450-3478, fused into the
+// final passes over RES) and, with norm, mat_frob_norm(RES) into scalar slot 0.
+struct CommEpi {
+  double *acc = nullptr;
+  double coef = 0.0;
+  bool norm = false;
+};
 // lcc_ready: cc[0] already holds the Pandya matrices of L (drivers reuse them
 // across the nested commutators of one BCH / Magnus series, where L = Omega).
 // Every launch covers the instances of `a`.
 static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *RES, unsigned mask,
-                         bool lcc_ready, const ActSet &a);
+                         bool lcc_ready, const ActSet &a, const CommEpi &epi = CommEpi());
 
 // Tile-shape autotuner: on the first FP64 commutator of a context, each 222
 // GEMM is timed (HIP events, best of 3) in both FP64 tile shapes on the
@@ -824,20 +835,20 @@ static int autotune(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, const ActSet &
 // nodes only; operands, workspaces and the instance table are fixed device
 // addresses, so a replay is the same work as the eager sequence.
 static int comm_impl(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *RES, unsigned mask,
-                     bool lcc_ready, const ActSet &a) {
+                     bool lcc_ready, const ActSet &a, const CommEpi &epi = CommEpi()) {
   if (c->prec == IMSRG_PREC_FP64 && c->shape_pp < 0 && a.n() > 0 && (mask & (T222PPHH | T221 | T222PH))) {
     int rc = autotune(c, L, R, a);
     if (rc) return rc;
   }
-  if (!c->graphs || c->prof_on || a.n() == 0) return comm_launches(c, L, R, RES, mask, lcc_ready, a);
+  if (!c->graphs || c->prof_on || a.n() == 0) return comm_launches(c, L, R, RES, mask, lcc_ready, a, epi);
   const imsrg_ctx_s::GraphKey key(L->d, R->d, RES->d, L->herm, R->herm, RES->herm, mask, lcc_ready ? 1 : 0, a.d,
-                                  a.n(), c->prec);
+                                  a.n(), c->prec, epi.acc, epi.coef, epi.norm ? 1 : 0);
   auto it = c->gexec.find(key);
   if (it == c->gexec.end()) {
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
-    int rc = comm_launches(c, L, R, RES, mask, lcc_ready, a);
+    int rc = comm_launches(c, L, R, RES, mask, lcc_ready, a, epi);
     c->capturing = false;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc) {
@@ -858,7 +869,7 @@ static int comm_impl(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *R
 }
 
 static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *RES, unsigned mask,
-                         bool lcc_ready, const ActSet &a) {
+                         bool lcc_ready, const ActSet &a, const CommEpi &epi) {
   hipStream_t s = c->stream;
   const DevLayout &D = c->dl;
   const int hL = L->herm, hR = R->herm, hres = RES->herm;
@@ -904,8 +915,11 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
     HIPCHK(hipStreamWaitEvent(s, c->ev_join, 0));
   }
   if (mask & T222PH) PROF(c, IMSRG_PROF_PH_APPLY, launch_ph_apply(s, D, c->tph, RES->d, ad, na));
-  PROF(c, IMSRG_PROF_MIRROR, launch_sym_mirror(s, D, RES->d, hres, ad, na));
-  PROF(c, IMSRG_PROF_ONEBODY, launch_ob_final(s, D, L->d, R->d, c->obpart, RES->d, hres, mask, ad, na));
+  double *np = epi.norm ? c->npart : nullptr;
+  PROF(c, IMSRG_PROF_MIRROR, launch_sym_mirror(s, D, RES->d, hres, ad, na, epi.acc, epi.coef, np));
+  PROF(c, IMSRG_PROF_ONEBODY,
+       launch_ob_final(s, D, L->d, R->d, c->obpart, RES->d, hres, mask, ad, na, epi.acc, epi.coef, np));
+  if (epi.norm) PROF(c, IMSRG_PROF_NORM, launch_norm_from_parts(s, D, c->npart, c->scal, ad, na));
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1069,7 +1083,6 @@ static int bch_impl(imsrg_ctx_s *c, imsrg_op_s *HS, imsrg_op_s *G, imsrg_op_s *H
   op_copy(H, AD, a);  // AD_1 = H
   AD->herm = 1;
   INT2->herm = 1;
-  const long long OS = c->lay.op_size;
   launch_pandya(c->stream, c->dl, G->d, c->cc[0], a.d, a.n());
   std::vector<int> pos(c->nbatch, -1);
   for (int k = 0; k < a.n(); k++) pos[a.h[k]] = k;
@@ -1078,11 +1091,15 @@ static int bch_impl(imsrg_ctx_s *c, imsrg_op_s *HS, imsrg_op_s *G, imsrg_op_s *H
   int iw;
   for (iw = 2; iw <= 40; iw++) {
     coef = coef / (iw - 1.0);
-    int rc = comm_impl(c, G, AD, INT2, IMSRG_TALL, true, cur);
+    // INT2 = [G, AD]; HS += coef * INT2 and ||INT2|| in the commutator's final passes
+    CommEpi epi;
+    epi.acc = HS->d;
+    epi.coef = coef;
+    epi.norm = true;
+    int rc = comm_impl(c, G, AD, INT2, IMSRG_TALL, true, cur, epi);
     if (rc) return rc;
-    PROF(c, IMSRG_PROF_AXPBY, launch_axpby(c->stream, OS, 1.0, HS->d, coef, INT2->d, HS->d, cur.d, cur.n(), OS));
     std::vector<double> nrm;
-    rc = op_norms(INT2, cur, nrm);
+    rc = read_scalars(c, 0, cur, nrm);
     if (rc) return rc;
     std::vector<int> next;
     for (int k = 0; k < cur.n(); k++) {
@@ -1114,7 +1131,6 @@ static int magexp_impl(imsrg_ctx_s *c, imsrg_op_s *DG, imsrg_op_s *G, const ActS
   if (a.n() == 0) return 0;
   imsrg_op_s *cur = tmp_op(c, 0, DG->herm), *oth = tmp_op(c, 1, DG->herm);
   if (!cur || !oth) return IMSRG_ENOMEM;
-  const long long OS = c->lay.op_size;
   op_copy(DG, cur, a);
   op_copy(DG, oth, a);
   std::vector<double> ad0, fullnorm;
@@ -1147,12 +1163,15 @@ static int magexp_impl(imsrg_ctx_s *c, imsrg_op_s *DG, imsrg_op_s *G, const ActS
     if (keep.empty()) break;
     if (keep.size() != live.size()) curset = to_set(keep, false);
     live = keep;
-    rc = comm_impl(c, G, cur, oth, IMSRG_TALL & ~(T110 | T220), true, curset);
+    // oth = [G, cur]; DG += cof * oth and ||oth|| in the commutator's final passes
+    CommEpi epi;
+    epi.acc = DG->d;
+    epi.coef = cof[i - 1];
+    epi.norm = true;
+    rc = comm_impl(c, G, cur, oth, IMSRG_TALL & ~(T110 | T220), true, curset, epi);
     if (rc) return rc;
-    PROF(c, IMSRG_PROF_AXPBY,
-         launch_axpby(c->stream, OS, 1.0, DG->d, cof[i - 1], oth->d, DG->d, curset.d, curset.n(), OS));
     std::vector<double> nrm;
-    rc = op_norms(oth, curset, nrm);
+    rc = read_scalars(c, 0, curset, nrm);
     if (rc) return rc;
     for (size_t m = 0; m < live.size(); m++) {
       advals[live[m]][i - 1] = nrm[m] * std::fabs(cof[i - 1]);

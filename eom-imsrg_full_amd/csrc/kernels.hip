@@ -1109,11 +1109,15 @@ void launch_ph_apply(hipStream_t s, const DevLayout &L, const double *Tbuf, doub
 // 388-395, 915-923).  32x32 tile pairs go through LDS so all reads and
 // writes are row-contiguous.
 __global__ __launch_bounds__(256) void k_sym_mirror(DevLayout L, const int4 *__restrict__ tiles, double *RES,
-                                                    int herm, const int *act) {
-  RES += inst_id(act) * L.op_size;
+                                                    int herm, const int *act, double *acc, double coef,
+                                                    double *npart, long long nstride) {
+  __shared__ double sh[256];
+  const long long inst = inst_id(act);
+  RES += inst * L.op_size;
   const int4 t = tiles[blockIdx.x];  // x = block, y = tile row, z = tile col (y <= z)
   const int q = t.x, dim = L.bdim[q];
   double *OV = RES + L.v_off + L.boff[q];
+  double *AV = acc ? acc + inst * L.op_size + L.v_off + L.boff[q] : nullptr;
   __shared__ double U[32][33], W[32][33];
   const int r0 = t.y * 32, c0 = t.z * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
@@ -1126,6 +1130,7 @@ __global__ __launch_bounds__(256) void k_sym_mirror(DevLayout L, const int4 *__r
   }
   __syncthreads();
   const bool diag = t.y == t.z;
+  double sq = 0.0;
   // upper: element (r0+k, c0+tx)
   for (int k = ty; k < 32; k += 8) {
     const int r = r0 + k, c = c0 + tx;
@@ -1133,18 +1138,33 @@ __global__ __launch_bounds__(256) void k_sym_mirror(DevLayout L, const int4 *__r
       double v = U[k][tx];
       if (!diag || k < tx) v += h * W[tx][k];
       else if (herm == -1) v = 0.0;
-      OV[(long long)r * dim + c] = v;
+      const long long o = (long long)r * dim + c;
+      OV[o] = v;
+      sq += v * v;
+      if (AV) AV[o] = AV[o] + coef * v;
     }
   }
   // lower: element (c0+k, r0+tx) = herm * upper (r0+tx, c0+k)
   for (int k = ty; k < 32; k += 8) {
     const int r = c0 + k, c = r0 + tx;
-    if (r < dim && c < dim && (!diag || tx < k)) OV[(long long)r * dim + c] = h * (U[tx][k] + h * W[k][tx]);
+    if (r < dim && c < dim && (!diag || tx < k)) {
+      const double v = h * (U[tx][k] + h * W[k][tx]);
+      const long long o = (long long)r * dim + c;
+      OV[o] = v;
+      sq += v * v;
+      if (AV) AV[o] = AV[o] + coef * v;
+    }
+  }
+  if (npart) {
+    sq = block_sum256(sq, sh);
+    if (threadIdx.x == 0) npart[inst * nstride + blockIdx.x] = sq;
   }
 }
-void launch_sym_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm, const int *act, int nact) {
+void launch_sym_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm, const int *act, int nact,
+                       double *acc, double coef, double *npart) {
   if (L.nmirror <= 0 || nact <= 0) return;
-  hipLaunchKernelGGL(k_sym_mirror, dim3(L.nmirror, nact), dim3(256), 0, s, L, L.mirror_tiles, RES, herm, act);
+  hipLaunchKernelGGL(k_sym_mirror, dim3(L.nmirror, nact), dim3(256), 0, s, L, L.mirror_tiles, RES, herm, act, acc,
+                     coef, npart, (long long)(L.nmirror + L.nob));
 }
 
 // ------------------------------------------------------------------ one-body (111 + 121 + 221)
@@ -1242,16 +1262,15 @@ __global__ __launch_bounds__(256) void k_ob_partial(DevLayout L, const double *_
 // 121 + 221 partials / (2j_p + 1); writes F(p,q) and F(q,p) = herm F(p,q)
 __global__ __launch_bounds__(64) void k_ob_final(DevLayout L, const double *__restrict__ Lop,
                                                  const double *__restrict__ Rop, const double *__restrict__ part,
-                                                 double *__restrict__ RES, int hres, unsigned mask, const int *act) {
+                                                 double *__restrict__ RES, int hres, unsigned mask, const int *act,
+                                                 double *acc, double coef, double *npart) {
   const int t = blockIdx.x;
   if (t >= L.nob) return;
-  {
-    const long long inst = inst_id(act);
-    Lop += inst * L.op_size;
-    Rop += inst * L.op_size;
-    RES += inst * L.op_size;
-    part += inst * (long long)L.nsame * L.ob_nchunk;
-  }
+  const long long inst = inst_id(act);
+  Lop += inst * L.op_size;
+  Rop += inst * L.op_size;
+  RES += inst * L.op_size;
+  part += inst * (long long)L.nsame * L.ob_nchunk;
   const int lane = threadIdx.x, p = L.ob_p[t], qo = L.ob_q[t], N = L.nsp;
   const double *FL = Lop + L.f_off, *FR = Rop + L.f_off;
   double val = 0.0;
@@ -1266,9 +1285,18 @@ __global__ __launch_bounds__(64) void k_ob_final(DevLayout L, const double *__re
       val += sm / (L.jj[p] + 1.0);
     }
     double *F = RES + L.f_off;
+    const double vt = (p != qo) ? hres * val : 0.0;  // F(q,p)
+    if (p == qo && hres == -1) val = 0.0;
     F[p * N + qo] = val;
-    if (p != qo) F[qo * N + p] = hres * val;
-    else if (hres == -1) F[p * N + p] = 0.0;
+    if (p != qo) F[qo * N + p] = vt;
+    if (acc) {
+      double *A = acc + inst * L.op_size;
+      double *AF = A + L.f_off;
+      AF[p * N + qo] = AF[p * N + qo] + coef * val;
+      if (p != qo) AF[qo * N + p] = AF[qo * N + p] + coef * vt;
+      if (t == 0) A[0] = A[0] + coef * RES[0];  // E0 (as add_sq_op)
+    }
+    if (npart) npart[inst * (long long)(L.nmirror + L.nob) + L.nmirror + t] = val * val + vt * vt;
   }
 }
 void launch_ob_partial(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop,
@@ -1278,9 +1306,11 @@ void launch_ob_partial(hipStream_t s, const DevLayout &L, const double *Lop, con
                      hL, hR, mask, act);
 }
 void launch_ob_final(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, const double *part,
-                     double *RES, int hres, unsigned mask, const int *act, int nact) {
+                     double *RES, int hres, unsigned mask, const int *act, int nact, double *acc, double coef,
+                     double *npart) {
   if (L.nob <= 0 || nact <= 0) return;
-  hipLaunchKernelGGL(k_ob_final, dim3(L.nob, nact), dim3(64), 0, s, L, Lop, Rop, part, RES, hres, mask, act);
+  hipLaunchKernelGGL(k_ob_final, dim3(L.nob, nact), dim3(64), 0, s, L, Lop, Rop, part, RES, hres, mask, act, acc,
+                     coef, npart);
 }
 
 // ------------------------------------------------------------------ reductions
@@ -1365,11 +1395,11 @@ __global__ __launch_bounds__(256) void k_sumsq_partial(const double *x, long lon
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 __global__ __launch_bounds__(256) void k_final_sum(const double *partials, int n, double *out, int do_sqrt,
-                                                    const int *act) {
+                                                    const int *act, long long pstride) {
   __shared__ double sh[256];
   {
     const long long inst = inst_id(act);
-    partials += inst * kPartials;
+    partials += inst * pstride;
     out += inst * kScal;
   }
   double s = 0.0;
@@ -1383,7 +1413,17 @@ void launch_sumsq(hipStream_t s, const DevLayout &L, const double *op, double *p
   const long long n = L.op_size - L.f_off;
   hipLaunchKernelGGL(k_sumsq_partial, dim3(kPartials, nact), dim3(256), 0, s, op + L.f_off, n, partials, act,
                      L.op_size);
-  hipLaunchKernelGGL(k_final_sum, dim3(1, nact), dim3(256), 0, s, partials, kPartials, out, 1, act);
+  hipLaunchKernelGGL(k_final_sum, dim3(1, nact), dim3(256), 0, s, partials, kPartials, out, 1, act,
+                     (long long)kPartials);
+}
+
+// the Frobenius norm from the partial sums of squares that k_sym_mirror and
+// k_ob_final wrote (nmirror + nob per instance), into out[inst * kScal]
+void launch_norm_from_parts(hipStream_t s, const DevLayout &L, const double *npart, double *out, const int *act,
+                            int nact) {
+  if (nact <= 0) return;
+  hipLaunchKernelGGL(k_final_sum, dim3(1, nact), dim3(256), 0, s, npart, L.nmirror + L.nob, out, 1, act,
+                     (long long)(L.nmirror + L.nob));
 }
 
 // mbpt2 (basic_IMSRG.f90:4030-4064): one workgroup per block
@@ -1415,7 +1455,8 @@ void launch_mbpt2(hipStream_t s, const DevLayout &L, const double *op, double *p
                   const int *act, int nact) {
   if (nact <= 0) return;
   hipLaunchKernelGGL(k_mbpt2_partial, dim3(L.nblocks, nact), dim3(256), 0, s, L, op, partials, act);
-  hipLaunchKernelGGL(k_final_sum, dim3(1, nact), dim3(256), 0, s, partials, L.nblocks, out, 0, act);
+  hipLaunchKernelGGL(k_final_sum, dim3(1, nact), dim3(256), 0, s, partials, L.nblocks, out, 0, act,
+                     (long long)kPartials);
 }
 
 // ------------------------------------------------------------------ generators

@@ -45,24 +45,29 @@ def test_graph_flow_matches_eager(e4):
     B, E = e4
     H = synthetic_hamiltonian(B, seed=0)
 
-    def run(graphs, ctx):
+    def run(graphs, ctx, stats):
         ctx.set_graphs(graphs)
         HS = ctx.op(1, H).normal_order()
         f = ctx.flow(HS, "white", 0.5)
         recs = []
         for _ in range(6):
             recs.append(f.step())
+            stats.append(ctx.graph_stats())
         f.close()
         return recs, HS.download()
 
     ctx = E.Context(B)
-    r0, h0 = run(False, ctx)
-    r1, h1 = run(True, ctx)
+    r0, h0 = run(False, ctx, [])
+    stats = []
+    r1, h1 = run(True, ctx, stats)
     assert [r["e0"] for r in r0] == [r["e0"] for r in r1]
     assert np.array_equal(h0, h1)
-    builds, launches = ctx.graph_stats()
-    print(f"[graph] flow: {builds} graphs built, {launches} replays", flush=True)
-    assert launches > 4 * builds
+    builds, launches = stats[-1]
+    print(f"[graph] flow: {builds} graphs built, {launches} replays; per step {stats}", flush=True)
+    # one graph per nested commutator of a series (the series coefficient is
+    # part of the key): once the series lengths have been seen, steps only
+    # replay graphs
+    assert stats[-1][0] == stats[-3][0] and launches > 2 * builds
 
 
 def test_graph_batched_flow_matches_eager(e4):
