@@ -49,8 +49,8 @@ struct DevLayout {
   const int4 *mirror_tiles;  // (block, tile row, tile col) with row <= col, 32x32 tiles
   int nmirror;
   long long f_off, v_off, op_size, vsize, ccsize, cwsize;  // per-instance strides
-  const int *apply_rows;      // k_ph_apply row order: pair rows sorted by (i, j, J)
-  XcdSplit xs_rows, xs_terms, xs_apply;  // k_122 rows, k_ph_terms chunks of 256 terms, apply_rows
+  const int2 *apply_chunks;   // k_ph_apply work items: (row, first output) per 64 outputs, rows in (i, j, J) order
+  XcdSplit xs_rows, xs_terms, xs_apply;  // k_122 rows, k_ph_terms chunks of 256 terms, apply_chunks
 };
 
 // A plain GEMM problem C = A * B (A M x K, B K x N, C M x N, all row-major)

@@ -393,13 +393,18 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
       if (Ly.pair_b[x] != Ly.pair_b[y]) return Ly.pair_b[x] < Ly.pair_b[y];
       return Ly.bJ[Ly.pair_q[x]] < Ly.bJ[Ly.pair_q[y]];
     });
-    std::vector<double> wa(Ly.npairs);
+    // one work item per 64 upper outputs of a row, rows in (i, j, J) order
+    std::vector<int2> chunks;
+    std::vector<double> wa;
     for (int k = 0; k < Ly.npairs; k++) {
-      const int r = arow[k], q = Ly.pair_q[r];
-      wa[k] = 64.0 + (Ly.bdim[q] - (r - Ly.pbase[q]));
+      const int r = arow[k], q = Ly.pair_q[r], rl = r - Ly.pbase[q];
+      for (int s0 = rl; s0 < Ly.bdim[q]; s0 += 64) {
+        chunks.push_back(make_int2(r, s0));
+        wa.push_back(16.0 + std::min(64, Ly.bdim[q] - s0));
+      }
     }
     D.xs_apply = make_split(wa);
-    UP(arow, apply_rows);
+    UP(chunks, apply_chunks);
   }
   std::vector<int4> mtiles;
   for (int q = 0; q < Ly.nblocks; q++) {

@@ -1029,9 +1029,10 @@ __global__ __launch_bounds__(256) void k_ph_terms(DevLayout L, const double *__r
 // The 6j symbols a row can use, {ji jj J; x y J'} with triangle (x, y, J) and
 // J' in the triangle ranges of the two J' sums, are one packed list per
 // (ji, jj, J), staged in LDS once per row (zero-padded to the longest list).
-// Rows run in (i, j, J) order, split into 8 contiguous ranges (one per XCD):
-// the rows of one orbit pair read the same quadruples' J' terms, so the
-// repeats hit L2.  What each output needs beyond its row (where its J' terms
+// Work items are 64 outputs of one row; rows run in (i, j, J) order, split
+// into 8 contiguous ranges (one per XCD): the rows of one orbit pair read the
+// same quadruples' J' terms, so the repeats hit L2.  What each output needs
+// beyond its row (where its J' terms
 // start, how many, where its 6j runs start in the list, signs) is one 64-bit
 // host-built descriptor.  Each output's two J' ranges are one contiguous run
 // in Tbuf, loaded 8 at a time so a thread keeps several loads in flight.
@@ -1051,7 +1052,8 @@ __global__ __launch_bounds__(64) void k_ph_apply(DevLayout L, const double *__re
   extern __shared__ double S6[];  // ap6_max doubles
   const int it = xcd_item(L.xs_apply);
   if (it < 0) return;
-  const int r = L.apply_rows[it];
+  const int2 ch = L.apply_chunks[it];
+  const int r = ch.x;
   {
     const long long inst = inst_id(act);
     Tbuf += inst * L.nphterm;
@@ -1068,7 +1070,9 @@ __global__ __launch_bounds__(64) void k_ph_apply(DevLayout L, const double *__re
   const double pre = (i == j) ? .70710678118 : 1.0;
   double *OV = RES + L.v_off + L.boff[q] + (long long)rl * dim;
   const unsigned long long *desc = L.ph_desc + L.ph_ubase[r] - rl;
-  for (int s = rl + (int)threadIdx.x; s < dim; s += 64) {
+  {
+    const int s = ch.y + (int)threadIdx.x;
+    if (s >= dim) return;
     int toff, n1, n, o1, o2;
     double g1, g2, pre2;
     ph_decode(desc[s], toff, n1, n, o1, o2, g1, g2, pre2);
