@@ -446,6 +446,7 @@ def main():
             "precision": head, "first_timed_s": recs[0]["s"], "E0_last": recs[-1]["e0"],
             "variants": variants,
             "speedup_vs_fp64": (steps_per_s / variants["fp64"]["steps_per_s"]) if "fp64" in variants else None,
+            "fp64_tile_shapes": ctx.tuning(),
             "batch": batch,
         },
         "roofline": roof,

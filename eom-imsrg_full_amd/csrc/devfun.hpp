@@ -15,7 +15,10 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 
 // Instance of a batched launch: blockIdx.y indexes the active-instance table
 // (null table: one instance, number 0).  Instance k of an operator, Pandya
-// matrix or workspace starts k strides after the buffer base.
+// matrix or workspace starts k strides after the buffer base.  A negative
+// entry marks an instance that has left a BCH or Magnus series (the series
+// gate kernels write it): every kernel returns at once for such a workgroup
+// row, before any barrier.
 __device__ __forceinline__ long long inst_id(const int *act) { return act ? act[blockIdx.y] : 0; }
 
 __device__ __forceinline__ int xcd_item(const XcdSplit &x) {

@@ -147,6 +147,11 @@ void launch_eta(hipStream_t s, const DevLayout &L, const double *Hden, const dou
 void launch_normal_order(hipStream_t s, const DevLayout &L, const double *T, double *H, const int *act,
                          int nact);
 void launch_gather_e0(hipStream_t s, const double *op, long long stride, double *out, const int *act, int nact);
+// truncation test of a BCH (mode 0) or Magnus (mode 1) series after a nested
+// commutator: instances that stop get -1 in the series table `act`; live[k]
+// (host memory) = 1 for the positions that go on
+void launch_series_gate(hipStream_t s, int *act, int nact, double *scal, int mode, double coef, int store,
+                        int test, double conv, double tiny, int *live);
 
 // ---- adjoint commutator (adjoint.hip): Ybar += d<C,[P,Y]>/dY term by term
 void launch_adj_e0(hipStream_t s, const DevLayout &L, const double *P, const double *C, double *Y, int hP, int hY,
@@ -172,5 +177,7 @@ void launch_adj_normal_order(hipStream_t s, const DevLayout &L, const double *Cb
 
 constexpr int kPartials = 1024;  // reduction partials per instance
 constexpr int kScal = 64;        // scalar slots per instance
+constexpr int kAdvSlot = 8;      // Magnus series: advals[0..6] in slots 8..14
+constexpr int kFullSlot = 15;    // Magnus series: ||Omega||
 
 }  // namespace imsrg

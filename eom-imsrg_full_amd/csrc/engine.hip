@@ -41,6 +41,12 @@ struct imsrg_ctx_s {
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool dual = true;        // use stream2 (imsrg_ctx_set_concurrency)
+  // BCH / Magnus series: the gate kernel after nested commutator n writes one
+  // live flag per series position into ring slot n % kGateRing of pinned host
+  // memory and ev_gate marks it; the host reads slot n-1 while commutator n runs
+  static constexpr int kGateRing = 4;
+  int *live_h = nullptr, *live_d = nullptr;  // host and device views, kGateRing * nbatch
+  hipEvent_t ev_gate[kGateRing] = {};
   bool capturing = false;  // graph capture records one stream only
   std::vector<void *> dallocs;
   int nbatch = 1;           // independent instances per operator
@@ -220,9 +226,9 @@ static ActSet all_set(imsrg_ctx_s *c) {
   return a;
 }
 // Writes `list` into table `slot` (an asynchronous copy on the context
-// stream).  Drivers rewrite a slot only after a stream synchronization (a
-// norm read-back), so the pinned source is never overwritten while a copy
-// from it is pending.
+// stream).  Drivers rewrite a slot only after a stream synchronization, so
+// the pinned source is never overwritten while a copy from it is pending.
+// Slots 1 and 2 are the BCH and Magnus series tables (series_set).
 static ActSet make_set(imsrg_ctx_s *c, int slot, const std::vector<int> &list) {
   ActSet a;
   a.d = c->act_d[slot];
@@ -311,6 +317,11 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
     imsrg_ctx_destroy(c);
     return fail(IMSRG_EDEVICE, "second stream / events failed");
   }
+  for (int k = 0; k < imsrg_ctx_s::kGateRing; k++)
+    if (hipEventCreateWithFlags(&c->ev_gate[k], hipEventDisableTiming) != hipSuccess) {
+      imsrg_ctx_destroy(c);
+      return fail(IMSRG_EDEVICE, "series gate events failed");
+    }
   const Layout &Ly = c->lay;
   DevLayout &D = c->dl;
   D.nsp = Ly.nsp; D.nhole = Ly.nhole; D.npart = Ly.npart; D.nblocks = Ly.nblocks; D.ncc = Ly.ncc;
@@ -445,6 +456,10 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
   }
   if (!rc && hipHostMalloc((void **)&c->hscal, kScal * B * sizeof(double), hipHostMallocDefault) != hipSuccess)
     rc = fail(IMSRG_ENOMEM, "hipHostMalloc failed");
+  if (!rc && (hipHostMalloc((void **)&c->live_h, imsrg_ctx_s::kGateRing * B * sizeof(int),
+                            hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+              hipHostGetDevicePointer((void **)&c->live_d, c->live_h, 0) != hipSuccess))
+    rc = fail(IMSRG_ENOMEM, "hipHostMalloc series gate ring failed");
   if (!rc &&
       hipHostMalloc((void **)&c->hstage, std::max<size_t>(Ly.op_size * B, Ly.ccsize) * sizeof(double),
                     hipHostMallocDefault) != hipSuccess)
@@ -559,6 +574,9 @@ int imsrg_ctx_destroy(imsrg_ctx c) {
   if (c->hstage) (void)hipHostFree(c->hstage);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  for (hipEvent_t e : c->ev_gate)
+    if (e) (void)hipEventDestroy(e);
+  if (c->live_h) (void)hipHostFree(c->live_h);
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -789,8 +807,11 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
 // GEMM is timed (HIP events, best of 3) in both FP64 tile shapes on the
 // context's real channel/Pandya blocks and the faster one is kept.  The
 // pp/hh timing writes nothing (no C, no W); the Pandya timing writes the Z
-// workspace, which every commutator recomputes before reading.
-static int autotune(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, const ActSet &a) {
+// workspace, which every commutator recomputes before reading.  The timing
+// covers every instance of the context (a series table may already mark
+// instances as stopped, and skipped workgroups would time nothing).
+static int autotune(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R) {
+  const ActSet a = all_set(c);
   hipStream_t s = c->stream;
   const DevLayout &D = c->dl;
   hipEvent_t e0, e1;
@@ -842,7 +863,7 @@ static int autotune(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, const ActSet &
 static int comm_impl(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *RES, unsigned mask,
                      bool lcc_ready, const ActSet &a, const CommEpi &epi = CommEpi()) {
   if (c->prec == IMSRG_PREC_FP64 && c->shape_pp < 0 && a.n() > 0 && (mask & (T222PPHH | T221 | T222PH))) {
-    int rc = autotune(c, L, R, a);
+    int rc = autotune(c, L, R);
     if (rc) return rc;
   }
   if (!c->graphs || c->prof_on || a.n() == 0) return comm_launches(c, L, R, RES, mask, lcc_ready, a, epi);
@@ -1073,11 +1094,47 @@ int imsrg_generator_atanxx(imsrg_ctx c, imsrg_op Hold, imsrg_op H, imsrg_op ETA)
 static const double kConv = (double)1e-6f;   // real(8), parameter :: conv = 1e-6
 static const double kTiny = (double)1e-9f;   // fullnorm < 1e-9 (default-real literal)
 
+// A BCH or Magnus series works on its own copy of the caller's instance
+// table (slot `slot`), which the gate kernels edit: an instance that stops
+// gets -1 there and every later kernel of the series skips it.  The copy is
+// made on the stream, behind the launches that read the slot before.
+static ActSet series_set(imsrg_ctx_s *c, int slot, const ActSet &a) {
+  ActSet s;
+  s.d = c->act_d[slot];
+  s.h = a.h;
+  (void)hipMemcpyAsync(c->act_d[slot], a.d, a.n() * sizeof(int), hipMemcpyDeviceToDevice, c->stream);
+  return s;
+}
+// The series gate after nested commutator n: the test on the device, its
+// live flags into ring slot n % kGateRing, marked by an event.
+static int series_gate(imsrg_ctx_s *c, int slot, const ActSet &set, int n, int mode, double coef, int store,
+                       int test) {
+  const int r = n % imsrg_ctx_s::kGateRing;
+  launch_series_gate(c->stream, c->act_d[slot], set.n(), c->scal, mode, coef, store, test, kConv, kTiny,
+                     c->live_d + (size_t)r * c->nbatch);
+  HIPCHK(hipEventRecord(c->ev_gate[r], c->stream));
+  return 0;
+}
+// Waits for the live flags of gate n; live[k] = 1 for positions that go on.
+static int series_live(imsrg_ctx_s *c, const ActSet &set, int n, std::vector<int> &live, int &nlive) {
+  const int r = n % imsrg_ctx_s::kGateRing;
+  HIPCHK(hipEventSynchronize(c->ev_gate[r]));
+  const volatile int *f = c->live_h + (size_t)r * c->nbatch;
+  live.assign(set.n(), 0);
+  nlive = 0;
+  for (int k = 0; k < set.n(); k++) nlive += (live[k] = f[k] ? 1 : 0);
+  return 0;
+}
+
 // BCH_EXPAND (IMSRG_MAGNUS.f90:149-234) on the instances of `a`.  Every
-// instance keeps its own truncation: after each nested commutator the
-// instances whose ||ad^n||/n! fell below conv leave the working set, so one
-// launch per kernel serves all instances still expanding.  nterms[k] is the
-// number of nested commutators instance a.h[k] used.
+// instance keeps its own truncation: after each nested commutator the gate
+// kernel drops the instances whose ||ad^n||/n! fell below conv, so one launch
+// per kernel serves all instances still expanding.  The host enqueues nested
+// commutator n+1 before it reads the outcome of the test after n-1 (its
+// flags are ready while n runs), so the device never waits for the host;
+// when every instance has stopped, the one commutator already enqueued
+// beyond the end is skipped by all its workgroups.  nterms[k] is the number
+// of nested commutators instance a.h[k] used.
 static int bch_impl(imsrg_ctx_s *c, imsrg_op_s *HS, imsrg_op_s *G, imsrg_op_s *H, const ActSet &a,
                     std::vector<int> &nterms) {
   imsrg_op_s *AD = tmp_op(c, 0, 1), *INT2 = tmp_op(c, 1, 1);
@@ -1088,13 +1145,12 @@ static int bch_impl(imsrg_ctx_s *c, imsrg_op_s *HS, imsrg_op_s *G, imsrg_op_s *H
   op_copy(H, AD, a);  // AD_1 = H
   AD->herm = 1;
   INT2->herm = 1;
-  launch_pandya(c->stream, c->dl, G->d, c->cc[0], a.d, a.n());
-  std::vector<int> pos(c->nbatch, -1);
-  for (int k = 0; k < a.n(); k++) pos[a.h[k]] = k;
-  ActSet cur = a;
+  const ActSet cur = series_set(c, 1, a);
+  launch_pandya(c->stream, c->dl, G->d, c->cc[0], cur.d, cur.n());
+  std::vector<int> live(a.n(), 1);  // positions in the series for nested commutator iw
+  int nlive = a.n();
   double coef = 1.0;
-  int iw;
-  for (iw = 2; iw <= 40; iw++) {
+  for (int iw = 2; iw <= 40; iw++) {
     coef = coef / (iw - 1.0);
     // INT2 = [G, AD]; HS += coef * INT2 and ||INT2|| in the commutator's final passes
     CommEpi epi;
@@ -1102,22 +1158,18 @@ static int bch_impl(imsrg_ctx_s *c, imsrg_op_s *HS, imsrg_op_s *G, imsrg_op_s *H
     epi.coef = coef;
     epi.norm = true;
     int rc = comm_impl(c, G, AD, INT2, IMSRG_TALL, true, cur, epi);
+    if (!rc) rc = series_gate(c, 1, cur, iw, 0, coef, -1, 0);
+    if (!rc && iw > 2) rc = series_live(c, cur, iw - 1, live, nlive);
     if (rc) return rc;
-    std::vector<double> nrm;
-    rc = read_scalars(c, 0, cur, nrm);
-    if (rc) return rc;
-    std::vector<int> next;
-    for (int k = 0; k < cur.n(); k++) {
-      nterms[pos[cur.h[k]]]++;
-      if (!(nrm[k] * coef < kConv)) next.push_back(cur.h[k]);
-    }
-    if (next.empty()) break;
+    if (nlive == 0) return 0;  // the test after iw-1 stopped every instance
+    for (int k = 0; k < cur.n(); k++) nterms[k] += live[k];
     std::swap(c->tmp[0], c->tmp[1]);
     AD = c->tmp[0];
     INT2 = c->tmp[1];
-    if (next.size() != cur.h.size()) cur = make_set(c, 1, next);
   }
-  if (iw > 40) return fail(IMSRG_ECLIP, "clipping BCH");
+  int rc = series_live(c, cur, 40, live, nlive);
+  if (rc) return rc;
+  if (nlive > 0) return fail(IMSRG_ECLIP, "clipping BCH");
   return 0;
 }
 int imsrg_bch_expand(imsrg_ctx c, imsrg_op HS, imsrg_op G, imsrg_op H, int *nterms) {
@@ -1129,7 +1181,8 @@ int imsrg_bch_expand(imsrg_ctx c, imsrg_op HS, imsrg_op G, imsrg_op H, int *nter
 }
 
 // MAGNUS_EXPAND (IMSRG_MAGNUS.f90:622-693) on the instances of `a`, each with
-// its own early exits; nterms as in bch_impl.
+// its own early exits, tested on the device and read back one nested
+// commutator late as in bch_impl; nterms as in bch_impl.
 static int magexp_impl(imsrg_ctx_s *c, imsrg_op_s *DG, imsrg_op_s *G, const ActSet &a, std::vector<int> &nterms) {
   static const double cof[7] = {1.0, -0.5, .0833333333, 0.0, -0.00138888888, 0.0, 3.306878e-5};
   nterms.assign(a.n(), 0);
@@ -1138,50 +1191,33 @@ static int magexp_impl(imsrg_ctx_s *c, imsrg_op_s *DG, imsrg_op_s *G, const ActS
   if (!cur || !oth) return IMSRG_ENOMEM;
   op_copy(DG, cur, a);
   op_copy(DG, oth, a);
-  std::vector<double> ad0, fullnorm;
-  int rc = op_norms(cur, a, ad0);
-  if (!rc) rc = op_norms(G, a, fullnorm);
+  const ActSet set = series_set(c, 2, a);
+  // advals(1) = ||DG|| and fullnorm = ||G|| into the series scalar slots; the
+  // first gate: fullnorm < 1e-9 returns, then the test on advals(1)
+  PROF(c, IMSRG_PROF_NORM,
+       launch_sumsq(c->stream, c->dl, cur->d, c->partials, c->scal + kAdvSlot, set.d, set.n()));
+  PROF(c, IMSRG_PROF_NORM,
+       launch_sumsq(c->stream, c->dl, G->d, c->partials, c->scal + kFullSlot, set.d, set.n()));
+  int rc = series_gate(c, 2, set, 1, 1, 0.0, -1, 0);
   if (rc) return rc;
-  // per instance (position k in a): advals(1..7)
-  std::vector<std::vector<double>> advals(a.n(), std::vector<double>(7, 0.0));
-  std::vector<int> live;  // positions in a
-  for (int k = 0; k < a.n(); k++) {
-    advals[k][0] = ad0[k];
-    if (!(fullnorm[k] < kTiny)) live.push_back(k);
-  }
-  if (live.empty()) return 0;
-  auto to_set = [&](const std::vector<int> &ks, bool same_as_a) {
-    if (same_as_a) return a;
-    std::vector<int> inst;
-    for (int k : ks) inst.push_back(a.h[k]);
-    return make_set(c, 2, inst);
-  };
-  ActSet curset = to_set(live, (int)live.size() == a.n());
-  launch_pandya(c->stream, c->dl, G->d, c->cc[0], curset.d, curset.n());
-  int q = 1;
+  launch_pandya(c->stream, c->dl, G->d, c->cc[0], set.d, set.n());
+  std::vector<int> live;
+  int nlive = 0;
   for (int i = 2; i <= 7; i++) {
-    const int qprev = q;
-    q = (std::fabs(cof[i - 1]) > kConv) ? 1 : 2;
-    std::vector<int> keep;
-    for (int k : live)
-      if (!(std::fabs(advals[k][i - qprev - 1] / fullnorm[k]) < kConv)) keep.push_back(k);
-    if (keep.empty()) break;
-    if (keep.size() != live.size()) curset = to_set(keep, false);
-    live = keep;
+    // q of the early exit before nested commutator i+1 (advals(i+1-q) is tested)
+    const int q = (std::fabs(cof[i - 1]) > kConv) ? 1 : 2;
     // oth = [G, cur]; DG += cof * oth and ||oth|| in the commutator's final passes
     CommEpi epi;
     epi.acc = DG->d;
     epi.coef = cof[i - 1];
     epi.norm = true;
-    rc = comm_impl(c, G, cur, oth, IMSRG_TALL & ~(T110 | T220), true, curset, epi);
+    rc = comm_impl(c, G, cur, oth, IMSRG_TALL & ~(T110 | T220), true, set, epi);
+    // advals(i) = ||oth|| |cof(i)|, then the test for commutator i+1
+    if (!rc && i < 7) rc = series_gate(c, 2, set, i, 1, std::fabs(cof[i - 1]), i - 1, i - q);
+    if (!rc) rc = series_live(c, set, i - 1, live, nlive);
     if (rc) return rc;
-    std::vector<double> nrm;
-    rc = read_scalars(c, 0, curset, nrm);
-    if (rc) return rc;
-    for (size_t m = 0; m < live.size(); m++) {
-      advals[live[m]][i - 1] = nrm[m] * std::fabs(cof[i - 1]);
-      nterms[live[m]]++;
-    }
+    if (nlive == 0) break;
+    for (int k = 0; k < set.n(); k++) nterms[k] += live[k];
     std::swap(cur, oth);
   }
   return 0;

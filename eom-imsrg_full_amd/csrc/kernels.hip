@@ -19,7 +19,9 @@ namespace imsrg {
 
 // ------------------------------------------------------------------ elementwise
 __global__ void k_fill(double *p, long long n, double v, const int *act, long long stride) {
-  p += inst_id(act) * stride;
+  const long long inst = inst_id(act);
+  if (inst < 0) return;
+  p += inst * stride;
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long st = (long long)gridDim.x * blockDim.x;
   for (; i < n; i += st) p[i] = v;
@@ -34,7 +36,9 @@ void launch_fill(hipStream_t s, double *p, long long n, double v, const int *act
 // add_sq_op (basic_IMSRG.f90:3450-3478): C = A*ax + B*bx elementwise
 __global__ void k_axpby(long long n, double a, const double *A, double b, const double *B, double *C,
                         const int *act, long long stride) {
-  const long long o = inst_id(act) * stride;
+  const long long inst = inst_id(act);
+  if (inst < 0) return;
+  const long long o = inst * stride;
   A += o;
   B += o;
   C += o;
@@ -52,7 +56,9 @@ void launch_axpby(hipStream_t s, long long n, double a, const double *A, double 
 
 // copy_sq_op (basic_IMSRG.f90:2942-2969) on the active instances
 __global__ void k_copy(long long n, const double *A, double *C, const int *act, long long stride) {
-  const long long o = inst_id(act) * stride;
+  const long long inst = inst_id(act);
+  if (inst < 0) return;
+  const long long o = inst * stride;
   A += o;
   C += o;
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -517,6 +523,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == 0 ?
   if (t >= ntiles) return;
   {
     const long long inst = inst_id(act);
+    if (inst < 0) return;
     X0 += inst * sx;
     X1 += inst * sx;
     if (C) C += inst * sc;
@@ -740,6 +747,7 @@ __global__ __launch_bounds__(256) void k_seg_gemm_w(const SegProb *__restrict__ 
   if (t >= ntiles) return;
   {
     const long long inst = inst_id(act);
+    if (inst < 0) return;
     X0 += inst * sx;
     X1 += inst * sx;
     if (C) C += inst * sc;
@@ -823,6 +831,7 @@ __global__ __launch_bounds__(256) void k_pandya(DevLayout L, const double *__res
   if (col >= L.nccol) return;
   {
     const long long inst = inst_id(act);
+    if (inst < 0) return;
     op += inst * L.op_size;
     cc += inst * L.ccsize;
   }
@@ -926,7 +935,9 @@ __global__ __launch_bounds__(64) void k_122(DevLayout L, const double *__restric
   const int r = xcd_item(L.xs_rows);
   if (r < 0) return;
   {
-    const long long o = inst_id(act) * L.op_size;
+    const long long inst = inst_id(act);
+    if (inst < 0) return;
+    const long long o = inst * L.op_size;
     Lop += o;
     Rop += o;
     RES += o;
@@ -973,7 +984,9 @@ void launch_122(hipStream_t s, const DevLayout &L, const double *Lop, const doub
 __global__ __launch_bounds__(64) void k_clear_rows(DevLayout L, double *RES, const int *act) {
   const int r = blockIdx.x;
   if (r >= L.npairs) return;
-  RES += inst_id(act) * L.op_size;
+  const long long inst = inst_id(act);
+  if (inst < 0) return;
+  RES += inst * L.op_size;
   const int q = L.pair_q[r], rl = r - L.pbase[q], dim = L.bdim[q];
   double *OV = RES + L.v_off + L.boff[q] + (long long)rl * dim;
   for (int s = threadIdx.x; s < dim; s += 64) OV[s] = 0.0;
@@ -1008,6 +1021,7 @@ __global__ __launch_bounds__(256) void k_ph_terms(DevLayout L, const double *__r
   if (chunk < 0 || t >= L.nphterm) return;
   {
     const long long inst = inst_id(act);
+    if (inst < 0) return;
     Z += inst * L.cwsize;
     Tbuf += inst * L.nphterm;
   }
@@ -1056,6 +1070,7 @@ __global__ __launch_bounds__(64) void k_ph_apply(DevLayout L, const double *__re
   const int r = ch.x;
   {
     const long long inst = inst_id(act);
+    if (inst < 0) return;
     Tbuf += inst * L.nphterm;
     RES += inst * L.op_size;
   }
@@ -1117,6 +1132,7 @@ __global__ __launch_bounds__(256) void k_sym_mirror(DevLayout L, const int4 *__r
                                                     double *npart, long long nstride) {
   __shared__ double sh[256];
   const long long inst = inst_id(act);
+  if (inst < 0) return;
   RES += inst * L.op_size;
   const int4 t = tiles[blockIdx.x];  // x = block, y = tile row, z = tile col (y <= z)
   const int q = t.x, dim = L.bdim[q];
@@ -1231,6 +1247,7 @@ __global__ __launch_bounds__(256) void k_ob_partial(DevLayout L, const double *_
   if (sp >= L.nsame) return;
   {
     const long long inst = inst_id(act);
+    if (inst < 0) return;
     Lop += inst * L.op_size;
     Rop += inst * L.op_size;
     Wpp += inst * L.vsize;
@@ -1271,6 +1288,7 @@ __global__ __launch_bounds__(64) void k_ob_final(DevLayout L, const double *__re
   const int t = blockIdx.x;
   if (t >= L.nob) return;
   const long long inst = inst_id(act);
+  if (inst < 0) return;
   Lop += inst * L.op_size;
   Rop += inst * L.op_size;
   RES += inst * L.op_size;
@@ -1325,6 +1343,7 @@ __global__ __launch_bounds__(256) void k_e0_partial(DevLayout L, const double *L
   __shared__ double sh[256];
   {
     const long long inst = inst_id(act);
+    if (inst < 0) return;
     Lop += inst * L.op_size;
     Rop += inst * L.op_size;
     partials += inst * kPartials;
@@ -1360,6 +1379,7 @@ __global__ __launch_bounds__(256) void k_e0_final(int nblocks, const double *par
   __shared__ double sh[256];
   {
     const long long inst = inst_id(act);
+    if (inst < 0) return;
     partials += inst * kPartials;
     RES += inst * rstride;
   }
@@ -1389,6 +1409,7 @@ __global__ __launch_bounds__(256) void k_sumsq_partial(const double *x, long lon
   __shared__ double sh[256];
   {
     const long long inst = inst_id(act);
+    if (inst < 0) return;
     x += inst * stride;
     partials += inst * kPartials;
   }
@@ -1403,6 +1424,7 @@ __global__ __launch_bounds__(256) void k_final_sum(const double *partials, int n
   __shared__ double sh[256];
   {
     const long long inst = inst_id(act);
+    if (inst < 0) return;
     partials += inst * pstride;
     out += inst * kScal;
   }
@@ -1436,6 +1458,7 @@ __global__ __launch_bounds__(256) void k_mbpt2_partial(DevLayout L, const double
   __shared__ double sh[256];
   {
     const long long inst = inst_id(act);
+    if (inst < 0) return;
     H += inst * L.op_size;
     partials += inst * kPartials;
   }
@@ -1468,6 +1491,7 @@ void launch_mbpt2(hipStream_t s, const DevLayout &L, const double *op, double *p
 __global__ void k_monopole(DevLayout L, const double *H, double *mono, const int *act) {
   {
     const long long inst = inst_id(act);
+    if (inst < 0) return;
     H += inst * L.op_size;
     mono += inst * L.nsp * L.nsp;
   }
@@ -1492,7 +1516,9 @@ void launch_monopole(hipStream_t s, const DevLayout &L, const double *H, double 
 __global__ void k_eta_onebody(DevLayout L, const double *Hden, const double *Hnum, double *ETA, int kind,
                               const int *act) {
   {
-    const long long o = inst_id(act) * L.op_size;
+    const long long inst = inst_id(act);
+    if (inst < 0) return;
+    const long long o = inst * L.op_size;
     Hden += o;
     Hnum += o;
     ETA += o;
@@ -1514,6 +1540,7 @@ __global__ __launch_bounds__(256) void k_eta_twobody(DevLayout L, const double *
                                                       const double *mono, double *ETA, int kind, const int *act) {
   {
     const long long inst = inst_id(act);
+    if (inst < 0) return;
     Hden += inst * L.op_size;
     Hnum += inst * L.op_size;
     ETA += inst * L.op_size;
@@ -1558,7 +1585,9 @@ void launch_eta(hipStream_t s, const DevLayout &L, const double *Hden, const dou
 __global__ __launch_bounds__(256) void k_no_e0(DevLayout L, const double *T, double *H, const int *act) {
   __shared__ double sh[256];
   {
-    const long long o = inst_id(act) * L.op_size;
+    const long long inst = inst_id(act);
+    if (inst < 0) return;
+    const long long o = inst * L.op_size;
     T += o;
     H += o;
   }
@@ -1579,7 +1608,9 @@ __global__ __launch_bounds__(256) void k_no_e0(DevLayout L, const double *T, dou
 }
 __global__ void k_no_f(DevLayout L, const double *T, double *H, int herm, const int *act) {
   {
-    const long long o = inst_id(act) * L.op_size;
+    const long long inst = inst_id(act);
+    if (inst < 0) return;
+    const long long o = inst * L.op_size;
     T += o;
     H += o;
   }
@@ -1610,10 +1641,49 @@ __global__ void k_gather_e0(const double *op, long long stride, double *out, con
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nact) return;
   const long long inst = act ? act[k] : 0;
+  if (inst < 0) return;
   out[inst * kScal] = op[inst * stride];
 }
 void launch_gather_e0(hipStream_t s, const double *op, long long stride, double *out, const int *act, int nact) {
   if (nact <= 0) return;
   hipLaunchKernelGGL(k_gather_e0, dim3((nact + 63) / 64), dim3(64), 0, s, op, stride, out, act, nact);
+}
+
+// The truncation tests of BCH_EXPAND (IMSRG_MAGNUS.f90:229-230) and
+// MAGNUS_EXPAND (IMSRG_MAGNUS.f90:656-674, 689) made on the device after each
+// nested commutator, so the host can enqueue the next commutator before the
+// norm is known.  Position k of the series table `act` holds instance act[k],
+// or a negative number once that instance has left the series; a leaving
+// instance gets -1, which every later kernel of the series skips.
+//   BCH (mode 0): goes on while !(||ad^n|| * coef < conv).
+//   Magnus (mode 1): advals[store] = ||ad^n|| * coef when store >= 0 (coef =
+//   |cof|; advals in scalar slots kAdvSlot.., ||Omega|| in kFullSlot), then goes
+//   on while !(fullnorm < tiny) and !(|advals[test] / fullnorm| < conv).
+// ||ad^n|| is scalar slot 0.  live[k] (host memory) gets 1 or 0 per position.
+__global__ void k_series_gate(int *act, int nact, double *scal, int mode, double coef, int store, int test,
+                              double conv, double tiny, int *live) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nact) return;
+  const int inst = act[k];
+  int on = 0;
+  if (inst >= 0) {
+    double *sc = scal + (long long)inst * kScal;
+    if (mode == 0) {
+      on = !(sc[0] * coef < conv);
+    } else {
+      if (store >= 0) sc[kAdvSlot + store] = sc[0] * coef;
+      const double full = sc[kFullSlot];
+      on = !(full < tiny) && !(fabs(sc[kAdvSlot + test] / full) < conv);
+    }
+    if (!on) act[k] = -1;
+  }
+  live[k] = on;
+  __threadfence_system();
+}
+void launch_series_gate(hipStream_t s, int *act, int nact, double *scal, int mode, double coef, int store,
+                        int test, double conv, double tiny, int *live) {
+  if (nact <= 0) return;
+  hipLaunchKernelGGL(k_series_gate, dim3((nact + 63) / 64), dim3(64), 0, s, act, nact, scal, mode, coef, store,
+                     test, conv, tiny, live);
 }
 }  // namespace imsrg

@@ -67,6 +67,31 @@ def test_batched_bch_per_instance_truncation(e4):
         assert np.array_equal(out[k], sHS.download())
 
 
+def test_batched_magnus_expand_per_instance_exits(e4):
+    """MAGNUS_EXPAND's early exits are tested per instance on the device: an
+    Omega of zero norm (no term), a tiny one (exits after the first test) and
+    ordinary ones give each instance the terms and the result of its own
+    single-instance run, and a second call on the same context agrees"""
+    B, E = e4
+    scales = (0.0, 1e-9, 1e-3, 0.3)
+    nb = len(scales)
+    G = np.stack([random_operator(B, -1, 50 + k) * s for k, s in enumerate(scales)])
+    eta = np.stack([random_operator(B, -1, 60 + k) for k in range(nb)])
+    bctx = E.Context(B, nbatch=nb)
+    res = []
+    for _ in range(2):
+        DG, nt = bctx.magnus_expand(bctx.op(-1, eta), bctx.op(-1, G))
+        res.append((DG.download(), np.array(nt)))
+    print(f"[batch] Magnus terms per instance {list(res[0][1])}", flush=True)
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+    assert res[0][1][0] == 0 and len(set(res[0][1])) > 1
+    single = E.Context(B)
+    for k in range(nb):
+        sDG, n1 = single.magnus_expand(single.op(-1, eta[k]), single.op(-1, G[k]))
+        assert n1 == res[0][1][k]
+        assert np.array_equal(res[0][0][k], sDG.download())
+
+
 def test_batched_flow_matches_single_flows(e4):
     B, E = e4
     nb = 4
