@@ -32,7 +32,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "eom-imsrg_full_amd"))
 
-PEAK = {"fp64_mfma_tflops": 78.6, "fp32_mfma_tflops": 157.3, "hbm_gbs": 8000.0}
+PEAK = {"fp64_mfma_tflops": 78.6, "fp32_mfma_tflops": 157.3, "int8_mfma_tops": 5000.0, "hbm_gbs": 8000.0}
 
 
 def log(*a):
@@ -47,7 +47,7 @@ def parse():
     p.add_argument("--emax", type=int, default=8)
     p.add_argument("--holes", type=int, default=6, help="filled orbits per isospin")
     p.add_argument("--generator", default="white", choices=["white", "atan", "w2"])
-    p.add_argument("--precision", default="all", choices=["all", "fp64", "split32", "bf16"])
+    p.add_argument("--precision", default="all", choices=["all", "fp64", "split32", "bf16", "int8x5"])
     p.add_argument("--cpu-baseline", default="auto", choices=["auto", "reference", "port", "none"])
     p.add_argument("--cpu-reps", type=int, default=3, help="commutators in the CPU sample")
     p.add_argument("--no-concurrency", action="store_true",
@@ -86,7 +86,12 @@ def roofline_of(cls, work, prof, head):
     bound, per_launch = work[cls]
     ms, n = prof[cls]
     avg_s = ms / 1e3 / n
-    if bound == "mfma":
+    if bound == "mfma" and head == "int8x5":
+        # ten int8 slice products per FP64 multiply-add, against the int8 dense peak
+        achieved = 10 * per_launch / avg_s / 1e12
+        peak = PEAK["int8_mfma_tops"]
+        unit = "TOP/s"
+    elif bound == "mfma":
         achieved = per_launch / avg_s / 1e12
         peak = {"fp64": PEAK["fp64_mfma_tflops"], "split32": PEAK["fp32_mfma_tflops"] / 3,
                 "bf16": 2500.0 / 6}[head]
@@ -217,8 +222,9 @@ def load_traffic():
     return out
 
 
-PRECS = {"fp64": 0, "split32": 1, "bf16": 2}
-DTYPE = {"fp64": "f64", "split32": "split-f32 (FP64 accum)", "bf16": "split-bf16x3 (FP64 accum)"}
+PRECS = {"fp64": 0, "split32": 1, "bf16": 2, "int8x5": 3}
+DTYPE = {"fp64": "f64", "split32": "split-f32 (FP64 accum)", "bf16": "split-bf16x3 (FP64 accum)",
+         "int8x5": "int8x5 slices (Ozaki scheme, exact int32, FP64 combine)"}
 
 
 def timed_flow(ctx, E, H0vec, a, prec, classes):
@@ -292,7 +298,7 @@ def batch_bench(a, E, Basis, batch_hamiltonians):
            "orbits": basis.nsp, "neq": basis.neq}
     bctx = E.Context(basis, nbatch=a.batch)
     sctx = E.Context(basis)
-    for prec in ("fp64", "split32"):
+    for prec in ("fp64", "split32", "int8x5"):
         bctx.set_precision(PRECS[prec])
         sctx.set_precision(PRECS[prec])
         bH = bctx.op(1, H).normal_order()
@@ -331,6 +337,7 @@ def batch_bench(a, E, Basis, batch_hamiltonians):
         log(f"batch {prec}: {nsteps / tb:.2f} instance-steps/s batched vs {nseq / ts:.2f} sequential "
             f"(x{(nsteps / tb) / (nseq / ts):.2f}); E0 identical: {same}")
     out["split32_speedup_vs_fp64"] = out["split32"]["instance_steps_per_s"] / out["fp64"]["instance_steps_per_s"]
+    out["int8x5_speedup_vs_fp64"] = out["int8x5"]["instance_steps_per_s"] / out["fp64"]["instance_steps_per_s"]
     bctx.close()
     sctx.close()
     return out
@@ -362,7 +369,7 @@ def main():
     H.normal_order()
     H0vec = H.download()
     eta_vec = ctx.generator(a.generator, H).download() if a.cpu_baseline != "none" else None
-    precs = ["fp64", "split32", "bf16"] if a.precision == "all" else [a.precision]
+    precs = ["fp64", "split32", "bf16", "int8x5"] if a.precision == "all" else [a.precision]
     checks = {}
     for prec in precs:
         if prec == "fp64":

@@ -13,6 +13,15 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 
 #define HIP_KERNEL_CHECK() ((void)0)
 
+// Z of one Pandya block (M x M, M even) is stored in 2x2 blocks: element
+// (R, C) at ((R/2)*(M/2) + C/2)*4 + (R%2)*2 + C%2.  Rows 2u, 2u+1 are the
+// pairs (a,b), (b,a) (or two (a,a) pairs), so the two Z elements of one ph J'
+// term share a block.
+// Only blocks u <= v are stored (Z(R,C) = sigma Z(C,R) gives the rest).
+__device__ __forceinline__ long long zidx(int R, int C, int M) {
+  return ((long long)(R >> 1) * (M >> 1) + (C >> 1)) * 4 + (R & 1) * 2 + (C & 1);
+}
+
 // Instance of a batched launch: blockIdx.y indexes the active-instance table
 // (null table: one instance, number 0).  Instance k of an operator, Pandya
 // matrix or workspace starts k strides after the buffer base.  A negative

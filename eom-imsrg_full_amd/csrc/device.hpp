@@ -74,6 +74,39 @@ struct SegProb {
   SegDesc seg[4];
 };
 
+// ---- int8-sliced products (the Ozaki scheme on the int8 matrix cores).  An
+// operand row r is held as x = s_r * sum_{i=1..5} a_i 2^(-7i): a power-of-two
+// row scale s_r (max |x| / s_r in [1/4, 1/2)) and five int8 planes of rounded
+// slices a_i in [-64, 64], each row K-contiguous and zero-padded to 64 rows /
+// 64 columns.  A product keeps the slice pairs with i + j <= 6 (fifteen int8
+// products, exact int32 sums per level i + j).
+constexpr int kI8Planes = 5;
+struct I8Seg {
+  int kofs, K, src, pad;  // K window of the segment in the block's plane rows (multiples of 64)
+  double sign;
+};
+// C (M x M) = sum of nseg segments  sign * A_src * B_other^T  over plane rows
+// of the block (A rows m, B rows n); c_off, w_off, nmid as in SegProb.
+struct I8Prob {
+  int M, nseg, nmid, ld;  // ld: row stride of the block in a plane (bytes)
+  long long c_off, w_off;
+  long long p_off;        // byte offset of the block's row 0 in a plane
+  long long s_off;        // index of the block's first row scale
+  I8Seg seg[4];
+};
+// Where the slices of an operand go: per channel block (pp/hh columns of the
+// V rows: pp at 0, hh at 64*ceil(np/64)) and per Pandya block (all columns).
+struct I8Layout {
+  const long long *v_poff, *v_soff;
+  const int *v_ld;
+  const long long *c_poff, *c_soff;
+  const int *c_ld;
+  const int2 *c_strips;  // (Pandya block, first row) per 64-row strip
+  int n_strips;
+  long long v_ps, c_ps;  // plane strides (bytes)
+  long long v_sr, c_sr;  // row scales per instance
+};
+
 // term bits (mirror the C ABI)
 enum : unsigned {
   T110 = 1u << 0, T111 = 1u << 1, T121 = 1u << 2, T122 = 1u << 3,
@@ -174,6 +207,21 @@ void launch_adj_generator(hipStream_t s, const DevLayout &L, const double *H, co
                           const double *Eb, double *Hb, double *mbar, int kind);
 // transpose of normal ordering: Hb (a copy of the cotangent Cb on entry) becomes d<Cb, NO(H)>/dH
 void launch_adj_normal_order(hipStream_t s, const DevLayout &L, const double *Cb, double *Hb, int herm);
+
+// int8 slices (ozaki.hip): the pp/hh columns of every V row of `op`, and the
+// rows of the Pandya matrices `cc`, into 4 planes + row scales
+void launch_slice_v(hipStream_t s, const DevLayout &L, const I8Layout &I, const double *op, signed char *planes,
+                    double *scale, const int *act, int nact);
+void launch_slice_cc(hipStream_t s, const DevLayout &L, const I8Layout &I, const double *cc, signed char *planes,
+                     double *scale, const int *act, int nact);
+// int8-sliced segmented GEMM on the 64 x 64 tile lists of the FP64 path:
+// epi 0 = 222_pp_hh (C += upper, W after nmid), epi 1 = Z (2x2-blocked upper).
+// Operand 0 / 1 planes P0 / P1 (plane stride ps), row scales S0 / S1; segment
+// factor sign * (src ? g1 : g0).
+void launch_seg_gemm_i8(hipStream_t s, int epi, const I8Prob *probs, const int4 *tiles, int ntiles,
+                        const signed char *P0, const signed char *P1, const double *S0, const double *S1,
+                        long long ps, double g0, double g1, double *C, double *W, double sigma, const int *act,
+                        int nact, long long sp, long long ss, long long sc, long long sw);
 
 constexpr int kPartials = 1024;  // reduction partials per instance
 constexpr int kScal = 64;        // scalar slots per instance

@@ -75,6 +75,13 @@ struct imsrg_ctx_s {
   int shape_pp = -1, shape_ph = -1;
   float tune_ms[4] = {0, 0, 0, 0};                      // pp 64, pp 128x64, ph 64, ph 128x64
   int prec = IMSRG_PREC_FP64;
+  // int8-sliced GEMM operands (IMSRG_PREC_INT8X5), built on first use:
+  // planes and row scales of operand 0 (L) and 1 (R), V rows and Pandya rows
+  bool i8_ready = false;
+  I8Layout i8{};
+  signed char *i8v[2] = {nullptr, nullptr}, *i8c[2] = {nullptr, nullptr};
+  double *i8vs[2] = {nullptr, nullptr}, *i8cs[2] = {nullptr, nullptr};
+  const I8Prob *pp_i8 = nullptr, *ph_i8 = nullptr;
   // active-instance tables: slot 0 = all instances (constant), 1.. = driver work sets
   static constexpr int kSlots = 4;
   int *act_d[kSlots] = {nullptr, nullptr, nullptr, nullptr};
@@ -602,10 +609,118 @@ int imsrg_ctx_info(imsrg_ctx c, double *info) {
   return 0;
 }
 
+// Tables and buffers of the int8-sliced GEMMs (ozaki.hip).  Channel block q:
+// plane rows 64*ceil(dim/64) of ld = 64*ceil(np/64) + 64*ceil(nh/64) bytes
+// (pp columns, then hh columns); Pandya block q: rows 64*ceil(r/64) of
+// 64*ceil(nph/64) bytes.  The problem tables follow the FP64 ones entry for
+// entry, so the FP64 64 x 64 tile lists serve both.  Padding stays zero: the
+// slice kernels rewrite every byte of a used row and never touch pad rows.
+static int i8_setup(imsrg_ctx_s *c) {
+  const Layout &Ly = c->lay;
+  const size_t B = c->nbatch;
+  auto pad = [](long long x) { return (x + 63) & ~63LL; };
+  std::vector<long long> vpoff(Ly.nblocks, 0), vsoff(Ly.nblocks, 0), cpoff(Ly.ncc, 0), csoff(Ly.ncc, 0);
+  std::vector<int> vld(Ly.nblocks, 0), cld(Ly.ncc, 0);
+  std::vector<int2> strips;
+  std::vector<I8Prob> pp, ph;
+  long long pb = 0, rb = 0;
+  int kmax = 0;
+  for (int q = 0; q < Ly.nblocks; q++) {
+    const int dim = Ly.bdim[q], np = Ly.bnp[q], nh = Ly.bnh[q];
+    if (dim == 0) continue;
+    const int ld = (int)(pad(np) + pad(nh));
+    vld[q] = ld;
+    vpoff[q] = pb;
+    vsoff[q] = rb;
+    I8Prob P{};
+    P.M = dim;
+    P.ld = ld;
+    P.c_off = P.w_off = Ly.boff[q];
+    P.p_off = pb;
+    P.s_off = rb;
+    P.nmid = 2;
+    P.nseg = nh > 0 ? 4 : 2;
+    P.seg[0] = I8Seg{0, (int)pad(np), 0, 0, 1.0};
+    P.seg[1] = I8Seg{0, (int)pad(np), 1, 0, -1.0};
+    P.seg[2] = I8Seg{(int)pad(np), (int)pad(nh), 0, 0, -1.0};
+    P.seg[3] = I8Seg{(int)pad(np), (int)pad(nh), 1, 0, 1.0};
+    pp.push_back(P);
+    kmax = std::max(kmax, ld);
+    pb += pad(dim) * ld;
+    rb += pad(dim);
+  }
+  c->i8.v_ps = (pb + 255) & ~255LL;
+  c->i8.v_sr = std::max<long long>(rb, 1);
+  pb = rb = 0;
+  for (int q = 0; q < Ly.ncc; q++) {
+    const int r = Ly.ccrlen[q], nbq = Ly.ccnph[q];
+    if (r == 0) continue;
+    const int ld = (int)pad(nbq);
+    cld[q] = ld;
+    cpoff[q] = pb;
+    csoff[q] = rb;
+    I8Prob P{};
+    P.M = r;
+    P.ld = ld;
+    P.c_off = Ly.cwoff[q];
+    P.w_off = -1;
+    P.p_off = pb;
+    P.s_off = rb;
+    P.nmid = P.nseg = 2;
+    P.seg[0] = I8Seg{0, ld, 0, 0, 1.0};
+    P.seg[1] = I8Seg{0, ld, 1, 0, 1.0};
+    ph.push_back(P);
+    if (nbq > 0)
+      for (int r0 = 0; r0 < r; r0 += 64) strips.push_back(make_int2(q, r0));
+    kmax = std::max(kmax, ld);
+    pb += pad(r) * ld;
+    rb += pad(r);
+  }
+  c->i8.c_ps = (pb + 255) & ~255LL;
+  c->i8.c_sr = std::max<long long>(rb, 1);
+  c->i8.n_strips = (int)strips.size();
+  // a level sums at most kI8Planes products of |a_i b_j| <= 64^2 over K
+  if ((long long)kmax * kI8Planes * 64 * 64 >= (1LL << 31)) return fail(IMSRG_EINVAL, "int8 slices: K too long");
+  int rc = upload_table(c, vpoff, &c->i8.v_poff);
+  if (!rc) rc = upload_table(c, vsoff, &c->i8.v_soff);
+  if (!rc) rc = upload_table(c, vld, &c->i8.v_ld);
+  if (!rc) rc = upload_table(c, cpoff, &c->i8.c_poff);
+  if (!rc) rc = upload_table(c, csoff, &c->i8.c_soff);
+  if (!rc) rc = upload_table(c, cld, &c->i8.c_ld);
+  if (!rc) rc = upload_table(c, strips, &c->i8.c_strips);
+  if (!rc) rc = upload_table(c, pp, &c->pp_i8);
+  if (!rc) rc = upload_table(c, ph, &c->ph_i8);
+  for (int k = 0; k < 2 && !rc; k++) {
+    void *p = nullptr;
+    const size_t vb = kI8Planes * (size_t)c->i8.v_ps * B, cb = kI8Planes * (size_t)c->i8.c_ps * B;
+    if (hipMalloc(&p, std::max<size_t>(vb, 256)) != hipSuccess) return fail(IMSRG_ENOMEM, "hipMalloc int8 planes failed");
+    c->dallocs.push_back(p);
+    c->i8v[k] = (signed char *)p;
+    HIPCHK(hipMemset(p, 0, std::max<size_t>(vb, 256)));
+    if (hipMalloc(&p, std::max<size_t>(cb, 256)) != hipSuccess) return fail(IMSRG_ENOMEM, "hipMalloc int8 planes failed");
+    c->dallocs.push_back(p);
+    c->i8c[k] = (signed char *)p;
+    HIPCHK(hipMemset(p, 0, std::max<size_t>(cb, 256)));
+    rc = dalloc(c, c->i8.v_sr * B, &c->i8vs[k]);
+    if (!rc) rc = dalloc(c, c->i8.c_sr * B, &c->i8cs[k]);
+    if (!rc) HIPCHK(hipMemset(c->i8vs[k], 0, sizeof(double) * c->i8.v_sr * B));
+    if (!rc) HIPCHK(hipMemset(c->i8cs[k], 0, sizeof(double) * c->i8.c_sr * B));
+  }
+  if (rc) return rc;
+  HIPCHK(hipDeviceSynchronize());
+  c->i8_ready = true;
+  return 0;
+}
+
 int imsrg_ctx_set_precision(imsrg_ctx c, int prec) {
   if (!c) return fail(IMSRG_EINVAL, "null context");
-  if (prec != IMSRG_PREC_FP64 && prec != IMSRG_PREC_SPLIT_FP32 && prec != IMSRG_PREC_SPLIT_BF16)
+  if (prec != IMSRG_PREC_FP64 && prec != IMSRG_PREC_SPLIT_FP32 && prec != IMSRG_PREC_SPLIT_BF16 &&
+      prec != IMSRG_PREC_INT8X5)
     return fail(IMSRG_EINVAL, "unknown precision");
+  if (prec == IMSRG_PREC_INT8X5 && !c->i8_ready) {
+    int rc = i8_setup(c);
+    if (rc) return rc;
+  }
   c->prec = prec;
   return 0;
 }
@@ -797,8 +912,9 @@ struct CommEpi {
   double coef = 0.0;
   bool norm = false;
 };
-// lcc_ready: cc[0] already holds the Pandya matrices of L (drivers reuse them
-// across the nested commutators of one BCH / Magnus series, where L = Omega).
+// lcc_ready: cc[0] already holds the Pandya matrices of L, and with int8
+// slices the slices of L too (prep_L; drivers reuse them across the nested
+// commutators of one BCH / Magnus series, where L = Omega).
 // Every launch covers the instances of `a`.
 static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *RES, unsigned mask,
                          bool lcc_ready, const ActSet &a, const CommEpi &epi = CommEpi());
@@ -916,7 +1032,18 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
     PROFS(c, IMSRG_PROF_E0, s2, launch_e0(s2, D, L->d, R->d, RES->d, hL, hR, mask, c->partials, ad, na, OS));
   if (mask & T122) PROFS(c, IMSRG_PROF_122, s2, launch_122(s2, D, L->d, R->d, RES->d, hres, ad, na));
   else launch_clear_blocks(s2, D, RES->d, ad, na);
-  if (mask & (T222PPHH | T221)) {
+  const bool i8 = c->prec == IMSRG_PREC_INT8X5;
+  if ((mask & (T222PPHH | T221)) && i8) {
+    // the slicing counts as part of the product; B rows are rows of the other
+    // operand: X[k][n] = herm * X[n][k]
+    PROFS(c, IMSRG_PROF_GEMM_PPHH, s2, {
+      if (!lcc_ready) launch_slice_v(s2, D, c->i8, L->d, c->i8v[0], c->i8vs[0], ad, na);
+      launch_slice_v(s2, D, c->i8, R->d, c->i8v[1], c->i8vs[1], ad, na);
+      launch_seg_gemm_i8(s2, 0, c->pp_i8, c->pp_tiles, c->n_pp_tiles, c->i8v[0], c->i8v[1], c->i8vs[0], c->i8vs[1],
+                         c->i8.v_ps, (double)hR, (double)hL, (mask & T222PPHH) ? RES->d + D.v_off : nullptr,
+                         (mask & T221) ? c->wpp : nullptr, 0.0, ad, na, kI8Planes * c->i8.v_ps, c->i8.v_sr, OS, D.vsize);
+    });
+  } else if (mask & (T222PPHH | T221)) {
     const bool w = c->prec == IMSRG_PREC_FP64 && c->shape_pp == 1;
     PROFS(c, IMSRG_PROF_GEMM_PPHH, s2,
           launch_seg_gemm_pphh(s2, w ? 3 : c->prec, c->pp_probs, w ? c->pp_tiles_w : c->pp_tiles,
@@ -929,11 +1056,21 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
   if (mask & T222PH) {
     if (!lcc_ready) PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, L->d, c->cc[0], ad, na));
     PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, R->d, c->cc[1], ad, na));
-    const bool w = c->prec == IMSRG_PREC_FP64 && c->shape_ph == 1;
-    PROF(c, IMSRG_PROF_GEMM_PH,
-         launch_seg_gemm_z(s, w ? 3 : c->prec, c->ph_probs, w ? c->ph_tiles_w : c->ph_tiles,
-                           w ? c->n_ph_tiles_w : c->n_ph_tiles, c->cc[0], c->cc[1], c->wph, (double)hL, (double)hR,
-                           ad, na, D.ccsize, D.cwsize));
+    if (i8) {
+      PROF(c, IMSRG_PROF_GEMM_PH, {
+        if (!lcc_ready) launch_slice_cc(s, D, c->i8, c->cc[0], c->i8c[0], c->i8cs[0], ad, na);
+        launch_slice_cc(s, D, c->i8, c->cc[1], c->i8c[1], c->i8cs[1], ad, na);
+        launch_seg_gemm_i8(s, 1, c->ph_i8, c->ph_tiles, c->n_ph_tiles, c->i8c[0], c->i8c[1], c->i8cs[0],
+                           c->i8cs[1], c->i8.c_ps, (double)hL, -(double)hR, c->wph, nullptr,
+                           -(double)hL * (double)hR, ad, na, kI8Planes * c->i8.c_ps, c->i8.c_sr, D.cwsize, 0);
+      });
+    } else {
+      const bool w = c->prec == IMSRG_PREC_FP64 && c->shape_ph == 1;
+      PROF(c, IMSRG_PROF_GEMM_PH,
+           launch_seg_gemm_z(s, w ? 3 : c->prec, c->ph_probs, w ? c->ph_tiles_w : c->ph_tiles,
+                             w ? c->n_ph_tiles_w : c->n_ph_tiles, c->cc[0], c->cc[1], c->wph, (double)hL,
+                             (double)hR, ad, na, D.ccsize, D.cwsize));
+    }
     PROF(c, IMSRG_PROF_PH_TERMS, launch_ph_terms(s, D, c->wph, c->tph, (double)hL, (double)hR, ad, na));
   }
   if (dual) {
@@ -1105,6 +1242,16 @@ static ActSet series_set(imsrg_ctx_s *c, int slot, const ActSet &a) {
   (void)hipMemcpyAsync(c->act_d[slot], a.d, a.n() * sizeof(int), hipMemcpyDeviceToDevice, c->stream);
   return s;
 }
+// The operand L = Omega of every nested commutator of a series, prepared once:
+// its Pandya matrices and, with int8 slices, the slices of its V rows and of
+// those matrices.
+static void prep_L(imsrg_ctx_s *c, imsrg_op_s *G, const ActSet &set) {
+  launch_pandya(c->stream, c->dl, G->d, c->cc[0], set.d, set.n());
+  if (c->prec == IMSRG_PREC_INT8X5) {
+    launch_slice_v(c->stream, c->dl, c->i8, G->d, c->i8v[0], c->i8vs[0], set.d, set.n());
+    launch_slice_cc(c->stream, c->dl, c->i8, c->cc[0], c->i8c[0], c->i8cs[0], set.d, set.n());
+  }
+}
 // The series gate after nested commutator n: the test on the device, its
 // live flags into ring slot n % kGateRing, marked by an event.
 static int series_gate(imsrg_ctx_s *c, int slot, const ActSet &set, int n, int mode, double coef, int store,
@@ -1146,7 +1293,7 @@ static int bch_impl(imsrg_ctx_s *c, imsrg_op_s *HS, imsrg_op_s *G, imsrg_op_s *H
   AD->herm = 1;
   INT2->herm = 1;
   const ActSet cur = series_set(c, 1, a);
-  launch_pandya(c->stream, c->dl, G->d, c->cc[0], cur.d, cur.n());
+  prep_L(c, G, cur);
   std::vector<int> live(a.n(), 1);  // positions in the series for nested commutator iw
   int nlive = a.n();
   double coef = 1.0;
@@ -1200,7 +1347,7 @@ static int magexp_impl(imsrg_ctx_s *c, imsrg_op_s *DG, imsrg_op_s *G, const ActS
        launch_sumsq(c->stream, c->dl, G->d, c->partials, c->scal + kFullSlot, set.d, set.n()));
   int rc = series_gate(c, 2, set, 1, 1, 0.0, -1, 0);
   if (rc) return rc;
-  launch_pandya(c->stream, c->dl, G->d, c->cc[0], set.d, set.n());
+  prep_L(c, G, set);
   std::vector<int> live;
   int nlive = 0;
   for (int i = 2; i <= 7; i++) {
@@ -1401,7 +1548,7 @@ static int nested_chain(imsrg_ctx_s *c, imsrg_op_s *Om, imsrg_op_s *X0, int K, u
                         std::vector<imsrg_op_s *> &A) {
   const ActSet all = all_set(c);
   op_copy(X0, A[0], all);
-  launch_pandya(c->stream, c->dl, Om->d, c->cc[0], all.d, all.n());
+  prep_L(c, Om, all);
   for (int k = 1; k <= K; k++) {
     A[k]->herm = -Om->herm * A[k - 1]->herm;
     int rc = comm_impl(c, Om, A[k - 1], A[k], mask, true, all);

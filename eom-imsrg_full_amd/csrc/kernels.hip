@@ -268,14 +268,6 @@ __device__ __forceinline__ int acc_col(int tj) {
   return ((threadIdx.x >> 6) & 1) * 32 + tj * 16 + (threadIdx.x & 15);
 }
 
-// Z of one Pandya block (M x M, M even) is stored in 2x2 blocks: element
-// (R, C) at ((R/2)*(M/2) + C/2)*4 + (R%2)*2 + C%2.  Rows 2u, 2u+1 are the
-// pairs (a,b), (b,a) (or two (a,a) pairs), so the two Z elements of one ph J'
-// term share a block.
-// Only blocks u <= v are stored (Z(R,C) = sigma Z(C,R) gives the rest).
-__device__ __forceinline__ long long zidx(int R, int C, int M) {
-  return ((long long)(R >> 1) * (M >> 1) + (C >> 1)) * 4 + (R & 1) * 2 + (C & 1);
-}
 
 // ---- split-precision segments: the same 64x64 tile and 2x2 quadrant layout;
 // products accumulate in FP32 inside one K slab and in FP64 across slabs.

@@ -1,0 +1,336 @@
+// ozaki.hip -- the 222 GEMMs on the int8 matrix cores of gfx950 (the Ozaki
+// scheme: error-free slicing into int8 planes, exact int32 products).
+//
+// Every operand row r is held as x = s_r * sum_{i=1..5} a_i 2^(-7i): s_r is a
+// power of two with max|row| / s_r in [1/4, 1/2), and a_i in [-64, 64] are
+// produced by rounding x / s_r * 2^7 to nearest and carrying the remainder
+// (each step exact in FP64; rounding rather than truncation keeps the
+// dropped remainders unbiased, which matters for sums of same-sign terms).
+// A dot product of two rows is then
+//   s_A s_B sum_{L=2..6} 2^(-7L) sum_{i+j=L} sum_k a_i[k] b_j[k],
+// with the inner sums exact in int32 on v_mfma_i32_16x16x64_i8 (|a_i b_j| <=
+// 2^12, at most five pairs per level and K < 2^17); the slice pairs with
+// i + j > 6 are dropped, so the error is about 2^-36 of max|A row| max|B row|
+// per product, against 2^-24 of each K-slab partial sum for split FP32.
+// The levels are combined and scaled in FP64, segment by segment, into the
+// same FP64 accumulators and epilogues (222_pp_hh C += upper, the 221 W
+// intermediate, the 2x2-blocked Z) as the FP64 path.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "device.hpp"
+#include "devfun.hpp"
+
+namespace imsrg {
+
+typedef int v4i_t __attribute__((ext_vector_type(4)));
+constexpr int NS = kI8Planes;  // slices per element
+
+// the NS rounded slices of x / 2^ex (|x| < 2^ex / 2), as bytes
+__device__ __forceinline__ void slices_of(double x, int ex, unsigned b[NS]) {
+  double t = ldexp(x, 7 - ex);
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    const double a = rint(t);
+    b[i] = (unsigned)((int)a & 255);
+    t = (t - a) * 128.0;
+  }
+}
+// exponent ex with max / 2^ex in [1/4, 1/2); the row scale is 2^ex (0 for a zero row)
+__device__ __forceinline__ int row_exp(double m) {
+  int ex = 0;
+  (void)frexp(m, &ex);
+  return ex + 1;
+}
+
+// V rows: one wavefront per pair row (four per workgroup).  The pp columns
+// [0, np) go to plane columns [0, np), the hh columns [np+nb, dim) to
+// [P, P+nh) with P = 64*ceil(np/64); the padding columns are written as zero.
+__global__ __launch_bounds__(256) void k_slice_v(DevLayout L, I8Layout I, const double *__restrict__ op,
+                                                 signed char *__restrict__ planes, double *__restrict__ scale,
+                                                 const int *act) {
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (r >= L.npairs) return;
+  {
+    const long long inst = inst_id(act);
+    if (inst < 0) return;
+    op += inst * L.op_size;
+    planes += inst * NS * I.v_ps;
+    scale += inst * I.v_sr;
+  }
+  const int q = L.pair_q[r], ld = I.v_ld[q];
+  if (ld == 0) return;
+  const int rl = r - L.pbase[q], dim = L.bdim[q], np = L.bnp[q], nb = L.bnb[q], nh = L.bnh[q];
+  const int P = (np + 63) & ~63;
+  const double *X = op + L.v_off + L.boff[q] + (long long)rl * dim;
+  double m = 0.0;
+  for (int j = lane; j < np + nh; j += 64) m = fmax(m, fabs(X[j < np ? j : nb + j]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  const int ex = row_exp(m);
+  if (lane == 0) scale[I.v_soff[q] + rl] = m > 0.0 ? ldexp(1.0, ex) : 0.0;
+  signed char *row = planes + I.v_poff[q] + (long long)rl * ld;
+  for (int j4 = lane * 4; j4 < ld; j4 += 256) {
+    unsigned w[NS] = {};
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int j = j4 + u;
+      const int c = j < np ? j : (j >= P && j < P + nh) ? np + nb + (j - P) : -1;
+      unsigned b[NS] = {};
+      if (c >= 0 && m > 0.0) slices_of(X[c], ex, b);
+#pragma unroll
+      for (int i = 0; i < NS; i++) w[i] |= b[i] << (8 * u);
+    }
+#pragma unroll
+    for (int i = 0; i < NS; i++) *(unsigned *)(row + i * I.v_ps + j4) = w[i];
+  }
+}
+
+// Pandya rows: one workgroup per 64-row strip of a Pandya block (column-major
+// r x K in `cc`).  Pass 1: row maxima (64 consecutive rows per wavefront load);
+// pass 2: per 64-column chunk each thread slices 16 consecutive columns of one
+// row and stores 16 bytes per plane.  Rows r..64*ceil(r/64) are written as zero.
+__global__ __launch_bounds__(256) void k_slice_cc(DevLayout L, I8Layout I, const double *__restrict__ cc,
+                                                  signed char *__restrict__ planes, double *__restrict__ scale,
+                                                  const int *act) {
+  __shared__ double red[4][64];
+  const int2 it = I.c_strips[blockIdx.x];
+  {
+    const long long inst = inst_id(act);
+    if (inst < 0) return;
+    cc += inst * L.ccsize;
+    planes += inst * NS * I.c_ps;
+    scale += inst * I.c_sr;
+  }
+  const int q = it.x, r0 = it.y, tid = threadIdx.x, rloc = tid & 63, kq = tid >> 6;
+  const int R = L.ccrlen[q], K = L.ccnph[q], ld = I.c_ld[q], row = r0 + rloc;
+  const double *src = cc + L.ccoff[q];
+  double m = 0.0;
+  if (row < R)
+    for (int k = kq; k < K; k += 4) m = fmax(m, fabs(src[(long long)k * R + row]));
+  red[kq][rloc] = m;
+  __syncthreads();
+  m = fmax(fmax(red[0][rloc], red[1][rloc]), fmax(red[2][rloc], red[3][rloc]));
+  const int ex = row_exp(m);
+  if (kq == 0) scale[I.c_soff[q] + row] = m > 0.0 ? ldexp(1.0, ex) : 0.0;
+  signed char *dst = planes + I.c_poff[q] + (long long)row * ld;
+  for (int kc = 0; kc < ld; kc += 64) {
+    const int k0 = kc + kq * 16;
+    unsigned w[NS][4];
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+#pragma unroll
+      for (int i = 0; i < NS; i++) w[i][g] = 0;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int k = k0 + 4 * g + u;
+        unsigned b[NS] = {};
+        if (row < R && k < K && m > 0.0) slices_of(src[(long long)k * R + row], ex, b);
+#pragma unroll
+        for (int i = 0; i < NS; i++) w[i][g] |= b[i] << (8 * u);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+      const v4i_t v = {(int)w[i][0], (int)w[i][1], (int)w[i][2], (int)w[i][3]};
+      *(v4i_t *)(dst + i * I.c_ps + k0) = v;
+    }
+  }
+}
+
+void launch_slice_v(hipStream_t s, const DevLayout &L, const I8Layout &I, const double *op, signed char *planes,
+                    double *scale, const int *act, int nact) {
+  if (nact <= 0 || L.npairs <= 0) return;
+  hipLaunchKernelGGL(k_slice_v, dim3((L.npairs + 3) / 4, nact), dim3(256), 0, s, L, I, op, planes, scale, act);
+}
+void launch_slice_cc(hipStream_t s, const DevLayout &L, const I8Layout &I, const double *cc, signed char *planes,
+                     double *scale, const int *act, int nact) {
+  if (nact <= 0 || I.n_strips <= 0) return;
+  hipLaunchKernelGGL(k_slice_cc, dim3(I.n_strips, nact), dim3(256), 0, s, L, I, cc, planes, scale, act);
+}
+
+// ---- the product.  A 64 x 64 output tile per workgroup; each K chunk of 64
+// bytes of the NS planes of A (64 rows) and B (64 rows) is loaded into
+// registers one chunk ahead and staged in LDS (16-byte column group g of row
+// r at g ^ ((r >> 2) & 3), so the 16 rows of a fragment read hit distinct
+// banks).  Wavefront (wm, wn) owns the 32 x 32 quadrant (rows 32 wm, columns
+// 32 wn) as 2 x 2 sub-tiles; fragments: lane l holds row l % 16 of a 16-row
+// group and K bytes 16 (l / 16) .. +15 of the chunk, for A and B alike; the
+// int32 result (i, j) sits in lane j + 16 (i / 4), element i % 4.
+struct I8Smem {
+  v4i_t a[NS][64][4];
+  v4i_t b[NS][64][4];
+};
+
+__device__ __forceinline__ int swz(int r, int g) { return g ^ ((r >> 2) & 3); }
+
+template <int EPI>
+__global__ __launch_bounds__(256, 2) void k_seg_gemm_i8(const I8Prob *__restrict__ probs,
+                                                        const int4 *__restrict__ tiles, int ntiles,
+                                                        const signed char *__restrict__ P0,
+                                                        const signed char *__restrict__ P1,
+                                                        const double *__restrict__ S0, const double *__restrict__ S1,
+                                                        long long ps, double g0, double g1, double *__restrict__ C,
+                                                        double *__restrict__ W, double sigma, const int *act,
+                                                        long long sp, long long ss, long long sc, long long sw) {
+  const int t = blockIdx.x;
+  if (t >= ntiles) return;
+  {
+    const long long inst = inst_id(act);
+    if (inst < 0) return;
+    P0 += inst * sp;
+    P1 += inst * sp;
+    S0 += inst * ss;
+    S1 += inst * ss;
+    if (C) C += inst * sc;
+    if (W) W += inst * sw;
+  }
+  const int4 tl = tiles[t];
+  if (tl.x < 0) return;  // padding slot of the XCD-balanced schedule
+  const I8Prob &p = probs[tl.x];
+  __shared__ I8Smem S;
+  const int M = p.M, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m0 = tl.y * 64, n0 = tl.z * 64, wm = wave >> 1, wn = wave & 1;
+  const int ra = m0 + wm * 32, cb = n0 + wn * 32;
+  // quadrants beyond the block or strictly below the diagonal only help load
+  const bool active = ra < M && cb < M && ra < cb + 32;
+  const long long ld = p.ld;
+  // staging: thread tid loads 16 bytes (group tid % 4) of tile row tid / 4 per plane
+  const int sr = tid >> 2, sg = tid & 3, sgp = swz(sr, sg);
+  // fragment reads: row (lane % 16) of each 16-row group, column group lane / 16
+  const int fr = lane & 15, fg = swz(fr, lane >> 4);
+  double4_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+
+  auto run = [&](int s0, int s1) {
+    for (int s = s0; s < s1; s++) {
+      const I8Seg &sgm = p.seg[s];
+      const int K = sgm.K;
+      if (K == 0) continue;
+      const bool o = sgm.src != 0;
+      const signed char *A = (o ? P1 : P0) + p.p_off + sgm.kofs + (long long)(m0 + sr) * ld + sg * 16;
+      const signed char *B = (o ? P0 : P1) + p.p_off + sgm.kofs + (long long)(n0 + sr) * ld + sg * 16;
+      v4i_t R[2][NS];  // the next chunk, in flight while this one is used
+      auto load = [&](int kc) {
+#pragma unroll
+        for (int i = 0; i < NS; i++) {
+          R[0][i] = *(const v4i_t *)(A + i * ps + kc);
+          R[1][i] = *(const v4i_t *)(B + i * ps + kc);
+        }
+      };
+      v4i_t lv[NS][2][2];
+#pragma unroll
+      for (int l = 0; l < NS; l++)
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+          for (int j = 0; j < 2; j++) lv[l][i][j] = (v4i_t){0, 0, 0, 0};
+      load(0);
+      for (int kc = 0; kc < K; kc += 64) {  // K is workgroup-uniform: every barrier is reached by all
+        __syncthreads();  // the previous chunk has been read
+#pragma unroll
+        for (int i = 0; i < NS; i++) {
+          S.a[i][sr][sgp] = R[0][i];
+          S.b[i][sr][sgp] = R[1][i];
+        }
+        __syncthreads();
+        if (kc + 64 < K) load(kc + 64);
+        if (active) {
+          // all B fragments, then A plane by plane: slice pair (i, j) adds to level i + j
+          v4i_t b[NS][2];
+#pragma unroll
+          for (int j = 0; j < NS; j++)
+#pragma unroll
+            for (int u = 0; u < 2; u++) b[j][u] = S.b[j][wn * 32 + u * 16 + fr][fg];
+#pragma unroll
+          for (int i = 0; i < NS; i++) {
+            v4i_t a[2];
+#pragma unroll
+            for (int u = 0; u < 2; u++) a[u] = S.a[i][wm * 32 + u * 16 + fr][fg];
+#pragma unroll
+            for (int j = 0; j + i < NS; j++)
+#pragma unroll
+              for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+                for (int tj = 0; tj < 2; tj++)
+                  lv[i + j][ti][tj] =
+                      __builtin_amdgcn_mfma_i32_16x16x64_i8(a[ti], b[j][tj], lv[i + j][ti][tj], 0, 0, 0);
+          }
+        }
+      }
+      if (active) {
+        const double *SA = (o ? S1 : S0) + p.s_off, *SB = (o ? S0 : S1) + p.s_off;
+        const double sgn = sgm.sign * (o ? g1 : g0);
+#pragma unroll
+        for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+          for (int tj = 0; tj < 2; tj++) {
+            const double sb = SB[cb + tj * 16 + fr] * sgn;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+              const double sa = SA[ra + ti * 16 + 4 * (lane >> 4) + v];
+              double lev = (double)lv[NS - 1][ti][tj][v];
+#pragma unroll
+              for (int l = NS - 2; l >= 0; l--) lev = (double)lv[l][ti][tj][v] + lev * 0x1p-7;
+              acc[ti][tj][v] += sa * sb * (lev * 0x1p-14);
+            }
+          }
+      }
+    }
+  };
+  auto each = [&](auto &&f) {
+#pragma unroll
+    for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+      for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int row = ra + ti * 16 + 4 * (lane >> 4) + v, col = cb + tj * 16 + fr;
+          if (row < M && col < M && row <= col) f(row, col, acc[ti][tj][v]);
+        }
+  };
+  if (EPI == 0) {
+    run(0, p.nmid);
+    if (W && active) {
+      double *Wp = W + p.w_off;
+      each([&](int row, int col, double x) { Wp[(long long)row * M + col] = x; });
+    }
+    run(p.nmid, p.nseg);
+    if (C && active) {
+      double *Cp = C + p.c_off;
+      each([&](int row, int col, double x) {
+        const long long o = (long long)row * M + col;
+        Cp[o] = Cp[o] + x;
+      });
+    }
+  } else {
+    run(0, p.nseg);
+    if (active) {
+      double *Zp = C + p.c_off;
+      each([&](int row, int col, double x) {
+        Zp[zidx(row, col, M)] = x;
+        if (row < col && (row >> 1) == (col >> 1)) Zp[zidx(col, row, M)] = sigma * x;
+      });
+    }
+  }
+}
+
+void launch_seg_gemm_i8(hipStream_t s, int epi, const I8Prob *probs, const int4 *tiles, int ntiles,
+                        const signed char *P0, const signed char *P1, const double *S0, const double *S1,
+                        long long ps, double g0, double g1, double *C, double *W, double sigma, const int *act,
+                        int nact, long long sp, long long ss, long long sc, long long sw) {
+  if (ntiles <= 0 || nact <= 0) return;
+  const dim3 g(ntiles, nact), b(256);
+  if (epi == 0)
+    hipLaunchKernelGGL(k_seg_gemm_i8<0>, g, b, 0, s, probs, tiles, ntiles, P0, P1, S0, S1, ps, g0, g1, C, W, sigma,
+                       act, sp, ss, sc, sw);
+  else
+    hipLaunchKernelGGL(k_seg_gemm_i8<1>, g, b, 0, s, probs, tiles, ntiles, P0, P1, S0, S1, ps, g0, g1, C, W, sigma,
+                       act, sp, ss, sc, sw);
+}
+
+}  // namespace imsrg

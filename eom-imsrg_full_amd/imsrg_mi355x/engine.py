@@ -21,7 +21,7 @@ HEADER = os.path.abspath(os.path.join(HERE, "..", "..", "include", "imsrg_mi355x
 
 T110, T111, T121, T122, T220, T221, T222PPHH, T222PH = (1 << k for k in range(8))
 TALL = 0xFF
-PREC_FP64, PREC_SPLIT_FP32, PREC_SPLIT_BF16 = 0, 1, 2
+PREC_FP64, PREC_SPLIT_FP32, PREC_SPLIT_BF16, PREC_INT8X5 = 0, 1, 2, 3
 GEN = {"white": 0, "atan": 1, "w2": 2}
 
 

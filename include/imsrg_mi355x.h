@@ -50,6 +50,9 @@ typedef struct imsrg_op_s *imsrg_op;
 #define IMSRG_PREC_FP64 0
 #define IMSRG_PREC_SPLIT_FP32 1   /* x = hi+lo in FP32; hi*hi+hi*lo+lo*hi on FP32 MFMA, FP64 across K slabs */
 #define IMSRG_PREC_SPLIT_BF16 2   /* x = hi+mid+lo in BF16; 6 products on BF16 MFMA, FP32 in-slab, FP64 across */
+#define IMSRG_PREC_INT8X5 3       /* x = s_row * sum_{i<=5} a_i 2^-7i in rounded int8 slices (Ozaki scheme);
+                                     the 15 slice products with i+j <= 6 on int8 MFMA, exact int32 sums,
+                                     FP64 combine */
 
 /* generator kinds (generators.f90) */
 #define IMSRG_GEN_WHITE 0         /* build_gs_white  generators.f90:4-106   */
