@@ -71,6 +71,13 @@ def kernel_work(info, basis):
     cc = info["ccsize"] * 8.0                     # Pandya matrices of one operator
     z = info["cwsize"] * 8.0                      # Z = hL*W - hR*W^T, all Pandya blocks
     t = info["nphterm"] * 8.0                     # ph recoupling J' terms
+    # int8 slicing: the pp/hh columns of one operator's V rows, and one Pandya
+    # matrix, read in FP64 and written as 5 int8 planes; one span of each per
+    # commutator, so a launch averages the two
+    from imsrg_mi355x.basis import pandya_blocks
+    vpp = sum(b.dim * (b.npp + b.nhh) for b in basis.blocks)
+    ccel = sum(r * nb for (_, r, nb) in pandya_blocks(basis))
+    sl = (vpp + ccel) * (8.0 + 5.0) / 2
     return {
         "gemm_pphh": ("mfma", info["flop_pphh"]),
         "gemm_ph": ("mfma", info["flop_ph"]),
@@ -79,6 +86,7 @@ def kernel_work(info, basis):
         "ph_terms": ("hbm", z + t),               # read Z once, write the terms
         "ph_apply": ("hbm", t + 2 * upper),       # read the terms, read-modify-write the upper triangles
         "sym_mirror": ("hbm", 2 * full),          # read and write whole blocks
+        "slice": ("hbm", sl),
     }
 
 
@@ -87,8 +95,9 @@ def roofline_of(cls, work, prof, head):
     ms, n = prof[cls]
     avg_s = ms / 1e3 / n
     if bound == "mfma" and head == "int8x5":
-        # ten int8 slice products per FP64 multiply-add, against the int8 dense peak
-        achieved = 10 * per_launch / avg_s / 1e12
+        # fifteen int8 slice products per FP64 multiply-add, against the int8 dense peak
+        # (the span also holds the slicing of the operand that changes every commutator)
+        achieved = 15 * per_launch / avg_s / 1e12
         peak = PEAK["int8_mfma_tops"]
         unit = "TOP/s"
     elif bound == "mfma":
@@ -141,8 +150,11 @@ KERNEL_NAME = {
     "gemm_pphh": "k_seg_gemm<false, 0, 0>", "gemm_ph": "k_seg_gemm<true, 0, 1>", "pandya": "k_pandya(",
     "k122": "k_122(", "ph_terms": "k_ph_terms(", "ph_apply": "k_ph_apply(", "sym_mirror": "k_sym_mirror(",
     "onebody": "k_ob_partial(",
+    # int8x5 products and slicing
+    "gemm_pphh_i8": "k_seg_gemm_i8<0>", "gemm_ph_i8": "k_seg_gemm_i8<1>", "slice_v": "k_slice_v(",
+    "slice_cc": "k_slice_cc(",
 }
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_emax8_fp64.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_emax8.txt")
 
 
 def pmc_summary(fetch_csv, write_csv, op_bytes):
@@ -210,8 +222,10 @@ def trace_summary(trace_csv):
     return "\n".join(out)
 
 
-def load_traffic():
-    """calibrated HBM bytes per launch per kernel class, if measured for this workload"""
+def load_traffic(head):
+    """calibrated HBM bytes per launch per kernel class, if measured for this
+    workload; with int8 slices the GEMM classes are the int8 products and a
+    slice span averages one k_slice_v and one k_slice_cc"""
     if not os.path.exists(TRAFFIC_FILE):
         return {}
     out = {}
@@ -219,6 +233,11 @@ def load_traffic():
         f = line.split()
         if len(f) == 4 and not line.startswith("#") and f[0] in KERNEL_NAME:
             out[f[0]] = float(f[3])
+    if head == "int8x5":
+        for cls in ("gemm_pphh", "gemm_ph"):
+            out[cls] = out.get(cls + "_i8")
+        if "slice_v" in out and "slice_cc" in out:
+            out["slice"] = (out["slice_v"] + out["slice_cc"]) / 2
     return out
 
 
@@ -412,7 +431,7 @@ def main():
     # of every class with a work model (from the fully profiled extra step)
     dom = max((k for k in work if prof[k][1]), key=lambda k: prof[k][0])
     roof = roofline_of(dom, work, prof, head)
-    traffic = load_traffic() if (head == "fp64" and a.emax == 8 and a.holes == 6) else {}
+    traffic = load_traffic(head) if (head in ("fp64", "int8x5") and a.emax == 8 and a.holes == 6) else {}
     roof["traffic"] = traffic.get(dom)
     if breakdown[dom][1]:
         iso = roofline_of(dom, work, breakdown, head)

@@ -1034,15 +1034,16 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
   else launch_clear_blocks(s2, D, RES->d, ad, na);
   const bool i8 = c->prec == IMSRG_PREC_INT8X5;
   if ((mask & (T222PPHH | T221)) && i8) {
-    // the slicing counts as part of the product; B rows are rows of the other
-    // operand: X[k][n] = herm * X[n][k]
-    PROFS(c, IMSRG_PROF_GEMM_PPHH, s2, {
+    PROFS(c, IMSRG_PROF_SLICE, s2, {
       if (!lcc_ready) launch_slice_v(s2, D, c->i8, L->d, c->i8v[0], c->i8vs[0], ad, na);
       launch_slice_v(s2, D, c->i8, R->d, c->i8v[1], c->i8vs[1], ad, na);
-      launch_seg_gemm_i8(s2, 0, c->pp_i8, c->pp_tiles, c->n_pp_tiles, c->i8v[0], c->i8v[1], c->i8vs[0], c->i8vs[1],
-                         c->i8.v_ps, (double)hR, (double)hL, (mask & T222PPHH) ? RES->d + D.v_off : nullptr,
-                         (mask & T221) ? c->wpp : nullptr, 0.0, ad, na, kI8Planes * c->i8.v_ps, c->i8.v_sr, OS, D.vsize);
     });
+    // B rows are rows of the other operand: X[k][n] = herm * X[n][k]
+    PROFS(c, IMSRG_PROF_GEMM_PPHH, s2,
+          launch_seg_gemm_i8(s2, 0, c->pp_i8, c->pp_tiles, c->n_pp_tiles, c->i8v[0], c->i8v[1], c->i8vs[0],
+                             c->i8vs[1], c->i8.v_ps, (double)hR, (double)hL,
+                             (mask & T222PPHH) ? RES->d + D.v_off : nullptr, (mask & T221) ? c->wpp : nullptr, 0.0,
+                             ad, na, kI8Planes * c->i8.v_ps, c->i8.v_sr, OS, D.vsize));
   } else if (mask & (T222PPHH | T221)) {
     const bool w = c->prec == IMSRG_PREC_FP64 && c->shape_pp == 1;
     PROFS(c, IMSRG_PROF_GEMM_PPHH, s2,
@@ -1057,13 +1058,14 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
     if (!lcc_ready) PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, L->d, c->cc[0], ad, na));
     PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, R->d, c->cc[1], ad, na));
     if (i8) {
-      PROF(c, IMSRG_PROF_GEMM_PH, {
+      PROF(c, IMSRG_PROF_SLICE, {
         if (!lcc_ready) launch_slice_cc(s, D, c->i8, c->cc[0], c->i8c[0], c->i8cs[0], ad, na);
         launch_slice_cc(s, D, c->i8, c->cc[1], c->i8c[1], c->i8cs[1], ad, na);
-        launch_seg_gemm_i8(s, 1, c->ph_i8, c->ph_tiles, c->n_ph_tiles, c->i8c[0], c->i8c[1], c->i8cs[0],
-                           c->i8cs[1], c->i8.c_ps, (double)hL, -(double)hR, c->wph, nullptr,
-                           -(double)hL * (double)hR, ad, na, kI8Planes * c->i8.c_ps, c->i8.c_sr, D.cwsize, 0);
       });
+      PROF(c, IMSRG_PROF_GEMM_PH,
+           launch_seg_gemm_i8(s, 1, c->ph_i8, c->ph_tiles, c->n_ph_tiles, c->i8c[0], c->i8c[1], c->i8cs[0],
+                              c->i8cs[1], c->i8.c_ps, (double)hL, -(double)hR, c->wph, nullptr,
+                              -(double)hL * (double)hR, ad, na, kI8Planes * c->i8.c_ps, c->i8.c_sr, D.cwsize, 0));
     } else {
       const bool w = c->prec == IMSRG_PREC_FP64 && c->shape_ph == 1;
       PROF(c, IMSRG_PROF_GEMM_PH,

@@ -65,6 +65,7 @@ __global__ __launch_bounds__(256) void k_slice_v(DevLayout L, I8Layout I, const 
   const int P = (np + 63) & ~63;
   const double *X = op + L.v_off + L.boff[q] + (long long)rl * dim;
   double m = 0.0;
+#pragma unroll 4
   for (int j = lane; j < np + nh; j += 64) m = fmax(m, fabs(X[j < np ? j : nb + j]));
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
@@ -107,8 +108,10 @@ __global__ __launch_bounds__(256) void k_slice_cc(DevLayout L, I8Layout I, const
   const int R = L.ccrlen[q], K = L.ccnph[q], ld = I.c_ld[q], row = r0 + rloc;
   const double *src = cc + L.ccoff[q];
   double m = 0.0;
-  if (row < R)
+  if (row < R) {
+#pragma unroll 8
     for (int k = kq; k < K; k += 4) m = fmax(m, fabs(src[(long long)k * R + row]));
+  }
   red[kq][rloc] = m;
   __syncthreads();
   m = fmax(fmax(red[0][rloc], red[1][rloc]), fmax(red[2][rloc], red[3][rloc]));

@@ -97,7 +97,7 @@ _SIG = {
     "imsrg_ctx_profile_read": (C.c_int, [_vp, _dp, C.POINTER(C.c_longlong), C.c_int, C.c_int]),
 }
 PROF_CLASSES = ["gemm_pphh", "gemm_ph", "pandya", "k122", "ph_apply", "ph_terms", "sym_mirror",
-                "onebody", "e0", "norm", "axpby", "generator"]
+                "onebody", "e0", "norm", "axpby", "generator", "slice"]
 for _name, (_res, _args) in _SIG.items():
     _f = getattr(_lib, _name)
     _f.restype = _res

@@ -223,6 +223,7 @@ int imsrg_flow_destroy(imsrg_flow flow);
 #define IMSRG_PROF_NORM 9
 #define IMSRG_PROF_AXPBY 10
 #define IMSRG_PROF_GENERATOR 11
+#define IMSRG_PROF_SLICE 12      /* int8 slicing of the GEMM operands (IMSRG_PREC_INT8X5) */
 #define IMSRG_PROF_NCLASS 16
 int imsrg_ctx_profile(imsrg_ctx ctx, int enable);
 /* bracket only the classes in mask (bit k = class k; 0 turns profiling off):
