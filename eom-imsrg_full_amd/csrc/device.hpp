@@ -1,5 +1,6 @@
 // device.hpp -- device-side view of the layout and the kernel launchers.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -101,8 +102,9 @@ struct I8Layout {
   const int *v_ld;
   const long long *c_poff, *c_soff;
   const int *c_ld;
-  const int2 *c_strips;  // (Pandya block, first row) per 64-row strip
-  int n_strips;
+  const int4 *c_items;   // (Pandya block, first row, first column, partial-max base) per 64 x 256 item
+  int n_items;
+  long long pm_size;      // partial row maxima per instance (64 per strip and 256-column chunk)
   long long v_ps, c_ps;  // plane strides (bytes)
   long long v_sr, c_sr;  // row scales per instance
 };
@@ -112,6 +114,25 @@ enum : unsigned {
   T110 = 1u << 0, T111 = 1u << 1, T121 = 1u << 2, T122 = 1u << 3,
   T220 = 1u << 4, T221 = 1u << 5, T222PPHH = 1u << 6, T222PH = 1u << 7
 };
+
+// Kernel-attached timing: a profiling bracket around one GEMM launch sets
+// these events, and the launch records them with the kernel dispatch itself
+// (hipExtLaunchKernelGGL) instead of as separate markers on the stream, which
+// would leave the GPU idle before and after the kernel.  Consumed (cleared)
+// by the next launch_k.
+struct KernelEvents {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+extern thread_local KernelEvents g_kernel_events;
+template <typename F, typename... Args>
+inline void launch_k(F kernel, dim3 g, dim3 b, unsigned shm, hipStream_t s, Args... args) {
+  const KernelEvents e = g_kernel_events;
+  g_kernel_events = KernelEvents();
+  if (e.start)
+    hipExtLaunchKernelGGL(kernel, g, b, shm, s, e.start, e.stop, 0u, args...);
+  else
+    hipLaunchKernelGGL(kernel, g, b, shm, s, args...);
+}
 
 // ---- launchers (kernels.hip) ----
 // Every launcher covers the instances listed in the device table `act`
@@ -212,8 +233,8 @@ void launch_adj_normal_order(hipStream_t s, const DevLayout &L, const double *Cb
 // rows of the Pandya matrices `cc`, into 4 planes + row scales
 void launch_slice_v(hipStream_t s, const DevLayout &L, const I8Layout &I, const double *op, signed char *planes,
                     double *scale, const int *act, int nact);
-void launch_slice_cc(hipStream_t s, const DevLayout &L, const I8Layout &I, const double *cc, signed char *planes,
-                     double *scale, const int *act, int nact);
+void launch_slice_cc(hipStream_t s, const DevLayout &L, const I8Layout &I, const double *cc, double *pmax,
+                     signed char *planes, double *scale, const int *act, int nact);
 // int8-sliced segmented GEMM on the 64 x 64 tile lists of the FP64 path:
 // epi 0 = 222_pp_hh (C += upper, W after nmid), epi 1 = Z (2x2-blocked upper).
 // Operand 0 / 1 planes P0 / P1 (plane stride ps), row scales S0 / S1; segment

@@ -40,6 +40,10 @@ struct imsrg_ctx_s {
   // partials) runs there while the ph chain runs on `stream`; fork/join events
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // an event recorded on `stream` after its last work (a series gate), which
+  // the next commutator's fork waits on instead of recording ev_fork; cleared
+  // by that commutator
+  hipEvent_t fork_mark = nullptr;
   bool dual = true;        // use stream2 (imsrg_ctx_set_concurrency)
   // BCH / Magnus series: the gate kernel after nested commutator n writes one
   // live flag per series position into ring slot n % kGateRing of pinned host
@@ -81,6 +85,7 @@ struct imsrg_ctx_s {
   I8Layout i8{};
   signed char *i8v[2] = {nullptr, nullptr}, *i8c[2] = {nullptr, nullptr};
   double *i8vs[2] = {nullptr, nullptr}, *i8cs[2] = {nullptr, nullptr};
+  double *i8pm = nullptr;  // partial row maxima of a Pandya matrix (k_rowmax_cc)
   const I8Prob *pp_i8 = nullptr, *ph_i8 = nullptr;
   // active-instance tables: slot 0 = all instances (constant), 1.. = driver work sets
   static constexpr int kSlots = 4;
@@ -105,6 +110,10 @@ struct imsrg_ctx_s {
   double prof_ms[IMSRG_PROF_NCLASS] = {};
   long long prof_n[IMSRG_PROF_NCLASS] = {};
 };
+
+namespace imsrg {
+thread_local KernelEvents g_kernel_events;
+}
 
 // Records the launch(es) in `stmt` as one timed interval of class `cls`
 // when profiling is on; events go on the context stream, nothing syncs.
@@ -132,6 +141,21 @@ static hipEvent_t prof_event(imsrg_ctx_s *c) {
     }                                                                        \
   } while (0)
 #define PROF(c, cls, stmt) PROFS(c, cls, (c)->stream, stmt)
+// The same for a statement that makes exactly one launch_k launch (the
+// GEMMs): the events are recorded with that kernel (KernelEvents), so the
+// bracket adds no markers to the stream.
+#define PROFK(c, cls, stmt)                                                  \
+  do {                                                                       \
+    hipEvent_t ea_ = nullptr, eb_ = nullptr;                                 \
+    if ((c)->prof_on && (((c)->prof_mask >> (cls)) & 1u)) {                 \
+      ea_ = prof_event(c);                                                   \
+      eb_ = prof_event(c);                                                   \
+      if (ea_ && eb_) g_kernel_events = KernelEvents{ea_, eb_};              \
+    }                                                                        \
+    stmt;                                                                    \
+    g_kernel_events = KernelEvents();                                        \
+    if (ea_ && eb_) (c)->pend.push_back({(cls), ea_, eb_});                  \
+  } while (0)
 
 struct imsrg_op_s {
   imsrg_ctx_s *ctx = nullptr;
@@ -621,9 +645,9 @@ static int i8_setup(imsrg_ctx_s *c) {
   auto pad = [](long long x) { return (x + 63) & ~63LL; };
   std::vector<long long> vpoff(Ly.nblocks, 0), vsoff(Ly.nblocks, 0), cpoff(Ly.ncc, 0), csoff(Ly.ncc, 0);
   std::vector<int> vld(Ly.nblocks, 0), cld(Ly.ncc, 0);
-  std::vector<int2> strips;
+  std::vector<int4> items;
   std::vector<I8Prob> pp, ph;
-  long long pb = 0, rb = 0;
+  long long pb = 0, rb = 0, pmb = 0;
   int kmax = 0;
   for (int q = 0; q < Ly.nblocks; q++) {
     const int dim = Ly.bdim[q], np = Ly.bnp[q], nh = Ly.bnh[q];
@@ -671,14 +695,19 @@ static int i8_setup(imsrg_ctx_s *c) {
     P.seg[1] = I8Seg{0, ld, 1, 0, 1.0};
     ph.push_back(P);
     if (nbq > 0)
-      for (int r0 = 0; r0 < r; r0 += 64) strips.push_back(make_int2(q, r0));
+      for (int r0 = 0; r0 < r; r0 += 64) {
+        const int nck = (ld + 255) / 256;
+        for (int k0 = 0; k0 < ld; k0 += 256) items.push_back(make_int4(q, r0, k0, (int)pmb));
+        pmb += 64LL * nck;
+      }
     kmax = std::max(kmax, ld);
     pb += pad(r) * ld;
     rb += pad(r);
   }
   c->i8.c_ps = (pb + 255) & ~255LL;
   c->i8.c_sr = std::max<long long>(rb, 1);
-  c->i8.n_strips = (int)strips.size();
+  c->i8.n_items = (int)items.size();
+  c->i8.pm_size = std::max<long long>(pmb, 1);
   // a level sums at most kI8Planes products of |a_i b_j| <= 64^2 over K
   if ((long long)kmax * kI8Planes * 64 * 64 >= (1LL << 31)) return fail(IMSRG_EINVAL, "int8 slices: K too long");
   int rc = upload_table(c, vpoff, &c->i8.v_poff);
@@ -687,7 +716,8 @@ static int i8_setup(imsrg_ctx_s *c) {
   if (!rc) rc = upload_table(c, cpoff, &c->i8.c_poff);
   if (!rc) rc = upload_table(c, csoff, &c->i8.c_soff);
   if (!rc) rc = upload_table(c, cld, &c->i8.c_ld);
-  if (!rc) rc = upload_table(c, strips, &c->i8.c_strips);
+  if (!rc) rc = upload_table(c, items, &c->i8.c_items);
+  if (!rc) rc = dalloc(c, c->i8.pm_size * B, &c->i8pm);
   if (!rc) rc = upload_table(c, pp, &c->pp_i8);
   if (!rc) rc = upload_table(c, ph, &c->ph_i8);
   for (int k = 0; k < 2 && !rc; k++) {
@@ -979,10 +1009,12 @@ static int autotune(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R) {
 static int comm_impl(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *RES, unsigned mask,
                      bool lcc_ready, const ActSet &a, const CommEpi &epi = CommEpi()) {
   if (c->prec == IMSRG_PREC_FP64 && c->shape_pp < 0 && a.n() > 0 && (mask & (T222PPHH | T221 | T222PH))) {
+    c->fork_mark = nullptr;  // the tuning launches follow the mark on the stream
     int rc = autotune(c, L, R);
     if (rc) return rc;
   }
   if (!c->graphs || c->prof_on || a.n() == 0) return comm_launches(c, L, R, RES, mask, lcc_ready, a, epi);
+  c->fork_mark = nullptr;  // a replayed graph runs on one stream
   const imsrg_ctx_s::GraphKey key(L->d, R->d, RES->d, L->herm, R->herm, RES->herm, mask, lcc_ready ? 1 : 0, a.d,
                                   a.n(), c->prec, epi.acc, epi.coef, epi.norm ? 1 : 0);
   auto it = c->gexec.find(key);
@@ -1017,7 +1049,10 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
   const int hL = L->herm, hR = R->herm, hres = RES->herm;
   const int *ad = a.d, na = a.n();
   const long long OS = D.op_size;
-  if (na == 0) return 0;
+  if (na == 0) {
+    c->fork_mark = nullptr;
+    return 0;
+  }
   // The pp/hh chain (E0, 122 or clear, pp/hh GEMM, one-body partials) and the
   // ph chain (Pandya, Z GEMM, J' terms) touch disjoint outputs and scratch,
   // so they run on two streams; both end before the ph apply accumulates
@@ -1025,9 +1060,10 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
   const bool dual = c->dual && !c->capturing;
   hipStream_t s2 = dual ? c->stream2 : s;
   if (dual) {
-    HIPCHK(hipEventRecord(c->ev_fork, s));
-    HIPCHK(hipStreamWaitEvent(s2, c->ev_fork, 0));
+    if (!c->fork_mark) HIPCHK(hipEventRecord(c->ev_fork, s));
+    HIPCHK(hipStreamWaitEvent(s2, c->fork_mark ? c->fork_mark : c->ev_fork, 0));
   }
+  c->fork_mark = nullptr;
   if (mask & (T110 | T220))
     PROFS(c, IMSRG_PROF_E0, s2, launch_e0(s2, D, L->d, R->d, RES->d, hL, hR, mask, c->partials, ad, na, OS));
   if (mask & T122) PROFS(c, IMSRG_PROF_122, s2, launch_122(s2, D, L->d, R->d, RES->d, hres, ad, na));
@@ -1039,14 +1075,14 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
       launch_slice_v(s2, D, c->i8, R->d, c->i8v[1], c->i8vs[1], ad, na);
     });
     // B rows are rows of the other operand: X[k][n] = herm * X[n][k]
-    PROFS(c, IMSRG_PROF_GEMM_PPHH, s2,
+    PROFK(c, IMSRG_PROF_GEMM_PPHH,
           launch_seg_gemm_i8(s2, 0, c->pp_i8, c->pp_tiles, c->n_pp_tiles, c->i8v[0], c->i8v[1], c->i8vs[0],
                              c->i8vs[1], c->i8.v_ps, (double)hR, (double)hL,
                              (mask & T222PPHH) ? RES->d + D.v_off : nullptr, (mask & T221) ? c->wpp : nullptr, 0.0,
                              ad, na, kI8Planes * c->i8.v_ps, c->i8.v_sr, OS, D.vsize));
   } else if (mask & (T222PPHH | T221)) {
     const bool w = c->prec == IMSRG_PREC_FP64 && c->shape_pp == 1;
-    PROFS(c, IMSRG_PROF_GEMM_PPHH, s2,
+    PROFK(c, IMSRG_PROF_GEMM_PPHH,
           launch_seg_gemm_pphh(s2, w ? 3 : c->prec, c->pp_probs, w ? c->pp_tiles_w : c->pp_tiles,
                                w ? c->n_pp_tiles_w : c->n_pp_tiles, L->d + D.v_off, R->d + D.v_off,
                                (mask & T222PPHH) ? RES->d + D.v_off : nullptr, (mask & T221) ? c->wpp : nullptr,
@@ -1059,17 +1095,17 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
     PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, R->d, c->cc[1], ad, na));
     if (i8) {
       PROF(c, IMSRG_PROF_SLICE, {
-        if (!lcc_ready) launch_slice_cc(s, D, c->i8, c->cc[0], c->i8c[0], c->i8cs[0], ad, na);
-        launch_slice_cc(s, D, c->i8, c->cc[1], c->i8c[1], c->i8cs[1], ad, na);
+        if (!lcc_ready) launch_slice_cc(s, D, c->i8, c->cc[0], c->i8pm, c->i8c[0], c->i8cs[0], ad, na);
+        launch_slice_cc(s, D, c->i8, c->cc[1], c->i8pm, c->i8c[1], c->i8cs[1], ad, na);
       });
-      PROF(c, IMSRG_PROF_GEMM_PH,
-           launch_seg_gemm_i8(s, 1, c->ph_i8, c->ph_tiles, c->n_ph_tiles, c->i8c[0], c->i8c[1], c->i8cs[0],
+      PROFK(c, IMSRG_PROF_GEMM_PH,
+            launch_seg_gemm_i8(s, 1, c->ph_i8, c->ph_tiles, c->n_ph_tiles, c->i8c[0], c->i8c[1], c->i8cs[0],
                               c->i8cs[1], c->i8.c_ps, (double)hL, -(double)hR, c->wph, nullptr,
                               -(double)hL * (double)hR, ad, na, kI8Planes * c->i8.c_ps, c->i8.c_sr, D.cwsize, 0));
     } else {
       const bool w = c->prec == IMSRG_PREC_FP64 && c->shape_ph == 1;
-      PROF(c, IMSRG_PROF_GEMM_PH,
-           launch_seg_gemm_z(s, w ? 3 : c->prec, c->ph_probs, w ? c->ph_tiles_w : c->ph_tiles,
+      PROFK(c, IMSRG_PROF_GEMM_PH,
+            launch_seg_gemm_z(s, w ? 3 : c->prec, c->ph_probs, w ? c->ph_tiles_w : c->ph_tiles,
                              w ? c->n_ph_tiles_w : c->n_ph_tiles, c->cc[0], c->cc[1], c->wph, (double)hL,
                              (double)hR, ad, na, D.ccsize, D.cwsize));
     }
@@ -1251,7 +1287,7 @@ static void prep_L(imsrg_ctx_s *c, imsrg_op_s *G, const ActSet &set) {
   launch_pandya(c->stream, c->dl, G->d, c->cc[0], set.d, set.n());
   if (c->prec == IMSRG_PREC_INT8X5) {
     launch_slice_v(c->stream, c->dl, c->i8, G->d, c->i8v[0], c->i8vs[0], set.d, set.n());
-    launch_slice_cc(c->stream, c->dl, c->i8, c->cc[0], c->i8c[0], c->i8cs[0], set.d, set.n());
+    launch_slice_cc(c->stream, c->dl, c->i8, c->cc[0], c->i8pm, c->i8c[0], c->i8cs[0], set.d, set.n());
   }
 }
 // The series gate after nested commutator n: the test on the device, its
@@ -1306,6 +1342,8 @@ static int bch_impl(imsrg_ctx_s *c, imsrg_op_s *HS, imsrg_op_s *G, imsrg_op_s *H
     epi.acc = HS->d;
     epi.coef = coef;
     epi.norm = true;
+    // nothing was enqueued since the last gate: the second stream can fork from its event
+    c->fork_mark = iw > 2 ? c->ev_gate[(iw - 1) % imsrg_ctx_s::kGateRing] : nullptr;
     int rc = comm_impl(c, G, AD, INT2, IMSRG_TALL, true, cur, epi);
     if (!rc) rc = series_gate(c, 1, cur, iw, 0, coef, -1, 0);
     if (!rc && iw > 2) rc = series_live(c, cur, iw - 1, live, nlive);
@@ -1360,6 +1398,7 @@ static int magexp_impl(imsrg_ctx_s *c, imsrg_op_s *DG, imsrg_op_s *G, const ActS
     epi.acc = DG->d;
     epi.coef = cof[i - 1];
     epi.norm = true;
+    c->fork_mark = i > 2 ? c->ev_gate[(i - 1) % imsrg_ctx_s::kGateRing] : nullptr;
     rc = comm_impl(c, G, cur, oth, IMSRG_TALL & ~(T110 | T220), true, set, epi);
     // advals(i) = ||oth|| |cof(i)|, then the test for commutator i+1
     if (!rc && i < 7) rc = series_gate(c, 2, set, i, 1, std::fabs(cof[i - 1]), i - 1, i - q);

@@ -773,18 +773,18 @@ void launch_seg_gemm_pphh(hipStream_t s, int mode, const SegProb *probs, const i
   if (ntiles <= 0 || nact <= 0) return;
   const dim3 g(ntiles, nact), b(256);
   if (mode == 3) {  // FP64 on 128 x 64 tiles (tiles built for that shape)
-    hipLaunchKernelGGL((k_seg_gemm_w<false, 0>), g, b, 0, s, probs, tiles, ntiles, X0, X1, 1.0, 1.0, C, W, 0.0, act,
+    launch_k(k_seg_gemm_w<false, 0>, g, b, 0, s, probs, tiles, ntiles, X0, X1, 1.0, 1.0, C, W, 0.0, act,
                        sop, sop, sw);
     return;
   }
   if (mode == 1)
-    hipLaunchKernelGGL((k_seg_gemm<false, 1, 0>), g, b, 0, s, probs, tiles, ntiles, X0, X1, 1.0, 1.0, C, W, 0.0,
+    launch_k(k_seg_gemm<false, 1, 0>, g, b, 0, s, probs, tiles, ntiles, X0, X1, 1.0, 1.0, C, W, 0.0,
                        act, sop, sop, sw);
   else if (mode == 2)
-    hipLaunchKernelGGL((k_seg_gemm<false, 2, 0>), g, b, 0, s, probs, tiles, ntiles, X0, X1, 1.0, 1.0, C, W, 0.0,
+    launch_k(k_seg_gemm<false, 2, 0>, g, b, 0, s, probs, tiles, ntiles, X0, X1, 1.0, 1.0, C, W, 0.0,
                        act, sop, sop, sw);
   else
-    hipLaunchKernelGGL((k_seg_gemm<false, 0, 0>), g, b, 0, s, probs, tiles, ntiles, X0, X1, 1.0, 1.0, C, W, 0.0,
+    launch_k(k_seg_gemm<false, 0, 0>, g, b, 0, s, probs, tiles, ntiles, X0, X1, 1.0, 1.0, C, W, 0.0,
                        act, sop, sop, sw);
 }
 void launch_seg_gemm_z(hipStream_t s, int mode, const SegProb *probs, const int4 *tiles, int ntiles,
@@ -794,18 +794,18 @@ void launch_seg_gemm_z(hipStream_t s, int mode, const SegProb *probs, const int4
   const dim3 g(ntiles, nact), b(256);
   const double g0 = hL, g1 = -hR, sigma = -hL * hR;
   if (mode == 3) {  // FP64 on 128 x 64 tiles
-    hipLaunchKernelGGL((k_seg_gemm_w<true, 1>), g, b, 0, s, probs, tiles, ntiles, LCC, RCC, g0, g1, Z, nullptr, sigma,
+    launch_k(k_seg_gemm_w<true, 1>, g, b, 0, s, probs, tiles, ntiles, LCC, RCC, g0, g1, Z, nullptr, sigma,
                        act, scc, sz, 0LL);
     return;
   }
   if (mode == 1)
-    hipLaunchKernelGGL((k_seg_gemm<true, 1, 1>), g, b, 0, s, probs, tiles, ntiles, LCC, RCC, g0, g1, Z, nullptr,
+    launch_k(k_seg_gemm<true, 1, 1>, g, b, 0, s, probs, tiles, ntiles, LCC, RCC, g0, g1, Z, nullptr,
                        sigma, act, scc, sz, 0LL);
   else if (mode == 2)
-    hipLaunchKernelGGL((k_seg_gemm<true, 2, 1>), g, b, 0, s, probs, tiles, ntiles, LCC, RCC, g0, g1, Z, nullptr,
+    launch_k(k_seg_gemm<true, 2, 1>, g, b, 0, s, probs, tiles, ntiles, LCC, RCC, g0, g1, Z, nullptr,
                        sigma, act, scc, sz, 0LL);
   else
-    hipLaunchKernelGGL((k_seg_gemm<true, 0, 1>), g, b, 0, s, probs, tiles, ntiles, LCC, RCC, g0, g1, Z, nullptr,
+    launch_k(k_seg_gemm<true, 0, 1>, g, b, 0, s, probs, tiles, ntiles, LCC, RCC, g0, g1, Z, nullptr,
                        sigma, act, scc, sz, 0LL);
 }
 

@@ -47,6 +47,11 @@ __device__ __forceinline__ int row_exp(double m) {
 // V rows: one wavefront per pair row (four per workgroup).  The pp columns
 // [0, np) go to plane columns [0, np), the hh columns [np+nb, dim) to
 // [P, P+nh) with P = 64*ceil(np/64); the padding columns are written as zero.
+// Lane l owns plane columns 4l + 256t .. +3; rows up to 1024 plane columns
+// keep their values in registers between the maximum and the slicing.
+__device__ __forceinline__ int v_col(int j, int np, int nb, int nh, int P) {
+  return j < np ? j : (j >= P && j < P + nh) ? np + nb + (j - P) : -1;
+}
 __global__ __launch_bounds__(256) void k_slice_v(DevLayout L, I8Layout I, const double *__restrict__ op,
                                                  signed char *__restrict__ planes, double *__restrict__ scale,
                                                  const int *act) {
@@ -64,62 +69,106 @@ __global__ __launch_bounds__(256) void k_slice_v(DevLayout L, I8Layout I, const 
   const int rl = r - L.pbase[q], dim = L.bdim[q], np = L.bnp[q], nb = L.bnb[q], nh = L.bnh[q];
   const int P = (np + 63) & ~63;
   const double *X = op + L.v_off + L.boff[q] + (long long)rl * dim;
+  constexpr int TC = 4;  // cached passes of 256 plane columns
+  double v[TC][4];
   double m = 0.0;
-#pragma unroll 4
-  for (int j = lane; j < np + nh; j += 64) m = fmax(m, fabs(X[j < np ? j : nb + j]));
+#pragma unroll
+  for (int t = 0; t < TC; t++)
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int j = 256 * t + 4 * lane + u;
+      const int c = j < ld ? v_col(j, np, nb, nh, P) : -1;
+      v[t][u] = c >= 0 ? X[c] : 0.0;
+      m = fmax(m, fabs(v[t][u]));
+    }
+  for (int j = 256 * TC + lane; j < ld; j += 64) {
+    const int c = v_col(j, np, nb, nh, P);
+    if (c >= 0) m = fmax(m, fabs(X[c]));
+  }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
   const int ex = row_exp(m);
   if (lane == 0) scale[I.v_soff[q] + rl] = m > 0.0 ? ldexp(1.0, ex) : 0.0;
   signed char *row = planes + I.v_poff[q] + (long long)rl * ld;
-  for (int j4 = lane * 4; j4 < ld; j4 += 256) {
+  auto put = [&](int j4, const double x[4]) {
     unsigned w[NS] = {};
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int j = j4 + u;
-      const int c = j < np ? j : (j >= P && j < P + nh) ? np + nb + (j - P) : -1;
       unsigned b[NS] = {};
-      if (c >= 0 && m > 0.0) slices_of(X[c], ex, b);
+      if (m > 0.0) slices_of(x[u], ex, b);
 #pragma unroll
       for (int i = 0; i < NS; i++) w[i] |= b[i] << (8 * u);
     }
 #pragma unroll
     for (int i = 0; i < NS; i++) *(unsigned *)(row + i * I.v_ps + j4) = w[i];
+  };
+#pragma unroll
+  for (int t = 0; t < TC; t++) {
+    const int j4 = 256 * t + 4 * lane;
+    if (j4 < ld) put(j4, v[t]);
+  }
+  for (int j4 = 256 * TC + 4 * lane; j4 < ld; j4 += 256) {
+    double x[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int c = v_col(j4 + u, np, nb, nh, P);
+      x[u] = c >= 0 ? X[c] : 0.0;
+    }
+    put(j4, x);
   }
 }
 
-// Pandya rows: one workgroup per 64-row strip of a Pandya block (column-major
-// r x K in `cc`).  Pass 1: row maxima (64 consecutive rows per wavefront load);
-// pass 2: per 64-column chunk each thread slices 16 consecutive columns of one
-// row and stores 16 bytes per plane.  Rows r..64*ceil(r/64) are written as zero.
-__global__ __launch_bounds__(256) void k_slice_cc(DevLayout L, I8Layout I, const double *__restrict__ cc,
-                                                  signed char *__restrict__ planes, double *__restrict__ scale,
-                                                  const int *act) {
+// Pandya rows (column-major r x K per block in `cc`), in items of 64 rows x
+// 256 columns.  k_rowmax_cc: the maximum of each row over the item's columns
+// (64 consecutive rows per wavefront load) into the strip's partial maxima;
+// k_slice_cc: the row maximum from those partials, then each thread slices 16
+// consecutive columns of one row per 64-column chunk and stores 16 bytes per
+// plane.  Rows r..64*ceil(r/64) are written as zero.
+__global__ __launch_bounds__(256) void k_rowmax_cc(DevLayout L, I8Layout I, const double *__restrict__ cc,
+                                                   double *__restrict__ pmax, const int *act) {
   __shared__ double red[4][64];
-  const int2 it = I.c_strips[blockIdx.x];
+  const int4 it = I.c_items[blockIdx.x];
   {
     const long long inst = inst_id(act);
     if (inst < 0) return;
     cc += inst * L.ccsize;
-    planes += inst * NS * I.c_ps;
-    scale += inst * I.c_sr;
+    pmax += inst * I.pm_size;
   }
-  const int q = it.x, r0 = it.y, tid = threadIdx.x, rloc = tid & 63, kq = tid >> 6;
-  const int R = L.ccrlen[q], K = L.ccnph[q], ld = I.c_ld[q], row = r0 + rloc;
+  const int q = it.x, r0 = it.y, k0 = it.z, tid = threadIdx.x, rloc = tid & 63, kq = tid >> 6;
+  const int R = L.ccrlen[q], K = L.ccnph[q], row = r0 + rloc, k1 = min(k0 + 256, K);
   const double *src = cc + L.ccoff[q];
   double m = 0.0;
   if (row < R) {
 #pragma unroll 8
-    for (int k = kq; k < K; k += 4) m = fmax(m, fabs(src[(long long)k * R + row]));
+    for (int k = k0 + kq; k < k1; k += 4) m = fmax(m, fabs(src[(long long)k * R + row]));
   }
   red[kq][rloc] = m;
   __syncthreads();
-  m = fmax(fmax(red[0][rloc], red[1][rloc]), fmax(red[2][rloc], red[3][rloc]));
+  if (kq == 0)
+    pmax[it.w + (k0 >> 8) * 64 + rloc] = fmax(fmax(red[0][rloc], red[1][rloc]), fmax(red[2][rloc], red[3][rloc]));
+}
+__global__ __launch_bounds__(256) void k_slice_cc(DevLayout L, I8Layout I, const double *__restrict__ cc,
+                                                  const double *__restrict__ pmax, signed char *__restrict__ planes,
+                                                  double *__restrict__ scale, const int *act) {
+  const int4 it = I.c_items[blockIdx.x];
+  {
+    const long long inst = inst_id(act);
+    if (inst < 0) return;
+    cc += inst * L.ccsize;
+    pmax += inst * I.pm_size;
+    planes += inst * NS * I.c_ps;
+    scale += inst * I.c_sr;
+  }
+  const int q = it.x, r0 = it.y, k0 = it.z, tid = threadIdx.x, rloc = tid & 63, kq = tid >> 6;
+  const int R = L.ccrlen[q], K = L.ccnph[q], ld = I.c_ld[q], row = r0 + rloc, nck = (ld + 255) >> 8;
+  double m = 0.0;
+  for (int c = 0; c < nck; c++) m = fmax(m, pmax[it.w + c * 64 + rloc]);
   const int ex = row_exp(m);
-  if (kq == 0) scale[I.c_soff[q] + row] = m > 0.0 ? ldexp(1.0, ex) : 0.0;
+  if (k0 == 0 && kq == 0) scale[I.c_soff[q] + row] = m > 0.0 ? ldexp(1.0, ex) : 0.0;
+  const double *src = cc + L.ccoff[q];
   signed char *dst = planes + I.c_poff[q] + (long long)row * ld;
-  for (int kc = 0; kc < ld; kc += 64) {
-    const int k0 = kc + kq * 16;
+  for (int kc = k0; kc < min(k0 + 256, ld); kc += 64) {
+    const int kb = kc + kq * 16;
     unsigned w[NS][4];
 #pragma unroll
     for (int g = 0; g < 4; g++) {
@@ -127,7 +176,7 @@ __global__ __launch_bounds__(256) void k_slice_cc(DevLayout L, I8Layout I, const
       for (int i = 0; i < NS; i++) w[i][g] = 0;
 #pragma unroll
       for (int u = 0; u < 4; u++) {
-        const int k = k0 + 4 * g + u;
+        const int k = kb + 4 * g + u;
         unsigned b[NS] = {};
         if (row < R && k < K && m > 0.0) slices_of(src[(long long)k * R + row], ex, b);
 #pragma unroll
@@ -137,7 +186,7 @@ __global__ __launch_bounds__(256) void k_slice_cc(DevLayout L, I8Layout I, const
 #pragma unroll
     for (int i = 0; i < NS; i++) {
       const v4i_t v = {(int)w[i][0], (int)w[i][1], (int)w[i][2], (int)w[i][3]};
-      *(v4i_t *)(dst + i * I.c_ps + k0) = v;
+      *(v4i_t *)(dst + i * I.c_ps + kb) = v;
     }
   }
 }
@@ -147,10 +196,11 @@ void launch_slice_v(hipStream_t s, const DevLayout &L, const I8Layout &I, const 
   if (nact <= 0 || L.npairs <= 0) return;
   hipLaunchKernelGGL(k_slice_v, dim3((L.npairs + 3) / 4, nact), dim3(256), 0, s, L, I, op, planes, scale, act);
 }
-void launch_slice_cc(hipStream_t s, const DevLayout &L, const I8Layout &I, const double *cc, signed char *planes,
-                     double *scale, const int *act, int nact) {
-  if (nact <= 0 || I.n_strips <= 0) return;
-  hipLaunchKernelGGL(k_slice_cc, dim3(I.n_strips, nact), dim3(256), 0, s, L, I, cc, planes, scale, act);
+void launch_slice_cc(hipStream_t s, const DevLayout &L, const I8Layout &I, const double *cc, double *pmax,
+                     signed char *planes, double *scale, const int *act, int nact) {
+  if (nact <= 0 || I.n_items <= 0) return;
+  hipLaunchKernelGGL(k_rowmax_cc, dim3(I.n_items, nact), dim3(256), 0, s, L, I, cc, pmax, act);
+  hipLaunchKernelGGL(k_slice_cc, dim3(I.n_items, nact), dim3(256), 0, s, L, I, cc, pmax, planes, scale, act);
 }
 
 // ---- the product.  A 64 x 64 output tile per workgroup; each K chunk of 64
@@ -329,10 +379,10 @@ void launch_seg_gemm_i8(hipStream_t s, int epi, const I8Prob *probs, const int4 
   if (ntiles <= 0 || nact <= 0) return;
   const dim3 g(ntiles, nact), b(256);
   if (epi == 0)
-    hipLaunchKernelGGL(k_seg_gemm_i8<0>, g, b, 0, s, probs, tiles, ntiles, P0, P1, S0, S1, ps, g0, g1, C, W, sigma,
+    launch_k(k_seg_gemm_i8<0>, g, b, 0, s, probs, tiles, ntiles, P0, P1, S0, S1, ps, g0, g1, C, W, sigma,
                        act, sp, ss, sc, sw);
   else
-    hipLaunchKernelGGL(k_seg_gemm_i8<1>, g, b, 0, s, probs, tiles, ntiles, P0, P1, S0, S1, ps, g0, g1, C, W, sigma,
+    launch_k(k_seg_gemm_i8<1>, g, b, 0, s, probs, tiles, ntiles, P0, P1, S0, S1, ps, g0, g1, C, W, sigma,
                        act, sp, ss, sc, sw);
 }
 
