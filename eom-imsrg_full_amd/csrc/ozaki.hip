@@ -206,15 +206,17 @@ void launch_slice_cc(hipStream_t s, const DevLayout &L, const I8Layout &I, const
 // ---- the product.  A 64 x 64 output tile per workgroup; each K chunk of 64
 // bytes of the NS planes of A (64 rows) and B (64 rows) is loaded into
 // registers one chunk ahead and staged in LDS (16-byte column group g of row
-// r at g ^ ((r >> 2) & 3), so the 16 rows of a fragment read hit distinct
-// banks).  Wavefront (wm, wn) owns the 32 x 32 quadrant (rows 32 wm, columns
-// 32 wn) as 2 x 2 sub-tiles; fragments: lane l holds row l % 16 of a 16-row
-// group and K bytes 16 (l / 16) .. +15 of the chunk, for A and B alike; the
-// int32 result (i, j) sits in lane j + 16 (i / 4), element i % 4.
+// r at g ^ ((r >> 2) & 3), so the 16 rows a quarter-wavefront reads hit
+// distinct banks).  Wavefront (wm, wn) owns the 32 x 32 quadrant (rows 32 wm,
+// columns 32 wn) as one v_mfma_i32_32x32x32_i8 tile per level (the 16x16x64
+// form issues at half its rate).  Fragments: lane l holds row l % 32 and the
+// K bytes 16 (l / 32) .. +15 of a 32-byte K step, for A and B alike; result
+// element v of lane l is (8 (v / 4) + 4 (l / 32) + v % 4, l % 32).
 struct I8Smem {
   v4i_t a[NS][64][4];
   v4i_t b[NS][64][4];
 };
+typedef int v16i_t __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ int swz(int r, int g) { return g ^ ((r >> 2) & 3); }
 
@@ -251,13 +253,12 @@ __global__ __launch_bounds__(256, 2) void k_seg_gemm_i8(const I8Prob *__restrict
   const long long ld = p.ld;
   // staging: thread tid loads 16 bytes (group tid % 4) of tile row tid / 4 per plane
   const int sr = tid >> 2, sg = tid & 3, sgp = swz(sr, sg);
-  // fragment reads: row (lane % 16) of each 16-row group, column group lane / 16
-  const int fr = lane & 15, fg = swz(fr, lane >> 4);
-  double4_t acc[2][2];
+  // fragment reads: row lane % 32 of the quadrant, column group 2 h + lane / 32 of K step h
+  const int fr = lane & 31, fh = lane >> 5;
+  const int fra = wm * 32 + fr, frb = wn * 32 + fr;
+  double acc[16];
 #pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  for (int v = 0; v < 16; v++) acc[v] = 0.0;
 
   auto run = [&](int s0, int s1) {
     for (int s = s0; s < s1; s++) {
@@ -275,13 +276,11 @@ __global__ __launch_bounds__(256, 2) void k_seg_gemm_i8(const I8Prob *__restrict
           R[1][i] = *(const v4i_t *)(B + i * ps + kc);
         }
       };
-      v4i_t lv[NS][2][2];
+      v16i_t lv[NS];
 #pragma unroll
       for (int l = 0; l < NS; l++)
 #pragma unroll
-        for (int i = 0; i < 2; i++)
-#pragma unroll
-          for (int j = 0; j < 2; j++) lv[l][i][j] = (v4i_t){0, 0, 0, 0};
+        for (int v = 0; v < 16; v++) lv[l][v] = 0;
       load(0);
       for (int kc = 0; kc < K; kc += 64) {  // K is workgroup-uniform: every barrier is reached by all
         __syncthreads();  // the previous chunk has been read
@@ -293,58 +292,42 @@ __global__ __launch_bounds__(256, 2) void k_seg_gemm_i8(const I8Prob *__restrict
         __syncthreads();
         if (kc + 64 < K) load(kc + 64);
         if (active) {
-          // all B fragments, then A plane by plane: slice pair (i, j) adds to level i + j
-          v4i_t b[NS][2];
 #pragma unroll
-          for (int j = 0; j < NS; j++)
+          for (int h = 0; h < 2; h++) {
+            // all B fragments of this K step, then A plane by plane: pair (i, j) adds to level i + j
+            v4i_t b[NS];
 #pragma unroll
-            for (int u = 0; u < 2; u++) b[j][u] = S.b[j][wn * 32 + u * 16 + fr][fg];
+            for (int j = 0; j < NS; j++) b[j] = S.b[j][frb][swz(frb, 2 * h + fh)];
 #pragma unroll
-          for (int i = 0; i < NS; i++) {
-            v4i_t a[2];
+            for (int i = 0; i < NS; i++) {
+              const v4i_t a = S.a[i][fra][swz(fra, 2 * h + fh)];
 #pragma unroll
-            for (int u = 0; u < 2; u++) a[u] = S.a[i][wm * 32 + u * 16 + fr][fg];
-#pragma unroll
-            for (int j = 0; j + i < NS; j++)
-#pragma unroll
-              for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-                for (int tj = 0; tj < 2; tj++)
-                  lv[i + j][ti][tj] =
-                      __builtin_amdgcn_mfma_i32_16x16x64_i8(a[ti], b[j][tj], lv[i + j][ti][tj], 0, 0, 0);
+              for (int j = 0; j + i < NS; j++)
+                lv[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b[j], lv[i + j], 0, 0, 0);
+            }
           }
         }
       }
       if (active) {
         const double *SA = (o ? S1 : S0) + p.s_off, *SB = (o ? S0 : S1) + p.s_off;
-        const double sgn = sgm.sign * (o ? g1 : g0);
+        const double sb = SB[cb + fr] * (sgm.sign * (o ? g1 : g0));
 #pragma unroll
-        for (int ti = 0; ti < 2; ti++)
+        for (int v = 0; v < 16; v++) {
+          const double sa = SA[ra + 8 * (v >> 2) + 4 * fh + (v & 3)];
+          double lev = (double)lv[NS - 1][v];
 #pragma unroll
-          for (int tj = 0; tj < 2; tj++) {
-            const double sb = SB[cb + tj * 16 + fr] * sgn;
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-              const double sa = SA[ra + ti * 16 + 4 * (lane >> 4) + v];
-              double lev = (double)lv[NS - 1][ti][tj][v];
-#pragma unroll
-              for (int l = NS - 2; l >= 0; l--) lev = (double)lv[l][ti][tj][v] + lev * 0x1p-7;
-              acc[ti][tj][v] += sa * sb * (lev * 0x1p-14);
-            }
-          }
+          for (int l = NS - 2; l >= 0; l--) lev = (double)lv[l][v] + lev * 0x1p-7;
+          acc[v] += sa * sb * (lev * 0x1p-14);
+        }
       }
     }
   };
   auto each = [&](auto &&f) {
 #pragma unroll
-    for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-      for (int tj = 0; tj < 2; tj++)
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-          const int row = ra + ti * 16 + 4 * (lane >> 4) + v, col = cb + tj * 16 + fr;
-          if (row < M && col < M && row <= col) f(row, col, acc[ti][tj][v]);
-        }
+    for (int v = 0; v < 16; v++) {
+      const int row = ra + 8 * (v >> 2) + 4 * fh + (v & 3), col = cb + fr;
+      if (row < M && col < M && row <= col) f(row, col, acc[v]);
+    }
   };
   if (EPI == 0) {
     run(0, p.nmid);
