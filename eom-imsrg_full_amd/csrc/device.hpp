@@ -77,10 +77,12 @@ struct SegProb {
 
 // ---- int8-sliced products (the Ozaki scheme on the int8 matrix cores).  An
 // operand row r is held as x = s_r * sum_{i=1..5} a_i 2^(-7i): a power-of-two
-// row scale s_r (max |x| / s_r in [1/4, 1/2)) and five int8 planes of rounded
-// slices a_i in [-64, 64], each row K-contiguous and zero-padded to 64 rows /
-// 64 columns.  A product keeps the slice pairs with i + j <= 6 (fifteen int8
-// products, exact int32 sums per level i + j).
+// row scale s_r (max |x| / s_r in [1/4, 1/2)) and five int8 slices a_i in
+// [-64, 64], zero-padded to 64 rows / 64 columns.  The five slices of each
+// 64-column chunk of a row lie together: slice i of column k of row r is at
+// byte r * 5 ld + (k / 64) * 320 + 64 i + k % 64 of the block, so one K chunk
+// of a row is 320 contiguous bytes.  A product keeps the slice pairs with
+// i + j <= 6 (fifteen int8 products, exact int32 sums per level i + j).
 constexpr int kI8Planes = 5;
 struct I8Seg {
   int kofs, K, src, pad;  // K window of the segment in the block's plane rows (multiples of 64)
@@ -89,9 +91,9 @@ struct I8Seg {
 // C (M x M) = sum of nseg segments  sign * A_src * B_other^T  over plane rows
 // of the block (A rows m, B rows n); c_off, w_off, nmid as in SegProb.
 struct I8Prob {
-  int M, nseg, nmid, ld;  // ld: row stride of the block in a plane (bytes)
+  int M, nseg, nmid, ld;  // ld: K extent of the block's rows (bytes per slice; the row stride is 5 ld)
   long long c_off, w_off;
-  long long p_off;        // byte offset of the block's row 0 in a plane
+  long long p_off;        // byte offset of the block's row 0
   long long s_off;        // index of the block's first row scale
   I8Seg seg[4];
 };
@@ -105,7 +107,7 @@ struct I8Layout {
   const int4 *c_items;   // (Pandya block, first row, first column, partial-max base) per 64 x 256 item
   int n_items;
   long long pm_size;      // partial row maxima per instance (64 per strip and 256-column chunk)
-  long long v_ps, c_ps;  // plane strides (bytes)
+  long long v_bytes, c_bytes;  // slices of one operand's V rows / one Pandya matrix, per instance
   long long v_sr, c_sr;  // row scales per instance
 };
 
@@ -237,12 +239,12 @@ void launch_slice_cc(hipStream_t s, const DevLayout &L, const I8Layout &I, const
                      signed char *planes, double *scale, const int *act, int nact);
 // int8-sliced segmented GEMM on the 64 x 64 tile lists of the FP64 path:
 // epi 0 = 222_pp_hh (C += upper, W after nmid), epi 1 = Z (2x2-blocked upper).
-// Operand 0 / 1 planes P0 / P1 (plane stride ps), row scales S0 / S1; segment
-// factor sign * (src ? g1 : g0).
+// Operand 0 / 1 slices P0 / P1, row scales S0 / S1; segment factor
+// sign * (src ? g1 : g0).
 void launch_seg_gemm_i8(hipStream_t s, int epi, const I8Prob *probs, const int4 *tiles, int ntiles,
                         const signed char *P0, const signed char *P1, const double *S0, const double *S1,
-                        long long ps, double g0, double g1, double *C, double *W, double sigma, const int *act,
-                        int nact, long long sp, long long ss, long long sc, long long sw);
+                        double g0, double g1, double *C, double *W, double sigma, const int *act, int nact,
+                        long long sp, long long ss, long long sc, long long sw);
 
 constexpr int kPartials = 1024;  // reduction partials per instance
 constexpr int kScal = 64;        // scalar slots per instance

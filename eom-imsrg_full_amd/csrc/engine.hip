@@ -634,9 +634,9 @@ int imsrg_ctx_info(imsrg_ctx c, double *info) {
 }
 
 // Tables and buffers of the int8-sliced GEMMs (ozaki.hip).  Channel block q:
-// plane rows 64*ceil(dim/64) of ld = 64*ceil(np/64) + 64*ceil(nh/64) bytes
-// (pp columns, then hh columns); Pandya block q: rows 64*ceil(r/64) of
-// 64*ceil(nph/64) bytes.  The problem tables follow the FP64 ones entry for
+// 64*ceil(dim/64) rows of ld = 64*ceil(np/64) + 64*ceil(nh/64) columns (pp
+// columns, then hh columns), 5 ld bytes each; Pandya block q: 64*ceil(r/64)
+// rows of 64*ceil(nph/64) columns.  The problem tables follow the FP64 ones entry for
 // entry, so the FP64 64 x 64 tile lists serve both.  Padding stays zero: the
 // slice kernels rewrite every byte of a used row and never touch pad rows.
 static int i8_setup(imsrg_ctx_s *c) {
@@ -670,10 +670,10 @@ static int i8_setup(imsrg_ctx_s *c) {
     P.seg[3] = I8Seg{(int)pad(np), (int)pad(nh), 1, 0, 1.0};
     pp.push_back(P);
     kmax = std::max(kmax, ld);
-    pb += pad(dim) * ld;
+    pb += pad(dim) * ld * kI8Planes;
     rb += pad(dim);
   }
-  c->i8.v_ps = (pb + 255) & ~255LL;
+  c->i8.v_bytes = (pb + 255) & ~255LL;
   c->i8.v_sr = std::max<long long>(rb, 1);
   pb = rb = 0;
   for (int q = 0; q < Ly.ncc; q++) {
@@ -701,10 +701,10 @@ static int i8_setup(imsrg_ctx_s *c) {
         pmb += 64LL * nck;
       }
     kmax = std::max(kmax, ld);
-    pb += pad(r) * ld;
+    pb += pad(r) * ld * kI8Planes;
     rb += pad(r);
   }
-  c->i8.c_ps = (pb + 255) & ~255LL;
+  c->i8.c_bytes = (pb + 255) & ~255LL;
   c->i8.c_sr = std::max<long long>(rb, 1);
   c->i8.n_items = (int)items.size();
   c->i8.pm_size = std::max<long long>(pmb, 1);
@@ -722,7 +722,7 @@ static int i8_setup(imsrg_ctx_s *c) {
   if (!rc) rc = upload_table(c, ph, &c->ph_i8);
   for (int k = 0; k < 2 && !rc; k++) {
     void *p = nullptr;
-    const size_t vb = kI8Planes * (size_t)c->i8.v_ps * B, cb = kI8Planes * (size_t)c->i8.c_ps * B;
+    const size_t vb = (size_t)c->i8.v_bytes * B, cb = (size_t)c->i8.c_bytes * B;
     if (hipMalloc(&p, std::max<size_t>(vb, 256)) != hipSuccess) return fail(IMSRG_ENOMEM, "hipMalloc int8 planes failed");
     c->dallocs.push_back(p);
     c->i8v[k] = (signed char *)p;
@@ -1077,9 +1077,9 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
     // B rows are rows of the other operand: X[k][n] = herm * X[n][k]
     PROFK(c, IMSRG_PROF_GEMM_PPHH,
           launch_seg_gemm_i8(s2, 0, c->pp_i8, c->pp_tiles, c->n_pp_tiles, c->i8v[0], c->i8v[1], c->i8vs[0],
-                             c->i8vs[1], c->i8.v_ps, (double)hR, (double)hL,
-                             (mask & T222PPHH) ? RES->d + D.v_off : nullptr, (mask & T221) ? c->wpp : nullptr, 0.0,
-                             ad, na, kI8Planes * c->i8.v_ps, c->i8.v_sr, OS, D.vsize));
+                             c->i8vs[1], (double)hR, (double)hL, (mask & T222PPHH) ? RES->d + D.v_off : nullptr,
+                             (mask & T221) ? c->wpp : nullptr, 0.0, ad, na, c->i8.v_bytes, c->i8.v_sr, OS,
+                             D.vsize));
   } else if (mask & (T222PPHH | T221)) {
     const bool w = c->prec == IMSRG_PREC_FP64 && c->shape_pp == 1;
     PROFK(c, IMSRG_PROF_GEMM_PPHH,
@@ -1100,8 +1100,8 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
       });
       PROFK(c, IMSRG_PROF_GEMM_PH,
             launch_seg_gemm_i8(s, 1, c->ph_i8, c->ph_tiles, c->n_ph_tiles, c->i8c[0], c->i8c[1], c->i8cs[0],
-                              c->i8cs[1], c->i8.c_ps, (double)hL, -(double)hR, c->wph, nullptr,
-                              -(double)hL * (double)hR, ad, na, kI8Planes * c->i8.c_ps, c->i8.c_sr, D.cwsize, 0));
+                              c->i8cs[1], (double)hL, -(double)hR, c->wph, nullptr, -(double)hL * (double)hR, ad,
+                              na, c->i8.c_bytes, c->i8.c_sr, D.cwsize, 0));
     } else {
       const bool w = c->prec == IMSRG_PREC_FP64 && c->shape_ph == 1;
       PROFK(c, IMSRG_PROF_GEMM_PH,

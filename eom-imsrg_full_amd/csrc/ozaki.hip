@@ -61,7 +61,7 @@ __global__ __launch_bounds__(256) void k_slice_v(DevLayout L, I8Layout I, const 
     const long long inst = inst_id(act);
     if (inst < 0) return;
     op += inst * L.op_size;
-    planes += inst * NS * I.v_ps;
+    planes += inst * I.v_bytes;
     scale += inst * I.v_sr;
   }
   const int q = L.pair_q[r], ld = I.v_ld[q];
@@ -89,7 +89,7 @@ __global__ __launch_bounds__(256) void k_slice_v(DevLayout L, I8Layout I, const 
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
   const int ex = row_exp(m);
   if (lane == 0) scale[I.v_soff[q] + rl] = m > 0.0 ? ldexp(1.0, ex) : 0.0;
-  signed char *row = planes + I.v_poff[q] + (long long)rl * ld;
+  signed char *row = planes + I.v_poff[q] + (long long)rl * NS * ld;
   auto put = [&](int j4, const double x[4]) {
     unsigned w[NS] = {};
 #pragma unroll
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void k_slice_v(DevLayout L, I8Layout I, const 
       for (int i = 0; i < NS; i++) w[i] |= b[i] << (8 * u);
     }
 #pragma unroll
-    for (int i = 0; i < NS; i++) *(unsigned *)(row + i * I.v_ps + j4) = w[i];
+    for (int i = 0; i < NS; i++) *(unsigned *)(row + (j4 >> 6) * (NS * 64) + i * 64 + (j4 & 63)) = w[i];
   };
 #pragma unroll
   for (int t = 0; t < TC; t++) {
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(256) void k_slice_cc(DevLayout L, I8Layout I, const
     if (inst < 0) return;
     cc += inst * L.ccsize;
     pmax += inst * I.pm_size;
-    planes += inst * NS * I.c_ps;
+    planes += inst * I.c_bytes;
     scale += inst * I.c_sr;
   }
   const int q = it.x, r0 = it.y, k0 = it.z, tid = threadIdx.x, rloc = tid & 63, kq = tid >> 6;
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void k_slice_cc(DevLayout L, I8Layout I, const
   const int ex = row_exp(m);
   if (k0 == 0 && kq == 0) scale[I.c_soff[q] + row] = m > 0.0 ? ldexp(1.0, ex) : 0.0;
   const double *src = cc + L.ccoff[q];
-  signed char *dst = planes + I.c_poff[q] + (long long)row * ld;
+  signed char *dst = planes + I.c_poff[q] + (long long)row * NS * ld;
   for (int kc = k0; kc < min(k0 + 256, ld); kc += 64) {
     const int kb = kc + kq * 16;
     unsigned w[NS][4];
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(256) void k_slice_cc(DevLayout L, I8Layout I, const
 #pragma unroll
     for (int i = 0; i < NS; i++) {
       const v4i_t v = {(int)w[i][0], (int)w[i][1], (int)w[i][2], (int)w[i][3]};
-      *(v4i_t *)(dst + i * I.c_ps + kb) = v;
+      *(v4i_t *)(dst + (kb >> 6) * (NS * 64) + i * 64 + (kb & 63)) = v;
     }
   }
 }
@@ -204,8 +204,8 @@ void launch_slice_cc(hipStream_t s, const DevLayout &L, const I8Layout &I, const
 }
 
 // ---- the product.  A 64 x 64 output tile per workgroup; each K chunk of 64
-// bytes of the NS planes of A (64 rows) and B (64 rows) is loaded into
-// registers one chunk ahead and staged in LDS (16-byte column group g of row
+// columns of the NS slices of A (64 rows) and B (64 rows, 320 contiguous bytes
+// per row) is loaded into registers one chunk ahead and staged in LDS (16-byte column group g of row
 // r at g ^ ((r >> 2) & 3), so the 16 rows a quarter-wavefront reads hit
 // distinct banks).  Wavefront (wm, wn) owns the 32 x 32 quadrant (rows 32 wm,
 // columns 32 wn) as one v_mfma_i32_32x32x32_i8 tile per level (the 16x16x64
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(256, 2) void k_seg_gemm_i8(const I8Prob *__restrict
                                                         const signed char *__restrict__ P0,
                                                         const signed char *__restrict__ P1,
                                                         const double *__restrict__ S0, const double *__restrict__ S1,
-                                                        long long ps, double g0, double g1, double *__restrict__ C,
+                                                        double g0, double g1, double *__restrict__ C,
                                                         double *__restrict__ W, double sigma, const int *act,
                                                         long long sp, long long ss, long long sc, long long sw) {
   const int t = blockIdx.x;
@@ -250,9 +250,17 @@ __global__ __launch_bounds__(256, 2) void k_seg_gemm_i8(const I8Prob *__restrict
   const int ra = m0 + wm * 32, cb = n0 + wn * 32;
   // quadrants beyond the block or strictly below the diagonal only help load
   const bool active = ra < M && cb < M && ra < cb + 32;
-  const long long ld = p.ld;
-  // staging: thread tid loads 16 bytes (group tid % 4) of tile row tid / 4 per plane
-  const int sr = tid >> 2, sg = tid & 3, sgp = swz(sr, sg);
+  const long long rs = (long long)NS * p.ld;  // row stride
+  // staging: the chunk of 64 rows x 320 bytes of one operand is 1280 pieces of
+  // 16 bytes; thread tid takes pieces tid + 256 q (consecutive threads read
+  // consecutive bytes), piece w of a row being slice w / 4, column group w % 4
+  int soff[NS], sdst[NS];
+#pragma unroll
+  for (int q = 0; q < NS; q++) {
+    const int pc = tid + 256 * q, r = pc / (4 * NS), w = pc - r * (4 * NS);
+    soff[q] = r * (int)rs + w * 16;
+    sdst[q] = ((w >> 2) * 64 + r) * 4 + swz(r, w & 3);  // index into a [NS][64][4] array of 16-byte groups
+  }
   // fragment reads: row lane % 32 of the quadrant, column group 2 h + lane / 32 of K step h
   const int fr = lane & 31, fh = lane >> 5;
   const int fra = wm * 32 + fr, frb = wn * 32 + fr;
@@ -266,14 +274,15 @@ __global__ __launch_bounds__(256, 2) void k_seg_gemm_i8(const I8Prob *__restrict
       const int K = sgm.K;
       if (K == 0) continue;
       const bool o = sgm.src != 0;
-      const signed char *A = (o ? P1 : P0) + p.p_off + sgm.kofs + (long long)(m0 + sr) * ld + sg * 16;
-      const signed char *B = (o ? P0 : P1) + p.p_off + sgm.kofs + (long long)(n0 + sr) * ld + sg * 16;
+      const signed char *A = (o ? P1 : P0) + p.p_off + (long long)m0 * rs + (sgm.kofs >> 6) * (NS * 64);
+      const signed char *B = (o ? P0 : P1) + p.p_off + (long long)n0 * rs + (sgm.kofs >> 6) * (NS * 64);
       v4i_t R[2][NS];  // the next chunk, in flight while this one is used
       auto load = [&](int kc) {
+        const int co = (kc >> 6) * (NS * 64);
 #pragma unroll
-        for (int i = 0; i < NS; i++) {
-          R[0][i] = *(const v4i_t *)(A + i * ps + kc);
-          R[1][i] = *(const v4i_t *)(B + i * ps + kc);
+        for (int q = 0; q < NS; q++) {
+          R[0][q] = *(const v4i_t *)(A + co + soff[q]);
+          R[1][q] = *(const v4i_t *)(B + co + soff[q]);
         }
       };
       v16i_t lv[NS];
@@ -285,9 +294,9 @@ __global__ __launch_bounds__(256, 2) void k_seg_gemm_i8(const I8Prob *__restrict
       for (int kc = 0; kc < K; kc += 64) {  // K is workgroup-uniform: every barrier is reached by all
         __syncthreads();  // the previous chunk has been read
 #pragma unroll
-        for (int i = 0; i < NS; i++) {
-          S.a[i][sr][sgp] = R[0][i];
-          S.b[i][sr][sgp] = R[1][i];
+        for (int q = 0; q < NS; q++) {
+          (&S.a[0][0][0])[sdst[q]] = R[0][q];
+          (&S.b[0][0][0])[sdst[q]] = R[1][q];
         }
         __syncthreads();
         if (kc + 64 < K) load(kc + 64);
@@ -357,15 +366,15 @@ __global__ __launch_bounds__(256, 2) void k_seg_gemm_i8(const I8Prob *__restrict
 
 void launch_seg_gemm_i8(hipStream_t s, int epi, const I8Prob *probs, const int4 *tiles, int ntiles,
                         const signed char *P0, const signed char *P1, const double *S0, const double *S1,
-                        long long ps, double g0, double g1, double *C, double *W, double sigma, const int *act,
-                        int nact, long long sp, long long ss, long long sc, long long sw) {
+                        double g0, double g1, double *C, double *W, double sigma, const int *act, int nact,
+                        long long sp, long long ss, long long sc, long long sw) {
   if (ntiles <= 0 || nact <= 0) return;
   const dim3 g(ntiles, nact), b(256);
   if (epi == 0)
-    launch_k(k_seg_gemm_i8<0>, g, b, 0, s, probs, tiles, ntiles, P0, P1, S0, S1, ps, g0, g1, C, W, sigma,
+    launch_k(k_seg_gemm_i8<0>, g, b, 0, s, probs, tiles, ntiles, P0, P1, S0, S1, g0, g1, C, W, sigma,
                        act, sp, ss, sc, sw);
   else
-    launch_k(k_seg_gemm_i8<1>, g, b, 0, s, probs, tiles, ntiles, P0, P1, S0, S1, ps, g0, g1, C, W, sigma,
+    launch_k(k_seg_gemm_i8<1>, g, b, 0, s, probs, tiles, ntiles, P0, P1, S0, S1, g0, g1, C, W, sigma,
                        act, sp, ss, sc, sw);
 }
 
