@@ -91,7 +91,7 @@ struct imsrg_ctx_s {
   static constexpr int kSlots = 4;
   int *act_d[kSlots] = {nullptr, nullptr, nullptr, nullptr};
   int *act_h[kSlots] = {nullptr, nullptr, nullptr, nullptr};
-  imsrg_op_s *tmp[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  imsrg_op_s *tmp[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // per-kernel-class HIP event timing on this context's stream
   bool prof_on = false;
   unsigned prof_mask = ~0u;  // kernel classes bracketed while profiling
@@ -1328,8 +1328,9 @@ static int bch_impl(imsrg_ctx_s *c, imsrg_op_s *HS, imsrg_op_s *G, imsrg_op_s *H
   if (a.n() == 0) return 0;
   op_copy(H, HS, a);
   op_copy(H, AD, a);  // AD_1 = H
-  AD->herm = 1;
-  INT2->herm = 1;
+  // [G, X] with G anti-hermitian keeps the hermiticity of X
+  AD->herm = H->herm;
+  INT2->herm = H->herm;
   const ActSet cur = series_set(c, 1, a);
   prep_L(c, G, cur);
   std::vector<int> live(a.n(), 1);  // positions in the series for nested commutator iw
@@ -1359,6 +1360,38 @@ static int bch_impl(imsrg_ctx_s *c, imsrg_op_s *HS, imsrg_op_s *G, imsrg_op_s *H
   if (nlive > 0) return fail(IMSRG_ECLIP, "clipping BCH");
   return 0;
 }
+// transform_sq_op_BCH (IMSRG_MAGNUS.f90:131-146) for a rank-0 operator:
+// BCH_EXPAND into a scratch operator, copied back over Op.
+int imsrg_transform_bch(imsrg_op Op, imsrg_op G, int *nterms) {
+  if (!Op || !G || Op == G || Op->ctx != G->ctx) return fail(IMSRG_EINVAL, "bad operands");
+  if (G->herm != -1) return fail(IMSRG_EINVAL, "Omega must be anti-hermitian");
+  imsrg_ctx_s *c = Op->ctx;
+  imsrg_op_s *T = tmp_op(c, 6, Op->herm);
+  if (!T) return IMSRG_ENOMEM;
+  const ActSet all = all_set(c);
+  std::vector<int> nt;
+  int rc = bch_impl(c, T, G, Op, all, nt);
+  if (rc) return rc;
+  op_copy(T, Op, all);
+  if (nterms) std::copy(nt.begin(), nt.end(), nterms);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// dHds (IMSRG_ODE.f90:416-463): DH = [eta, HS] with eta the generator of
+// `kind` built from HS; every term, E0 from 110 + 220.
+int imsrg_dhds(imsrg_op HS, int kind, imsrg_op DH) {
+  if (!HS || !DH || HS == DH || HS->ctx != DH->ctx) return fail(IMSRG_EINVAL, "bad operands");
+  if (DH->herm != HS->herm) return fail(IMSRG_EINVAL, "DH herm must equal herm(HS)");
+  imsrg_ctx_s *c = HS->ctx;
+  imsrg_op_s *ETA = tmp_op(c, 6, -1);
+  if (!ETA) return IMSRG_ENOMEM;
+  const ActSet all = all_set(c);
+  int rc = gen_impl(c, kind, HS, ETA, all);
+  if (!rc) rc = comm_impl(c, ETA, HS, DH, IMSRG_TALL, false, all);
+  return rc;
+}
+
 int imsrg_bch_expand(imsrg_ctx c, imsrg_op HS, imsrg_op G, imsrg_op H, int *nterms) {
   if (!c || !HS || !G || !H || HS == G || HS == H) return fail(IMSRG_EINVAL, "bad operands");
   std::vector<int> nt;

@@ -29,6 +29,7 @@ module imsrg_gpu
        gpu_commutator_121, gpu_commutator_122, gpu_commutator_222_pp_hh, gpu_commutator_221, &
        gpu_commutator_222_ph, gpu_calculate_cross_coupled
   public :: gpu_BCH_EXPAND, gpu_MAGNUS_EXPAND, gpu_build_generator, gpu_magnus_decouple
+  public :: gpu_transform_sq_op_BCH, gpu_dHds
 
   type(c_ptr), save :: ctx = c_null_ptr
   integer(c_int64_t), save :: neq = 0
@@ -110,6 +111,16 @@ module imsrg_gpu
        type(c_ptr), value :: c, HS, G, H
        integer(c_int) :: nterms
      end function c_bch
+     integer(c_int) function c_transform_bch(Op, G, nterms) bind(C, name='imsrg_transform_bch')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: Op, G
+       integer(c_int) :: nterms
+     end function c_transform_bch
+     integer(c_int) function c_dhds(HS, kind, DH) bind(C, name='imsrg_dhds')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: HS, DH
+       integer(c_int), value :: kind
+     end function c_dhds
      integer(c_int) function c_magexp(c, DG, G, nterms) bind(C, name='imsrg_magnus_expand')
        import :: c_int, c_ptr
        type(c_ptr), value :: c, DG, G
@@ -372,6 +383,37 @@ contains
     call pull(hs_, HS)
     call drop(hg) ; call drop(hh) ; call drop(hs_)
   end subroutine gpu_BCH_EXPAND
+
+  ! transform_sq_op_BCH (IMSRG_MAGNUS.f90:131-146) for rank-0 operators:
+  ! Op <- e^G Op e^-G
+  subroutine gpu_transform_sq_op_BCH(Op, G, jbas, quads)
+    type(sq_op) :: Op, G
+    type(spd) :: jbas
+    character(1) :: quads
+    type(c_ptr) :: ho, hg
+    integer(c_int) :: nterms
+    if (Op%rank > 0) error stop 'imsrg_gpu: tensor operators (BCH_TENSOR) are not provided'
+    if (quads == 'y') error stop 'imsrg_gpu: quadrupole restoration is not provided'
+    ho = push(Op) ; hg = push(G)
+    call check(c_transform_bch(ho, hg, nterms), 'imsrg_transform_bch')
+    call pull(ho, Op)
+    call drop(ho) ; call drop(hg)
+  end subroutine gpu_transform_sq_op_BCH
+
+  ! dHds (IMSRG_ODE.f90:416-463), the derivative the ODE solver evaluates: like
+  ! the reference it builds the generator from HS (the device generator `kind`
+  ! in place of the external build_gen) and returns vectorize([eta, HS]) in yp
+  subroutine gpu_dHds(t, yy, yp, HS, jbas, kind)
+    real(8) :: t, yy(*), yp(*)
+    type(sq_op) :: HS
+    type(spd) :: jbas
+    integer, intent(in) :: kind
+    type(c_ptr) :: hh, hd
+    hh = push(HS) ; hd = fresh(HS%herm)
+    call check(c_dhds(hh, int(kind, c_int), hd), 'imsrg_dhds')
+    call check(c_op_download(hd, yp, neq), 'imsrg_op_download')
+    call drop(hh) ; call drop(hd)
+  end subroutine gpu_dHds
 
   ! MAGNUS_EXPAND (IMSRG_MAGNUS.f90:622-693): DG <- dOmega/ds
   subroutine gpu_MAGNUS_EXPAND(DG, G, AD, jbas)

@@ -94,6 +94,8 @@ _SIG = {
     "imsrg_ctx_profile": (C.c_int, [_vp, C.c_int]),
     "imsrg_ctx_profile_classes": (C.c_int, [_vp, C.c_uint]),
     "imsrg_ctx_set_concurrency": (C.c_int, [_vp, C.c_int]),
+    "imsrg_transform_bch": (C.c_int, [_vp, _vp, C.POINTER(C.c_int)]),
+    "imsrg_dhds": (C.c_int, [_vp, C.c_int, _vp]),
     "imsrg_ctx_profile_read": (C.c_int, [_vp, _dp, C.POINTER(C.c_longlong), C.c_int, C.c_int]),
 }
 PROF_CLASSES = ["gemm_pphh", "gemm_ph", "pandya", "k122", "ph_apply", "ph_terms", "sym_mirror",
@@ -244,6 +246,18 @@ class Context:
         n = np.zeros(self.nbatch, np.int32)
         _chk(_lib.imsrg_bch_expand(self.h, HS.h, G.h, H.h, _iptr(n)))
         return HS, self._per_inst(n)
+
+    def transform_bch(self, Op, G):
+        """Op <- e^G Op e^-G in place (transform_sq_op_BCH); the nested-commutator count"""
+        n = np.zeros(self.nbatch, np.int32)
+        _chk(_lib.imsrg_transform_bch(Op.h, G.h, _iptr(n)))
+        return self._per_inst(n)
+
+    def dhds(self, HS, kind="white", DH=None):
+        """DH = [eta(HS), HS] with the generator `kind` built from HS (dHds)"""
+        DH = DH or Operator(self, HS.herm)
+        _chk(_lib.imsrg_dhds(HS.h, GEN[kind], DH.h))
+        return DH
 
     def magnus_expand(self, DG, G):
         n = np.zeros(self.nbatch, np.int32)

@@ -184,6 +184,13 @@ int imsrg_generator_atanxx(imsrg_ctx ctx, imsrg_op Hold, imsrg_op H, imsrg_op ET
 int imsrg_bch_expand(imsrg_ctx ctx, imsrg_op HS, imsrg_op G, imsrg_op H, int *nterms);
 /* DG <- dOmega/ds series (MAGNUS_EXPAND :622-693); nterms as above */
 int imsrg_magnus_expand(imsrg_ctx ctx, imsrg_op DG, imsrg_op G, int *nterms);
+/* Op <- e^G Op e^-G for a rank-0 observable (transform_sq_op_BCH :131-146, the
+ * BCH_EXPAND branch); G anti-hermitian, Op of either hermiticity; nterms as
+ * above */
+int imsrg_transform_bch(imsrg_op Op, imsrg_op G, int *nterms);
+/* DH = [eta(HS), HS], eta the generator `kind` of HS (dHds, IMSRG_ODE.f90:416-463:
+ * the derivative the reference's ODE solver evaluates); DH herm = herm(HS) */
+int imsrg_dhds(imsrg_op HS, int kind, imsrg_op DH);
 /* full flow (magnus_decouple :15-128, no checkpointing; nbatch = 1): HS in = normal-ordered
  * H(0), out = decoupled H(s); G out = Omega(s).  Per-step records (arrays of
  * length max_steps, any may be NULL): s, E0, E0+MBPT2, |MBPT2|, #Magnus and

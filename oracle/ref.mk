@@ -16,7 +16,7 @@ LIBS := -L$(MKLDIR) -Wl,-rpath,$(MKLDIR) -lmkl_rt -lz
 MODS := gzipmod basic_IMSRG cross_coupled commutators isospin_operators \
         three_body_routines HF_mod TS_commutators tensor_products gaulag operators \
         operator_commutators IMSRG_MAGNUS EOM_scalar_commutators EOM_TS_commutators \
-        EOM_dTZ_commutators brute_force_testing
+        EOM_dTZ_commutators brute_force_testing adams_ode IMSRG_ODE
 OBJS := $(OUT)/anglib.o $(addprefix $(OUT)/,$(addsuffix .o,$(MODS))) $(OUT)/generators.o
 
 LIBDIR := $(abspath $(HERE)../eom-imsrg_full_amd/imsrg_mi355x/lib)
@@ -63,6 +63,10 @@ $(OUT)/EOM_TS_commutators.o: $(REFDIR)/EOM_TS_commutators.f90 $(OUT)/cross_coupl
 $(OUT)/EOM_dTZ_commutators.o: $(REFDIR)/EOM_dTZ_commutators.f90 $(OUT)/cross_coupled.o $(OUT)/isospin_operators.o
 	$(FC) $(FFLAGS) -c -o $@ $< -J$(OUT)
 $(OUT)/brute_force_testing.o: $(REFDIR)/brute_force_testing.f90 $(OUT)/EOM_scalar_commutators.o $(OUT)/EOM_TS_commutators.o $(OUT)/EOM_dTZ_commutators.o $(OUT)/TS_commutators.o $(OUT)/operator_commutators.o $(OUT)/operators.o $(OUT)/commutators.o
+	$(FC) $(FFLAGS) -c -o $@ $< -J$(OUT)
+$(OUT)/adams_ode.o: $(REFDIR)/adams_ode.f90 $(OUT)/basic_IMSRG.o
+	$(FC) $(FFLAGS) -c -o $@ $< -J$(OUT)
+$(OUT)/IMSRG_ODE.o: $(REFDIR)/IMSRG_ODE.f90 $(OUT)/adams_ode.o $(OUT)/operators.o $(OUT)/commutators.o
 	$(FC) $(FFLAGS) -c -o $@ $< -J$(OUT)
 $(OUT)/generators.o: $(REFDIR)/generators.f90 $(OUT)/commutators.o
 	$(FC) $(FFLAGS) -c -o $@ $< -J$(OUT)

@@ -18,7 +18,8 @@ program drop_in_test
   character(len=1024) :: fb, fl, fr, fh
   integer :: nfail, q
   real(8) :: e1, e2
-  external :: build_gs_white, build_gs_w2
+  real(8), allocatable :: yy(:), yp1(:), yp2(:)
+  external :: build_gs_white, build_gs_w2, dHds
 
   call getarg(1, fb) ; call getarg(2, fl) ; call getarg(3, fr) ; call getarg(4, fh)
   checkpointing = .false.
@@ -84,6 +85,18 @@ program drop_in_test
   call copy_sq_op(ETA1, DG1) ; call copy_sq_op(ETA1, DG2)
   call MAGNUS_EXPAND(DG1, G, AD, jbas) ; call gpu_MAGNUS_EXPAND(DG2, G, AD, jbas)
   call cmp('MAGNUS_EXPAND', DG1, DG2)
+
+  ! observable transform (the rank-0 branch of transform_sq_op_BCH) of R
+  call copy_sq_op(R, A) ; call copy_sq_op(R, B)
+  call transform_sq_op_BCH(A, G, jbas, 'n') ; call gpu_transform_sq_op_BCH(B, G, jbas, 'n')
+  call cmp('transform_sq_op_BCH', A, B)
+
+  ! the ODE solver's derivative dHds with the White generator
+  allocate(yy(HS%neq), yp1(HS%neq), yp2(HS%neq))
+  call vectorize(H, yy)
+  call dHds(0.d0, yy, yp1, H, jbas, build_gs_white)
+  call gpu_dHds(0.d0, yy, yp2, H, jbas, GEN_WHITE)
+  call report('dHds', maxval(abs(yp1 - yp2)) / max(maxval(abs(yp1)), 1d-300))
 
   ! the whole flow: magnus_decouple vs gpu_magnus_decouple
   call copy_sq_op(H, A) ; call copy_sq_op(H, B)

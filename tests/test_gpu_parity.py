@@ -104,6 +104,25 @@ def test_bch_and_magnus_expand(env):
     assert_close(got_dg.download(), want_dg.vec(), what="MAGNUS_EXPAND")
 
 
+def test_transform_bch_and_dhds(env):
+    """transform_sq_op_BCH (rank 0) of a random hermitian observable, in place,
+    against the oracle's BCH_EXPAND of it; dHds = [eta(H), H] against the
+    oracle's generator and commutator"""
+    B, ctx, ob, E = env
+    oH = O.normal_order(ob.op(synthetic_hamiltonian(B, seed=0), 1))
+    Hn = oH.vec()
+    eta = O.generator("white", oH)
+    G = 0.5 * eta.vec()
+    obs = random_operator(B, 1, 17)
+    want, want_n = O.bch(ob.op(G, -1), ob.op(obs, 1))
+    Op = ctx.op(1, obs)
+    n = ctx.transform_bch(Op, ctx.op(-1, G))
+    assert n == want_n
+    assert_close(Op.download(), want.vec(), what="transform_sq_op_BCH")
+    got = ctx.dhds(ctx.op(1, Hn), "white").download()
+    assert_close(got, O.commutator(eta, oH).vec(), what="dHds")
+
+
 def test_magnus_flow(env):
     B, ctx, ob, E = env
     oH = O.normal_order(ob.op(synthetic_hamiltonian(B, seed=0), 1))
