@@ -85,7 +85,8 @@ struct SegProb {
 // i + j <= 6 (fifteen int8 products, exact int32 sums per level i + j).
 constexpr int kI8Planes = 5;
 struct I8Seg {
-  int kofs, K, src, pad;  // K window of the segment in the block's plane rows (multiples of 64)
+  int kofs, K, src;  // K window of the segment in the block's rows (multiples of 64)
+  int K32;           // its columns rounded up to 32: a last half chunk skips its second K step
   double sign;
 };
 // C (M x M) = sum of nseg segments  sign * A_src * B_other^T  over plane rows

@@ -643,6 +643,7 @@ static int i8_setup(imsrg_ctx_s *c) {
   const Layout &Ly = c->lay;
   const size_t B = c->nbatch;
   auto pad = [](long long x) { return (x + 63) & ~63LL; };
+  auto pad32 = [](long long x) { return (x + 31) & ~31LL; };
   std::vector<long long> vpoff(Ly.nblocks, 0), vsoff(Ly.nblocks, 0), cpoff(Ly.ncc, 0), csoff(Ly.ncc, 0);
   std::vector<int> vld(Ly.nblocks, 0), cld(Ly.ncc, 0);
   std::vector<int4> items;
@@ -664,10 +665,10 @@ static int i8_setup(imsrg_ctx_s *c) {
     P.s_off = rb;
     P.nmid = 2;
     P.nseg = nh > 0 ? 4 : 2;
-    P.seg[0] = I8Seg{0, (int)pad(np), 0, 0, 1.0};
-    P.seg[1] = I8Seg{0, (int)pad(np), 1, 0, -1.0};
-    P.seg[2] = I8Seg{(int)pad(np), (int)pad(nh), 0, 0, -1.0};
-    P.seg[3] = I8Seg{(int)pad(np), (int)pad(nh), 1, 0, 1.0};
+    P.seg[0] = I8Seg{0, (int)pad(np), 0, (int)pad32(np), 1.0};
+    P.seg[1] = I8Seg{0, (int)pad(np), 1, (int)pad32(np), -1.0};
+    P.seg[2] = I8Seg{(int)pad(np), (int)pad(nh), 0, (int)pad32(nh), -1.0};
+    P.seg[3] = I8Seg{(int)pad(np), (int)pad(nh), 1, (int)pad32(nh), 1.0};
     pp.push_back(P);
     kmax = std::max(kmax, ld);
     pb += pad(dim) * ld * kI8Planes;
@@ -691,8 +692,8 @@ static int i8_setup(imsrg_ctx_s *c) {
     P.p_off = pb;
     P.s_off = rb;
     P.nmid = P.nseg = 2;
-    P.seg[0] = I8Seg{0, ld, 0, 0, 1.0};
-    P.seg[1] = I8Seg{0, ld, 1, 0, 1.0};
+    P.seg[0] = I8Seg{0, ld, 0, (int)pad32(nbq), 1.0};
+    P.seg[1] = I8Seg{0, ld, 1, (int)pad32(nbq), 1.0};
     ph.push_back(P);
     if (nbq > 0)
       for (int r0 = 0; r0 < r; r0 += 64) {

@@ -301,8 +301,10 @@ __global__ __launch_bounds__(256, 2) void k_seg_gemm_i8(const I8Prob *__restrict
         __syncthreads();
         if (kc + 64 < K) load(kc + 64);
         if (active) {
+          const int nh = kc + 32 < sgm.K32 ? 2 : 1;  // workgroup-uniform
 #pragma unroll
           for (int h = 0; h < 2; h++) {
+            if (h >= nh) break;
             // all B fragments of this K step, then A plane by plane: pair (i, j) adds to level i + j
             v4i_t b[NS];
 #pragma unroll
