@@ -1465,25 +1465,38 @@ struct imsrg_flow_s {
   std::vector<int> steps;
 };
 
-int imsrg_flow_create(imsrg_ctx c, imsrg_op HS, imsrg_op G, int gen_kind, double ds, imsrg_flow *out) {
+// Set-up of magnus_decouple (IMSRG_MAGNUS.f90:31-97).  Fresh start: Omega = 0,
+// DG = generator(H(0)), crit = |MBPT2|.  Checkpoint restart (:83-97): Omega(s0)
+// given in G, HS = BCH(Omega, H(0)), DG = generator(HS), s = s0, steps =
+// nint(s0/ds) and crit = 10 (the reference's initial value, so a step follows).
+static int flow_setup(imsrg_ctx c, imsrg_op HS, imsrg_op G, int gen_kind, double ds, bool restart, double s0,
+                      imsrg_flow *out) {
   if (!c || !HS || !G || HS == G || !out) return fail(IMSRG_EINVAL, "bad operands");
   if (HS->ctx != c || G->ctx != c) return fail(IMSRG_EINVAL, "operands of another context");
+  if (restart && G->herm != -1) return fail(IMSRG_EINVAL, "Omega must be anti-hermitian");
   imsrg_flow_s *f = new imsrg_flow_s;
   f->ctx = c; f->HS = HS; f->G = G; f->gen_kind = gen_kind; f->ds = ds;
   const int B = c->nbatch;
-  f->s.assign(B, 0.0); f->crit.assign(B, 0.0); f->emp2.assign(B, 0.0); f->steps.assign(B, 0);
+  f->s.assign(B, restart ? s0 : 0.0); f->crit.assign(B, 0.0); f->emp2.assign(B, 0.0);
+  f->steps.assign(B, restart ? (int)std::lround(s0 / ds) : 0);
   const ActSet all = all_set(c);
   int rc = op_new(c, 1, &f->H0);
   if (!rc) rc = op_new(c, -1, &f->DG);
-  if (!rc) {
+  if (!rc && restart) {
+    op_copy(HS, f->H0, all);
+    std::vector<int> nt;
+    rc = bch_impl(c, HS, G, f->H0, all, nt);
+    if (!rc) rc = gen_impl(c, gen_kind, HS, f->DG, all);
+    for (int k = 0; k < B; k++) f->crit[k] = 10.0;
+  } else if (!rc) {
     launch_fill(c->stream, G->d, c->lay.op_size * B, 0.0);
     G->herm = -1;
     rc = gen_impl(c, gen_kind, HS, f->DG, all);
-  }
-  if (!rc) {
-    op_copy(HS, f->H0, all);
-    rc = mbpt2_set(HS, all, f->emp2);
-    for (int k = 0; k < B; k++) f->crit[k] = std::fabs(f->emp2[k]);
+    if (!rc) {
+      op_copy(HS, f->H0, all);
+      rc = mbpt2_set(HS, all, f->emp2);
+      for (int k = 0; k < B; k++) f->crit[k] = std::fabs(f->emp2[k]);
+    }
   }
   if (rc) {
     imsrg_flow_destroy(f);
@@ -1491,6 +1504,44 @@ int imsrg_flow_create(imsrg_ctx c, imsrg_op HS, imsrg_op G, int gen_kind, double
   }
   *out = f;
   return 0;
+}
+int imsrg_flow_create(imsrg_ctx c, imsrg_op HS, imsrg_op G, int gen_kind, double ds, imsrg_flow *out) {
+  return flow_setup(c, HS, G, gen_kind, ds, false, 0.0, out);
+}
+int imsrg_flow_restart(imsrg_ctx c, imsrg_op HS, imsrg_op G, int gen_kind, double ds, double s, imsrg_flow *out) {
+  return flow_setup(c, HS, G, gen_kind, ds, true, s, out);
+}
+
+// write_twobody_operator / write_omega_checkpoint and read_twobody_operator /
+// read_omega_checkpoint (basic_IMSRG.f90:3069-3420) store an operator as its
+// vectorize stream, neq FP64 values in an unformatted stream file, no header.
+int imsrg_op_write_stream(imsrg_op op, const char *path) {
+  if (!op || !path) return fail(IMSRG_EINVAL, "null argument");
+  if (int rc = single_only(op->ctx, "imsrg_op_write_stream")) return rc;
+  const int64_t n = op->ctx->lay.neq;
+  std::vector<double> v(n);
+  int rc = imsrg_op_download(op, v.data(), n);
+  if (rc) return rc;
+  FILE *fp = std::fopen(path, "wb");
+  if (!fp) return fail(IMSRG_EINVAL, std::string("cannot open ") + path + " for writing");
+  const size_t w = std::fwrite(v.data(), sizeof(double), (size_t)n, fp);
+  const int cl = std::fclose(fp);
+  if (w != (size_t)n || cl != 0) return fail(IMSRG_EINVAL, std::string("short write to ") + path);
+  return 0;
+}
+int imsrg_op_read_stream(imsrg_op op, const char *path) {
+  if (!op || !path) return fail(IMSRG_EINVAL, "null argument");
+  if (int rc = single_only(op->ctx, "imsrg_op_read_stream")) return rc;
+  const int64_t n = op->ctx->lay.neq;
+  FILE *fp = std::fopen(path, "rb");
+  if (!fp) return fail(IMSRG_EINVAL, std::string("cannot open ") + path);
+  std::vector<double> v(n);
+  const size_t r = std::fread(v.data(), sizeof(double), (size_t)n, fp);
+  const bool more = std::fgetc(fp) != EOF;
+  std::fclose(fp);
+  if (r != (size_t)n || more)
+    return fail(IMSRG_EINVAL, std::string(path) + ": not a stream of " + std::to_string(n) + " FP64 values");
+  return imsrg_op_upload(op, v.data(), n);
 }
 
 // rec: 8 doubles per instance (s, E0, E0+MBPT2, crit, Magnus terms, BCH

@@ -133,6 +133,13 @@ module imsrg_gpu
        real(c_double), value :: ds
        type(c_ptr) :: out
      end function c_flow_create
+     integer(c_int) function c_flow_restart(c, HS, G, kind, ds, s, out) bind(C, name='imsrg_flow_restart')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: c, HS, G
+       integer(c_int), value :: kind
+       real(c_double), value :: ds, s
+       type(c_ptr) :: out
+     end function c_flow_restart
      integer(c_int) function c_flow_step(f, rec) bind(C, name='imsrg_flow_step')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: f
@@ -447,15 +454,44 @@ contains
     integer, intent(in) :: kind
     type(c_ptr) :: hs_, hg, flow
     real(c_double) :: rec(8)
+    real(8) :: s, sx
     integer(c_int) :: done
+    integer :: steps, chk
+    logical :: restart, first
     if (.not. allocated(G%mat)) call duplicate_sq_op(HS, G)
+    ! checkpoints as in magnus_decouple (IMSRG_MAGNUS.f90:55-66, 83-97, 101-108):
+    ! resume from the last Omega checkpoint the reference's reader finds, and
+    ! write Omega with the reference's writer every chk steps (not on the
+    ! first such step after a restart)
+    chk = 24
+    if (HS%eMax == 14) chk = 4
+    if (HS%eMax > 14) chk = 1
+    restart = .false.
+    if (checkpointing) restart = .not. read_omega_checkpoint(G, sx)
     G%herm = -1
-    hs_ = push(HS) ; hg = fresh(-1)
-    call check(c_flow_create(ctx, hs_, hg, int(kind, c_int), 0.5d0, flow), 'imsrg_flow_create')
+    hs_ = push(HS)
+    if (restart) then
+       hg = push(G)
+       call check(c_flow_restart(ctx, hs_, hg, int(kind, c_int), 0.5d0, sx, flow), 'imsrg_flow_restart')
+       s = sx ; steps = nint(sx / 0.5d0) ; first = .true.
+    else
+       hg = fresh(-1)
+       call check(c_flow_create(ctx, hs_, hg, int(kind, c_int), 0.5d0, flow), 'imsrg_flow_create')
+       s = 0.d0 ; steps = 0 ; first = .false.
+    end if
     call check(c_flow_converged(flow, done), 'imsrg_flow_converged')
     do while (done == 0)
+       if (checkpointing .and. mod(steps + 1, chk) == 0) then
+          if (first) then
+             first = .false.
+          else
+             call pull(hg, G)
+             call write_omega_checkpoint(G, s)
+          end if
+       end if
        call check(c_flow_step(flow, rec), 'imsrg_flow_step')
-       write(*,'(I6,4(e15.7))') nint(rec(7)), rec(1), rec(2), rec(3), rec(4)
+       s = rec(1) ; steps = nint(rec(7))
+       write(*,'(I6,4(e15.7))') steps, rec(1), rec(2), rec(3), rec(4)
        call check(c_flow_converged(flow, done), 'imsrg_flow_converged')
     end do
     call check(c_flow_destroy(flow), 'imsrg_flow_destroy')

@@ -88,6 +88,9 @@ _SIG = {
     "imsrg_magnus_flow": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_double, C.c_int, _dp, _dp, _dp, _dp,
                                     _ip, _ip, _ip]),
     "imsrg_flow_create": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_double, C.POINTER(_vp)]),
+    "imsrg_flow_restart": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_double, C.c_double, C.POINTER(_vp)]),
+    "imsrg_op_write_stream": (C.c_int, [_vp, C.c_char_p]),
+    "imsrg_op_read_stream": (C.c_int, [_vp, C.c_char_p]),
     "imsrg_flow_step": (C.c_int, [_vp, _dp]),
     "imsrg_flow_converged": (C.c_int, [_vp, _ip]),
     "imsrg_flow_destroy": (C.c_int, [_vp]),
@@ -276,8 +279,8 @@ class Context:
         return dict(steps=n, s=s[:n], e0=e0[:n], e0_mbpt2=emp2[:n], crit=crit[:n], n_magnus=nm[:n],
                     n_bch=nb[:n], HS=HS, G=G)
 
-    def flow(self, HS, kind="white", ds=0.5, G=None):
-        return Flow(self, HS, kind, ds, G)
+    def flow(self, HS, kind="white", ds=0.5, G=None, restart_s=None):
+        return Flow(self, HS, kind, ds, G, restart_s)
 
     def profile(self, enable=True, classes=None):
         """time every kernel class with HIP events, or only those named in classes"""
@@ -334,6 +337,14 @@ class Operator:
         """vectorize-order vector, or (nbatch, neq) array for a batched context"""
         v = np.ascontiguousarray(vec, dtype=np.float64)
         _chk(_lib.imsrg_op_upload(self.h, _dptr(v), v.size))
+        return self
+
+    def write_stream(self, path):
+        """the reference's .bin operator file: the vectorize stream, no header"""
+        _chk(_lib.imsrg_op_write_stream(self.h, os.fsencode(path)))
+
+    def read_stream(self, path):
+        _chk(_lib.imsrg_op_read_stream(self.h, os.fsencode(path)))
         return self
 
     def download(self):
@@ -409,12 +420,18 @@ def commutator_220(L, R):
 class Flow:
     """magnus_decouple one step at a time (imsrg_flow_*)."""
 
-    def __init__(self, ctx, HS, kind="white", ds=0.5, G=None):
+    def __init__(self, ctx, HS, kind="white", ds=0.5, G=None, restart_s=None):
+        """HS = normal-ordered H(0).  With restart_s, G holds Omega(restart_s) (a
+        checkpoint) and the flow resumes there (imsrg_flow_restart)."""
         self.ctx = ctx
         self.HS = HS
         self.G = G or Operator(ctx, -1)
         h = _vp()
-        _chk(_lib.imsrg_flow_create(ctx.h, HS.h, self.G.h, GEN[kind], float(ds), C.byref(h)))
+        if restart_s is None:
+            _chk(_lib.imsrg_flow_create(ctx.h, HS.h, self.G.h, GEN[kind], float(ds), C.byref(h)))
+        else:
+            _chk(_lib.imsrg_flow_restart(ctx.h, HS.h, self.G.h, GEN[kind], float(ds), float(restart_s),
+                                         C.byref(h)))
         self.h = h
 
     def step(self):

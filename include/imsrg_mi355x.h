@@ -209,9 +209,25 @@ int imsrg_magnus_flow(imsrg_ctx ctx, imsrg_op HS, imsrg_op G, int gen_kind, doub
  * HS and G stay owned by the caller and must outlive the flow. */
 typedef struct imsrg_flow_s *imsrg_flow;
 int imsrg_flow_create(imsrg_ctx ctx, imsrg_op HS, imsrg_op G, int gen_kind, double ds, imsrg_flow *out);
+/* The checkpoint-restart branch of magnus_decouple (IMSRG_MAGNUS.f90:83-97): HS
+ * in = normal-ordered H(0), G in = Omega(s) (for instance read back with
+ * imsrg_op_read_stream from a write_omega_checkpoint file); HS becomes
+ * BCH(Omega, H(0)), the generator is rebuilt, and the flow goes on from s with
+ * nint(s/ds) steps counted, taking at least one more step. */
+int imsrg_flow_restart(imsrg_ctx ctx, imsrg_op HS, imsrg_op G, int gen_kind, double ds, double s,
+                       imsrg_flow *out);
 int imsrg_flow_step(imsrg_flow flow, double *rec8);
 int imsrg_flow_converged(imsrg_flow flow, int *yes);   /* |MBPT2| <= float32(1e-6) */
 int imsrg_flow_destroy(imsrg_flow flow);
+
+/* ---- operator files ---------------------------------------------------------
+ * The reference's `.bin` operator dumps and Omega checkpoints
+ * (write_twobody_operator / write_omega_checkpoint, basic_IMSRG.f90:3069-3226;
+ * read_twobody_operator / read_omega_checkpoint :3228-3420): the vectorize
+ * stream of an operator, neq FP64 values, no header.  Single-instance contexts;
+ * a file of another length is rejected with IMSRG_EINVAL. */
+int imsrg_op_write_stream(imsrg_op op, const char *path);
+int imsrg_op_read_stream(imsrg_op op, const char *path);
 
 /* ---- measurement (no counterpart in the reference, which only prints
  * omp_get_wtime, main_IMSRG.f90:46,308-309) ---------------------------------

@@ -108,6 +108,22 @@ program drop_in_test
   call cmp('magnus_decouple_H', A, B)
   call cmp('magnus_decouple_Omega', ETA1, ETA2)
 
+  ! checkpoints: a first reference flow writes Omega every 24 steps; a second
+  ! reference flow and the drop-in then both resume from the last checkpoint
+  checkpointing = .true.
+  prefix = 'dropin_hw20'
+  scratch = './'
+  call copy_sq_op(H, A) ; call clear_sq_op(ETA1) ; ETA1%herm = -1
+  call magnus_decouple(A, ETA1, jbas, 'n', 'n', build_gs_white)
+  call copy_sq_op(H, A) ; call clear_sq_op(ETA1) ; ETA1%herm = -1
+  call magnus_decouple(A, ETA1, jbas, 'n', 'n', build_gs_white)
+  call copy_sq_op(H, B) ; call clear_sq_op(ETA2) ; ETA2%herm = -1
+  call gpu_magnus_decouple(B, ETA2, jbas, GEN_WHITE)
+  checkpointing = .false.
+  A%neq = HS%neq ; B%neq = HS%neq
+  call cmp('checkpoint_restart_H', A, B)
+  call cmp('checkpoint_restart_Omega', ETA1, ETA2)
+
   call gpu_finalize()
   if (nfail > 0) then
      write(*,*) 'FAILURES ', nfail

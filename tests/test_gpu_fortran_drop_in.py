@@ -3,7 +3,8 @@ against the reference's own routines on the same sq_op inputs: a program built
 from the reference objects (oracle/_ref/drop_in_test) calls, e.g.,
 commutator_111 and gpu_commutator_111, BCH_EXPAND and gpu_BCH_EXPAND,
 transform_sq_op_BCH and gpu_transform_sq_op_BCH, dHds and gpu_dHds,
-magnus_decouple and gpu_magnus_decouple, and reports relative differences."""
+magnus_decouple and gpu_magnus_decouple (also resumed from the reference's own
+Omega checkpoint files), and reports relative differences."""
 import os
 import subprocess
 
@@ -35,5 +36,5 @@ def test_fortran_drop_in(tmp_path, emax, k):
     names = {c[1] for c in checks}
     assert {"full_commutator", "per_routine_sequence", "cross_coupled", "build_gs_w2", "BCH_EXPAND",
             "MAGNUS_EXPAND", "magnus_decouple_H", "magnus_decouple_Omega", "transform_sq_op_BCH",
-            "dHds"} <= names
+            "dHds", "checkpoint_restart_H", "checkpoint_restart_Omega"} <= names
     assert all(float(c[2]) <= 1e-10 for c in checks)

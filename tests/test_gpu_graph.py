@@ -117,3 +117,43 @@ def test_two_stream_commutators_match_one_stream():
             res.append((HS.download(), np.atleast_1d(nt)))
         print(f"[streams] emax=4 nbatch={nb} BCH terms {res[0][1]}", flush=True)
         assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+
+
+def test_checkpoint_restart_continues_the_flow(tmp_path):
+    """magnus_decouple's checkpoint restart: Omega(s) written as the
+    reference's .bin stream after 3 steps, read back into a fresh context, and
+    the flow resumed there (HS = BCH(Omega, H(0)), generator rebuilt) takes the
+    same later steps bit for bit as the uninterrupted flow"""
+    from imsrg_mi355x import engine as E
+    B = Basis.ho(4, 3)
+    H = synthetic_hamiltonian(B, seed=0)
+    ctx = E.Context(B)
+    HS = ctx.op(1, H).normal_order()
+    H0 = HS.download()
+    f = ctx.flow(HS, "white", 0.5)
+    recs = [f.step() for _ in range(6)]
+    f.close()
+    path = tmp_path / "omega_s1.500.bin"
+    # replay the first 3 steps and checkpoint Omega(1.5)
+    HS2 = ctx.op(1, H0)
+    f2 = ctx.flow(HS2, "white", 0.5)
+    for _ in range(3):
+        f2.step()
+    f2.G.write_stream(path)
+    assert np.array_equal(np.fromfile(path, dtype=np.float64), f2.G.download())
+    f2.close()
+    ctx2 = E.Context(B)
+    G = ctx2.op(-1).read_stream(path)
+    HS3 = ctx2.op(1, H0)
+    f3 = ctx2.flow(HS3, "white", 0.5, G=G, restart_s=1.5)
+    later = [f3.step() for _ in range(3)]
+    f3.close()
+    print(f"[restart] E0 uninterrupted {[r['e0'] for r in recs[3:]]}, resumed {[r['e0'] for r in later]}", flush=True)
+    for a, b in zip(recs[3:], later):
+        assert a["e0"] == b["e0"] and a["s"] == b["s"] and a["step"] == b["step"]
+        assert a["n_magnus"] == b["n_magnus"] and a["n_bch"] == b["n_bch"]
+    with pytest.raises(E.ImsrgError):
+        ctx2.op(1).read_stream(tmp_path / "missing.bin")
+    (tmp_path / "short.bin").write_bytes(b"\0" * 16)
+    with pytest.raises(E.ImsrgError):
+        ctx2.op(1).read_stream(tmp_path / "short.bin")
