@@ -213,6 +213,7 @@ __global__ __launch_bounds__(256) void k_adj_ph_terms(DevLayout L, const double 
   const double v = Tbar[t];
   if (v == 0.0) return;
   const unsigned tc = L.ph_terms[t];
+  if (tc == 0xFFFFFFFFu) return;  // padding term
   const unsigned code = L.ph_quads[tc >> 6];
   const int side = (tc >> 5) & 1, m = tc & 31;
   const int i = code >> 24, j = (code >> 16) & 255, k = (code >> 8) & 255, l = code & 255;

@@ -50,8 +50,19 @@ struct DevLayout {
   const int4 *mirror_tiles;  // (block, tile row, tile col) with row <= col, 32x32 tiles
   int nmirror;
   long long f_off, v_off, op_size, vsize, ccsize, cwsize;  // per-instance strides
-  const int2 *apply_chunks;   // k_ph_apply work items: (row, first output) per 64 outputs, rows in (i, j, J) order
-  XcdSplit xs_rows, xs_terms, xs_apply;  // k_122 rows, k_ph_terms chunks of 256 terms, apply_chunks
+  const struct PhChunk *ph_chunks;  // k_ph_apply work items, rows in (i, j, J) order
+  XcdSplit xs_rows, xs_terms, xs_apply;  // k_122 rows, k_ph_terms chunks of 256 terms, ph_chunks
+};
+
+// One k_ph_apply work item: up to 64 consecutive upper outputs of one row
+// (ij; J), with what the kernel would otherwise look up through four levels
+// of dependent table reads.
+struct PhChunk {
+  long long vrow;   // operator offset of the first output
+  long long dbase;  // ph_desc index of the first output
+  int n;            // outputs, 1..64
+  int b0, len;      // the row key's packed 6j list (ap6_val) start and length
+  int half;         // 1 when i == j (the outputs carry 1/sqrt 2)
 };
 
 // A plain GEMM problem C = A * B (A M x K, B K x N, C M x N, all row-major)

@@ -436,17 +436,26 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
       return Ly.bJ[Ly.pair_q[x]] < Ly.bJ[Ly.pair_q[y]];
     });
     // one work item per 64 upper outputs of a row, rows in (i, j, J) order
-    std::vector<int2> chunks;
+    std::vector<PhChunk> chunks;
     std::vector<double> wa;
     for (int k = 0; k < Ly.npairs; k++) {
-      const int r = arow[k], q = Ly.pair_q[r], rl = r - Ly.pbase[q];
-      for (int s0 = rl; s0 < Ly.bdim[q]; s0 += 64) {
-        chunks.push_back(make_int2(r, s0));
-        wa.push_back(16.0 + std::min(64, Ly.bdim[q] - s0));
+      const int r = arow[k], q = Ly.pair_q[r], rl = r - Ly.pbase[q], dim = Ly.bdim[q];
+      const int i = Ly.pair_a[r], j = Ly.pair_b[r];
+      const int key = ((Ly.jj[i] >> 1) * Ly.nhalf + (Ly.jj[j] >> 1)) * Ly.nJ + (Ly.bJ[q] >> 1);
+      for (int s0 = rl; s0 < dim; s0 += 64) {
+        PhChunk h;
+        h.vrow = Ly.v_off + boff[q] + (long long)rl * dim + s0;
+        h.dbase = Ly.ph_ubase[r] - rl + s0;
+        h.n = std::min(64, dim - s0);
+        h.b0 = Ly.ap6_base[key];
+        h.len = Ly.ap6_base[key + 1] - Ly.ap6_base[key];
+        h.half = i == j;
+        chunks.push_back(h);
+        wa.push_back(16.0 + h.n);
       }
     }
     D.xs_apply = make_split(wa);
-    UP(chunks, apply_chunks);
+    UP(chunks, ph_chunks);
   }
   std::vector<int4> mtiles;
   for (int q = 0; q < Ly.nblocks; q++) {

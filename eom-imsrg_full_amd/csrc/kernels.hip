@@ -844,23 +844,64 @@ __global__ __launch_bounds__(256) void k_pandya(DevLayout L, const double *__res
   int *BD = PS + nJ * N;
   int *OQ = BD + NB;                           // jj | l << 8 | (2tz + 1) << 16
   {
+    // every staging load of a batch is issued before the first LDS store, so
+    // a thread waits for one memory latency per batch, not one per element
     const double *six0 = L.sixj + (((size_t)(jp >> 1) * h6 + (jh >> 1)) * nJ + (JC >> 1)) * (size_t)nsl;
-    for (int e = threadIdx.x; e < nsl; e += 256) S6[e] = six0[e];
+    for (int e0 = 0; e0 < nsl; e0 += 256 * 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int e = e0 + (int)threadIdx.x + 256 * u;
+        v[u] = e < nsl ? six0[e] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int e = e0 + (int)threadIdx.x + 256 * u;
+        if (e < nsl) S6[e] = v[u];
+      }
+    }
     for (int e = threadIdx.x; e < NB; e += 256) {
       BO[e] = L.boff[e];
       BD[e] = L.bdim[e];
     }
-    for (int e = threadIdx.x; e < nJ * N; e += 256) {
-      const int J2 = e / N, x = e - J2 * N;
-      const int *pj = L.pidx + (size_t)J2 * NN;
-      PR[e] = h > x ? pj[x * N + h] : pj[h * N + x];
-      PS[e] = p > x ? pj[x * N + p] : pj[p * N + x];
+    for (int e0 = 0; e0 < nJ * N; e0 += 256 * 8) {
+      int vr[8], vs[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int e = e0 + (int)threadIdx.x + 256 * u;
+        vr[u] = vs[u] = -1;
+        if (e < nJ * N) {
+          const int J2 = e / N, x = e - J2 * N;
+          const int *pj = L.pidx + (size_t)J2 * NN;
+          vr[u] = h > x ? pj[x * N + h] : pj[h * N + x];
+          vs[u] = p > x ? pj[x * N + p] : pj[p * N + x];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int e = e0 + (int)threadIdx.x + 256 * u;
+        if (e < nJ * N) {
+          PR[e] = vr[u];
+          PS[e] = vs[u];
+        }
+      }
     }
     for (int e = threadIdx.x; e < N; e += 256) OQ[e] = L.jj[e] | (L.l[e] << 8) | ((L.tz[e] + 1) << 16);
   }
   __syncthreads();
+  // the next row's orbits are loaded while the current row's terms are summed
+  const int *rowa = L.ccrow_a + L.ccrbase[q1], *rowb = L.ccrow_b + L.ccrbase[q1];
+  int a_next = -1, b_next = -1;
+  if ((int)threadIdx.x < rlen) {
+    a_next = rowa[threadIdx.x];
+    b_next = rowb[threadIdx.x];
+  }
   for (int r = threadIdx.x; r < rlen; r += blockDim.x) {
-    const int a = L.ccrow_a[L.ccrbase[q1] + r], b = L.ccrow_b[L.ccrbase[q1] + r];
+    const int a = a_next, b = b_next;
+    if (r + (int)blockDim.x < rlen) {
+      a_next = rowa[r + blockDim.x];
+      b_next = rowb[r + blockDim.x];
+    }
     if (a < 0) {  // zero row that makes the row count even
       out[r] = 0.0;
       continue;
@@ -877,10 +918,10 @@ __global__ __launch_bounds__(256) void k_pandya(DevLayout L, const double *__res
       const bool sw1 = h > a, sw2 = p > b;
       const double *six = S6 + ((ja >> 1) * h6 + (jb >> 1)) * nJ;
       double sm = 0.0;
-      for (int J0 = Jmin; J0 <= Jmax; J0 += 8) {
-        double vv[4], cf[4];
+      for (int J0 = Jmin; J0 <= Jmax; J0 += 16) {
+        double vv[8], cf[8];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
+        for (int u = 0; u < 8; u++) {
           const int JT = J0 + 2 * u;
           vv[u] = 0.0;
           cf[u] = 0.0;
@@ -896,7 +937,7 @@ __global__ __launch_bounds__(256) void k_pandya(DevLayout L, const double *__res
           }
         }
 #pragma unroll
-        for (int u = 0; u < 4; u++) sm = sm + cf[u] * vv[u];
+        for (int u = 0; u < 8; u++) sm = sm + cf[u] * vv[u];
       }
       val = sm * sgnp((jh + jb + JC) >> 1) * sq;
     }
@@ -1034,14 +1075,20 @@ __global__ __launch_bounds__(256) void k_ph_terms(DevLayout L, const double *__r
 
 // The 6j symbols a row can use, {ji jj J; x y J'} with triangle (x, y, J) and
 // J' in the triangle ranges of the two J' sums, are one packed list per
-// (ji, jj, J), staged in LDS once per row (zero-padded to the longest list).
-// Work items are 64 outputs of one row; rows run in (i, j, J) order, split
-// into 8 contiguous ranges (one per XCD): the rows of one orbit pair read the
-// same quadruples' J' terms, so the repeats hit L2.  What each output needs
-// beyond its row (where its J' terms
-// start, how many, where its 6j runs start in the list, signs) is one 64-bit
-// host-built descriptor.  Each output's two J' ranges are one contiguous run
-// in Tbuf, loaded 8 at a time so a thread keeps several loads in flight.
+// (ji, jj, J), staged in LDS once per work item.  A work item is up to 64
+// outputs of one row (one wavefront); rows run in (i, j, J) order, split into
+// 8 contiguous ranges (one per XCD): the rows of one orbit pair read the same
+// quadruples' J' terms, so the repeats hit L2.  What each output needs beyond
+// its row (where its J' terms start, how many, where its 6j runs start in the
+// list, signs) is one 64-bit host-built descriptor; what the row needs is one
+// PhChunk header.  An output's two J' ranges are one contiguous run in Tbuf
+// that starts at an even offset (runs are padded to even lengths with zero
+// terms).  Four lanes read an output's run together, each two terms with one
+// 16-byte load, so the four read 64 contiguous bytes per instruction (one or
+// two cache lines, where one lane per output touched a line per term); the
+// wavefront covers its 64 outputs as four groups of 16, with every group's
+// descriptor, old value and first two loads issued before any is used, and
+// two lane exchanges add each output's four partial sums.
 __device__ __forceinline__ void ph_decode(unsigned long long d, int &toff, int &n1, int &n, int &o1, int &o2,
                                           double &g1, double &g2, double &pre2) {
   toff = (int)(d & 0x7FFFFFFull);
@@ -1053,49 +1100,88 @@ __device__ __forceinline__ void ph_decode(unsigned long long d, int &toff, int &
   g2 = ((d >> 59) & 1) ? 1.0 : 0.0;
   pre2 = ((d >> 60) & 1) ? .70710678118 : 1.0;
 }
+// 6j weight of term m of an output (0 past its run); index 0 when unused
+__device__ __forceinline__ double ph_coef(const double *S6, int m, int n, int n1, int o1, int o2, double g1,
+                                          double g2) {
+  const bool s1 = m < n1, in = m < n;
+  const int e = in ? (s1 ? o1 + m : o2 + m) : 0;
+  return (in ? (s1 ? g1 : g2) : 0.0) * S6[e];
+}
 __global__ __launch_bounds__(64) void k_ph_apply(DevLayout L, const double *__restrict__ Tbuf,
                                                  double *__restrict__ RES, const int *act) {
-  extern __shared__ double S6[];  // ap6_max doubles
+  extern __shared__ double S6[];  // max(ap6_max, 64) doubles
   const int it = xcd_item(L.xs_apply);
   if (it < 0) return;
-  const int2 ch = L.apply_chunks[it];
-  const int r = ch.x;
+  const PhChunk h = L.ph_chunks[it];
   {
     const long long inst = inst_id(act);
     if (inst < 0) return;
     Tbuf += inst * L.nphterm;
     RES += inst * L.op_size;
   }
-  const int q = L.pair_q[r], rl = r - L.pbase[q], dim = L.bdim[q], J = L.bJ[q];
-  const int i = L.pair_a[r], j = L.pair_b[r];
-  {
-    const int key = ((L.jj[i] >> 1) * L.nhalf + (L.jj[j] >> 1)) * L.nJ + (J >> 1);
-    const int b0 = L.ap6_base[key], len = L.ap6_base[key + 1] - b0;
-    for (int e = threadIdx.x; e < L.ap6_max; e += 64) S6[e] = e < len ? L.ap6_val[b0 + e] : 0.0;
-  }
-  __syncthreads();
-  const double pre = (i == j) ? .70710678118 : 1.0;
-  double *OV = RES + L.v_off + L.boff[q] + (long long)rl * dim;
-  const unsigned long long *desc = L.ph_desc + L.ph_ubase[r] - rl;
-  {
-    const int s = ch.y + (int)threadIdx.x;
-    if (s >= dim) return;
-    int toff, n1, n, o1, o2;
-    double g1, g2, pre2;
-    ph_decode(desc[s], toff, n1, n, o1, o2, g1, g2, pre2);
-    const double *T = Tbuf + toff;
-    double acc = 0.0;
-    for (int m0 = 0; m0 < n; m0 += 8) {
-      double t[8];
+  double *OV = RES + h.vrow;
+  const unsigned long long *desc = L.ph_desc + h.dbase;
+  const int g = (int)(threadIdx.x >> 2), sub = (int)(threadIdx.x & 3);
+  // descriptors and old values of outputs g, g + 16, g + 32, g + 48
+  unsigned long long d[4];
+  double old[4];
 #pragma unroll
-      for (int u = 0; u < 8; u++) t[u] = (m0 + u < n) ? T[m0 + u] : 0.0;
+  for (int u = 0; u < 4; u++) {
+    const int o = g + 16 * u;
+    d[u] = o < h.n ? desc[o] : 0ull;
+    old[u] = (o < h.n && sub == 0) ? OV[o] : 0.0;
+  }
+  // the list, zero-padded to 64 entries: an output whose J' sum is absent
+  // reads (weight 0) entries below its run length, never above 62
+  {
+    const int ns = max(h.len, 64);
+    for (int e0 = 0; e0 < ns; e0 += 512) {
+      double v[8];
 #pragma unroll
       for (int u = 0; u < 8; u++) {
-        const int m = m0 + u;
-        if (m < n) acc += (m < n1 ? g1 * S6[o1 + m] : g2 * S6[o2 + m]) * t[u];
+        const int e = e0 + (int)threadIdx.x + 64 * u;
+        v[u] = e < h.len ? L.ap6_val[h.b0 + e] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int e = e0 + (int)threadIdx.x + 64 * u;
+        if (e < ns) S6[e] = v[u];
       }
     }
-    OV[s] = OV[s] + acc * pre * pre2;
+  }
+  // terms m, m + 1 for m = 2 sub + 8 w: two loads per output in flight cover n <= 16
+  const int m0 = 2 * sub;
+  double2 t[4][2];
+  int toff[4], n1[4], n[4], o1[4], o2[4];
+  double g1[4], g2[4], pre2[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    ph_decode(d[u], toff[u], n1[u], n[u], o1[u], o2[u], g1[u], g2[u], pre2[u]);
+    const double2 *T = reinterpret_cast<const double2 *>(Tbuf + toff[u]);
+#pragma unroll
+    for (int w = 0; w < 2; w++) t[u][w] = (m0 + 8 * w < n[u]) ? T[(m0 >> 1) + 4 * w] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  const double pre = h.half ? .70710678118 : 1.0;
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    double acc = 0.0;
+#pragma unroll
+    for (int w = 0; w < 2; w++) {
+      const int m = m0 + 8 * w;
+      acc += ph_coef(S6, m, n[u], n1[u], o1[u], o2[u], g1[u], g2[u]) * t[u][w].x +
+             ph_coef(S6, m + 1, n[u], n1[u], o1[u], o2[u], g1[u], g2[u]) * t[u][w].y;
+    }
+    const double2 *T = reinterpret_cast<const double2 *>(Tbuf + toff[u]);
+    for (int m = m0 + 16; m < n[u]; m += 8) {
+      const double2 tt = T[m >> 1];
+      acc += ph_coef(S6, m, n[u], n1[u], o1[u], o2[u], g1[u], g2[u]) * tt.x +
+             ph_coef(S6, m + 1, n[u], n1[u], o1[u], o2[u], g1[u], g2[u]) * tt.y;
+    }
+    acc += __shfl_xor(acc, 1, 64);
+    acc += __shfl_xor(acc, 2, 64);
+    const int o = g + 16 * u;
+    if (sub == 0 && o < h.n) OV[o] = old[u] + acc * pre2[u] * pre;
   }
 }
 void launch_ph_terms(hipStream_t s, const DevLayout &L, const double *Z, double *Tbuf, double hL, double hR,
@@ -1106,7 +1192,7 @@ void launch_ph_terms(hipStream_t s, const DevLayout &L, const double *Z, double 
 void launch_ph_apply(hipStream_t s, const DevLayout &L, const double *Tbuf, double *RES, const int *act,
                      int nact) {
   if (L.nphterm <= 0 || nact <= 0) return;
-  const size_t lds = sizeof(double) * L.ap6_max;
+  const size_t lds = sizeof(double) * std::max(L.ap6_max, 64);  // the list is staged zero-padded to >= 64
   hipLaunchKernelGGL(k_ph_apply, dim3(8 * L.xs_apply.len, nact), dim3(64), lds, s, L, Tbuf, RES, act);
 }
 

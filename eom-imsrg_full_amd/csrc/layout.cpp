@@ -298,6 +298,7 @@ std::string Layout::build(int nsp_, const int *n_, const int *l_, const int *jj_
       qtoff[t] = (uint32_t)ph_terms.size();
       for (int JP = a1, m = 0; JP <= b1; JP += 2, m++) ph_terms.push_back(((uint32_t)t << 6) | (uint32_t)m);
       for (int JP = a2, m = 0; JP <= b2; JP += 2, m++) ph_terms.push_back(((uint32_t)t << 6) | 32u | (uint32_t)m);
+      if (ph_terms.size() & 1) ph_terms.push_back(kPhPadTerm);  // runs start at even offsets
     }
     if (ph_terms.size() >= 0xFFFFFFFFull) return "too many ph terms";
     // per upper element (block-major, row-major upper triangle): its quadruple's toff
@@ -327,7 +328,9 @@ std::string Layout::build(int nsp_, const int *n_, const int *l_, const int *jj_
     ph_zsel.assign(ph_terms.size(), 0);
     const int JT1 = JTM + 1;
     for (size_t t = 0; t < ph_terms.size(); t++) {
-      const uint32_t tc = ph_terms[t], c = ph_quads[tc >> 6];
+      const uint32_t tc = ph_terms[t];
+      if (tc == kPhPadTerm) continue;  // a zero term (selector 0)
+      const uint32_t c = ph_quads[tc >> 6];
       const int side = (tc >> 5) & 1, m = tc & 31;
       const int oi = c >> 24, oj = (c >> 16) & 255, ok = (c >> 8) & 255, ol = c & 255;
       int qx, R1, C1, R2, C2;

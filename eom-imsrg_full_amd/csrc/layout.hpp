@@ -69,7 +69,7 @@ struct Layout {
   // (i,j,k,l) quadruples owning at least one upper element (ij,kl;J), packed
   // i<<24|j<<16|k<<8|l and sorted by (i,k,j,l) for the ph recoupling kernel
   std::vector<uint32_t> ph_quads;
-  std::vector<uint32_t> ph_terms;   // quad<<6 | side<<5 | m (side 0: T1, 1: T2)
+  std::vector<uint32_t> ph_terms;   // quad<<6 | side<<5 | m (side 0: T1, 1: T2), or kPhPadTerm
   std::vector<uint32_t> ph_toff;    // per upper element: offset of its quadruple's T1 terms
   std::vector<int64_t> ph_ubase;    // per global pair row: first upper element
   std::vector<uint64_t> ph_desc;    // per upper element: T offset, run lengths, 6j runs, flags
@@ -111,5 +111,8 @@ struct Layout {
 };
 
 double racah6j(int j1, int j2, int j3, int j4, int j5, int j6);  // doubled arguments
+
+// a zero J' term that pads a quadruple's run to an even length
+constexpr uint32_t kPhPadTerm = 0xFFFFFFFFu;
 
 }  // namespace imsrg

@@ -113,6 +113,7 @@ program drop_in_test
   checkpointing = .true.
   prefix = 'dropin_hw20'
   scratch = './'
+  resubmitter = ''   ! no batch-job resubmission after a checkpoint (main_IMSRG reads it from argument 2)
   call copy_sq_op(H, A) ; call clear_sq_op(ETA1) ; ETA1%herm = -1
   call magnus_decouple(A, ETA1, jbas, 'n', 'n', build_gs_white)
   call copy_sq_op(H, A) ; call clear_sq_op(ETA1) ; ETA1%herm = -1
