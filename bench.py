@@ -53,6 +53,8 @@ def parse():
     p.add_argument("--no-concurrency", action="store_true",
                    help="run each commutator on one stream (default: pp/hh and ph chains on two)")
     p.add_argument("--batch", type=int, default=16, help="instances of the batched measurement (0: skip)")
+    p.add_argument("--hf-reps", type=int, default=3,
+                   help="timed Hartree-Fock solutions of the bare workload Hamiltonian (0: skip)")
     p.add_argument("--batch-emax", type=int, default=6)
     p.add_argument("--batch-holes", type=int, default=3)
     p.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -304,6 +306,20 @@ def verify(ctx, E, H0vec, a, prec):
     return out
 
 
+def hf_bench(ctx, bare_vec, reps):
+    """imsrg_hartree_fock on the bare workload Hamiltonian: wall time of the
+    whole call (self-consistent loop with one host read of the convergence
+    test per iteration, then the one- and two-body transforms)"""
+    ts = []
+    for r in range(reps + 1):  # the first call also allocates the workspace
+        H = ctx.op(1, bare_vec)
+        t0 = time.perf_counter()
+        D, e, it = H.hartree_fock()
+        if r:
+            ts.append(time.perf_counter() - t0)
+    return {"ms": 1e3 * float(np.median(ts)), "iterations": it, "e_hf_MeV": e, "reps": reps}
+
+
 def batch_bench(a, E, Basis, batch_hamiltonians):
     """Config-3 style measurement: `a.batch` independent Hamiltonians (seeds
     0.., hw in [16,24] MeV, sigma2 in [0.5,1.5] MeV) on one batched context,
@@ -384,7 +400,8 @@ def main():
     if a.no_concurrency:
         ctx.set_concurrency(False)
     info = ctx.info()
-    H = ctx.op(1, synthetic_hamiltonian(basis, seed=0, hw=20.0, sigma1=0.5, sigma2=1.0))
+    bare = synthetic_hamiltonian(basis, seed=0, hw=20.0, sigma1=0.5, sigma2=1.0)
+    H = ctx.op(1, bare)
     H.normal_order()
     H0vec = H.download()
     eta_vec = ctx.generator(a.generator, H).download() if a.cpu_baseline != "none" else None
@@ -443,6 +460,11 @@ def main():
         by_kernel[k]["traffic"] = traffic.get(k)
     kernels = {k: {"ms_total": round(v[0], 3), "launches": v[1]} for k, v in breakdown.items() if v[1]}
     log("kernel classes of one profiled step (ms, launches):", json.dumps(kernels))
+    hf = None
+    if a.hf_reps > 0:
+        log("Hartree-Fock of the bare workload Hamiltonian")
+        hf = hf_bench(ctx, bare, a.hf_reps)
+        log(f"  {hf['ms']:.2f} ms, {hf['iterations']} iterations, E_HF={hf['e_hf_MeV']:.10f}")
     batch = None
     if a.batch > 0:
         log(f"batched flows: {a.batch} instances at emax={a.batch_emax}")
@@ -474,6 +496,7 @@ def main():
             "speedup_vs_fp64": (steps_per_s / variants["fp64"]["steps_per_s"]) if "fp64" in variants else None,
             "fp64_tile_shapes": ctx.tuning(),
             "batch": batch,
+            "hartree_fock": hf,
         },
         "roofline": roof,
         "roofline_by_kernel": {k: {"achieved": round(v["achieved"], 2), "unit": v["unit"],

@@ -27,6 +27,10 @@ struct DevLayout {
   const int *pidx;
   const int *nb_start, *nb_pos, *nb_x, *nb_i;
   const double *nb_coef;
+  // Hartree-Fock: groups in the reference's block order, same-group orbit
+  // pairs (all; a <= c), per pair row the rows the pair-basis transform couples
+  const int *hf_gorder, *hf_b, *hf_d, *hf_a, *hf_c, *hf_nb_start, *hf_nb;
+  int ngrp, n_hf_bd, n_hf_ac;
   const int *ob_p, *ob_q;
   const int *php_a, *php_i;  // same-(l,j,tz) (particle, hole) pairs
   int nsame, ob_nchunk;      // one-body outputs with 121/221 terms; 256-term chunks per output
@@ -218,6 +222,16 @@ void launch_gather_e0(hipStream_t s, const double *op, long long stride, double 
 // truncation test of a BCH (mode 0) or Magnus (mode 1) series after a nested
 // commutator: instances that stop get -1 in the series table `act`; live[k]
 // (host memory) = 1 for the positions that go on
+// Hartree-Fock (hf.hip)
+constexpr int kHFMaxGroup = 16;
+void launch_hf_init(hipStream_t s, const DevLayout &L, const double *op, double *D, double *Tm, double *eig,
+                    int reset);
+void launch_hf_iteration(hipStream_t s, const DevLayout &L, const double *op, double *D, double *rho,
+                         const double *Tm, double *Vg, double *Fm, double *eig, double *cpart, double *crit);
+void launch_hf_finish_1b(hipStream_t s, const DevLayout &L, double *D, const double *M, double *op, int herm,
+                         const double *Tm, const double *Vg, double *tv, double *ehf);
+void launch_hf_2b(hipStream_t s, const DevLayout &L, const double *D, double *op, double *W, int herm);
+
 void launch_series_gate(hipStream_t s, int *act, int nact, double *scal, int mode, double coef, int store,
                         int test, double conv, double tiny, int *live);
 

@@ -92,6 +92,11 @@ struct imsrg_ctx_s {
   int *act_d[kSlots] = {nullptr, nullptr, nullptr, nullptr};
   int *act_h[kSlots] = {nullptr, nullptr, nullptr, nullptr};
   imsrg_op_s *tmp[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  // Hartree-Fock workspace (allocated on first use): D, rho, gamma, Fock and
+  // bare one-body matrices (nsp x nsp), eigenvalues, e_HF diagonals,
+  // per-group eigenvalue changes, [crit, e_HF]
+  double *hf_D = nullptr, *hf_rho = nullptr, *hf_Vg = nullptr, *hf_Fm = nullptr, *hf_Tm = nullptr;
+  double *hf_eig = nullptr, *hf_tv = nullptr, *hf_cp = nullptr, *hf_sc = nullptr;
   // per-kernel-class HIP event timing on this context's stream
   bool prof_on = false;
   unsigned prof_mask = ~0u;  // kernel classes bracketed while profiling
@@ -399,6 +404,11 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
   UP(Ly.pbase, pbase); UP(Ly.pair_a, pair_a); UP(Ly.pair_b, pair_b); UP(Ly.pair_q, pair_q);
   UP(Ly.pidx, pidx);
   UP(Ly.nb_start, nb_start); UP(Ly.nb_pos, nb_pos); UP(Ly.nb_x, nb_x); UP(Ly.nb_i, nb_i);
+  UP(Ly.hf_gorder, hf_gorder); UP(Ly.hf_b, hf_b); UP(Ly.hf_d, hf_d); UP(Ly.hf_a, hf_a); UP(Ly.hf_c, hf_c);
+  UP(Ly.hf_nb_start, hf_nb_start); UP(Ly.hf_nb, hf_nb);
+  D.ngrp = (int)Ly.grp_start.size();
+  D.n_hf_bd = (int)Ly.hf_b.size();
+  D.n_hf_ac = (int)Ly.hf_a.size();
   UP(Ly.nb_coef, nb_coef);
   UP(Ly.ob_p, ob_p); UP(Ly.ob_q, ob_q); UP(php_a, php_a); UP(php_i, php_i);
   UP(same_ob, same_ob); UP(ob_same, ob_same);
@@ -941,6 +951,79 @@ int imsrg_normal_order(imsrg_op H) {
   launch_normal_order(c->stream, c->dl, T->d, H->d, all.d, all.n());
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+// ---------------------------------------------------------------- Hartree-Fock
+static int hf_setup(imsrg_ctx_s *c) {
+  if (c->hf_D) return 0;
+  const Layout &Ly = c->lay;
+  for (int n : Ly.grp_cnt)
+    if (n > kHFMaxGroup) return fail(IMSRG_EINVAL, "Hartree-Fock: an (l,j,tz) group has more than 16 orbits");
+  const size_t n2 = (size_t)Ly.nsp * Ly.nsp;
+  double **mats[5] = {&c->hf_D, &c->hf_rho, &c->hf_Vg, &c->hf_Fm, &c->hf_Tm};
+  for (double **m : mats) {
+    if (int rc = dalloc(c, n2, m)) return rc;
+    HIPCHK(hipMemsetAsync(*m, 0, n2 * sizeof(double), c->stream));
+  }
+  if (int rc = dalloc(c, Ly.nsp, &c->hf_eig)) return rc;
+  if (int rc = dalloc(c, 2 * (size_t)Ly.nsp, &c->hf_tv)) return rc;
+  if (int rc = dalloc(c, Ly.grp_start.size(), &c->hf_cp)) return rc;
+  if (int rc = dalloc(c, 2, &c->hf_sc)) return rc;
+  return 0;
+}
+
+int imsrg_hartree_fock(imsrg_op H, double *coefs, int *iterations, double *e_hf) {
+  if (!H) return fail(IMSRG_EINVAL, "null operand");
+  imsrg_ctx_s *c = H->ctx;
+  if (int rc = single_only(c, "imsrg_hartree_fock")) return rc;
+  if (H->herm != 1) return fail(IMSRG_EINVAL, "imsrg_hartree_fock: H must be hermitian");
+  if (int rc = hf_setup(c)) return rc;
+  const DevLayout &D = c->dl;
+  launch_hf_init(c->stream, D, H->d, c->hf_D, c->hf_Tm, c->hf_eig, 1);
+  // the reference's loop test `crit > 1e-6` compares with a default-real literal
+  const double conv = (double)1e-6f;
+  double crit = 10.0;
+  int it = 0;
+  while (crit > conv) {
+    if (it >= 10000) return fail(IMSRG_ECONV, "Hartree-Fock did not converge in 10000 iterations");
+    launch_hf_iteration(c->stream, D, H->d, c->hf_D, c->hf_rho, c->hf_Tm, c->hf_Vg, c->hf_Fm, c->hf_eig, c->hf_cp,
+                        c->hf_sc);
+    HIPCHK(hipMemcpyAsync(c->hscal, c->hf_sc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    crit = c->hscal[0];
+    it++;
+  }
+  launch_hf_finish_1b(c->stream, D, c->hf_D, c->hf_Fm, H->d, 1, c->hf_Tm, c->hf_Vg, c->hf_tv, c->hf_sc + 1);
+  imsrg_op_s *W = tmp_op(c, 5, 1);
+  if (!W) return IMSRG_ENOMEM;
+  launch_hf_2b(c->stream, D, c->hf_D, H->d, W->d, 1);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(c->hscal, c->hf_sc + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (coefs)
+    HIPCHK(hipMemcpyAsync(coefs, c->hf_D, sizeof(double) * c->lay.nsp * c->lay.nsp, hipMemcpyDeviceToHost,
+                          c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (e_hf) *e_hf = c->hscal[0];
+  if (iterations) *iterations = it;
+  return 0;
+}
+
+int imsrg_observable_to_hf(imsrg_op O, const double *coefs) {
+  if (!O || !coefs) return fail(IMSRG_EINVAL, "null argument");
+  imsrg_ctx_s *c = O->ctx;
+  if (int rc = single_only(c, "imsrg_observable_to_hf")) return rc;
+  if (int rc = hf_setup(c)) return rc;
+  const DevLayout &D = c->dl;
+  HIPCHK(hipMemcpyAsync(c->hf_D, coefs, sizeof(double) * c->lay.nsp * c->lay.nsp, hipMemcpyHostToDevice,
+                        c->stream));
+  launch_hf_init(c->stream, D, O->d, c->hf_D, c->hf_Tm, nullptr, 0);
+  launch_hf_finish_1b(c->stream, D, c->hf_D, c->hf_Tm, O->d, O->herm, nullptr, nullptr, nullptr, nullptr);
+  imsrg_op_s *W = tmp_op(c, 5, O->herm);
+  if (!W) return IMSRG_ENOMEM;
+  launch_hf_2b(c->stream, D, c->hf_D, O->d, W->d, O->herm);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));  // the caller's coefs may go away
+  return imsrg_normal_order(O);
 }
 
 // ---------------------------------------------------------------- commutator

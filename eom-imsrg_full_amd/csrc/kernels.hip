@@ -1135,53 +1135,54 @@ __global__ __launch_bounds__(64) void k_ph_apply(DevLayout L, const double *__re
   // reads (weight 0) entries below its run length, never above 62
   {
     const int ns = max(h.len, 64);
-    for (int e0 = 0; e0 < ns; e0 += 512) {
-      double v[8];
+    for (int e0 = 0; e0 < ns; e0 += 256) {
+      double v[4];
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
+      for (int u = 0; u < 4; u++) {
         const int e = e0 + (int)threadIdx.x + 64 * u;
         v[u] = e < h.len ? L.ap6_val[h.b0 + e] : 0.0;
       }
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
+      for (int u = 0; u < 4; u++) {
         const int e = e0 + (int)threadIdx.x + 64 * u;
         if (e < ns) S6[e] = v[u];
       }
     }
   }
   // terms m, m + 1 for m = 2 sub + 8 w: two loads per output in flight cover n <= 16
+  // (only the run offset and length are decoded here; the rest at use, so
+  // that few registers stay live across the loads)
   const int m0 = 2 * sub;
   double2 t[4][2];
-  int toff[4], n1[4], n[4], o1[4], o2[4];
-  double g1[4], g2[4], pre2[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
-    ph_decode(d[u], toff[u], n1[u], n[u], o1[u], o2[u], g1[u], g2[u], pre2[u]);
-    const double2 *T = reinterpret_cast<const double2 *>(Tbuf + toff[u]);
+    const int toff = (int)(d[u] & 0x7FFFFFFull), n = (int)((d[u] >> 27) & 31) + (int)((d[u] >> 32) & 31);
+    const double2 *T = reinterpret_cast<const double2 *>(Tbuf + toff);
 #pragma unroll
-    for (int w = 0; w < 2; w++) t[u][w] = (m0 + 8 * w < n[u]) ? T[(m0 >> 1) + 4 * w] : make_double2(0.0, 0.0);
+    for (int w = 0; w < 2; w++) t[u][w] = (m0 + 8 * w < n) ? T[(m0 >> 1) + 4 * w] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   const double pre = h.half ? .70710678118 : 1.0;
 #pragma unroll
   for (int u = 0; u < 4; u++) {
+    int toff, n1, n, o1, o2;
+    double g1, g2, pre2;
+    ph_decode(d[u], toff, n1, n, o1, o2, g1, g2, pre2);
     double acc = 0.0;
 #pragma unroll
     for (int w = 0; w < 2; w++) {
       const int m = m0 + 8 * w;
-      acc += ph_coef(S6, m, n[u], n1[u], o1[u], o2[u], g1[u], g2[u]) * t[u][w].x +
-             ph_coef(S6, m + 1, n[u], n1[u], o1[u], o2[u], g1[u], g2[u]) * t[u][w].y;
+      acc += ph_coef(S6, m, n, n1, o1, o2, g1, g2) * t[u][w].x + ph_coef(S6, m + 1, n, n1, o1, o2, g1, g2) * t[u][w].y;
     }
-    const double2 *T = reinterpret_cast<const double2 *>(Tbuf + toff[u]);
-    for (int m = m0 + 16; m < n[u]; m += 8) {
+    const double2 *T = reinterpret_cast<const double2 *>(Tbuf + toff);
+    for (int m = m0 + 16; m < n; m += 8) {
       const double2 tt = T[m >> 1];
-      acc += ph_coef(S6, m, n[u], n1[u], o1[u], o2[u], g1[u], g2[u]) * tt.x +
-             ph_coef(S6, m + 1, n[u], n1[u], o1[u], o2[u], g1[u], g2[u]) * tt.y;
+      acc += ph_coef(S6, m, n, n1, o1, o2, g1, g2) * tt.x + ph_coef(S6, m + 1, n, n1, o1, o2, g1, g2) * tt.y;
     }
     acc += __shfl_xor(acc, 1, 64);
     acc += __shfl_xor(acc, 2, 64);
     const int o = g + 16 * u;
-    if (sub == 0 && o < h.n) OV[o] = old[u] + acc * pre2[u] * pre;
+    if (sub == 0 && o < h.n) OV[o] = old[u] + acc * pre2 * pre;
   }
 }
 void launch_ph_terms(hipStream_t s, const DevLayout &L, const double *Z, double *Tbuf, double hL, double hR,

@@ -180,6 +180,52 @@ std::string Layout::build(int nsp_, const int *n_, const int *l_, const int *jj_
     }
   }
   nb_start[npairs] = (int)nb_pos.size();
+  // Hartree-Fock tables
+  {
+    const int ng = (int)grp_start.size();
+    hf_gorder.resize(ng);
+    for (int g = 0; g < ng; g++) hf_gorder[g] = g;
+    std::sort(hf_gorder.begin(), hf_gorder.end(), [&](int x, int y) {
+      const int a = grp_list[grp_start[x]], b = grp_list[grp_start[y]];
+      if (tz[a] != tz[b]) return tz[a] < tz[b];
+      if (l[a] != l[b]) return l[a] < l[b];
+      return jj[a] < jj[b];
+    });
+    hf_b.clear(); hf_d.clear(); hf_a.clear(); hf_c.clear();
+    for (int g : hf_gorder)
+      for (int x = 0; x < grp_cnt[g]; x++)
+        for (int y = 0; y < grp_cnt[g]; y++) {
+          const int a = grp_list[grp_start[g] + x], c = grp_list[grp_start[g] + y];
+          hf_b.push_back(a);
+          hf_d.push_back(c);
+          if (x <= y) {
+            hf_a.push_back(a);
+            hf_c.push_back(c);
+          }
+        }
+    hf_nb_start.assign(npairs + 1, 0);
+    hf_nb.clear();
+    std::vector<int> cand;
+    for (int r = 0; r < npairs; r++) {
+      hf_nb_start[r] = (int)hf_nb.size();
+      const int i = pair_a[r], j = pair_b[r], J = bJ[pair_q[r]];
+      cand.clear();
+      for (int side = 0; side < 2; side++) {
+        const int g1 = grp_of[side == 0 ? i : j], g2 = grp_of[side == 0 ? j : i];
+        for (int x = 0; x < grp_cnt[g1]; x++)
+          for (int y = 0; y < grp_cnt[g2]; y++) {
+            const int a = grp_list[grp_start[g1] + x], b = grp_list[grp_start[g2] + y];
+            const int p = pidx[(size_t)(J / 2) * nn2 + (size_t)std::min(a, b) * nsp + std::max(a, b)];
+            if (p < 0) return "internal: Hartree-Fock pair outside block";
+            cand.push_back(p);
+          }
+      }
+      std::sort(cand.begin(), cand.end());
+      cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+      hf_nb.insert(hf_nb.end(), cand.begin(), cand.end());
+    }
+    hf_nb_start[npairs] = (int)hf_nb.size();
+  }
   // Pandya blocks
   ncc = (JTM + 1) * 4;
   ccJ.assign(ncc, 0); ccrlen.assign(ncc, 0); ccnph.assign(ncc, 0); ccTz.assign(ncc, 0); ccP.assign(ncc, 0);

@@ -39,6 +39,15 @@ struct Layout {
   // pidx[(J/2)*nsp*nsp + a*nsp + b] (a <= b): position inside block's full matrix
   std::vector<int> pidx;
 
+  // Hartree-Fock tables.  hf_gorder: the (l,j,tz) groups in the reference's
+  // single-particle block order (2tz, l, 2j; allocate_sp_mat, basic_IMSRG.f90:
+  // 2600-2660); hf_b/hf_d: every ordered orbit pair of one group, groups in
+  // that order; hf_a/hf_c: the pairs with a <= c.  hf_nb[hf_nb_start[r] ..]:
+  // for pair row r = (i,j) of block q, the rows (a,b) of q with a, b in the
+  // groups of i, j (either order): the non-zeros of column r of the pair-basis
+  // transform (transform_2b_to_HF, HF_mod.f90:494-619), positions in the block.
+  std::vector<int> hf_gorder, hf_b, hf_d, hf_a, hf_c, hf_nb_start, hf_nb;
+
   // one-body "upper" outputs: (p,q) with p<=q inside hh/pp, all (particle,hole)
   std::vector<int> ob_p, ob_q;
 

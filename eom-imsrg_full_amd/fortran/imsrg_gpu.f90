@@ -17,6 +17,7 @@ module imsrg_gpu
   use iso_c_binding
   use basic_IMSRG
   use cross_coupled
+  use three_body_routines, only: three_body_force
   implicit none
   private
 
@@ -29,7 +30,7 @@ module imsrg_gpu
        gpu_commutator_121, gpu_commutator_122, gpu_commutator_222_pp_hh, gpu_commutator_221, &
        gpu_commutator_222_ph, gpu_calculate_cross_coupled
   public :: gpu_BCH_EXPAND, gpu_MAGNUS_EXPAND, gpu_build_generator, gpu_magnus_decouple
-  public :: gpu_transform_sq_op_BCH, gpu_dHds
+  public :: gpu_transform_sq_op_BCH, gpu_dHds, gpu_calc_HF, gpu_observable_to_HF
 
   type(c_ptr), save :: ctx = c_null_ptr
   integer(c_int64_t), save :: neq = 0
@@ -116,6 +117,17 @@ module imsrg_gpu
        type(c_ptr), value :: Op, G
        integer(c_int) :: nterms
      end function c_transform_bch
+     integer(c_int) function c_hartree_fock(H, coefs, iterations, e_hf) bind(C, name='imsrg_hartree_fock')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: H
+       real(c_double) :: coefs(*), e_hf
+       integer(c_int) :: iterations
+     end function c_hartree_fock
+     integer(c_int) function c_observable_to_hf(O, coefs) bind(C, name='imsrg_observable_to_hf')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: O
+       real(c_double) :: coefs(*)
+     end function c_observable_to_hf
      integer(c_int) function c_dhds(HS, kind, DH) bind(C, name='imsrg_dhds')
        import :: c_int, c_ptr
        type(c_ptr), value :: HS, DH
@@ -406,6 +418,64 @@ contains
     call pull(ho, Op)
     call drop(ho) ; call drop(hg)
   end subroutine gpu_transform_sq_op_BCH
+
+  ! calc_HF (HF_mod.f90:14-106) without a three-body force: the bare H becomes
+  ! the normal-ordered Hamiltonian in the Hartree-Fock basis, and D (unallocated
+  ! on entry, as the reference allocates it) the coefficients in the reference's
+  ! block form (allocate_sp_mat), eigval = the Fock matrix's diagonal
+  subroutine gpu_calc_HF(H, THREEBOD, jbas, D)
+    type(sq_op) :: H
+    type(three_body_force) :: THREEBOD
+    type(spd) :: jbas
+    type(full_sp_block_mat) :: D
+    type(c_ptr) :: hh
+    real(c_double), allocatable :: cf(:,:)
+    real(c_double) :: ehf
+    integer(c_int) :: it
+    integer :: q, i, j
+    if (allocated(THREEBOD%mat)) error stop 'imsrg_gpu: three-body forces in Hartree-Fock are not provided'
+    allocate(cf(jbas%total_orbits, jbas%total_orbits))
+    hh = push(H)
+    call check(c_hartree_fock(hh, cf, it, ehf), 'imsrg_hartree_fock')
+    call pull(hh, H)
+    call drop(hh)
+    write(*,'(A22,f12.7)') ' Hartree Fock Energy: ', ehf
+    ! the library's coefs[a*nsp + i] is cf(i+1, a+1) here
+    call allocate_sp_mat(jbas, D)
+    do q = 1, D%blocks
+       do i = 1, D%map(q)
+          do j = 1, D%map(q)
+             D%blkM(q)%matrix(i, j) = cf(D%blkM(q)%states(j), D%blkM(q)%states(i))
+          end do
+          D%blkM(q)%eigval(i) = f_elem(D%blkM(q)%states(i), D%blkM(q)%states(i), H, jbas)
+       end do
+    end do
+  end subroutine gpu_calc_HF
+
+  ! sqop_to_HF (HF_mod.f90:108-136) for a scalar observable
+  subroutine gpu_observable_to_HF(Op, coefs, jbas)
+    type(sq_op) :: Op
+    type(full_sp_block_mat) :: coefs
+    type(spd) :: jbas
+    type(c_ptr) :: ho
+    real(c_double), allocatable :: cf(:,:)
+    integer :: q, i, j
+    if (allocated(Op%tblck)) error stop 'imsrg_gpu: tensor observables (transform_1b_to_HF_tensor) are not provided'
+    if (.not. allocated(Op%mat)) return
+    allocate(cf(jbas%total_orbits, jbas%total_orbits))
+    cf = 0.d0
+    do q = 1, coefs%blocks
+       do i = 1, coefs%map(q)
+          do j = 1, coefs%map(q)
+             cf(coefs%blkM(q)%states(j), coefs%blkM(q)%states(i)) = coefs%blkM(q)%matrix(i, j)
+          end do
+       end do
+    end do
+    ho = push(Op)
+    call check(c_observable_to_hf(ho, cf), 'imsrg_observable_to_hf')
+    call pull(ho, Op)
+    call drop(ho)
+  end subroutine gpu_observable_to_HF
 
   ! dHds (IMSRG_ODE.f90:416-463), the derivative the ODE solver evaluates: like
   ! the reference it builds the generator from HS (the device generator `kind`

@@ -98,6 +98,8 @@ _SIG = {
     "imsrg_ctx_profile_classes": (C.c_int, [_vp, C.c_uint]),
     "imsrg_ctx_set_concurrency": (C.c_int, [_vp, C.c_int]),
     "imsrg_transform_bch": (C.c_int, [_vp, _vp, C.POINTER(C.c_int)]),
+    "imsrg_hartree_fock": (C.c_int, [_vp, _dp, C.POINTER(C.c_int), _dp]),
+    "imsrg_observable_to_hf": (C.c_int, [_vp, _dp]),
     "imsrg_dhds": (C.c_int, [_vp, C.c_int, _vp]),
     "imsrg_ctx_profile_read": (C.c_int, [_vp, _dp, C.POINTER(C.c_longlong), C.c_int, C.c_int]),
 }
@@ -389,6 +391,26 @@ class Operator:
 
     def normal_order(self):
         _chk(_lib.imsrg_normal_order(self.h))
+        return self
+
+    def hartree_fock(self):
+        """calc_HF on this bare Hamiltonian, in place: it becomes the normal-ordered
+        Hamiltonian in the Hartree-Fock basis.  Returns (D, E_HF, iterations),
+        D[a, i] = component of old orbit a in new orbit i."""
+        n = self.ctx.basis.nsp
+        D = np.zeros((n, n))
+        it = np.zeros(1, np.int32)
+        e = np.zeros(1)
+        _chk(_lib.imsrg_hartree_fock(self.h, _dptr(D), _iptr(it), _dptr(e)))
+        return D, float(e[0]), int(it[0])
+
+    def to_hf(self, D):
+        """sqop_to_HF on this bare scalar observable, in place, with coefficients D"""
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        n = self.ctx.basis.nsp
+        if D.shape != (n, n):
+            raise ValueError("D must be nsp x nsp")
+        _chk(_lib.imsrg_observable_to_hf(self.h, _dptr(D)))
         return self
 
     def normal_order_adjoint(self):

@@ -33,6 +33,7 @@ typedef struct imsrg_op_s *imsrg_op;
 #define IMSRG_EDEVICE -2    /* HIP runtime error                              */
 #define IMSRG_ENOMEM -3     /* device allocation failed                        */
 #define IMSRG_ECLIP -4      /* BCH series did not converge in 40 terms        */
+#define IMSRG_ECONV -5      /* Hartree-Fock did not converge in 10000 iterations */
 
 /* term mask bits for imsrg_commutator (one per reference routine) */
 #define IMSRG_T110 (1u << 0)      /* commutator_110      commutators.f90:10-33   */
@@ -132,6 +133,21 @@ int imsrg_op_e0_batch(imsrg_op op, double *e0);
 int imsrg_op_norm_batch(imsrg_op op, double *out);
 int imsrg_mbpt2_batch(imsrg_op H, double *out);
 int imsrg_normal_order(imsrg_op H);                                 /* normal_order :2375-2506 */
+
+/* calc_HF (HF_mod.f90:14-106) for a two-body Hamiltonian (no three-body force):
+ * H holds the bare Hamiltonian (before normal ordering) and is replaced by the
+ * normal-ordered Hamiltonian in the Hartree-Fock basis: E0 = e_HF (:376-405),
+ * one-body = Fock matrix (transform_1b_to_HF :408-491), two-body transformed
+ * (transform_2b_to_HF :494-619).  coefs (nsp*nsp doubles, may be NULL)
+ * receives D(old orbit a, new orbit i) at coefs[a*nsp + i]; iterations and
+ * e_hf may be NULL.  The self-consistency loop, the (l,j,tz)-block
+ * eigenproblems (diagonalize_blocks, basic_IMSRG.f90:2163-2180) and the
+ * transforms run on the device.  nbatch = 1; groups of at most 16 orbits. */
+int imsrg_hartree_fock(imsrg_op H, double *coefs, int *iterations, double *e_hf);
+/* sqop_to_HF (HF_mod.f90:108-136) for a scalar observable O (bare): the
+ * one- and two-body transforms with the coefficients of imsrg_hartree_fock,
+ * then normal_order. */
+int imsrg_observable_to_hf(imsrg_op O, const double *coefs);
 
 /* ---- commutator engine --------------------------------------------------
  * RES = [L, R] restricted to the terms in mask, in the reference's call

@@ -260,6 +260,141 @@ def mbpt2(A: OOp):
 
 
 # ----------------------------------------------------------------------
+# Hartree-Fock without three-body forces (HF_mod.f90), restated in numpy on the
+# C oracle's f_elem/v_elem.  The block eigenproblems go to numpy.linalg.eigh
+# where the reference calls LAPACK dsyev (basic_IMSRG.f90:2163-2180).
+HF_CONV = float(np.float32(1e-6))  # the reference's `crit > 1e-6` (a default-real literal)
+
+
+def sp_blocks(B):
+    """single-particle blocks of allocate_sp_mat (basic_IMSRG.f90:2600-2660):
+    2tz outer, then l, then 2j; members in orbit order; empty blocks dropped"""
+    out = []
+    for tz in (-1, 1):
+        for l in range(int(B.l.max()) + 1):
+            for j in range(1, B.jtot_max + 1, 2):
+                m = [i for i in range(B.nsp) if B.l[i] == l and B.jj[i] == j and B.tz[i] == tz]
+                if m:
+                    out.append(np.array(m))
+    return out
+
+
+def _write_1b(B, f, groups, Fb):
+    """transform_1b_to_HF's store (HF_mod.f90:461-488): for a <= b in a
+    block, pp and hh take F(a,b) in both orders; a particle-hole entry takes
+    F(particle, hole) of the block matrix"""
+    for g, M in zip(groups, Fb):
+        for x in range(len(g)):
+            for y in range(x, len(g)):
+                a, b = g[x], g[y]
+                if B.con[a] == B.con[b]:
+                    f[a, b] = f[b, a] = M[x, y]
+                elif B.con[a] == 0:
+                    f[a, b] = M[x, y]
+                else:
+                    f[b, a] = M[y, x]
+
+
+def transform_2b(B, mats, D):
+    """transform_2b_to_HF (HF_mod.f90:494-619): per (J,P,Tz) block
+    V' = C^T V C - (-1)^J (C^T V Crev + Crev^T V C) + Crev^T V Crev with
+    C(ab,ij) = D(a,i) D(b,j) sqrt(1+d_ab)/sqrt(1+d_ij) and
+    Crev(ab,ij) = D(b,i) D(a,j) (1-d_ab) (-1)^((ja+jb)/2) sqrt(1+d_ab)/sqrt(1+d_ij)"""
+    out = []
+    for blk, M in zip(B.blocks, mats):
+        pr = np.concatenate(blk.pairs).reshape(-1, 2)
+        if len(pr) == 0:
+            out.append(M)
+            continue
+        a, b = pr[:, 0], pr[:, 1]
+        dab = (a == b).astype(float)
+        nrm = np.sqrt(1.0 + dab)[:, None] / np.sqrt(1.0 + dab)[None, :]
+        C = D[a[:, None], a[None, :]] * D[b[:, None], b[None, :]] * nrm
+        ph = (-1.0) ** ((B.jj[a] + B.jj[b]) // 2)
+        Cr = D[b[:, None], a[None, :]] * D[a[:, None], b[None, :]] * ((1.0 - dab) * ph)[:, None] * nrm
+        s = (-1.0) ** (blk.J // 2)
+        out.append(C.T @ M @ C - s * (C.T @ M @ Cr + Cr.T @ M @ C) + Cr.T @ M @ Cr)
+    return out
+
+
+def hartree_fock(B, Hvec, max_iter=100000):
+    """calc_HF (HF_mod.f90:14-106) on the bare Hamiltonian stream Hvec:
+    returns (stream of H in the normal-ordered HF basis, D[a, i] (old orbit a,
+    new orbit i), E_HF, iterations)"""
+    ob = OBasis(B)
+    H = OOp(ob, Hvec, 1)
+    groups = sp_blocks(B)
+    con = B.con.astype(float)
+    T = [np.array([[H.f(a, b) for b in g] for a in g]) for g in groups]   # write_kin_matrix
+    # gamma_matrix (HF_mod.f90:169-232) is linear in rho:
+    #   gam(a,c) = 1/(ja+1) sum_{blocks, b, d} rho(d,b) sum_J (J+1) v(a,b,c,d;J)
+    # (the reference scales rho by 2j+1 and divides by it again); tabulated once
+    K = {}
+    for p, g in enumerate(groups):
+        ja = int(B.jj[g[0]])
+        for q, h in enumerate(groups):
+            jr = int(B.jj[h[0]])
+            k = np.zeros((len(g), len(g), len(h), len(h)))
+            for x, a in enumerate(g):
+                for y in range(x, len(g)):
+                    c = g[y]
+                    for u, b in enumerate(h):
+                        for w, d in enumerate(h):
+                            k[x, y, u, w] = sum((J + 1.0) * H.v(int(a), int(b), int(c), int(d), J)
+                                                for J in range(abs(jr - ja), jr + ja + 1, 2))
+            K[p, q] = k / (ja + 1.0)
+    D = [np.eye(len(g)) for g in groups]
+    eig = [np.zeros(len(g)) for g in groups]
+    crit, it = 10.0, 0
+    while crit > HF_CONV:
+        rho = []
+        for Dq, g in zip(D, groups):       # density_matrix (HF_mod.f90:346-373)
+            Dx = Dq * con[g][None, :]
+            rho.append(Dx @ Dx.T)
+        Vg = []
+        for p, g in enumerate(groups):
+            m = np.zeros((len(g), len(g)))
+            for q in range(len(groups)):
+                m += np.einsum("xyuw,wu->xy", K[p, q], rho[q])
+            m = np.triu(m) + np.triu(m, 1).T
+            Vg.append(m)
+        F = [t + v for t, v in zip(T, Vg)]
+        crit = 0.0
+        for q in range(len(groups)):
+            w, v = np.linalg.eigh(F[q])
+            crit += np.sqrt(np.sum((eig[q] - w) ** 2))
+            D[q], eig[q] = v, w
+        it += 1
+        if it > max_iter:
+            raise RuntimeError("Hartree-Fock did not converge")
+    for q in range(len(groups)):          # columns with a negative diagonal flip sign
+        D[q] = D[q] * np.where(np.diag(D[q]) < 0.0, -1.0, 1.0)[None, :]
+    Ft = [Dq.T @ Fq @ Dq for Dq, Fq in zip(D, F)]
+    Tt = [Dq.T @ Tq @ Dq for Dq, Tq in zip(D, T)]
+    Vt = [Dq.T @ Vq @ Dq for Dq, Vq in zip(D, Vg)]
+    ehf = 0.0                              # e_HF (HF_mod.f90:376-405)
+    for g, t, v in zip(groups, Tt, Vt):
+        for i in range(len(g)):
+            ehf += (t[i, i] + 0.5 * v[i, i]) * con[g[i]] * (B.jj[g[i]] + 1.0)
+    Dfull = np.zeros((B.nsp, B.nsp))
+    for g, Dq in zip(groups, D):
+        Dfull[np.ix_(g, g)] = Dq
+    _, f, mats = B.full_from_vec(Hvec, 1)
+    _write_1b(B, f, groups, Ft)
+    return B.vec_from_full(ehf, f, transform_2b(B, mats, Dfull)), Dfull, ehf, it
+
+
+def observable_to_hf(B, Ovec, Dfull, herm=1):
+    """sqop_to_HF (HF_mod.f90:108-136) for a scalar observable: one-body
+    D^T f D per block, the two-body transform, then normal_order"""
+    groups = sp_blocks(B)
+    e0, f, mats = B.full_from_vec(Ovec, herm)
+    _write_1b(B, f, groups, [Dfull[np.ix_(g, g)].T @ f[np.ix_(g, g)] @ Dfull[np.ix_(g, g)] for g in groups])
+    v = B.vec_from_full(e0, f, transform_2b(B, mats, Dfull))
+    return normal_order(OOp(OBasis(B), v, herm)).vec()
+
+
+# ----------------------------------------------------------------------
 # the reference-built harness (exists only where oracle/ref.mk was run)
 def ref_available():
     return os.path.exists(REF_HARNESS)
@@ -321,6 +456,18 @@ class RefRunner:
     def gen(self, kind, H):
         self.run("gen", kind, self._w("H.bin", H), "g.bin")
         return self._r("g.bin")
+
+    def hf(self, H, O=None):
+        """calc_HF on the bare H (and sqop_to_HF on O): (H_HF, D, E_HF[, O_HF])"""
+        args = ["hf", self._w("H.bin", H), "hf.bin", "D.bin"]
+        if O is not None:
+            args += [self._w("O.bin", O), "hfo.bin"]
+        out = self.run(*args)
+        e = float([ln for ln in out.splitlines() if ln.startswith("EHF ")][-1].split()[1])
+        n = self.basis.nsp
+        D = self._r("D.bin").reshape(n, n).T  # written column-major: D[a, i]
+        res = (self._r("hf.bin"), D, e)
+        return res + (self._r("hfo.bin"),) if O is not None else res
 
     def normal(self, H):
         self.run("normal", self._w("H.bin", H), "n.bin")

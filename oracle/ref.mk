@@ -79,7 +79,7 @@ $(OUT)/ref_harness: $(HERE)ref_harness.f90 $(OBJS) $(OUT)/harness_basis.o
 
 # the product's Fortran drop-in, compiled against the reference modules, and
 # its test program (reference routines vs drop-in on the same sq_op inputs)
-$(OUT)/imsrg_gpu.o: $(HERE)../eom-imsrg_full_amd/fortran/imsrg_gpu.f90 $(OUT)/cross_coupled.o
+$(OUT)/imsrg_gpu.o: $(HERE)../eom-imsrg_full_amd/fortran/imsrg_gpu.f90 $(OUT)/cross_coupled.o $(OUT)/three_body_routines.o
 	$(FC) $(FFLAGS) -c -o $@ $< -J$(OUT)
 
 $(OUT)/drop_in_test: $(HERE)../tests/fortran/drop_in_test.f90 $(OBJS) $(OUT)/harness_basis.o $(OUT)/imsrg_gpu.o $(LIBDIR)/libimsrg_mi355x.so

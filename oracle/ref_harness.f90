@@ -23,12 +23,16 @@
 !   flow   kind HS.bin maxsteps outprefix -> magnus_decouple loop, full precision
 !   brute  hA A.bin hB B.bin points.txt out.txt -> brute-force J-coupled sums
 !   time   hL L.bin hR R.bin reps        -> wall seconds per full commutator
+!   hf     H.bin out.bin D.bin [O.bin Oout.bin] -> calc_HF (no three-body force) on the bare H;
+!          D(old orbit, new orbit) as an nsp x nsp column-major stream; prints
+!          E_HF; with O, also sqop_to_HF(O) with those coefficients
 program ref_harness
   use basic_IMSRG
   use cross_coupled
   use commutators
   use IMSRG_MAGNUS
   use brute_force_testing
+  use HF_mod
   use harness_basis
   implicit none
 
@@ -75,6 +79,8 @@ program ref_harness
      call do_brute()
   case ('time')
      call do_time()
+  case ('hf')
+     call do_hf()
   case default
      print*, 'unknown command ', trim(cmd)
      stop 2
@@ -289,6 +295,42 @@ contains
     call read_op(trim(sarg(4)), A)
     write(*,'(2es26.17)') mat_frob_norm(A), mbpt2(A, jbas)
   end subroutine do_scal
+
+  subroutine do_hf()
+    ! calc_HF (HF_mod.f90:14-106) with an unallocated three-body force, and
+    ! sqop_to_HF (HF_mod.f90:108-136) for a scalar observable
+    type(sq_op) :: H, O
+    type(three_body_force) :: tb
+    type(full_sp_block_mat) :: coefs
+    real(8), allocatable :: dfull(:,:)
+    integer :: q, i, j, n
+    call duplicate_sq_op(HS, H)
+    H%herm = 1
+    call read_op(trim(sarg(3)), H)
+    call calc_HF(H, tb, jbas, coefs)
+    write(*,'(A,es26.17)') 'EHF ', H%E0
+    call write_op(trim(sarg(4)), H)
+    n = jbas%total_orbits
+    allocate(dfull(n, n))
+    dfull = 0.d0
+    do q = 1, coefs%blocks
+       do i = 1, coefs%map(q)
+          do j = 1, coefs%map(q)
+             dfull(coefs%blkM(q)%states(i), coefs%blkM(q)%states(j)) = coefs%blkM(q)%matrix(i, j)
+          end do
+       end do
+    end do
+    open(unit=43, file=trim(sarg(5)), form='unformatted', access='stream', status='replace')
+    write(43) dfull
+    close(43)
+    if (len_trim(sarg(6)) > 0) then
+       call duplicate_sq_op(HS, O)
+       O%herm = 1
+       call read_op(trim(sarg(6)), O)
+       call sqop_to_HF(O, coefs, jbas)
+       call write_op(trim(sarg(7)), O)
+    end if
+  end subroutine do_hf
 
   subroutine do_flow()
     ! body of magnus_decouple, IMSRG_MAGNUS.f90:15-128, with full-precision output

@@ -2,7 +2,7 @@
 ! own routines on the same sq_op inputs.  Built by oracle/ref.mk into
 ! oracle/_ref/drop_in_test from the reference objects; run by
 ! tests/test_gpu_fortran_drop_in.py on the GPU.
-!   drop_in_test <basis> <L.bin herm -1> <R.bin herm 1> <Hn.bin normal-ordered H>
+!   drop_in_test <basis> <L.bin herm -1> <R.bin herm 1> <Hn.bin normal-ordered H> <H0.bin bare H>
 ! Prints "CHECK <name> <relative error>" lines; exit status 1 on any > 1e-10.
 program drop_in_test
   use basic_IMSRG
@@ -10,18 +10,21 @@ program drop_in_test
   use commutators
   use IMSRG_MAGNUS
   use harness_basis
+  use HF_mod
   use imsrg_gpu
   implicit none
   type(spd) :: jbas
   type(sq_op) :: HS, L, R, H, A, B, w1, w2, ETA1, ETA2, G, DG1, DG2, AD
   type(cc_mat) :: LCC, RCC, WCC, GCC
-  character(len=1024) :: fb, fl, fr, fh
+  character(len=1024) :: fb, fl, fr, fh, fh0
+  type(three_body_force) :: tb
+  type(full_sp_block_mat) :: C1, C2
   integer :: nfail, q
   real(8) :: e1, e2
   real(8), allocatable :: yy(:), yp1(:), yp2(:)
   external :: build_gs_white, build_gs_w2, dHds
 
-  call getarg(1, fb) ; call getarg(2, fl) ; call getarg(3, fr) ; call getarg(4, fh)
+  call getarg(1, fb) ; call getarg(2, fl) ; call getarg(3, fr) ; call getarg(4, fh) ; call getarg(5, fh0)
   checkpointing = .false.
   OUTPUT_DIR = './'
   prefix = 'drop_in_test'
@@ -124,6 +127,24 @@ program drop_in_test
   A%neq = HS%neq ; B%neq = HS%neq
   call cmp('checkpoint_restart_H', A, B)
   call cmp('checkpoint_restart_Omega', ETA1, ETA2)
+
+  ! Hartree-Fock: calc_HF vs gpu_calc_HF on the bare H, then sqop_to_HF vs
+  ! gpu_observable_to_HF on a hermitian observable (R) with each one's coefficients
+  A%herm = 1 ; call load(trim(fh0), A)
+  B%herm = 1 ; call load(trim(fh0), B)
+  call calc_HF(A, tb, jbas, C1)
+  call gpu_calc_HF(B, tb, jbas, C2)
+  call cmp('calc_HF', A, B)
+  e1 = 0.d0
+  do q = 1, C1%blocks
+     if (C1%map(q) > 0) e1 = max(e1, maxval(abs(C1%blkM(q)%matrix - C2%blkM(q)%matrix)))
+  end do
+  call report('calc_HF_coefficients', e1)
+  A%herm = 1 ; call load(trim(fr), A)
+  B%herm = 1 ; call load(trim(fr), B)
+  call sqop_to_HF(A, C1, jbas)
+  call gpu_observable_to_HF(B, C2, jbas)
+  call cmp('sqop_to_HF', A, B)
 
   call gpu_finalize()
   if (nfail > 0) then

@@ -2,7 +2,7 @@
 compiled FP64 routines (oracle/_ref/ref_harness, built by oracle/ref.mk from
 the reference sources).  Run in the build container only:
 
-    make -f oracle/ref.mk && python tests/golden/make_golden.py
+    make -f oracle/ref.mk && python tests/golden/make_golden.py [emax2] [emax4] [hf]
 
 The fixtures hold inputs and outputs (vectorize-order FP64 streams), never
 reference source.  Inputs are stored too, so nothing depends on a random
@@ -104,8 +104,28 @@ def emax4():
     print("emax4_k3.npz", sum(v.nbytes for v in d.values()) / 1e6, "MB")
 
 
+def hartree_fock():
+    """calc_HF on the bare synthetic Hamiltonian and sqop_to_HF on a random
+    hermitian observable, from the reference harness (no three-body force)"""
+    for emax, k, seed_o in ((2, 2, 11), (4, 3, 12)):
+        B = Basis.ho(emax, k)
+        ref = O.RefRunner(B, threads=4)
+        H0 = synthetic_hamiltonian(B, seed=0)
+        Ob = random_operator(B, 1, seed_o)
+        h, D, e, o = ref.hf(H0, Ob)
+        d = dict(H0=H0, O=Ob, H_hf=h, D=D, E_hf=np.array([e]), O_hf=o)
+        name = f"hf_emax{emax}_k{k}.npz"
+        np.savez_compressed(os.path.join(OUT, name), **d)
+        print(name, sum(v.nbytes for v in d.values()) / 1e6, "MB")
+
+
 if __name__ == "__main__":
     if not O.ref_available():
         sys.exit("oracle/_ref/ref_harness is missing: run make -f oracle/ref.mk first")
-    emax2()
-    emax4()
+    what = sys.argv[1:] or ["emax2", "emax4", "hf"]
+    if "emax2" in what:
+        emax2()
+    if "emax4" in what:
+        emax4()
+    if "hf" in what:
+        hartree_fock()

@@ -27,8 +27,9 @@ def test_fortran_drop_in(tmp_path, emax, k):
     random_operator(B, -1, 1).tofile(tmp_path / "L.bin")
     random_operator(B, 1, 2).tofile(tmp_path / "R.bin")
     O.normal_order(ob.op(synthetic_hamiltonian(B, 0), 1)).vec().tofile(tmp_path / "H.bin")
+    synthetic_hamiltonian(B, 0).tofile(tmp_path / "H0.bin")
     print(f"[drop-in] emax={emax}", flush=True)
-    r = subprocess.run([BIN, "basis.txt", "L.bin", "R.bin", "H.bin"], cwd=tmp_path, capture_output=True,
+    r = subprocess.run([BIN, "basis.txt", "L.bin", "R.bin", "H.bin", "H0.bin"], cwd=tmp_path, capture_output=True,
                        text=True, timeout=600)
     checks = [ln.split() for ln in r.stdout.splitlines() if ln.startswith("CHECK")]
     print("\n".join(" ".join(c) for c in checks))
@@ -36,5 +37,6 @@ def test_fortran_drop_in(tmp_path, emax, k):
     names = {c[1] for c in checks}
     assert {"full_commutator", "per_routine_sequence", "cross_coupled", "build_gs_w2", "BCH_EXPAND",
             "MAGNUS_EXPAND", "magnus_decouple_H", "magnus_decouple_Omega", "transform_sq_op_BCH",
-            "dHds", "checkpoint_restart_H", "checkpoint_restart_Omega"} <= names
+            "dHds", "checkpoint_restart_H", "checkpoint_restart_Omega", "calc_HF", "calc_HF_coefficients",
+            "sqop_to_HF"} <= names
     assert all(float(c[2]) <= 1e-10 for c in checks)
