@@ -244,7 +244,7 @@ __global__ __launch_bounds__(256, 2) void k_seg_gemm_i8(const I8Prob *__restrict
   const int4 tl = tiles[t];
   if (tl.x < 0) return;  // padding slot of the XCD-balanced schedule
   const I8Prob &p = probs[tl.x];
-  __shared__ I8Smem S;
+  __shared__ I8Smem S2[2];  // double-buffered chunks: 80 KB, two workgroups fill a CU's 160 KB
   const int M = p.M, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int m0 = tl.y * 64, n0 = tl.z * 64, wm = wave >> 1, wn = wave & 1;
   const int ra = m0 + wm * 32, cb = n0 + wn * 32;
@@ -290,16 +290,23 @@ __global__ __launch_bounds__(256, 2) void k_seg_gemm_i8(const I8Prob *__restrict
       for (int l = 0; l < NS; l++)
 #pragma unroll
         for (int v = 0; v < 16; v++) lv[l][v] = 0;
-      load(0);
-      for (int kc = 0; kc < K; kc += 64) {  // K is workgroup-uniform: every barrier is reached by all
-        __syncthreads();  // the previous chunk has been read
+      auto store = [&](I8Smem &S) {
 #pragma unroll
         for (int q = 0; q < NS; q++) {
           (&S.a[0][0][0])[sdst[q]] = R[0][q];
           (&S.b[0][0][0])[sdst[q]] = R[1][q];
         }
-        __syncthreads();
-        if (kc + 64 < K) load(kc + 64);
+      };
+      // chunk kc is read from buffer (kc / 64) % 2 while chunk kc + 64 is
+      // stored into the other one; one barrier per chunk.  Every segment ends
+      // on a barrier, so both buffers are free when the next one starts.
+      load(0);
+      store(S2[0]);
+      if (64 < K) load(64);
+      __syncthreads();
+      int buf = 0;
+      for (int kc = 0; kc < K; kc += 64) {  // K is workgroup-uniform: every barrier is reached by all
+        const I8Smem &S = S2[buf];
         if (active) {
           const int nh = kc + 32 < sgm.K32 ? 2 : 1;  // workgroup-uniform
 #pragma unroll
@@ -318,6 +325,12 @@ __global__ __launch_bounds__(256, 2) void k_seg_gemm_i8(const I8Prob *__restrict
             }
           }
         }
+        if (kc + 64 < K) {
+          store(S2[buf ^ 1]);
+          if (kc + 128 < K) load(kc + 128);
+        }
+        __syncthreads();
+        buf ^= 1;
       }
       if (active) {
         const double *SA = (o ? S1 : S0) + p.s_off, *SB = (o ? S0 : S1) + p.s_off;
