@@ -433,7 +433,7 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
     UP(ub, ph_ubase);
   }
   {
-    std::vector<double> wr(Ly.npairs), wt((Ly.ph_terms.size() + 255) / 256, 1.0);
+    std::vector<double> wr(Ly.npairs), wt((Ly.ph_terms.size() + 1023) / 1024, 1.0);
     for (int r = 0; r < Ly.npairs; r++)
       wr[r] = Ly.bdim[Ly.pair_q[r]] * (double)(Ly.nb_start[r + 1] - Ly.nb_start[r] + 1);
     D.xs_rows = make_split(wr);

@@ -1044,33 +1044,52 @@ void launch_clear_blocks(hipStream_t s, const DevLayout &L, double *RES, const i
 // Z is stored in 2x2 blocks, upper blocks only: Z(R1,C1) and Z(R2,C2), where
 // (R1,R2) and (C1,C2) are the two orders of one orbit pair each, lie in one
 // block (R1/2, C1/2), or in its mirror (C1/2, R1/2) times sigma.
-// One thread per J' term: T = z1 + ph*sigma*z2 (side 0) or sigma*z1 + ph*z2
-// (side 1), both Z elements read with one 32-byte load of the block the
-// host-built address table names.
+// Four J' terms per thread (t + 256 u of a chunk of 1024): T = z1 + ph*sigma*z2
+// (side 0) or sigma*z1 + ph*z2 (side 1), both Z elements read with one
+// 32-byte load of the block the host-built address table names; the four
+// terms' table reads, then their four block reads, are in flight together.
 __global__ __launch_bounds__(256) void k_ph_terms(DevLayout L, const double *__restrict__ Z,
                                                   double *__restrict__ Tbuf, double sigma, const int *act) {
   const int chunk = xcd_item(L.xs_terms);
-  const long long t = (long long)chunk * 256 + threadIdx.x;
-  if (chunk < 0 || t >= L.nphterm) return;
+  if (chunk < 0) return;
   {
     const long long inst = inst_id(act);
     if (inst < 0) return;
     Z += inst * L.cwsize;
     Tbuf += inst * L.nphterm;
   }
-  const unsigned sel = L.ph_zsel[t];
-  double v = 0.0;
-  if (sel & 128) {
-    const double4_t b = reinterpret_cast<const double4_t *>(Z)[L.ph_zblk[t]];
-    double z1 = b[sel & 3], z2 = b[(sel >> 2) & 3];
-    if (sel & 16) {
-      z1 = sigma * z1;
-      z2 = sigma * z2;
+  const long long t0 = (long long)chunk * 1024 + threadIdx.x;
+  unsigned sel[4], blk[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const long long t = t0 + 256 * u;
+    sel[u] = 0;
+    blk[u] = 0;
+    if (t < L.nphterm) {
+      sel[u] = L.ph_zsel[t];
+      blk[u] = L.ph_zblk[t];
     }
-    const double ph = (sel & 32) ? -1.0 : 1.0;
-    v = (sel & 64) ? sigma * z1 + ph * z2 : z1 + ph * sigma * z2;
   }
-  Tbuf[t] = v;
+  double4_t b[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+    if (sel[u] & 128) b[u] = reinterpret_cast<const double4_t *>(Z)[blk[u]];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const long long t = t0 + 256 * u;
+    if (t >= L.nphterm) break;
+    double v = 0.0;
+    if (sel[u] & 128) {
+      double z1 = b[u][sel[u] & 3], z2 = b[u][(sel[u] >> 2) & 3];
+      if (sel[u] & 16) {
+        z1 = sigma * z1;
+        z2 = sigma * z2;
+      }
+      const double ph = (sel[u] & 32) ? -1.0 : 1.0;
+      v = (sel[u] & 64) ? sigma * z1 + ph * z2 : z1 + ph * sigma * z2;
+    }
+    Tbuf[t] = v;
+  }
 }
 
 // The 6j symbols a row can use, {ji jj J; x y J'} with triangle (x, y, J) and
