@@ -143,6 +143,22 @@ def cpu_baseline(mode, basis, Hn_vec, eta_vec, comm_per_step, reps):
                       f"{sec:.3f} s each, x {comm_per_step:.2f} commutators per Magnus step"}
 
 
+def hf_cpu_baseline(mode, basis, bare_vec):
+    """The reference's calc_HF (oracle/_ref ref_harness hf: MKL, OpenMP) on the
+    same bare Hamiltonian, wall seconds including the harness's basis set-up and
+    file I/O; None without the reference build"""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    if mode not in ("auto", "reference") or not O.ref_available():
+        return None
+    threads = min(16, os.cpu_count() or 1)
+    ref = O.RefRunner(basis, threads=threads)
+    t0 = time.perf_counter()
+    ref.hf(bare_vec)
+    return {"seconds": time.perf_counter() - t0, "cores": threads, "kind": "reference",
+            "note": "ref_harness hf wall time, including its basis set-up and file I/O"}
+
+
 def basis_tag(basis):
     return int(max(2 * basis.n + basis.l))
 
@@ -473,6 +489,10 @@ def main():
     if a.cpu_baseline != "none":
         log("timing the CPU baseline sample")
         cpu = cpu_baseline(a.cpu_baseline, basis, H0vec, eta_vec, comm_per_step, a.cpu_reps)
+        if hf is not None:
+            hf["cpu_baseline"] = hf_cpu_baseline(a.cpu_baseline, basis, bare)
+            if hf["cpu_baseline"]:
+                log(f"  reference calc_HF on the CPU: {hf['cpu_baseline']['seconds']:.2f} s")
     out = {
         "metric": "Magnus BCH steps/s at emax=8: split-FP32 vs FP64 GPU at |dE0|<1e-6 MeV",
         "value": steps_per_s,
