@@ -6,7 +6,9 @@
  * IMSRG_MAGNUS.f90).  Operators are device-resident; they cross the boundary
  * only as FP64 streams in the reference's own vectorize order
  * (basic_IMSRG.f90:3794-3854), so a Fortran caller moves an sq_op with
- * vectorize/repackage and an ISO_C_BINDING interface.
+ * vectorize/repackage and an ISO_C_BINDING interface (the module
+ * eom-imsrg_full_amd/fortran/imsrg_gpu.f90 holds one for every entry point
+ * the reference's call sites need).
  *
  * Conventions: angular momenta doubled (2j, 2tz); orbits 0-based on the C side;
  * every function returns 0 on success and a negative IMSRG_E* code on failure,
