@@ -226,8 +226,11 @@ void launch_gather_e0(hipStream_t s, const double *op, long long stride, double 
 constexpr int kHFMaxGroup = 16;
 void launch_hf_init(hipStream_t s, const DevLayout &L, const double *op, double *D, double *Tm, double *eig,
                     int reset);
+// one self-consistency iteration; state = {converged, iterations}: every kernel
+// returns at once when converged, the last one counts and tests crit > conv
 void launch_hf_iteration(hipStream_t s, const DevLayout &L, const double *op, double *D, double *rho,
-                         const double *Tm, double *Vg, double *Fm, double *eig, double *cpart, double *crit);
+                         const double *Tm, double *Vg, double *Fm, double *eig, double *cpart, double *crit,
+                         double *state, double conv);
 void launch_hf_finish_1b(hipStream_t s, const DevLayout &L, double *D, const double *M, double *op, int herm,
                          const double *Tm, const double *Vg, double *tv, double *ehf);
 void launch_hf_2b(hipStream_t s, const DevLayout &L, const double *D, double *op, double *W, int herm);

@@ -28,9 +28,12 @@ __global__ void k_hf_init(DevLayout L, const double *__restrict__ op, double *__
 
 // density_matrix (HF_mod.f90:346-373) without the (2j+1) that gamma_matrix
 // divides out again: rho(b,d) = sum over hole orbits i of the group of D(b,i) D(d,i)
-__global__ void k_hf_density(DevLayout L, const double *__restrict__ D, double *__restrict__ rho) {
+// state[0] != 0: the loop has converged and every kernel of a further
+// iteration returns at once (the host enqueues iterations in groups)
+__global__ void k_hf_density(DevLayout L, const double *__restrict__ D, double *__restrict__ rho,
+                             const double *__restrict__ state) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= L.n_hf_bd) return;
+  if (t >= L.n_hf_bd || state[0] != 0.0) return;
   const int b = L.hf_b[t], d = L.hf_d[t], N = L.nsp, g = L.grp_of[b];
   const int *mem = L.grp_list + L.grp_start[g];
   double s = 0.0;
@@ -47,10 +50,11 @@ __global__ void k_hf_density(DevLayout L, const double *__restrict__ D, double *
 // T + gam is written with it (both triangles)
 __global__ __launch_bounds__(256) void k_hf_gamma(DevLayout L, const double *__restrict__ op,
                                                   const double *__restrict__ rho, const double *__restrict__ Tm,
-                                                  double *__restrict__ Vg, double *__restrict__ Fm) {
+                                                  double *__restrict__ Vg, double *__restrict__ Fm,
+                                                  const double *__restrict__ state) {
   __shared__ double sh[256];
   const int t = blockIdx.x;
-  if (t >= L.n_hf_ac) return;
+  if (t >= L.n_hf_ac || state[0] != 0.0) return;
   const int a = L.hf_a[t], c = L.hf_c[t], ja = L.jj[a], N = L.nsp;
   const double *V = op + L.v_off;
   double s = 0.0;
@@ -77,11 +81,12 @@ __global__ __launch_bounds__(256) void k_hf_gamma(DevLayout L, const double *__r
 // ascending with their vectors.  Rotations run one after another; lane k
 // updates row / column k of each.  cpart[g] = |eig_old - eig_new|_2 of the group.
 __global__ __launch_bounds__(64) void k_hf_diag(DevLayout L, const double *__restrict__ Fm, double *__restrict__ D,
-                                                double *__restrict__ eig, double *__restrict__ cpart) {
+                                                double *__restrict__ eig, double *__restrict__ cpart,
+                                                const double *__restrict__ state) {
   __shared__ double A[kHFMaxGroup][kHFMaxGroup], Q[kHFMaxGroup][kHFMaxGroup];
   __shared__ double off_s;
   const int g = blockIdx.x;
-  if (g >= L.ngrp) return;
+  if (g >= L.ngrp || state[0] != 0.0) return;
   const int m = L.grp_cnt[g], N = L.nsp, k = threadIdx.x;
   const int *mem = L.grp_list + L.grp_start[g];
   if (k < m)
@@ -166,12 +171,17 @@ __global__ __launch_bounds__(64) void k_hf_diag(DevLayout L, const double *__res
   cpart[g] = sqrt(cr);
 }
 
-// crit = sum of the groups' eigenvalue changes, in the reference's block order
-__global__ void k_hf_crit(DevLayout L, const double *__restrict__ cpart, double *__restrict__ crit) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// crit = sum of the groups' eigenvalue changes, in the reference's block
+// order; counts the iteration (state[1]) and ends the loop (state[0] = 1)
+// once crit > conv fails, as the reference's `do while (crit > 1e-6)`
+__global__ void k_hf_crit(DevLayout L, const double *__restrict__ cpart, double *__restrict__ crit,
+                          double *__restrict__ state, double conv) {
+  if (blockIdx.x != 0 || threadIdx.x != 0 || state[0] != 0.0) return;
   double s = 0.0;
   for (int k = 0; k < L.ngrp; k++) s += cpart[L.hf_gorder[k]];
   crit[0] = s;
+  state[1] += 1.0;
+  if (!(s > conv)) state[0] = 1.0;
 }
 
 // eigenvectors with a negative diagonal component change sign (HF_mod.f90:84-89)
@@ -319,11 +329,12 @@ void launch_hf_init(hipStream_t s, const DevLayout &L, const double *op, double 
   hipLaunchKernelGGL(k_hf_init, dim3(nblk(L.n_hf_bd, 256)), dim3(256), 0, s, L, op, D, Tm, eig, reset);
 }
 void launch_hf_iteration(hipStream_t s, const DevLayout &L, const double *op, double *D, double *rho,
-                         const double *Tm, double *Vg, double *Fm, double *eig, double *cpart, double *crit) {
-  hipLaunchKernelGGL(k_hf_density, dim3(nblk(L.n_hf_bd, 256)), dim3(256), 0, s, L, D, rho);
-  hipLaunchKernelGGL(k_hf_gamma, dim3(L.n_hf_ac), dim3(256), 0, s, L, op, rho, Tm, Vg, Fm);
-  hipLaunchKernelGGL(k_hf_diag, dim3(L.ngrp), dim3(64), 0, s, L, Fm, D, eig, cpart);
-  hipLaunchKernelGGL(k_hf_crit, dim3(1), dim3(64), 0, s, L, cpart, crit);
+                         const double *Tm, double *Vg, double *Fm, double *eig, double *cpart, double *crit,
+                         double *state, double conv) {
+  hipLaunchKernelGGL(k_hf_density, dim3(nblk(L.n_hf_bd, 256)), dim3(256), 0, s, L, D, rho, state);
+  hipLaunchKernelGGL(k_hf_gamma, dim3(L.n_hf_ac), dim3(256), 0, s, L, op, rho, Tm, Vg, Fm, state);
+  hipLaunchKernelGGL(k_hf_diag, dim3(L.ngrp), dim3(64), 0, s, L, Fm, D, eig, cpart, state);
+  hipLaunchKernelGGL(k_hf_crit, dim3(1), dim3(64), 0, s, L, cpart, crit, state, conv);
 }
 void launch_hf_finish_1b(hipStream_t s, const DevLayout &L, double *D, const double *M, double *op, int herm,
                          const double *Tm, const double *Vg, double *tv, double *ehf) {
