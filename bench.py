@@ -7,18 +7,20 @@ emax=8 workload (90 HO orbits, 40Ca-like closed shell: lowest 6 orbits per
 isospin filled, hw = 20 MeV, Gaussian one-/two-body matrix elements, seed 0,
 normal-ordered on the device).
 
-1. Verification (untimed): an FP64 flow and a flow of every split-precision
-   variant run in lockstep from s = 0 for W+K steps; per step |dE0| and
-   ||dOmega||_F of the split flow against the FP64 flow.
+1. Verification (untimed): the FP64 flow and the flow of every reduced-
+   precision variant run in lockstep from s = 0 until both have converged
+   (|MBPT2| < 1e-6, the reference's stop test); per step |dE0| and
+   ||dOmega||_F of the variant against the FP64 flow.  The FP64 flow is the
+   one tests/test_gpu_emax8_oracle.py pins against the oracle (its first two
+   steps at 1e-10 relative, every term of the commutator at this size).
 2. Timing, per precision: a fresh flow from s = 0, W untimed warmup steps,
    then exactly K steps of the same flow timed between two stream
    synchronizations.  Inputs are resident in HBM when the timer starts.
 
-The headline value is the steps/s of the fastest variant that meets the
-|dE0| < 1e-6 MeV bar: the FP64 path (which matches the FP64 oracle to 1e-10
-relative in the parity tests) or a split variant whose lockstep |dE0| stays
-below 1e-6 MeV.  Every variant is reported with its speed and error.
-Prints ONE JSON line on stdout.
+The headline value is the steps/s of the fastest variant whose |dE0| stays
+below 1e-6 MeV over the whole flow: the FP64 path itself or a reduced-
+precision variant.  Every variant is reported with its speed and error
+(top-level "precision_gate").  Prints ONE JSON line on stdout.
 """
 import argparse
 import json
@@ -57,11 +59,33 @@ def parse():
                    help="timed Hartree-Fock solutions of the bare workload Hamiltonian (0: skip)")
     p.add_argument("--batch-emax", type=int, default=6)
     p.add_argument("--batch-holes", type=int, default=3)
+    p.add_argument("--batch8", type=int, default=4,
+                   help="instances of the batched emax=8 measurement, FP64 and int8x5 (0: skip)")
+    p.add_argument("--gate-steps", type=int, default=200,
+                   help="at most this many lockstep steps in the precision gate (the flow converges earlier)")
     p.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                    help="print the calibrated HBM traffic table from two rocprofv3 --pmc runs and exit")
     p.add_argument("--trace-summary", metavar="KERNEL_TRACE_CSV",
                    help="print the per-kernel table of a rocprofv3 --kernel-trace run and exit")
     return p.parse_args()
+
+
+def z_upper_bytes(basis):
+    """bytes of Z = hL*W - hR*W^T that the Pandya product writes: per Pandya
+    block with ph columns, the upper 2x2 blocks of its M x M matrix (M = the
+    reference's rows plus one zero row when the count of (a,a) rows is odd)"""
+    from imsrg_mi355x.basis import pandya_blocks
+    jtm = basis.jtot_max
+    out = 0
+    for q, (JC, r, nb) in enumerate(pandya_blocks(basis)):
+        if r == 0 or nb == 0:
+            continue
+        par = q // (2 * jtm + 2)
+        tz = (q % (2 * jtm + 2)) // (jtm + 1)
+        same = sum(1 for a in range(basis.nsp) if JC <= 2 * basis.jj[a]) if (par == 0 and tz == 0) else 0
+        m = (r + (same & 1)) // 2
+        out += 32 * m * (m + 1) // 2
+    return float(out)
 
 
 def kernel_work(info, basis):
@@ -71,7 +95,7 @@ def kernel_work(info, basis):
     full = float((dims * dims).sum()) * 8.0      # all channel blocks of one operator
     upper = info["nupper"] * 8.0                  # their upper triangles
     cc = info["ccsize"] * 8.0                     # Pandya matrices of one operator
-    z = info["cwsize"] * 8.0                      # Z = hL*W - hR*W^T, all Pandya blocks
+    z = z_upper_bytes(basis)                      # Z, the upper 2x2 blocks the product writes
     t = info["nphterm"] * 8.0                     # ph recoupling J' terms
     # int8 slicing: the pp/hh columns of one operator's V rows, and one Pandya
     # matrix, read in FP64 and written as 5 int8 planes; one span of each per
@@ -96,12 +120,19 @@ def roofline_of(cls, work, prof, head):
     bound, per_launch = work[cls]
     ms, n = prof[cls]
     avg_s = ms / 1e3 / n
+    extra = {}
     if bound == "mfma" and head == "int8x5":
-        # fifteen int8 slice products per FP64 multiply-add, against the int8 dense peak
-        # (the span also holds the slicing of the operand that changes every commutator)
-        achieved = 15 * per_launch / avg_s / 1e12
+        # the algorithmic FP64 flop of the product against the int8 dense peak;
+        # beside it the same rate against the FP64 MFMA peak, and the int8
+        # slice products actually issued (15 per FP64 multiply-add, tile
+        # padding excluded) against the int8 peak
+        achieved = per_launch / avg_s / 1e12
         peak = PEAK["int8_mfma_tops"]
         unit = "TOP/s"
+        extra = {"fp64_equivalent": {"achieved_tflops": achieved, "peak_tflops": PEAK["fp64_mfma_tflops"],
+                                     "frac": achieved / PEAK["fp64_mfma_tflops"]},
+                 "int8_slice_ops": {"per_launch": 15 * per_launch, "achieved_tops": 15 * achieved,
+                                    "frac_of_int8_peak": 15 * achieved / PEAK["int8_mfma_tops"]}}
     elif bound == "mfma":
         achieved = per_launch / avg_s / 1e12
         peak = {"fp64": PEAK["fp64_mfma_tflops"], "split32": PEAK["fp32_mfma_tflops"] / 3,
@@ -111,8 +142,32 @@ def roofline_of(cls, work, prof, head):
         achieved = per_launch / avg_s / 1e9
         peak = PEAK["hbm_gbs"]
         unit = "GB/s"
-    return {"kernel": cls, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
-            "frac": achieved / peak, "per_launch": per_launch, "avg_launch_ms": avg_s * 1e3, "launches": n}
+    out = {"kernel": cls, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+           "frac": achieved / peak, "per_launch": per_launch, "avg_launch_ms": avg_s * 1e3, "launches": n}
+    out.update(extra)
+    return out
+
+
+def host_cpu():
+    """threads the CPU baselines use, and what the host reports about itself:
+    OMP_NUM_THREADS when set (the box's CPU share), else every core the
+    process may run on"""
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(env) if env.isdigit() and int(env) > 0 else allowed
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return threads, {"nproc": os.cpu_count(), "affinity": allowed, "omp_num_threads": env or None,
+                     "cpu_model": model}
 
 
 def cpu_baseline(mode, basis, Hn_vec, eta_vec, comm_per_step, reps):
@@ -121,7 +176,7 @@ def cpu_baseline(mode, basis, Hn_vec, eta_vec, comm_per_step, reps):
     steps/s with the measured commutators per step."""
     sys.path.insert(0, ROOT)
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    threads, hostinfo = host_cpu()
     kind = None
     if mode in ("auto", "reference") and O.ref_available():
         kind = "reference"
@@ -140,7 +195,9 @@ def cpu_baseline(mode, basis, Hn_vec, eta_vec, comm_per_step, reps):
         return None
     return {"value": 1.0 / (sec * comm_per_step), "unit": "steps/s", "cores": threads, "kind": kind,
             "sample": f"{reps} emax={basis_tag(basis)} commutators [eta,H] on {threads} host threads, "
-                      f"{sec:.3f} s each, x {comm_per_step:.2f} commutators per Magnus step"}
+                      f"{sec:.3f} s each, x {comm_per_step:.2f} commutators per Magnus step (the GPU "
+                      f"flow's measured mean over the timed steps)",
+            "seconds_per_commutator": sec, "host": hostinfo}
 
 
 def hf_cpu_baseline(mode, basis, bare_vec):
@@ -151,7 +208,7 @@ def hf_cpu_baseline(mode, basis, bare_vec):
     from oracle import oracle as O
     if mode not in ("auto", "reference") or not O.ref_available():
         return None
-    threads = min(16, os.cpu_count() or 1)
+    threads, _ = host_cpu()
     ref = O.RefRunner(basis, threads=threads)
     t0 = time.perf_counter()
     ref.hf(bare_vec)
@@ -301,20 +358,29 @@ def timed_flow(ctx, E, H0vec, a, prec, classes):
 
 
 def verify(ctx, E, H0vec, a, prec):
-    """lockstep FP64 vs `prec` flows: per step |dE0| and ||dOmega||_F"""
+    """The precision gate: the FP64 flow and the `prec` flow in lockstep from
+    s = 0 until both have converged (or a.gate_steps); per step |dE0| and
+    ||dOmega||_F of the `prec` flow against the FP64 flow, and whether both
+    used the same Magnus/BCH term counts."""
     HA, HB = ctx.op(1, H0vec), ctx.op(1, H0vec)
     ctx.set_precision(0)
     fa = ctx.flow(HA, a.generator, 0.5)
     ctx.set_precision(PRECS[prec])
     fb = ctx.flow(HB, a.generator, 0.5)
     out = []
-    for _ in range(a.warmup + a.steps):
-        ctx.set_precision(0)
-        ra = fa.step()
-        ctx.set_precision(PRECS[prec])
-        rb = fb.step()
+    ra = rb = None
+    for _ in range(a.gate_steps):
+        if fa.converged and fb.converged:
+            break
+        if not fa.converged:
+            ctx.set_precision(0)
+            ra = fa.step()
+        if not fb.converged:
+            ctx.set_precision(PRECS[prec])
+            rb = fb.step()
         dO = float(np.linalg.norm(fa.G.download() - fb.G.download()))
-        out.append({"s": float(ra["s"]), "dE0": float(abs(ra["e0"] - rb["e0"])), "dOmega_F": dO,
+        out.append({"s": float(ra["s"]), "s_prec": float(rb["s"]), "dE0": float(abs(ra["e0"] - rb["e0"])),
+                    "dOmega_F": dO, "crit": float(ra["crit"]),
                     "same_terms": bool(ra["n_magnus"] == rb["n_magnus"] and ra["n_bch"] == rb["n_bch"])})
     fa.close()
     fb.close()
@@ -322,10 +388,22 @@ def verify(ctx, E, H0vec, a, prec):
     return out
 
 
+def gate_summary(checks):
+    """compact per-variant gate record"""
+    return {"max_abs_dE0_MeV": float(max(c["dE0"] for c in checks)),
+            "final_abs_dE0_MeV": float(checks[-1]["dE0"]),
+            "max_dOmega_F": float(max(c["dOmega_F"] for c in checks)),
+            "lockstep_steps": len(checks),
+            "converged_at_s": [checks[-1]["s"], checks[-1]["s_prec"]],
+            "same_term_counts_every_step": bool(all(c["same_terms"] for c in checks)),
+            "passes_1e-6_MeV": bool(max(c["dE0"] for c in checks) < 1e-6)}
+
+
 def hf_bench(ctx, bare_vec, reps):
     """imsrg_hartree_fock on the bare workload Hamiltonian: wall time of the
-    whole call (self-consistent loop with one host read of the convergence
-    test per iteration, then the one- and two-body transforms)"""
+    whole call (self-consistent loop enqueued eight iterations at a time, the
+    convergence test evaluated on the device and read by the host once per
+    eight iterations, then the one- and two-body transforms)"""
     ts = []
     for r in range(reps + 1):  # the first call also allocates the workspace
         H = ctx.op(1, bare_vec)
@@ -337,32 +415,37 @@ def hf_bench(ctx, bare_vec, reps):
 
 
 def batch_bench(a, E, Basis, batch_hamiltonians):
-    """Config-3 style measurement: `a.batch` independent Hamiltonians (seeds
-    0.., hw in [16,24] MeV, sigma2 in [0.5,1.5] MeV) on one batched context,
-    every kernel launched once for all active instances, against the same
-    instances flowed one after another on a single-instance context.  Both
-    run W untimed + K timed Magnus steps per instance."""
+    """The batching workload: `a.batch` independent Hamiltonians (seeds 0..,
+    hw in [16,24] MeV, sigma2 in [0.5,1.5] MeV) on one batched context, every
+    kernel launched once for all active instances and every commutator
+    replayed from a hipGraph, against the same instances flowed one after
+    another on a single-instance context (eager).  Both run W untimed + K
+    timed Magnus steps per instance."""
     basis = Basis.ho(a.batch_emax, a.batch_holes)
     H = batch_hamiltonians(basis, range(a.batch))
     out = {"workload": f"{a.batch} Hamiltonians emax={a.batch_emax}, {a.batch_holes} holes/isospin, "
                        f"seeds 0-{a.batch - 1}, hw U[16,24], sigma2 U[0.5,1.5], {a.generator} generator, ds=0.5",
-           "orbits": basis.nsp, "neq": basis.neq}
+           "orbits": basis.nsp, "neq": basis.neq, "graphs": True}
     bctx = E.Context(basis, nbatch=a.batch)
+    bctx.set_graphs(True)
     sctx = E.Context(basis)
     for prec in ("fp64", "split32", "int8x5"):
         bctx.set_precision(PRECS[prec])
         sctx.set_precision(PRECS[prec])
         bH = bctx.op(1, H).normal_order()
         f = bctx.flow(bH, a.generator, 0.5)
+        g0 = bctx.graph_stats()
         for _ in range(a.warmup):
             f.step()
         bctx.sync()
+        g1 = bctx.graph_stats()
         t0 = time.perf_counter()
         nsteps = 0
         for _ in range(a.steps):
             nsteps += sum(r["stepped"] for r in f.step())
         bctx.sync()
         tb = time.perf_counter() - t0
+        g2 = bctx.graph_stats()
         e0b = np.array(bH.e0)
         f.close()
         ts, nseq, e0s = 0.0, 0, []
@@ -384,13 +467,55 @@ def batch_bench(a, E, Basis, batch_hamiltonians):
         same = bool(np.array_equal(e0b, np.array(e0s)))
         out[prec] = {"instance_steps_per_s": nsteps / tb, "sequential_instance_steps_per_s": nseq / ts,
                      "speedup_vs_sequential": (nsteps / tb) / (nseq / ts), "ms_per_batched_step": 1e3 * tb / a.steps,
-                     "E0_identical_to_sequential": same}
-        log(f"batch {prec}: {nsteps / tb:.2f} instance-steps/s batched vs {nseq / ts:.2f} sequential "
-            f"(x{(nsteps / tb) / (nseq / ts):.2f}); E0 identical: {same}")
+                     "E0_identical_to_sequential": same,
+                     "graph_builds_warmup": g1[0] - g0[0], "graph_builds_timed": g2[0] - g1[0],
+                     "graph_replays_timed": g2[1] - g1[1]}
+        log(f"batch {prec}: {nsteps / tb:.2f} instance-steps/s batched (graphs: {g2[0] - g1[0]} builds, "
+            f"{g2[1] - g1[1]} replays timed) vs {nseq / ts:.2f} sequential (x{(nsteps / tb) / (nseq / ts):.2f}); "
+            f"E0 identical: {same}")
     out["split32_speedup_vs_fp64"] = out["split32"]["instance_steps_per_s"] / out["fp64"]["instance_steps_per_s"]
     out["int8x5_speedup_vs_fp64"] = out["int8x5"]["instance_steps_per_s"] / out["fp64"]["instance_steps_per_s"]
     bctx.close()
     sctx.close()
+    return out
+
+
+def batch8_bench(a, E, basis, H0vec, single_fp64_steps_per_s):
+    """The precision workload batched: a.batch8 copies of the emax=8 flow
+    state at s = 0 (the benchmark Hamiltonian and a.batch8 - 1 others, seeds
+    1.., hw 20 MeV), flowed together, FP64 and int8x5, with W untimed + K
+    timed steps; instance-steps/s against the single-instance FP64 flow."""
+    from imsrg_mi355x import synthetic_hamiltonian
+    nb = a.batch8
+    vecs = [H0vec]
+    tmp = E.Context(basis)
+    for k in range(1, nb):
+        h = tmp.op(1, synthetic_hamiltonian(basis, seed=k, hw=20.0, sigma1=0.5, sigma2=1.0))
+        h.normal_order()
+        vecs.append(h.download())
+    tmp.close()
+    bctx = E.Context(basis, nbatch=nb)
+    out = {"workload": f"{nb} emax={a.emax} Hamiltonians (seeds 0-{nb - 1}, hw 20), {a.generator} generator",
+           "instances": nb}
+    for prec in ("fp64", "int8x5"):
+        bctx.set_precision(PRECS[prec])
+        bH = bctx.op(1, np.stack(vecs))
+        f = bctx.flow(bH, a.generator, 0.5)
+        for _ in range(a.warmup):
+            f.step()
+        bctx.sync()
+        t0 = time.perf_counter()
+        n = 0
+        for _ in range(a.steps):
+            n += sum(r["stepped"] for r in f.step())
+        bctx.sync()
+        tb = time.perf_counter() - t0
+        f.close()
+        out[prec] = {"instance_steps_per_s": n / tb, "ms_per_batched_step": 1e3 * tb / a.steps,
+                     "vs_single_instance_fp64": (n / tb) / single_fp64_steps_per_s}
+        log(f"batch8 {prec}: {n / tb:.2f} instance-steps/s ({nb} instances), "
+            f"x{(n / tb) / single_fp64_steps_per_s:.2f} vs single-instance FP64")
+    bctx.close()
     return out
 
 
@@ -426,10 +551,11 @@ def main():
     for prec in precs:
         if prec == "fp64":
             continue
-        log(f"verification: {prec} vs fp64, {a.warmup + a.steps} lockstep steps")
+        log(f"precision gate: {prec} vs fp64 in lockstep until both converge")
         checks[prec] = verify(ctx, E, H0vec, a, prec)
         for c in checks[prec]:
-            log(f"  s={c['s']} |dE0|={c['dE0']:.3e} MeV ||dOmega||_F={c['dOmega_F']:.3e} same terms={c['same_terms']}")
+            log(f"  s={c['s']} |dE0|={c['dE0']:.3e} MeV ||dOmega||_F={c['dOmega_F']:.3e} crit={c['crit']:.2e} "
+                f"same terms={c['same_terms']}")
     runs = {}
     work = kernel_work(info, basis)
     for prec in precs:
@@ -444,13 +570,18 @@ def main():
     for prec, (recs, elapsed, prof, breakdown) in runs.items():
         v = {"steps_per_s": a.steps / elapsed, "ms_per_step": 1e3 * elapsed / a.steps}
         if prec in checks:
-            v["max_abs_dE0_MeV"] = float(max(c["dE0"] for c in checks[prec]))
-            v["max_dOmega_F"] = float(max(c["dOmega_F"] for c in checks[prec]))
-            v["meets_1e-6_MeV"] = bool(v["max_abs_dE0_MeV"] < 1e-6)
+            v.update(gate_summary(checks[prec]))
         variants[prec] = v
+    gate = {p: {k: variants[p][k] for k in ("max_abs_dE0_MeV", "lockstep_steps", "passes_1e-6_MeV")}
+            for p in variants if p in checks}
     if "fp64" in variants:
-        variants["fp64"]["meets_1e-6_MeV"] = True  # it is the FP64 reference path
-    ok = [p for p in variants if variants[p].get("meets_1e-6_MeV")]
+        # the reference of the gate: the FP64 flow itself, pinned to the oracle
+        # (first two steps and every commutator term at emax=8, 1e-10 relative)
+        # by tests/test_gpu_emax8_oracle.py
+        variants["fp64"]["passes_1e-6_MeV"] = True
+        gate["fp64"] = {"max_abs_dE0_MeV": 0.0, "passes_1e-6_MeV": True,
+                        "note": "gate reference; matches the FP64 oracle to 1e-10 relative in the parity tests"}
+    ok = [p for p in variants if variants[p].get("passes_1e-6_MeV")]
     head = max(ok, key=lambda p: variants[p]["steps_per_s"]) if ok else precs[0]
     for p in variants:
         if "fp64" in variants:
@@ -485,6 +616,10 @@ def main():
     if a.batch > 0:
         log(f"batched flows: {a.batch} instances at emax={a.batch_emax}")
         batch = batch_bench(a, E, Basis, batch_hamiltonians)
+    batch8 = None
+    if a.batch8 > 0 and "fp64" in variants:
+        log(f"batched flows: {a.batch8} instances at emax={a.emax}")
+        batch8 = batch8_bench(a, E, basis, H0vec, variants["fp64"]["steps_per_s"])
     cpu = None
     if a.cpu_baseline != "none":
         log("timing the CPU baseline sample")
@@ -516,8 +651,10 @@ def main():
             "speedup_vs_fp64": (steps_per_s / variants["fp64"]["steps_per_s"]) if "fp64" in variants else None,
             "fp64_tile_shapes": ctx.tuning(),
             "batch": batch,
+            "batch_emax8": batch8,
             "hartree_fock": hf,
         },
+        "precision_gate": gate,
         "roofline": roof,
         "roofline_by_kernel": {k: {"achieved": round(v["achieved"], 2), "unit": v["unit"],
                                    "frac": round(v["frac"], 4), "avg_launch_ms": round(v["avg_launch_ms"], 4),

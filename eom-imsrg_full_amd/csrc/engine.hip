@@ -94,17 +94,18 @@ struct imsrg_ctx_s {
   imsrg_op_s *tmp[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // Hartree-Fock workspace (allocated on first use): D, rho, gamma, Fock and
   // bare one-body matrices (nsp x nsp), eigenvalues, e_HF diagonals,
-  // per-group eigenvalue changes, [crit, e_HF]
+  // per-group eigenvalue changes, and [crit, e_HF, converged flag, iterations]
   double *hf_D = nullptr, *hf_rho = nullptr, *hf_Vg = nullptr, *hf_Fm = nullptr, *hf_Tm = nullptr;
   double *hf_eig = nullptr, *hf_tv = nullptr, *hf_cp = nullptr, *hf_sc = nullptr;
   // per-kernel-class HIP event timing on this context's stream
   bool prof_on = false;
   unsigned prof_mask = ~0u;  // kernel classes bracketed while profiling
   // hipGraph replay of whole commutators: one executable graph per distinct
-  // (operands, herms, mask, Pandya reuse, instance table, precision)
+  // (operands, herms, mask, Pandya reuse, instance table, precision, series
+  // epilogue, FP64 tile shapes)
   bool graphs = false;
   typedef std::tuple<const void *, const void *, const void *, int, int, int, unsigned, int, const void *, int, int,
-                     const void *, double, int>
+                     const void *, double, int, int, int>
       GraphKey;
   std::map<GraphKey, hipGraphExec_t> gexec;
   long long graph_launches = 0, graph_builds = 0;
@@ -1113,7 +1114,7 @@ static int comm_impl(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *R
   if (!c->graphs || c->prof_on || a.n() == 0) return comm_launches(c, L, R, RES, mask, lcc_ready, a, epi);
   c->fork_mark = nullptr;  // a replayed graph runs on one stream
   const imsrg_ctx_s::GraphKey key(L->d, R->d, RES->d, L->herm, R->herm, RES->herm, mask, lcc_ready ? 1 : 0, a.d,
-                                  a.n(), c->prec, epi.acc, epi.coef, epi.norm ? 1 : 0);
+                                  a.n(), c->prec, epi.acc, epi.coef, epi.norm ? 1 : 0, c->shape_pp, c->shape_ph);
   auto it = c->gexec.find(key);
   if (it == c->gexec.end()) {
     hipGraph_t g = nullptr;

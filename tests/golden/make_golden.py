@@ -2,11 +2,14 @@
 compiled FP64 routines (oracle/_ref/ref_harness, built by oracle/ref.mk from
 the reference sources).  Run in the build container only:
 
-    make -f oracle/ref.mk && python tests/golden/make_golden.py [emax2] [emax4] [hf]
+    make -f oracle/ref.mk && python tests/golden/make_golden.py [emax2] [emax4] [hf] [emax6] [emax8]
 
 The fixtures hold inputs and outputs (vectorize-order FP64 streams), never
 reference source.  Inputs are stored too, so nothing depends on a random
-number generator being stable across numpy versions.
+number generator being stable across numpy versions.  At emax=6 and 8 one
+stream is 3.9 and 29 MB, so those fixtures hold sketches instead (tests/
+helpers.py: norm, Gaussian projections, sampled entries, and the SHA-256 of
+each input, which the tests check before comparing).
 """
 import os
 import sys
@@ -17,6 +20,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 sys.path.insert(0, os.path.join(ROOT, "eom-imsrg_full_amd"))
 sys.path.insert(0, ROOT)
 
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from helpers import sketch_store  # noqa: E402
 from imsrg_mi355x import Basis, random_operator, synthetic_hamiltonian  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 
@@ -119,6 +125,34 @@ def hartree_fock():
         print(name, sum(v.nbytes for v in d.values()) / 1e6, "MB")
 
 
+def terms_sketch(emax, k, flow_steps):
+    """every term and the full commutator of [L, R] (L anti-hermitian seed 1,
+    R hermitian seed 2), and the first `flow_steps` Magnus steps from the
+    synthetic Hamiltonian (seed 0) normal-ordered by the reference, as
+    sketches"""
+    B = Basis.ho(emax, k)
+    ref = O.RefRunner(B, threads=os.cpu_count())
+    d = {}
+    L, R = random_operator(B, -1, 1), random_operator(B, 1, 2)
+    sketch_store(d, "input_L", L)
+    sketch_store(d, "input_R", R)
+    t = ref.terms(-1, L, 1, R)
+    d["e110"], d["e220"] = np.float64(t["e110"]), np.float64(t["e220"])
+    for name in ("111", "121", "122", "222pphh", "221", "222ph"):
+        sketch_store(d, f"term_{name}", t[name])
+    sketch_store(d, "full", ref.comm(-1, L, 1, R))
+    if flow_steps:
+        Hn = ref.normal(synthetic_hamiltonian(B, seed=0, hw=20.0, sigma1=0.5, sigma2=1.0))
+        sketch_store(d, "input_Hn", Hn)
+        st, hs, g = ref.flow("white", Hn, flow_steps)
+        d["flow_rows"] = st
+        sketch_store(d, "flow_HS", hs)
+        sketch_store(d, "flow_G", g)
+    name = f"emax{emax}_k{k}_sketch.npz"
+    np.savez_compressed(os.path.join(OUT, name), **d)
+    print(name, sum(np.asarray(v).nbytes for v in d.values()) / 1e6, "MB")
+
+
 if __name__ == "__main__":
     if not O.ref_available():
         sys.exit("oracle/_ref/ref_harness is missing: run make -f oracle/ref.mk first")
@@ -129,3 +163,7 @@ if __name__ == "__main__":
         emax4()
     if "hf" in what:
         hartree_fock()
+    if "emax6" in what:
+        terms_sketch(6, 3, 0)
+    if "emax8" in what:
+        terms_sketch(8, 6, 2)

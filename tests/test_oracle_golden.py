@@ -130,3 +130,26 @@ def test_brute_force_golden(g2):
             else:
                 got = O.lib().or_brute_2body(oA.h, oB.h, a, b, c, dd, J)
             assert abs(got - w) <= 1e-10 * max(1.0, abs(w)), (tag, p, got, w)
+
+
+@pytest.mark.parametrize("emax,k", [(6, 3), (8, 6)], ids=["emax6", "emax8"])
+def test_terms_at_scale_vs_reference_sketch(emax, k):
+    """every term and the full commutator of [L, R] at the batching (emax=6)
+    and precision (emax=8) sizes against the reference build's sketches
+    (norm, projections, sampled entries; tests/helpers.py)"""
+    from helpers import sketch_check
+    from imsrg_mi355x import random_operator
+    d = np.load(os.path.join(GOLD, f"emax{emax}_k{k}_sketch.npz"), allow_pickle=False)
+    B = Basis.ho(emax, k)
+    O.lib().or_set_threads(os.cpu_count() or 1)
+    ob = O.OBasis(B)
+    L, R = random_operator(B, -1, 1), random_operator(B, 1, 2)
+    sketch_check(d, "input_L", L, exact=True)
+    sketch_check(d, "input_R", R, exact=True)
+    oL, oR = ob.op(L, -1), ob.op(R, 1)
+    t = O.terms(oL, oR)
+    assert abs(t["e110"] - float(d["e110"])) <= 1e-10 * max(1.0, abs(float(d["e110"])))
+    assert abs(t["e220"] - float(d["e220"])) <= 1e-10 * max(1.0, abs(float(d["e220"])))
+    for name in ("111", "121", "122", "222pphh", "221", "222ph"):
+        sketch_check(d, f"term_{name}", t[name])
+    sketch_check(d, "full", O.commutator(oL, oR).vec())
