@@ -188,11 +188,14 @@ void launch_ph_terms(hipStream_t s, const DevLayout &L, const double *Z, double 
 void launch_ph_apply(hipStream_t s, const DevLayout &L, const double *Tbuf, double *RES, const int *act,
                      int nact);
 // upper += herm * (strict lower)^T, then lower = herm * upper (diagonal zeroed for herm = -1).
-// Series epilogue (acc non-null): acc += coef * RES over every channel block;
+// Series epilogue (acc non-null): acc += coef * RES over every channel block,
+// coef read from *coefp when coefp is non-null (graph replays: the value is
+// written before each replay, so one graph serves every term of a series);
 // npart non-null: one sum of squares of the final RES per tile pair (nmirror +
 // nob partials per instance, the rest written by launch_ob_final)
 void launch_sym_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm, const int *act, int nact,
-                       double *acc = nullptr, double coef = 0.0, double *npart = nullptr);
+                       double *acc = nullptr, double coef = 0.0, const double *coefp = nullptr,
+                       double *npart = nullptr);
 // 111 + 121 + 221 one-body outputs in two launches: the 121/221 partial sums
 // (Wpp = pp intermediate of launch_seg_gemm_pphh; part: nsame * ob_nchunk
 // doubles per instance), then 111 plus the partials into F (and its mirror)
@@ -200,7 +203,7 @@ void launch_ob_partial(hipStream_t s, const DevLayout &L, const double *Lop, con
                        const double *Wpp, double *part, int hL, int hR, unsigned mask, const int *act, int nact);
 void launch_ob_final(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, const double *part,
                      double *RES, int hres, unsigned mask, const int *act, int nact, double *acc = nullptr,
-                     double coef = 0.0, double *npart = nullptr);
+                     double coef = 0.0, const double *coefp = nullptr, double *npart = nullptr);
 // sqrt of the sum of the nmirror + nob partials into out[inst * kScal]
 void launch_norm_from_parts(hipStream_t s, const DevLayout &L, const double *npart, double *out, const int *act,
                             int nact);

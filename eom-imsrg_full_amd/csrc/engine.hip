@@ -62,6 +62,7 @@ struct imsrg_ctx_s {
   double *obpart = nullptr;  // one-body 121/221 partial sums, nsame * ob_nchunk per instance
   double *npart = nullptr;   // sums of squares of a commutator result, nmirror + nob per instance
   double *scal = nullptr;   // device scalars, kScal per instance
+  double *coef_d = nullptr; // series coefficient of the next graph replay
   double *hscal = nullptr;  // pinned host mirror of scal
   double *mono = nullptr;
   double *mbar = nullptr;   // generator-transpose scratch (nsp * nsp)
@@ -249,11 +250,16 @@ static void build_seg_tiles(const std::vector<SegProb> &probs, std::vector<int4>
 }
 
 // A set of active instances: device table (what the kernels index with
-// blockIdx.y) and its host copy.
+// blockIdx.y) and its host copy.  In graph mode a flow's table is padded with
+// -1 entries (which every kernel skips) to the context's full batch, so the
+// captured launches stay valid while instances converge one after another:
+// the kernels launch over g() >= n() entries, the host reads the first n().
 struct ActSet {
   const int *d = nullptr;
   std::vector<int> h;
+  int ng = -1;  // table entries the kernels launch over; -1: n()
   int n() const { return (int)h.size(); }
+  int g() const { return ng < 0 ? n() : ng; }
 };
 static ActSet all_set(imsrg_ctx_s *c) {
   ActSet a;
@@ -266,14 +272,17 @@ static ActSet all_set(imsrg_ctx_s *c) {
 // stream).  Drivers rewrite a slot only after a stream synchronization, so
 // the pinned source is never overwritten while a copy from it is pending.
 // Slots 1 and 2 are the BCH and Magnus series tables (series_set).
-static ActSet make_set(imsrg_ctx_s *c, int slot, const std::vector<int> &list) {
+// With pad > list size the table is padded with -1 entries to pad entries.
+static ActSet make_set(imsrg_ctx_s *c, int slot, const std::vector<int> &list, int pad = 0) {
   ActSet a;
   a.d = c->act_d[slot];
   a.h = list;
-  if (!list.empty()) {
-    std::memcpy(c->act_h[slot], list.data(), list.size() * sizeof(int));
-    (void)hipMemcpyAsync(c->act_d[slot], c->act_h[slot], list.size() * sizeof(int), hipMemcpyHostToDevice,
-                         c->stream);
+  const int n = (int)list.size();
+  if (pad > n) a.ng = pad;
+  if (a.g() > 0) {
+    std::memcpy(c->act_h[slot], list.data(), n * sizeof(int));
+    for (int k = n; k < a.g(); k++) c->act_h[slot][k] = -1;
+    (void)hipMemcpyAsync(c->act_d[slot], c->act_h[slot], a.g() * sizeof(int), hipMemcpyHostToDevice, c->stream);
   }
   return a;
 }
@@ -487,6 +496,7 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
   if (!rc) rc = dalloc(c, (size_t)std::max(D.nsame, 1) * D.ob_nchunk * B, &c->obpart);
   if (!rc) rc = dalloc(c, (size_t)(D.nmirror + D.nob) * B, &c->npart);
   if (!rc) rc = dalloc(c, kScal * B, &c->scal);
+  if (!rc) rc = dalloc(c, 1, &c->coef_d);
   if (!rc) rc = dalloc(c, (size_t)Ly.nsp * Ly.nsp * B, &c->mono);
   if (!rc) rc = dalloc(c, (size_t)Ly.nsp * Ly.nsp, &c->mbar);
   for (int k = 0; k < imsrg_ctx_s::kSlots && !rc; k++) {
@@ -1038,6 +1048,7 @@ int imsrg_observable_to_hf(imsrg_op O, const double *coefs) {
 struct CommEpi {
   double *acc = nullptr;
   double coef = 0.0;
+  const double *coefp = nullptr;  // coef in device memory (graph replays)
   bool norm = false;
 };
 // lcc_ready: cc[0] already holds the Pandya matrices of L, and with int8
@@ -1113,14 +1124,22 @@ static int comm_impl(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *R
   }
   if (!c->graphs || c->prof_on || a.n() == 0) return comm_launches(c, L, R, RES, mask, lcc_ready, a, epi);
   c->fork_mark = nullptr;  // a replayed graph runs on one stream
+  // the series coefficient goes through device memory, written before the
+  // replay, so the terms of a series share one graph
+  CommEpi ge = epi;
+  if (ge.acc) {
+    launch_fill(c->stream, c->coef_d, 1, epi.coef);
+    ge.coef = 0.0;
+    ge.coefp = c->coef_d;
+  }
   const imsrg_ctx_s::GraphKey key(L->d, R->d, RES->d, L->herm, R->herm, RES->herm, mask, lcc_ready ? 1 : 0, a.d,
-                                  a.n(), c->prec, epi.acc, epi.coef, epi.norm ? 1 : 0, c->shape_pp, c->shape_ph);
+                                  a.g(), c->prec, ge.acc, 0.0, ge.norm ? 1 : 0, c->shape_pp, c->shape_ph);
   auto it = c->gexec.find(key);
   if (it == c->gexec.end()) {
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
-    int rc = comm_launches(c, L, R, RES, mask, lcc_ready, a, epi);
+    int rc = comm_launches(c, L, R, RES, mask, lcc_ready, a, ge);
     c->capturing = false;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc) {
@@ -1145,7 +1164,7 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
   hipStream_t s = c->stream;
   const DevLayout &D = c->dl;
   const int hL = L->herm, hR = R->herm, hres = RES->herm;
-  const int *ad = a.d, na = a.n();
+  const int *ad = a.d, na = a.g();
   const long long OS = D.op_size;
   if (na == 0) {
     c->fork_mark = nullptr;
@@ -1215,9 +1234,9 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
   }
   if (mask & T222PH) PROF(c, IMSRG_PROF_PH_APPLY, launch_ph_apply(s, D, c->tph, RES->d, ad, na));
   double *np = epi.norm ? c->npart : nullptr;
-  PROF(c, IMSRG_PROF_MIRROR, launch_sym_mirror(s, D, RES->d, hres, ad, na, epi.acc, epi.coef, np));
+  PROF(c, IMSRG_PROF_MIRROR, launch_sym_mirror(s, D, RES->d, hres, ad, na, epi.acc, epi.coef, epi.coefp, np));
   PROF(c, IMSRG_PROF_ONEBODY,
-       launch_ob_final(s, D, L->d, R->d, c->obpart, RES->d, hres, mask, ad, na, epi.acc, epi.coef, np));
+       launch_ob_final(s, D, L->d, R->d, c->obpart, RES->d, hres, mask, ad, na, epi.acc, epi.coef, epi.coefp, np));
   if (epi.norm) PROF(c, IMSRG_PROF_NORM, launch_norm_from_parts(s, D, c->npart, c->scal, ad, na));
   HIPCHK(hipGetLastError());
   return 0;
@@ -1375,7 +1394,8 @@ static ActSet series_set(imsrg_ctx_s *c, int slot, const ActSet &a) {
   ActSet s;
   s.d = c->act_d[slot];
   s.h = a.h;
-  (void)hipMemcpyAsync(c->act_d[slot], a.d, a.n() * sizeof(int), hipMemcpyDeviceToDevice, c->stream);
+  s.ng = a.ng;
+  (void)hipMemcpyAsync(c->act_d[slot], a.d, a.g() * sizeof(int), hipMemcpyDeviceToDevice, c->stream);
   return s;
 }
 // The operand L = Omega of every nested commutator of a series, prepared once:
@@ -1393,7 +1413,7 @@ static void prep_L(imsrg_ctx_s *c, imsrg_op_s *G, const ActSet &set) {
 static int series_gate(imsrg_ctx_s *c, int slot, const ActSet &set, int n, int mode, double coef, int store,
                        int test) {
   const int r = n % imsrg_ctx_s::kGateRing;
-  launch_series_gate(c->stream, c->act_d[slot], set.n(), c->scal, mode, coef, store, test, kConv, kTiny,
+  launch_series_gate(c->stream, c->act_d[slot], set.g(), c->scal, mode, coef, store, test, kConv, kTiny,
                      c->live_d + (size_t)r * c->nbatch);
   HIPCHK(hipEventRecord(c->ev_gate[r], c->stream));
   return 0;
@@ -1654,7 +1674,7 @@ int imsrg_flow_step(imsrg_flow f, double *rec) {
   std::vector<double> e0(B, 0.0);
   int rc = 0;
   if (!live.empty()) {
-    const ActSet a = ((int)live.size() == B) ? all_set(c) : make_set(c, 3, live);
+    const ActSet a = ((int)live.size() == B) ? all_set(c) : make_set(c, 3, live, c->graphs ? B : 0);
     std::vector<int> tm, tb;
     rc = magexp_impl(c, f->DG, f->G, a, tm);
     if (rc) return rc;

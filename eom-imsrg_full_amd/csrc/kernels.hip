@@ -1227,10 +1227,11 @@ void launch_ph_apply(hipStream_t s, const DevLayout &L, const double *Tbuf, doub
 // writes are row-contiguous.
 __global__ __launch_bounds__(256) void k_sym_mirror(DevLayout L, const int4 *__restrict__ tiles, double *RES,
                                                     int herm, const int *act, double *acc, double coef,
-                                                    double *npart, long long nstride) {
+                                                    const double *coefp, double *npart, long long nstride) {
   __shared__ double sh[256];
   const long long inst = inst_id(act);
   if (inst < 0) return;
+  if (coefp) coef = *coefp;
   RES += inst * L.op_size;
   const int4 t = tiles[blockIdx.x];  // x = block, y = tile row, z = tile col (y <= z)
   const int q = t.x, dim = L.bdim[q];
@@ -1279,10 +1280,10 @@ __global__ __launch_bounds__(256) void k_sym_mirror(DevLayout L, const int4 *__r
   }
 }
 void launch_sym_mirror(hipStream_t s, const DevLayout &L, double *RES, int herm, const int *act, int nact,
-                       double *acc, double coef, double *npart) {
+                       double *acc, double coef, const double *coefp, double *npart) {
   if (L.nmirror <= 0 || nact <= 0) return;
   hipLaunchKernelGGL(k_sym_mirror, dim3(L.nmirror, nact), dim3(256), 0, s, L, L.mirror_tiles, RES, herm, act, acc,
-                     coef, npart, (long long)(L.nmirror + L.nob));
+                     coef, coefp, npart, (long long)(L.nmirror + L.nob));
 }
 
 // ------------------------------------------------------------------ one-body (111 + 121 + 221)
@@ -1382,11 +1383,12 @@ __global__ __launch_bounds__(256) void k_ob_partial(DevLayout L, const double *_
 __global__ __launch_bounds__(64) void k_ob_final(DevLayout L, const double *__restrict__ Lop,
                                                  const double *__restrict__ Rop, const double *__restrict__ part,
                                                  double *__restrict__ RES, int hres, unsigned mask, const int *act,
-                                                 double *acc, double coef, double *npart) {
+                                                 double *acc, double coef, const double *coefp, double *npart) {
   const int t = blockIdx.x;
   if (t >= L.nob) return;
   const long long inst = inst_id(act);
   if (inst < 0) return;
+  if (coefp) coef = *coefp;
   Lop += inst * L.op_size;
   Rop += inst * L.op_size;
   RES += inst * L.op_size;
@@ -1427,10 +1429,10 @@ void launch_ob_partial(hipStream_t s, const DevLayout &L, const double *Lop, con
 }
 void launch_ob_final(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, const double *part,
                      double *RES, int hres, unsigned mask, const int *act, int nact, double *acc, double coef,
-                     double *npart) {
+                     const double *coefp, double *npart) {
   if (L.nob <= 0 || nact <= 0) return;
   hipLaunchKernelGGL(k_ob_final, dim3(L.nob, nact), dim3(64), 0, s, L, Lop, Rop, part, RES, hres, mask, act, acc,
-                     coef, npart);
+                     coef, coefp, npart);
 }
 
 // ------------------------------------------------------------------ reductions
