@@ -237,6 +237,11 @@ void launch_hf_iteration(hipStream_t s, const DevLayout &L, const double *op, do
 void launch_hf_finish_1b(hipStream_t s, const DevLayout &L, double *D, const double *M, double *op, int herm,
                          const double *Tm, const double *Vg, double *tv, double *ehf);
 void launch_hf_2b(hipStream_t s, const DevLayout &L, const double *D, double *op, double *W, int herm);
+// the same transform for the pair rows [r0, r0 + nrows), which must cover whole
+// channel blocks: W holds only their blocks, W[0] = element v_off + wofs of a
+// full operator layout (wofs = the first block's offset inside V)
+void launch_hf_2b_window(hipStream_t s, const DevLayout &L, const double *D, double *op, double *W, int herm,
+                         int r0, int nrows, long long wofs);
 
 void launch_series_gate(hipStream_t s, int *act, int nact, double *scal, int mode, double coef, int store,
                         int test, double conv, double tiny, int *live);

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -98,6 +99,9 @@ struct imsrg_ctx_s {
   // per-group eigenvalue changes, and [crit, e_HF, converged flag, iterations]
   double *hf_D = nullptr, *hf_rho = nullptr, *hf_Vg = nullptr, *hf_Fm = nullptr, *hf_Tm = nullptr;
   double *hf_eig = nullptr, *hf_tv = nullptr, *hf_cp = nullptr, *hf_sc = nullptr;
+  // this device's copy of the static Hartree-Fock scratch array (the
+  // allocation-free entry point imsrg_hartree_fock_noalloc)
+  double *hf_scratch = nullptr;
   // per-kernel-class HIP event timing on this context's stream
   bool prof_on = false;
   unsigned prof_mask = ~0u;  // kernel classes bracketed while profiling
@@ -329,6 +333,35 @@ static int single_only(imsrg_ctx_s *c, const char *what) {
   return c->nbatch == 1 ? 0 : fail(IMSRG_EINVAL, std::string(what) + ": batched context, use the _batch variant");
 }
 
+// ---------------------------------------------------------------- static scratch
+// Device memory of imsrg_hartree_fock_noalloc: the self-consistency state
+// (five nsp x nsp matrices, eigenvalues, e_HF diagonals, per-group changes,
+// four scalars) and, in the rest, one window of channel blocks for the
+// two-body transform.  Each GPU the library runs on holds its own copy of the
+// array until the process ends; host code uses the address that
+// hipGetSymbolAddress returns for the current device, never the array's name.
+static constexpr long long kHFScratchDoubles = 2LL << 20;  // 16 MiB
+static __device__ __align__(256) double scratch[kHFScratchDoubles];
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kMaxDevices] = {};
+
+// Once per device, with that device current (context creation): the device
+// address of `scratch`.
+static int scratch_setup(imsrg_ctx_s *c) {
+  if (c->device < 0 || c->device >= kMaxDevices) return fail(IMSRG_EDEVICE, "device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[c->device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(IMSRG_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[c->device] = static_cast<double *>(p);
+  }
+  c->hf_scratch = g_scratch_ptr[c->device];
+  return 0;
+}
+
 // ---------------------------------------------------------------- context
 int imsrg_ctx_create(int nsp, const int *n, const int *l, const int *jj, const int *tz, const int *con,
                      int device, imsrg_ctx *out) {
@@ -353,6 +386,10 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
     }
   }
   (void)hipGetDevice(&c->device);
+  if (int rc = scratch_setup(c)) {
+    delete c;
+    return rc;
+  }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return fail(IMSRG_EDEVICE, "hipStreamCreate failed");
@@ -1017,6 +1054,83 @@ int imsrg_hartree_fock(imsrg_op H, double *coefs, int *iterations, double *e_hf)
   if (coefs)
     HIPCHK(hipMemcpyAsync(coefs, c->hf_D, sizeof(double) * c->lay.nsp * c->lay.nsp, hipMemcpyDeviceToHost,
                           c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (e_hf) *e_hf = c->hscal[0];
+  if (iterations) *iterations = it;
+  return 0;
+}
+
+// imsrg_hartree_fock with every buffer in the static scratch array: the
+// self-consistency state at the front, then the two-body transform block by
+// block through windows of whole channel blocks that fit the rest.  Nothing is
+// allocated; the state is cleared by a fill kernel on every call.
+int imsrg_hartree_fock_noalloc(imsrg_op H, double *coefs, int *iterations, double *e_hf) {
+  if (!H) return fail(IMSRG_EINVAL, "null operand");
+  imsrg_ctx_s *c = H->ctx;
+  if (c->nbatch != 1) return fail(IMSRG_EINVAL, "imsrg_hartree_fock_noalloc: batched context");
+  if (H->herm != 1) return fail(IMSRG_EINVAL, "imsrg_hartree_fock_noalloc: H must be hermitian");
+  if (!c->hf_scratch) return fail(IMSRG_EDEVICE, "imsrg_hartree_fock_noalloc: scratch not set up");
+  const Layout &Ly = c->lay;
+  for (int n : Ly.grp_cnt)
+    if (n > kHFMaxGroup) return fail(IMSRG_EINVAL, "Hartree-Fock: an (l,j,tz) group has more than 16 orbits");
+  auto a32 = [](long long x) { return (x + 31) & ~31LL; };
+  const long long n2 = (long long)Ly.nsp * Ly.nsp;
+  double *const base = c->hf_scratch;
+  long long o = 0;
+  double *m[5];
+  for (double *&p : m) {
+    p = base + o;
+    o += a32(n2);
+  }
+  double *eig = base + o;
+  o += a32(Ly.nsp);
+  double *tv = base + o;
+  o += a32(2LL * Ly.nsp);
+  double *cp = base + o;
+  o += a32((long long)Ly.grp_start.size());
+  double *sc = base + o;  // crit, e_HF, converged flag, iterations
+  o += 32;
+  const long long state = o, wcap = kHFScratchDoubles - state;
+  if (wcap <= 0)
+    return fail(IMSRG_EINVAL, "imsrg_hartree_fock_noalloc: " + std::to_string(Ly.nsp) +
+                                  " orbits need more than the 16 MiB scratch (" + std::to_string(state * 8) +
+                                  " bytes of state)");
+  for (int q = 0; q < Ly.nblocks; q++)
+    if ((long long)Ly.bdim[q] * Ly.bdim[q] > wcap)
+      return fail(IMSRG_EINVAL, "imsrg_hartree_fock_noalloc: channel block " + std::to_string(q) + " (dim " +
+                                    std::to_string(Ly.bdim[q]) + ") exceeds the scratch window of " +
+                                    std::to_string(wcap) + " doubles");
+  double *D = m[0], *rho = m[1], *Vg = m[2], *Fm = m[3], *Tm = m[4], *W = base + state;
+  const DevLayout &DL = c->dl;
+  launch_fill(c->stream, base, state, 0.0);
+  launch_hf_init(c->stream, DL, H->d, D, Tm, eig, 1);
+  const double conv = (double)1e-6f;  // the reference's default-real loop test
+  int it = 0;
+  for (;;) {
+    for (int k = 0; k < 8; k++)
+      launch_hf_iteration(c->stream, DL, H->d, D, rho, Tm, Vg, Fm, eig, cp, sc, sc + 2, conv);
+    HIPCHK(hipMemcpyAsync(c->hscal, sc, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    it = (int)c->hscal[3];
+    if (c->hscal[2] != 0.0) break;
+    if (it >= 10000) return fail(IMSRG_ECONV, "Hartree-Fock did not converge in 10000 iterations");
+  }
+  launch_hf_finish_1b(c->stream, DL, D, Fm, H->d, 1, Tm, Vg, tv, sc + 1);
+  // windows of consecutive channel blocks [q0, q1) whose blocks fit wcap
+  for (int q0 = 0; q0 < Ly.nblocks;) {
+    int q1 = q0 + 1;
+    while (q1 < Ly.nblocks &&
+           Ly.boff[q1] + (long long)Ly.bdim[q1] * Ly.bdim[q1] - Ly.boff[q0] <= wcap)
+      q1++;
+    const int r0 = Ly.pbase[q0];
+    const int nrows = Ly.pbase[q1 - 1] + Ly.bdim[q1 - 1] - r0;
+    launch_hf_2b_window(c->stream, DL, D, H->d, W, 1, r0, nrows, (long long)Ly.boff[q0]);
+    q0 = q1;
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(c->hscal, sc + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (coefs)
+    HIPCHK(hipMemcpyAsync(coefs, D, sizeof(double) * n2, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (e_hf) *e_hf = c->hscal[0];
   if (iterations) *iterations = it;

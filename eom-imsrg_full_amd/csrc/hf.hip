@@ -268,13 +268,14 @@ __device__ __forceinline__ double hf_u(const DevLayout &L, const double *D, int 
 // pass 1: W = U^T V, row by row: W(ij, s) = sum over the rows (ab) that U
 // couples to (ij) of U(ab,ij) V(ab, s); one wavefront per pair row
 __global__ __launch_bounds__(64) void k_hf_rows(DevLayout L, const double *__restrict__ D,
-                                                const double *__restrict__ op, double *__restrict__ W) {
-  const int r = blockIdx.x;
+                                                const double *__restrict__ op, double *__restrict__ W, int r0,
+                                                long long wofs) {
+  const int r = r0 + (int)blockIdx.x;
   if (r >= L.npairs) return;
   const int q = L.pair_q[r], base = L.pbase[q], rl = r - base, dim = L.bdim[q], J = L.bJ[q];
   const int i = L.pair_a[r], j = L.pair_b[r];
   const double *VB = op + L.v_off + L.boff[q];
-  double *WR = W + L.v_off + L.boff[q] + (long long)rl * dim;
+  double *WR = W + (L.v_off + L.boff[q] - wofs) + (long long)rl * dim;
   const int e0 = L.hf_nb_start[r], e1 = L.hf_nb_start[r + 1];
   for (int s0 = 0; s0 < dim; s0 += 256) {
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -300,11 +301,12 @@ __global__ __launch_bounds__(64) void k_hf_rows(DevLayout L, const double *__res
 // U couples to (kl) of W(ij, ab) U(ab, kl); the lower triangle is herm times
 // the upper (diagonal zero for herm = -1); one wavefront per pair row (ij)
 __global__ __launch_bounds__(64) void k_hf_cols(DevLayout L, const double *__restrict__ D,
-                                                const double *__restrict__ W, double *__restrict__ op, int herm) {
-  const int r = blockIdx.x;
+                                                const double *__restrict__ W, double *__restrict__ op, int herm,
+                                                int r0, long long wofs) {
+  const int r = r0 + (int)blockIdx.x;
   if (r >= L.npairs) return;
   const int q = L.pair_q[r], base = L.pbase[q], rl = r - base, dim = L.bdim[q], J = L.bJ[q];
-  const double *WR = W + L.v_off + L.boff[q] + (long long)rl * dim;
+  const double *WR = W + (L.v_off + L.boff[q] - wofs) + (long long)rl * dim;
   double *VB = op + L.v_off + L.boff[q];
   for (int s = rl + (int)threadIdx.x; s < dim; s += 64) {
     const int k = L.pair_a[base + s], l = L.pair_b[base + s];
@@ -344,8 +346,14 @@ void launch_hf_finish_1b(hipStream_t s, const DevLayout &L, double *D, const dou
 }
 void launch_hf_2b(hipStream_t s, const DevLayout &L, const double *D, double *op, double *W, int herm) {
   if (L.npairs <= 0) return;
-  hipLaunchKernelGGL(k_hf_rows, dim3(L.npairs), dim3(64), 0, s, L, D, op, W);
-  hipLaunchKernelGGL(k_hf_cols, dim3(L.npairs), dim3(64), 0, s, L, D, W, op, herm);
+  hipLaunchKernelGGL(k_hf_rows, dim3(L.npairs), dim3(64), 0, s, L, D, op, W, 0, 0LL);
+  hipLaunchKernelGGL(k_hf_cols, dim3(L.npairs), dim3(64), 0, s, L, D, W, op, herm, 0, 0LL);
+}
+void launch_hf_2b_window(hipStream_t s, const DevLayout &L, const double *D, double *op, double *W, int herm,
+                         int r0, int nrows, long long wofs) {
+  if (nrows <= 0) return;
+  hipLaunchKernelGGL(k_hf_rows, dim3(nrows), dim3(64), 0, s, L, D, op, W, r0, wofs);
+  hipLaunchKernelGGL(k_hf_cols, dim3(nrows), dim3(64), 0, s, L, D, W, op, herm, r0, wofs);
 }
 
 }  // namespace imsrg

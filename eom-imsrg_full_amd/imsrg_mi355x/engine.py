@@ -99,6 +99,7 @@ _SIG = {
     "imsrg_ctx_set_concurrency": (C.c_int, [_vp, C.c_int]),
     "imsrg_transform_bch": (C.c_int, [_vp, _vp, C.POINTER(C.c_int)]),
     "imsrg_hartree_fock": (C.c_int, [_vp, _dp, C.POINTER(C.c_int), _dp]),
+    "imsrg_hartree_fock_noalloc": (C.c_int, [_vp, _dp, C.POINTER(C.c_int), _dp]),
     "imsrg_observable_to_hf": (C.c_int, [_vp, _dp]),
     "imsrg_dhds": (C.c_int, [_vp, C.c_int, _vp]),
     "imsrg_ctx_profile_read": (C.c_int, [_vp, _dp, C.POINTER(C.c_longlong), C.c_int, C.c_int]),

@@ -16,7 +16,7 @@
  * A context is bound to one device and one stream; calls on one context are
  * stream-ordered and must come from one host thread at a time.  Different
  * contexts are independent (no global mutable state besides the per-thread
- * error string).
+ * error string and the per-GPU scratch array of imsrg_hartree_fock_noalloc).
  */
 #ifndef IMSRG_MI355X_H
 #define IMSRG_MI355X_H
@@ -146,6 +146,17 @@ int imsrg_normal_order(imsrg_op H);                                 /* normal_or
  * eigenproblems (diagonalize_blocks, basic_IMSRG.f90:2163-2180) and the
  * transforms run on the device.  nbatch = 1; groups of at most 16 orbits. */
 int imsrg_hartree_fock(imsrg_op H, double *coefs, int *iterations, double *e_hf);
+/* imsrg_hartree_fock without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * results and errors as imsrg_hartree_fock (bit for bit), but every work
+ * buffer lives in one static 16 MiB device array per GPU (held until the
+ * process ends): the self-consistency state (5 nsp^2 + 3 nsp doubles and a
+ * few more) and windows of whole channel blocks for the two-body transform.
+ * Returns IMSRG_EINVAL when the state or one channel block does not fit.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (two host threads, two contexts on one device) race. */
+int imsrg_hartree_fock_noalloc(imsrg_op H, double *coefs, int *iterations, double *e_hf);
 /* sqop_to_HF (HF_mod.f90:108-136) for a scalar observable O (bare): the
  * one- and two-body transforms with the coefficients of imsrg_hartree_fock,
  * then normal_order. */
