@@ -1165,12 +1165,15 @@ struct CommEpi {
   const double *coefp = nullptr;  // coef in device memory (graph replays)
   bool norm = false;
 };
-// lcc_ready: cc[0] already holds the Pandya matrices of L, and with int8
-// slices the slices of L too (prep_L; drivers reuse them across the nested
-// commutators of one BCH / Magnus series, where L = Omega).
+// cc_ready bits: kLPrepped -- cc[0] already holds the Pandya matrices of L,
+// and with int8 slices the slices of L too (prep_L; drivers reuse them across
+// the nested commutators of one BCH / Magnus series, where L = Omega);
+// kLPandya / kRPandya -- cc[0] / cc[1] hold the Pandya matrices of L / R
+// (uploaded by the caller), their int8 slices still to be made.
 // Every launch covers the instances of `a`.
+enum { kLPrepped = 1, kLPandya = 2, kRPandya = 4 };
 static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *RES, unsigned mask,
-                         bool lcc_ready, const ActSet &a, const CommEpi &epi = CommEpi());
+                         int cc_ready, const ActSet &a, const CommEpi &epi = CommEpi());
 
 // Tile-shape autotuner: on the first FP64 commutator of a context, each 222
 // GEMM is timed (HIP events, best of 3) in both FP64 tile shapes on the
@@ -1230,13 +1233,13 @@ static int autotune(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R) {
 // nodes only; operands, workspaces and the instance table are fixed device
 // addresses, so a replay is the same work as the eager sequence.
 static int comm_impl(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *RES, unsigned mask,
-                     bool lcc_ready, const ActSet &a, const CommEpi &epi = CommEpi()) {
+                     int cc_ready, const ActSet &a, const CommEpi &epi = CommEpi()) {
   if (c->prec == IMSRG_PREC_FP64 && c->shape_pp < 0 && a.n() > 0 && (mask & (T222PPHH | T221 | T222PH))) {
     c->fork_mark = nullptr;  // the tuning launches follow the mark on the stream
     int rc = autotune(c, L, R);
     if (rc) return rc;
   }
-  if (!c->graphs || c->prof_on || a.n() == 0) return comm_launches(c, L, R, RES, mask, lcc_ready, a, epi);
+  if (!c->graphs || c->prof_on || a.n() == 0) return comm_launches(c, L, R, RES, mask, cc_ready, a, epi);
   c->fork_mark = nullptr;  // a replayed graph runs on one stream
   // the series coefficient goes through device memory, written before the
   // replay, so the terms of a series share one graph
@@ -1246,14 +1249,14 @@ static int comm_impl(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *R
     ge.coef = 0.0;
     ge.coefp = c->coef_d;
   }
-  const imsrg_ctx_s::GraphKey key(L->d, R->d, RES->d, L->herm, R->herm, RES->herm, mask, lcc_ready ? 1 : 0, a.d,
+  const imsrg_ctx_s::GraphKey key(L->d, R->d, RES->d, L->herm, R->herm, RES->herm, mask, cc_ready, a.d,
                                   a.g(), c->prec, ge.acc, 0.0, ge.norm ? 1 : 0, c->shape_pp, c->shape_ph);
   auto it = c->gexec.find(key);
   if (it == c->gexec.end()) {
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
-    int rc = comm_launches(c, L, R, RES, mask, lcc_ready, a, ge);
+    int rc = comm_launches(c, L, R, RES, mask, cc_ready, a, ge);
     c->capturing = false;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc) {
@@ -1274,7 +1277,8 @@ static int comm_impl(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *R
 }
 
 static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_s *RES, unsigned mask,
-                         bool lcc_ready, const ActSet &a, const CommEpi &epi) {
+                         int cc_ready, const ActSet &a, const CommEpi &epi) {
+  const bool lprep = cc_ready & kLPrepped, lcc = lprep || (cc_ready & kLPandya), rcc = cc_ready & kRPandya;
   hipStream_t s = c->stream;
   const DevLayout &D = c->dl;
   const int hL = L->herm, hR = R->herm, hres = RES->herm;
@@ -1302,7 +1306,7 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
   const bool i8 = c->prec == IMSRG_PREC_INT8X5;
   if ((mask & (T222PPHH | T221)) && i8) {
     PROFS(c, IMSRG_PROF_SLICE, s2, {
-      if (!lcc_ready) launch_slice_v(s2, D, c->i8, L->d, c->i8v[0], c->i8vs[0], ad, na);
+      if (!lprep) launch_slice_v(s2, D, c->i8, L->d, c->i8v[0], c->i8vs[0], ad, na);
       launch_slice_v(s2, D, c->i8, R->d, c->i8v[1], c->i8vs[1], ad, na);
     });
     // B rows are rows of the other operand: X[k][n] = herm * X[n][k]
@@ -1322,11 +1326,11 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
   PROFS(c, IMSRG_PROF_ONEBODY, s2,
         launch_ob_partial(s2, D, L->d, R->d, c->wpp, c->obpart, hL, hR, mask, ad, na));
   if (mask & T222PH) {
-    if (!lcc_ready) PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, L->d, c->cc[0], ad, na));
-    PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, R->d, c->cc[1], ad, na));
+    if (!lcc) PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, L->d, c->cc[0], ad, na));
+    if (!rcc) PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, R->d, c->cc[1], ad, na));
     if (i8) {
       PROF(c, IMSRG_PROF_SLICE, {
-        if (!lcc_ready) launch_slice_cc(s, D, c->i8, c->cc[0], c->i8pm, c->i8c[0], c->i8cs[0], ad, na);
+        if (!lprep) launch_slice_cc(s, D, c->i8, c->cc[0], c->i8pm, c->i8c[0], c->i8cs[0], ad, na);
         launch_slice_cc(s, D, c->i8, c->cc[1], c->i8pm, c->i8c[1], c->i8cs[1], ad, na);
       });
       PROFK(c, IMSRG_PROF_GEMM_PH,
@@ -1361,7 +1365,7 @@ int imsrg_commutator(imsrg_ctx c, imsrg_op L, imsrg_op R, imsrg_op RES, unsigned
   if (L->ctx != c || R->ctx != c || RES->ctx != c) return fail(IMSRG_EINVAL, "operands of another context");
   if (RES == L || RES == R) return fail(IMSRG_EINVAL, "RES must not alias L or R");
   if (RES->herm != -L->herm * R->herm) return fail(IMSRG_EINVAL, "RES herm must be -herm(L)*herm(R)");
-  return comm_impl(c, L, R, RES, mask & IMSRG_TALL, false, all_set(c));
+  return comm_impl(c, L, R, RES, mask & IMSRG_TALL, 0, all_set(c));
 }
 
 static int scalar_term(imsrg_op L, imsrg_op R, unsigned mask, double *out) {
@@ -1404,6 +1408,43 @@ int imsrg_cross_coupled(imsrg_op op, double *host_out, int64_t n) {
       for (int r = 0; r < rr; r++) host_out[k++] = blk[(int64_t)col * rd + ref[r]];
   }
   return 0;
+}
+
+// commutator_222_ph (commutators.f90:771-925) from Pandya matrices the caller
+// made with calculate_cross_coupled: the two streams (reference row order,
+// as imsrg_cross_coupled writes them) go into the device's paired-row layout,
+// then the Z product, the J' terms and the 6j-weighted sums run as in
+// imsrg_commutator with mask IMSRG_T222PH.
+int imsrg_commutator_222_ph_cc(imsrg_ctx c, const double *lcc, int hL, const double *rcc, int hR, int64_t n,
+                               imsrg_op RES) {
+  if (!c || !lcc || !rcc || !RES) return fail(IMSRG_EINVAL, "null argument");
+  if (RES->ctx != c) return fail(IMSRG_EINVAL, "RES belongs to another context");
+  if (int rc = single_only(c, "imsrg_commutator_222_ph_cc")) return rc;
+  if ((hL != 1 && hL != -1) || (hR != 1 && hR != -1)) return fail(IMSRG_EINVAL, "herm must be +1 or -1");
+  if (RES->herm != -hL * hR) return fail(IMSRG_EINVAL, "RES herm must be -herm(LCC)*herm(RCC)");
+  if (n != imsrg_cross_coupled_size(c)) return fail(IMSRG_EINVAL, "Pandya stream size mismatch");
+  const Layout &Ly = c->lay;
+  const double *src[2] = {lcc, rcc};
+  for (int k = 0; k < 2; k++) {
+    std::memset(c->hstage, 0, sizeof(double) * Ly.ccsize);
+    int64_t o = 0;
+    for (int q = 0; q < Ly.ncc; q++) {
+      const int rr = Ly.ccrlen_ref[q], rd = Ly.ccrlen[q], nb = Ly.ccnph[q];
+      double *blk = c->hstage + Ly.ccoff[q];
+      const int *ref = Ly.ccref.data() + Ly.ccrefbase[q];
+      for (int col = 0; col < nb; col++)
+        for (int r = 0; r < rr; r++) blk[(int64_t)col * rd + ref[r]] = src[k][o++];
+    }
+    HIPCHK(hipMemcpyAsync(c->cc[k], c->hstage, sizeof(double) * Ly.ccsize, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));  // hstage is rewritten next
+  }
+  // operands stand-ins carrying the hermiticities: with mask T222PH and both
+  // Pandya matrices in place no kernel reads their storage
+  imsrg_op_s Lh = *RES, Rh = *RES;
+  Lh.herm = hL;
+  Rh.herm = hR;
+  c->fork_mark = nullptr;
+  return comm_launches(c, &Lh, &Rh, RES, T222PH, kLPandya | kRPandya, all_set(c));
 }
 
 // ---------------------------------------------------------------- adjoint commutator
@@ -1474,7 +1515,7 @@ static int gen_impl(imsrg_ctx_s *c, int kind, imsrg_op_s *H, imsrg_op_s *ETA, co
     imsrg_op_s *E1 = tmp_op(c, 2, -1), *H2 = tmp_op(c, 3, 1), *E2 = tmp_op(c, 4, -1);
     if (!E1 || !H2 || !E2) return IMSRG_ENOMEM;
     gen_simple(c, H, H, E1, 1, a);
-    int rc = comm_impl(c, E1, H, H2, IMSRG_TALL, false, a);
+    int rc = comm_impl(c, E1, H, H2, IMSRG_TALL, 0, a);
     if (rc) return rc;
     gen_simple(c, H, H2, E2, 1, a);
     launch_axpby(c->stream, c->lay.op_size, 1.5, E1->d, 0.5, E2->d, ETA->d, a.d, a.n(), c->lay.op_size);
@@ -1577,7 +1618,7 @@ static int bch_impl(imsrg_ctx_s *c, imsrg_op_s *HS, imsrg_op_s *G, imsrg_op_s *H
     epi.norm = true;
     // nothing was enqueued since the last gate: the second stream can fork from its event
     c->fork_mark = iw > 2 ? c->ev_gate[(iw - 1) % imsrg_ctx_s::kGateRing] : nullptr;
-    int rc = comm_impl(c, G, AD, INT2, IMSRG_TALL, true, cur, epi);
+    int rc = comm_impl(c, G, AD, INT2, IMSRG_TALL, kLPrepped, cur, epi);
     if (!rc) rc = series_gate(c, 1, cur, iw, 0, coef, -1, 0);
     if (!rc && iw > 2) rc = series_live(c, cur, iw - 1, live, nlive);
     if (rc) return rc;
@@ -1620,7 +1661,7 @@ int imsrg_dhds(imsrg_op HS, int kind, imsrg_op DH) {
   if (!ETA) return IMSRG_ENOMEM;
   const ActSet all = all_set(c);
   int rc = gen_impl(c, kind, HS, ETA, all);
-  if (!rc) rc = comm_impl(c, ETA, HS, DH, IMSRG_TALL, false, all);
+  if (!rc) rc = comm_impl(c, ETA, HS, DH, IMSRG_TALL, 0, all);
   return rc;
 }
 
@@ -1664,7 +1705,7 @@ static int magexp_impl(imsrg_ctx_s *c, imsrg_op_s *DG, imsrg_op_s *G, const ActS
     epi.coef = cof[i - 1];
     epi.norm = true;
     c->fork_mark = i > 2 ? c->ev_gate[(i - 1) % imsrg_ctx_s::kGateRing] : nullptr;
-    rc = comm_impl(c, G, cur, oth, IMSRG_TALL & ~(T110 | T220), true, set, epi);
+    rc = comm_impl(c, G, cur, oth, IMSRG_TALL & ~(T110 | T220), kLPrepped, set, epi);
     // advals(i) = ||oth|| |cof(i)|, then the test for commutator i+1
     if (!rc && i < 7) rc = series_gate(c, 2, set, i, 1, std::fabs(cof[i - 1]), i - 1, i - q);
     if (!rc) rc = series_live(c, set, i - 1, live, nlive);
@@ -1908,7 +1949,7 @@ static int nested_chain(imsrg_ctx_s *c, imsrg_op_s *Om, imsrg_op_s *X0, int K, u
   prep_L(c, Om, all);
   for (int k = 1; k <= K; k++) {
     A[k]->herm = -Om->herm * A[k - 1]->herm;
-    int rc = comm_impl(c, Om, A[k - 1], A[k], mask, true, all);
+    int rc = comm_impl(c, Om, A[k - 1], A[k], mask, kLPrepped, all);
     if (rc) return rc;
   }
   return 0;

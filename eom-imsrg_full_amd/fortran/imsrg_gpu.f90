@@ -34,6 +34,11 @@ module imsrg_gpu
 
   type(c_ptr), save :: ctx = c_null_ptr
   integer(c_int64_t), save :: neq = 0
+  ! the 221 one-body term of the last gpu_commutator_222_pp_hh call, for the
+  ! gpu_commutator_221 call that passes the same w1, w2
+  real(8), allocatable, save :: ob_fhh(:,:), ob_fph(:,:), ob_fpp(:,:)
+  type(c_ptr), save :: tok_w1 = c_null_ptr, tok_w2 = c_null_ptr
+  logical, save :: have221 = .false.
 
   interface
      function c_last_error() bind(C, name='imsrg_last_error') result(p)
@@ -339,20 +344,32 @@ contains
     call add_twobody(RES, T, .true.)
   end subroutine gpu_commutator_122
 
-  ! accumulates two-body; w1/w2 are kept in the argument list for call-site
-  ! compatibility -- the device keeps its own pp/hh intermediates
+  ! accumulates two-body; one pp/hh product also gives the 221 one-body term,
+  ! kept for the gpu_commutator_221 call that passes the same w1, w2 (which are
+  ! not filled: the device keeps its own pp/hh intermediates)
   subroutine gpu_commutator_222_pp_hh(L, R, RES, w1, w2, jbas)
-    type(sq_op) :: L, R, RES, w1, w2, T
+    type(sq_op) :: L, R, RES, T
+    type(sq_op), target :: w1, w2
     type(spd) :: jbas
-    call term(L, R, RES, T222PPHH, T)
+    call term(L, R, RES, ior(T222PPHH, T221), T)
     call add_twobody(RES, T, .false.)
+    ob_fhh = T%fhh ; ob_fph = T%fph ; ob_fpp = T%fpp
+    tok_w1 = c_loc(w1)
+    tok_w2 = c_loc(w2)
+    have221 = .true.
   end subroutine gpu_commutator_222_pp_hh
 
   subroutine gpu_commutator_221(L, R, RES, w1, w2, jbas)  ! accumulates one-body
-    type(sq_op) :: L, R, RES, w1, w2, T
+    type(sq_op) :: L, R, RES, T
+    type(sq_op), target :: w1, w2
     type(spd) :: jbas
-    call term(L, R, RES, T221, T)
-    call add_onebody(RES, T, .false.)
+    if (have221 .and. c_associated(tok_w1, c_loc(w1)) .and. c_associated(tok_w2, c_loc(w2))) then
+       RES%fhh = RES%fhh + ob_fhh ; RES%fph = RES%fph + ob_fph ; RES%fpp = RES%fpp + ob_fpp
+    else
+       call term(L, R, RES, T221, T)
+       call add_onebody(RES, T, .false.)
+    end if
+    have221 = .false.
   end subroutine gpu_commutator_221
 
   ! the device forms the Pandya matrices of both operands itself, so this takes

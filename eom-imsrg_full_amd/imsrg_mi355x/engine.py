@@ -81,6 +81,7 @@ _SIG = {
     "imsrg_commutator_220": (C.c_int, [_vp, _vp, _dp]),
     "imsrg_cross_coupled": (C.c_int, [_vp, _dp, _i64]),
     "imsrg_cross_coupled_size": (_i64, [_vp]),
+    "imsrg_commutator_222_ph_cc": (C.c_int, [_vp, _dp, C.c_int, _dp, C.c_int, _i64, _vp]),
     "imsrg_generator": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "imsrg_generator_atanxx": (C.c_int, [_vp, _vp, _vp, _vp]),
     "imsrg_bch_expand": (C.c_int, [_vp, _vp, _vp, _vp, _ip]),

@@ -176,6 +176,15 @@ int imsrg_commutator_220(imsrg_op L, imsrg_op R, double *out);
  * copied to the host column-major per non-empty CC block in reference order */
 int imsrg_cross_coupled(imsrg_op op, double *host_out, int64_t n);
 int64_t imsrg_cross_coupled_size(imsrg_ctx ctx);
+/* commutator_222_ph(LCC, RCC, RES, WCC, jbas) (commutators.f90:771-925) on
+ * Pandya matrices the caller made (calculate_cross_coupled): lcc and rcc in
+ * the layout imsrg_cross_coupled writes (n = imsrg_cross_coupled_size),
+ * hL/hR their hermiticities (cc_mat%herm).  RES (herm -hL*hR) receives the
+ * 222_ph term alone, as imsrg_commutator with mask IMSRG_T222PH: one- and
+ * two-body parts overwritten, E0 untouched.  The workspace WCC of the
+ * reference is the context's.  nbatch = 1. */
+int imsrg_commutator_222_ph_cc(imsrg_ctx ctx, const double *lcc, int hL, const double *rcc, int hR, int64_t n,
+                               imsrg_op RES);
 
 /* Adjoint commutator (no counterpart in the reference; the gradient path):
  * Ybar = d <Cbar, [P, Y]> / dY for Y of hermiticity hY, for the terms in

@@ -16,6 +16,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "build", "liboracle_imsrg.so")
 REF_HARNESS = os.path.join(HERE, "_ref", "ref_harness")
+# the same harness linked against the product's substitute module commutators
+SUB_HARNESS = os.path.join(HERE, "_ref", "sub_harness")
 
 _lib = None
 
@@ -401,10 +403,12 @@ def ref_available():
 
 
 class RefRunner:
-    """Runs oracle/_ref/ref_harness on operator streams in a temp directory."""
+    """Runs oracle/_ref/ref_harness (or another build of it, `binary`) on
+    operator streams in a temp directory."""
 
-    def __init__(self, basis, threads=None):
+    def __init__(self, basis, threads=None, binary=None):
         self.basis = basis
+        self.binary = binary or REF_HARNESS
         self.tmp = tempfile.mkdtemp(prefix="refh_")
         self.bfile = os.path.join(self.tmp, "basis.txt")
         basis.write_basis_file(self.bfile)
@@ -422,7 +426,7 @@ class RefRunner:
         return np.fromfile(os.path.join(self.tmp, name), dtype=np.float64)
 
     def run(self, *args):
-        r = subprocess.run([REF_HARNESS, self.bfile] + [str(a) for a in args], cwd=self.tmp,
+        r = subprocess.run([self.binary, self.bfile] + [str(a) for a in args], cwd=self.tmp,
                            env=self.env, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"ref_harness {args[0]} failed: {r.stdout[-2000:]} {r.stderr[-2000:]}")
@@ -490,6 +494,16 @@ class RefRunner:
                 f.write(" ".join(str(int(x)) for x in p) + "\n")
         self.run("brute", hA, self._w("A.bin", A), hB, self._w("B.bin", B), pf, "bo.txt")
         return [float(ln.split()[-1]) for ln in open(os.path.join(self.tmp, "bo.txt")) if ln.strip()]
+
+    def decouple(self, kind, HS):
+        """magnus_decouple itself: (H(s), Omega(s), its printed step lines)"""
+        self.run("decouple", kind, self._w("HSd.bin", HS), "d")
+        rows = [ln.split() for ln in open(os.path.join(self.tmp, "d_0b_magnus_flow.dat")) if ln.strip()]
+        return self._r("d_HS.bin"), self._r("d_G.bin"), np.array([[float(x) for x in r] for r in rows])
+
+    def dhds(self, kind, HS):
+        self.run("dhds", kind, self._w("HSh.bin", HS), "dh.bin")
+        return self._r("dh.bin")
 
     def time_comm(self, hL, L, hR, R, reps=1):
         out = self.run("time", hL, self._w("L.bin", L), hR, self._w("R.bin", R), reps)

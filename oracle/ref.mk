@@ -21,7 +21,7 @@ OBJS := $(OUT)/anglib.o $(addprefix $(OUT)/,$(addsuffix .o,$(MODS))) $(OUT)/gene
 
 LIBDIR := $(abspath $(HERE)../eom-imsrg_full_amd/imsrg_mi355x/lib)
 
-all: $(OUT)/ref_harness $(if $(wildcard $(LIBDIR)/libimsrg_mi355x.so),$(OUT)/drop_in_test)
+all: $(OUT)/ref_harness $(if $(wildcard $(LIBDIR)/libimsrg_mi355x.so),$(OUT)/drop_in_test $(OUT)/sub_harness)
 
 $(OUT):
 	@mkdir -p $(OUT)
@@ -85,6 +85,36 @@ $(OUT)/imsrg_gpu.o: $(HERE)../eom-imsrg_full_amd/fortran/imsrg_gpu.f90 $(OUT)/cr
 $(OUT)/drop_in_test: $(HERE)../tests/fortran/drop_in_test.f90 $(OBJS) $(OUT)/harness_basis.o $(OUT)/imsrg_gpu.o $(LIBDIR)/libimsrg_mi355x.so
 	$(FC) $(FFLAGS) -o $@ $(HERE)../tests/fortran/drop_in_test.f90 $(OBJS) $(OUT)/harness_basis.o \
 	    $(OUT)/imsrg_gpu.o -J$(OUT) $(LIBS) -L$(LIBDIR) -limsrg_mi355x \
+	    -Wl,-rpath,'$$ORIGIN/../../eom-imsrg_full_amd/imsrg_mi355x/lib'
+
+# The link-time substitute: the product's `module commutators`
+# (eom-imsrg_full_amd/fortran/commutators_gpu.f90) in place of the reference's,
+# and the reference's callers of that module -- IMSRG_MAGNUS, generators,
+# IMSRG_ODE, brute_force_testing -- compiled unmodified against it into
+# $(SUB); ref_harness.f90 linked the same way (with -DSUBSTITUTE, which adds
+# the device generators to its generator choices) is sub_harness.
+SUB := $(OUT)/sub
+SUBFLAGS := -J$(SUB) -I$(SUB) -I$(OUT)
+SUBOBJS := $(OUT)/anglib.o $(addprefix $(OUT)/,$(addsuffix .o,gzipmod basic_IMSRG cross_coupled \
+        isospin_operators three_body_routines HF_mod TS_commutators tensor_products gaulag operators \
+        operator_commutators EOM_scalar_commutators EOM_TS_commutators EOM_dTZ_commutators adams_ode)) \
+        $(addprefix $(SUB)/,commutators.o IMSRG_MAGNUS.o brute_force_testing.o IMSRG_ODE.o generators.o)
+
+$(SUB)/commutators.o: $(HERE)../eom-imsrg_full_amd/fortran/commutators_gpu.f90 $(OUT)/cross_coupled.o
+	@mkdir -p $(SUB)
+	$(FC) $(FFLAGS) -c -o $@ $< $(SUBFLAGS)
+$(SUB)/IMSRG_MAGNUS.o: $(REFDIR)/IMSRG_MAGNUS.f90 $(SUB)/commutators.o $(OUT)/HF_mod.o $(OUT)/TS_commutators.o $(OUT)/operators.o $(OUT)/operator_commutators.o
+	$(FC) $(FFLAGS) -c -o $@ $< $(SUBFLAGS)
+$(SUB)/brute_force_testing.o: $(REFDIR)/brute_force_testing.f90 $(SUB)/commutators.o $(OUT)/EOM_scalar_commutators.o $(OUT)/EOM_TS_commutators.o $(OUT)/EOM_dTZ_commutators.o $(OUT)/TS_commutators.o $(OUT)/operator_commutators.o $(OUT)/operators.o
+	$(FC) $(FFLAGS) -c -o $@ $< $(SUBFLAGS)
+$(SUB)/IMSRG_ODE.o: $(REFDIR)/IMSRG_ODE.f90 $(SUB)/commutators.o $(OUT)/adams_ode.o $(OUT)/operators.o
+	$(FC) $(FFLAGS) -c -o $@ $< $(SUBFLAGS)
+$(SUB)/generators.o: $(REFDIR)/generators.f90 $(SUB)/commutators.o
+	$(FC) $(FFLAGS) -c -o $@ $< $(SUBFLAGS)
+
+$(OUT)/sub_harness: $(HERE)ref_harness.f90 $(SUBOBJS) $(OUT)/harness_basis.o $(LIBDIR)/libimsrg_mi355x.so
+	$(FC) $(FFLAGS) -cpp -DSUBSTITUTE -o $@ $(HERE)ref_harness.f90 $(SUBOBJS) $(OUT)/harness_basis.o \
+	    $(SUBFLAGS) $(LIBS) -L$(LIBDIR) -limsrg_mi355x \
 	    -Wl,-rpath,'$$ORIGIN/../../eom-imsrg_full_amd/imsrg_mi355x/lib'
 
 clean:

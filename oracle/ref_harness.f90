@@ -26,6 +26,14 @@
 !   hf     H.bin out.bin D.bin [O.bin Oout.bin] -> calc_HF (no three-body force) on the bare H;
 !          D(old orbit, new orbit) as an nsp x nsp column-major stream; prints
 !          E_HF; with O, also sqop_to_HF(O) with those coefficients
+!   decouple kind HS.bin outprefix      -> magnus_decouple itself (IMSRG_MAGNUS.f90:15-128)
+!   dhds   kind HS.bin out.bin          -> dHds (IMSRG_ODE.f90:416-463): vectorize([eta(HS), HS])
+!
+! Built twice by oracle/ref.mk: ref_harness against the reference's own
+! module commutators, and sub_harness (-DSUBSTITUTE) against the product's
+! substitute for it (eom-imsrg_full_amd/fortran/commutators_gpu.f90), with
+! IMSRG_MAGNUS, generators and IMSRG_ODE compiled unmodified against that;
+! sub_harness also takes the device generators gpu_white, gpu_atan, gpu_w2.
 program ref_harness
   use basic_IMSRG
   use cross_coupled
@@ -39,7 +47,10 @@ program ref_harness
   type(spd) :: jbas
   type(sq_op) :: HS
   character(len=1024) :: basisfile, cmd
-  external :: build_gs_white, build_gs_w2, build_gs_atan
+  external :: build_gs_white, build_gs_w2, build_gs_atan, dHds
+#ifdef SUBSTITUTE
+  external :: gpu_build_gs_white, gpu_build_gs_w2, gpu_build_gs_atan
+#endif
 
   call getarg(1, basisfile)
   call getarg(2, cmd)
@@ -81,6 +92,10 @@ program ref_harness
      call do_time()
   case ('hf')
      call do_hf()
+  case ('decouple')
+     call do_decouple()
+  case ('dhds')
+     call do_dhds()
   case default
      print*, 'unknown command ', trim(cmd)
      stop 2
@@ -263,10 +278,73 @@ contains
        call build_gs_atan(H, ETA, jbas)
     case ('w2')
        call build_gs_w2(H, ETA, jbas)
+#ifdef SUBSTITUTE
+    case ('gpu_white')
+       call gpu_build_gs_white(H, ETA, jbas)
+    case ('gpu_atan')
+       call gpu_build_gs_atan(H, ETA, jbas)
+    case ('gpu_w2')
+       call gpu_build_gs_w2(H, ETA, jbas)
+#endif
     case default
        stop 'bad generator'
     end select
   end subroutine run_gen
+
+  subroutine do_decouple()
+    ! magnus_decouple (IMSRG_MAGNUS.f90:15-128) with the generator passed as
+    ! its external build_generator; writes H(s), Omega(s) and the step lines
+    ! it prints into <prefix>_0b_magnus_flow.dat
+    type(sq_op) :: G
+    integer :: neq0
+    neq0 = HS%neq
+    HS%herm = 1
+    call read_op(trim(sarg(4)), HS)
+    prefix = trim(sarg(5))
+    select case (trim(sarg(3)))
+    case ('white')
+       call magnus_decouple(HS, G, jbas, 'n', 'n', build_gs_white)
+    case ('w2')
+       call magnus_decouple(HS, G, jbas, 'n', 'n', build_gs_w2)
+#ifdef SUBSTITUTE
+    case ('gpu_white')
+       call magnus_decouple(HS, G, jbas, 'n', 'n', gpu_build_gs_white)
+    case ('gpu_w2')
+       call magnus_decouple(HS, G, jbas, 'n', 'n', gpu_build_gs_w2)
+#endif
+    case default
+       stop 'bad generator'
+    end select
+    HS%neq = neq0
+    call write_op(trim(sarg(5))//'_HS.bin', HS)
+    call write_op(trim(sarg(5))//'_G.bin', G)
+  end subroutine do_decouple
+
+  subroutine do_dhds()
+    ! dHds (IMSRG_ODE.f90:416-463) at HS with the generator `kind`
+    real(8) :: t
+    real(8), allocatable :: yy(:), yp(:)
+    HS%herm = 1
+    call read_op(trim(sarg(4)), HS)
+    allocate(yy(HS%neq), yp(HS%neq))
+    call vectorize(HS, yy)
+    t = 0.d0
+    select case (trim(sarg(3)))
+    case ('white')
+       call dHds(t, yy, yp, HS, jbas, build_gs_white)
+    case ('w2')
+       call dHds(t, yy, yp, HS, jbas, build_gs_w2)
+#ifdef SUBSTITUTE
+    case ('gpu_white')
+       call dHds(t, yy, yp, HS, jbas, gpu_build_gs_white)
+#endif
+    case default
+       stop 'bad generator'
+    end select
+    open(unit=42, file=trim(sarg(5)), access='stream', form='unformatted', status='replace')
+    write(42) yp
+    close(42)
+  end subroutine do_dhds
 
   subroutine do_gen()
     type(sq_op) :: H, ETA
