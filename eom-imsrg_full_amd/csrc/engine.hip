@@ -1816,6 +1816,60 @@ int imsrg_op_read_stream(imsrg_op op, const char *path) {
   return imsrg_op_upload(op, v.data(), n);
 }
 
+// interaction files (interaction_IO.f90): the operator's image to the host,
+// the reader's writes into it, back to the device
+int imsrg_read_interaction(imsrg_op H, int format, const char *path, const char *path_1b,
+                           const imsrg_interaction_params *p) {
+  if (!H || !path || !p) return fail(IMSRG_EINVAL, "null argument");
+  if (format == IMSRG_INT_ME2B && !path_1b) return fail(IMSRG_EINVAL, "me2b needs its one-body file");
+  imsrg_ctx_s *c = H->ctx;
+  if (int rc = single_only(c, "imsrg_read_interaction")) return rc;
+  if (H->herm != 1) return fail(IMSRG_EINVAL, "imsrg_read_interaction: H must be hermitian");
+  ReadParams rp;
+  rp.hw = p->hw;
+  rp.com_hw = p->com_hw;
+  rp.lawson_beta = p->lawson_beta;
+  rp.A = p->A;
+  rp.htype = p->htype;
+  rp.emax_file = p->emax_file;
+  rp.lmax_file = p->lmax_file;
+  const Layout &Ly = c->lay;
+  HIPCHK(hipMemcpyAsync(c->hstage, H->d, sizeof(double) * Ly.op_size, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::string err;
+  if (read_interaction_file(Ly, format, path, path_1b, rp, c->hstage, err) != 0)
+    return fail(IMSRG_EINVAL, "imsrg_read_interaction: " + err);
+  HIPCHK(hipMemcpyAsync(H->d, c->hstage, sizeof(double) * Ly.op_size, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// the same readers on a host vectorize stream (no device involved)
+int imsrg_read_interaction_vec(int nsp, const int *n, const int *l, const int *jj, const int *tz, const int *con,
+                               int format, const char *path, const char *path_1b,
+                               const imsrg_interaction_params *p, double *vec, int64_t neq) {
+  if (!n || !l || !jj || !tz || !con || !path || !p || !vec) return fail(IMSRG_EINVAL, "null argument");
+  if (format == IMSRG_INT_ME2B && !path_1b) return fail(IMSRG_EINVAL, "me2b needs its one-body file");
+  Layout Ly;
+  std::string err = Ly.build(nsp, n, l, jj, tz, con);
+  if (!err.empty()) return fail(IMSRG_EINVAL, err);
+  if (neq != Ly.neq) return fail(IMSRG_EINVAL, "stream length must be the basis' neq");
+  ReadParams rp;
+  rp.hw = p->hw;
+  rp.com_hw = p->com_hw;
+  rp.lawson_beta = p->lawson_beta;
+  rp.A = p->A;
+  rp.htype = p->htype;
+  rp.emax_file = p->emax_file;
+  rp.lmax_file = p->lmax_file;
+  std::vector<double> img(Ly.op_size, 0.0);
+  Ly.unpack_vec(vec, 1, img.data());
+  if (read_interaction_file(Ly, format, path, path_1b, rp, img.data(), err) != 0)
+    return fail(IMSRG_EINVAL, "imsrg_read_interaction_vec: " + err);
+  Ly.pack_vec(img.data(), vec);
+  return 0;
+}
+
 // rec: 8 doubles per instance (s, E0, E0+MBPT2, crit, Magnus terms, BCH
 // terms, steps taken, 1 if the instance stepped in this call else 0)
 int imsrg_flow_step(imsrg_flow f, double *rec) {

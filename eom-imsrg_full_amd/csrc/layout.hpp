@@ -121,6 +121,19 @@ struct Layout {
 
 double racah6j(int j1, int j2, int j3, int j4, int j5, int j6);  // doubled arguments
 
+// Interaction files (interaction_IO.cpp): the reference's H%hospace, H%com_hw,
+// H%lawson_beta, Aneut + Aprot and ham_type, and for me2j files the file's own
+// single-particle basis cuts.
+struct ReadParams {
+  double hw = 0.0, com_hw = 0.0, lawson_beta = 0.0;
+  int A = 0, htype = 0, emax_file = 0, lmax_file = 0;
+};
+// Sets the matrix elements the file lists in `img`, the host image of one
+// operator in this layout; format 0 ascii, 1 gz, 2 me2j, 3 me2b (path2 = its
+// one-body .me1b file).  0 or -1 with err.
+int read_interaction_file(const Layout &L, int format, const char *path, const char *path2, const ReadParams &p,
+                          double *img, std::string &err);
+
 // a zero J' term that pads a quadruple's run to an even length
 constexpr uint32_t kPhPadTerm = 0xFFFFFFFFu;
 

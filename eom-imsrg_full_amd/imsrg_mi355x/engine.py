@@ -39,6 +39,15 @@ def _load():
 
 _lib = _load()
 _vp, _ip, _dp, _i64 = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int64
+
+
+class InteractionParams(C.Structure):
+    """imsrg_interaction_params"""
+    _fields_ = [("hw", C.c_double), ("com_hw", C.c_double), ("lawson_beta", C.c_double), ("A", C.c_int),
+                ("htype", C.c_int), ("emax_file", C.c_int), ("lmax_file", C.c_int)]
+
+
+INTERACTION_FORMATS = {"ascii": 0, "gz": 1, "me2j": 2, "me2b": 3}
 _SIG = {
     "imsrg_last_error": (C.c_char_p, []),
     "imsrg_ctx_create": (C.c_int, [C.c_int, _ip, _ip, _ip, _ip, _ip, C.c_int, C.POINTER(_vp)]),
@@ -101,6 +110,9 @@ _SIG = {
     "imsrg_transform_bch": (C.c_int, [_vp, _vp, C.POINTER(C.c_int)]),
     "imsrg_hartree_fock": (C.c_int, [_vp, _dp, C.POINTER(C.c_int), _dp]),
     "imsrg_hartree_fock_noalloc": (C.c_int, [_vp, _dp, C.POINTER(C.c_int), _dp]),
+    "imsrg_read_interaction": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_char_p, C.POINTER(InteractionParams)]),
+    "imsrg_read_interaction_vec": (C.c_int, [C.c_int, _ip, _ip, _ip, _ip, _ip, C.c_int, C.c_char_p, C.c_char_p,
+                                             C.POINTER(InteractionParams), _dp, _i64]),
     "imsrg_observable_to_hf": (C.c_int, [_vp, _dp]),
     "imsrg_dhds": (C.c_int, [_vp, C.c_int, _vp]),
     "imsrg_ctx_profile_read": (C.c_int, [_vp, _dp, C.POINTER(C.c_longlong), C.c_int, C.c_int]),
@@ -415,6 +427,15 @@ class Operator:
         _chk(_lib.imsrg_observable_to_hf(self.h, _dptr(D)))
         return self
 
+    def read_interaction(self, fmt, path, path_1b=None, hw=0.0, com_hw=0.0, lawson_beta=0.0, A=0, htype=0,
+                         emax_file=0, lmax_file=0):
+        """set the two-body matrix elements of an interaction file (interaction_IO.f90
+        formats: ascii, gz, me2j, me2b) into this hermitian operator, in place"""
+        p = InteractionParams(hw, com_hw, lawson_beta, A, htype, emax_file, lmax_file)
+        _chk(_lib.imsrg_read_interaction(self.h, INTERACTION_FORMATS[fmt], str(path).encode(),
+                                         str(path_1b).encode() if path_1b else None, C.byref(p)))
+        return self
+
     def normal_order_adjoint(self):
         """transpose of normal ordering applied to this cotangent (a new operator)"""
         out = Operator(self.ctx, self.herm)
@@ -427,6 +448,19 @@ class Operator:
         v = np.zeros(self.ctx.neq * B)
         _chk(_lib.imsrg_op_download_cotangent(self.h, _dptr(v), v.size))
         return v if B == 1 else v.reshape(B, self.ctx.neq)
+
+
+def read_interaction_vec(basis, fmt, path, path_1b=None, vec=None, hw=0.0, com_hw=0.0, lawson_beta=0.0, A=0,
+                         htype=0, emax_file=0, lmax_file=0):
+    """an interaction file (interaction_IO.f90 formats: ascii, gz, me2j, me2b) read
+    on the host into a vectorize stream of `basis` (zeros, or a copy of vec)"""
+    n, l, jj, tz, con = (np.ascontiguousarray(x, dtype=np.int32) for x in basis.orbit_table().T)
+    v = np.zeros(basis.neq) if vec is None else np.array(vec, dtype=np.float64)
+    p = InteractionParams(hw, com_hw, lawson_beta, A, htype, emax_file, lmax_file)
+    _chk(_lib.imsrg_read_interaction_vec(basis.nsp, _iptr(n), _iptr(l), _iptr(jj), _iptr(tz), _iptr(con),
+                                         INTERACTION_FORMATS[fmt], str(path).encode(),
+                                         str(path_1b).encode() if path_1b else None, C.byref(p), _dptr(v), v.size))
+    return v
 
 
 def commutator_110(L, R):

@@ -267,6 +267,41 @@ int imsrg_flow_destroy(imsrg_flow flow);
 int imsrg_op_write_stream(imsrg_op op, const char *path);
 int imsrg_op_read_stream(imsrg_op op, const char *path);
 
+/* ---- interaction files (interaction_IO.f90) --------------------------------
+ * The two-body matrix elements a file lists are set into H (herm +1, single
+ * instance) as the reference's reader sets them: each element and its
+ * hermitian mirror, in file order (a repeated element keeps the last value);
+ * everything else in H is kept, except that IMSRG_INT_ME2B also sets E0 and
+ * the one-body part from its one-body file.  Orbit labels follow each
+ * format's convention against the context's orbits (matched by n, l, 2j,
+ * 2tz where the file names orbits by quantum numbers). */
+#define IMSRG_INT_ASCII 0  /* read_interaction      interaction_IO.f90:8-107: 8 header lines, then
+                              "Tz Par J a b c d V" (orbits 1-based, J doubled) */
+#define IMSRG_INT_GZ 1     /* read_gz               :110-231: gzip, a count, then one element per line */
+#define IMSRG_INT_ME2J 2   /* read_me2j_interaction :282-568: gzip, isospin-coupled values ten per line
+                              over the file's basis (hk order up to emax_file, lmax_file), to pn */
+#define IMSRG_INT_ME2B 3   /* read_me2b_interaction :570-1151: gzip two-body blocks, then E0 and the
+                              one-body part from path_1b */
+typedef struct {
+  double hw;          /* H%hospace (MeV) */
+  double com_hw;      /* H%com_hw */
+  double lawson_beta; /* H%lawson_beta: adds beta (p1.p2 + com_hw^2/hw^2 r1.r2) hw/A */
+  int A;              /* H%Aneut + H%Aprot */
+  int htype;          /* ham_type: 1 subtracts the centre-of-mass kinetic term p1.p2 hw/A */
+  int emax_file;      /* IMSRG_INT_ME2J: the file's eMax and lMax */
+  int lmax_file;
+} imsrg_interaction_params;
+int imsrg_read_interaction(imsrg_op H, int format, const char *path, const char *path_1b,
+                           const imsrg_interaction_params *p);
+/* The same readers on a host vectorize stream (basis_IMSRG.f90:3794-3918
+ * order, neq = the basis' stream length) instead of a device operator: vec in
+ * = the operator before reading (zeros for a fresh one), out = after.  The
+ * basis is given as for imsrg_ctx_create; no device is touched, so a Fortran
+ * caller can repackage the result into an sq_op. */
+int imsrg_read_interaction_vec(int nsp, const int *n, const int *l, const int *jj, const int *tz, const int *con,
+                               int format, const char *path, const char *path_1b,
+                               const imsrg_interaction_params *p, double *vec, int64_t neq);
+
 /* ---- measurement (no counterpart in the reference, which only prints
  * omp_get_wtime, main_IMSRG.f90:46,308-309) ---------------------------------
  * While enabled, every engine kernel launch is bracketed by HIP events on the

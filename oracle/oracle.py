@@ -505,6 +505,18 @@ class RefRunner:
         self.run("dhds", kind, self._w("HSh.bin", HS), "dh.bin")
         return self._r("dh.bin")
 
+    def readint(self, fmt, path, hw=0.0, com_hw=0.0, beta=0.0, A=0, htype=0, emax_file=0, lmax_file=0):
+        """the reference's reader `fmt` on path (copied beside the harness; an me2b
+        file brings its .me1b file) into a zero operator"""
+        import shutil
+        name = os.path.basename(path)
+        shutil.copy(path, os.path.join(self.tmp, name))
+        if fmt == "me2b":
+            shutil.copy(path[:-5] + "1b.gz", os.path.join(self.tmp, name[:-5] + "1b.gz"))
+        self.run("readint", fmt, name, "ri.bin", repr(float(hw)), repr(float(com_hw)), repr(float(beta)), A, htype,
+                 emax_file, lmax_file)
+        return self._r("ri.bin")
+
     def time_comm(self, hL, L, hR, R, reps=1):
         out = self.run("time", hL, self._w("L.bin", L), hR, self._w("R.bin", R), reps)
         for ln in out.splitlines():

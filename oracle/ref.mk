@@ -17,7 +17,7 @@ MODS := gzipmod basic_IMSRG cross_coupled commutators isospin_operators \
         three_body_routines HF_mod TS_commutators tensor_products gaulag operators \
         operator_commutators IMSRG_MAGNUS EOM_scalar_commutators EOM_TS_commutators \
         EOM_dTZ_commutators brute_force_testing adams_ode IMSRG_ODE
-OBJS := $(OUT)/anglib.o $(addprefix $(OUT)/,$(addsuffix .o,$(MODS))) $(OUT)/generators.o
+OBJS := $(OUT)/anglib.o $(addprefix $(OUT)/,$(addsuffix .o,$(MODS))) $(OUT)/generators.o $(OUT)/interaction_IO.o
 
 LIBDIR := $(abspath $(HERE)../eom-imsrg_full_amd/imsrg_mi355x/lib)
 
@@ -70,6 +70,8 @@ $(OUT)/IMSRG_ODE.o: $(REFDIR)/IMSRG_ODE.f90 $(OUT)/adams_ode.o $(OUT)/operators.
 	$(FC) $(FFLAGS) -c -o $@ $< -J$(OUT)
 $(OUT)/generators.o: $(REFDIR)/generators.f90 $(OUT)/commutators.o
 	$(FC) $(FFLAGS) -c -o $@ $< -J$(OUT)
+$(OUT)/interaction_IO.o: $(REFDIR)/interaction_IO.f90 $(OUT)/basic_IMSRG.o
+	$(FC) $(FFLAGS) -c -o $@ $< -J$(OUT)
 
 $(OUT)/harness_basis.o: $(HERE)harness_basis.f90 $(OUT)/basic_IMSRG.o
 	$(FC) $(FFLAGS) -c -o $@ $< -J$(OUT)
@@ -97,7 +99,8 @@ SUB := $(OUT)/sub
 SUBFLAGS := -J$(SUB) -I$(SUB) -I$(OUT)
 SUBOBJS := $(OUT)/anglib.o $(addprefix $(OUT)/,$(addsuffix .o,gzipmod basic_IMSRG cross_coupled \
         isospin_operators three_body_routines HF_mod TS_commutators tensor_products gaulag operators \
-        operator_commutators EOM_scalar_commutators EOM_TS_commutators EOM_dTZ_commutators adams_ode)) \
+        operator_commutators EOM_scalar_commutators EOM_TS_commutators EOM_dTZ_commutators adams_ode \
+        interaction_IO)) \
         $(addprefix $(SUB)/,commutators.o IMSRG_MAGNUS.o brute_force_testing.o IMSRG_ODE.o generators.o)
 
 $(SUB)/commutators.o: $(HERE)../eom-imsrg_full_amd/fortran/commutators_gpu.f90 $(OUT)/cross_coupled.o

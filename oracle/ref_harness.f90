@@ -28,6 +28,9 @@
 !          E_HF; with O, also sqop_to_HF(O) with those coefficients
 !   decouple kind HS.bin outprefix      -> magnus_decouple itself (IMSRG_MAGNUS.f90:15-128)
 !   dhds   kind HS.bin out.bin          -> dHds (IMSRG_ODE.f90:416-463): vectorize([eta(HS), HS])
+!   readint fmt file out.bin hw com_hw beta A htype emax_file lmax_file
+!                                        -> the reference's interaction reader fmt (ascii, gz, me2j,
+!          me2b) on file into a zero operator (interaction_IO.f90), written as a stream
 !
 ! Built twice by oracle/ref.mk: ref_harness against the reference's own
 ! module commutators, and sub_harness (-DSUBSTITUTE) against the product's
@@ -41,6 +44,7 @@ program ref_harness
   use IMSRG_MAGNUS
   use brute_force_testing
   use HF_mod
+  use interaction_IO
   use harness_basis
   implicit none
 
@@ -96,6 +100,8 @@ program ref_harness
      call do_decouple()
   case ('dhds')
      call do_dhds()
+  case ('readint')
+     call do_readint()
   case default
      print*, 'unknown command ', trim(cmd)
      stop 2
@@ -319,6 +325,77 @@ contains
     call write_op(trim(sarg(5))//'_HS.bin', HS)
     call write_op(trim(sarg(5))//'_G.bin', G)
   end subroutine do_decouple
+
+  subroutine do_readint()
+    ! the reference's readers with their globals set: TBME_DIR/intfile name
+    ! the file, H carries hw, com_hw, the Lawson beta and A; for me2j the
+    ! file's basis jbx is the hk list up to emax_file / lmax_file (what
+    ! read_sp_basis builds from the file's .sps), jbx%con its orbit's index
+    ! in jbas
+    type(spd) :: jbx
+    integer :: htype, ef, lf, e, l, jj, tz, ix, i
+    TBME_DIR = ''
+    intfile = trim(sarg(4))
+    HS%herm = 1
+    call clear_sq_op(HS)
+    HS%hospace = rarg(6)
+    HS%com_hw = rarg(7)
+    HS%lawson_beta = rarg(8)
+    HS%Aprot = iarg(9)
+    HS%Aneut = 0
+    htype = iarg(10)
+    ef = iarg(11)
+    lf = iarg(12)
+    HS%lmax = maxval(jbas%ll)
+    select case (trim(sarg(3)))
+    case ('ascii')
+       call read_interaction(HS, jbas, htype)
+    case ('gz')
+       call read_gz(HS, jbas, htype)
+    case ('me2j')
+       ix = 0
+       do e = 0, ef
+          do l = mod(e, 2), min(e, lf), 2
+             do jj = abs(2*l - 1), 2*l + 1, 2
+                do tz = -1, 1, 2
+                   ix = ix + 1
+                end do
+             end do
+          end do
+       end do
+       jbx%total_orbits = ix
+       allocate(jbx%nn(ix), jbx%ll(ix), jbx%jj(ix), jbx%itzp(ix), jbx%con(ix))
+       ix = 0
+       do e = 0, ef
+          do l = mod(e, 2), min(e, lf), 2
+             do jj = abs(2*l - 1), 2*l + 1, 2
+                do tz = -1, 1, 2
+                   ix = ix + 1
+                   jbx%nn(ix) = (e - l) / 2 ; jbx%ll(ix) = l ; jbx%jj(ix) = jj ; jbx%itzp(ix) = tz
+                   jbx%con(ix) = 0
+                   do i = 1, jbas%total_orbits
+                      if (jbas%nn(i) == jbx%nn(ix) .and. jbas%ll(i) == l .and. jbas%jj(i) == jj .and. &
+                           jbas%itzp(i) == tz) jbx%con(ix) = i
+                   end do
+                end do
+             end do
+          end do
+       end do
+       call read_me2j_interaction(HS, jbas, jbx, htype)
+    case ('me2b')
+       call read_me2b_interaction(HS, jbas, htype)
+    case default
+       stop 'bad interaction format'
+    end select
+    call write_op(trim(sarg(5)), HS)
+  end subroutine do_readint
+
+  real(8) function rarg(k)
+    integer :: k
+    character(len=64) :: s
+    call getarg(k, s)
+    read(s,*) rarg
+  end function rarg
 
   subroutine do_dhds()
     ! dHds (IMSRG_ODE.f90:416-463) at HS with the generator `kind`
