@@ -67,6 +67,7 @@ struct PhChunk {
   int n;            // outputs, 1..64
   int b0, len;      // the row key's packed 6j list (ap6_val) start and length
   int half;         // 1 when i == j (the outputs carry 1/sqrt 2)
+  long long tbase;  // the row's first J' term: descriptors hold term offsets relative to it
 };
 
 // A plain GEMM problem C = A * B (A M x K, B K x N, C M x N, all row-major)
@@ -177,6 +178,7 @@ void launch_seg_gemm_z(hipStream_t s, int mode, const SegProb *probs, const int4
                        int nact, long long scc, long long sz);
 void launch_pandya(hipStream_t s, const DevLayout &L, const double *op, double *cc, const int *act, int nact);
 size_t pandya_lds_bytes(const DevLayout &L);  // per-workgroup LDS of the Pandya kernel
+hipError_t pandya_allow_lds(size_t bytes);
 // 122: M = A_L R - A_R L over whole channel blocks (diagonal symmetrised);
 // launch_sym_mirror later forms RES = M + herm*M^T
 void launch_122(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES,

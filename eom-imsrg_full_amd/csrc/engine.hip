@@ -421,10 +421,21 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
       }
   D.nphp = (int)php_a.size();
   D.nsp = Ly.nsp; D.nJ = Ly.nJ; D.nhalf = Ly.nhalf; D.nblocks = Ly.nblocks;
-  if (pandya_lds_bytes(D) > 65536) {
-    imsrg_ctx_destroy(c);
-    return fail(IMSRG_EINVAL, "basis too large: the Pandya kernel stages " + std::to_string(pandya_lds_bytes(D)) +
-                                  " bytes of tables per workgroup (limit 65536)");
+  {
+    // the Pandya kernel stages its tables in LDS: up to the device's
+    // per-workgroup limit (160 KiB on gfx950), above 64 KiB by request
+    int lds_max = 65536;
+    (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+    const size_t need = pandya_lds_bytes(D);
+    if (need > (size_t)lds_max) {
+      imsrg_ctx_destroy(c);
+      return fail(IMSRG_EINVAL, "basis too large: the Pandya kernel stages " + std::to_string(need) +
+                                    " bytes of tables per workgroup (device limit " + std::to_string(lds_max) + ")");
+    }
+    if (need > 65536 && pandya_allow_lds(need) != hipSuccess) {
+      imsrg_ctx_destroy(c);
+      return fail(IMSRG_EDEVICE, "the Pandya kernel cannot take " + std::to_string(need) + " bytes of LDS");
+    }
   }
   std::vector<int> same_ob, ob_same(Ly.ob_p.size(), -1);
   for (size_t t = 0; t < Ly.ob_p.size(); t++) {
@@ -507,6 +518,7 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
         h.b0 = Ly.ap6_base[key];
         h.len = Ly.ap6_base[key + 1] - Ly.ap6_base[key];
         h.half = i == j;
+        h.tbase = Ly.ph_tbase[r];
         chunks.push_back(h);
         wa.push_back(16.0 + h.n);
       }

@@ -948,6 +948,11 @@ size_t pandya_lds_bytes(const DevLayout &L) {
   return sizeof(double) * L.nhalf * L.nhalf * L.nJ + sizeof(long long) * L.nblocks +
          sizeof(int) * (2 * L.nJ * L.nsp + L.nblocks + L.nsp);
 }
+// lets k_pandya take more than 64 KiB of dynamic LDS (up to the device's per-workgroup limit)
+hipError_t pandya_allow_lds(size_t bytes) {
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pandya), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes);
+}
 void launch_pandya(hipStream_t s, const DevLayout &L, const double *op, double *cc, const int *act, int nact) {
   if (L.nccol <= 0 || nact <= 0) return;
   hipLaunchKernelGGL(k_pandya, dim3(L.nccol, nact), dim3(256), pandya_lds_bytes(L), s, L, op, cc, act);
@@ -1110,11 +1115,11 @@ __global__ __launch_bounds__(256) void k_ph_terms(DevLayout L, const double *__r
 // two lane exchanges add each output's four partial sums.
 __device__ __forceinline__ void ph_decode(unsigned long long d, int &toff, int &n1, int &n, int &o1, int &o2,
                                           double &g1, double &g2, double &pre2) {
-  toff = (int)(d & 0x7FFFFFFull);
-  n1 = (int)((d >> 27) & 31);
-  n = n1 + (int)((d >> 32) & 31);
-  o1 = (int)((d >> 37) & 1023);
-  o2 = (int)((d >> 47) & 1023) - n1;
+  toff = (int)(d & 0x7FFFFFull);
+  n1 = (int)((d >> 23) & 31);
+  n = n1 + (int)((d >> 28) & 31);
+  o1 = (int)((d >> 33) & 4095);
+  o2 = (int)((d >> 45) & 4095) - n1;
   g1 = ((d >> 57) & 1) ? (((d >> 58) & 1) ? -1.0 : 1.0) : 0.0;
   g2 = ((d >> 59) & 1) ? 1.0 : 0.0;
   pre2 = ((d >> 60) & 1) ? .70710678118 : 1.0;
@@ -1135,7 +1140,7 @@ __global__ __launch_bounds__(64) void k_ph_apply(DevLayout L, const double *__re
   {
     const long long inst = inst_id(act);
     if (inst < 0) return;
-    Tbuf += inst * L.nphterm;
+    Tbuf += inst * L.nphterm + h.tbase;  // descriptors hold offsets relative to the row's first term
     RES += inst * L.op_size;
   }
   double *OV = RES + h.vrow;
@@ -1175,7 +1180,7 @@ __global__ __launch_bounds__(64) void k_ph_apply(DevLayout L, const double *__re
   double2 t[4][2];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
-    const int toff = (int)(d[u] & 0x7FFFFFFull), n = (int)((d[u] >> 27) & 31) + (int)((d[u] >> 32) & 31);
+    const int toff = (int)(d[u] & 0x7FFFFFull), n = (int)((d[u] >> 23) & 31) + (int)((d[u] >> 28) & 31);
     const double2 *T = reinterpret_cast<const double2 *>(Tbuf + toff);
 #pragma unroll
     for (int w = 0; w < 2; w++) t[u][w] = (m0 + 8 * w < n) ? T[(m0 >> 1) + 4 * w] : make_double2(0.0, 0.0);

@@ -462,13 +462,23 @@ std::string Layout::build(int nsp_, const int *n_, const int *l_, const int *jj_
     ap6_max = std::max(ap6_max, 2 * nJ);  // room for an empty run read from position 0
   }
   // Per upper element (ij,kl;J) of the ph apply, everything its output needs
-  // beyond the row: bits 0-26 first T1 term of its quadruple, 27-31 n1 (T1
-  // terms), 32-36 n2 (T2 terms), 37-46 / 47-56 start of the {ji jj J; jl jk J'}
-  // / {ji jj J; jk jl J'} runs in the row key's packed 6j list, 57 T1 sum used,
-  // 58 its sign is -1, 59 T2 sum used, 60 k == l (1/sqrt 2).
+  // beyond the row: bits 0-22 first T1 term of its quadruple (relative to the
+  // row's first term), 23-27 n1 (T1 terms), 28-32 n2 (T2 terms), 33-44 / 45-56
+  // start of the {ji jj J; jl jk J'} / {ji jj J; jk jl J'} runs in the row
+  // key's packed 6j list, 57 T1 sum used, 58 its sign is -1, 59 T2 sum used,
+  // 60 k == l (1/sqrt 2).
   {
     const int h = nhalf;
     ph_desc.assign(ph_toff.size(), 0);
+    // term offsets are stored relative to the row's smallest one: the terms of
+    // one row (i, j) all have quadruples (i, k, j, l), which the (i, k, j, l)
+    // order keeps within the terms of orbit i
+    ph_tbase.assign(npairs, 0);
+    for (int r = 0; r < npairs; r++) {
+      uint64_t mn = ~0ull;
+      for (int64_t u = ph_ubase[r]; u < ph_ubase[r + 1]; u++) mn = std::min<uint64_t>(mn, ph_toff[u]);
+      ph_tbase[r] = ph_ubase[r + 1] > ph_ubase[r] ? (int64_t)mn : 0;
+    }
     for (int qq = 0; qq < nblocks; qq++) {
       const int dim = bdim[qq], base = pbase[qq], J = bJ[qq];
       for (int r = 0; r < dim; r++) {
@@ -490,11 +500,11 @@ std::string Layout::build(int nsp_, const int *n_, const int *l_, const int *jj_
           o1 = v1 ? o1 : 0;
           o2 = v2 ? o2 : 0;
           const size_t u = ph_ubase[base + r] + (s2 - r);
-          const uint64_t toff = ph_toff[u];
-          if (toff >= (1ull << 27) || n1 > 31 || n2 > 31 || o1 > 1023 || o2 > 1023)
+          const uint64_t toff = ph_toff[u] - (uint64_t)ph_tbase[base + r];
+          if (toff >= (1ull << 23) || n1 > 31 || n2 > 31 || o1 > 4095 || o2 > 4095)
             return "basis too large for the ph apply descriptors";
           const bool neg1 = ((((jl + jk + J) >> 1) & 1) == 0);  // -(-1)^((jl+jk+J)/2) = -1
-          ph_desc[u] = toff | (n1 << 27) | (n2 << 32) | ((uint64_t)o1 << 37) | ((uint64_t)o2 << 47) |
+          ph_desc[u] = toff | (n1 << 23) | (n2 << 28) | ((uint64_t)o1 << 33) | ((uint64_t)o2 << 45) |
                        ((uint64_t)v1 << 57) | ((uint64_t)neg1 << 58) | ((uint64_t)v2 << 59) |
                        ((uint64_t)(k == l2) << 60);
         }

@@ -81,6 +81,7 @@ struct Layout {
   std::vector<uint32_t> ph_terms;   // quad<<6 | side<<5 | m (side 0: T1, 1: T2), or kPhPadTerm
   std::vector<uint32_t> ph_toff;    // per upper element: offset of its quadruple's T1 terms
   std::vector<int64_t> ph_ubase;    // per global pair row: first upper element
+  std::vector<int64_t> ph_tbase;    // per global pair row: the smallest ph_toff of its upper elements
   std::vector<uint64_t> ph_desc;    // per upper element: T offset, run lengths, 6j runs, flags
   std::vector<uint32_t> ph_zblk;    // per J' term: 2x2 Z block (units of 4 doubles)
   std::vector<uint8_t> ph_zsel;     // per J' term: element selector / flags

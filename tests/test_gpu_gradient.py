@@ -1,5 +1,7 @@
 """Gradient of E0(s_N) after N Magnus steps with respect to the bare
-Hamiltonian (the device adjoint flow, imsrg_flow_gradient):
+Hamiltonian (the device adjoint flow, imsrg_flow_gradient), with N the step
+at which the unperturbed flow converges (|MBPT2| < 1e-6), so the loss is the
+decoupled E0(s_final):
 
 * the transposes it is built from (normal ordering, repackage) satisfy the
   adjoint identity <C, A x> = <A^T C, x>;
@@ -55,8 +57,10 @@ def test_gradient_vs_central_differences_64_directions(e4):
     from imsrg_mi355x import autograd as AG
     from imsrg_mi355x import engine as E
     B, ctx, _ = e4
-    nsteps, ncheck = 16, 8
     h0 = synthetic_hamiltonian(B, seed=0)
+    nsteps = ctx.magnus_flow(ctx.op(1, h0).normal_order(), "white", 0.5, 200)["steps"]
+    ncheck = 8
+    print(f"[grad] emax=4 flow converges after {nsteps} steps", flush=True)
     e0, g = AG.energy_and_gradient(ctx, h0, "white", 0.5, nsteps, ncheck)
     D = np.stack([gaussian_direction(B, k) for k in range(64)])
     hs = np.concatenate([h0 + EPS * D, h0 - EPS * D])
@@ -81,15 +85,17 @@ def test_gradient_vs_oracle_reference_emax2():
     from imsrg_mi355x import engine as E
     B = Basis.ho(2, 2)
     ctx = E.Context(B)
-    nsteps, ncheck = 6, 3
     h0 = synthetic_hamiltonian(B, seed=0)
-    e0, g = AG.energy_and_gradient(ctx, h0, "white", 0.5, nsteps, ncheck)
     ob = O.OBasis(B)
+    nsteps = abs(O.magnus_flow(O.normal_order(ob.op(h0, 1)), "white", 0.5, 200)["steps"])
+    ncheck = 4
+    print(f"[grad] emax=2 flow converges after {nsteps} steps", flush=True)
+    e0, g = AG.energy_and_gradient(ctx, h0, "white", 0.5, nsteps, ncheck)
 
     def loss(h):
         H = O.normal_order(ob.op(h, 1))
         r = O.magnus_flow(H, "white", 0.5, max_steps=nsteps)
-        assert abs(r["steps"]) == nsteps  # not converged before s_N
+        assert abs(r["steps"]) == nsteps  # the perturbed flow converges at the same step
         return r["e0"][-1]
 
     assert abs(loss(h0) - e0) <= 1e-10 * abs(e0)
