@@ -1,0 +1,1111 @@
+// tensor.hip -- spherical tensor operators (rank > 0) and the scalar-tensor
+// commutator [G, T] with its BCH series on the device: allocate_tensor and
+// tensor_elem (basic_IMSRG.f90:644-907, 1504-1641), TS_commutator_111 ..
+// _222_ph (TS_commutators.f90:10-1849), the tensor Pandya transform
+// (calculate_single_pandya, cross_coupled.f90:1416-1634) and BCH_TENSOR
+// (IMSRG_MAGNUS.f90:389-465).
+//
+// Layout.  A tensor operator of rank R (doubled) and parity change dpar is one
+// FP64 buffer [E0 | pad][F: nsp x nsp row-major][V]: F holds f_tensor_elem(a, b)
+// for every orbit pair (the (hole, particle) entries are the herm-phased
+// (particle, hole) ones), V holds, per tensor block (J1 <= J2, Tz, P1) in the
+// reference's order, the FULL n1 x n2 row-major matrix between the scalar
+// channels (J1, Tz, P1) and (J2, Tz, P1 + dpar/2): rows and columns are the
+// channels' pair lists [pp | ph | hh] of the scalar layout, so the nine
+// reference sub-blocks tgam(1..9) tile it and every 222_pp_hh product is a
+// plain GEMM against the scalar full channel blocks of G.
+//
+// Kernels.  One thread per output element for the gathers (111, 121, 211, 221,
+// 122, 212, tensor Pandya, ph recoupling), FP64 MFMA for the products (the
+// pp/hh ladders and the two Pandya products of each cross-coupled block).  The
+// ph recoupling is a gather: every stored output element sums, over the
+// sixteen index patterns through which TS_commutator_222_ph scatters into it
+// and over the cross-coupled (J3, J4) blocks, the 9j-weighted product entries
+// that pass the reference's |W| > 1e-6 test.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "devfun.hpp"
+#include "engine_internal.hpp"
+
+using namespace imsrg;
+
+namespace {
+
+// ---------------------------------------------------------------- device view
+struct DevTensor {
+  int rank, dpar, nblk, ncc, nJ, nh, hx, nhole, npart;
+  long long nelem, nx, nr;      // tensor V elements; CCX / CCR elements
+  const int *holes, *parts;
+  const int *J1, *J2, *q1, *q2, *n1, *n2;
+  const long long *off, *tstart;  // per tensor block: V offset; first flat element
+  const int *bidx;                // [((J1/2)*nJ + J2/2)*6 + (Tz+1)*2 + P1] -> block (J1 <= J2) or -1
+  const int *cJ3, *cJ4, *cq3, *cq4;
+  const long long *xoff, *roff, *woff, *xstart, *rstart;
+  const int *cidx;                // [((J3/2)*nJ + J4/2)*4 + Tz*2 + P] -> cross-coupled block or -1
+  const int *ccol;                // [(scalar Pandya block * nsp + p) * nsp + h] -> column of (p,h)
+  const double *x6, *s6;
+  long long f_off, v_off;
+};
+
+// A product C (M x N, row-major, ldc) = sum of nseg segments alpha * A * B
+// with A(m,k) = X[asrc][a_off + m*a_rs + k*a_cs], B(k,n) = X[bsrc][b_off + k*b_rs + n*b_cs].
+struct TSeg {
+  long long a_off, b_off;
+  int K, a_rs, a_cs, b_rs, b_cs, asrc, bsrc;
+  double alpha;
+};
+struct TProb {
+  int M, N, ldc, nseg;
+  long long c_off;
+  TSeg seg[2];
+};
+
+
+// {J1 J2 R; j4 j5 j6} with J1, J2 integer (doubled), j4 up to jmax + R, j5, j6 of the basis
+__device__ __forceinline__ double tx6(const DevTensor &T, int J1, int J2, int j4, int j5, int j6) {
+  return T.x6[((((size_t)(J1 >> 1) * T.nJ + (J2 >> 1)) * T.hx + (j4 >> 1)) * T.nh + (j5 >> 1)) * T.nh + (j6 >> 1)];
+}
+// {c d J2; b x J4}: c, d, b of the basis, x up to jmax + R
+__device__ __forceinline__ double ts6(const DevTensor &T, int c, int d, int J2, int b, int x, int J4) {
+  return T.s6[(((((size_t)(c >> 1) * T.nh + (d >> 1)) * T.nJ + (J2 >> 1)) * T.nh + (b >> 1)) * T.hx + (x >> 1)) *
+                  T.nJ + (J4 >> 1)];
+}
+// ninej (basic_IMSRG.f90:4290-4308): {a b J1; c d J2; J3 J4 R} from three 6j per x
+__device__ inline double t9j(const DevTensor &T, int a, int b, int J1, int c, int d, int J2, int J3, int J4) {
+  const int R = T.rank;
+  const int xmin = max(max(abs(J2 - b), abs(a - R)), abs(J4 - c));
+  const int xmax = min(min(J2 + b, a + R), J4 + c);
+  double sm = 0.0;
+  for (int x = xmin; x <= xmax; x += 2)
+    sm -= (x + 1.0) * tx6(T, J1, J2, x, a, b) * ts6(T, c, d, J2, b, x, J4) * tx6(T, J3, J4, x, a, c);
+  return sm;
+}
+
+// tensor_elem (basic_IMSRG.f90:1504-1641): position in V of T^{J1x J2x}_{ax bx, cx dx}
+// and the factor applied to the stored value (switch phase, exchange phase, sqrt 2)
+__device__ inline long long t_pos(const DevLayout &L, const DevTensor &T, int herm, int ax, int bx, int cx, int dx,
+                                  int J1x, int J2x, double &pre) {
+  pre = 0.0;
+  int a = ax, b = bx, c = cx, d = dx, J1 = J1x, J2 = J2x;
+  bool sw = false;
+  if (J1x > J2x) {
+    a = cx; b = dx; c = ax; d = bx; J1 = J2x; J2 = J1x;
+    sw = true;
+  }
+  if (!dtri(J1, J2, T.rank)) return -1;
+  const int ja = L.jj[a], jb = L.jj[b], jc = L.jj[c], jd = L.jj[d];
+  if (!(dtri(ja, jb, J1) && dtri(jc, jd, J2))) return -1;
+  const int P = (L.l[a] + L.l[b]) & 1;
+  if (((L.l[c] + L.l[d]) & 1) != (P ^ ((T.dpar >> 1) & 1))) return -1;
+  const int T2 = L.tz[a] + L.tz[b];
+  if (L.tz[c] + L.tz[d] != T2) return -1;
+  const int q = T.bidx[((J1 >> 1) * T.nJ + (J2 >> 1)) * 6 + (T2 / 2 + 1) * 2 + P];
+  if (q < 0) return -1;
+  double f = sw ? (double)(sgnp((J2 - J1) >> 1) * herm) : 1.0;
+  int r, s;
+  if (a > b) {
+    r = dpos(L, J1, b, a);
+    f *= sgnp(1 + ((ja + jb - J1) >> 1));
+  } else {
+    if (a == b) f *= M_SQRT2;
+    r = dpos(L, J1, a, b);
+  }
+  if (c > d) {
+    s = dpos(L, J2, d, c);
+    f *= sgnp(1 + ((jc + jd - J2) >> 1));
+  } else {
+    if (c == d) f *= M_SQRT2;
+    s = dpos(L, J2, c, d);
+  }
+  if ((r | s) < 0) return -1;
+  pre = f;
+  return T.off[q] + (long long)r * T.n2[q] + s;
+}
+__device__ __forceinline__ double t_elem(const DevLayout &L, const DevTensor &T, const double *V, int herm, int a,
+                                         int b, int c, int d, int J1, int J2) {
+  double f;
+  const long long p = t_pos(L, T, herm, a, b, c, d, J1, J2, f);
+  return p < 0 ? 0.0 : V[p] * f;
+}
+
+// flat element e of a prefix table (n + 1 entries) -> item
+__device__ __forceinline__ int find_item(const long long *start, int n, long long e) {
+  int lo = 0, hi = n;  // start[lo] <= e < start[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (start[mid] <= e) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------- one-body terms
+// TS_commutator_111 (TS_commutators.f90:10-55): overwrites fhh, fph, fpp of RES.
+// LF: the scalar operator's full one-body matrix, RF the tensor's.
+__global__ void k_t111(DevLayout L, DevTensor T, const double *__restrict__ LF, const double *__restrict__ RF,
+                       double *__restrict__ F, int hL, int hR, int hres) {
+  const int A = T.nhole, P = T.npart, n = L.nsp;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)A * A + (long long)P * A + (long long)P * P) return;
+  auto lf = [&](int a, int b) { return LF[(size_t)a * n + b]; };
+  auto rf = [&](int a, int b) { return RF[(size_t)a * n + b]; };
+  if (t < (long long)A * A) {
+    const int i = T.holes[t / A], j = T.holes[t % A];
+    // th1 = Lhh Rhh, th2 = Lph^T Rph at (i,j) and (j,i)
+    double th1 = 0, th2 = 0, th1t = 0, th2t = 0;
+    for (int k = 0; k < A; k++) {
+      const int hk = T.holes[k];
+      th1 += lf(i, hk) * rf(hk, j);
+      th1t += lf(j, hk) * rf(hk, i);
+    }
+    for (int k = 0; k < P; k++) {
+      const int p = T.parts[k];
+      th2 += lf(p, i) * rf(p, j);
+      th2t += lf(p, j) * rf(p, i);
+    }
+    const double ph = sgnp((L.jj[i] - L.jj[j]) >> 1);
+    F[(size_t)i * n + j] = th1 + th2 * hL - (th1t * hL + th2t) * hR * ph;
+  } else if (t < (long long)A * A + (long long)P * A) {
+    const long long u = t - (long long)A * A;
+    const int p = T.parts[u / A], h = T.holes[u % A];
+    double sm = 0;
+    for (int k = 0; k < A; k++) {
+      const int hk = T.holes[k];
+      sm += lf(p, hk) * rf(hk, h);
+    }
+    for (int k = 0; k < P; k++) {
+      const int q = T.parts[k];
+      sm += lf(p, q) * rf(q, h);
+    }
+    double sm2 = 0;
+    for (int k = 0; k < A; k++) {
+      const int hk = T.holes[k];
+      sm2 += rf(p, hk) * lf(hk, h);
+    }
+    for (int k = 0; k < P; k++) {
+      const int q = T.parts[k];
+      sm2 += rf(p, q) * lf(q, h);
+    }
+    const double v = sm - sm2;
+    F[(size_t)p * n + h] = v;
+    F[(size_t)h * n + p] = v * hres * sgnp((L.jj[h] - L.jj[p]) >> 1);
+  } else {
+    const long long u = t - (long long)A * A - (long long)P * A;
+    const int p = T.parts[u / P], q = T.parts[u % P];
+    // tp1 = Lph Rph^T, tp2 = Lpp Rpp at (p,q) and (q,p)
+    double tp1 = 0, tp1t = 0, tp2 = 0, tp2t = 0;
+    for (int k = 0; k < A; k++) {
+      const int h = T.holes[k];
+      tp1 += lf(p, h) * rf(q, h);
+      tp1t += lf(q, h) * rf(p, h);
+    }
+    for (int k = 0; k < P; k++) {
+      const int r = T.parts[k];
+      tp2 += lf(p, r) * rf(r, q);
+      tp2t += lf(q, r) * rf(r, p);
+    }
+    const double ph = sgnp((L.jj[p] - L.jj[q]) >> 1);
+    F[(size_t)p * n + q] = tp1 * hR * ph + tp2 - (tp1t + tp2t * hR * ph) * hL;
+  }
+}
+
+// TS_commutator_121 (kind 0, :58-303), _211 (kind 1, :307-552) and _221
+// (kind 2, :809-1067): per upper output (p,q) allowed by the tensor's
+// selection rules, F(p,q) += sum, then the lower entry becomes
+// herm * (-1)^((jp-jq)/2) F(p,q) (diagonal included), exactly as each
+// reference routine leaves it; (particle, hole) outputs also refresh the
+// derived (hole, particle) entry.
+__global__ void k_t_onebody(DevLayout L, DevTensor T, int kind, const double *__restrict__ LF,
+                            const double *__restrict__ RF, const double *__restrict__ RV,
+                            const double *__restrict__ gcc, const double *__restrict__ W1,
+                            const double *__restrict__ W2, double *__restrict__ F, int hL, int hR, int hres) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L.nob) return;
+  const int p = L.ob_p[t], q = L.ob_q[t], n = L.nsp, R = T.rank;
+  const int jp = L.jj[p], jq = L.jj[q], lp = L.l[p], lq = L.l[q];
+  if ((lq & 1) != ((lp + (T.dpar >> 1)) & 1)) return;
+  if (L.tz[q] != L.tz[p]) return;
+  if (!dtri(jq, jp, R)) return;
+  double sm = 0.0;
+  if (kind == 0) {
+    for (int u = 0; u < L.nphp; u++) {
+      const int a = L.php_a[u], i = L.php_i[u];
+      const int ja = L.jj[a], ji = L.jj[i];
+      double smx = 0.0, smy = 0.0;
+      for (int J2 = abs(ji - jq); J2 <= ji + jq; J2 += 2)
+        for (int J1 = abs(ja - jp); J1 <= ja + jp; J1 += 2)
+          smx += t_elem(L, T, RV, hR, a, p, i, q, J1, J2) * sqrt((J1 + 1.0) * (J2 + 1.0)) * sgnp(J1 >> 1) *
+                 tx6(T, J1, J2, jq, jp, ji);
+      for (int J1 = abs(ji - jp); J1 <= ji + jp; J1 += 2)
+        for (int J2 = abs(ja - jq); J2 <= ja + jq; J2 += 2)
+          smy += t_elem(L, T, RV, hR, i, p, a, q, J1, J2) * sqrt((J1 + 1.0) * (J2 + 1.0)) * sgnp(J1 >> 1) *
+                 tx6(T, J1, J2, jq, jp, ji);
+      sm += LF[(size_t)a * n + i] * (hL * smx - smy) * sgnp((R + jq + ji) >> 1);
+    }
+  } else if (kind == 1) {
+    const int PAR = (lp + lq) & 1;
+    const int qx = (R >> 1) + 2 * PAR * (L.jtot_max + 1);  // Pandya block (J = R, Tz = 0, PAR)
+    const int rpq = L.ccrow[((size_t)qx * n + p) * n + q], rqp = L.ccrow[((size_t)qx * n + q) * n + p];
+    const long long base = L.ccoff[qx];
+    const int rlen = L.ccrlen[qx];
+    const double inv = 1.0 / sqrt(R + 1.0);
+    const double c1 = sgnp((jp - jq + R) >> 1), c2 = (double)(hR * hL * sgnp(R >> 1));
+    for (int ai = 0; ai < T.npart; ai++) {
+      const int a = T.parts[ai], ja = L.jj[a], la = L.l[a];
+      for (int ii = 0; ii < T.nhole; ii++) {
+        const int i = T.holes[ii], ji = L.jj[i], li = L.l[i];
+        if ((li & 1) != ((la + (T.dpar >> 1)) & 1)) continue;
+        if (L.tz[i] != L.tz[a]) continue;
+        if (!dtri(ja, ji, R)) continue;
+        if (((la + li) & 1) != PAR) continue;
+        const int col = T.ccol[((size_t)qx * n + a) * n + i];
+        if (col < 0 || rpq < 0 || rqp < 0) continue;  // unreachable for allowed outputs
+        sm += RF[(size_t)a * n + i] *
+              (c1 * gcc[base + (long long)col * rlen + rpq] - c2 * gcc[base + (long long)col * rlen + rqp]) * inv;
+      }
+    }
+  } else {
+    const int pm = hL * hR, i = p, j = q, ji = jp, jj = jq;
+    for (int pass = 0; pass < 2; pass++) {  // c over holes with W1, over particles with W2
+      const int nc = pass == 0 ? T.nhole : T.npart;
+      const int *cl = pass == 0 ? T.holes : T.parts;
+      const double *W = pass == 0 ? W1 : W2;
+      for (int ck = 0; ck < nc; ck++) {
+        const int c = cl[ck], jc = L.jj[c];
+        double s1 = 0.0;
+        for (int J1 = abs(jc - ji); J1 <= jc + ji; J1 += 2) {
+          for (int J2 = abs(jc - jj); J2 <= min(J1 - 2, jc + jj); J2 += 2)
+            s1 -= sqrt((J1 + 1.0) * (J2 + 1.0)) * tx6(T, J1, J2, jj, ji, jc) *
+                  t_elem(L, T, W, hR, c, j, c, i, J2, J1) * sgnp(J2 >> 1) * pm;
+          for (int J2 = max(J1, abs(jc - jj)); J2 <= jc + jj; J2 += 2)
+            s1 += sqrt((J1 + 1.0) * (J2 + 1.0)) * tx6(T, J1, J2, jj, ji, jc) *
+                  t_elem(L, T, W, hR, c, i, c, j, J1, J2) * sgnp(J1 >> 1);
+        }
+        sm += s1 * sgnp((jc + jj) >> 1);
+      }
+    }
+    sm *= sgnp(R >> 1);
+  }
+  const size_t pq = (size_t)p * n + q, qp = (size_t)q * n + p;
+  const double v = F[pq] + sm;
+  F[pq] = v;
+  if (L.con[p] == L.con[q])
+    F[qp] = v * hres * sgnp((jp - jq) >> 1);
+  else
+    F[qp] = v * hres * sgnp((jq - jp) >> 1);  // (hole, particle): f_tensor_elem's phase
+}
+
+// ---------------------------------------------------------------- two-body gathers
+__device__ __forceinline__ bool elem_of(const DevLayout &L, const DevTensor &T, long long e, int &q, int &a, int &b,
+                                        int &c, int &d, long long &vpos) {
+  if (e >= T.nelem) return false;
+  q = find_item(T.tstart, T.nblk, e);
+  const long long u = e - T.tstart[q];
+  const int n2 = T.n2[q];
+  const int r = (int)(u / n2), s = (int)(u % n2);
+  const int g1 = L.pbase[T.q1[q]] + r, g2 = L.pbase[T.q2[q]] + s;
+  a = L.pair_a[g1]; b = L.pair_b[g1]; c = L.pair_a[g2]; d = L.pair_b[g2];
+  vpos = T.off[q] + (long long)r * n2 + s;
+  return true;
+}
+
+// TS_commutator_122 (TS_commutators.f90:555-663): overwrites every element
+__global__ void k_t122(DevLayout L, DevTensor T, const double *__restrict__ LF, const double *__restrict__ RV,
+                       double *__restrict__ OV, int hR) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int q, a, b, c, d;
+  long long vp;
+  if (!elem_of(L, T, e, q, a, b, c, d, vp)) return;
+  const int J1 = T.J1[q], J2 = T.J2[q], n = L.nsp;
+  double sm = 0.0;
+  int g = L.grp_of[a];
+  for (int k = 0; k < L.grp_cnt[g]; k++) {
+    const int i = L.grp_list[L.grp_start[g] + k];
+    sm += LF[(size_t)a * n + i] * t_elem(L, T, RV, hR, i, b, c, d, J1, J2);
+  }
+  g = L.grp_of[b];
+  for (int k = 0; k < L.grp_cnt[g]; k++) {
+    const int i = L.grp_list[L.grp_start[g] + k];
+    sm += LF[(size_t)b * n + i] * t_elem(L, T, RV, hR, a, i, c, d, J1, J2);
+  }
+  g = L.grp_of[c];
+  for (int k = 0; k < L.grp_cnt[g]; k++) {
+    const int i = L.grp_list[L.grp_start[g] + k];
+    sm -= LF[(size_t)i * n + c] * t_elem(L, T, RV, hR, a, b, i, d, J1, J2);
+  }
+  g = L.grp_of[d];
+  for (int k = 0; k < L.grp_cnt[g]; k++) {
+    const int i = L.grp_list[L.grp_start[g] + k];
+    sm -= LF[(size_t)i * n + d] * t_elem(L, T, RV, hR, a, b, c, i, J1, J2);
+  }
+  sm = sm / sqrt(1.0 + (a == b)) / sqrt(1.0 + (c == d));
+  OV[vp] = sm;
+}
+
+// TS_commutator_212 (TS_commutators.f90:666-806): accumulates into every element.
+// GV: the scalar operator's V region (v_elem), RF: the tensor's one-body matrix.
+__global__ void k_t212(DevLayout L, DevTensor T, const double *__restrict__ GV, const double *__restrict__ RF,
+                       double *__restrict__ OV) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int q, a, b, c, d;
+  long long vp;
+  if (!elem_of(L, T, e, q, a, b, c, d, vp)) return;
+  const int J1 = T.J1[q], J2 = T.J2[q], n = L.nsp, R = T.rank, dp = (T.dpar >> 1) & 1;
+  const int ja = L.jj[a], jb = L.jj[b], jc = L.jj[c], jd = L.jj[d];
+  const int ta = L.tz[a], tb = L.tz[b], tc = L.tz[c], td = L.tz[d];
+  const int la = L.l[a] & 1, lb = L.l[b] & 1, lc = L.l[c] & 1, ld = L.l[d] & 1;
+  double sm = 0.0;
+  for (int i = 0; i < n; i++) {
+    const int ji = L.jj[i], ti = L.tz[i], li = (L.l[i] + dp) & 1;
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+    if (ti == ta && li == la && dtri(ji, ja, R))
+      s1 = -tx6(T, J1, J2, ji, ja, jb) * RF[(size_t)a * n + i] * dv(L, GV, i, b, c, d, J2);
+    s1 *= sgnp((ja + jb - J2) >> 1);
+    if (ti == tb && li == lb && dtri(ji, jb, R))
+      s2 = tx6(T, J1, J2, ji, jb, ja) * RF[(size_t)b * n + i] * dv(L, GV, i, a, c, d, J2);
+    s2 *= sgnp((J1 + J2) >> 1);
+    if (ti == td && li == ld && dtri(ji, jd, R))
+      s3 = tx6(T, J1, J2, jd, ji, jc) * RF[(size_t)i * n + d] * dv(L, GV, a, b, c, i, J1);
+    s3 *= sgnp((jc + jd - J1) >> 1);
+    if (ti == tc && li == lc && dtri(ji, jc, R))
+      s4 = -tx6(T, J1, J2, jc, ji, jd) * RF[(size_t)i * n + c] * dv(L, GV, a, b, d, i, J1);
+    s4 *= sgnp((J2 + J1) >> 1);
+    sm += s1 + s2 + s3 + s4;
+  }
+  sm *= sqrt((J1 + 1.0) * (J2 + 1.0) / (1.0 + (a == b)) / (1.0 + (c == d))) * sgnp(R >> 1);
+  OV[vp] += sm;
+}
+
+// Tensor Pandya matrices of one cross-coupled block (calculate_single_pandya,
+// cross_coupled.f90:1416-1634).  mode 0: CCX (r3 x nb4, rows = ordered pairs at
+// J3 in the scalar Pandya row order, columns = (p,h) at J4); mode 1: CCR
+// (nb3 x r4, rows = (p,h) at J3, columns = ordered pairs (b,a) at J4).
+__global__ void k_tpandya(DevLayout L, DevTensor T, int mode, const double *__restrict__ RV, double *__restrict__ out,
+                          int hR) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long ntot = mode == 0 ? T.nx : T.nr;
+  if (e >= ntot) return;
+  const long long *st = mode == 0 ? T.xstart : T.rstart;
+  const int x = find_item(st, T.ncc, e);
+  const long long u = e - st[x];
+  const int J3 = T.cJ3[x], J4 = T.cJ4[x], q3 = T.cq3[x], q4 = T.cq4[x];
+  int a, b, p, h;
+  if (mode == 0) {
+    const int nb4 = L.ccnph[q4];
+    const int R1 = (int)(u / nb4), k = (int)(u % nb4);
+    a = L.ccrow_a[L.ccrbase[q3] + R1];
+    b = L.ccrow_b[L.ccrbase[q3] + R1];
+    p = L.cccol_p[L.cccbase[q4] + k];
+    h = L.cccol_h[L.cccbase[q4] + k];
+  } else {
+    const int r4 = L.ccrlen[q4];
+    const int k = (int)(u / r4), G = (int)(u % r4);
+    p = L.cccol_p[L.cccbase[q3] + k];
+    h = L.cccol_h[L.cccbase[q3] + k];
+    b = L.ccrow_a[L.ccrbase[q4] + G];  // the row pair is (b, a)
+    a = L.ccrow_b[L.ccrbase[q4] + G];
+  }
+  double v = 0.0;
+  if (a >= 0) {
+    const int ja = L.jj[a], jb = L.jj[b], jp = L.jj[p], jh = L.jj[h];
+    if (((L.l[a] + L.l[h]) & 1) == ((L.l[b] + L.l[p] + (T.dpar >> 1)) & 1) && L.tz[a] + L.tz[h] == L.tz[b] + L.tz[p]) {
+      double sm = 0.0;
+      for (int J3p = abs(ja - jh); J3p <= ja + jh; J3p += 2)
+        for (int J4p = abs(jp - jb); J4p <= jp + jb; J4p += 2) {
+          double f;
+          const long long pos = mode == 0 ? t_pos(L, T, hR, a, h, p, b, J3p, J4p, f)
+                                          : t_pos(L, T, hR, h, a, b, p, J3p, J4p, f);
+          if (pos < 0) continue;
+          const double nj = mode == 0 ? t9j(T, ja, jb, J3, jh, jp, J4, J3p, J4p) : t9j(T, jh, jp, J3, ja, jb, J4, J3p, J4p);
+          sm -= sgnp(J4p >> 1) * sqrt((J3p + 1.0) * (J4p + 1.0)) * nj * RV[pos] * f;
+        }
+      const double s34 = sqrt((J3 + 1.0) * (J4 + 1.0));
+      v = mode == 0 ? sm * sgnp((jh + jb + J4) >> 1) * s34
+                    : sm * sgnp((jp + ja + J4) >> 1) * s34 * sgnp((jp + jh) >> 1);
+    }
+  }
+  out[(mode == 0 ? T.xoff[x] : T.roff[x]) + u] = v;
+}
+
+// The sixteen ways TS_commutator_222_ph (TS_commutators.f90:1459-1849) adds a
+// product entry into a stored element (x1 x2 | x3 x4; J1 J2): the source
+// orbits (a,b,c,d) in terms of x, the orbit-order condition of the
+// reference's branch, and which of its 9j argument orders, phase and herm
+// factor apply.  The source entry is W(IX = (d,a) at J3, JX = (c,b) at J4).
+__device__ inline void ph_pattern(int k, const int x[4], int &a, int &b, int &c, int &d, bool &ok) {
+  ok = true;
+  switch (k) {
+    case 0: a = x[0]; b = x[1]; c = x[2]; d = x[3]; break;                          // X: (a,b,c,d)
+    case 1: c = x[0]; d = x[1]; a = x[2]; b = x[3]; break;                          // X: (c,d,a,b)
+    case 2: a = x[0]; b = x[1]; d = x[2]; c = x[3]; ok = x[2] < x[3]; break;        // X: (a,b,d,c)
+    case 3: d = x[0]; c = x[1]; a = x[2]; b = x[3]; ok = x[0] < x[1]; break;        // X: (d,c,a,b)
+    case 4: b = x[0]; a = x[1]; c = x[2]; d = x[3]; ok = x[0] < x[1]; break;        // X: (b,a,c,d)
+    case 5: c = x[0]; d = x[1]; b = x[2]; a = x[3]; ok = x[2] < x[3]; break;        // X: (c,d,b,a)
+    case 6: b = x[0]; a = x[1]; d = x[2]; c = x[3]; ok = x[0] < x[1] && x[2] < x[3]; break;  // X: (b,a,d,c)
+    case 7: d = x[0]; c = x[1]; b = x[2]; a = x[3]; ok = x[0] < x[1] && x[2] < x[3]; break;  // X: (d,c,b,a)
+    case 8: d = x[0]; b = x[1]; c = x[2]; a = x[3]; break;                          // Y: (d,b,c,a)
+    case 9: c = x[0]; a = x[1]; d = x[2]; b = x[3]; break;                          // Y: (c,a,d,b)
+    case 10: d = x[0]; b = x[1]; a = x[2]; c = x[3]; ok = x[2] < x[3]; break;       // Y: (d,b,a,c)
+    case 11: a = x[0]; c = x[1]; d = x[2]; b = x[3]; ok = x[0] < x[1]; break;       // Y: (a,c,d,b)
+    case 12: b = x[0]; d = x[1]; c = x[2]; a = x[3]; ok = x[0] < x[1]; break;       // Y: (b,d,c,a)
+    case 13: c = x[0]; a = x[1]; b = x[2]; d = x[3]; ok = x[2] < x[3]; break;       // Y: (c,a,b,d)
+    case 14: b = x[0]; d = x[1]; a = x[2]; c = x[3]; ok = x[0] < x[1] && x[2] < x[3]; break;  // Y: (b,d,a,c)
+    default: a = x[0]; c = x[1]; b = x[2]; d = x[3]; ok = x[0] < x[1] && x[2] < x[3]; break; // Y: (a,c,b,d)
+  }
+}
+
+__global__ void k_t222ph(DevLayout L, DevTensor T, const double *__restrict__ Wx, const double *__restrict__ Wy,
+                         double *__restrict__ OV, int hL, int hR) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int q, x[4];
+  long long vp;
+  if (!elem_of(L, T, e, q, x[0], x[1], x[2], x[3], vp)) return;
+  const int J1 = T.J1[q], J2 = T.J2[q], R = T.rank, n = L.nsp, dp = (T.dpar >> 1) & 1, JM = L.jtot_max;
+  if (x[0] == x[1] && ((J1 >> 1) & 1)) return;
+  if (x[2] == x[3] && ((J2 >> 1) & 1)) return;
+  const double s12 = sqrt((J1 + 1.0) * (J2 + 1.0));
+  const double thr = (double)1e-6f;
+  double acc = 0.0;
+  for (int k = 0; k < 16; k++) {
+    int a, b, c, d;
+    bool ok;
+    ph_pattern(k, x, a, b, c, d, ok);
+    if (!ok) continue;
+    const bool isx = k < 8;
+    const int ja = L.jj[a], jb = L.jj[b], jc = L.jj[c], jd = L.jj[d];
+    const int Tz = abs(L.tz[d] - L.tz[a]) >> 1, PAR = (L.l[d] + L.l[a]) & 1, PAR2 = PAR ^ dp;
+    if (((L.l[c] + L.l[b]) & 1) != PAR2 || (abs(L.tz[c] - L.tz[b]) >> 1) != Tz) continue;
+    const double abcd = sgnp((ja + jb + jc + jd) >> 1);
+    for (int J3 = abs(jd - ja); J3 <= jd + ja; J3 += 2) {
+      const int q3 = (J3 >> 1) + Tz * (JM + 1) + 2 * PAR * (JM + 1);
+      const int IX = L.ccrow[((size_t)q3 * n + d) * n + a];
+      if (IX < 0) continue;
+      for (int J4 = max(abs(J3 - R), J3); J4 <= min(J3 + R, 2 * JM); J4 += 2) {
+        if (!dtri(jc, jb, J4)) continue;
+        if (!isx && J4 == J3) continue;  // no Y product where J3 = J4
+        const int qx = T.cidx[((J3 >> 1) * T.nJ + (J4 >> 1)) * 4 + Tz * 2 + PAR];
+        if (qx < 0) continue;
+        const int q4 = T.cq4[qx];
+        const int JX = L.ccrow[((size_t)q4 * n + c) * n + b];
+        if (JX < 0) continue;
+        const long long wi = T.woff[qx] + (long long)IX * L.ccrlen[q4] + JX;
+        const double w = isx ? hL * Wx[wi] : Wy[wi];  // Wx carries LCC%herm (TS_commutators.f90:1490)
+        if (!(fabs(w) > thr)) continue;
+        const double P = s12 * sqrt((J3 + 1.0) * (J4 + 1.0)), ph34 = sgnp((J3 + J4) >> 1);
+        double v;
+        switch (k) {
+          case 0: v = t9j(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((jb + jd + J2) >> 1) * ph34 * hL; break;
+          case 1: v = -t9j(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((jc + ja + J1 + R) >> 1) * hR; break;
+          case 2: v = t9j(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((jb + jc) >> 1) * ph34 * hL; break;
+          case 3: v = t9j(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((ja - jd + R) >> 1) * hR; break;
+          case 4: v = t9j(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((ja + jd + J1 + J2) >> 1) * ph34 * hL; break;
+          case 5: v = t9j(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((jc - jb + J1 + J2 + R) >> 1) * hR; break;
+          case 6: v = t9j(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((ja + jc + J1) >> 1) * ph34 * hL; break;
+          case 7: v = t9j(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((jb - jd + J2 + R) >> 1) * hR; break;
+          case 8: v = t9j(T, jd, ja, J3, jb, jc, J4, J1, J2) * sgnp((jc + ja + J2) >> 1) * hL; break;
+          case 9: v = t9j(T, ja, jd, J3, jc, jb, J4, J1, J2) * sgnp((jd - jb + J1 + R) >> 1) * ph34 * hR; break;
+          case 10: v = -t9j(T, jd, ja, J3, jb, jc, J4, J1, J2) * hL; break;
+          case 11: v = t9j(T, ja, jd, J3, jc, jb, J4, J1, J2) * abcd * ph34 * sgnp(R >> 1) * hR; break;
+          case 12: v = -t9j(T, jd, ja, J3, jb, jc, J4, J1, J2) * abcd * sgnp((J1 + J2) >> 1) * hL; break;
+          case 13: v = t9j(T, ja, jd, J3, jc, jb, J4, J1, J2) * sgnp((J1 + J2 + R) >> 1) * ph34 * hR; break;
+          case 14: v = t9j(T, jd, ja, J3, jb, jc, J4, J1, J2) * sgnp((jb + jd + J1) >> 1) * hL; break;
+          default: v = t9j(T, ja, jd, J3, jc, jb, J4, J1, J2) * sgnp((ja - jc + J2 + R) >> 1) * ph34 * hR; break;
+        }
+        acc += P * v * w;
+      }
+    }
+  }
+  // add_elem_to_tensor stores V * pre, pre = sqrt 2 per identical pair
+  const double pre = (x[0] == x[1] ? M_SQRT2 : 1.0) * (x[2] == x[3] ? M_SQRT2 : 1.0);
+  OV[vp] += acc * pre;
+}
+
+// ---------------------------------------------------------------- products
+// 64 x 64 tiles, 4 waves of 32 x 32 on v_mfma_f64_16x16x4; tiles (problem, tile row, tile column)
+__global__ __launch_bounds__(256) void k_tgemm(const TProb *__restrict__ probs, const int4 *__restrict__ tiles,
+                                               int ntiles, const double *__restrict__ X0,
+                                               const double *__restrict__ X1, double *__restrict__ C) {
+  const int t = blockIdx.x;
+  if (t >= ntiles) return;
+  const int4 tl = tiles[t];
+  const TProb &p = probs[tl.x];
+  const int m0 = tl.y * 64, n0 = tl.z * 64;
+  __shared__ double As[16][65], Bs[16][65];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wm = wave >> 1, wn = wave & 1;
+  double4_t acc[2][2];
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  for (int sgi = 0; sgi < p.nseg; sgi++) {
+    const TSeg sg = p.seg[sgi];
+    const double *Ap = (sg.asrc ? X1 : X0) + sg.a_off, *Bp = (sg.bsrc ? X1 : X0) + sg.b_off;
+    const bool arow = sg.a_cs == 1, brow = sg.b_cs == 1;
+    for (int k0 = 0; k0 < sg.K; k0 += 16) {
+      for (int e = tid; e < 16 * 64; e += 256) {
+        int kk, mm;
+        if (arow) { kk = e & 15; mm = e >> 4; } else { mm = e & 63; kk = e >> 6; }
+        const int m = m0 + mm, k = k0 + kk;
+        As[kk][mm] = (m < p.M && k < sg.K) ? sg.alpha * Ap[(long long)m * sg.a_rs + (long long)k * sg.a_cs] : 0.0;
+        int kb, nn;
+        if (brow) { nn = e & 63; kb = e >> 6; } else { kb = e & 15; nn = e >> 4; }
+        const int n = n0 + nn, k2 = k0 + kb;
+        Bs[kb][nn] = (n < p.N && k2 < sg.K) ? Bp[(long long)k2 * sg.b_rs + (long long)n * sg.b_cs] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int kq = 0; kq < 4; kq++) {
+        const int k = kq * 4 + (lane >> 4);
+        const double a0 = As[k][wm * 32 + (lane & 15)], a1 = As[k][wm * 32 + 16 + (lane & 15)];
+        const double b0 = Bs[k][wn * 32 + (lane & 15)], b1 = Bs[k][wn * 32 + 16 + (lane & 15)];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      }
+      __syncthreads();
+    }
+  }
+  double *Cp = C + p.c_off;
+  const int g = lane >> 4;
+  for (int ti = 0; ti < 2; ti++)
+    for (int tj = 0; tj < 2; tj++)
+      for (int v = 0; v < 4; v++) {  // v_mfma_f64_16x16x4: row lane/16 + 4v, column lane%16
+        const int row = m0 + wm * 32 + ti * 16 + g + 4 * v, col = n0 + wn * 32 + tj * 16 + (lane & 15);
+        if (row < p.M && col < p.N) Cp[(long long)row * p.ldc + col] = acc[ti][tj][v];
+      }
+}
+
+// mat_frob_norm of a tensor (basic_IMSRG.f90:3921-3952): fhh^2 + 2 fph^2 + fpp^2
+// + sum over blocks (1 if J1 = J2, else 2) x block^2; one partial per workgroup
+__global__ void k_tnorm(DevLayout L, DevTensor T, const double *__restrict__ op, double *__restrict__ part) {
+  __shared__ double sh[256];
+  const int n = L.nsp;
+  const long long nf = (long long)n * n, ntot = nf + T.nelem;
+  double s = 0.0;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < ntot; e += (long long)gridDim.x * blockDim.x) {
+    if (e < nf) {
+      const int a = (int)(e / n), b = (int)(e % n);
+      const double v = op[T.f_off + e];
+      if (L.con[a] == L.con[b]) s += v * v;
+      else if (L.con[a] == 0) s += 2.0 * v * v;  // (particle, hole); (hole, particle) is derived
+    } else {
+      int q, a, b, c, d;
+      long long vp;
+      elem_of(L, T, e - nf, q, a, b, c, d, vp);
+      const double v = op[T.v_off + vp];
+      s += (T.J1[q] == T.J2[q] ? 1.0 : 2.0) * v * v;
+    }
+  }
+  const double r = block_sum256(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+constexpr int kNormParts = 512;
+
+}  // namespace
+
+// ---------------------------------------------------------------- host side
+struct TensorSpace {
+  int rank = 0, dpar = 0, nblk = 0, ncc = 0, hx = 0;
+  std::vector<int> J1, J2, q1, q2, n1, n2, bidx;
+  std::vector<long long> off, tstart;
+  long long vsize = 0, v_off = 0, op_size = 0, neq = 0, nelem = 0;
+  std::vector<int> cJ3, cJ4, cq3, cq4, cidx, ccol;
+  std::vector<long long> xoff, roff, woff, xstart, rstart;
+  long long xsize = 0, rsize = 0, wsize = 0;
+  std::vector<double> x6, s6;
+  DevTensor dt{};
+  double *gcc = nullptr, *ccx = nullptr, *ccr = nullptr, *wx = nullptr, *wy = nullptr;
+  double *w1 = nullptr, *w2 = nullptr, *part = nullptr, *hpart = nullptr;
+  const TProb *pp_probs[2] = {nullptr, nullptr}, *cc_probs[2] = {nullptr, nullptr};
+  const int4 *pp_tiles = nullptr, *cc_tiles[2] = {nullptr, nullptr};
+  int n_pp_tiles = 0, n_cc_tiles[2] = {0, 0};
+  ~TensorSpace() {
+    if (hpart) (void)hipHostFree(hpart);
+  }
+};
+
+struct imsrg_tensor_s {
+  imsrg_ctx_s *ctx = nullptr;
+  TensorSpace *ts = nullptr;
+  double *d = nullptr;
+  int herm = 1;
+};
+
+static long long align32(long long x) { return (x + 31) & ~31LL; }
+
+static void add_tiles(int pi, int M, int N, std::vector<int4> &tiles) {
+  for (int i = 0; i < (M + 63) / 64; i++)
+    for (int j = 0; j < (N + 63) / 64; j++) tiles.push_back(make_int4(pi, i, j, 0));
+}
+
+static int tspace_build(imsrg_ctx_s *c, int rank, int dpar, TensorSpace &S) {
+  const Layout &Ly = c->lay;
+  const int JM = Ly.jtot_max, nJ = Ly.nJ, nsp = Ly.nsp, nh = Ly.nhalf;
+  S.rank = rank;
+  S.dpar = dpar;
+  const int dp = (dpar >> 1) & 1;
+  // tensor blocks, allocate_tensor's order (basic_IMSRG.f90:690-770)
+  S.bidx.assign((size_t)nJ * nJ * 6, -1);
+  long long off = 0, el = 0;
+  for (int J1 = 0; J1 <= 2 * JM; J1 += 2)
+    for (int J2 = std::max(std::abs(J1 - rank), J1); J2 <= J1 + rank; J2 += 2)
+      for (int Tz = -1; Tz <= 1; Tz++)
+        for (int P1 = 0; P1 <= 1; P1++) {
+          const int q = (int)S.J1.size();
+          const bool live = J2 <= 2 * JM;
+          const int b1 = live ? 3 * J1 + 2 * (Tz + 1) + P1 : -1;
+          const int b2 = live ? 3 * J2 + 2 * (Tz + 1) + (P1 ^ dp) : -1;
+          S.J1.push_back(J1);
+          S.J2.push_back(J2);
+          S.q1.push_back(b1);
+          S.q2.push_back(b2);
+          const int m1 = live ? Ly.bdim[b1] : 0, m2 = live ? Ly.bdim[b2] : 0;
+          S.n1.push_back(m1);
+          S.n2.push_back(m2);
+          S.off.push_back(off);
+          S.tstart.push_back(el);
+          off = align32(off + (long long)m1 * m2);
+          el += (long long)m1 * m2;
+          if (live) S.bidx[((size_t)(J1 >> 1) * nJ + (J2 >> 1)) * 6 + (Tz + 1) * 2 + P1] = q;
+        }
+  S.nblk = (int)S.J1.size();
+  S.tstart.push_back(el);
+  S.nelem = el;
+  S.vsize = off;
+  S.v_off = align32(Ly.f_off + (long long)nsp * nsp);
+  S.op_size = S.v_off + S.vsize;
+  S.neq = 1 + (long long)Ly.nhole * Ly.nhole + (long long)Ly.npart * Ly.npart + (long long)Ly.npart * Ly.nhole + el;
+  // cross-coupled tensor blocks (allocate_small_tensor_CCMAT, cross_coupled.f90:498-700):
+  // J3, J4 >= J3 within rank, |Tz|, parity of the J3 pairs
+  S.cidx.assign((size_t)nJ * nJ * 4, -1);
+  long long xo = 0, ro = 0, wo = 0, xe = 0, re = 0;
+  for (int J3 = 0; J3 <= 2 * JM; J3 += 2)
+    for (int J4 = std::max(std::abs(J3 - rank), J3); J4 <= std::min(J3 + rank, 2 * JM); J4 += 2)
+      for (int Tz = 0; Tz <= 1; Tz++)
+        for (int P = 0; P <= 1; P++) {
+          const int x = (int)S.cJ3.size();
+          const int q3 = (J3 >> 1) + Tz * (JM + 1) + 2 * P * (JM + 1);
+          const int q4 = (J4 >> 1) + Tz * (JM + 1) + 2 * (P ^ dp) * (JM + 1);
+          S.cJ3.push_back(J3);
+          S.cJ4.push_back(J4);
+          S.cq3.push_back(q3);
+          S.cq4.push_back(q4);
+          S.cidx[((size_t)(J3 >> 1) * nJ + (J4 >> 1)) * 4 + Tz * 2 + P] = x;
+          const long long r3 = Ly.ccrlen[q3], r4 = Ly.ccrlen[q4], nb3 = Ly.ccnph[q3], nb4 = Ly.ccnph[q4];
+          S.xoff.push_back(xo);
+          S.roff.push_back(ro);
+          S.woff.push_back(wo);
+          S.xstart.push_back(xe);
+          S.rstart.push_back(re);
+          xo = align32(xo + r3 * nb4);
+          ro = align32(ro + nb3 * r4);
+          wo = align32(wo + r3 * r4);
+          xe += r3 * nb4;
+          re += nb3 * r4;
+        }
+  S.ncc = (int)S.cJ3.size();
+  S.xstart.push_back(xe);
+  S.rstart.push_back(re);
+  S.xsize = xo;
+  S.rsize = ro;
+  S.wsize = wo;
+  // column of (p,h) inside each scalar Pandya block
+  S.ccol.assign((size_t)Ly.ncc * nsp * nsp, -1);
+  for (int q = 0; q < Ly.ncc; q++)
+    for (int k = Ly.cccbase[q]; k < Ly.cccbase[q + 1]; k++)
+      S.ccol[((size_t)q * nsp + Ly.cccol_p[k]) * nsp + Ly.cccol_h[k]] = k - Ly.cccbase[q];
+  // 6j tables of the tensor terms (xxxsixj, store_6j_3halfint, basic_IMSRG.f90:2291-2370, 2558-2595)
+  S.hx = (JM + rank + 1) / 2;
+  const int hx = S.hx;
+  S.x6.assign((size_t)nJ * nJ * hx * nh * nh, 0.0);
+  for (int a1 = 0; a1 < nJ; a1++)
+    for (int a2 = 0; a2 < nJ; a2++) {
+      if (!(2 * a2 <= 2 * a1 + rank && 2 * a2 >= std::abs(2 * a1 - rank))) continue;
+      for (int i4 = 0; i4 < hx; i4++)
+        for (int i5 = 0; i5 < nh; i5++)
+          for (int i6 = 0; i6 < nh; i6++)
+            S.x6[((((size_t)a1 * nJ + a2) * hx + i4) * nh + i5) * nh + i6] =
+                racah6j(2 * a1, 2 * a2, rank, 2 * i4 + 1, 2 * i5 + 1, 2 * i6 + 1);
+    }
+  S.s6.assign((size_t)nh * nh * nJ * nh * hx * nJ, 0.0);
+  for (int i1 = 0; i1 < nh; i1++)
+    for (int i2 = 0; i2 < nh; i2++)
+      for (int a3 = 0; a3 < nJ; a3++) {
+        if (!(2 * a3 <= 2 * i1 + 1 + 2 * i2 + 1 && 2 * a3 >= std::abs(2 * i1 - 2 * i2))) continue;
+        for (int i4 = 0; i4 < nh; i4++)
+          for (int i5 = 0; i5 < hx; i5++)
+            for (int a6 = 0; a6 < nJ; a6++)
+              S.s6[(((((size_t)i1 * nh + i2) * nJ + a3) * nh + i4) * hx + i5) * nJ + a6] =
+                  racah6j(2 * i1 + 1, 2 * i2 + 1, 2 * a3, 2 * i4 + 1, 2 * i5 + 1, 2 * a6);
+      }
+  // device tables
+  DevTensor &D = S.dt;
+  D.rank = rank; D.dpar = dpar; D.nblk = S.nblk; D.ncc = S.ncc; D.nJ = nJ; D.nh = nh; D.hx = hx;
+  D.nhole = Ly.nhole; D.npart = Ly.npart; D.nelem = S.nelem; D.nx = xe; D.nr = re;
+  D.f_off = Ly.f_off; D.v_off = S.v_off;
+#define TUP(vec, field) \
+  if (upload_table(c, vec, &D.field) != 0) return IMSRG_ENOMEM
+  TUP(Ly.holes, holes); TUP(Ly.parts, parts);
+  TUP(S.J1, J1); TUP(S.J2, J2); TUP(S.q1, q1); TUP(S.q2, q2); TUP(S.n1, n1); TUP(S.n2, n2);
+  TUP(S.off, off); TUP(S.tstart, tstart); TUP(S.bidx, bidx);
+  TUP(S.cJ3, cJ3); TUP(S.cJ4, cJ4); TUP(S.cq3, cq3); TUP(S.cq4, cq4);
+  TUP(S.xoff, xoff); TUP(S.roff, roff); TUP(S.woff, woff); TUP(S.xstart, xstart); TUP(S.rstart, rstart);
+  TUP(S.cidx, cidx); TUP(S.ccol, ccol); TUP(S.x6, x6); TUP(S.s6, s6);
+#undef TUP
+  // products: the pp and hh ladders per tensor block (W1, W2)
+  {
+    std::vector<TProb> pr[2];
+    std::vector<int4> tiles;
+    for (int q = 0; q < S.nblk; q++) {
+      const int m1 = S.n1[q], m2 = S.n2[q];
+      if (m1 == 0 || m2 == 0) continue;
+      const int b1 = S.q1[q], b2 = S.q2[q];
+      const int np1 = Ly.bnp[b1], nb1 = Ly.bnb[b1], nh1 = Ly.bnh[b1];
+      const int np2 = Ly.bnp[b2], nb2 = Ly.bnb[b2], nh2 = Ly.bnh[b2];
+      for (int w = 0; w < 2; w++) {  // w 0: pp intermediate, 1: hh
+        const int k1 = w ? np1 + nb1 : 0, K1 = w ? nh1 : np1, k2 = w ? np2 + nb2 : 0, K2 = w ? nh2 : np2;
+        TProb p{};
+        p.M = m1; p.N = m2; p.ldc = m2; p.nseg = 2; p.c_off = S.off[q];
+        // G(J1)[:, k] T[k, :]  (operand 0: G's V, 1: T's V)
+        p.seg[0] = TSeg{Ly.boff[b1] + k1, S.off[q] + (long long)k1 * m2, K1, m1, 1, m2, 1, 0, 1, 1.0};
+        // - T[:, k] G(J2)[k, :]
+        p.seg[1] = TSeg{S.off[q] + k2, Ly.boff[b2] + (long long)k2 * m2, K2, m2, 1, m2, 1, 1, 0, -1.0};
+        pr[w].push_back(p);
+      }
+      add_tiles((int)pr[0].size() - 1, m1, m2, tiles);
+    }
+    for (int w = 0; w < 2; w++)
+      if (upload_table(c, pr[w], &S.pp_probs[w]) != 0) return IMSRG_ENOMEM;
+    if (upload_table(c, tiles, &S.pp_tiles) != 0) return IMSRG_ENOMEM;
+    S.n_pp_tiles = (int)tiles.size();
+  }
+  // the two Pandya products per cross-coupled block (TS_commutator_222_ph, TS_commutators.f90:1479-1515):
+  // alpha carries 1/sqrt(J3+1) and 1/sqrt(J4+1); the herm of G that the reference also folds into Wx
+  // is applied where k_t222ph reads Wx, so these tables serve every G
+  {
+    std::vector<TProb> pr[2];
+    std::vector<int4> tiles[2];
+    for (int x = 0; x < S.ncc; x++) {
+      const int q3 = S.cq3[x], q4 = S.cq4[x];
+      const int r3 = Ly.ccrlen[q3], r4 = Ly.ccrlen[q4], nb3 = Ly.ccnph[q3], nb4 = Ly.ccnph[q4];
+      if (r3 == 0 || r4 == 0) continue;
+      TProb px{};  // Wx = LCC(J3) CCR / sqrt(J3+1): operand 0 the Pandya matrices of G, 1 CCR
+      px.M = r3; px.N = r4; px.ldc = r4; px.c_off = S.woff[x];
+      px.nseg = nb3 > 0 ? 1 : 0;
+      px.seg[0] = TSeg{Ly.ccoff[q3], S.roff[x], nb3, 1, r3, r4, 1, 0, 1, 1.0 / std::sqrt(S.cJ3[x] + 1.0)};
+      pr[0].push_back(px);
+      add_tiles((int)pr[0].size() - 1, r3, r4, tiles[0]);
+      TProb py{};  // Wy = CCX LCC(J4)^T / sqrt(J4+1): operand 0 CCX, 1 the Pandya matrices of G
+      py.M = r3; py.N = r4; py.ldc = r4; py.c_off = S.woff[x];
+      py.nseg = (nb4 > 0 && S.cJ3[x] != S.cJ4[x]) ? 1 : 0;
+      py.seg[0] = TSeg{S.xoff[x], Ly.ccoff[q4], nb4, nb4, 1, r4, 1, 0, 1, 1.0 / std::sqrt(S.cJ4[x] + 1.0)};
+      pr[1].push_back(py);
+      add_tiles((int)pr[1].size() - 1, r3, r4, tiles[1]);
+    }
+    for (int w = 0; w < 2; w++) {
+      if (upload_table(c, pr[w], &S.cc_probs[w]) != 0) return IMSRG_ENOMEM;
+      if (upload_table(c, tiles[w], &S.cc_tiles[w]) != 0) return IMSRG_ENOMEM;
+      S.n_cc_tiles[w] = (int)tiles[w].size();
+    }
+  }
+  // workspaces
+  if (dalloc(c, Ly.ccsize, &S.gcc) || dalloc(c, S.xsize, &S.ccx) || dalloc(c, S.rsize, &S.ccr) ||
+      dalloc(c, S.wsize, &S.wx) || dalloc(c, S.wsize, &S.wy) || dalloc(c, S.vsize, &S.w1) ||
+      dalloc(c, S.vsize, &S.w2) || dalloc(c, kNormParts, &S.part))
+    return IMSRG_ENOMEM;
+  launch_fill(c->stream, S.w1, S.vsize, 0.0);
+  launch_fill(c->stream, S.w2, S.vsize, 0.0);
+  launch_fill(c->stream, S.wx, S.wsize, 0.0);
+  launch_fill(c->stream, S.wy, S.wsize, 0.0);
+  HIPCHK(hipHostMalloc((void **)&S.hpart, kNormParts * sizeof(double)));
+  return 0;
+}
+
+static int tspace_get(imsrg_ctx_s *c, int rank, int dpar, TensorSpace **out) {
+  auto key = std::make_pair(rank, dpar);
+  auto it = c->tspaces.find(key);
+  if (it != c->tspaces.end()) {
+    *out = it->second.get();
+    return 0;
+  }
+  auto S = std::make_shared<TensorSpace>();
+  const int rc = tspace_build(c, rank, dpar, *S);
+  if (rc != 0) return rc;
+  c->tspaces[key] = S;
+  *out = S.get();
+  return 0;
+}
+
+// ---------------------------------------------------------------- C ABI
+static unsigned nblk(long long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+int imsrg_tensor_create(imsrg_ctx c, int rank, int dpar, int herm, imsrg_tensor *out) {
+  if (!c || !out) return fail(IMSRG_EINVAL, "null argument");
+  if (c->nbatch != 1) return fail(IMSRG_EINVAL, "tensor operators need a single-instance context");
+  if (rank <= 0 || (rank & 1)) return fail(IMSRG_EINVAL, "rank must be doubled and positive (2 lambda)");
+  if (dpar != 0 && dpar != 2) return fail(IMSRG_EINVAL, "dpar must be 0 or 2 (doubled)");
+  if (herm != 1 && herm != -1) return fail(IMSRG_EINVAL, "herm must be +1 or -1");
+  TensorSpace *S = nullptr;
+  if (int rc = tspace_get(c, rank, dpar, &S)) return rc;
+  imsrg_tensor_s *t = new imsrg_tensor_s;
+  t->ctx = c;
+  t->ts = S;
+  t->herm = herm;
+  if (hipMalloc(&t->d, sizeof(double) * S->op_size) != hipSuccess) {
+    delete t;
+    return fail(IMSRG_ENOMEM, "hipMalloc tensor operator failed");
+  }
+  launch_fill(c->stream, t->d, S->op_size, 0.0);
+  *out = t;
+  return 0;
+}
+
+int imsrg_tensor_destroy(imsrg_tensor t) {
+  if (!t) return 0;
+  (void)hipStreamSynchronize(t->ctx->stream);
+  (void)hipFree(t->d);
+  delete t;
+  return 0;
+}
+
+int64_t imsrg_tensor_neq(imsrg_tensor t) { return t ? t->ts->neq : -1; }
+int imsrg_tensor_set_herm(imsrg_tensor t, int herm) {
+  if (!t || (herm != 1 && herm != -1)) return fail(IMSRG_EINVAL, "bad argument");
+  t->herm = herm;
+  return 0;
+}
+int imsrg_tensor_get_herm(imsrg_tensor t, int *herm) {
+  if (!t || !herm) return fail(IMSRG_EINVAL, "null argument");
+  *herm = t->herm;
+  return 0;
+}
+
+// stream <-> device image (see the header for the stream order)
+static void stream_to_image(const Layout &Ly, const TensorSpace &S, int herm, const double *v, double *img) {
+  std::fill(img, img + S.op_size, 0.0);
+  const int A = Ly.nhole, P = Ly.npart, n = Ly.nsp;
+  long long k = 0;
+  img[0] = v[k++];
+  double *F = img + Ly.f_off;
+  for (int j = 0; j < A; j++)
+    for (int i = 0; i < A; i++) F[(size_t)Ly.holes[i] * n + Ly.holes[j]] = v[k + i + (long long)j * A];
+  k += (long long)A * A;
+  for (int j = 0; j < P; j++)
+    for (int i = 0; i < P; i++) F[(size_t)Ly.parts[i] * n + Ly.parts[j]] = v[k + i + (long long)j * P];
+  k += (long long)P * P;
+  for (int j = 0; j < A; j++)
+    for (int i = 0; i < P; i++) {
+      const int p = Ly.parts[i], h = Ly.holes[j];
+      const double x = v[k + i + (long long)j * P];
+      F[(size_t)p * n + h] = x;
+      F[(size_t)h * n + p] = x * herm * (((Ly.jj[h] - Ly.jj[p]) / 2) & 1 ? -1.0 : 1.0);
+    }
+  k += (long long)P * A;
+  double *V = img + S.v_off;
+  for (int q = 0; q < S.nblk; q++) {
+    const int m1 = S.n1[q], m2 = S.n2[q];
+    if (m1 == 0 || m2 == 0) continue;
+    const int b1 = S.q1[q], b2 = S.q2[q];
+    const int s1[4] = {0, Ly.bnp[b1], Ly.bnp[b1] + Ly.bnb[b1], m1};
+    const int s2[4] = {0, Ly.bnp[b2], Ly.bnp[b2] + Ly.bnb[b2], m2};
+    static const int sea1[9] = {0, 0, 0, 1, 2, 1, 2, 1, 2}, sea2[9] = {0, 1, 2, 1, 2, 2, 0, 0, 1};
+    for (int g = 0; g < 9; g++) {
+      const int r0 = s1[sea1[g]], nr = s1[sea1[g] + 1] - r0, c0 = s2[sea2[g]], nc = s2[sea2[g] + 1] - c0;
+      for (int jc = 0; jc < nc; jc++)
+        for (int ir = 0; ir < nr; ir++) V[S.off[q] + (long long)(r0 + ir) * m2 + c0 + jc] = v[k++];
+    }
+  }
+}
+static void image_to_stream(const Layout &Ly, const TensorSpace &S, const double *img, double *v) {
+  const int A = Ly.nhole, P = Ly.npart, n = Ly.nsp;
+  long long k = 0;
+  v[k++] = img[0];
+  const double *F = img + Ly.f_off;
+  for (int j = 0; j < A; j++)
+    for (int i = 0; i < A; i++) v[k++] = F[(size_t)Ly.holes[i] * n + Ly.holes[j]];
+  for (int j = 0; j < P; j++)
+    for (int i = 0; i < P; i++) v[k++] = F[(size_t)Ly.parts[i] * n + Ly.parts[j]];
+  for (int j = 0; j < A; j++)
+    for (int i = 0; i < P; i++) v[k++] = F[(size_t)Ly.parts[i] * n + Ly.holes[j]];
+  const double *V = img + S.v_off;
+  for (int q = 0; q < S.nblk; q++) {
+    const int m1 = S.n1[q], m2 = S.n2[q];
+    if (m1 == 0 || m2 == 0) continue;
+    const int b1 = S.q1[q], b2 = S.q2[q];
+    const int s1[4] = {0, Ly.bnp[b1], Ly.bnp[b1] + Ly.bnb[b1], m1};
+    const int s2[4] = {0, Ly.bnp[b2], Ly.bnp[b2] + Ly.bnb[b2], m2};
+    static const int sea1[9] = {0, 0, 0, 1, 2, 1, 2, 1, 2}, sea2[9] = {0, 1, 2, 1, 2, 2, 0, 0, 1};
+    for (int g = 0; g < 9; g++) {
+      const int r0 = s1[sea1[g]], nr = s1[sea1[g] + 1] - r0, c0 = s2[sea2[g]], nc = s2[sea2[g] + 1] - c0;
+      for (int jc = 0; jc < nc; jc++)
+        for (int ir = 0; ir < nr; ir++) v[k++] = V[S.off[q] + (long long)(r0 + ir) * m2 + c0 + jc];
+    }
+  }
+}
+
+int imsrg_tensor_upload(imsrg_tensor t, const double *vec, int64_t n) {
+  if (!t || !vec) return fail(IMSRG_EINVAL, "null argument");
+  const TensorSpace &S = *t->ts;
+  if (n != S.neq) return fail(IMSRG_EINVAL, "tensor stream length " + std::to_string(n) + " != " + std::to_string(S.neq));
+  std::vector<double> img(S.op_size);
+  stream_to_image(t->ctx->lay, S, t->herm, vec, img.data());
+  HIPCHK(hipMemcpyAsync(t->d, img.data(), sizeof(double) * S.op_size, hipMemcpyHostToDevice, t->ctx->stream));
+  HIPCHK(hipStreamSynchronize(t->ctx->stream));
+  return 0;
+}
+
+int imsrg_tensor_download(imsrg_tensor t, double *vec, int64_t n) {
+  if (!t || !vec) return fail(IMSRG_EINVAL, "null argument");
+  const TensorSpace &S = *t->ts;
+  if (n != S.neq) return fail(IMSRG_EINVAL, "tensor stream length " + std::to_string(n) + " != " + std::to_string(S.neq));
+  std::vector<double> img(S.op_size);
+  HIPCHK(hipMemcpyAsync(img.data(), t->d, sizeof(double) * S.op_size, hipMemcpyDeviceToHost, t->ctx->stream));
+  HIPCHK(hipStreamSynchronize(t->ctx->stream));
+  image_to_stream(t->ctx->lay, S, img.data(), vec);
+  return 0;
+}
+
+static int same_space(imsrg_tensor a, imsrg_tensor b) {
+  if (!a || !b) return fail(IMSRG_EINVAL, "null argument");
+  if (a->ts != b->ts || a->ctx != b->ctx) return fail(IMSRG_EINVAL, "tensors of different rank, parity or context");
+  return 0;
+}
+
+int imsrg_tensor_copy(imsrg_tensor src, imsrg_tensor dst) {
+  if (int rc = same_space(src, dst)) return rc;
+  HIPCHK(hipMemcpyAsync(dst->d, src->d, sizeof(double) * src->ts->op_size, hipMemcpyDeviceToDevice, src->ctx->stream));
+  dst->herm = src->herm;
+  return 0;
+}
+
+int imsrg_tensor_clear(imsrg_tensor t) {
+  if (!t) return fail(IMSRG_EINVAL, "null argument");
+  launch_fill(t->ctx->stream, t->d, t->ts->op_size, 0.0);
+  return 0;
+}
+
+int imsrg_tensor_add(imsrg_tensor A, double ax, imsrg_tensor B, double bx, imsrg_tensor C) {
+  if (int rc = same_space(A, B)) return rc;
+  if (int rc = same_space(A, C)) return rc;
+  launch_axpby(A->ctx->stream, A->ts->op_size, ax, A->d, bx, B->d, C->d, nullptr, 1, 0);
+  return 0;
+}
+
+static int tnorm(imsrg_tensor t, double *out) {
+  imsrg_ctx_s *c = t->ctx;
+  TensorSpace &S = *t->ts;
+  hipLaunchKernelGGL(k_tnorm, dim3(kNormParts), dim3(256), 0, c->stream, c->dl, S.dt, (const double *)t->d, S.part);
+  HIPCHK(hipMemcpyAsync(S.hpart, S.part, kNormParts * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double s = 0.0;
+  for (int k = 0; k < kNormParts; k++) s += S.hpart[k];
+  *out = std::sqrt(s);
+  return 0;
+}
+
+int imsrg_tensor_norm(imsrg_tensor t, double *out) {
+  if (!t || !out) return fail(IMSRG_EINVAL, "null argument");
+  return tnorm(t, out);
+}
+
+// [G, T] into RES in BCH_TENSOR's call order (IMSRG_MAGNUS.f90:427-446):
+// 111 (overwrites the one-body part), 121, 122 (overwrites the two-body
+// part), 212, 222_pp_hh, 221, 211, 222_ph; the others accumulate.  221 reads
+// the pp/hh ladder products, computed here whenever 221 is asked for.
+static int ts_comm(imsrg_op_s *G, imsrg_tensor T, imsrg_tensor RES, unsigned mask) {
+  imsrg_ctx_s *c = G->ctx;
+  TensorSpace &S = *T->ts;
+  const DevLayout &L = c->dl;
+  const DevTensor &D = S.dt;
+  hipStream_t s = c->stream;
+  const int hL = G->herm, hR = T->herm, hres = RES->herm;
+  const double *GF = G->d + c->lay.f_off, *GV = G->d + c->lay.v_off;
+  const double *RF = T->d + c->lay.f_off, *RV = T->d + S.v_off;
+  double *OF = RES->d + c->lay.f_off, *OV = RES->d + S.v_off;
+  const long long nf = (long long)D.nhole * D.nhole + (long long)D.npart * D.nhole + (long long)D.npart * D.npart;
+  if (mask & IMSRG_TS111)
+    hipLaunchKernelGGL(k_t111, dim3(nblk(nf)), dim3(256), 0, s, L, D, GF, RF, OF, hL, hR, hres);
+  if (mask & IMSRG_TS121)
+    hipLaunchKernelGGL(k_t_onebody, dim3(nblk(L.nob)), dim3(256), 0, s, L, D, 0, GF, RF, RV, (const double *)S.gcc,
+                       (const double *)S.w1, (const double *)S.w2, OF, hL, hR, hres);
+  if (S.nelem > 0 && (mask & IMSRG_TS122))
+    hipLaunchKernelGGL(k_t122, dim3(nblk(S.nelem)), dim3(256), 0, s, L, D, GF, RV, OV, hR);
+  if (S.nelem > 0 && (mask & IMSRG_TS212))
+    hipLaunchKernelGGL(k_t212, dim3(nblk(S.nelem)), dim3(256), 0, s, L, D, GV, RF, OV);
+  if ((mask & (IMSRG_TS222PPHH | IMSRG_TS221)) && S.n_pp_tiles > 0) {
+    hipLaunchKernelGGL(k_tgemm, dim3(S.n_pp_tiles), dim3(256), 0, s, S.pp_probs[0], S.pp_tiles, S.n_pp_tiles, GV, RV,
+                       S.w1);
+    hipLaunchKernelGGL(k_tgemm, dim3(S.n_pp_tiles), dim3(256), 0, s, S.pp_probs[1], S.pp_tiles, S.n_pp_tiles, GV, RV,
+                       S.w2);
+  }
+  if (mask & IMSRG_TS222PPHH) {
+    launch_axpby(s, S.vsize, 1.0, OV, 1.0, S.w1, OV, nullptr, 1, 0);
+    launch_axpby(s, S.vsize, 1.0, OV, -1.0, S.w2, OV, nullptr, 1, 0);
+  }
+  if (mask & IMSRG_TS221)
+    hipLaunchKernelGGL(k_t_onebody, dim3(nblk(L.nob)), dim3(256), 0, s, L, D, 2, GF, RF, RV, (const double *)S.gcc,
+                       (const double *)S.w1, (const double *)S.w2, OF, hL, hR, hres);
+  if (mask & (IMSRG_TS211 | IMSRG_TS222PH)) launch_pandya(s, L, G->d, S.gcc, nullptr, 1);
+  if (mask & IMSRG_TS211)
+    hipLaunchKernelGGL(k_t_onebody, dim3(nblk(L.nob)), dim3(256), 0, s, L, D, 1, GF, RF, RV, (const double *)S.gcc,
+                       (const double *)S.w1, (const double *)S.w2, OF, hL, hR, hres);
+  if ((mask & IMSRG_TS222PH) && S.nelem > 0) {
+    if (D.nx > 0) hipLaunchKernelGGL(k_tpandya, dim3(nblk(D.nx)), dim3(256), 0, s, L, D, 0, RV, S.ccx, hR);
+    if (D.nr > 0) hipLaunchKernelGGL(k_tpandya, dim3(nblk(D.nr)), dim3(256), 0, s, L, D, 1, RV, S.ccr, hR);
+    if (S.n_cc_tiles[0] > 0)
+      hipLaunchKernelGGL(k_tgemm, dim3(S.n_cc_tiles[0]), dim3(256), 0, s, S.cc_probs[0], S.cc_tiles[0],
+                         S.n_cc_tiles[0], (const double *)S.gcc, (const double *)S.ccr, S.wx);
+    if (S.n_cc_tiles[1] > 0)
+      hipLaunchKernelGGL(k_tgemm, dim3(S.n_cc_tiles[1]), dim3(256), 0, s, S.cc_probs[1], S.cc_tiles[1],
+                         S.n_cc_tiles[1], (const double *)S.ccx, (const double *)S.gcc, S.wy);
+    hipLaunchKernelGGL(k_t222ph, dim3(nblk(S.nelem)), dim3(256), 0, s, L, D, (const double *)S.wx,
+                       (const double *)S.wy, OV, hL, hR);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int imsrg_tensor_commutator(imsrg_op G, imsrg_tensor T, imsrg_tensor RES, unsigned mask) {
+  if (!G || !T || !RES) return fail(IMSRG_EINVAL, "null argument");
+  if (int rc = same_space(T, RES)) return rc;
+  if (G->ctx != T->ctx) return fail(IMSRG_EINVAL, "G and T belong to different contexts");
+  if (T == RES) return fail(IMSRG_EINVAL, "RES must differ from T");
+  return ts_comm(G, T, RES, mask);
+}
+
+int imsrg_bch_tensor(imsrg_op G, imsrg_tensor HS, int *nterms) {
+  if (!G || !HS) return fail(IMSRG_EINVAL, "null argument");
+  imsrg_ctx_s *c = HS->ctx;
+  if (G->ctx != c) return fail(IMSRG_EINVAL, "G and HS belong to different contexts");
+  TensorSpace &S = *HS->ts;
+  imsrg_tensor_s INT2, AD;
+  INT2.ctx = AD.ctx = c;
+  INT2.ts = AD.ts = &S;
+  double *buf = nullptr;
+  if (hipMalloc(&buf, sizeof(double) * S.op_size * 2) != hipSuccess)
+    return fail(IMSRG_ENOMEM, "hipMalloc BCH_TENSOR workspace failed");
+  INT2.d = buf;
+  AD.d = buf + S.op_size;
+  // INT2 = HS (copy_sq_op keeps HS's herm, IMSRG_MAGNUS.f90:419-422)
+  int rc = imsrg_tensor_copy(HS, &INT2);
+  const double conv = (double)1e-6f;  // real(8), parameter :: conv = 1e-6
+  double coef = 1.0;
+  int n = 0;
+  for (int iw = 2; iw <= 30 && rc == 0; iw++) {
+    coef = coef / (iw - 1.0);
+    if ((rc = imsrg_tensor_copy(&INT2, &AD))) break;
+    launch_fill(c->stream, INT2.d, S.op_size, 0.0);
+    if ((rc = ts_comm(G, &AD, &INT2, IMSRG_TSALL))) break;
+    n++;
+    launch_axpby(c->stream, S.op_size, 1.0, HS->d, coef, INT2.d, HS->d, nullptr, 1, 0);
+    double nrm = 0.0;
+    if ((rc = tnorm(&INT2, &nrm))) break;
+    if (nrm * coef < conv) break;
+  }
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(buf);
+  if (nterms) *nterms = n;
+  return rc;
+}

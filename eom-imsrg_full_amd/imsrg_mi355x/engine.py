@@ -21,6 +21,10 @@ HEADER = os.path.abspath(os.path.join(HERE, "..", "..", "include", "imsrg_mi355x
 
 T110, T111, T121, T122, T220, T221, T222PPHH, T222PH = (1 << k for k in range(8))
 TALL = 0xFF
+# scalar-tensor commutator terms (TS_commutators.f90), IMSRG_TS* in the header
+TS_TERMS = {"111": 1 << 0, "121": 1 << 1, "211": 1 << 2, "122": 1 << 3, "212": 1 << 4,
+            "222pphh": 1 << 5, "221": 1 << 6, "222ph": 1 << 7}
+TSALL = 0xFF
 PREC_FP64, PREC_SPLIT_FP32, PREC_SPLIT_BF16, PREC_INT8X5 = 0, 1, 2, 3
 GEN = {"white": 0, "atan": 1, "w2": 2}
 
@@ -116,6 +120,19 @@ _SIG = {
     "imsrg_observable_to_hf": (C.c_int, [_vp, _dp]),
     "imsrg_dhds": (C.c_int, [_vp, C.c_int, _vp]),
     "imsrg_ctx_profile_read": (C.c_int, [_vp, _dp, C.POINTER(C.c_longlong), C.c_int, C.c_int]),
+    "imsrg_tensor_create": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "imsrg_tensor_destroy": (C.c_int, [_vp]),
+    "imsrg_tensor_neq": (_i64, [_vp]),
+    "imsrg_tensor_set_herm": (C.c_int, [_vp, C.c_int]),
+    "imsrg_tensor_get_herm": (C.c_int, [_vp, _ip]),
+    "imsrg_tensor_upload": (C.c_int, [_vp, _dp, _i64]),
+    "imsrg_tensor_download": (C.c_int, [_vp, _dp, _i64]),
+    "imsrg_tensor_copy": (C.c_int, [_vp, _vp]),
+    "imsrg_tensor_clear": (C.c_int, [_vp]),
+    "imsrg_tensor_add": (C.c_int, [_vp, C.c_double, _vp, C.c_double, _vp]),
+    "imsrg_tensor_norm": (C.c_int, [_vp, _dp]),
+    "imsrg_tensor_commutator": (C.c_int, [_vp, _vp, _vp, C.c_uint]),
+    "imsrg_bch_tensor": (C.c_int, [_vp, _vp, _ip]),
 }
 PROF_CLASSES = ["gemm_pphh", "gemm_ph", "pandya", "k122", "ph_apply", "ph_terms", "sym_mirror",
                 "onebody", "e0", "norm", "axpby", "generator", "slice"]
@@ -223,6 +240,26 @@ class Context:
             o.upload(vec)
         return o
 
+    def tensor(self, rank, dpar=0, herm=1, vec=None):
+        """a rank > 0 operator; rank and dpar doubled as the reference stores them"""
+        t = TensorOp(self, rank, dpar, herm)
+        if vec is not None:
+            t.upload(vec)
+        return t
+
+    def tensor_commutator(self, G, T, RES=None, mask=TSALL, herm=None):
+        """[G, T] terms of TS_commutators.f90 into RES (herm -herm(G) herm(T) unless given)"""
+        if RES is None:
+            RES = TensorOp(self, T.rank, T.dpar, herm if herm is not None else -G.herm * T.herm)
+        _chk(_lib.imsrg_tensor_commutator(G.h, T.h, RES.h, int(mask)))
+        return RES
+
+    def bch_tensor(self, G, HS):
+        """BCH_TENSOR(G, HS) in place; returns the number of commutators evaluated"""
+        n = C.c_int(0)
+        _chk(_lib.imsrg_bch_tensor(G.h, HS.h, C.byref(n)))
+        return n.value
+
     # -- engine
     def commutator(self, L, R, RES=None, mask=TALL):
         if RES is None:
@@ -321,6 +358,62 @@ class Context:
         out = np.zeros(n)
         _chk(_lib.imsrg_cross_coupled(A.h, _dptr(out), n))
         return out
+
+
+class TensorOp:
+    """A device-resident rank > 0 sq_op (imsrg_tensor)."""
+
+    def __init__(self, ctx: Context, rank, dpar=0, herm=1):
+        self.ctx, self.rank, self.dpar = ctx, int(rank), int(dpar)
+        h = _vp()
+        _chk(_lib.imsrg_tensor_create(ctx.h, self.rank, self.dpar, int(herm), C.byref(h)))
+        self.h = h
+        self.neq = int(_lib.imsrg_tensor_neq(h))
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                _lib.imsrg_tensor_destroy(self.h)
+        except Exception:
+            pass
+
+    @property
+    def herm(self):
+        v = C.c_int(0)
+        _chk(_lib.imsrg_tensor_get_herm(self.h, C.byref(v)))
+        return v.value
+
+    @herm.setter
+    def herm(self, h):
+        _chk(_lib.imsrg_tensor_set_herm(self.h, int(h)))
+
+    def upload(self, vec):
+        v = np.ascontiguousarray(vec, dtype=np.float64)
+        _chk(_lib.imsrg_tensor_upload(self.h, _dptr(v), v.size))
+        return self
+
+    def download(self):
+        v = np.zeros(self.neq)
+        _chk(_lib.imsrg_tensor_download(self.h, _dptr(v), v.size))
+        return v
+
+    def copy_from(self, other):
+        _chk(_lib.imsrg_tensor_copy(other.h, self.h))
+        return self
+
+    def clear(self):
+        _chk(_lib.imsrg_tensor_clear(self.h))
+        return self
+
+    def add(self, ax, B, bx, out=None):
+        out = out or self
+        _chk(_lib.imsrg_tensor_add(self.h, float(ax), B.h, float(bx), out.h))
+        return out
+
+    def norm(self):
+        v = C.c_double(0.0)
+        _chk(_lib.imsrg_tensor_norm(self.h, C.byref(v)))
+        return v.value
 
 
 class Operator:

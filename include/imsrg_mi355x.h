@@ -302,6 +302,53 @@ int imsrg_read_interaction_vec(int nsp, const int *n, const int *l, const int *j
                                int format, const char *path, const char *path_1b,
                                const imsrg_interaction_params *p, double *vec, int64_t neq);
 
+/* ---- spherical tensor operators and the scalar-tensor commutator ---------
+ * A rank > 0 operator (allocate_tensor, basic_IMSRG.f90:644-907) on a
+ * single-instance context: rank and dpar doubled as the reference stores them
+ * (op%rank = 2 lambda, op%dpar = 0 or 2), herm +1 or -1.  Its host stream
+ * (imsrg_tensor_upload / download, imsrg_tensor_neq doubles) is the
+ * reference's storage flattened in declaration order, since the reference has
+ * no vectorize for tensors: E0, fhh (A x A), fpp (P x P), fph (P x A), then for
+ * every tensor block in allocate_tensor's order (J1, J2 = max(|J1-rank|, J1)
+ * .. J1+rank, Tz = -1..1, P1 = 0..1) its sub-blocks tgam(1..9), each
+ * column-major.  The (hole, particle) one-body entries are implied by fph
+ * with f_tensor_elem's phase herm * (-1)^((jh-jp)/2). */
+typedef struct imsrg_tensor_s *imsrg_tensor;
+int imsrg_tensor_create(imsrg_ctx ctx, int rank, int dpar, int herm, imsrg_tensor *out);
+int imsrg_tensor_destroy(imsrg_tensor t);
+int64_t imsrg_tensor_neq(imsrg_tensor t);
+int imsrg_tensor_set_herm(imsrg_tensor t, int herm);
+int imsrg_tensor_get_herm(imsrg_tensor t, int *herm);
+int imsrg_tensor_upload(imsrg_tensor t, const double *vec, int64_t n);
+int imsrg_tensor_download(imsrg_tensor t, double *vec, int64_t n);
+int imsrg_tensor_copy(imsrg_tensor src, imsrg_tensor dst);    /* copy_sq_op   basic_IMSRG.f90:2942-2969 */
+int imsrg_tensor_clear(imsrg_tensor t);                         /* clear_sq_op  :3537-3562 */
+/* C = ax A + bx B                                                 append_operator :3481-3510 */
+int imsrg_tensor_add(imsrg_tensor A, double ax, imsrg_tensor B, double bx, imsrg_tensor C);
+int imsrg_tensor_norm(imsrg_tensor t, double *out);             /* mat_frob_norm, tensor branch :3921-3952 */
+
+/* Terms of the scalar-tensor commutator [G, T] (TS_commutators.f90; G scalar,
+ * T the tensor).  imsrg_tensor_commutator runs the terms in mask in the order
+ * BCH_TENSOR calls them (IMSRG_MAGNUS.f90:427-446): 111 and 122 overwrite
+ * RES's one- and two-body parts, the others add to RES; RES's herm is the
+ * caller's (the reference's tests use -herm(G) herm(T), BCH_TENSOR herm(T)). */
+#define IMSRG_TS111 (1u << 0)      /* TS_commutator_111       TS_commutators.f90:10-55     */
+#define IMSRG_TS121 (1u << 1)      /* TS_commutator_121       :58-303                      */
+#define IMSRG_TS211 (1u << 2)      /* TS_commutator_211       :307-552                     */
+#define IMSRG_TS122 (1u << 3)      /* TS_commutator_122       :555-663                     */
+#define IMSRG_TS212 (1u << 4)      /* TS_commutator_212       :666-806                     */
+#define IMSRG_TS222PPHH (1u << 5)  /* TS_commutator_222_pp_hh :1070-1456                   */
+#define IMSRG_TS221 (1u << 6)      /* TS_commutator_221       :809-1067 (ladder products recomputed) */
+#define IMSRG_TS222PH (1u << 7)    /* TS_commutator_222_ph    :1459-1849 with calculate_single_pandya,
+                                      cross_coupled.f90:1416-1634 */
+#define IMSRG_TSALL 0xFFu
+int imsrg_tensor_commutator(imsrg_op G, imsrg_tensor T, imsrg_tensor RES, unsigned mask);
+/* BCH_TENSOR (IMSRG_MAGNUS.f90:389-465): HS <- sum_n ad_G^n HS / n!, n up to
+ * 29, stopping after the first term with ||term|| / n! < float32(1e-6);
+ * nterms = commutators evaluated.  Replaces transform_sq_op_BCH for rank > 0
+ * operators (:131-146). */
+int imsrg_bch_tensor(imsrg_op G, imsrg_tensor HS, int *nterms);
+
 /* ---- measurement (no counterpart in the reference, which only prints
  * omp_get_wtime, main_IMSRG.f90:46,308-309) ---------------------------------
  * While enabled, every engine kernel launch is bracketed by HIP events on the

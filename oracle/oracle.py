@@ -441,6 +441,22 @@ class RefRunner:
             out[k] = self._r(f"t_{k}.bin")
         return out
 
+    TENSOR_TERMS = ("111", "121", "211", "122", "212", "222pphh", "221", "222ph", "full")
+
+    def tensor(self, hG, G, rank, dpar, hT, seed):
+        """a random tensor T of the reference's generator (rank, dpar doubled)
+        and the TS_commutator terms of [G, T]: (T, {term: stream})"""
+        self.run("tensor", hG, self._w("G.bin", G), rank, dpar, hT, seed, os.path.join(self.tmp, "ts"))
+        return self._r("ts_T.bin"), {k: self._r(f"ts_{k}.bin") for k in self.TENSOR_TERMS}
+
+    def bch_tensor(self, hG, G, rank, dpar, hT, T):
+        self.run("tbch", hG, self._w("G.bin", G), rank, dpar, hT, self._w("T.bin", T), "tb.bin")
+        return self._r("tb.bin")
+
+    def time_tensor(self, hG, G, rank, dpar, hT, T, reps):
+        out = self.run("ttime", hG, self._w("G.bin", G), rank, dpar, hT, self._w("T.bin", T), reps)
+        return float(out.split("SECONDS_PER_TENSOR_COMMUTATOR")[1].split()[0])
+
     def comm(self, hL, L, hR, R):
         self.run("comm", hL, self._w("L.bin", L), hR, self._w("R.bin", R), "c.bin")
         return self._r("c.bin")

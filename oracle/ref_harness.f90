@@ -31,6 +31,15 @@
 !   readint fmt file out.bin hw com_hw beta A htype emax_file lmax_file
 !                                        -> the reference's interaction reader fmt (ascii, gz, me2j,
 !          me2b) on file into a zero operator (interaction_IO.f90), written as a stream
+!   tensor hG G.bin rank dpar hT seed outprefix -> a random tensor T (construct_random_rankX,
+!          brute_force_testing.f90:123-265; rank and dpar doubled) written to <outprefix>_T.bin,
+!          then each TS_commutator term of [G, T] alone (<outprefix>_<term>.bin) and the
+!          whole scalar-tensor commutator in BCH_TENSOR's call order (<outprefix>_full.bin)
+!   tbch   hG G.bin rank dpar hT T.bin out.bin -> BCH_TENSOR(G, T) (IMSRG_MAGNUS.f90:389-465)
+!   ttime  hG G.bin rank dpar hT T.bin reps   -> wall seconds per scalar-tensor commutator
+! Tensor operators travel as raw FP64 streams: E0, fhh, fpp, fph, then for every
+! tensor block (allocate_tensor's order, basic_IMSRG.f90:644-907) its nine
+! sub-blocks tgam(1..9), each column-major.
 !
 ! Built twice by oracle/ref.mk: ref_harness against the reference's own
 ! module commutators, and sub_harness (-DSUBSTITUTE) against the product's
@@ -42,6 +51,7 @@ program ref_harness
   use cross_coupled
   use commutators
   use IMSRG_MAGNUS
+  use TS_commutators
   use brute_force_testing
   use HF_mod
   use interaction_IO
@@ -102,6 +112,12 @@ program ref_harness
      call do_dhds()
   case ('readint')
      call do_readint()
+  case ('tensor')
+     call do_tensor()
+  case ('tbch')
+     call do_tbch()
+  case ('ttime')
+     call do_ttime()
   case default
      print*, 'unknown command ', trim(cmd)
      stop 2
@@ -389,6 +405,152 @@ contains
     end select
     call write_op(trim(sarg(5)), HS)
   end subroutine do_readint
+
+  subroutine write_tensor(fname, T)
+    character(len=*) :: fname
+    type(sq_op) :: T
+    integer :: q, g
+    open(unit=48, file=fname, access='stream', form='unformatted', status='replace')
+    write(48) T%E0
+    write(48) T%fhh
+    write(48) T%fpp
+    write(48) T%fph
+    do q = 1, T%nblocks
+       do g = 1, 9
+          if (size(T%tblck(q)%tgam(g)%X) > 0) write(48) T%tblck(q)%tgam(g)%X
+       end do
+    end do
+    close(48)
+  end subroutine write_tensor
+
+  subroutine read_tensor(fname, T)
+    character(len=*) :: fname
+    type(sq_op) :: T
+    integer :: q, g
+    open(unit=49, file=fname, access='stream', form='unformatted', status='old')
+    read(49) T%E0
+    read(49) T%fhh
+    read(49) T%fpp
+    read(49) T%fph
+    do q = 1, T%nblocks
+       do g = 1, 9
+          if (size(T%tblck(q)%tgam(g)%X) > 0) read(49) T%tblck(q)%tgam(g)%X
+       end do
+    end do
+    close(49)
+  end subroutine read_tensor
+
+  subroutine tensor_setup(G, T)
+    ! G (scalar, args 3-4) and the shape of T (rank, dpar in args 5-6; herm arg 7);
+    ! xmap_tensor / half6j slot 1 as in test_scalar_tensor_commutator
+    ! (brute_force_testing.f90:912-960) and its caller
+    type(sq_op) :: G, T
+    integer :: N
+    call duplicate_sq_op(HS, G)
+    G%herm = iarg(3)
+    call read_op(trim(sarg(4)), G)
+    T%rank = iarg(5)
+    T%dpar = iarg(6)
+    T%xindx = 1
+    T%pphh_ph = .false.
+    N = jbas%total_orbits
+    allocate(jbas%xmap_tensor(1, N*(N+1)/2))
+    allocate(half6j(1))
+    call allocate_tensor(jbas, T, HS)
+    T%herm = iarg(7)
+    T%E0 = 0.d0
+  end subroutine tensor_setup
+
+  subroutine scalar_tensor_comm(G, T, GCC, TCC, RES, w1, w2)
+    ! the call order of BCH_TENSOR (IMSRG_MAGNUS.f90:427-446)
+    type(sq_op) :: G, T, RES, w1, w2
+    type(cc_mat) :: GCC
+    type(pandya_mat) :: TCC
+    call TS_commutator_111(G, T, RES, jbas)
+    call TS_commutator_121(G, T, RES, jbas)
+    call TS_commutator_122(G, T, RES, jbas)
+    call TS_commutator_212(G, T, RES, jbas)
+    call TS_commutator_222_pp_hh(G, T, RES, w1, w2, jbas)
+    call TS_commutator_221(w1, w2, G%herm*T%herm, RES, jbas)
+    call TS_commutator_211(GCC, T, RES, jbas)
+    call TS_commutator_222_ph(GCC, TCC, T, RES, jbas)
+  end subroutine scalar_tensor_comm
+
+  subroutine do_tensor()
+    type(sq_op) :: G, T, RES, w1, w2
+    type(cc_mat) :: GCC
+    type(pandya_mat) :: TCC
+    character(len=1024) :: pfx
+    integer :: ns, i
+    integer, allocatable :: sd(:)
+    call tensor_setup(G, T)
+    call random_seed(size=ns)
+    allocate(sd(ns))
+    sd = [(iarg(8) * 7919 + 104729 * i, i = 1, ns)]
+    call random_seed(put=sd)
+    call construct_random_rankX(T, iarg(7), jbas)
+    pfx = sarg(9)
+    call write_tensor(trim(pfx)//'_T.bin', T)
+    call duplicate_sq_op(T, RES)
+    call duplicate_sq_op(T, w1)
+    call duplicate_sq_op(T, w2)
+    RES%herm = -1 * G%herm * T%herm
+    call init_ph_mat(G, GCC, jbas)
+    call allocate_small_tensor_CCMAT(T, TCC, jbas)
+    call calculate_cross_coupled(G, GCC, jbas)
+    call clear_sq_op(RES) ; call TS_commutator_111(G, T, RES, jbas)
+    call write_tensor(trim(pfx)//'_111.bin', RES)
+    call clear_sq_op(RES) ; call TS_commutator_121(G, T, RES, jbas)
+    call write_tensor(trim(pfx)//'_121.bin', RES)
+    call clear_sq_op(RES) ; call TS_commutator_211(GCC, T, RES, jbas)
+    call write_tensor(trim(pfx)//'_211.bin', RES)
+    call clear_sq_op(RES) ; call TS_commutator_122(G, T, RES, jbas)
+    call write_tensor(trim(pfx)//'_122.bin', RES)
+    call clear_sq_op(RES) ; call TS_commutator_212(G, T, RES, jbas)
+    call write_tensor(trim(pfx)//'_212.bin', RES)
+    call clear_sq_op(RES) ; call TS_commutator_222_pp_hh(G, T, RES, w1, w2, jbas)
+    call write_tensor(trim(pfx)//'_222pphh.bin', RES)
+    call clear_sq_op(RES) ; call TS_commutator_221(w1, w2, G%herm*T%herm, RES, jbas)
+    call write_tensor(trim(pfx)//'_221.bin', RES)
+    call clear_sq_op(RES) ; call TS_commutator_222_ph(GCC, TCC, T, RES, jbas)
+    call write_tensor(trim(pfx)//'_222ph.bin', RES)
+    call clear_sq_op(RES)
+    call scalar_tensor_comm(G, T, GCC, TCC, RES, w1, w2)
+    call write_tensor(trim(pfx)//'_full.bin', RES)
+  end subroutine do_tensor
+
+  subroutine do_tbch()
+    type(sq_op) :: G, T
+    call tensor_setup(G, T)
+    call read_tensor(trim(sarg(8)), T)
+    call BCH_TENSOR(G, T, jbas, 'n')
+    call write_tensor(trim(sarg(9)), T)
+  end subroutine do_tbch
+
+  subroutine do_ttime()
+    type(sq_op) :: G, T, RES, w1, w2
+    type(cc_mat) :: GCC
+    type(pandya_mat) :: TCC
+    integer :: reps, k
+    real(8) :: t0, t1, omp_get_wtime
+    call tensor_setup(G, T)
+    call read_tensor(trim(sarg(8)), T)
+    reps = iarg(9)
+    call duplicate_sq_op(T, RES)
+    call duplicate_sq_op(T, w1)
+    call duplicate_sq_op(T, w2)
+    RES%herm = -1 * G%herm * T%herm
+    call init_ph_mat(G, GCC, jbas)
+    call allocate_small_tensor_CCMAT(T, TCC, jbas)
+    t0 = omp_get_wtime()
+    do k = 1, reps
+       call clear_sq_op(RES)
+       call calculate_cross_coupled(G, GCC, jbas)
+       call scalar_tensor_comm(G, T, GCC, TCC, RES, w1, w2)
+    end do
+    t1 = omp_get_wtime()
+    write(*,'(A,es16.8)') 'SECONDS_PER_TENSOR_COMMUTATOR ', (t1 - t0) / reps
+  end subroutine do_ttime
 
   real(8) function rarg(k)
     integer :: k
