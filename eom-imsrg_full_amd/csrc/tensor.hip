@@ -47,6 +47,8 @@ struct DevTensor {
   const int *cidx;                // [((J3/2)*nJ + J4/2)*4 + Tz*2 + P] -> cross-coupled block or -1
   const int *ccol;                // [(scalar Pandya block * nsp + p) * nsp + h] -> column of (p,h)
   const double *x6, *s6;
+  const double *n9;               // 9j table of the ph recoupling (k_fill9j), null: evaluate on the fly
+  int nd;                         // rank/2 + 1: J4 - J3 and J2 - J1 steps the table holds
   long long f_off, v_off;
 };
 
@@ -82,6 +84,35 @@ __device__ inline double t9j(const DevTensor &T, int a, int b, int J1, int c, in
   for (int x = xmin; x <= xmax; x += 2)
     sm -= (x + 1.0) * tx6(T, J1, J2, x, a, b) * ts6(T, c, d, J2, b, x, J4) * tx6(T, J3, J4, x, a, c);
   return sm;
+}
+
+// The 9j symbols of the ph recoupling, {j1 j2 J3; j3 j4 J4; J1 J2 R} with
+// J3 <= J4 <= J3 + R and J1 <= J2 <= J1 + R, tabulated once per tensor space:
+// index (((((((k1*nh + k2)*nh + k3)*nh + k4)*nJ + J3/2)*nd + (J4-J3)/2)*nJ + J1/2)*nd + (J2-J1)/2
+__device__ __forceinline__ long long n9_index(const DevTensor &T, int j1, int j2, int j3, int j4, int J3, int J4,
+                                              int J1, int J2) {
+  const int h = T.nh;
+  return ((((((((long long)(j1 >> 1) * h + (j2 >> 1)) * h + (j3 >> 1)) * h + (j4 >> 1)) * T.nJ + (J3 >> 1)) * T.nd +
+            ((J4 - J3) >> 1)) * T.nJ + (J1 >> 1)) * T.nd + ((J2 - J1) >> 1));
+}
+__device__ __forceinline__ double t9j_ph(const DevTensor &T, int a, int b, int J3, int c, int d, int J4, int J1,
+                                         int J2) {
+  return T.n9 ? T.n9[n9_index(T, a, b, c, d, J3, J4, J1, J2)] : t9j(T, a, b, J3, c, d, J4, J1, J2);
+}
+__global__ void k_fill9j(DevTensor T, double *__restrict__ n9, long long n) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  long long u = e;
+  const int d2 = (int)(u % T.nd); u /= T.nd;
+  const int J1 = 2 * (int)(u % T.nJ); u /= T.nJ;
+  const int d4 = (int)(u % T.nd); u /= T.nd;
+  const int J3 = 2 * (int)(u % T.nJ); u /= T.nJ;
+  const int k4 = (int)(u % T.nh); u /= T.nh;
+  const int k3 = (int)(u % T.nh); u /= T.nh;
+  const int k2 = (int)(u % T.nh); u /= T.nh;
+  const int k1 = (int)u;
+  const int J4 = J3 + 2 * d4, J2 = J1 + 2 * d2, JM2 = 2 * (T.nJ - 1);
+  n9[e] = (J4 <= JM2 && J2 <= JM2) ? t9j(T, 2 * k1 + 1, 2 * k2 + 1, J3, 2 * k3 + 1, 2 * k4 + 1, J4, J1, J2) : 0.0;
 }
 
 // tensor_elem (basic_IMSRG.f90:1504-1641): position in V of T^{J1x J2x}_{ax bx, cx dx}
@@ -216,12 +247,14 @@ __global__ void k_t111(DevLayout L, DevTensor T, const double *__restrict__ LF, 
 // selection rules, F(p,q) += sum, then the lower entry becomes
 // herm * (-1)^((jp-jq)/2) F(p,q) (diagonal included), exactly as each
 // reference routine leaves it; (particle, hole) outputs also refresh the
-// derived (hole, particle) entry.
-__global__ void k_t_onebody(DevLayout L, DevTensor T, int kind, const double *__restrict__ LF,
-                            const double *__restrict__ RF, const double *__restrict__ RV,
-                            const double *__restrict__ gcc, const double *__restrict__ W1,
-                            const double *__restrict__ W2, double *__restrict__ F, int hL, int hR, int hres) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// derived (hole, particle) entry.  One wavefront per output: the lanes split
+// the intermediate orbits and reduce.
+__global__ __launch_bounds__(64) void k_t_onebody(DevLayout L, DevTensor T, int kind, const double *__restrict__ LF,
+                                                  const double *__restrict__ RF, const double *__restrict__ RV,
+                                                  const double *__restrict__ gcc, const double *__restrict__ W1,
+                                                  const double *__restrict__ W2, double *__restrict__ F, int hL,
+                                                  int hR, int hres) {
+  const int t = blockIdx.x, lane = threadIdx.x;
   if (t >= L.nob) return;
   const int p = L.ob_p[t], q = L.ob_q[t], n = L.nsp, R = T.rank;
   const int jp = L.jj[p], jq = L.jj[q], lp = L.l[p], lq = L.l[q];
@@ -230,7 +263,7 @@ __global__ void k_t_onebody(DevLayout L, DevTensor T, int kind, const double *__
   if (!dtri(jq, jp, R)) return;
   double sm = 0.0;
   if (kind == 0) {
-    for (int u = 0; u < L.nphp; u++) {
+    for (int u = lane; u < L.nphp; u += 64) {
       const int a = L.php_a[u], i = L.php_i[u];
       const int ja = L.jj[a], ji = L.jj[i];
       double smx = 0.0, smy = 0.0;
@@ -252,42 +285,38 @@ __global__ void k_t_onebody(DevLayout L, DevTensor T, int kind, const double *__
     const int rlen = L.ccrlen[qx];
     const double inv = 1.0 / sqrt(R + 1.0);
     const double c1 = sgnp((jp - jq + R) >> 1), c2 = (double)(hR * hL * sgnp(R >> 1));
-    for (int ai = 0; ai < T.npart; ai++) {
-      const int a = T.parts[ai], ja = L.jj[a], la = L.l[a];
-      for (int ii = 0; ii < T.nhole; ii++) {
-        const int i = T.holes[ii], ji = L.jj[i], li = L.l[i];
-        if ((li & 1) != ((la + (T.dpar >> 1)) & 1)) continue;
-        if (L.tz[i] != L.tz[a]) continue;
-        if (!dtri(ja, ji, R)) continue;
-        if (((la + li) & 1) != PAR) continue;
-        const int col = T.ccol[((size_t)qx * n + a) * n + i];
-        if (col < 0 || rpq < 0 || rqp < 0) continue;  // unreachable for allowed outputs
-        sm += RF[(size_t)a * n + i] *
-              (c1 * gcc[base + (long long)col * rlen + rpq] - c2 * gcc[base + (long long)col * rlen + rqp]) * inv;
-      }
+    for (int u = lane; u < T.npart * T.nhole; u += 64) {
+      const int a = T.parts[u / T.nhole], i = T.holes[u % T.nhole];
+      const int ja = L.jj[a], la = L.l[a], ji = L.jj[i], li = L.l[i];
+      if ((li & 1) != ((la + (T.dpar >> 1)) & 1)) continue;
+      if (L.tz[i] != L.tz[a]) continue;
+      if (!dtri(ja, ji, R)) continue;
+      if (((la + li) & 1) != PAR) continue;
+      const int col = T.ccol[((size_t)qx * n + a) * n + i];
+      if (col < 0 || rpq < 0 || rqp < 0) continue;  // unreachable for allowed outputs
+      sm += RF[(size_t)a * n + i] *
+            (c1 * gcc[base + (long long)col * rlen + rpq] - c2 * gcc[base + (long long)col * rlen + rqp]) * inv;
     }
   } else {
     const int pm = hL * hR, i = p, j = q, ji = jp, jj = jq;
-    for (int pass = 0; pass < 2; pass++) {  // c over holes with W1, over particles with W2
-      const int nc = pass == 0 ? T.nhole : T.npart;
-      const int *cl = pass == 0 ? T.holes : T.parts;
-      const double *W = pass == 0 ? W1 : W2;
-      for (int ck = 0; ck < nc; ck++) {
-        const int c = cl[ck], jc = L.jj[c];
-        double s1 = 0.0;
-        for (int J1 = abs(jc - ji); J1 <= jc + ji; J1 += 2) {
-          for (int J2 = abs(jc - jj); J2 <= min(J1 - 2, jc + jj); J2 += 2)
-            s1 -= sqrt((J1 + 1.0) * (J2 + 1.0)) * tx6(T, J1, J2, jj, ji, jc) *
-                  t_elem(L, T, W, hR, c, j, c, i, J2, J1) * sgnp(J2 >> 1) * pm;
-          for (int J2 = max(J1, abs(jc - jj)); J2 <= jc + jj; J2 += 2)
-            s1 += sqrt((J1 + 1.0) * (J2 + 1.0)) * tx6(T, J1, J2, jj, ji, jc) *
-                  t_elem(L, T, W, hR, c, i, c, j, J1, J2) * sgnp(J1 >> 1);
-        }
-        sm += s1 * sgnp((jc + jj) >> 1);
+    for (int u = lane; u < n; u += 64) {  // c over holes with W1, over particles with W2
+      const int c = u < T.nhole ? T.holes[u] : T.parts[u - T.nhole], jc = L.jj[c];
+      const double *W = u < T.nhole ? W1 : W2;
+      double s1 = 0.0;
+      for (int J1 = abs(jc - ji); J1 <= jc + ji; J1 += 2) {
+        for (int J2 = abs(jc - jj); J2 <= min(J1 - 2, jc + jj); J2 += 2)
+          s1 -= sqrt((J1 + 1.0) * (J2 + 1.0)) * tx6(T, J1, J2, jj, ji, jc) *
+                t_elem(L, T, W, hR, c, j, c, i, J2, J1) * sgnp(J2 >> 1) * pm;
+        for (int J2 = max(J1, abs(jc - jj)); J2 <= jc + jj; J2 += 2)
+          s1 += sqrt((J1 + 1.0) * (J2 + 1.0)) * tx6(T, J1, J2, jj, ji, jc) *
+                t_elem(L, T, W, hR, c, i, c, j, J1, J2) * sgnp(J1 >> 1);
       }
+      sm += s1 * sgnp((jc + jj) >> 1);
     }
     sm *= sgnp(R >> 1);
   }
+  sm = wave_sum(sm);
+  if (lane != 0) return;
   const size_t pq = (size_t)p * n + q, qp = (size_t)q * n + p;
   const double v = F[pq] + sm;
   F[pq] = v;
@@ -496,22 +525,22 @@ __global__ void k_t222ph(DevLayout L, DevTensor T, const double *__restrict__ Wx
         const double P = s12 * sqrt((J3 + 1.0) * (J4 + 1.0)), ph34 = sgnp((J3 + J4) >> 1);
         double v;
         switch (k) {
-          case 0: v = t9j(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((jb + jd + J2) >> 1) * ph34 * hL; break;
-          case 1: v = -t9j(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((jc + ja + J1 + R) >> 1) * hR; break;
-          case 2: v = t9j(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((jb + jc) >> 1) * ph34 * hL; break;
-          case 3: v = t9j(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((ja - jd + R) >> 1) * hR; break;
-          case 4: v = t9j(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((ja + jd + J1 + J2) >> 1) * ph34 * hL; break;
-          case 5: v = t9j(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((jc - jb + J1 + J2 + R) >> 1) * hR; break;
-          case 6: v = t9j(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((ja + jc + J1) >> 1) * ph34 * hL; break;
-          case 7: v = t9j(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((jb - jd + J2 + R) >> 1) * hR; break;
-          case 8: v = t9j(T, jd, ja, J3, jb, jc, J4, J1, J2) * sgnp((jc + ja + J2) >> 1) * hL; break;
-          case 9: v = t9j(T, ja, jd, J3, jc, jb, J4, J1, J2) * sgnp((jd - jb + J1 + R) >> 1) * ph34 * hR; break;
-          case 10: v = -t9j(T, jd, ja, J3, jb, jc, J4, J1, J2) * hL; break;
-          case 11: v = t9j(T, ja, jd, J3, jc, jb, J4, J1, J2) * abcd * ph34 * sgnp(R >> 1) * hR; break;
-          case 12: v = -t9j(T, jd, ja, J3, jb, jc, J4, J1, J2) * abcd * sgnp((J1 + J2) >> 1) * hL; break;
-          case 13: v = t9j(T, ja, jd, J3, jc, jb, J4, J1, J2) * sgnp((J1 + J2 + R) >> 1) * ph34 * hR; break;
-          case 14: v = t9j(T, jd, ja, J3, jb, jc, J4, J1, J2) * sgnp((jb + jd + J1) >> 1) * hL; break;
-          default: v = t9j(T, ja, jd, J3, jc, jb, J4, J1, J2) * sgnp((ja - jc + J2 + R) >> 1) * ph34 * hR; break;
+          case 0: v = t9j_ph(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((jb + jd + J2) >> 1) * ph34 * hL; break;
+          case 1: v = -t9j_ph(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((jc + ja + J1 + R) >> 1) * hR; break;
+          case 2: v = t9j_ph(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((jb + jc) >> 1) * ph34 * hL; break;
+          case 3: v = t9j_ph(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((ja - jd + R) >> 1) * hR; break;
+          case 4: v = t9j_ph(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((ja + jd + J1 + J2) >> 1) * ph34 * hL; break;
+          case 5: v = t9j_ph(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((jc - jb + J1 + J2 + R) >> 1) * hR; break;
+          case 6: v = t9j_ph(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((ja + jc + J1) >> 1) * ph34 * hL; break;
+          case 7: v = t9j_ph(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((jb - jd + J2 + R) >> 1) * hR; break;
+          case 8: v = t9j_ph(T, jd, ja, J3, jb, jc, J4, J1, J2) * sgnp((jc + ja + J2) >> 1) * hL; break;
+          case 9: v = t9j_ph(T, ja, jd, J3, jc, jb, J4, J1, J2) * sgnp((jd - jb + J1 + R) >> 1) * ph34 * hR; break;
+          case 10: v = -t9j_ph(T, jd, ja, J3, jb, jc, J4, J1, J2) * hL; break;
+          case 11: v = t9j_ph(T, ja, jd, J3, jc, jb, J4, J1, J2) * abcd * ph34 * sgnp(R >> 1) * hR; break;
+          case 12: v = -t9j_ph(T, jd, ja, J3, jb, jc, J4, J1, J2) * abcd * sgnp((J1 + J2) >> 1) * hL; break;
+          case 13: v = t9j_ph(T, ja, jd, J3, jc, jb, J4, J1, J2) * sgnp((J1 + J2 + R) >> 1) * ph34 * hR; break;
+          case 14: v = t9j_ph(T, jd, ja, J3, jb, jc, J4, J1, J2) * sgnp((jb + jd + J1) >> 1) * hL; break;
+          default: v = t9j_ph(T, ja, jd, J3, jc, jb, J4, J1, J2) * sgnp((ja - jc + J2 + R) >> 1) * ph34 * hR; break;
         }
         acc += P * v * w;
       }
@@ -615,6 +644,7 @@ struct TensorSpace {
   std::vector<long long> xoff, roff, woff, xstart, rstart;
   long long xsize = 0, rsize = 0, wsize = 0;
   std::vector<double> x6, s6;
+  long long n9size = 0;
   DevTensor dt{};
   double *gcc = nullptr, *ccx = nullptr, *ccr = nullptr, *wx = nullptr, *wy = nullptr;
   double *w1 = nullptr, *w2 = nullptr, *part = nullptr, *hpart = nullptr;
@@ -754,6 +784,19 @@ static int tspace_build(imsrg_ctx_s *c, int rank, int dpar, TensorSpace &S) {
   TUP(S.xoff, xoff); TUP(S.roff, roff); TUP(S.woff, woff); TUP(S.xstart, xstart); TUP(S.rstart, rstart);
   TUP(S.cidx, cidx); TUP(S.ccol, ccol); TUP(S.x6, x6); TUP(S.s6, s6);
 #undef TUP
+  // 9j table of the recoupling, when it fits in a few GiB
+  D.nd = rank / 2 + 1;
+  D.n9 = nullptr;
+  {
+    const long long n9 = (long long)nh * nh * nh * nh * nJ * nJ * D.nd * D.nd;
+    if (n9 <= (1LL << 29)) {
+      double *t9 = nullptr;
+      if (dalloc(c, n9, &t9)) return IMSRG_ENOMEM;
+      hipLaunchKernelGGL(k_fill9j, dim3((unsigned)((n9 + 255) / 256)), dim3(256), 0, c->stream, D, t9, n9);
+      D.n9 = t9;
+      S.n9size = n9;
+    }
+  }
   // products: the pp and hh ladders per tensor block (W1, W2)
   {
     std::vector<TProb> pr[2];
@@ -1028,7 +1071,7 @@ static int ts_comm(imsrg_op_s *G, imsrg_tensor T, imsrg_tensor RES, unsigned mas
   if (mask & IMSRG_TS111)
     hipLaunchKernelGGL(k_t111, dim3(nblk(nf)), dim3(256), 0, s, L, D, GF, RF, OF, hL, hR, hres);
   if (mask & IMSRG_TS121)
-    hipLaunchKernelGGL(k_t_onebody, dim3(nblk(L.nob)), dim3(256), 0, s, L, D, 0, GF, RF, RV, (const double *)S.gcc,
+    hipLaunchKernelGGL(k_t_onebody, dim3(L.nob), dim3(64), 0, s, L, D, 0, GF, RF, RV, (const double *)S.gcc,
                        (const double *)S.w1, (const double *)S.w2, OF, hL, hR, hres);
   if (S.nelem > 0 && (mask & IMSRG_TS122))
     hipLaunchKernelGGL(k_t122, dim3(nblk(S.nelem)), dim3(256), 0, s, L, D, GF, RV, OV, hR);
@@ -1045,11 +1088,11 @@ static int ts_comm(imsrg_op_s *G, imsrg_tensor T, imsrg_tensor RES, unsigned mas
     launch_axpby(s, S.vsize, 1.0, OV, -1.0, S.w2, OV, nullptr, 1, 0);
   }
   if (mask & IMSRG_TS221)
-    hipLaunchKernelGGL(k_t_onebody, dim3(nblk(L.nob)), dim3(256), 0, s, L, D, 2, GF, RF, RV, (const double *)S.gcc,
+    hipLaunchKernelGGL(k_t_onebody, dim3(L.nob), dim3(64), 0, s, L, D, 2, GF, RF, RV, (const double *)S.gcc,
                        (const double *)S.w1, (const double *)S.w2, OF, hL, hR, hres);
   if (mask & (IMSRG_TS211 | IMSRG_TS222PH)) launch_pandya(s, L, G->d, S.gcc, nullptr, 1);
   if (mask & IMSRG_TS211)
-    hipLaunchKernelGGL(k_t_onebody, dim3(nblk(L.nob)), dim3(256), 0, s, L, D, 1, GF, RF, RV, (const double *)S.gcc,
+    hipLaunchKernelGGL(k_t_onebody, dim3(L.nob), dim3(64), 0, s, L, D, 1, GF, RF, RV, (const double *)S.gcc,
                        (const double *)S.w1, (const double *)S.w2, OF, hL, hR, hres);
   if ((mask & IMSRG_TS222PH) && S.nelem > 0) {
     if (D.nx > 0) hipLaunchKernelGGL(k_tpandya, dim3(nblk(D.nx)), dim3(256), 0, s, L, D, 0, RV, S.ccx, hR);
