@@ -63,6 +63,10 @@ def parse():
                    help="instances of the batched emax=8 measurement, FP64 and int8x5 (0: skip)")
     p.add_argument("--gate-steps", type=int, default=200,
                    help="at most this many lockstep steps in the precision gate (the flow converges earlier)")
+    p.add_argument("--tensor-reps", type=int, default=5,
+                   help="scalar-tensor commutators timed at --emax (0: skip the tensor leg)")
+    p.add_argument("--tensor-rank", type=int, default=4, help="rank of the tensor leg, doubled (4: E2)")
+    p.add_argument("--tensor-only", action="store_true", help="run the tensor leg alone and print its record")
     p.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                    help="print the calibrated HBM traffic table from two rocprofv3 --pmc runs and exit")
     p.add_argument("--trace-summary", metavar="KERNEL_TRACE_CSV",
@@ -414,6 +418,75 @@ def hf_bench(ctx, bare_vec, reps):
     return {"ms": 1e3 * float(np.median(ts)), "iterations": it, "e_hf_MeV": e, "reps": reps}
 
 
+def tensor_bench(a, E, ctx, basis, eta_vec, cpu_mode):
+    """The tensor observable path: [G, T] for a rank a.tensor_rank (doubled)
+    even-parity tensor T with random elements and G = the white generator of
+    the workload Hamiltonian (anti-hermitian), timed over a.tensor_reps calls
+    at the workload size; BCH_TENSOR(ds * G, T) once; and, beside them, the
+    same commutator at emax=4 on the GPU and by the reference's
+    TS_commutators on the host cores (oracle/_ref ref_harness ttime, one call)."""
+    from imsrg_mi355x import Basis as _B, synthetic_hamiltonian
+    rank = a.tensor_rank
+    rng = np.random.default_rng(7)
+    G = ctx.op(-1, eta_vec)
+    T = ctx.tensor(rank, 0, 1)
+    tvec = rng.uniform(-1.0, 1.0, T.neq)
+    T.upload(tvec)
+    RES = ctx.tensor(rank, 0, 1)
+    ctx.tensor_commutator(G, T, RES)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.tensor_reps):
+        ctx.tensor_commutator(G, T, RES)
+    ctx.sync()
+    ms = 1e3 * (time.perf_counter() - t0) / a.tensor_reps
+    terms = {}
+    for name, m in E.TS_TERMS.items():
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.tensor_commutator(G, T, RES, mask=m)
+        ctx.sync()
+        terms[name] = round(1e3 * (time.perf_counter() - t0), 3)
+    Gs = ctx.op(-1, 0.5 * eta_vec)
+    Ts = ctx.tensor(rank, 0, 1, vec=tvec)
+    t0 = time.perf_counter()
+    nb = ctx.bch_tensor(Gs, Ts)
+    ctx.sync()
+    bch_ms = 1e3 * (time.perf_counter() - t0)
+    out = {"workload": f"[eta, T] at emax={basis_tag(basis)}: eta the white generator of the workload H, "
+                       f"T a rank-{rank // 2} even-parity tensor with uniform(-1,1) elements",
+           "tensor_neq": T.neq, "ms_per_commutator": ms, "commutators_per_s": 1e3 / ms, "ms_per_term": terms,
+           "bch_tensor": {"G": "0.5 eta", "commutators": nb, "ms": bch_ms}}
+    # like for like at emax=4 against the reference on the host cores
+    b4 = _B.ho(4, 3)
+    c4 = E.Context(b4)
+    H4 = c4.op(1, synthetic_hamiltonian(b4, seed=0, hw=20.0, sigma1=0.5, sigma2=1.0)).normal_order()
+    eta4 = c4.generator("white", H4).download()
+    T4 = c4.tensor(rank, 0, 1)
+    t4 = rng.uniform(-1.0, 1.0, T4.neq)
+    T4.upload(t4)
+    R4 = c4.tensor(rank, 0, 1)
+    G4 = c4.op(-1, eta4)
+    c4.tensor_commutator(G4, T4, R4)
+    c4.sync()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        c4.tensor_commutator(G4, T4, R4)
+    c4.sync()
+    out["emax4_gpu_ms"] = 1e3 * (time.perf_counter() - t0) / 10
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    if cpu_mode in ("auto", "reference") and O.ref_available():
+        threads, _ = host_cpu()
+        sec = O.RefRunner(b4, threads=threads).time_tensor(-1, eta4, rank, 0, 1, t4, 1)
+        out["cpu_baseline"] = {"value": 1.0 / sec, "unit": "scalar-tensor commutators/s", "cores": threads,
+                               "kind": "reference",
+                               "sample": f"one [eta, T] at emax=4 by TS_commutators on {threads} host threads, "
+                                         f"{sec:.3f} s (GPU at emax=4: {out['emax4_gpu_ms']:.3f} ms)"}
+    c4.close()
+    return out
+
+
 def batch_bench(a, E, Basis, batch_hamiltonians):
     """The batching workload: `a.batch` independent Hamiltonians (seeds 0..,
     hw in [16,24] MeV, sigma2 in [0.5,1.5] MeV) on one batched context, every
@@ -533,6 +606,12 @@ def main():
         return
     log(f"emax={a.emax} holes/iso={a.holes} orbits={basis.nsp} blocks={len(basis.blocks)} neq={basis.neq}")
     ctx = E.Context(basis)
+    if a.tensor_only:
+        H = ctx.op(1, synthetic_hamiltonian(basis, seed=0, hw=20.0, sigma1=0.5, sigma2=1.0))
+        H.normal_order()
+        eta = ctx.generator(a.generator, H).download()
+        print(json.dumps({"tensor": tensor_bench(a, E, ctx, basis, eta, a.cpu_baseline)}))
+        return
     if a.no_concurrency:
         ctx.set_concurrency(False)
     info = ctx.info()
@@ -615,6 +694,13 @@ def main():
     if a.batch8 > 0 and "fp64" in variants:
         log(f"batched flows: {a.batch8} instances at emax={a.emax}")
         batch8 = batch8_bench(a, E, basis, H0vec, variants["fp64"]["steps_per_s"])
+    tensor = None
+    if a.tensor_reps > 0:
+        log(f"scalar-tensor commutators (rank {a.tensor_rank // 2} tensor) at emax={a.emax}")
+        tensor = tensor_bench(a, E, ctx, basis, eta_vec if eta_vec is not None else
+                              ctx.generator(a.generator, ctx.op(1, H0vec)).download(), a.cpu_baseline)
+        log(f"  {tensor['ms_per_commutator']:.2f} ms per commutator; BCH_TENSOR "
+            f"{tensor['bch_tensor']['commutators']} commutators {tensor['bch_tensor']['ms']:.1f} ms")
     cpu = None
     if a.cpu_baseline != "none":
         log("timing the CPU baseline sample")
@@ -648,6 +734,7 @@ def main():
             "batch": batch,
             "batch_emax8": batch8,
             "hartree_fock": hf,
+            "tensor": tensor,
         },
         "precision_gate": gate,
         "roofline": roof,
