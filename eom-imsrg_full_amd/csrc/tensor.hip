@@ -520,7 +520,7 @@ __global__ void k_t222ph(DevLayout L, DevTensor T, const double *__restrict__ Wx
         const int JX = L.ccrow[((size_t)q4 * n + c) * n + b];
         if (JX < 0) continue;
         const long long wi = T.woff[qx] + (long long)IX * L.ccrlen[q4] + JX;
-        const double w = isx ? hL * Wx[wi] : Wy[wi];  // Wx carries LCC%herm (TS_commutators.f90:1490)
+        const double w = isx ? hL * Wx[wi] : Wy[wi];  // Wx carries LCC%herm (TS_commutators.f90:1506)
         if (!(fabs(w) > thr)) continue;
         const double P = s12 * sqrt((J3 + 1.0) * (J4 + 1.0)), ph34 = sgnp((J3 + J4) >> 1);
         double v;
@@ -824,7 +824,7 @@ static int tspace_build(imsrg_ctx_s *c, int rank, int dpar, TensorSpace &S) {
     if (upload_table(c, tiles, &S.pp_tiles) != 0) return IMSRG_ENOMEM;
     S.n_pp_tiles = (int)tiles.size();
   }
-  // the two Pandya products per cross-coupled block (TS_commutator_222_ph, TS_commutators.f90:1479-1515):
+  // the two Pandya products per cross-coupled block (TS_commutator_222_ph, TS_commutators.f90:1496-1521):
   // alpha carries 1/sqrt(J3+1) and 1/sqrt(J4+1); the herm of G that the reference also folds into Wx
   // is applied where k_t222ph reads Wx, so these tables serve every G
   {

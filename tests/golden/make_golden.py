@@ -2,7 +2,7 @@
 compiled FP64 routines (oracle/_ref/ref_harness, built by oracle/ref.mk from
 the reference sources).  Run in the build container only:
 
-    make -f oracle/ref.mk && python tests/golden/make_golden.py [emax2] [emax4] [hf] [emax6] [emax8]
+    make -f oracle/ref.mk && python tests/golden/make_golden.py [emax2] [emax4] [hf] [emax6] [emax8] [tensor]
 
 The fixtures hold inputs and outputs (vectorize-order FP64 streams), never
 reference source.  Inputs are stored too, so nothing depends on a random
@@ -153,6 +153,30 @@ def terms_sketch(emax, k, flow_steps):
     print(name, sum(np.asarray(v).nbytes for v in d.values()) / 1e6, "MB")
 
 
+TENSOR_CASES = {"E2": (4, 0, 1, -1), "E1hermG": (2, 2, 1, 1)}  # rank, dpar (doubled), herm T, herm G
+
+
+def tensor_fixtures():
+    """emax=2: for each tensor case, the scalar G (random, seed 11 + rank + dpar),
+    the tensor T of the reference's generator (construct_random_rankX, seed
+    5 + rank), every TS_commutator term of [G, T] alone, the whole commutator,
+    and BCH_TENSOR(0.02 G', T) with G' anti-hermitian (seed 23)"""
+    B = Basis.ho(2, 2)
+    ref = O.RefRunner(B, threads=2)
+    d = {}
+    for name, (rank, dpar, hT, hG) in TENSOR_CASES.items():
+        G = random_operator(B, hG, 11 + rank + dpar)
+        T, terms = ref.tensor(hG, G, rank, dpar, hT, 5 + rank)
+        d[f"{name}_G"], d[f"{name}_T"] = G, T
+        for k, v in terms.items():
+            d[f"{name}_{k}"] = v
+        Gs = random_operator(B, -1, 23) * 0.02
+        d[f"{name}_bchG"] = Gs
+        d[f"{name}_bch"] = ref.bch_tensor(-1, Gs, rank, dpar, hT, T)
+    np.savez_compressed(os.path.join(OUT, "tensor_emax2_k2.npz"), **d)
+    print("tensor_emax2_k2.npz", sum(np.asarray(v).nbytes for v in d.values()) / 1e6, "MB")
+
+
 if __name__ == "__main__":
     if not O.ref_available():
         sys.exit("oracle/_ref/ref_harness is missing: run make -f oracle/ref.mk first")
@@ -167,3 +191,5 @@ if __name__ == "__main__":
         terms_sketch(6, 3, 0)
     if "emax8" in what:
         terms_sketch(8, 6, 2)
+    if "tensor" in what:
+        tensor_fixtures()

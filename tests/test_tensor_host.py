@@ -40,6 +40,20 @@ def test_harness_tensor_stream_layout(case):
     assert np.linalg.norm(s - full) <= 1e-12 * np.linalg.norm(full)
 
 
+@needs_ref
+def test_tensor_fixture_matches_live_reference():
+    """the committed tensor fixture is what the reference build writes today"""
+    import os
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "tensor_emax2_k2.npz"))
+    B = Basis.ho(2, 2)
+    ref = O.RefRunner(B, threads=2)
+    rank, dpar, hT, hG = 4, 0, 1, -1
+    T, terms = ref.tensor(hG, d["E2_G"], rank, dpar, hT, 5 + rank)
+    assert np.array_equal(T, d["E2_T"])
+    for k, v in terms.items():
+        assert np.linalg.norm(v - d[f"E2_{k}"]) <= 1e-13 * np.linalg.norm(v), k
+
+
 def test_tensor_symbols_exported():
     from imsrg_mi355x import engine as E
     names = [n for n in E.header_symbols() if "tensor" in n]
