@@ -276,7 +276,8 @@ def trace_summary(trace_csv):
     import csv
     groups = {}
     for r in csv.DictReader(open(trace_csv)):
-        name = r["Kernel_Name"].split("(")[0].replace("imsrg::", "").replace("void ", "")
+        name = r["Kernel_Name"].replace("(anonymous namespace)::", "").split("(")[0]
+        name = name.replace("imsrg::", "").replace("void ", "")
         key = (int(r["Grid_Size_X"]), int(r["Grid_Size_Y"]), int(r["Grid_Size_Z"]))
         dur = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
         groups.setdefault(name, {}).setdefault(key, []).append(dur)

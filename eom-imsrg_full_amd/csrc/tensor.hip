@@ -26,6 +26,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <utility>
 
 #include "devfun.hpp"
 #include "engine_internal.hpp"
@@ -459,95 +460,104 @@ __global__ void k_tpandya(DevLayout L, DevTensor T, int mode, const double *__re
 }
 
 // The sixteen ways TS_commutator_222_ph (TS_commutators.f90:1459-1849) adds a
-// product entry into a stored element (x1 x2 | x3 x4; J1 J2): the source
+// product entry into a stored element (x0 x1 | x2 x3; J1 J2): the source
 // orbits (a,b,c,d) in terms of x, the orbit-order condition of the
 // reference's branch, and which of its 9j argument orders, phase and herm
-// factor apply.  The source entry is W(IX = (d,a) at J3, JX = (c,b) at J4).
-__device__ inline void ph_pattern(int k, const int x[4], int &a, int &b, int &c, int &d, bool &ok) {
-  ok = true;
-  switch (k) {
-    case 0: a = x[0]; b = x[1]; c = x[2]; d = x[3]; break;                          // X: (a,b,c,d)
-    case 1: c = x[0]; d = x[1]; a = x[2]; b = x[3]; break;                          // X: (c,d,a,b)
-    case 2: a = x[0]; b = x[1]; d = x[2]; c = x[3]; ok = x[2] < x[3]; break;        // X: (a,b,d,c)
-    case 3: d = x[0]; c = x[1]; a = x[2]; b = x[3]; ok = x[0] < x[1]; break;        // X: (d,c,a,b)
-    case 4: b = x[0]; a = x[1]; c = x[2]; d = x[3]; ok = x[0] < x[1]; break;        // X: (b,a,c,d)
-    case 5: c = x[0]; d = x[1]; b = x[2]; a = x[3]; ok = x[2] < x[3]; break;        // X: (c,d,b,a)
-    case 6: b = x[0]; a = x[1]; d = x[2]; c = x[3]; ok = x[0] < x[1] && x[2] < x[3]; break;  // X: (b,a,d,c)
-    case 7: d = x[0]; c = x[1]; b = x[2]; a = x[3]; ok = x[0] < x[1] && x[2] < x[3]; break;  // X: (d,c,b,a)
-    case 8: d = x[0]; b = x[1]; c = x[2]; a = x[3]; break;                          // Y: (d,b,c,a)
-    case 9: c = x[0]; a = x[1]; d = x[2]; b = x[3]; break;                          // Y: (c,a,d,b)
-    case 10: d = x[0]; b = x[1]; a = x[2]; c = x[3]; ok = x[2] < x[3]; break;       // Y: (d,b,a,c)
-    case 11: a = x[0]; c = x[1]; d = x[2]; b = x[3]; ok = x[0] < x[1]; break;       // Y: (a,c,d,b)
-    case 12: b = x[0]; d = x[1]; c = x[2]; a = x[3]; ok = x[0] < x[1]; break;       // Y: (b,d,c,a)
-    case 13: c = x[0]; a = x[1]; b = x[2]; d = x[3]; ok = x[2] < x[3]; break;       // Y: (c,a,b,d)
-    case 14: b = x[0]; d = x[1]; a = x[2]; c = x[3]; ok = x[0] < x[1] && x[2] < x[3]; break;  // Y: (b,d,a,c)
-    default: a = x[0]; c = x[1]; b = x[2]; d = x[3]; ok = x[0] < x[1] && x[2] < x[3]; break; // Y: (a,c,b,d)
+// factor apply.  The source entry is W(IX = (d,a) at J3, JX = (c,b) at J4),
+// Wx for K < 8, Wy for K >= 8.  K is a template argument so that every
+// pattern's orbit selection and coefficient resolve at compile time.
+template <int K>
+__device__ __forceinline__ void ph_add(const DevLayout &L, const DevTensor &T, const double *__restrict__ Wx,
+                                       const double *__restrict__ Wy, int x0, int x1, int x2, int x3, int J1, int J2,
+                                       double s12, int hL, int hR, double &acc) {
+  int a, b, c, d;
+  bool ok = true;
+  if constexpr (K == 0) { a = x0; b = x1; c = x2; d = x3; }                                  // X: (a,b,c,d)
+  if constexpr (K == 1) { c = x0; d = x1; a = x2; b = x3; }                                  // X: (c,d,a,b)
+  if constexpr (K == 2) { a = x0; b = x1; d = x2; c = x3; ok = x2 < x3; }                    // X: (a,b,d,c)
+  if constexpr (K == 3) { d = x0; c = x1; a = x2; b = x3; ok = x0 < x1; }                    // X: (d,c,a,b)
+  if constexpr (K == 4) { b = x0; a = x1; c = x2; d = x3; ok = x0 < x1; }                    // X: (b,a,c,d)
+  if constexpr (K == 5) { c = x0; d = x1; b = x2; a = x3; ok = x2 < x3; }                    // X: (c,d,b,a)
+  if constexpr (K == 6) { b = x0; a = x1; d = x2; c = x3; ok = x0 < x1 && x2 < x3; }         // X: (b,a,d,c)
+  if constexpr (K == 7) { d = x0; c = x1; b = x2; a = x3; ok = x0 < x1 && x2 < x3; }         // X: (d,c,b,a)
+  if constexpr (K == 8) { d = x0; b = x1; c = x2; a = x3; }                                  // Y: (d,b,c,a)
+  if constexpr (K == 9) { c = x0; a = x1; d = x2; b = x3; }                                  // Y: (c,a,d,b)
+  if constexpr (K == 10) { d = x0; b = x1; a = x2; c = x3; ok = x2 < x3; }                   // Y: (d,b,a,c)
+  if constexpr (K == 11) { a = x0; c = x1; d = x2; b = x3; ok = x0 < x1; }                   // Y: (a,c,d,b)
+  if constexpr (K == 12) { b = x0; d = x1; c = x2; a = x3; ok = x0 < x1; }                   // Y: (b,d,c,a)
+  if constexpr (K == 13) { c = x0; a = x1; b = x2; d = x3; ok = x2 < x3; }                   // Y: (c,a,b,d)
+  if constexpr (K == 14) { b = x0; d = x1; a = x2; c = x3; ok = x0 < x1 && x2 < x3; }        // Y: (b,d,a,c)
+  if constexpr (K == 15) { a = x0; c = x1; b = x2; d = x3; ok = x0 < x1 && x2 < x3; }        // Y: (a,c,b,d)
+  if (!ok) return;
+  constexpr bool isx = K < 8;
+  const int R = T.rank, n = L.nsp, JM = L.jtot_max;
+  const int ja = L.jj[a], jb = L.jj[b], jc = L.jj[c], jd = L.jj[d];
+  const int Tz = abs(L.tz[d] - L.tz[a]) >> 1, PAR = (L.l[d] + L.l[a]) & 1, PAR2 = PAR ^ ((T.dpar >> 1) & 1);
+  if (((L.l[c] + L.l[b]) & 1) != PAR2 || (abs(L.tz[c] - L.tz[b]) >> 1) != Tz) return;
+  const double abcd = sgnp((ja + jb + jc + jd) >> 1);
+  const double thr = (double)1e-6f;
+  for (int J3 = abs(jd - ja); J3 <= jd + ja; J3 += 2) {
+    const int q3 = (J3 >> 1) + Tz * (JM + 1) + 2 * PAR * (JM + 1);
+    const int IX = L.ccrow[((size_t)q3 * n + d) * n + a];
+    if (IX < 0) continue;
+    for (int J4 = max(abs(J3 - R), J3); J4 <= min(J3 + R, 2 * JM); J4 += 2) {
+      if (!dtri(jc, jb, J4)) continue;
+      if (!isx && J4 == J3) continue;  // no Y product where J3 = J4
+      const int qx = T.cidx[((J3 >> 1) * T.nJ + (J4 >> 1)) * 4 + Tz * 2 + PAR];
+      if (qx < 0) continue;
+      const int q4 = T.cq4[qx];
+      const int JX = L.ccrow[((size_t)q4 * n + c) * n + b];
+      if (JX < 0) continue;
+      const long long wi = T.woff[qx] + (long long)IX * L.ccrlen[q4] + JX;
+      const double w = isx ? hL * Wx[wi] : Wy[wi];  // Wx carries LCC%herm (TS_commutators.f90:1506)
+      if (!(fabs(w) > thr)) continue;
+      const double P = s12 * sqrt((J3 + 1.0) * (J4 + 1.0)), ph34 = sgnp((J3 + J4) >> 1);
+      double v;
+      if constexpr (K == 0) v = t9j_ph(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((jb + jd + J2) >> 1) * ph34 * hL;
+      if constexpr (K == 1) v = -t9j_ph(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((jc + ja + J1 + R) >> 1) * hR;
+      if constexpr (K == 2) v = t9j_ph(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((jb + jc) >> 1) * ph34 * hL;
+      if constexpr (K == 3) v = t9j_ph(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((ja - jd + R) >> 1) * hR;
+      if constexpr (K == 4)
+        v = t9j_ph(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((ja + jd + J1 + J2) >> 1) * ph34 * hL;
+      if constexpr (K == 5) v = t9j_ph(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((jc - jb + J1 + J2 + R) >> 1) * hR;
+      if constexpr (K == 6) v = t9j_ph(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((ja + jc + J1) >> 1) * ph34 * hL;
+      if constexpr (K == 7) v = t9j_ph(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((jb - jd + J2 + R) >> 1) * hR;
+      if constexpr (K == 8) v = t9j_ph(T, jd, ja, J3, jb, jc, J4, J1, J2) * sgnp((jc + ja + J2) >> 1) * hL;
+      if constexpr (K == 9) v = t9j_ph(T, ja, jd, J3, jc, jb, J4, J1, J2) * sgnp((jd - jb + J1 + R) >> 1) * ph34 * hR;
+      if constexpr (K == 10) v = -t9j_ph(T, jd, ja, J3, jb, jc, J4, J1, J2) * hL;
+      if constexpr (K == 11) v = t9j_ph(T, ja, jd, J3, jc, jb, J4, J1, J2) * abcd * ph34 * sgnp(R >> 1) * hR;
+      if constexpr (K == 12) v = -t9j_ph(T, jd, ja, J3, jb, jc, J4, J1, J2) * abcd * sgnp((J1 + J2) >> 1) * hL;
+      if constexpr (K == 13) v = t9j_ph(T, ja, jd, J3, jc, jb, J4, J1, J2) * sgnp((J1 + J2 + R) >> 1) * ph34 * hR;
+      if constexpr (K == 14) v = t9j_ph(T, jd, ja, J3, jb, jc, J4, J1, J2) * sgnp((jb + jd + J1) >> 1) * hL;
+      if constexpr (K == 15)
+        v = t9j_ph(T, ja, jd, J3, jc, jb, J4, J1, J2) * sgnp((ja - jc + J2 + R) >> 1) * ph34 * hR;
+      acc += P * v * w;
+    }
   }
+}
+
+template <int... K>
+__device__ __forceinline__ void ph_add_all(std::integer_sequence<int, K...>, const DevLayout &L, const DevTensor &T,
+                                           const double *__restrict__ Wx, const double *__restrict__ Wy, int x0,
+                                           int x1, int x2, int x3, int J1, int J2, double s12, int hL, int hR,
+                                           double &acc) {
+  (ph_add<K>(L, T, Wx, Wy, x0, x1, x2, x3, J1, J2, s12, hL, hR, acc), ...);
 }
 
 __global__ void k_t222ph(DevLayout L, DevTensor T, const double *__restrict__ Wx, const double *__restrict__ Wy,
                          double *__restrict__ OV, int hL, int hR) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  int q, x[4];
+  int q, x0, x1, x2, x3;
   long long vp;
-  if (!elem_of(L, T, e, q, x[0], x[1], x[2], x[3], vp)) return;
-  const int J1 = T.J1[q], J2 = T.J2[q], R = T.rank, n = L.nsp, dp = (T.dpar >> 1) & 1, JM = L.jtot_max;
-  if (x[0] == x[1] && ((J1 >> 1) & 1)) return;
-  if (x[2] == x[3] && ((J2 >> 1) & 1)) return;
+  if (!elem_of(L, T, e, q, x0, x1, x2, x3, vp)) return;
+  const int J1 = T.J1[q], J2 = T.J2[q];
+  if (x0 == x1 && ((J1 >> 1) & 1)) return;
+  if (x2 == x3 && ((J2 >> 1) & 1)) return;
   const double s12 = sqrt((J1 + 1.0) * (J2 + 1.0));
-  const double thr = (double)1e-6f;
   double acc = 0.0;
-  for (int k = 0; k < 16; k++) {
-    int a, b, c, d;
-    bool ok;
-    ph_pattern(k, x, a, b, c, d, ok);
-    if (!ok) continue;
-    const bool isx = k < 8;
-    const int ja = L.jj[a], jb = L.jj[b], jc = L.jj[c], jd = L.jj[d];
-    const int Tz = abs(L.tz[d] - L.tz[a]) >> 1, PAR = (L.l[d] + L.l[a]) & 1, PAR2 = PAR ^ dp;
-    if (((L.l[c] + L.l[b]) & 1) != PAR2 || (abs(L.tz[c] - L.tz[b]) >> 1) != Tz) continue;
-    const double abcd = sgnp((ja + jb + jc + jd) >> 1);
-    for (int J3 = abs(jd - ja); J3 <= jd + ja; J3 += 2) {
-      const int q3 = (J3 >> 1) + Tz * (JM + 1) + 2 * PAR * (JM + 1);
-      const int IX = L.ccrow[((size_t)q3 * n + d) * n + a];
-      if (IX < 0) continue;
-      for (int J4 = max(abs(J3 - R), J3); J4 <= min(J3 + R, 2 * JM); J4 += 2) {
-        if (!dtri(jc, jb, J4)) continue;
-        if (!isx && J4 == J3) continue;  // no Y product where J3 = J4
-        const int qx = T.cidx[((J3 >> 1) * T.nJ + (J4 >> 1)) * 4 + Tz * 2 + PAR];
-        if (qx < 0) continue;
-        const int q4 = T.cq4[qx];
-        const int JX = L.ccrow[((size_t)q4 * n + c) * n + b];
-        if (JX < 0) continue;
-        const long long wi = T.woff[qx] + (long long)IX * L.ccrlen[q4] + JX;
-        const double w = isx ? hL * Wx[wi] : Wy[wi];  // Wx carries LCC%herm (TS_commutators.f90:1506)
-        if (!(fabs(w) > thr)) continue;
-        const double P = s12 * sqrt((J3 + 1.0) * (J4 + 1.0)), ph34 = sgnp((J3 + J4) >> 1);
-        double v;
-        switch (k) {
-          case 0: v = t9j_ph(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((jb + jd + J2) >> 1) * ph34 * hL; break;
-          case 1: v = -t9j_ph(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((jc + ja + J1 + R) >> 1) * hR; break;
-          case 2: v = t9j_ph(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((jb + jc) >> 1) * ph34 * hL; break;
-          case 3: v = t9j_ph(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((ja - jd + R) >> 1) * hR; break;
-          case 4: v = t9j_ph(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((ja + jd + J1 + J2) >> 1) * ph34 * hL; break;
-          case 5: v = t9j_ph(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((jc - jb + J1 + J2 + R) >> 1) * hR; break;
-          case 6: v = t9j_ph(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((ja + jc + J1) >> 1) * ph34 * hL; break;
-          case 7: v = t9j_ph(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((jb - jd + J2 + R) >> 1) * hR; break;
-          case 8: v = t9j_ph(T, jd, ja, J3, jb, jc, J4, J1, J2) * sgnp((jc + ja + J2) >> 1) * hL; break;
-          case 9: v = t9j_ph(T, ja, jd, J3, jc, jb, J4, J1, J2) * sgnp((jd - jb + J1 + R) >> 1) * ph34 * hR; break;
-          case 10: v = -t9j_ph(T, jd, ja, J3, jb, jc, J4, J1, J2) * hL; break;
-          case 11: v = t9j_ph(T, ja, jd, J3, jc, jb, J4, J1, J2) * abcd * ph34 * sgnp(R >> 1) * hR; break;
-          case 12: v = -t9j_ph(T, jd, ja, J3, jb, jc, J4, J1, J2) * abcd * sgnp((J1 + J2) >> 1) * hL; break;
-          case 13: v = t9j_ph(T, ja, jd, J3, jc, jb, J4, J1, J2) * sgnp((J1 + J2 + R) >> 1) * ph34 * hR; break;
-          case 14: v = t9j_ph(T, jd, ja, J3, jb, jc, J4, J1, J2) * sgnp((jb + jd + J1) >> 1) * hL; break;
-          default: v = t9j_ph(T, ja, jd, J3, jc, jb, J4, J1, J2) * sgnp((ja - jc + J2 + R) >> 1) * ph34 * hR; break;
-        }
-        acc += P * v * w;
-      }
-    }
-  }
+  ph_add_all(std::make_integer_sequence<int, 16>{}, L, T, Wx, Wy, x0, x1, x2, x3, J1, J2, s12, hL, hR, acc);
   // add_elem_to_tensor stores V * pre, pre = sqrt 2 per identical pair
-  const double pre = (x[0] == x[1] ? M_SQRT2 : 1.0) * (x[2] == x[3] ? M_SQRT2 : 1.0);
+  const double pre = (x0 == x1 ? M_SQRT2 : 1.0) * (x2 == x3 ? M_SQRT2 : 1.0);
   OV[vp] += acc * pre;
 }
 
@@ -1056,17 +1066,20 @@ int imsrg_tensor_norm(imsrg_tensor t, double *out) {
 // [G, T] into RES in BCH_TENSOR's call order (IMSRG_MAGNUS.f90:427-446):
 // 111 (overwrites the one-body part), 121, 122 (overwrites the two-body
 // part), 212, 222_pp_hh, 221, 211, 222_ph; the others accumulate.  221 reads
-// the pp/hh ladder products, computed here whenever 221 is asked for.
-static int ts_comm(imsrg_op_s *G, imsrg_tensor T, imsrg_tensor RES, unsigned mask) {
-  imsrg_ctx_s *c = G->ctx;
+// the pp/hh ladder products, computed here whenever 221 is asked for.  Gd:
+// the scalar operator's buffer (may be null when only 211 / 222_ph run on
+// Pandya matrices already in S.gcc, gcc_ready).
+static int ts_comm(imsrg_ctx_s *c, const double *Gd, int hG, imsrg_tensor T, imsrg_tensor RES, unsigned mask,
+                   bool gcc_ready) {
   TensorSpace &S = *T->ts;
   const DevLayout &L = c->dl;
   const DevTensor &D = S.dt;
   hipStream_t s = c->stream;
-  const int hL = G->herm, hR = T->herm, hres = RES->herm;
-  const double *GF = G->d + c->lay.f_off, *GV = G->d + c->lay.v_off;
+  const int hL = hG, hR = T->herm, hres = RES->herm;
+  const double *GF = Gd ? Gd + c->lay.f_off : nullptr, *GV = Gd ? Gd + c->lay.v_off : nullptr;
   const double *RF = T->d + c->lay.f_off, *RV = T->d + S.v_off;
   double *OF = RES->d + c->lay.f_off, *OV = RES->d + S.v_off;
+  if (!Gd && (mask & ~(IMSRG_TS211 | IMSRG_TS222PH))) return fail(IMSRG_EINVAL, "terms need the scalar operator");
   const long long nf = (long long)D.nhole * D.nhole + (long long)D.npart * D.nhole + (long long)D.npart * D.npart;
   if (mask & IMSRG_TS111)
     hipLaunchKernelGGL(k_t111, dim3(nblk(nf)), dim3(256), 0, s, L, D, GF, RF, OF, hL, hR, hres);
@@ -1090,7 +1103,7 @@ static int ts_comm(imsrg_op_s *G, imsrg_tensor T, imsrg_tensor RES, unsigned mas
   if (mask & IMSRG_TS221)
     hipLaunchKernelGGL(k_t_onebody, dim3(L.nob), dim3(64), 0, s, L, D, 2, GF, RF, RV, (const double *)S.gcc,
                        (const double *)S.w1, (const double *)S.w2, OF, hL, hR, hres);
-  if (mask & (IMSRG_TS211 | IMSRG_TS222PH)) launch_pandya(s, L, G->d, S.gcc, nullptr, 1);
+  if ((mask & (IMSRG_TS211 | IMSRG_TS222PH)) && !gcc_ready) launch_pandya(s, L, Gd, S.gcc, nullptr, 1);
   if (mask & IMSRG_TS211)
     hipLaunchKernelGGL(k_t_onebody, dim3(L.nob), dim3(64), 0, s, L, D, 1, GF, RF, RV, (const double *)S.gcc,
                        (const double *)S.w1, (const double *)S.w2, OF, hL, hR, hres);
@@ -1115,7 +1128,70 @@ int imsrg_tensor_commutator(imsrg_op G, imsrg_tensor T, imsrg_tensor RES, unsign
   if (int rc = same_space(T, RES)) return rc;
   if (G->ctx != T->ctx) return fail(IMSRG_EINVAL, "G and T belong to different contexts");
   if (T == RES) return fail(IMSRG_EINVAL, "RES must differ from T");
-  return ts_comm(G, T, RES, mask);
+  return ts_comm(G->ctx, G->d, G->herm, T, RES, mask, false);
+}
+
+int imsrg_tensor_commutator_cc(imsrg_tensor T, const double *gcc, int hG, int64_t n, imsrg_tensor RES,
+                               unsigned mask) {
+  if (!T || !gcc || !RES) return fail(IMSRG_EINVAL, "null argument");
+  if (int rc = same_space(T, RES)) return rc;
+  if (T == RES) return fail(IMSRG_EINVAL, "RES must differ from T");
+  if (hG != 1 && hG != -1) return fail(IMSRG_EINVAL, "herm must be +1 or -1");
+  if (mask & ~(IMSRG_TS211 | IMSRG_TS222PH)) return fail(IMSRG_EINVAL, "only 211 and 222_ph run on Pandya matrices");
+  imsrg_ctx_s *c = T->ctx;
+  if (n != imsrg_cross_coupled_size(c)) return fail(IMSRG_EINVAL, "Pandya stream size mismatch");
+  // the reference's row order (all ordered pairs, first orbit major) into the
+  // device's paired rows; the zero rows stay zero
+  const Layout &Ly = c->lay;
+  std::vector<double> img(Ly.ccsize, 0.0);
+  int64_t o = 0;
+  for (int q = 0; q < Ly.ncc; q++) {
+    const int rr = Ly.ccrlen_ref[q], rd = Ly.ccrlen[q], nb = Ly.ccnph[q];
+    const int *ref = Ly.ccref.data() + Ly.ccrefbase[q];
+    for (int col = 0; col < nb; col++)
+      for (int r = 0; r < rr; r++) img[Ly.ccoff[q] + (int64_t)col * rd + ref[r]] = gcc[o++];
+  }
+  HIPCHK(hipMemcpyAsync(T->ts->gcc, img.data(), sizeof(double) * Ly.ccsize, hipMemcpyHostToDevice, c->stream));
+  const int rc = ts_comm(c, nullptr, hG, T, RES, mask, true);
+  HIPCHK(hipStreamSynchronize(c->stream));  // img is a local buffer
+  return rc;
+}
+
+int imsrg_tensor_ladders(imsrg_op G, imsrg_tensor T, imsrg_tensor W1, imsrg_tensor W2) {
+  if (!G || !T || !W1 || !W2) return fail(IMSRG_EINVAL, "null argument");
+  if (int rc = same_space(T, W1)) return rc;
+  if (int rc = same_space(T, W2)) return rc;
+  if (G->ctx != T->ctx) return fail(IMSRG_EINVAL, "G and T belong to different contexts");
+  imsrg_ctx_s *c = T->ctx;
+  TensorSpace &S = *T->ts;
+  hipStream_t s = c->stream;
+  const double *GV = G->d + c->lay.v_off, *RV = T->d + S.v_off;
+  launch_fill(s, W1->d, S.op_size, 0.0);
+  launch_fill(s, W2->d, S.op_size, 0.0);
+  if (S.n_pp_tiles > 0) {
+    hipLaunchKernelGGL(k_tgemm, dim3(S.n_pp_tiles), dim3(256), 0, s, S.pp_probs[0], S.pp_tiles, S.n_pp_tiles, GV, RV,
+                       W1->d + S.v_off);
+    hipLaunchKernelGGL(k_tgemm, dim3(S.n_pp_tiles), dim3(256), 0, s, S.pp_probs[1], S.pp_tiles, S.n_pp_tiles, GV, RV,
+                       W2->d + S.v_off);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int imsrg_tensor_commutator_221(imsrg_tensor W1, imsrg_tensor W2, int pm, imsrg_tensor RES) {
+  if (!W1 || !W2 || !RES) return fail(IMSRG_EINVAL, "null argument");
+  if (int rc = same_space(W1, RES)) return rc;
+  if (int rc = same_space(W2, RES)) return rc;
+  if (pm != 1 && pm != -1) return fail(IMSRG_EINVAL, "pm must be +1 or -1");
+  imsrg_ctx_s *c = RES->ctx;
+  TensorSpace &S = *RES->ts;
+  double *OF = RES->d + c->lay.f_off;
+  const double *RV = RES->d + S.v_off;  // not read by the 221 kernel
+  hipLaunchKernelGGL(k_t_onebody, dim3(c->dl.nob), dim3(64), 0, c->stream, c->dl, S.dt, 2, (const double *)OF,
+                     (const double *)OF, RV, (const double *)S.gcc, (const double *)(W1->d + S.v_off),
+                     (const double *)(W2->d + S.v_off), OF, pm, 1, RES->herm);
+  HIPCHK(hipGetLastError());
+  return 0;
 }
 
 int imsrg_bch_tensor(imsrg_op G, imsrg_tensor HS, int *nterms) {
@@ -1140,7 +1216,7 @@ int imsrg_bch_tensor(imsrg_op G, imsrg_tensor HS, int *nterms) {
     coef = coef / (iw - 1.0);
     if ((rc = imsrg_tensor_copy(&INT2, &AD))) break;
     launch_fill(c->stream, INT2.d, S.op_size, 0.0);
-    if ((rc = ts_comm(G, &AD, &INT2, IMSRG_TSALL))) break;
+    if ((rc = ts_comm(c, G->d, G->herm, &AD, &INT2, IMSRG_TSALL, false))) break;
     n++;
     launch_axpby(c->stream, S.op_size, 1.0, HS->d, coef, INT2.d, HS->d, nullptr, 1, 0);
     double nrm = 0.0;

@@ -348,6 +348,19 @@ int imsrg_tensor_commutator(imsrg_op G, imsrg_tensor T, imsrg_tensor RES, unsign
  * nterms = commutators evaluated.  Replaces transform_sq_op_BCH for rank > 0
  * operators (:131-146). */
 int imsrg_bch_tensor(imsrg_op G, imsrg_tensor HS, int *nterms);
+/* The reference's argument lists that carry Pandya matrices or ladder
+ * products instead of G, for a caller that holds them:
+ * TS_commutator_211(LCC,R,RES) and TS_commutator_222_ph(LCC,RCC,R,RES)
+ * (TS_commutators.f90:307, 1459) from G's Pandya matrices in the reference's
+ * cc_mat layout (the imsrg_cross_coupled stream, hG = LCC%herm; mask within
+ * IMSRG_TS211 | IMSRG_TS222PH; RCC is workspace there); the ladder products
+ * w1 (pp) and w2 (hh) that TS_commutator_222_pp_hh leaves for
+ * TS_commutator_221 (:1070-1456, two-body parts, the rest zero); and
+ * TS_commutator_221(w1,w2,pm,RES) (:809-1067) from given w1, w2. */
+int imsrg_tensor_commutator_cc(imsrg_tensor T, const double *gcc, int hG, int64_t n, imsrg_tensor RES,
+                               unsigned mask);
+int imsrg_tensor_ladders(imsrg_op G, imsrg_tensor T, imsrg_tensor W1, imsrg_tensor W2);
+int imsrg_tensor_commutator_221(imsrg_tensor W1, imsrg_tensor W2, int pm, imsrg_tensor RES);
 
 /* ---- measurement (no counterpart in the reference, which only prints
  * omp_get_wtime, main_IMSRG.f90:46,308-309) ---------------------------------

@@ -89,26 +89,31 @@ $(OUT)/drop_in_test: $(HERE)../tests/fortran/drop_in_test.f90 $(OBJS) $(OUT)/har
 	    $(OUT)/imsrg_gpu.o -J$(OUT) $(LIBS) -L$(LIBDIR) -limsrg_mi355x \
 	    -Wl,-rpath,'$$ORIGIN/../../eom-imsrg_full_amd/imsrg_mi355x/lib'
 
-# The link-time substitute: the product's `module commutators`
-# (eom-imsrg_full_amd/fortran/commutators_gpu.f90) in place of the reference's,
-# and the reference's callers of that module -- IMSRG_MAGNUS, generators,
-# IMSRG_ODE, brute_force_testing -- compiled unmodified against it into
+# The link-time substitutes: the product's `module commutators`
+# (eom-imsrg_full_amd/fortran/commutators_gpu.f90) and `module TS_commutators`
+# (ts_commutators_gpu.f90) in place of the reference's, and the reference's
+# callers of those modules -- IMSRG_MAGNUS, generators, IMSRG_ODE,
+# brute_force_testing -- compiled unmodified against them into
 # $(SUB); ref_harness.f90 linked the same way (with -DSUBSTITUTE, which adds
 # the device generators to its generator choices) is sub_harness.
 SUB := $(OUT)/sub
 SUBFLAGS := -J$(SUB) -I$(SUB) -I$(OUT)
 SUBOBJS := $(OUT)/anglib.o $(addprefix $(OUT)/,$(addsuffix .o,gzipmod basic_IMSRG cross_coupled \
-        isospin_operators three_body_routines HF_mod TS_commutators tensor_products gaulag operators \
+        isospin_operators three_body_routines HF_mod tensor_products gaulag operators \
         operator_commutators EOM_scalar_commutators EOM_TS_commutators EOM_dTZ_commutators adams_ode \
         interaction_IO)) \
-        $(addprefix $(SUB)/,commutators.o IMSRG_MAGNUS.o brute_force_testing.o IMSRG_ODE.o generators.o)
+        $(addprefix $(SUB)/,commutators.o TS_commutators.o IMSRG_MAGNUS.o brute_force_testing.o IMSRG_ODE.o \
+        generators.o)
 
 $(SUB)/commutators.o: $(HERE)../eom-imsrg_full_amd/fortran/commutators_gpu.f90 $(OUT)/cross_coupled.o
 	@mkdir -p $(SUB)
 	$(FC) $(FFLAGS) -c -o $@ $< $(SUBFLAGS)
-$(SUB)/IMSRG_MAGNUS.o: $(REFDIR)/IMSRG_MAGNUS.f90 $(SUB)/commutators.o $(OUT)/HF_mod.o $(OUT)/TS_commutators.o $(OUT)/operators.o $(OUT)/operator_commutators.o
+$(SUB)/TS_commutators.o: $(HERE)../eom-imsrg_full_amd/fortran/ts_commutators_gpu.f90 $(OUT)/cross_coupled.o
+	@mkdir -p $(SUB)
 	$(FC) $(FFLAGS) -c -o $@ $< $(SUBFLAGS)
-$(SUB)/brute_force_testing.o: $(REFDIR)/brute_force_testing.f90 $(SUB)/commutators.o $(OUT)/EOM_scalar_commutators.o $(OUT)/EOM_TS_commutators.o $(OUT)/EOM_dTZ_commutators.o $(OUT)/TS_commutators.o $(OUT)/operator_commutators.o $(OUT)/operators.o
+$(SUB)/IMSRG_MAGNUS.o: $(REFDIR)/IMSRG_MAGNUS.f90 $(SUB)/commutators.o $(OUT)/HF_mod.o $(SUB)/TS_commutators.o $(OUT)/operators.o $(OUT)/operator_commutators.o
+	$(FC) $(FFLAGS) -c -o $@ $< $(SUBFLAGS)
+$(SUB)/brute_force_testing.o: $(REFDIR)/brute_force_testing.f90 $(SUB)/commutators.o $(OUT)/EOM_scalar_commutators.o $(OUT)/EOM_TS_commutators.o $(OUT)/EOM_dTZ_commutators.o $(SUB)/TS_commutators.o $(OUT)/operator_commutators.o $(OUT)/operators.o
 	$(FC) $(FFLAGS) -c -o $@ $< $(SUBFLAGS)
 $(SUB)/IMSRG_ODE.o: $(REFDIR)/IMSRG_ODE.f90 $(SUB)/commutators.o $(OUT)/adams_ode.o $(OUT)/operators.o
 	$(FC) $(FFLAGS) -c -o $@ $< $(SUBFLAGS)

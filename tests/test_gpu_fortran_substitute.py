@@ -1,5 +1,6 @@
-"""The link-time substitute for the reference's `module commutators`
-(eom-imsrg_full_amd/fortran/commutators_gpu.f90) against the reference itself.
+"""The link-time substitutes for the reference's `module commutators` and
+`module TS_commutators` (eom-imsrg_full_amd/fortran/commutators_gpu.f90,
+ts_commutators_gpu.f90) against the reference itself.
 
 oracle/ref.mk builds the same harness program twice: ref_harness against the
 reference's own module commutators, and sub_harness against the substitute,
@@ -15,7 +16,11 @@ same inputs; every output must agree to 1e-10 relative:
   BCH_EXPAND, MAGNUS_EXPAND, IMSRG_ODE's dHds;
 * magnus_decouple itself (its step lines, H(s), Omega(s)), with the
   reference's generators and with the device generators passed as its
-  external build_generator.
+  external build_generator;
+* every TS_commutator routine in its contract (TS_commutator_221 on the w1,
+  w2 that TS_commutator_222_pp_hh left, 211 and 222_ph on the caller's
+  Pandya matrices), BCH_TENSOR's call sequence and BCH_TENSOR itself
+  (IMSRG_MAGNUS.f90:389-465, compiled unmodified against the substitute).
 """
 import os
 
@@ -85,3 +90,19 @@ def test_magnus_decouple(pair, kind, ref_kind):
     assert np.allclose(rows[1:, 3], rows0[1:, 3], rtol=1e-6, atol=0)
     assert_close(hs, hs0, what=f"magnus_decouple H(s) {kind}")
     assert_close(g, g0, what=f"magnus_decouple Omega(s) {kind}")
+
+
+@pytest.mark.parametrize("case", [(4, 0, 1, -1), (2, 2, -1, 1)], ids=["E2", "E1anti"])
+def test_ts_routines_and_bch_tensor(pair, case):
+    B, ref, sub, Hn = pair
+    rank, dpar, hT, hG = case
+    G = random_operator(B, hG, 11 + rank)
+    T, want = ref.tensor(hG, G, rank, dpar, hT, 5 + rank)
+    T2, got = sub.tensor(hG, G, rank, dpar, hT, 5 + rank)
+    assert np.array_equal(T, T2)  # the reference's generator, same seed, in both programs
+    for k in ref.TENSOR_TERMS:
+        print(f"[substitute] TS {k}: rel {rel(got[k], want[k]):.2e}", flush=True)
+        assert_close(got[k], want[k], what=f"TS_commutator {k}")
+    Gs = random_operator(B, -1, 23) * 0.02
+    assert_close(sub.bch_tensor(-1, Gs, rank, dpar, hT, T), ref.bch_tensor(-1, Gs, rank, dpar, hT, T),
+                 what="BCH_TENSOR through the substitute")
