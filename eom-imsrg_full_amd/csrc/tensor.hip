@@ -50,6 +50,8 @@ struct DevTensor {
   const double *x6, *s6;
   const double *n9;               // 9j table of the ph recoupling (k_fill9j), null: evaluate on the fly
   int nd;                         // rank/2 + 1: J4 - J3 and J2 - J1 steps the table holds
+  const unsigned *quads;          // orbit quadruples x0|x1<<8|x2<<16|x3<<24 (x0 <= x1, x2 <= x3) with outputs
+  long long nquad;
   long long f_off, v_off;
 };
 
@@ -561,6 +563,161 @@ __global__ void k_t222ph(DevLayout L, DevTensor T, const double *__restrict__ Wx
   OV[vp] += acc * pre;
 }
 
+// The same recoupling by orbit quadruple (x0 <= x1 | x2 <= x3), one wavefront
+// each.  The product entries a quadruple's outputs read do not depend on the
+// outputs' (J1, J2): they are gathered once, over (pattern, J3, J4), with the
+// parts of their weights that do not depend on (J1, J2) (sign, herm, (J3,J4)
+// phase and norm, the |W| test) and the (J1, J2)-independent part of their 9j
+// table index, into a compacted LDS list; each output (J1, J2) of the
+// quadruple then sums the list across the lanes.  Needs the 9j table.
+struct PhSlot {
+  double c;      // sign * herm * phase(J3,J4) * sqrt((2J3+1)(2J4+1)) * W
+  unsigned b9;   // 9j table index without the (J1, J2) part
+  int ab;        // bit 0: (-1)^(J1/2) applies, bit 1: (-1)^(J2/2) applies
+};
+__global__ __launch_bounds__(64) void k_t222ph_q(DevLayout L, DevTensor T, const double *__restrict__ Wx,
+                                                 const double *__restrict__ Wy, double *__restrict__ OV, int hL,
+                                                 int hR) {
+  extern __shared__ double ph_lds[];
+  PhSlot *slot = reinterpret_cast<PhSlot *>(ph_lds);
+  __shared__ int pa[16], pb[16], pc[16], pd[16], pok[16], pab[16], pph[16], pj[16][4];
+  __shared__ double pcs[16];
+  const long long qd = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (qd >= T.nquad) return;
+  const unsigned code = T.quads[qd];
+  const int x0 = code & 255, x1 = (code >> 8) & 255, x2 = (code >> 16) & 255, x3 = (int)(code >> 24);
+  const int R = T.rank, n = L.nsp, JM = L.jtot_max, nd = T.nd, nJ = T.nJ, h = T.nh;
+  if (lane < 16) {
+    // (a, b, c, d) as indices into (x0, x1, x2, x3), two bits each, pattern k in bits 2k..2k+1
+    const int k = lane;
+    const unsigned ia = 0x2727DD88u, ib = 0x88DD88DDu, ic = 0x72727272u, id = 0xDD882727u;
+    auto pick = [&](unsigned tbl) {
+      const int i = (tbl >> (2 * k)) & 3;
+      return i == 0 ? x0 : i == 1 ? x1 : i == 2 ? x2 : x3;
+    };
+    const int a = pick(ia), b = pick(ib), c = pick(ic), d = pick(id);
+    // the branch condition of the reference: 1 x0 < x1, 2 x2 < x3, 3 both
+    const unsigned cond = (0xF960F960u >> (2 * k)) & 3;
+    bool ok = (!(cond & 1) || x0 < x1) && (!(cond & 2) || x2 < x3);
+    const int ja = L.jj[a], jb = L.jj[b], jc = L.jj[c], jd = L.jj[d];
+    const int Tz = abs(L.tz[d] - L.tz[a]) >> 1, PAR = (L.l[d] + L.l[a]) & 1, PAR2 = PAR ^ ((T.dpar >> 1) & 1);
+    if (((L.l[c] + L.l[b]) & 1) != PAR2 || (abs(L.tz[c] - L.tz[b]) >> 1) != Tz) ok = false;
+    // constant sign exponent (halves), J1/J2 flags and the (J3,J4) phase of each pattern
+    // (TS_commutators.f90:1562-1849, in the order of ph_add)
+    int e = 0, ab = 0;
+    double sg = 1.0;
+    const int abcd = (ja + jb + jc + jd) >> 1;
+    switch (k) {
+      case 0: e = (jb + jd) >> 1; ab = 2; break;
+      case 1: e = (jc + ja + R) >> 1; ab = 1; sg = -1.0; break;
+      case 2: e = (jb + jc) >> 1; break;
+      case 3: e = (ja - jd + R) >> 1; break;
+      case 4: e = (ja + jd) >> 1; ab = 3; break;
+      case 5: e = (jc - jb + R) >> 1; ab = 3; break;
+      case 6: e = (ja + jc) >> 1; ab = 1; break;
+      case 7: e = (jb - jd + R) >> 1; ab = 2; break;
+      case 8: e = (jc + ja) >> 1; ab = 2; break;
+      case 9: e = (jd - jb + R) >> 1; ab = 1; break;
+      case 10: sg = -1.0; break;
+      case 11: e = abcd + (R >> 1); break;
+      case 12: e = abcd; ab = 3; sg = -1.0; break;
+      case 13: e = R >> 1; ab = 3; break;
+      case 14: e = (jb + jd) >> 1; ab = 1; break;
+      default: e = (ja - jc + R) >> 1; ab = 2; break;
+    }
+    const bool ph34 = (k < 8) ? !(k & 1) : (k & 1);
+    // the 9j argument order of the pattern: (ja,jd | jb,jc), (jd,ja | jc,jb), (jd,ja | jb,jc), (ja,jd | jc,jb)
+    const int var = (k < 8) ? (k & 1) : 2 + (k & 1);
+    pa[k] = a; pb[k] = b; pc[k] = c; pd[k] = d;
+    pok[k] = ok ? (Tz | (PAR << 1) | 4) : 0;
+    pab[k] = ab;
+    pph[k] = ph34;
+    pcs[k] = sg * sgnp(e) * ((k & 1) ? hR : hL);
+    pj[k][0] = (var == 0 || var == 3) ? ja : jd;
+    pj[k][1] = (var == 0 || var == 3) ? jd : ja;
+    pj[k][2] = (var == 0 || var == 2) ? jb : jc;
+    pj[k][3] = (var == 0 || var == 2) ? jc : jb;
+  }
+  __syncthreads();
+  // gather: slots (pattern, J3, J4 - J3), compacted to the entries that pass
+  const int S = nJ * nd, nslot = 16 * S;
+  const double thr = (double)1e-6f;
+  int count = 0;
+  for (int s0 = 0; s0 < nslot; s0 += 64) {
+    const int s = s0 + lane;
+    bool live = false;
+    double cval = 0.0;
+    unsigned b9 = 0;
+    int abk = 0;
+    if (s < nslot) {
+      const int k = s / S, r = s - k * S, J3 = 2 * (r / nd), J4 = J3 + 2 * (r % nd);
+      const int okc = pok[k];
+      if (okc) {
+        const int a = pa[k], b = pb[k], c = pc[k], d = pd[k], Tz = okc & 1, PAR = (okc >> 1) & 1;
+        const int ja = L.jj[a], jb = L.jj[b], jc = L.jj[c], jd = L.jj[d];
+        const bool isx = k < 8;
+        if (dtri(jd, ja, J3) && J4 <= 2 * JM && dtri(J3, J4, R) && dtri(jc, jb, J4) && (isx || J4 != J3)) {
+          const int q3 = (J3 >> 1) + Tz * (JM + 1) + 2 * PAR * (JM + 1);
+          const int IX = L.ccrow[((size_t)q3 * n + d) * n + a];
+          const int qx = T.cidx[((J3 >> 1) * nJ + (J4 >> 1)) * 4 + Tz * 2 + PAR];
+          if (IX >= 0 && qx >= 0) {
+            const int q4 = T.cq4[qx];
+            const int JX = L.ccrow[((size_t)q4 * n + c) * n + b];
+            if (JX >= 0) {
+              const long long wi = T.woff[qx] + (long long)IX * L.ccrlen[q4] + JX;
+              const double w = isx ? hL * Wx[wi] : Wy[wi];  // Wx carries LCC%herm (TS_commutators.f90:1506)
+              if (fabs(w) > thr) {
+                live = true;
+                cval = pcs[k] * (pph[k] ? (double)sgnp((J3 + J4) >> 1) : 1.0) * sqrt((J3 + 1.0) * (J4 + 1.0)) * w;
+                b9 = (unsigned)((((((pj[k][0] >> 1) * h + (pj[k][1] >> 1)) * h + (pj[k][2] >> 1)) * h +
+                                  (pj[k][3] >> 1)) * nJ + (J3 >> 1)) * nd + ((J4 - J3) >> 1)) * (unsigned)(nJ * nd);
+                abk = pab[k];
+              }
+            }
+          }
+        }
+      }
+    }
+    const unsigned long long m = __ballot(live);
+    if (live) {
+      const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
+      slot[pos].c = cval;
+      slot[pos].b9 = b9;
+      slot[pos].ab = abk;
+    }
+    count += __popcll(m);
+  }
+  __syncthreads();
+  if (count == 0) return;
+  // outputs (x0 x1 | x2 x3; J1 <= J2) of the quadruple
+  const int j0 = L.jj[x0], j1 = L.jj[x1], j2 = L.jj[x2], j3 = L.jj[x3];
+  const int Tzq = (L.tz[x0] + L.tz[x1]) / 2 + 1, P1 = (L.l[x0] + L.l[x1]) & 1;
+  const double pre = (x0 == x1 ? M_SQRT2 : 1.0) * (x2 == x3 ? M_SQRT2 : 1.0);
+  for (int J1 = abs(j0 - j1); J1 <= j0 + j1; J1 += 2) {
+    if (x0 == x1 && ((J1 >> 1) & 1)) continue;
+    for (int J2 = max(J1, abs(j2 - j3)); J2 <= min(j2 + j3, J1 + R); J2 += 2) {
+      if (x2 == x3 && ((J2 >> 1) & 1)) continue;
+      if (J1 + J2 < R) continue;
+      const int q = T.bidx[((J1 >> 1) * nJ + (J2 >> 1)) * 6 + Tzq * 2 + P1];
+      if (q < 0) continue;
+      const int r = dpos(L, J1, x0, x1), sc = dpos(L, J2, x2, x3);
+      if ((r | sc) < 0) continue;
+      const unsigned off = (unsigned)((J1 >> 1) * nd + ((J2 - J1) >> 1));
+      const int s1 = (J1 >> 1) & 1, s2 = (J2 >> 1) & 1;
+      double acc = 0.0;
+      for (int t = lane; t < count; t += 64) {
+        const PhSlot sl = slot[t];
+        const int e = ((sl.ab & 1) ? s1 : 0) + ((sl.ab & 2) ? s2 : 0);
+        const double v = sl.c * T.n9[sl.b9 + off];
+        acc += (e & 1) ? -v : v;
+      }
+      acc = wave_sum(acc);
+      if (lane == 0) OV[T.off[q] + (long long)r * T.n2[q] + sc] += acc * sqrt((J1 + 1.0) * (J2 + 1.0)) * pre;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- products
 // 64 x 64 tiles, 4 waves of 32 x 32 on v_mfma_f64_16x16x4; tiles (problem, tile row, tile column)
 __global__ __launch_bounds__(256) void k_tgemm(const TProb *__restrict__ probs, const int4 *__restrict__ tiles,
@@ -806,6 +963,38 @@ static int tspace_build(imsrg_ctx_s *c, int rank, int dpar, TensorSpace &S) {
       D.n9 = t9;
       S.n9size = n9;
     }
+  }
+  // orbit quadruples of the recoupling gather (k_t222ph_q): row pair x0 <= x1,
+  // column pair x2 <= x3, with at least one stored output
+  D.quads = nullptr;
+  D.nquad = 0;
+  if (D.n9 && nsp <= 256) {
+    std::vector<unsigned> quads;
+    std::vector<int> pa_, pb_;
+    for (int a = 0; a < nsp; a++)
+      for (int b = a; b < nsp; b++) { pa_.push_back(a); pb_.push_back(b); }
+    for (size_t u = 0; u < pa_.size(); u++) {
+      const int x0 = pa_[u], x1 = pb_[u], j0 = Ly.jj[x0], j1 = Ly.jj[x1];
+      const int t1 = Ly.tz[x0] + Ly.tz[x1], p1 = (Ly.l[x0] + Ly.l[x1]) & 1;
+      for (size_t v = 0; v < pa_.size(); v++) {
+        const int x2 = pa_[v], x3 = pb_[v];
+        if (Ly.tz[x2] + Ly.tz[x3] != t1 || ((Ly.l[x2] + Ly.l[x3]) & 1) != (p1 ^ dp)) continue;
+        const int j2 = Ly.jj[x2], j3 = Ly.jj[x3];
+        bool any = false;
+        for (int J1 = std::abs(j0 - j1); J1 <= j0 + j1 && !any; J1 += 2) {
+          if (x0 == x1 && ((J1 >> 1) & 1)) continue;
+          for (int J2 = std::max(J1, std::abs(j2 - j3)); J2 <= std::min(j2 + j3, J1 + rank); J2 += 2) {
+            if (x2 == x3 && ((J2 >> 1) & 1)) continue;
+            if (J1 + J2 < rank) continue;
+            any = true;
+            break;
+          }
+        }
+        if (any) quads.push_back((unsigned)x0 | (unsigned)x1 << 8 | (unsigned)x2 << 16 | (unsigned)x3 << 24);
+      }
+    }
+    if (upload_table(c, quads, &D.quads) != 0) return IMSRG_ENOMEM;
+    D.nquad = (long long)quads.size();
   }
   // products: the pp and hh ladders per tensor block (W1, W2)
   {
@@ -1116,8 +1305,13 @@ static int ts_comm(imsrg_ctx_s *c, const double *Gd, int hG, imsrg_tensor T, ims
     if (S.n_cc_tiles[1] > 0)
       hipLaunchKernelGGL(k_tgemm, dim3(S.n_cc_tiles[1]), dim3(256), 0, s, S.cc_probs[1], S.cc_tiles[1],
                          S.n_cc_tiles[1], (const double *)S.ccx, (const double *)S.gcc, S.wy);
-    hipLaunchKernelGGL(k_t222ph, dim3(nblk(S.nelem)), dim3(256), 0, s, L, D, (const double *)S.wx,
-                       (const double *)S.wy, OV, hL, hR);
+    if (D.quads && D.nquad > 0)
+      hipLaunchKernelGGL(k_t222ph_q, dim3((unsigned)D.nquad), dim3(64),
+                         (unsigned)(sizeof(PhSlot) * 16 * D.nJ * D.nd), s, L, D, (const double *)S.wx,
+                         (const double *)S.wy, OV, hL, hR);
+    else
+      hipLaunchKernelGGL(k_t222ph, dim3(nblk(S.nelem)), dim3(256), 0, s, L, D, (const double *)S.wx,
+                         (const double *)S.wy, OV, hL, hR);
   }
   HIPCHK(hipGetLastError());
   return 0;
