@@ -102,6 +102,16 @@ __device__ __forceinline__ double t9j_ph(const DevTensor &T, int a, int b, int J
                                          int J2) {
   return T.n9 ? T.n9[n9_index(T, a, b, c, d, J3, J4, J1, J2)] : t9j(T, a, b, J3, c, d, J4, J1, J2);
 }
+// {a b J3; c d J4; J3p J4p R} with J3 <= J4 <= J3 + R (a cross-coupled block) and any (J3p, J4p):
+// from the table, through the exchange of the first two columns (a factor (-1)^(sum of all nine))
+// when J4p < J3p
+__device__ __forceinline__ double t9j_cc(const DevTensor &T, int a, int b, int J3, int c, int d, int J4, int J3p,
+                                         int J4p) {
+  if (!T.n9 || J4p - J3p > T.rank || J3p - J4p > T.rank) return t9j(T, a, b, J3, c, d, J4, J3p, J4p);
+  if (J3p <= J4p) return T.n9[n9_index(T, a, b, c, d, J3, J4, J3p, J4p)];
+  const double v = T.n9[n9_index(T, b, a, d, c, J3, J4, J4p, J3p)];
+  return ((a + b + J3 + c + d + J4 + J3p + J4p + T.rank) >> 1) & 1 ? -v : v;
+}
 __global__ void k_fill9j(DevTensor T, double *__restrict__ n9, long long n) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
@@ -450,7 +460,7 @@ __global__ void k_tpandya(DevLayout L, DevTensor T, int mode, const double *__re
           const long long pos = mode == 0 ? t_pos(L, T, hR, a, h, p, b, J3p, J4p, f)
                                           : t_pos(L, T, hR, h, a, b, p, J3p, J4p, f);
           if (pos < 0) continue;
-          const double nj = mode == 0 ? t9j(T, ja, jb, J3, jh, jp, J4, J3p, J4p) : t9j(T, jh, jp, J3, ja, jb, J4, J3p, J4p);
+          const double nj = mode == 0 ? t9j_cc(T, ja, jb, J3, jh, jp, J4, J3p, J4p) : t9j_cc(T, jh, jp, J3, ja, jb, J4, J3p, J4p);
           sm -= sgnp(J4p >> 1) * sqrt((J3p + 1.0) * (J4p + 1.0)) * nj * RV[pos] * f;
         }
       const double s34 = sqrt((J3 + 1.0) * (J4 + 1.0));
@@ -580,7 +590,7 @@ __global__ __launch_bounds__(64) void k_t222ph_q(DevLayout L, DevTensor T, const
                                                  int hR) {
   extern __shared__ double ph_lds[];
   PhSlot *slot = reinterpret_cast<PhSlot *>(ph_lds);
-  __shared__ int pa[16], pb[16], pc[16], pd[16], pok[16], pab[16], pph[16], pj[16][4];
+  __shared__ int pa[16], pb[16], pc[16], pd[16], pok[16], pab[16], pph[16], pj[16][4], pj3[16], pstart[17];
   __shared__ double pcs[16];
   const long long qd = blockIdx.x;
   const int lane = threadIdx.x;
@@ -631,6 +641,8 @@ __global__ __launch_bounds__(64) void k_t222ph_q(DevLayout L, DevTensor T, const
     const int var = (k < 8) ? (k & 1) : 2 + (k & 1);
     pa[k] = a; pb[k] = b; pc[k] = c; pd[k] = d;
     pok[k] = ok ? (Tz | (PAR << 1) | 4) : 0;
+    pj3[k] = abs(jd - ja);                                   // the pattern's J3 run: |jd-ja| .. jd+ja
+    pstart[k + 1] = ok ? (((jd + ja - abs(jd - ja)) >> 1) + 1) * nd : 0;  // its slots (J3, J4 - J3)
     pab[k] = ab;
     pph[k] = ph34;
     pcs[k] = sg * sgnp(e) * ((k & 1) ? hR : hL);
@@ -640,8 +652,13 @@ __global__ __launch_bounds__(64) void k_t222ph_q(DevLayout L, DevTensor T, const
     pj[k][3] = (var == 0 || var == 2) ? jc : jb;
   }
   __syncthreads();
-  // gather: slots (pattern, J3, J4 - J3), compacted to the entries that pass
-  const int S = nJ * nd, nslot = 16 * S;
+  if (lane == 0) {
+    pstart[0] = 0;
+    for (int k = 0; k < 16; k++) pstart[k + 1] += pstart[k];
+  }
+  __syncthreads();
+  // gather: slots (pattern, J3 of its run, J4 - J3), compacted to the entries that pass
+  const int nslot = pstart[16];
   const double thr = (double)1e-6f;
   int count = 0;
   for (int s0 = 0; s0 < nslot; s0 += 64) {
@@ -651,7 +668,9 @@ __global__ __launch_bounds__(64) void k_t222ph_q(DevLayout L, DevTensor T, const
     unsigned b9 = 0;
     int abk = 0;
     if (s < nslot) {
-      const int k = s / S, r = s - k * S, J3 = 2 * (r / nd), J4 = J3 + 2 * (r % nd);
+      int k = 0;
+      while (pstart[k + 1] <= s) k++;
+      const int r = s - pstart[k], J3 = pj3[k] + 2 * (r / nd), J4 = J3 + 2 * (r % nd);
       const int okc = pok[k];
       if (okc) {
         const int a = pa[k], b = pb[k], c = pc[k], d = pd[k], Tz = okc & 1, PAR = (okc >> 1) & 1;
