@@ -48,7 +48,7 @@ struct DevTensor {
   const int *cidx;                // [((J3/2)*nJ + J4/2)*4 + Tz*2 + P] -> cross-coupled block or -1
   const int *ccol;                // [(scalar Pandya block * nsp + p) * nsp + h] -> column of (p,h)
   const double *x6, *s6;
-  const double *n9;               // 9j table of the ph recoupling (k_fill9j), null: evaluate on the fly
+  const double *n9;               // 9j table of the ph recoupling and the tensor Pandya (k_fill9j)
   int nd;                         // rank/2 + 1: J4 - J3 and J2 - J1 steps the table holds
   const unsigned *quads;          // orbit quadruples x0|x1<<8|x2<<16|x3<<24 (x0 <= x1, x2 <= x3) with outputs
   long long nquad;
@@ -98,16 +98,12 @@ __device__ __forceinline__ long long n9_index(const DevTensor &T, int j1, int j2
   return ((((((((long long)(j1 >> 1) * h + (j2 >> 1)) * h + (j3 >> 1)) * h + (j4 >> 1)) * T.nJ + (J3 >> 1)) * T.nd +
             ((J4 - J3) >> 1)) * T.nJ + (J1 >> 1)) * T.nd + ((J2 - J1) >> 1));
 }
-__device__ __forceinline__ double t9j_ph(const DevTensor &T, int a, int b, int J3, int c, int d, int J4, int J1,
-                                         int J2) {
-  return T.n9 ? T.n9[n9_index(T, a, b, c, d, J3, J4, J1, J2)] : t9j(T, a, b, J3, c, d, J4, J1, J2);
-}
 // {a b J3; c d J4; J3p J4p R} with J3 <= J4 <= J3 + R (a cross-coupled block) and any (J3p, J4p):
 // from the table, through the exchange of the first two columns (a factor (-1)^(sum of all nine))
 // when J4p < J3p
 __device__ __forceinline__ double t9j_cc(const DevTensor &T, int a, int b, int J3, int c, int d, int J4, int J3p,
                                          int J4p) {
-  if (!T.n9 || J4p - J3p > T.rank || J3p - J4p > T.rank) return t9j(T, a, b, J3, c, d, J4, J3p, J4p);
+  if (J4p - J3p > T.rank || J3p - J4p > T.rank) return 0.0;  // no triangle (J3p, J4p, R)
   if (J3p <= J4p) return T.n9[n9_index(T, a, b, c, d, J3, J4, J3p, J4p)];
   const double v = T.n9[n9_index(T, b, a, d, c, J3, J4, J4p, J3p)];
   return ((a + b + J3 + c + d + J4 + J3p + J4p + T.rank) >> 1) & 1 ? -v : v;
@@ -471,115 +467,43 @@ __global__ void k_tpandya(DevLayout L, DevTensor T, int mode, const double *__re
   out[(mode == 0 ? T.xoff[x] : T.roff[x]) + u] = v;
 }
 
-// The sixteen ways TS_commutator_222_ph (TS_commutators.f90:1459-1849) adds a
-// product entry into a stored element (x0 x1 | x2 x3; J1 J2): the source
-// orbits (a,b,c,d) in terms of x, the orbit-order condition of the
-// reference's branch, and which of its 9j argument orders, phase and herm
-// factor apply.  The source entry is W(IX = (d,a) at J3, JX = (c,b) at J4),
-// Wx for K < 8, Wy for K >= 8.  K is a template argument so that every
-// pattern's orbit selection and coefficient resolve at compile time.
-template <int K>
-__device__ __forceinline__ void ph_add(const DevLayout &L, const DevTensor &T, const double *__restrict__ Wx,
-                                       const double *__restrict__ Wy, int x0, int x1, int x2, int x3, int J1, int J2,
-                                       double s12, int hL, int hR, double &acc) {
-  int a, b, c, d;
-  bool ok = true;
-  if constexpr (K == 0) { a = x0; b = x1; c = x2; d = x3; }                                  // X: (a,b,c,d)
-  if constexpr (K == 1) { c = x0; d = x1; a = x2; b = x3; }                                  // X: (c,d,a,b)
-  if constexpr (K == 2) { a = x0; b = x1; d = x2; c = x3; ok = x2 < x3; }                    // X: (a,b,d,c)
-  if constexpr (K == 3) { d = x0; c = x1; a = x2; b = x3; ok = x0 < x1; }                    // X: (d,c,a,b)
-  if constexpr (K == 4) { b = x0; a = x1; c = x2; d = x3; ok = x0 < x1; }                    // X: (b,a,c,d)
-  if constexpr (K == 5) { c = x0; d = x1; b = x2; a = x3; ok = x2 < x3; }                    // X: (c,d,b,a)
-  if constexpr (K == 6) { b = x0; a = x1; d = x2; c = x3; ok = x0 < x1 && x2 < x3; }         // X: (b,a,d,c)
-  if constexpr (K == 7) { d = x0; c = x1; b = x2; a = x3; ok = x0 < x1 && x2 < x3; }         // X: (d,c,b,a)
-  if constexpr (K == 8) { d = x0; b = x1; c = x2; a = x3; }                                  // Y: (d,b,c,a)
-  if constexpr (K == 9) { c = x0; a = x1; d = x2; b = x3; }                                  // Y: (c,a,d,b)
-  if constexpr (K == 10) { d = x0; b = x1; a = x2; c = x3; ok = x2 < x3; }                   // Y: (d,b,a,c)
-  if constexpr (K == 11) { a = x0; c = x1; d = x2; b = x3; ok = x0 < x1; }                   // Y: (a,c,d,b)
-  if constexpr (K == 12) { b = x0; d = x1; c = x2; a = x3; ok = x0 < x1; }                   // Y: (b,d,c,a)
-  if constexpr (K == 13) { c = x0; a = x1; b = x2; d = x3; ok = x2 < x3; }                   // Y: (c,a,b,d)
-  if constexpr (K == 14) { b = x0; d = x1; a = x2; c = x3; ok = x0 < x1 && x2 < x3; }        // Y: (b,d,a,c)
-  if constexpr (K == 15) { a = x0; c = x1; b = x2; d = x3; ok = x0 < x1 && x2 < x3; }        // Y: (a,c,b,d)
-  if (!ok) return;
-  constexpr bool isx = K < 8;
-  const int R = T.rank, n = L.nsp, JM = L.jtot_max;
-  const int ja = L.jj[a], jb = L.jj[b], jc = L.jj[c], jd = L.jj[d];
-  const int Tz = abs(L.tz[d] - L.tz[a]) >> 1, PAR = (L.l[d] + L.l[a]) & 1, PAR2 = PAR ^ ((T.dpar >> 1) & 1);
-  if (((L.l[c] + L.l[b]) & 1) != PAR2 || (abs(L.tz[c] - L.tz[b]) >> 1) != Tz) return;
-  const double abcd = sgnp((ja + jb + jc + jd) >> 1);
-  const double thr = (double)1e-6f;
-  for (int J3 = abs(jd - ja); J3 <= jd + ja; J3 += 2) {
-    const int q3 = (J3 >> 1) + Tz * (JM + 1) + 2 * PAR * (JM + 1);
-    const int IX = L.ccrow[((size_t)q3 * n + d) * n + a];
-    if (IX < 0) continue;
-    for (int J4 = max(abs(J3 - R), J3); J4 <= min(J3 + R, 2 * JM); J4 += 2) {
-      if (!dtri(jc, jb, J4)) continue;
-      if (!isx && J4 == J3) continue;  // no Y product where J3 = J4
-      const int qx = T.cidx[((J3 >> 1) * T.nJ + (J4 >> 1)) * 4 + Tz * 2 + PAR];
-      if (qx < 0) continue;
-      const int q4 = T.cq4[qx];
-      const int JX = L.ccrow[((size_t)q4 * n + c) * n + b];
-      if (JX < 0) continue;
-      const long long wi = T.woff[qx] + (long long)IX * L.ccrlen[q4] + JX;
-      const double w = isx ? hL * Wx[wi] : Wy[wi];  // Wx carries LCC%herm (TS_commutators.f90:1506)
-      if (!(fabs(w) > thr)) continue;
-      const double P = s12 * sqrt((J3 + 1.0) * (J4 + 1.0)), ph34 = sgnp((J3 + J4) >> 1);
-      double v;
-      if constexpr (K == 0) v = t9j_ph(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((jb + jd + J2) >> 1) * ph34 * hL;
-      if constexpr (K == 1) v = -t9j_ph(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((jc + ja + J1 + R) >> 1) * hR;
-      if constexpr (K == 2) v = t9j_ph(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((jb + jc) >> 1) * ph34 * hL;
-      if constexpr (K == 3) v = t9j_ph(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((ja - jd + R) >> 1) * hR;
-      if constexpr (K == 4)
-        v = t9j_ph(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((ja + jd + J1 + J2) >> 1) * ph34 * hL;
-      if constexpr (K == 5) v = t9j_ph(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((jc - jb + J1 + J2 + R) >> 1) * hR;
-      if constexpr (K == 6) v = t9j_ph(T, ja, jd, J3, jb, jc, J4, J1, J2) * sgnp((ja + jc + J1) >> 1) * ph34 * hL;
-      if constexpr (K == 7) v = t9j_ph(T, jd, ja, J3, jc, jb, J4, J1, J2) * sgnp((jb - jd + J2 + R) >> 1) * hR;
-      if constexpr (K == 8) v = t9j_ph(T, jd, ja, J3, jb, jc, J4, J1, J2) * sgnp((jc + ja + J2) >> 1) * hL;
-      if constexpr (K == 9) v = t9j_ph(T, ja, jd, J3, jc, jb, J4, J1, J2) * sgnp((jd - jb + J1 + R) >> 1) * ph34 * hR;
-      if constexpr (K == 10) v = -t9j_ph(T, jd, ja, J3, jb, jc, J4, J1, J2) * hL;
-      if constexpr (K == 11) v = t9j_ph(T, ja, jd, J3, jc, jb, J4, J1, J2) * abcd * ph34 * sgnp(R >> 1) * hR;
-      if constexpr (K == 12) v = -t9j_ph(T, jd, ja, J3, jb, jc, J4, J1, J2) * abcd * sgnp((J1 + J2) >> 1) * hL;
-      if constexpr (K == 13) v = t9j_ph(T, ja, jd, J3, jc, jb, J4, J1, J2) * sgnp((J1 + J2 + R) >> 1) * ph34 * hR;
-      if constexpr (K == 14) v = t9j_ph(T, jd, ja, J3, jb, jc, J4, J1, J2) * sgnp((jb + jd + J1) >> 1) * hL;
-      if constexpr (K == 15)
-        v = t9j_ph(T, ja, jd, J3, jc, jb, J4, J1, J2) * sgnp((ja - jc + J2 + R) >> 1) * ph34 * hR;
-      acc += P * v * w;
-    }
-  }
-}
-
-template <int... K>
-__device__ __forceinline__ void ph_add_all(std::integer_sequence<int, K...>, const DevLayout &L, const DevTensor &T,
-                                           const double *__restrict__ Wx, const double *__restrict__ Wy, int x0,
-                                           int x1, int x2, int x3, int J1, int J2, double s12, int hL, int hR,
-                                           double &acc) {
-  (ph_add<K>(L, T, Wx, Wy, x0, x1, x2, x3, J1, J2, s12, hL, hR, acc), ...);
-}
-
-__global__ void k_t222ph(DevLayout L, DevTensor T, const double *__restrict__ Wx, const double *__restrict__ Wy,
-                         double *__restrict__ OV, int hL, int hR) {
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  int q, x0, x1, x2, x3;
-  long long vp;
-  if (!elem_of(L, T, e, q, x0, x1, x2, x3, vp)) return;
-  const int J1 = T.J1[q], J2 = T.J2[q];
-  if (x0 == x1 && ((J1 >> 1) & 1)) return;
-  if (x2 == x3 && ((J2 >> 1) & 1)) return;
-  const double s12 = sqrt((J1 + 1.0) * (J2 + 1.0));
-  double acc = 0.0;
-  ph_add_all(std::make_integer_sequence<int, 16>{}, L, T, Wx, Wy, x0, x1, x2, x3, J1, J2, s12, hL, hR, acc);
-  // add_elem_to_tensor stores V * pre, pre = sqrt 2 per identical pair
-  const double pre = (x0 == x1 ? M_SQRT2 : 1.0) * (x2 == x3 ? M_SQRT2 : 1.0);
-  OV[vp] += acc * pre;
-}
-
-// The same recoupling by orbit quadruple (x0 <= x1 | x2 <= x3), one wavefront
-// each.  The product entries a quadruple's outputs read do not depend on the
+// The ph recoupling of TS_commutator_222_ph (TS_commutators.f90:1459-1849) as
+// a gather by orbit quadruple (x0 <= x1 | x2 <= x3), one wavefront each.  The
+// reference adds every product entry W(IX = (d,a) at J3, JX = (c,b) at J4)
+// (Wx or Wy) into up to 16 x (J1, J2) outputs; read backwards, an output
+// (x0 x1 | x2 x3; J1 J2) receives entries through sixteen patterns: the source
+// orbits (a,b,c,d) as a permutation of x, the orbit-order condition of the
+// reference's branch, one of four 9j argument orders, a phase and a herm
+// factor.  The entries a quadruple's outputs read do not depend on the
 // outputs' (J1, J2): they are gathered once, over (pattern, J3, J4), with the
 // parts of their weights that do not depend on (J1, J2) (sign, herm, (J3,J4)
 // phase and norm, the |W| test) and the (J1, J2)-independent part of their 9j
 // table index, into a compacted LDS list; each output (J1, J2) of the
-// quadruple then sums the list across the lanes.  Needs the 9j table.
+// quadruple then sums the list across the lanes.
+//
+// The sixteen patterns (TS_commutators.f90:1562-1849): the output receives
+// W(d,a;c,b) times P * 9j * sign * herm, P = sqrt((2J1+1)(2J2+1)(2J3+1)(2J4+1)),
+// ph34 = (-1)^((J3+J4)/2), abcd = (-1)^((ja+jb+jc+jd)/2), all angular momenta doubled.
+//  k  W   output = V^{J1 J2} of   branch          9j {. . J3; . . J4; J1 J2 R}   factor
+//  0  Wx  (a,b,c,d)               b>=a, d>=c       (ja jd | jb jc)   (-1)^((jb+jd+J2)/2) ph34 hL
+//  1  Wx  (c,d,a,b)               b>=a, d>=c       (jd ja | jc jb)  -(-1)^((jc+ja+J1+R)/2) hR
+//  2  Wx  (a,b,d,c)               b>=a, d<c        (ja jd | jb jc)   (-1)^((jb+jc)/2) ph34 hL
+//  3  Wx  (d,c,a,b)               b>=a, d<c        (jd ja | jc jb)   (-1)^((ja-jd+R)/2) hR
+//  4  Wx  (b,a,c,d)               b<a, d>=c        (ja jd | jb jc)   (-1)^((ja+jd+J1+J2)/2) ph34 hL
+//  5  Wx  (c,d,b,a)               b<a, d>=c        (jd ja | jc jb)   (-1)^((jc-jb+J1+J2+R)/2) hR
+//  6  Wx  (b,a,d,c)               b<a, d<c         (ja jd | jb jc)   (-1)^((ja+jc+J1)/2) ph34 hL
+//  7  Wx  (d,c,b,a)               b<a, d<c         (jd ja | jc jb)   (-1)^((jb-jd+J2+R)/2) hR
+//  8  Wy  (d,b,c,a)               b>=d, a>=c       (jd ja | jb jc)   (-1)^((jc+ja+J2)/2) hL
+//  9  Wy  (c,a,d,b)               b>=d, a>=c       (ja jd | jc jb)   (-1)^((jd-jb+J1+R)/2) ph34 hR
+// 10  Wy  (d,b,a,c)               b>=d, a<c        (jd ja | jb jc)  -hL
+// 11  Wy  (a,c,d,b)               b>=d, a<c        (ja jd | jc jb)   abcd ph34 (-1)^(R/2) hR
+// 12  Wy  (b,d,c,a)               b<d, a>=c        (jd ja | jb jc)  -abcd (-1)^((J1+J2)/2) hL
+// 13  Wy  (c,a,b,d)               b<d, a>=c        (ja jd | jc jb)   (-1)^((J1+J2+R)/2) ph34 hR
+// 14  Wy  (b,d,a,c)               b<d, a<c         (jd ja | jb jc)   (-1)^((jb+jd+J1)/2) hL
+// 15  Wy  (a,c,b,d)               b<d, a<c         (ja jd | jc jb)   (-1)^((ja-jc+J2+R)/2) ph34 hR
+// Matching the output (x0 x1 | x2 x3) to the k-th form gives (a,b,c,d) as a
+// permutation of x (the packed tables ia..id below) and turns the branch into
+// x0 < x1 and / or x2 < x3; no Wy where J3 = J4 (the reference skips it).
 struct PhSlot {
   double c;      // sign * herm * phase(J3,J4) * sqrt((2J3+1)(2J4+1)) * W
   unsigned b9;   // 9j table index without the (J1, J2) part
@@ -614,7 +538,7 @@ __global__ __launch_bounds__(64) void k_t222ph_q(DevLayout L, DevTensor T, const
     const int Tz = abs(L.tz[d] - L.tz[a]) >> 1, PAR = (L.l[d] + L.l[a]) & 1, PAR2 = PAR ^ ((T.dpar >> 1) & 1);
     if (((L.l[c] + L.l[b]) & 1) != PAR2 || (abs(L.tz[c] - L.tz[b]) >> 1) != Tz) ok = false;
     // constant sign exponent (halves), J1/J2 flags and the (J3,J4) phase of each pattern
-    // (TS_commutators.f90:1562-1849, in the order of ph_add)
+    // (TS_commutators.f90:1562-1849)
     int e = 0, ab = 0;
     double sg = 1.0;
     const int abcd = (ja + jb + jc + jd) >> 1;
@@ -970,24 +894,22 @@ static int tspace_build(imsrg_ctx_s *c, int rank, int dpar, TensorSpace &S) {
   TUP(S.xoff, xoff); TUP(S.roff, roff); TUP(S.woff, woff); TUP(S.xstart, xstart); TUP(S.rstart, rstart);
   TUP(S.cidx, cidx); TUP(S.ccol, ccol); TUP(S.x6, x6); TUP(S.s6, s6);
 #undef TUP
-  // 9j table of the recoupling, when it fits in a few GiB
+  // 9j table of the recoupling and the tensor Pandya transform (up to 4 GiB:
+  // rank 2 to emax=14, rank 3 to emax=12)
   D.nd = rank / 2 + 1;
-  D.n9 = nullptr;
   {
     const long long n9 = (long long)nh * nh * nh * nh * nJ * nJ * D.nd * D.nd;
-    if (n9 <= (1LL << 29)) {
-      double *t9 = nullptr;
-      if (dalloc(c, n9, &t9)) return IMSRG_ENOMEM;
-      hipLaunchKernelGGL(k_fill9j, dim3((unsigned)((n9 + 255) / 256)), dim3(256), 0, c->stream, D, t9, n9);
-      D.n9 = t9;
-      S.n9size = n9;
-    }
+    if (n9 > (1LL << 29)) return fail(IMSRG_EINVAL, "tensor rank too large for the 9j table at this basis");
+    double *t9 = nullptr;
+    if (dalloc(c, n9, &t9)) return IMSRG_ENOMEM;
+    hipLaunchKernelGGL(k_fill9j, dim3((unsigned)((n9 + 255) / 256)), dim3(256), 0, c->stream, D, t9, n9);
+    D.n9 = t9;
+    S.n9size = n9;
   }
   // orbit quadruples of the recoupling gather (k_t222ph_q): row pair x0 <= x1,
-  // column pair x2 <= x3, with at least one stored output
-  D.quads = nullptr;
-  D.nquad = 0;
-  if (D.n9 && nsp <= 256) {
+  // column pair x2 <= x3, with at least one stored output (orbits packed in 8 bits)
+  if (nsp > 256) return fail(IMSRG_EINVAL, "tensor operators need at most 256 orbits");
+  {
     std::vector<unsigned> quads;
     std::vector<int> pa_, pb_;
     for (int a = 0; a < nsp; a++)
@@ -1044,7 +966,7 @@ static int tspace_build(imsrg_ctx_s *c, int rank, int dpar, TensorSpace &S) {
   }
   // the two Pandya products per cross-coupled block (TS_commutator_222_ph, TS_commutators.f90:1496-1521):
   // alpha carries 1/sqrt(J3+1) and 1/sqrt(J4+1); the herm of G that the reference also folds into Wx
-  // is applied where k_t222ph reads Wx, so these tables serve every G
+  // is applied where k_t222ph_q reads Wx, so these tables serve every G
   {
     std::vector<TProb> pr[2];
     std::vector<int4> tiles[2];
@@ -1324,12 +1246,9 @@ static int ts_comm(imsrg_ctx_s *c, const double *Gd, int hG, imsrg_tensor T, ims
     if (S.n_cc_tiles[1] > 0)
       hipLaunchKernelGGL(k_tgemm, dim3(S.n_cc_tiles[1]), dim3(256), 0, s, S.cc_probs[1], S.cc_tiles[1],
                          S.n_cc_tiles[1], (const double *)S.ccx, (const double *)S.gcc, S.wy);
-    if (D.quads && D.nquad > 0)
+    if (D.nquad > 0)
       hipLaunchKernelGGL(k_t222ph_q, dim3((unsigned)D.nquad), dim3(64),
                          (unsigned)(sizeof(PhSlot) * 16 * D.nJ * D.nd), s, L, D, (const double *)S.wx,
-                         (const double *)S.wy, OV, hL, hR);
-    else
-      hipLaunchKernelGGL(k_t222ph, dim3(nblk(S.nelem)), dim3(256), 0, s, L, D, (const double *)S.wx,
                          (const double *)S.wy, OV, hL, hR);
   }
   HIPCHK(hipGetLastError());
