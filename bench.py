@@ -230,7 +230,7 @@ KERNEL_NAME = {
     "k122": "k_122(", "ph_terms": "k_ph_terms(", "ph_apply": "k_ph_apply(", "sym_mirror": "k_sym_mirror(",
     "onebody": "k_ob_partial(",
     # int8x5 products and slicing
-    "gemm_pphh_i8": "k_seg_gemm_i8<0>", "gemm_ph_i8": "k_seg_gemm_i8<1>", "slice_v": "k_slice_v(",
+    "gemm_pphh_i8": "k_seg_gemm_i8", "gemm_ph_i8": "k_zprod_i8", "slice_v": "k_slice_v(",
     "slice_cc": "k_slice_cc(",
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_emax8.txt")

@@ -94,10 +94,14 @@ struct SegProb {
 // ---- int8-sliced products (the Ozaki scheme on the int8 matrix cores).  An
 // operand row r is held as x = s_r * sum_{i=1..5} a_i 2^(-7i): a power-of-two
 // row scale s_r (max |x| / s_r in [1/4, 1/2)) and five int8 slices a_i in
-// [-64, 64], zero-padded to 64 rows / 64 columns.  The five slices of each
-// 64-column chunk of a row lie together: slice i of column k of row r is at
+// [-64, 64], zero-padded to 64 rows / 64 columns.  V rows: the five slices of
+// each 64-column chunk of a row lie together, slice i of column k of row r at
 // byte r * 5 ld + (k / 64) * 320 + 64 i + k % 64 of the block, so one K chunk
-// of a row is 320 contiguous bytes.  A product keeps the slice pairs with
+// of a row is 320 contiguous bytes.  Pandya rows: fragment order, slice i of
+// columns 32 s .. 32 s + 31 of the 32 rows of group g in one 1 KB piece at
+// byte ((g * ld / 32 + s) * 5 + i) * 1024 of the block (lane l of a matrix
+// instruction reads the 16 bytes at 16 l: row 32 g + l % 32, columns
+// 32 s + 16 (l / 32) .. +15).  A product keeps the slice pairs with
 // i + j <= 6 (fifteen int8 products, exact int32 sums per level i + j).
 constexpr int kI8Planes = 5;
 struct I8Seg {
@@ -276,14 +280,19 @@ void launch_slice_v(hipStream_t s, const DevLayout &L, const I8Layout &I, const 
                     double *scale, const int *act, int nact);
 void launch_slice_cc(hipStream_t s, const DevLayout &L, const I8Layout &I, const double *cc, double *pmax,
                      signed char *planes, double *scale, const int *act, int nact);
-// int8-sliced segmented GEMM on the 64 x 64 tile lists of the FP64 path:
-// epi 0 = 222_pp_hh (C += upper, W after nmid), epi 1 = Z (2x2-blocked upper).
-// Operand 0 / 1 slices P0 / P1, row scales S0 / S1; segment factor
-// sign * (src ? g1 : g0).
-void launch_seg_gemm_i8(hipStream_t s, int epi, const I8Prob *probs, const int4 *tiles, int ntiles,
-                        const signed char *P0, const signed char *P1, const double *S0, const double *S1,
-                        double g0, double g1, double *C, double *W, double sigma, const int *act, int nact,
-                        long long sp, long long ss, long long sc, long long sw);
+// int8-sliced 222_pp_hh GEMM on the 64 x 64 tile lists of the FP64 path
+// (C += upper, W after nmid).  Operand 0 / 1 slices P0 / P1, row scales
+// S0 / S1; segment factor sign * (src ? g1 : g0).
+void launch_seg_gemm_i8(hipStream_t s, const I8Prob *probs, const int4 *tiles, int ntiles, const signed char *P0,
+                        const signed char *P1, const double *S0, const double *S1, double g0, double g1, double *C,
+                        double *W, const int *act, int nact, long long sp, long long ss, long long sc, long long sw);
+// int8-sliced Z product of the ph channel (2x2-blocked upper Z) on the
+// fragment-ordered Pandya planes, over a list of 32 x 32 upper tiles (x =
+// problem, y / z = row / column group; x = -1 a padding slot), four per
+// workgroup.
+void launch_zprod_i8(hipStream_t s, const I8Prob *probs, const int4 *tiles, int ntiles, const signed char *P0,
+                     const signed char *P1, const double *S0, const double *S1, double g0, double g1, double *Z,
+                     double sigma, const int *act, int nact, long long sp, long long ss, long long sz);
 
 constexpr int kPartials = 1024;  // reduction partials per instance
 constexpr int kScal = 64;        // scalar slots per instance

@@ -92,6 +92,35 @@ static void upper_tiles(int M, int TM, int TN, int pi, std::vector<int4> &out) {
     for (int j = 0; j * TN < M; j++)
       if (i * TM <= j * TN + TN - 1) out.push_back(make_int4(pi, i, j, 0));
 }
+// The int8 Z product's schedule: 32 x 32 upper tiles, one per wavefront and
+// four per workgroup.  Problems go to the XCD bins by weight (K x tiles) as
+// above; each bin lists its tiles row group by row group and is padded to
+// whole workgroups, and workgroup 8 i + x takes tiles 4 i .. 4 i + 3 of bin x,
+// so the wavefronts of a workgroup share a row group of A.
+static void build_wave_tiles(const std::vector<I8Prob> &probs, std::vector<int4> &tiles) {
+  const int NX = 8;
+  std::vector<double> w(probs.size());
+  std::vector<int> order(probs.size());
+  for (size_t i = 0; i < probs.size(); i++) {
+    std::vector<int4> t;
+    upper_tiles(probs[i].M, 32, 32, 0, t);
+    w[i] = (double)(probs[i].seg[0].K32 + 32) * (double)t.size();  // + 32: the stores of a K = 0 block
+    order[i] = (int)i;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return w[a] > w[b]; });
+  std::vector<std::vector<int4>> bins(NX);
+  std::vector<double> load(NX, 0.0);
+  for (int pi : order) {
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    upper_tiles(probs[pi].M, 32, 32, pi, bins[x]);
+    load[x] += w[pi];
+  }
+  size_t len = 0;
+  for (auto &b : bins) len = std::max(len, (b.size() + 3) / 4);
+  tiles.assign(len * NX * 4, make_int4(-1, 0, 0, 0));
+  for (int x = 0; x < NX; x++)
+    for (size_t i = 0; i < bins[x].size(); i++) tiles[((i / 4) * NX + x) * 4 + i % 4] = bins[x][i];
+}
 static void build_seg_tiles(const std::vector<SegProb> &probs, std::vector<int4> &tiles, int TM = 64, int TN = 64) {
   const int NX = 8;
   std::vector<double> w(probs.size());
@@ -666,6 +695,12 @@ static int i8_setup(imsrg_ctx_s *c) {
   if (!rc) rc = dalloc(c, c->i8.pm_size * B, &c->i8pm);
   if (!rc) rc = upload_table(c, pp, &c->pp_i8);
   if (!rc) rc = upload_table(c, ph, &c->ph_i8);
+  if (!rc) {
+    std::vector<int4> zt;
+    build_wave_tiles(ph, zt);
+    c->n_ph_i8_tiles = (int)zt.size();
+    rc = upload_table(c, zt, &c->ph_i8_tiles);
+  }
   for (int k = 0; k < 2 && !rc; k++) {
     void *p = nullptr;
     const size_t vb = (size_t)c->i8.v_bytes * B, cb = (size_t)c->i8.c_bytes * B;
@@ -1189,10 +1224,9 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
     });
     // B rows are rows of the other operand: X[k][n] = herm * X[n][k]
     PROFK(c, IMSRG_PROF_GEMM_PPHH,
-          launch_seg_gemm_i8(s2, 0, c->pp_i8, c->pp_tiles, c->n_pp_tiles, c->i8v[0], c->i8v[1], c->i8vs[0],
+          launch_seg_gemm_i8(s2, c->pp_i8, c->pp_tiles, c->n_pp_tiles, c->i8v[0], c->i8v[1], c->i8vs[0],
                              c->i8vs[1], (double)hR, (double)hL, (mask & T222PPHH) ? RES->d + D.v_off : nullptr,
-                             (mask & T221) ? c->wpp : nullptr, 0.0, ad, na, c->i8.v_bytes, c->i8.v_sr, OS,
-                             D.vsize));
+                             (mask & T221) ? c->wpp : nullptr, ad, na, c->i8.v_bytes, c->i8.v_sr, OS, D.vsize));
   } else if (mask & (T222PPHH | T221)) {
     const bool w = c->prec == IMSRG_PREC_FP64 && c->shape_pp == 1;
     PROFK(c, IMSRG_PROF_GEMM_PPHH,
@@ -1212,9 +1246,9 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
         launch_slice_cc(s, D, c->i8, c->cc[1], c->i8pm, c->i8c[1], c->i8cs[1], ad, na);
       });
       PROFK(c, IMSRG_PROF_GEMM_PH,
-            launch_seg_gemm_i8(s, 1, c->ph_i8, c->ph_tiles, c->n_ph_tiles, c->i8c[0], c->i8c[1], c->i8cs[0],
-                              c->i8cs[1], (double)hL, -(double)hR, c->wph, nullptr, -(double)hL * (double)hR, ad,
-                              na, c->i8.c_bytes, c->i8.c_sr, D.cwsize, 0));
+            launch_zprod_i8(s, c->ph_i8, c->ph_i8_tiles, c->n_ph_i8_tiles, c->i8c[0], c->i8c[1], c->i8cs[0],
+                            c->i8cs[1], (double)hL, -(double)hR, c->wph, -(double)hL * (double)hR, ad, na,
+                            c->i8.c_bytes, c->i8.c_sr, D.cwsize));
     } else {
       const bool w = c->prec == IMSRG_PREC_FP64 && c->shape_ph == 1;
       PROFK(c, IMSRG_PROF_GEMM_PH,

@@ -90,6 +90,8 @@ struct imsrg_ctx_s {
   double *i8vs[2] = {nullptr, nullptr}, *i8cs[2] = {nullptr, nullptr};
   double *i8pm = nullptr;  // partial row maxima of a Pandya matrix (k_rowmax_cc)
   const I8Prob *pp_i8 = nullptr, *ph_i8 = nullptr;
+  const int4 *ph_i8_tiles = nullptr;  // 32 x 32 tiles of the int8 Z product, four per workgroup
+  int n_ph_i8_tiles = 0;
   // active-instance tables: slot 0 = all instances (constant), 1.. = driver work sets
   static constexpr int kSlots = 4;
   int *act_d[kSlots] = {nullptr, nullptr, nullptr, nullptr};

@@ -118,12 +118,21 @@ __global__ __launch_bounds__(256) void k_slice_v(DevLayout L, I8Layout I, const 
   }
 }
 
+// Pandya planes are stored in fragment order: the rows of a block in groups
+// of 32; for group g, K step s (32 columns, nks = ld / 32 per row) and slice
+// i, 1 KB holds lane l's operand of one v_mfma_i32_32x32x32_i8, the 16 bytes
+// of row 32 g + l % 32, columns 32 s + 16 (l / 32) .. +15, at 16 l.
+__device__ __forceinline__ long long frag_off(int g, int s, int i, int nks) {
+  return ((long long)(g * nks + s) * NS + i) * 1024;
+}
+
 // Pandya rows (column-major r x K per block in `cc`), in items of 64 rows x
 // 256 columns.  k_rowmax_cc: the maximum of each row over the item's columns
 // (64 consecutive rows per wavefront load) into the strip's partial maxima;
 // k_slice_cc: the row maximum from those partials, then each thread slices 16
 // consecutive columns of one row per 64-column chunk and stores 16 bytes per
-// plane.  Rows r..64*ceil(r/64) are written as zero.
+// plane (a wavefront's stores are two runs of 512 contiguous bytes).  Rows
+// r..64*ceil(r/64) are written as zero.
 __global__ __launch_bounds__(256) void k_rowmax_cc(DevLayout L, I8Layout I, const double *__restrict__ cc,
                                                    double *__restrict__ pmax, const int *act) {
   __shared__ double red[4][64];
@@ -166,7 +175,8 @@ __global__ __launch_bounds__(256) void k_slice_cc(DevLayout L, I8Layout I, const
   const int ex = row_exp(m);
   if (k0 == 0 && kq == 0) scale[I.c_soff[q] + row] = m > 0.0 ? ldexp(1.0, ex) : 0.0;
   const double *src = cc + L.ccoff[q];
-  signed char *dst = planes + I.c_poff[q] + (long long)row * NS * ld;
+  signed char *dst = planes + I.c_poff[q];
+  const int nks = ld >> 5;
   for (int kc = k0; kc < min(k0 + 256, ld); kc += 64) {
     const int kb = kc + kq * 16;
     unsigned w[NS][4];
@@ -186,7 +196,7 @@ __global__ __launch_bounds__(256) void k_slice_cc(DevLayout L, I8Layout I, const
 #pragma unroll
     for (int i = 0; i < NS; i++) {
       const v4i_t v = {(int)w[i][0], (int)w[i][1], (int)w[i][2], (int)w[i][3]};
-      *(v4i_t *)(dst + (kb >> 6) * (NS * 64) + i * 64 + (kb & 63)) = v;
+      *(v4i_t *)(dst + frag_off(row >> 5, kb >> 5, i, nks) + (((row & 31) + 32 * ((kb >> 4) & 1)) << 4)) = v;
     }
   }
 }
@@ -203,7 +213,8 @@ void launch_slice_cc(hipStream_t s, const DevLayout &L, const I8Layout &I, const
   hipLaunchKernelGGL(k_slice_cc, dim3(I.n_items, nact), dim3(256), 0, s, L, I, cc, pmax, planes, scale, act);
 }
 
-// ---- the product.  A 64 x 64 output tile per workgroup; each K chunk of 64
+// ---- the 222_pp_hh product (C += upper, the 221 intermediate W after
+// segment nmid) on the V-row planes.  A 64 x 64 output tile per workgroup; each K chunk of 64
 // columns of the NS slices of A (64 rows) and B (64 rows, 320 contiguous bytes
 // per row) is loaded into registers one chunk ahead and staged in LDS (16-byte column group g of row
 // r at g ^ ((r >> 2) & 3), so the 16 rows a quarter-wavefront reads hit
@@ -220,15 +231,14 @@ typedef int v16i_t __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ int swz(int r, int g) { return g ^ ((r >> 2) & 3); }
 
-template <int EPI>
 __global__ __launch_bounds__(256, 2) void k_seg_gemm_i8(const I8Prob *__restrict__ probs,
                                                         const int4 *__restrict__ tiles, int ntiles,
                                                         const signed char *__restrict__ P0,
                                                         const signed char *__restrict__ P1,
                                                         const double *__restrict__ S0, const double *__restrict__ S1,
                                                         double g0, double g1, double *__restrict__ C,
-                                                        double *__restrict__ W, double sigma, const int *act,
-                                                        long long sp, long long ss, long long sc, long long sw) {
+                                                        double *__restrict__ W, const int *act, long long sp,
+                                                        long long ss, long long sc, long long sw) {
   const int t = blockIdx.x;
   if (t >= ntiles) return;
   {
@@ -353,44 +363,166 @@ __global__ __launch_bounds__(256, 2) void k_seg_gemm_i8(const I8Prob *__restrict
       if (row < M && col < M && row <= col) f(row, col, acc[v]);
     }
   };
-  if (EPI == 0) {
-    run(0, p.nmid);
-    if (W && active) {
-      double *Wp = W + p.w_off;
-      each([&](int row, int col, double x) { Wp[(long long)row * M + col] = x; });
+  run(0, p.nmid);
+  if (W && active) {
+    double *Wp = W + p.w_off;
+    each([&](int row, int col, double x) { Wp[(long long)row * M + col] = x; });
+  }
+  run(p.nmid, p.nseg);
+  if (C && active) {
+    double *Cp = C + p.c_off;
+    each([&](int row, int col, double x) {
+      const long long o = (long long)row * M + col;
+      Cp[o] = Cp[o] + x;
+    });
+  }
+}
+
+void launch_seg_gemm_i8(hipStream_t s, const I8Prob *probs, const int4 *tiles, int ntiles, const signed char *P0,
+                        const signed char *P1, const double *S0, const double *S1, double g0, double g1, double *C,
+                        double *W, const int *act, int nact, long long sp, long long ss, long long sc, long long sw) {
+  if (ntiles <= 0 || nact <= 0) return;
+  launch_k(k_seg_gemm_i8, dim3(ntiles, nact), dim3(256), 0, s, probs, tiles, ntiles, P0, P1, S0, S1, g0, g1, C, W,
+           act, sp, ss, sc, sw);
+}
+
+// ---- the Z product of the ph channel over each Pandya block,
+//   Z(m, n) = g0 L_m . R_n + g1 R_m . L_n     (rows of the fragment-ordered planes),
+// whose K (the block's ph pairs, at most 116 at emax = 8) is short: the
+// product is bound by the latency of its operand loads and by its Z stores,
+// not by the matrix cores.  One wavefront per 32 x 32 upper tile, four tiles
+// (consecutive column groups of one row group, so they share A through the
+// L1 and L2) per workgroup; no barriers.  Every fragment is one coalesced
+// 1 KB load; the ten fragments of K step t + 1 are in flight while the
+// fifteen v_mfma_i32_32x32x32_i8 of step t run.  The two segments are one
+// stream of 2 nk steps; each segment's level sums are scaled into the FP64
+// accumulators when its last step is done, with row scales the wavefront
+// staged in its own LDS slot at the start (so that no global load of the
+// scaling waits behind the operand loads in flight).  Result element v of
+// lane l is row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32 of the tile;
+// lanes c, c ^ 1 swap one value per row pair so that each writes 16 bytes
+// of a 2x2 Z block, and the 32 lanes of a row pair write 512 contiguous bytes.
+__global__ __launch_bounds__(256, 2) void k_zprod_i8(const I8Prob *__restrict__ probs, const int4 *__restrict__ tiles,
+                                                  int ntiles, const signed char *__restrict__ P0,
+                                                  const signed char *__restrict__ P1, const double *__restrict__ S0,
+                                                  const double *__restrict__ S1, double g0, double g1,
+                                                  double *__restrict__ Z, double sigma, const int *act, long long sp,
+                                                  long long ss, long long sz) {
+  __shared__ double scl[4][128];  // per wavefront: A row scales of segments 0, 1; B row scales of segments 0, 1
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int item = (int)blockIdx.x * 4 + wave;
+  if (item >= ntiles) return;  // no barriers in this kernel: a wavefront may leave on its own
+  {
+    const long long inst = inst_id(act);
+    if (inst < 0) return;
+    P0 += inst * sp;
+    P1 += inst * sp;
+    S0 += inst * ss;
+    S1 += inst * ss;
+    Z += inst * sz;
+  }
+  const int4 tl = tiles[item];
+  if (tl.x < 0) return;  // padding slot of the XCD-balanced schedule
+  const I8Prob &p = probs[tl.x];
+  const int M = p.M, lane = threadIdx.x & 63, fr = lane & 31, fh = lane >> 5;
+  const int ra = tl.y * 32, cb = tl.z * 32;
+  const int nks = p.ld >> 5, nk = p.seg[0].K32 >> 5, T = 2 * nk;
+  double *sw = scl[wave];
+  const double f0 = p.seg[0].sign * g0, f1 = p.seg[1].sign * g1;  // segment factors (uniform)
+  // the row scales first (the oldest loads, so that waiting for them leaves
+  // the operand loads issued after them in flight)
+  double sx, sy;
+  {
+    const double *sp0 = S0 + p.s_off, *sp1 = S1 + p.s_off;
+    sx = fh ? sp1[ra + fr] : sp0[ra + fr];  // A rows: L (segment 0), R (segment 1)
+    sy = fh ? sp0[cb + fr] : sp1[cb + fr];  // B rows: R (segment 0), L (segment 1)
+  }
+  const unsigned lo = 16u * (unsigned)lane;
+  const signed char *A0 = P0 + p.p_off + frag_off(tl.y, 0, 0, nks), *B0 = P1 + p.p_off + frag_off(tl.z, 0, 0, nks);
+  const signed char *A1 = P1 + p.p_off + frag_off(tl.y, 0, 0, nks), *B1 = P0 + p.p_off + frag_off(tl.z, 0, 0, nks);
+  v16i_t lv[NS];
+  double acc[16];
+  v4i_t fa[2][NS], fb[2][NS];  // two fragment sets (three spill at two wavefronts per SIMD)
+  // step t (clamped to T - 1: the reloads past the last step are never used)
+  auto load = [&](int t, v4i_t(&a)[NS], v4i_t(&b)[NS]) {
+    t = min(t, T - 1);
+    const bool s1 = t >= nk;
+    const long long o = (long long)(s1 ? t - nk : t) * (NS * 1024);
+    const signed char *pa = (s1 ? A1 : A0) + o, *pb = (s1 ? B1 : B0) + o;
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+      a[i] = *(const v4i_t *)(pa + i * 1024 + lo);
+      b[i] = *(const v4i_t *)(pb + i * 1024 + lo);
     }
-    run(p.nmid, p.nseg);
-    if (C && active) {
-      double *Cp = C + p.c_off;
-      each([&](int row, int col, double x) {
-        const long long o = (long long)row * M + col;
-        Cp[o] = Cp[o] + x;
-      });
+    // the sets stay whole and in program order, so a wait for the oldest
+    // set leaves the younger ones in flight
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  auto mma = [&](const v4i_t(&a)[NS], const v4i_t(&b)[NS]) {
+#pragma unroll
+    for (int i = 0; i < NS; i++)
+#pragma unroll
+      for (int j = 0; i + j < NS; j++) lv[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], lv[i + j], 0, 0, 0);
+  };
+  auto fold = [&](int sg) {
+    const double sb = sw[64 + 32 * sg + fr];
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+      const double sa = sw[32 * sg + 8 * (v >> 2) + 4 * fh + (v & 3)];
+      double lev = (double)lv[NS - 1][v];
+#pragma unroll
+      for (int l = NS - 2; l >= 0; l--) lev = (double)lv[l][v] + lev * 0x1p-7;
+      acc[v] += sa * sb * (lev * 0x1p-14);
     }
-  } else {
-    run(0, p.nseg);
-    if (active) {
-      double *Zp = C + p.c_off;
-      each([&](int row, int col, double x) {
-        Zp[zidx(row, col, M)] = x;
-        if (row < col && (row >> 1) == (col >> 1)) Zp[zidx(col, row, M)] = sigma * x;
-      });
+#pragma unroll
+    for (int l = 0; l < NS; l++)
+#pragma unroll
+      for (int v = 0; v < 16; v++) lv[l][v] = 0;
+  };
+#pragma unroll
+  for (int l = 0; l < NS; l++)
+#pragma unroll
+    for (int v = 0; v < 16; v++) lv[l][v] = 0;
+#pragma unroll
+  for (int v = 0; v < 16; v++) acc[v] = 0.0;
+  if (T > 0) load(0, fa[0], fb[0]);  // T = 2 nk >= 2
+  sw[lane] = sx;
+  sw[64 + lane] = sy * (fh ? f1 : f0);
+  if (T > 0) {
+    // the two fragment sets in rotation, one step ahead (T is even)
+    for (int t = 0; t < T; t += 2) {
+      load(t + 1, fa[1], fb[1]);
+      mma(fa[0], fb[0]);
+      if (t == nk - 1) fold(0);
+      load(t + 2, fa[0], fb[0]);
+      mma(fa[1], fb[1]);
+      if (t + 1 == nk - 1) fold(0);
+    }
+    fold(1);
+  }
+  double *Zp = Z + p.c_off;
+  const int col = cb + fr;
+  const bool odd = fr & 1;
+#pragma unroll
+  for (int v = 0; v < 16; v += 2) {
+    const int row = ra + 8 * (v >> 2) + 4 * fh + (v & 3);  // even: rows row, row + 1 are v, v + 1
+    const double y = __shfl_xor(odd ? acc[v] : acc[v + 1], 1, 64);
+    if (row < M && col < M && (row >> 1) <= (col >> 1)) {
+      const long long e = zidx(row, col & ~1, M);
+      if (odd)  // Z(row+1, col-1), Z(row+1, col); a diagonal block takes sigma Z(row, col) for the first
+        *(double2 *)(Zp + e + 2) = make_double2((row >> 1) == (col >> 1) ? sigma * acc[v] : y, acc[v + 1]);
+      else  // Z(row, col), Z(row, col+1)
+        *(double2 *)(Zp + e) = make_double2(acc[v], y);
     }
   }
 }
 
-void launch_seg_gemm_i8(hipStream_t s, int epi, const I8Prob *probs, const int4 *tiles, int ntiles,
-                        const signed char *P0, const signed char *P1, const double *S0, const double *S1,
-                        double g0, double g1, double *C, double *W, double sigma, const int *act, int nact,
-                        long long sp, long long ss, long long sc, long long sw) {
+void launch_zprod_i8(hipStream_t s, const I8Prob *probs, const int4 *tiles, int ntiles, const signed char *P0,
+                     const signed char *P1, const double *S0, const double *S1, double g0, double g1, double *Z,
+                     double sigma, const int *act, int nact, long long sp, long long ss, long long sz) {
   if (ntiles <= 0 || nact <= 0) return;
-  const dim3 g(ntiles, nact), b(256);
-  if (epi == 0)
-    launch_k(k_seg_gemm_i8<0>, g, b, 0, s, probs, tiles, ntiles, P0, P1, S0, S1, g0, g1, C, W, sigma,
-                       act, sp, ss, sc, sw);
-  else
-    launch_k(k_seg_gemm_i8<1>, g, b, 0, s, probs, tiles, ntiles, P0, P1, S0, S1, g0, g1, C, W, sigma,
-                       act, sp, ss, sc, sw);
+  launch_k(k_zprod_i8, dim3((ntiles + 3) / 4, nact), dim3(256), 0, s, probs, tiles, ntiles, P0, P1, S0, S1, g0, g1,
+           Z, sigma, act, sp, ss, sz);
 }
 
 }  // namespace imsrg
