@@ -10,16 +10,21 @@
 namespace imsrg {
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
+// two doubles at any 8-byte boundary: one 16-byte access (global_load/
+// store_dwordx4; the hardware takes them unaligned), half the vector memory
+// instructions of two 8-byte accesses
+typedef double dpair_t __attribute__((ext_vector_type(2), aligned(8)));
 
 #define HIP_KERNEL_CHECK() ((void)0)
 
-// Z of one Pandya block (M x M, M even) is stored in 2x2 blocks: element
-// (R, C) at ((R/2)*(M/2) + C/2)*4 + (R%2)*2 + C%2.  Rows 2u, 2u+1 are the
-// pairs (a,b), (b,a) (or two (a,a) pairs), so the two Z elements of one ph J'
-// term share a block.
+// Z of one Pandya block (M x M, M even) is stored in 2x2 blocks: block
+// (R/2, C/2) at ((R/2)*(M/2) + C/2)*4, and inside it the diagonal pair
+// first, Z(2u,2w), Z(2u+1,2w+1), then Z(2u,2w+1), Z(2u+1,2w).  Rows 2u, 2u+1
+// are the pairs (a,b), (b,a) (or two (a,a) pairs), so the two Z elements
+// Z(R,C), Z(R^1,C^1) of one ph J' term are one aligned 16-byte half.
 // Only blocks u <= v are stored (Z(R,C) = sigma Z(C,R) gives the rest).
 __device__ __forceinline__ long long zidx(int R, int C, int M) {
-  return ((long long)(R >> 1) * (M >> 1) + (C >> 1)) * 4 + (R & 1) * 2 + (C & 1);
+  return ((long long)(R >> 1) * (M >> 1) + (C >> 1)) * 4 + (((R ^ C) & 1) << 1) + (R & 1);
 }
 
 // Instance of a batched launch: blockIdx.y indexes the active-instance table

@@ -46,8 +46,10 @@ struct DevLayout {
   long long nquad;
   const unsigned *ph_terms;  // J' terms: quad<<6 | side<<5 | m, in quadruple order
   long long nphterm;
-  const unsigned *ph_zblk;     // per J' term: 2x2 block of Z holding its two elements (units of 4 doubles)
+  const unsigned *ph_zblk;     // per J' term: half of a 2x2 Z block holding its two elements (units of 2 doubles)
   const unsigned char *ph_zsel;  // per J' term: element positions, mirror, phase, side, valid
+  const unsigned *ph_two;        // the terms with a second half-block load
+  long long nphtwo;
   const unsigned *ph_toff;   // per upper element (row-major per block): first T1 term of its quadruple
   const unsigned long long *ph_desc;  // per upper element: T offset, run lengths, 6j runs, flags (layout.cpp)
   const long long *ph_ubase; // per global pair row r: index of its first upper element

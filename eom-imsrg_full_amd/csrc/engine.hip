@@ -376,7 +376,8 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
   UP(Ly.ph_terms, ph_terms);
   D.nphterm = (long long)Ly.ph_terms.size();
   UP(Ly.ph_toff, ph_toff);
-  UP(Ly.ph_zblk, ph_zblk); UP(Ly.ph_zsel, ph_zsel);
+  UP(Ly.ph_zblk, ph_zblk); UP(Ly.ph_zsel, ph_zsel); UP(Ly.ph_two, ph_two);
+  D.nphtwo = (long long)Ly.ph_two.size();
   {
     std::vector<unsigned long long> pd(Ly.ph_desc.begin(), Ly.ph_desc.end());
     UP(pd, ph_desc);

@@ -964,8 +964,8 @@ void launch_pandya(hipStream_t s, const DevLayout &L, const double *op, double *
 //   M(r,s) = sum_{e in nb(r)} c_e [fL(x,i) R(r_e,s) - fR(x,i) L(r_e,s)],
 // the right-multiplication half equals -hL*hR*M(s,r) (hermiticity of f and
 // V), so RES(r,s) = M(r,s) + herm(RES)*M(s,r).  This kernel writes M over the
-// whole block (one wavefront per row, coalesced row reads, wave-uniform
-// neighbour entries); the diagonal is written already symmetrised,
+// whole block (one wavefront per row, coalesced 16-byte row reads, wave-
+// uniform neighbour entries); the diagonal is written already symmetrised,
 // M(r,r)*(1+herm).  k_sym_mirror adds the transpose at the end.
 __global__ __launch_bounds__(64) void k_122(DevLayout L, const double *__restrict__ Lop,
                                             const double *__restrict__ Rop, double *__restrict__ RES,
@@ -986,29 +986,41 @@ __global__ __launch_bounds__(64) void k_122(DevLayout L, const double *__restric
   double *OV = RES + bo + (long long)rl * dim;
   const double *FL = Lop + L.f_off, *FR = Rop + L.f_off;
   const int e0 = L.nb_start[r], e1 = L.nb_start[r + 1];
-  // four outputs per thread (s0 + 64u + lane): each neighbour entry issues 8
-  // independent row loads and its scalars are read once for all four
-  for (int s0 = 0; s0 < dim; s0 += 256) {
-    double val[4] = {0.0, 0.0, 0.0, 0.0};
+  // eight outputs per lane, as four pairs s, s + 1 (s = s0 + 128 u + 2 lane,
+  // 16-byte accesses); each neighbour entry issues 8 independent row loads
+  // and its scalars are read once for all eight
+  for (int s0 = 0; s0 < dim; s0 += 512) {
+    dpair_t val[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) val[u] = (dpair_t){0.0, 0.0};
     for (int e = e0; e < e1; e++) {
       const long long pe = L.nb_pos[e];
       const int xi = L.nb_x[e] * N + L.nb_i[e];
       const double ce = L.nb_coef[e], fl = FL[xi], fr = FR[xi];
       const double *rrow = RV + pe * dim, *lrow = LV + pe * dim;
-      double rv[4], lv[4];
+      dpair_t rv[4], lv[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
-        const int s = s0 + 64 * u + (int)threadIdx.x;
-        rv[u] = s < dim ? rrow[s] : 0.0;
-        lv[u] = s < dim ? lrow[s] : 0.0;
+        const int s = s0 + 128 * u + 2 * (int)threadIdx.x;
+        if (s + 1 < dim) {
+          rv[u] = *(const dpair_t *)(rrow + s);
+          lv[u] = *(const dpair_t *)(lrow + s);
+        } else {  // the last element of an odd row, or past the row
+          rv[u] = (dpair_t){s < dim ? rrow[s] : 0.0, 0.0};
+          lv[u] = (dpair_t){s < dim ? lrow[s] : 0.0, 0.0};
+        }
       }
 #pragma unroll
       for (int u = 0; u < 4; u++) val[u] += ce * (fl * rv[u] - fr * lv[u]);
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int s = s0 + 64 * u + (int)threadIdx.x;
-      if (s < dim) OV[s] = (s == rl) ? val[u] * (1.0 + hres) : val[u];
+      const int s = s0 + 128 * u + 2 * (int)threadIdx.x;
+      dpair_t v = val[u];
+      if (s == rl) v.x *= 1.0 + hres;
+      if (s + 1 == rl) v.y *= 1.0 + hres;
+      if (s + 1 < dim) *(dpair_t *)(OV + s) = v;
+      else if (s < dim) OV[s] = v.x;
     }
   }
 }
@@ -1048,53 +1060,73 @@ void launch_clear_blocks(hipStream_t s, const DevLayout &L, double *RES, const i
 // sums to every upper element, one wavefront per RES row.
 // Z is stored in 2x2 blocks, upper blocks only: Z(R1,C1) and Z(R2,C2), where
 // (R1,R2) and (C1,C2) are the two orders of one orbit pair each, lie in one
-// block (R1/2, C1/2), or in its mirror (C1/2, R1/2) times sigma.
+// aligned 16-byte half of block (R1/2, C1/2), or of its mirror (C1/2, R1/2)
+// times sigma (see zidx), unless a pair is (a,a): then R2 = R1 or C2 = C1 and
+// the second element is in the other half of the block.
 // Four J' terms per thread (t + 256 u of a chunk of 1024): T = z1 + ph*sigma*z2
 // (side 0) or sigma*z1 + ph*z2 (side 1), both Z elements read with one
-// 32-byte load of the block the host-built address table names; the four
-// terms' table reads, then their four block reads, are in flight together.
+// 16-byte load of the half the host-built address table names (a second one
+// for the (a,a) terms); the four terms' table reads, then their four
+// half-block reads, are in flight together.
+__device__ __forceinline__ double ph_term(unsigned sel, double2 b, double2 c, double sigma) {
+  const double2 h2 = (sel & 2) ? c : b;
+  double z1 = (sel & 1) ? b.y : b.x, z2 = (sel & 4) ? h2.y : h2.x;
+  if (sel & 16) {
+    z1 = sigma * z1;
+    z2 = sigma * z2;
+  }
+  const double ph = (sel & 32) ? -1.0 : 1.0;
+  const double v = (sel & 64) ? sigma * z1 + ph * z2 : z1 + ph * sigma * z2;
+  return (sel & 128) ? v : 0.0;
+}
+// Work items past the XCD split's are the (a,a) terms, one per thread; the
+// chunks skip them.  Every load is unconditional (past the end: the last
+// term; invalid terms: half 0), so the four terms' loads stay in flight together.
 __global__ __launch_bounds__(256) void k_ph_terms(DevLayout L, const double *__restrict__ Z,
                                                   double *__restrict__ Tbuf, double sigma, const int *act) {
-  const int chunk = xcd_item(L.xs_terms);
-  if (chunk < 0) return;
   {
     const long long inst = inst_id(act);
     if (inst < 0) return;
     Z += inst * L.cwsize;
     Tbuf += inst * L.nphterm;
   }
+  const double2 *Z2 = reinterpret_cast<const double2 *>(Z);
+  const int nchunk = 8 * L.xs_terms.len;
+  if ((int)blockIdx.x >= nchunk) {
+    const long long k = (long long)(blockIdx.x - nchunk) * 256 + threadIdx.x;
+    if (k >= L.nphtwo) return;
+    const unsigned t = L.ph_two[k], sel = L.ph_zsel[t], blk = L.ph_zblk[t];
+    Tbuf[t] = ph_term(sel, Z2[blk], Z2[blk ^ 1u], sigma);
+    return;
+  }
+  const int chunk = xcd_item(L.xs_terms);
+  if (chunk < 0) return;
   const long long t0 = (long long)chunk * 1024 + threadIdx.x;
   unsigned sel[4], blk[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
-    const long long t = t0 + 256 * u;
-    sel[u] = 0;
-    blk[u] = 0;
-    if (t < L.nphterm) {
-      sel[u] = L.ph_zsel[t];
-      blk[u] = L.ph_zblk[t];
-    }
+    const long long t = min(t0 + 256 * u, L.nphterm - 1);
+    sel[u] = L.ph_zsel[t];
+    blk[u] = L.ph_zblk[t];
   }
-  double4_t b[4];
+  double2 b[4];
 #pragma unroll
-  for (int u = 0; u < 4; u++)
-    if (sel[u] & 128) b[u] = reinterpret_cast<const double4_t *>(Z)[blk[u]];
+  for (int u = 0; u < 4; u++) b[u] = Z2[blk[u]];
+  // the four half-block reads are all issued here (a value used only under
+  // the store's condition would otherwise be loaded inside it, one at a time)
+  asm volatile("" ::"v"(b[0].x), "v"(b[0].y), "v"(b[1].x), "v"(b[1].y), "v"(b[2].x), "v"(b[2].y), "v"(b[3].x),
+               "v"(b[3].y));
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     const long long t = t0 + 256 * u;
-    if (t >= L.nphterm) break;
-    double v = 0.0;
-    if (sel[u] & 128) {
-      double z1 = b[u][sel[u] & 3], z2 = b[u][(sel[u] >> 2) & 3];
-      if (sel[u] & 16) {
-        z1 = sigma * z1;
-        z2 = sigma * z2;
-      }
-      const double ph = (sel[u] & 32) ? -1.0 : 1.0;
-      v = (sel[u] & 64) ? sigma * z1 + ph * z2 : z1 + ph * sigma * z2;
-    }
-    Tbuf[t] = v;
+    if (t < L.nphterm && (sel[u] & 130) != 130) Tbuf[t] = ph_term(sel[u], b[u], b[u], sigma);
   }
+}
+void launch_ph_terms(hipStream_t s, const DevLayout &L, const double *Z, double *Tbuf, double hL, double hR,
+                     const int *act, int nact) {
+  if (L.nphterm <= 0 || nact <= 0) return;
+  const unsigned g = 8 * L.xs_terms.len + (unsigned)((L.nphtwo + 255) / 256);
+  hipLaunchKernelGGL(k_ph_terms, dim3(g, nact), dim3(256), 0, s, L, Z, Tbuf, -hL * hR, act);
 }
 
 // The 6j symbols a row can use, {ji jj J; x y J'} with triangle (x, y, J) and
@@ -1208,11 +1240,6 @@ __global__ __launch_bounds__(64) void k_ph_apply(DevLayout L, const double *__re
     const int o = g + 16 * u;
     if (sub == 0 && o < h.n) OV[o] = old[u] + acc * pre2 * pre;
   }
-}
-void launch_ph_terms(hipStream_t s, const DevLayout &L, const double *Z, double *Tbuf, double hL, double hR,
-                     const int *act, int nact) {
-  if (L.nphterm <= 0 || nact <= 0) return;
-  hipLaunchKernelGGL(k_ph_terms, dim3(8 * L.xs_terms.len, nact), dim3(256), 0, s, L, Z, Tbuf, -hL * hR, act);
 }
 void launch_ph_apply(hipStream_t s, const DevLayout &L, const double *Tbuf, double *RES, const int *act,
                      int nact) {

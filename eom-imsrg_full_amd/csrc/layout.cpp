@@ -365,9 +365,12 @@ std::string Layout::build(int nsp_, const int *n_, const int *l_, const int *jj_
       }
     }
     ph_ubase[npairs] = (int64_t)ph_toff.size();
-    // Z address of every J' term: the 2x2 block of Z (index in units of 4
-    // doubles from the Z base) holding both Z elements the term combines, and
-    // a selector: bits 0-1 / 2-3 their positions in the stored block, 4 the
+    // Z address of every J' term: the 16-byte half of a 2x2 block of Z
+    // (index in units of 2 doubles from the Z base) holding the first Z
+    // element the term combines, and a selector: bit 0 that element's
+    // position in the half, bit 1 the second element lies in the other half
+    // of the block (only where a pair is (a,a): otherwise the two elements
+    // Z(R1,C1), Z(R1^1,C1^1) share a half), bit 2 its position there, 4 the
     // block is the mirror (x sigma), 5 the phase (-1)^((ji+jj+jk+jl)/2) is
     // -1, 6 side (T2), 7 valid (all four Pandya rows exist)
     ph_zblk.assign(ph_terms.size(), 0);
@@ -396,22 +399,22 @@ std::string Layout::build(int nsp_, const int *n_, const int *l_, const int *jj_
       if (((jj[oi] + jj[oj] + jj[ok] + jj[ol]) >> 1) & 1) sel |= 32;
       if ((R1 | C1 | R2 | C2) >= 0) {
         const int u = R1 >> 1, w = C1 >> 1, h2 = ccrlen[qx] >> 1;
-        int e1, e2;
-        if (u <= w) {
-          e1 = (R1 & 1) * 2 + (C1 & 1);
-          e2 = (R2 & 1) * 2 + (C2 & 1);
-        } else {
-          e1 = (C1 & 1) * 2 + (R1 & 1);
-          e2 = (C2 & 1) * 2 + (R2 & 1);
-          sel |= 16;
-        }
-        const int64_t blk = cwoff[qx] / 4 + (int64_t)std::min(u, w) * h2 + std::max(u, w);
-        if (blk > 0xFFFFFFFFll) return "Z too large for 32-bit block indices";
-        ph_zblk[t] = (uint32_t)blk;
-        sel |= (uint8_t)(128 | e1 | (e2 << 2));
+        if ((R2 >> 1) != u || (C2 >> 1) != w) return "internal: ph term elements in two Z blocks";
+        // the stored element (a, b) of the block (a = its row, b = its column
+        // parity, transposed in a mirror block) sits at 2 (a ^ b) + a
+        auto pos = [&](int R, int C) { return u <= w ? (((R ^ C) & 1) << 1) | (R & 1) : (((R ^ C) & 1) << 1) | (C & 1); };
+        const int e1 = pos(R1, C1), e2 = pos(R2, C2);
+        if (u > w) sel |= 16;
+        const int64_t half = (cwoff[qx] / 4 + (int64_t)std::min(u, w) * h2 + std::max(u, w)) * 2 + (e1 >> 1);
+        if (half > 0xFFFFFFFFll) return "Z too large for 32-bit half-block indices";
+        ph_zblk[t] = (uint32_t)half;
+        sel |= (uint8_t)(128 | (e1 & 1) | (((e1 ^ e2) >> 1) << 1) | ((e2 & 1) << 2));
       }
       ph_zsel[t] = sel;
     }
+    ph_two.clear();
+    for (size_t t = 0; t < ph_terms.size(); t++)
+      if ((ph_zsel[t] & 130) == 130) ph_two.push_back((uint32_t)t);
   }
   // 6j table
   nhalf = (JTM + 1) / 2;

@@ -83,8 +83,9 @@ struct Layout {
   std::vector<int64_t> ph_ubase;    // per global pair row: first upper element
   std::vector<int64_t> ph_tbase;    // per global pair row: the smallest ph_toff of its upper elements
   std::vector<uint64_t> ph_desc;    // per upper element: T offset, run lengths, 6j runs, flags
-  std::vector<uint32_t> ph_zblk;    // per J' term: 2x2 Z block (units of 4 doubles)
+  std::vector<uint32_t> ph_zblk;    // per J' term: 16-byte half of a 2x2 Z block (units of 2 doubles)
   std::vector<uint8_t> ph_zsel;     // per J' term: element selector / flags
+  std::vector<uint32_t> ph_two;     // the J' terms whose two Z elements lie in two halves of a block
 
   // dense 6j table {j1 j2 J3; j4 j5 J6}: j1,j2,j4,j5 half-integer, J3,J6 integer,
   // index order [j1][j2][J3][j4][j5][J6] so that the (j4,j5,J6) a wavefront

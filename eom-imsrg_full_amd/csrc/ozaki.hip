@@ -400,8 +400,10 @@ void launch_seg_gemm_i8(hipStream_t s, const I8Prob *probs, const int4 *tiles, i
 // staged in its own LDS slot at the start (so that no global load of the
 // scaling waits behind the operand loads in flight).  Result element v of
 // lane l is row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32 of the tile;
-// lanes c, c ^ 1 swap one value per row pair so that each writes 16 bytes
-// of a 2x2 Z block, and the 32 lanes of a row pair write 512 contiguous bytes.
+// lanes c, c ^ 1 swap one value per row pair so that each writes one 16-byte
+// half of a 2x2 Z block (the even lane Z(r,c), Z(r+1,c+1), the odd lane
+// Z(r,c+1), Z(r+1,c); see zidx), and the 32 lanes of a row pair write 512
+// contiguous bytes.
 __global__ __launch_bounds__(256, 2) void k_zprod_i8(const I8Prob *__restrict__ probs, const int4 *__restrict__ tiles,
                                                   int ntiles, const signed char *__restrict__ P0,
                                                   const signed char *__restrict__ P1, const double *__restrict__ S0,
@@ -506,12 +508,12 @@ __global__ __launch_bounds__(256, 2) void k_zprod_i8(const I8Prob *__restrict__ 
 #pragma unroll
   for (int v = 0; v < 16; v += 2) {
     const int row = ra + 8 * (v >> 2) + 4 * fh + (v & 3);  // even: rows row, row + 1 are v, v + 1
-    const double y = __shfl_xor(odd ? acc[v] : acc[v + 1], 1, 64);
+    const double y = __shfl_xor(acc[v + 1], 1, 64);  // the partner's row + 1 element
     if (row < M && col < M && (row >> 1) <= (col >> 1)) {
       const long long e = zidx(row, col & ~1, M);
-      if (odd)  // Z(row+1, col-1), Z(row+1, col); a diagonal block takes sigma Z(row, col) for the first
-        *(double2 *)(Zp + e + 2) = make_double2((row >> 1) == (col >> 1) ? sigma * acc[v] : y, acc[v + 1]);
-      else  // Z(row, col), Z(row, col+1)
+      if (odd)  // Z(row, col), Z(row+1, col-1); a diagonal block takes sigma Z(row, col) for the second
+        *(double2 *)(Zp + e + 2) = make_double2(acc[v], (row >> 1) == (col >> 1) ? sigma * acc[v] : y);
+      else  // Z(row, col), Z(row+1, col+1)
         *(double2 *)(Zp + e) = make_double2(acc[v], y);
     }
   }
