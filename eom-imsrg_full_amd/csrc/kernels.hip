@@ -1273,17 +1273,37 @@ __global__ __launch_bounds__(256) void k_sym_mirror(DevLayout L, const int4 *__r
   const int r0 = t.y * 32, c0 = t.z * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
   const double h = (double)herm;
-  for (int k = ty; k < 32; k += 8) {
-    const int r = r0 + k, c = c0 + tx;
-    U[k][tx] = (r < dim && c < dim) ? OV[(long long)r * dim + c] : 0.0;   // upper tile (r0, c0)
-    const int r2 = c0 + k, c2 = r0 + tx;
-    W[k][tx] = (r2 < dim && c2 < dim) ? OV[(long long)r2 * dim + c2] : 0.0;  // lower tile (c0, r0)
+  // a thread writes exactly the elements it loads (upper (r0+k, c0+tx) and
+  // lower (c0+k, r0+tx)), so the accumulator's old values are loaded here
+  // too: every load of the tile pair is in flight at once, the positions past
+  // the block read element 0 and are never stored
+  const double *AR = AV ? AV : OV;
+  double au[4], al[4], x[4], y[4];
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    const int k = ty + 8 * m;
+    const int r = r0 + k, c = c0 + tx, r2 = c0 + k, c2 = r0 + tx;
+    const long long o1 = (r < dim && c < dim) ? (long long)r * dim + c : 0;
+    const long long o2 = (r2 < dim && c2 < dim) ? (long long)r2 * dim + c2 : 0;
+    x[m] = OV[o1];
+    y[m] = OV[o2];
+    au[m] = AR[o1];  // (RES itself when there is no accumulator: a load off any branch)
+    al[m] = AR[o2];
+  }
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    const int k = ty + 8 * m;
+    const bool in1 = r0 + k < dim && c0 + tx < dim, in2 = c0 + k < dim && r0 + tx < dim;
+    U[k][tx] = in1 ? x[m] : 0.0;  // upper tile (r0, c0)
+    W[k][tx] = in2 ? y[m] : 0.0;  // lower tile (c0, r0)
   }
   __syncthreads();
   const bool diag = t.y == t.z;
   double sq = 0.0;
   // upper: element (r0+k, c0+tx)
-  for (int k = ty; k < 32; k += 8) {
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    const int k = ty + 8 * m;
     const int r = r0 + k, c = c0 + tx;
     if (r < dim && c < dim && (!diag || k <= tx)) {
       double v = U[k][tx];
@@ -1292,18 +1312,20 @@ __global__ __launch_bounds__(256) void k_sym_mirror(DevLayout L, const int4 *__r
       const long long o = (long long)r * dim + c;
       OV[o] = v;
       sq += v * v;
-      if (AV) AV[o] = AV[o] + coef * v;
+      if (AV) AV[o] = au[m] + coef * v;
     }
   }
   // lower: element (c0+k, r0+tx) = herm * upper (r0+tx, c0+k)
-  for (int k = ty; k < 32; k += 8) {
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    const int k = ty + 8 * m;
     const int r = c0 + k, c = r0 + tx;
     if (r < dim && c < dim && (!diag || tx < k)) {
       const double v = h * (U[tx][k] + h * W[k][tx]);
       const long long o = (long long)r * dim + c;
       OV[o] = v;
       sq += v * v;
-      if (AV) AV[o] = AV[o] + coef * v;
+      if (AV) AV[o] = al[m] + coef * v;
     }
   }
   if (npart) {
