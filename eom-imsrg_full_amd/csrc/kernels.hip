@@ -466,7 +466,10 @@ __device__ __forceinline__ void seg_tile(const SegProb &p, int m0, int n0, const
     }
     run_segments<A_KMAJOR, MODE, FULL>(p, p.nmid, p.nseg, X0, X1, g0, g1, m0, n0, S, acc, need);
     if (C && active) {
+      // every old value of C first (element 0 outside the upper triangle),
+      // so the loads are in flight together, then the stores
       double *Cp = C + p.c_off;
+      double old[2][2][4];
 #pragma unroll
       for (int ti = 0; ti < 2; ti++)
 #pragma unroll
@@ -474,10 +477,16 @@ __device__ __forceinline__ void seg_tile(const SegProb &p, int m0, int n0, const
 #pragma unroll
           for (int v = 0; v < 4; v++) {
             const int row = m0 + acc_row<MODE>(ti, v), col = n0 + acc_col(tj);
-            if (row < M && col < M && row <= col) {
-              const long long o = (long long)row * M + col;
-              Cp[o] = Cp[o] + acc[ti][tj][v];
-            }
+            old[ti][tj][v] = Cp[(row < M && col < M && row <= col) ? (long long)row * M + col : 0];
+          }
+#pragma unroll
+      for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+        for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int row = m0 + acc_row<MODE>(ti, v), col = n0 + acc_col(tj);
+            if (row < M && col < M && row <= col) Cp[(long long)row * M + col] = old[ti][tj][v] + acc[ti][tj][v];
           }
     }
   } else {
@@ -696,7 +705,10 @@ __device__ __forceinline__ void seg_tile_w(const SegProb &p, int m0, int n0, con
     }
     run(p.nmid, p.nseg);
     if (C && need) {
+      // every old value of C first (element 0 outside the upper triangle),
+      // so the loads are in flight together, then the stores
       double *Cp = C + p.c_off;
+      double old[2][4][4];
 #pragma unroll
       for (int ti = 0; ti < 2; ti++)
 #pragma unroll
@@ -704,10 +716,16 @@ __device__ __forceinline__ void seg_tile_w(const SegProb &p, int m0, int n0, con
 #pragma unroll
           for (int v = 0; v < 4; v++) {
             const int row = m0 + wave * 32 + ti * 16 + g + 4 * v, col = n0 + tj * 16 + (lane & 15);
-            if (row < M && col < M && row <= col) {
-              const long long o = (long long)row * M + col;
-              Cp[o] = Cp[o] + acc[ti][tj][v];
-            }
+            old[ti][tj][v] = Cp[(row < M && col < M && row <= col) ? (long long)row * M + col : 0];
+          }
+#pragma unroll
+      for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+        for (int tj = 0; tj < 4; tj++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int row = m0 + wave * 32 + ti * 16 + g + 4 * v, col = n0 + tj * 16 + (lane & 15);
+            if (row < M && col < M && row <= col) Cp[(long long)row * M + col] = old[ti][tj][v] + acc[ti][tj][v];
           }
     }
   } else {
@@ -850,10 +868,7 @@ __global__ __launch_bounds__(256) void k_pandya(DevLayout L, const double *__res
     for (int e0 = 0; e0 < nsl; e0 += 256 * 8) {
       double v[8];
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const int e = e0 + (int)threadIdx.x + 256 * u;
-        v[u] = e < nsl ? six0[e] : 0.0;
-      }
+      for (int u = 0; u < 8; u++) v[u] = six0[min(e0 + (int)threadIdx.x + 256 * u, nsl - 1)];  // off any branch
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const int e = e0 + (int)threadIdx.x + 256 * u;
@@ -868,14 +883,11 @@ __global__ __launch_bounds__(256) void k_pandya(DevLayout L, const double *__res
       int vr[8], vs[8];
 #pragma unroll
       for (int u = 0; u < 8; u++) {
-        const int e = e0 + (int)threadIdx.x + 256 * u;
-        vr[u] = vs[u] = -1;
-        if (e < nJ * N) {
-          const int J2 = e / N, x = e - J2 * N;
-          const int *pj = L.pidx + (size_t)J2 * NN;
-          vr[u] = h > x ? pj[x * N + h] : pj[h * N + x];
-          vs[u] = p > x ? pj[x * N + p] : pj[p * N + x];
-        }
+        const int e = min(e0 + (int)threadIdx.x + 256 * u, nJ * N - 1);  // off any branch
+        const int J2 = e / N, x = e - J2 * N;
+        const int *pj = L.pidx + (size_t)J2 * NN;
+        vr[u] = pj[h > x ? x * N + h : h * N + x];
+        vs[u] = pj[p > x ? x * N + p : p * N + x];
       }
 #pragma unroll
       for (int u = 0; u < 8; u++) {
@@ -918,26 +930,31 @@ __global__ __launch_bounds__(256) void k_pandya(DevLayout L, const double *__res
       const bool sw1 = h > a, sw2 = p > b;
       const double *six = S6 + ((ja >> 1) * h6 + (jb >> 1)) * nJ;
       double sm = 0.0;
+      // eight J' terms a batch; the element offsets and weights first (a J'
+      // past Jmax, or an element the block does not hold, reads element 0
+      // with weight 0), then the eight gathers, none of them on a branch, so
+      // they are in flight together
       for (int J0 = Jmin; J0 <= Jmax; J0 += 16) {
-        double vv[8], cf[8];
+        long long off[8];
+        double w[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-          const int JT = J0 + 2 * u;
-          vv[u] = 0.0;
-          cf[u] = 0.0;
-          if (JT <= Jmax) {
-            const int rr = PR[(JT >> 1) * N + a], ss = PS[(JT >> 1) * N + b], q = 3 * JT + qb;
-            double pre = 1.0;
-            if (sw1) pre = sgnp(1 + ((jh + ja - JT) >> 1));
-            else if (h == a) pre = pre * M_SQRT2;
-            if (sw2) pre = pre * sgnp(1 + ((jp + jb - JT) >> 1));
-            else if (p == b) pre = pre * M_SQRT2;
-            if ((rr | ss) >= 0) vv[u] = V[BO[q] + (long long)rr * BD[q] + ss] * pre;
-            cf[u] = (double)(sgnp(JT >> 1) * (JT + 1)) * six[JT >> 1];
-          }
+          const int JT = min(J0 + 2 * u, Jmax);
+          const int rr = PR[(JT >> 1) * N + a], ss = PS[(JT >> 1) * N + b], q = 3 * JT + qb;
+          double pre = 1.0;
+          if (sw1) pre = sgnp(1 + ((jh + ja - JT) >> 1));
+          else if (h == a) pre = pre * M_SQRT2;
+          if (sw2) pre = pre * sgnp(1 + ((jp + jb - JT) >> 1));
+          else if (p == b) pre = pre * M_SQRT2;
+          const bool ok = J0 + 2 * u <= Jmax && (rr | ss) >= 0;
+          off[u] = ok ? BO[q] + (long long)rr * BD[q] + ss : 0;
+          w[u] = ok ? pre * (double)(sgnp(JT >> 1) * (JT + 1)) * six[JT >> 1] : 0.0;
         }
+        double vv[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) sm = sm + cf[u] * vv[u];
+        for (int u = 0; u < 8; u++) vv[u] = V[off[u]];
+#pragma unroll
+        for (int u = 0; u < 8; u++) sm = sm + w[u] * vv[u];
       }
       val = sm * sgnp((jh + jb + JC) >> 1) * sq;
     }

@@ -78,7 +78,8 @@ __global__ __launch_bounds__(256) void k_slice_v(DevLayout L, I8Layout I, const 
     for (int u = 0; u < 4; u++) {
       const int j = 256 * t + 4 * lane + u;
       const int c = j < ld ? v_col(j, np, nb, nh, P) : -1;
-      v[t][u] = c >= 0 ? X[c] : 0.0;
+      v[t][u] = X[max(c, 0)];  // off any branch: the sixteen loads are in flight together
+      if (c < 0) v[t][u] = 0.0;
       m = fmax(m, fabs(v[t][u]));
     }
   for (int j = 256 * TC + lane; j < ld; j += 64) {
@@ -179,6 +180,12 @@ __global__ __launch_bounds__(256) void k_slice_cc(DevLayout L, I8Layout I, const
   const int nks = ld >> 5;
   for (int kc = k0; kc < min(k0 + 256, ld); kc += 64) {
     const int kb = kc + kq * 16;
+    double x[16];  // the sixteen loads first, off any branch, so they are in flight together
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      const int k = kb + e;
+      x[e] = src[(row < R && k < K) ? (long long)k * R + row : 0];
+    }
     unsigned w[NS][4];
 #pragma unroll
     for (int g = 0; g < 4; g++) {
@@ -188,7 +195,7 @@ __global__ __launch_bounds__(256) void k_slice_cc(DevLayout L, I8Layout I, const
       for (int u = 0; u < 4; u++) {
         const int k = kb + 4 * g + u;
         unsigned b[NS] = {};
-        if (row < R && k < K && m > 0.0) slices_of(src[(long long)k * R + row], ex, b);
+        if (row < R && k < K && m > 0.0) slices_of(x[4 * g + u], ex, b);
 #pragma unroll
         for (int i = 0; i < NS; i++) w[i][g] |= b[i] << (8 * u);
       }
@@ -370,11 +377,20 @@ __global__ __launch_bounds__(256, 2) void k_seg_gemm_i8(const I8Prob *__restrict
   }
   run(p.nmid, p.nseg);
   if (C && active) {
+    // C's old values are all loaded before the first store (element 0 where
+    // the tile leaves the upper triangle), so the sixteen loads are in flight together
     double *Cp = C + p.c_off;
-    each([&](int row, int col, double x) {
-      const long long o = (long long)row * M + col;
-      Cp[o] = Cp[o] + x;
-    });
+    double old[16];
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+      const int row = ra + 8 * (v >> 2) + 4 * fh + (v & 3), col = cb + fr;
+      old[v] = Cp[(row < M && col < M && row <= col) ? (long long)row * M + col : 0];
+    }
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+      const int row = ra + 8 * (v >> 2) + 4 * fh + (v & 3), col = cb + fr;
+      if (row < M && col < M && row <= col) Cp[(long long)row * M + col] = old[v] + acc[v];
+    }
   }
 }
 
