@@ -52,6 +52,10 @@ struct DevTensor {
   int nd;                         // rank/2 + 1: J4 - J3 and J2 - J1 steps the table holds
   const unsigned *quads;          // orbit quadruples x0|x1<<8|x2<<16|x3<<24 (x0 <= x1, x2 <= x3) with outputs
   long long nquad;
+  // the orbits i the tensor's one-body part couples to orbit x (same tz, l + dpar/2
+  // of the same parity, triangle (ji, jx, R)): part[pstart[x] .. pstart[x+1]),
+  // each i | ji << 8
+  const int *pstart, *part;
   long long f_off, v_off;
 };
 
@@ -384,33 +388,75 @@ __global__ void k_t122(DevLayout L, DevTensor T, const double *__restrict__ LF, 
 
 // TS_commutator_212 (TS_commutators.f90:666-806): accumulates into every element.
 // GV: the scalar operator's V region (v_elem), RF: the tensor's one-body matrix.
+// Per output (ab|cd; J1 J2) four sums over the orbits i the tensor's one-body
+// part couples to a, b, d, c (the partner lists):
+//   -{J1 J2 R; ji ja jb} RF(a,i) v(ib,cd;J2) (-1)^((ja+jb-J2)/2)
+//   +{J1 J2 R; ji jb ja} RF(b,i) v(ia,cd;J2) (-1)^((J1+J2)/2)
+//   +{J1 J2 R; jd ji jc} RF(i,d) v(ab,ci;J1) (-1)^((jc+jd-J1)/2)
+//   -{J1 J2 R; jc ji jd} RF(i,c) v(ab,di;J1) (-1)^((J1+J2)/2)
+// Each v is an element of the scalar channel block of (cd) at J2 or of (ab) at
+// J1 whose fixed pair is the output's own column or row, so it costs one pair
+// lookup and one load; the partners go four at a time with every load off any
+// branch (clamped to a valid entry, weight 0 past the list), so a batch makes
+// three dependent round trips instead of three per partner.
 __global__ void k_t212(DevLayout L, DevTensor T, const double *__restrict__ GV, const double *__restrict__ RF,
                        double *__restrict__ OV) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   int q, a, b, c, d;
   long long vp;
   if (!elem_of(L, T, e, q, a, b, c, d, vp)) return;
-  const int J1 = T.J1[q], J2 = T.J2[q], n = L.nsp, R = T.rank, dp = (T.dpar >> 1) & 1;
+  const int J1 = T.J1[q], J2 = T.J2[q], n = L.nsp, R = T.rank;
   const int ja = L.jj[a], jb = L.jj[b], jc = L.jj[c], jd = L.jj[d];
-  const int ta = L.tz[a], tb = L.tz[b], tc = L.tz[c], td = L.tz[d];
-  const int la = L.l[a] & 1, lb = L.l[b] & 1, lc = L.l[c] & 1, ld = L.l[d] & 1;
+  const int q1 = T.q1[q], q2 = T.q2[q];
+  const long long u = e - T.tstart[q];
+  const int r1 = (int)(u / T.n2[q]), s2 = (int)(u % T.n2[q]);  // row of (ab) in block q1, column of (cd) in q2
+  const double *V1 = GV + L.boff[q1], *V2 = GV + L.boff[q2];
+  const int d1 = L.bdim[q1], d2 = L.bdim[q2];
+  const int *P1 = L.pidx + (size_t)(J1 >> 1) * n * n, *P2 = L.pidx + (size_t)(J2 >> 1) * n * n;
+  const double rab = a == b ? M_SQRT2 : 1.0, rcd = c == d ? M_SQRT2 : 1.0;
+  const double *x6 = T.x6 + (((size_t)(J1 >> 1) * T.nJ + (J2 >> 1)) * T.hx) * T.nh * T.nh;  // {J1 J2 R; . . .}
+  auto six = [&](int j4, int j5, int j6) { return x6[((size_t)(j4 >> 1) * T.nh + (j5 >> 1)) * T.nh + (j6 >> 1)]; };
   double sm = 0.0;
-  for (int i = 0; i < n; i++) {
-    const int ji = L.jj[i], ti = L.tz[i], li = (L.l[i] + dp) & 1;
-    double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
-    if (ti == ta && li == la && dtri(ji, ja, R))
-      s1 = -tx6(T, J1, J2, ji, ja, jb) * RF[(size_t)a * n + i] * dv(L, GV, i, b, c, d, J2);
-    s1 *= sgnp((ja + jb - J2) >> 1);
-    if (ti == tb && li == lb && dtri(ji, jb, R))
-      s2 = tx6(T, J1, J2, ji, jb, ja) * RF[(size_t)b * n + i] * dv(L, GV, i, a, c, d, J2);
-    s2 *= sgnp((J1 + J2) >> 1);
-    if (ti == td && li == ld && dtri(ji, jd, R))
-      s3 = tx6(T, J1, J2, jd, ji, jc) * RF[(size_t)i * n + d] * dv(L, GV, a, b, c, i, J1);
-    s3 *= sgnp((jc + jd - J1) >> 1);
-    if (ti == tc && li == lc && dtri(ji, jc, R))
-      s4 = -tx6(T, J1, J2, jc, ji, jd) * RF[(size_t)i * n + c] * dv(L, GV, a, b, d, i, J1);
-    s4 *= sgnp((J2 + J1) >> 1);
-    sm += s1 + s2 + s3 + s4;
+  // kind 0: x = a, pair (i,b) at J2; 1: x = b, pair (i,a) at J2; 2: x = d, pair (c,i) at J1; 3: x = c, pair (d,i) at J1
+#pragma unroll
+  for (int kind = 0; kind < 4; kind++) {
+    const int x = kind == 0 ? a : kind == 1 ? b : kind == 2 ? d : c;
+    const int o = kind == 0 ? b : kind == 1 ? a : kind == 2 ? c : d;  // the pair's other orbit
+    const int jo = L.jj[o], J = kind < 2 ? J2 : J1;
+    const int k0 = T.pstart[x], k1 = T.pstart[x + 1];
+    const double sg = kind == 0 ? -(double)sgnp((ja + jb - J2) >> 1)
+                      : kind == 1 ? (double)sgnp((J1 + J2) >> 1)
+                      : kind == 2 ? (double)sgnp((jc + jd - J1) >> 1)
+                                  : -(double)sgnp((J1 + J2) >> 1);
+    for (int k = k0; k < k1; k += 4) {
+      int pi[4];
+#pragma unroll
+      for (int v = 0; v < 4; v++) pi[v] = T.part[min(k + v, k1 - 1)];
+      long long pos[4];
+      double w[4];
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = pi[v] & 255, ji = pi[v] >> 8;
+        // the pair's position in its block and v_elem's factor: kinds 0, 1 put i
+        // first (bra (i,o) against the fixed ket (cd)), kinds 2, 3 second (ket (o,i))
+        const int lo = min(i, o), hi = max(i, o);
+        const int rr = (kind < 2 ? P2 : P1)[lo * n + hi];
+        const bool swp = kind < 2 ? i > o : o > i;
+        double f = swp ? (double)sgnp(1 + ((ji + jo - J) >> 1)) : (i == o ? M_SQRT2 : 1.0);
+        const double rf = kind == 0 ? RF[(size_t)a * n + i] : kind == 1 ? RF[(size_t)b * n + i]
+                        : kind == 2 ? RF[(size_t)i * n + d] : RF[(size_t)i * n + c];
+        const double sx = kind == 0 ? six(ji, ja, jb) : kind == 1 ? six(ji, jb, ja)
+                        : kind == 2 ? six(jd, ji, jc) : six(jc, ji, jd);
+        const bool ok = k + v < k1 && rr >= 0;
+        pos[v] = !ok ? 0 : kind < 2 ? (long long)rr * d2 + s2 : (long long)r1 * d1 + rr;
+        w[v] = ok ? f * rf * sx : 0.0;
+      }
+      double g[4];
+#pragma unroll
+      for (int v = 0; v < 4; v++) g[v] = (kind < 2 ? V2 : V1)[pos[v]];
+#pragma unroll
+      for (int v = 0; v < 4; v++) sm += sg * w[v] * g[v] * (kind < 2 ? rcd : rab);
+    }
   }
   sm *= sqrt((J1 + 1.0) * (J2 + 1.0) / (1.0 + (a == b)) / (1.0 + (c == d))) * sgnp(R >> 1);
   OV[vp] += sm;
@@ -893,6 +939,20 @@ static int tspace_build(imsrg_ctx_s *c, int rank, int dpar, TensorSpace &S) {
   TUP(S.cJ3, cJ3); TUP(S.cJ4, cJ4); TUP(S.cq3, cq3); TUP(S.cq4, cq4);
   TUP(S.xoff, xoff); TUP(S.roff, roff); TUP(S.woff, woff); TUP(S.xstart, xstart); TUP(S.rstart, rstart);
   TUP(S.cidx, cidx); TUP(S.ccol, ccol); TUP(S.x6, x6); TUP(S.s6, s6);
+  {
+    std::vector<int> ps(nsp + 1, 0), pl;
+    for (int x = 0; x < nsp; x++) {
+      ps[x] = (int)pl.size();
+      for (int i = 0; i < nsp; i++)
+        if (Ly.tz[i] == Ly.tz[x] && ((Ly.l[i] + dp) & 1) == (Ly.l[x] & 1) && std::abs(Ly.jj[i] - Ly.jj[x]) <= rank &&
+            rank <= Ly.jj[i] + Ly.jj[x])
+          pl.push_back(i | Ly.jj[i] << 8);
+    }
+    ps[nsp] = (int)pl.size();
+    if (pl.empty()) pl.push_back(0);  // a readable entry for the clamped loads
+    TUP(ps, pstart);
+    TUP(pl, part);
+  }
 #undef TUP
   // 9j table of the recoupling and the tensor Pandya transform (up to 4 GiB:
   // rank 2 to emax=14, rank 3 to emax=12)
