@@ -168,6 +168,30 @@ __device__ inline long long t_pos(const DevLayout &L, const DevTensor &T, int he
   pre = f;
   return T.off[q] + (long long)r * T.n2[q] + s;
 }
+// t_pos with every table read made (indices in range whatever the tests
+// say) and no early return: an element the tensor does not hold gets
+// position 0 and factor 0.  For batches whose loads should be in flight
+// together.  chan: the caller's test that the pairs' parities (with the
+// tensor's) and Tz match; j*x the orbits' 2j.
+__device__ inline long long t_pos_all(const DevLayout &L, const DevTensor &T, int herm, int ax, int bx, int cx, int dx,
+                                      int jax, int jbx, int jcx, int jdx, int J1x, int J2x, bool chan, double &pre) {
+  const bool sw = J1x > J2x;
+  const int a = sw ? cx : ax, b = sw ? dx : bx, c = sw ? ax : cx, d = sw ? bx : dx;
+  const int ja = sw ? jcx : jax, jb = sw ? jdx : jbx, jc = sw ? jax : jcx, jd = sw ? jbx : jdx;
+  const int J1 = sw ? J2x : J1x, J2 = sw ? J1x : J2x;
+  const int P = (L.l[a] + L.l[b]) & 1, T2 = L.tz[a] + L.tz[b];
+  const int q = T.bidx[((J1 >> 1) * T.nJ + (J2 >> 1)) * 6 + (T2 / 2 + 1) * 2 + P];
+  const int r = dpos(L, J1, min(a, b), max(a, b)), s = dpos(L, J2, min(c, d), max(c, d));
+  const int qc = max(q, 0);
+  const long long o = T.off[qc];
+  const int n2 = T.n2[qc];
+  double f = sw ? (double)(sgnp((J2 - J1) >> 1) * herm) : 1.0;
+  f *= a > b ? (double)sgnp(1 + ((ja + jb - J1) >> 1)) : (a == b ? M_SQRT2 : 1.0);
+  f *= c > d ? (double)sgnp(1 + ((jc + jd - J2) >> 1)) : (c == d ? M_SQRT2 : 1.0);
+  const bool ok = chan && dtri(J1, J2, T.rank) && dtri(ja, jb, J1) && dtri(jc, jd, J2) && q >= 0 && (r | s) >= 0;
+  pre = ok ? f : 0.0;
+  return ok ? o + (long long)r * n2 + s : 0;
+}
 __device__ __forceinline__ double t_elem(const DevLayout &L, const DevTensor &T, const double *V, int herm, int a,
                                          int b, int c, int d, int J1, int J2) {
   double f;
@@ -495,16 +519,34 @@ __global__ void k_tpandya(DevLayout L, DevTensor T, int mode, const double *__re
   if (a >= 0) {
     const int ja = L.jj[a], jb = L.jj[b], jp = L.jj[p], jh = L.jj[h];
     if (((L.l[a] + L.l[h]) & 1) == ((L.l[b] + L.l[p] + (T.dpar >> 1)) & 1) && L.tz[a] + L.tz[h] == L.tz[b] + L.tz[p]) {
+      // the (J3', J4') terms four at a time: positions, factors and 9j of a
+      // batch first (every table read off any branch), then its four tensor
+      // reads, so a batch makes three dependent round trips instead of one
+      // chain per term.  The channel test is the same for every term.
+      const int a3 = abs(ja - jh), n3 = (ja + jh - a3) / 2 + 1;
+      const int a4 = abs(jp - jb), n4 = (jp + jb - a4) / 2 + 1, nt = n3 * n4;
+      const bool chan = true;  // the parity and Tz tests above hold for both pairs of every term
       double sm = 0.0;
-      for (int J3p = abs(ja - jh); J3p <= ja + jh; J3p += 2)
-        for (int J4p = abs(jp - jb); J4p <= jp + jb; J4p += 2) {
+      for (int t0 = 0; t0 < nt; t0 += 4) {
+        long long pos[4];
+        double w[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const int t = min(t0 + k, nt - 1);
+          const int J3p = a3 + 2 * (t / n4), J4p = a4 + 2 * (t % n4);
           double f;
-          const long long pos = mode == 0 ? t_pos(L, T, hR, a, h, p, b, J3p, J4p, f)
-                                          : t_pos(L, T, hR, h, a, b, p, J3p, J4p, f);
-          if (pos < 0) continue;
-          const double nj = mode == 0 ? t9j_cc(T, ja, jb, J3, jh, jp, J4, J3p, J4p) : t9j_cc(T, jh, jp, J3, ja, jb, J4, J3p, J4p);
-          sm -= sgnp(J4p >> 1) * sqrt((J3p + 1.0) * (J4p + 1.0)) * nj * RV[pos] * f;
+          pos[k] = mode == 0 ? t_pos_all(L, T, hR, a, h, p, b, ja, jh, jp, jb, J3p, J4p, chan, f)
+                             : t_pos_all(L, T, hR, h, a, b, p, jh, ja, jb, jp, J3p, J4p, chan, f);
+          const double nj = mode == 0 ? t9j_cc(T, ja, jb, J3, jh, jp, J4, J3p, J4p)
+                                      : t9j_cc(T, jh, jp, J3, ja, jb, J4, J3p, J4p);
+          w[k] = t0 + k < nt ? -sgnp(J4p >> 1) * sqrt((J3p + 1.0) * (J4p + 1.0)) * nj * f : 0.0;
         }
+        double g[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) g[k] = RV[pos[k]];
+#pragma unroll
+        for (int k = 0; k < 4; k++) sm += w[k] * g[k];
+      }
       const double s34 = sqrt((J3 + 1.0) * (J4 + 1.0));
       v = mode == 0 ? sm * sgnp((jh + jb + J4) >> 1) * s34
                     : sm * sgnp((jp + ja + J4) >> 1) * s34 * sgnp((jp + jh) >> 1);
