@@ -52,6 +52,7 @@ struct DevTensor {
   int nd;                         // rank/2 + 1: J4 - J3 and J2 - J1 steps the table holds
   const unsigned *quads;          // orbit quadruples x0|x1<<8|x2<<16|x3<<24 (x0 <= x1, x2 <= x3) with outputs
   long long nquad;
+  int nslotmax;                   // most gather slots of any quadruple (k_t222ph_q's LDS list)
   // the orbits i the tensor's one-body part couples to orbit x (same tz, l + dpar/2
   // of the same parity, triangle (ji, jx, R)): part[pstart[x] .. pstart[x+1]),
   // each i | ji << 8
@@ -592,6 +593,11 @@ __global__ void k_tpandya(DevLayout L, DevTensor T, int mode, const double *__re
 // Matching the output (x0 x1 | x2 x3) to the k-th form gives (a,b,c,d) as a
 // permutation of x (the packed tables ia..id below) and turns the branch into
 // x0 < x1 and / or x2 < x3; no Wy where J3 = J4 (the reference skips it).
+// the patterns' (a, b, c, d) as indices into (x0, x1, x2, x3), two bits each,
+// pattern k in bits 2k..2k+1, and their branch conditions (1: x0 < x1,
+// 2: x2 < x3, 3: both)
+constexpr unsigned kPhIa = 0x2727DD88u, kPhIb = 0x88DD88DDu, kPhIc = 0x72727272u, kPhId = 0xDD882727u;
+constexpr unsigned kPhCond = 0xF960F960u;
 struct PhSlot {
   double c;      // sign * herm * phase(J3,J4) * sqrt((2J3+1)(2J4+1)) * W
   unsigned b9;   // 9j table index without the (J1, J2) part
@@ -611,16 +617,14 @@ __global__ __launch_bounds__(64) void k_t222ph_q(DevLayout L, DevTensor T, const
   const int x0 = code & 255, x1 = (code >> 8) & 255, x2 = (code >> 16) & 255, x3 = (int)(code >> 24);
   const int R = T.rank, n = L.nsp, JM = L.jtot_max, nd = T.nd, nJ = T.nJ, h = T.nh;
   if (lane < 16) {
-    // (a, b, c, d) as indices into (x0, x1, x2, x3), two bits each, pattern k in bits 2k..2k+1
     const int k = lane;
-    const unsigned ia = 0x2727DD88u, ib = 0x88DD88DDu, ic = 0x72727272u, id = 0xDD882727u;
     auto pick = [&](unsigned tbl) {
       const int i = (tbl >> (2 * k)) & 3;
       return i == 0 ? x0 : i == 1 ? x1 : i == 2 ? x2 : x3;
     };
-    const int a = pick(ia), b = pick(ib), c = pick(ic), d = pick(id);
-    // the branch condition of the reference: 1 x0 < x1, 2 x2 < x3, 3 both
-    const unsigned cond = (0xF960F960u >> (2 * k)) & 3;
+    const int a = pick(kPhIa), b = pick(kPhIb), c = pick(kPhIc), d = pick(kPhId);
+    // the branch condition of the reference
+    const unsigned cond = (kPhCond >> (2 * k)) & 3;
     bool ok = (!(cond & 1) || x0 < x1) && (!(cond & 2) || x2 < x3);
     const int ja = L.jj[a], jb = L.jj[b], jc = L.jj[c], jd = L.jj[d];
     const int Tz = abs(L.tz[d] - L.tz[a]) >> 1, PAR = (L.l[d] + L.l[a]) & 1, PAR2 = PAR ^ ((T.dpar >> 1) & 1);
@@ -712,7 +716,8 @@ __global__ __launch_bounds__(64) void k_t222ph_q(DevLayout L, DevTensor T, const
     }
     const unsigned long long m = __ballot(live);
     if (live) {
-      const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
+      // pos < this quadruple's slot count <= nslotmax (the host counts the same slots)
+      const int pos = min(count + __popcll(m & ((1ull << lane) - 1ull)), T.nslotmax - 1);
       slot[pos].c = cval;
       slot[pos].b9 = b9;
       slot[pos].ab = abk;
@@ -721,31 +726,65 @@ __global__ __launch_bounds__(64) void k_t222ph_q(DevLayout L, DevTensor T, const
   }
   __syncthreads();
   if (count == 0) return;
-  // outputs (x0 x1 | x2 x3; J1 <= J2) of the quadruple
+  // outputs (x0 x1 | x2 x3; J1 <= J2) of the quadruple, 64 at a time: lane o
+  // takes J1 = |j0 - j1| + 2 (o / nd), J2 = J1 + 2 (o % nd), finds its element
+  // and reads its old value (every table read off any branch, for all the
+  // outputs at once); the outputs' sums over the slots then go four at a time
+  // (their 9j reads in flight together), each lane keeps its own output's sum,
+  // and the lanes write their elements together.
   const int j0 = L.jj[x0], j1 = L.jj[x1], j2 = L.jj[x2], j3 = L.jj[x3];
   const int Tzq = (L.tz[x0] + L.tz[x1]) / 2 + 1, P1 = (L.l[x0] + L.l[x1]) & 1;
   const double pre = (x0 == x1 ? M_SQRT2 : 1.0) * (x2 == x3 ? M_SQRT2 : 1.0);
-  for (int J1 = abs(j0 - j1); J1 <= j0 + j1; J1 += 2) {
-    if (x0 == x1 && ((J1 >> 1) & 1)) continue;
-    for (int J2 = max(J1, abs(j2 - j3)); J2 <= min(j2 + j3, J1 + R); J2 += 2) {
-      if (x2 == x3 && ((J2 >> 1) & 1)) continue;
-      if (J1 + J2 < R) continue;
-      const int q = T.bidx[((J1 >> 1) * nJ + (J2 >> 1)) * 6 + Tzq * 2 + P1];
-      if (q < 0) continue;
-      const int r = dpos(L, J1, x0, x1), sc = dpos(L, J2, x2, x3);
-      if ((r | sc) < 0) continue;
-      const unsigned off = (unsigned)((J1 >> 1) * nd + ((J2 - J1) >> 1));
-      const int s1 = (J1 >> 1) & 1, s2 = (J2 >> 1) & 1;
-      double acc = 0.0;
+  const int Jlo = abs(j0 - j1), nout = (min(j0, j1) + 1) * nd;
+  for (int o0 = 0; o0 < nout; o0 += 64) {
+    const int o = o0 + lane;
+    const int J1 = Jlo + 2 * (o / nd), J2 = J1 + 2 * (o % nd);
+    bool val = o < nout && J1 <= j0 + j1 && J2 >= abs(j2 - j3) && J2 <= j2 + j3 && J1 + J2 >= R &&
+               !(x0 == x1 && ((J1 >> 1) & 1)) && !(x2 == x3 && ((J2 >> 1) & 1));
+    const int J1c = val ? J1 : Jlo, J2c = val ? J2 : Jlo;  // in the tables' range
+    const int q = T.bidx[((J1c >> 1) * nJ + (J2c >> 1)) * 6 + Tzq * 2 + P1];
+    const int r = dpos(L, J1c, x0, x1), sc = dpos(L, J2c, x2, x3);
+    const int qc = max(q, 0);
+    val = val && q >= 0 && (r | sc) >= 0;
+    const long long po = val ? T.off[qc] + (long long)r * T.n2[qc] + sc : 0;
+    const double old = OV[po];
+    unsigned long long vm = __ballot(val);
+    double res = 0.0;
+    while (vm) {
+      int ok_[4];
+      unsigned offk[4];
+      int ek[4][2];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        ok_[k] = vm != 0;
+        const int ol = vm ? __ffsll((long long)vm) - 1 : 0;
+        vm &= vm - 1;
+        const int J1k = __shfl(J1, ol, 64), J2k = __shfl(J2, ol, 64);
+        offk[k] = (unsigned)((J1k >> 1) * nd + ((J2k - J1k) >> 1));
+        ek[k][0] = (J1k >> 1) & 1;
+        ek[k][1] = (J2k >> 1) & 1;
+        ok_[k] = ok_[k] ? ol : -1;
+      }
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
       for (int t = lane; t < count; t += 64) {
         const PhSlot sl = slot[t];
-        const int e = ((sl.ab & 1) ? s1 : 0) + ((sl.ab & 2) ? s2 : 0);
-        const double v = sl.c * T.n9[sl.b9 + off];
-        acc += (e & 1) ? -v : v;
+        double v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = T.n9[sl.b9 + offk[k]];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const int e = ((sl.ab & 1) ? ek[k][0] : 0) + ((sl.ab & 2) ? ek[k][1] : 0);
+          const double x = sl.c * v[k];
+          acc[k] += (e & 1) ? -x : x;
+        }
       }
-      acc = wave_sum(acc);
-      if (lane == 0) OV[T.off[q] + (long long)r * T.n2[q] + sc] += acc * sqrt((J1 + 1.0) * (J2 + 1.0)) * pre;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const double a = wave_sum(acc[k]);
+        if (lane == ok_[k]) res = a;
+      }
     }
+    if (val) OV[po] = old + res * sqrt((J1 + 1.0) * (J2 + 1.0)) * pre;
   }
 }
 
@@ -1014,6 +1053,7 @@ static int tspace_build(imsrg_ctx_s *c, int rank, int dpar, TensorSpace &S) {
   {
     std::vector<unsigned> quads;
     std::vector<int> pa_, pb_;
+    int nslotmax = 1;
     for (int a = 0; a < nsp; a++)
       for (int b = a; b < nsp; b++) { pa_.push_back(a); pb_.push_back(b); }
     for (size_t u = 0; u < pa_.size(); u++) {
@@ -1033,11 +1073,28 @@ static int tspace_build(imsrg_ctx_s *c, int rank, int dpar, TensorSpace &S) {
             break;
           }
         }
-        if (any) quads.push_back((unsigned)x0 | (unsigned)x1 << 8 | (unsigned)x2 << 16 | (unsigned)x3 << 24);
+        if (!any) continue;
+        quads.push_back((unsigned)x0 | (unsigned)x1 << 8 | (unsigned)x2 << 16 | (unsigned)x3 << 24);
+        // the quadruple's gather slots (k_t222ph_q): a run of J3 times the
+        // J4 - J3 steps per pattern that passes the branch, parity and Tz tests
+        const int xs[4] = {x0, x1, x2, x3};
+        int ns = 0;
+        for (int k = 0; k < 16; k++) {
+          const int a = xs[(kPhIa >> (2 * k)) & 3], b = xs[(kPhIb >> (2 * k)) & 3];
+          const int cc = xs[(kPhIc >> (2 * k)) & 3], dd = xs[(kPhId >> (2 * k)) & 3];
+          const unsigned cond = (kPhCond >> (2 * k)) & 3;
+          if (((cond & 1) && !(x0 < x1)) || ((cond & 2) && !(x2 < x3))) continue;
+          const int Tz = std::abs(Ly.tz[dd] - Ly.tz[a]) >> 1, PAR2 = ((Ly.l[dd] + Ly.l[a]) & 1) ^ dp;
+          if (((Ly.l[cc] + Ly.l[b]) & 1) != PAR2 || (std::abs(Ly.tz[cc] - Ly.tz[b]) >> 1) != Tz) continue;
+          const int jd = Ly.jj[dd], ja = Ly.jj[a];
+          ns += ((jd + ja - std::abs(jd - ja)) / 2 + 1) * (rank / 2 + 1);
+        }
+        nslotmax = std::max(nslotmax, ns);
       }
     }
     if (upload_table(c, quads, &D.quads) != 0) return IMSRG_ENOMEM;
     D.nquad = (long long)quads.size();
+    D.nslotmax = nslotmax;
   }
   // products: the pp and hh ladders per tensor block (W1, W2)
   {
@@ -1350,7 +1407,7 @@ static int ts_comm(imsrg_ctx_s *c, const double *Gd, int hG, imsrg_tensor T, ims
                          S.n_cc_tiles[1], (const double *)S.ccx, (const double *)S.gcc, S.wy);
     if (D.nquad > 0)
       hipLaunchKernelGGL(k_t222ph_q, dim3((unsigned)D.nquad), dim3(64),
-                         (unsigned)(sizeof(PhSlot) * 16 * D.nJ * D.nd), s, L, D, (const double *)S.wx,
+                         (unsigned)(sizeof(PhSlot) * D.nslotmax), s, L, D, (const double *)S.wx,
                          (const double *)S.wy, OV, hL, hR);
   }
   HIPCHK(hipGetLastError());
