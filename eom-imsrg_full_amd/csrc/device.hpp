@@ -297,6 +297,7 @@ void launch_zprod_i8(hipStream_t s, const I8Prob *probs, const int4 *tiles, int 
                      double sigma, const int *act, int nact, long long sp, long long ss, long long sz);
 
 constexpr int kPartials = 1024;  // reduction partials per instance
+constexpr int kE0Split = 8;      // E0 (220) workgroups per channel block
 constexpr int kScal = 64;        // scalar slots per instance
 constexpr int kAdvSlot = 8;      // Magnus series: advals[0..6] in slots 8..14
 constexpr int kFullSlot = 15;    // Magnus series: ||Omega||

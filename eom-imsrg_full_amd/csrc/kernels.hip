@@ -1435,8 +1435,12 @@ __global__ __launch_bounds__(256) void k_ob_partial(DevLayout L, const double *_
     const int a = L.php_a[e], i = L.php_i[e], ji = L.jj[i], jq = L.jj[qo];
     const int JT = abs(ji - jq) + 2 * m;
     if (JT <= ji + jq) {
-      const double smx = dv(L, VR, a, p, i, qo, JT), smx2 = dv(L, VL, a, p, i, qo, JT);
-      const double smy = dv(L, VR, i, p, a, qo, JT), smy2 = dv(L, VL, i, p, a, qo, JT);
+      // the two elements' positions (the same in L and R), then the four
+      // reads together (position 0 and factor 0 for a structural zero)
+      double fx, fy;
+      const long long ox = dv_pos(L, a, p, i, qo, JT, fx), oy = dv_pos(L, i, p, a, qo, JT, fy);
+      const long long oxc = ox < 0 ? 0 : ox, oyc = oy < 0 ? 0 : oy;
+      const double smx = VR[oxc] * fx, smx2 = VL[oxc] * fx, smy = VR[oyc] * fy, smy2 = VL[oyc] * fy;
       v = (JT + 1) * (FL[a * N + i] * (hL * smx - smy) - FR[a * N + i] * (hR * smx2 - smy2));
     }
   } else if (it < n121 + n221) {
@@ -1508,9 +1512,10 @@ void launch_ob_final(hipStream_t s, const DevLayout &L, const double *Lop, const
 
 // ------------------------------------------------------------------ reductions
 
-// 220 per block (commutators.f90:399-432) into partials[q]; 110 into partials[nblocks]
+// 220 per block (commutators.f90:399-432) into partials[q * nsplit + split]; 110 into
+// partials[nblocks * nsplit] (nsplit workgroups per block, as many as kPartials allows, at most kE0Split)
 __global__ __launch_bounds__(256) void k_e0_partial(DevLayout L, const double *Lop, const double *Rop,
-                                                     int hL, int hR, double *partials, const int *act) {
+                                                     int hL, int hR, double *partials, const int *act, int nsplit) {
   __shared__ double sh[256];
   {
     const long long inst = inst_id(act);
@@ -1520,17 +1525,19 @@ __global__ __launch_bounds__(256) void k_e0_partial(DevLayout L, const double *L
     partials += inst * kPartials;
   }
   const int b = blockIdx.x;
-  if (b < L.nblocks) {
-    const int np = L.bnp[b], nb = L.bnb[b], nh = L.bnh[b], dim = L.bdim[b];
-    const double *A = Lop + L.v_off + L.boff[b], *B = Rop + L.v_off + L.boff[b];
+  if (b < L.nblocks * nsplit) {
+    // nsplit workgroups per block, each a strided share of its pphh elements
+    const int q = b / nsplit, sp = b - q * nsplit;
+    const int np = L.bnp[q], nb = L.bnb[q], nh = L.bnh[q], dim = L.bdim[q];
+    const double *A = Lop + L.v_off + L.boff[q], *B = Rop + L.v_off + L.boff[q];
     double s = 0.0;
-    for (int e = threadIdx.x; e < np * nh; e += 256) {
+    for (int e = sp * 256 + (int)threadIdx.x; e < np * nh; e += 256 * nsplit) {
       const int ix = e / nh, jx = e % nh;
       const long long o = (long long)ix * dim + np + nb + jx;
       s = s - A[o] * B[o] * hR;
     }
     s = block_sum256(s, sh);
-    if (threadIdx.x == 0) partials[b] = s * (L.bJ[b] + 1);
+    if (threadIdx.x == 0) partials[b] = s * (L.bJ[q] + 1);
   } else {
     const int N = L.nsp;
     const double *FL = Lop + L.f_off, *FR = Rop + L.f_off;
@@ -1545,7 +1552,7 @@ __global__ __launch_bounds__(256) void k_e0_partial(DevLayout L, const double *L
     if (threadIdx.x == 0) partials[b] = s;
   }
 }
-__global__ __launch_bounds__(256) void k_e0_final(int nblocks, const double *partials, double *RES,
+__global__ __launch_bounds__(256) void k_e0_final(int nparts, const double *partials, double *RES,
                                                    int use110, int use220, const int *act, long long rstride) {
   __shared__ double sh[256];
   {
@@ -1555,11 +1562,11 @@ __global__ __launch_bounds__(256) void k_e0_final(int nblocks, const double *par
     RES += inst * rstride;
   }
   double s = 0.0;
-  for (int q = threadIdx.x; q < nblocks; q += 256) s += partials[q];
+  for (int q = threadIdx.x; q < nparts; q += 256) s += partials[q];
   s = block_sum256(s, sh);
   if (threadIdx.x == 0) {
     double e = 0.0;
-    if (use110) e = partials[nblocks];
+    if (use110) e = partials[nparts];
     if (use220) e = e + s * 2.0;
     RES[0] = e;
   }
@@ -1567,10 +1574,11 @@ __global__ __launch_bounds__(256) void k_e0_final(int nblocks, const double *par
 void launch_e0(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES,
                int hL, int hR, unsigned mask, double *partials, const int *act, int nact, long long rstride) {
   if (nact <= 0) return;
-  hipLaunchKernelGGL(k_e0_partial, dim3(L.nblocks + 1, nact), dim3(256), 0, s, L, Lop, Rop, hL, hR, partials,
-                     act);
+  const int nsplit = std::max(1, std::min(kE0Split, (kPartials - 1) / std::max(L.nblocks, 1)));
+  hipLaunchKernelGGL(k_e0_partial, dim3(L.nblocks * nsplit + 1, nact), dim3(256), 0, s, L, Lop, Rop, hL, hR,
+                     partials, act, nsplit);
   const int use220 = (mask & T220) && (hL * hR == -1);
-  hipLaunchKernelGGL(k_e0_final, dim3(1, nact), dim3(256), 0, s, L.nblocks, partials, RES,
+  hipLaunchKernelGGL(k_e0_final, dim3(1, nact), dim3(256), 0, s, L.nblocks * nsplit, partials, RES,
                      (mask & T110) ? 1 : 0, use220 ? 1 : 0, act, rstride);
 }
 
