@@ -1539,12 +1539,12 @@ __global__ __launch_bounds__(256) void k_e0_partial(DevLayout L, const double *L
     s = block_sum256(s, sh);
     if (threadIdx.x == 0) partials[b] = s * (L.bJ[q] + 1);
   } else {
-    const int N = L.nsp;
+    // the (particle, hole) pairs: the last P*A entries of the one-body output list
+    const int N = L.nsp, A = L.nhole, P = L.npart, t0 = A * (A + 1) / 2 + P * (P + 1) / 2;
     const double *FL = Lop + L.f_off, *FR = Rop + L.f_off;
     double s = 0.0;
-    for (int e = threadIdx.x; e < N * N; e += 256) {
-      const int a = e / N, i = e % N;
-      if (L.con[a] != 0 || L.con[i] != 1) continue;
+    for (int e = threadIdx.x; e < P * A; e += 256) {
+      const int a = L.ob_p[t0 + e], i = L.ob_q[t0 + e];
       const double lf = FL[a * N + i], rf = FR[a * N + i];
       s = s + (lf * rf * hL - rf * lf * hR) * (L.jj[i] + 1);
     }
