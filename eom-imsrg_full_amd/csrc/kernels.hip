@@ -1393,9 +1393,22 @@ __device__ __forceinline__ double w221(const DevLayout &L, const double *Wpp, co
   } else {
     const double *Lb = VL + bo, *Rb = VR + bo;
     val = 0.0;
-    for (int k = L.bnp[q] + L.bnb[q]; k < dim; k++)
-      val += Lb[(long long)r * dim + k] * Rb[(long long)k * dim + s] -
-             Rb[(long long)r * dim + k] * Lb[(long long)k * dim + s];
+    // the hh columns four at a time, reads clamped into the block (weight 0
+    // past its end) so that the sixteen reads are in flight together
+    for (int k0 = L.bnp[q] + L.bnb[q]; k0 < dim; k0 += 4) {
+      double lr[4], rk[4], rr[4], lk[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int k = min(k0 + u, dim - 1);
+        lr[u] = Lb[(long long)r * dim + k];
+        rk[u] = Rb[(long long)k * dim + s];
+        rr[u] = Rb[(long long)r * dim + k];
+        lk[u] = Lb[(long long)k * dim + s];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (k0 + u < dim) val += lr[u] * rk[u] - rr[u] * lk[u];
+    }
   }
   return val * pre;
 }
