@@ -1605,9 +1605,18 @@ __global__ __launch_bounds__(256) void k_sumsq_partial(const double *x, long lon
     x += inst * stride;
     partials += inst * kPartials;
   }
+  // eight strided elements per round, read off any branch (clamped to the
+  // last element, discarded past it): the same summation order as one at a time
   double s = 0.0;
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
-    s += x[i] * x[i];
+  const long long st = (long long)gridDim.x * 256;
+  for (long long i0 = (long long)blockIdx.x * 256 + threadIdx.x; i0 < n; i0 += 8 * st) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = x[min(i0 + u * st, n - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (i0 + u * st < n) s += v[u] * v[u];
+  }
   s = block_sum256(s, sh);
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
@@ -1620,8 +1629,15 @@ __global__ __launch_bounds__(256) void k_final_sum(const double *partials, int n
     partials += inst * pstride;
     out += inst * kScal;
   }
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) s += partials[i];
+  double s = 0.0;  // eight partials per round off any branch, in the one-at-a-time order
+  for (int i0 = threadIdx.x; i0 < n; i0 += 8 * 256) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = partials[min(i0 + 256 * u, n - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (i0 + 256 * u < n) s += v[u];
+  }
   s = block_sum256(s, sh);
   if (threadIdx.x == 0) out[0] = do_sqrt ? sqrt(s) : s;
 }
