@@ -227,7 +227,7 @@ def basis_tag(basis):
 # kernel class -> kernel symbol of the FP64 path (rocprofv3 names)
 KERNEL_NAME = {
     "gemm_pphh": "k_seg_gemm<false, 0, 0>", "gemm_ph": "k_seg_gemm<true, 0, 1>", "pandya": "k_pandya(",
-    "k122": "k_122(", "ph_terms": "k_ph_terms(", "ph_apply": "k_ph_apply(", "sym_mirror": "k_sym_mirror(",
+    "k122": "k_122s(", "ph_terms": "k_ph_terms(", "ph_apply": "k_ph_apply(", "sym_mirror": "k_sym_mirror(",
     "onebody": "k_ob_partial(",
     # int8x5 products and slicing
     "gemm_pphh_i8": "k_seg_gemm_i8", "gemm_ph_i8": "k_zprod_i8", "slice_v": "k_slice_v(",

@@ -27,6 +27,14 @@ struct DevLayout {
   const int *pidx;
   const int *nb_start, *nb_pos, *nb_x, *nb_i;
   const double *nb_coef;
+  const struct Set122 *s122_sets;  // 122 row sets
+  const int *s122_rowl;            // per set row: its row in the block
+  const int2 *s122_items;          // set kernel work items (set, 64-column chunk)
+  const int2 *s122_sx;             // per slot: f-matrix indices of its two entries
+  const double *s122_sc;           // per slot: their two coefficients
+  int n122_items, n122_slots;
+  const int *s122_frows;                                 // rows of the sets past 32 rows (row kernel)
+  int n122_frows;
   // Hartree-Fock: groups in the reference's block order, same-group orbit
   // pairs (all; a <= c), per pair row the rows the pair-basis transform couples
   const int *hf_gorder, *hf_b, *hf_d, *hf_a, *hf_c, *hf_nb_start, *hf_nb;
@@ -57,7 +65,15 @@ struct DevLayout {
   int nmirror;
   long long f_off, v_off, op_size, vsize, ccsize, cwsize;  // per-instance strides
   const struct PhChunk *ph_chunks;  // k_ph_apply work items, rows in (i, j, J) order
-  XcdSplit xs_rows, xs_terms, xs_apply;  // k_122 rows, k_ph_terms chunks of 1024 terms, ph_chunks
+  XcdSplit xs_rows, xs_terms, xs_apply;  // k_122 rows (of s122_frows), k_ph_terms chunks of 1024 terms, ph_chunks
+};
+
+// A 122 row set: rows s122_rowl[rs .. rs + n) of block q (dimension dim,
+// matrix at bo doubles into the two-body part); its n x 2n weights start at
+// ab in the weight table (-1: the row kernel takes the set)
+struct alignas(16) Set122 {
+  int rs, n, dim, q;
+  long long bo, ab;
 };
 
 // One k_ph_apply work item: up to 64 consecutive upper outputs of one row
@@ -188,7 +204,7 @@ hipError_t pandya_allow_lds(size_t bytes);
 // 122: M = A_L R - A_R L over whole channel blocks (diagonal symmetrised);
 // launch_sym_mirror later forms RES = M + herm*M^T
 void launch_122(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES,
-                int hres, const int *act, int nact);
+                int hres, const int *act, int nact, double *W);
 void launch_clear_blocks(hipStream_t s, const DevLayout &L, double *RES, const int *act, int nact);
 // ph recoupling: J' terms T(quadruple, J') from Z, then RES(upper) += 6j-weighted sums
 void launch_ph_terms(hipStream_t s, const DevLayout &L, const double *Z, double *Tbuf, double hL, double hR,

@@ -353,6 +353,23 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
   if (!rc) rc = upload_table(c, vec, &D.field)
   UP(Ly.jj, jj); UP(Ly.l, l); UP(Ly.tz, tz); UP(Ly.con, con);
   UP(Ly.grp_of, grp_of); UP(Ly.grp_start, grp_start); UP(Ly.grp_cnt, grp_cnt); UP(Ly.grp_list, grp_list);
+  UP(Ly.s122_frows, s122_frows);
+  D.n122_frows = (int)Ly.s122_frows.size();
+  {
+    std::vector<Set122> sets;
+    std::vector<int> rowl(Ly.s122_rows.size());
+    for (size_t k = 0; k < Ly.s122_q.size(); k++) {
+      const int q = Ly.s122_q[k], rs = Ly.s122_start[k], n = Ly.s122_start[k + 1] - rs;
+      sets.push_back(Set122{rs, n, Ly.bdim[q], q, (long long)Ly.boff[q], Ly.s122_ab[k]});
+      for (int j = rs; j < rs + n; j++) rowl[j] = Ly.s122_rows[j] - Ly.pbase[q];
+    }
+    std::vector<int2> items(Ly.s122_items.size() / 2), sx(Ly.s122_sx.size() / 2);
+    for (size_t k = 0; k < items.size(); k++) items[k] = make_int2(Ly.s122_items[2 * k], Ly.s122_items[2 * k + 1]);
+    for (size_t k = 0; k < sx.size(); k++) sx[k] = make_int2(Ly.s122_sx[2 * k], Ly.s122_sx[2 * k + 1]);
+    D.n122_items = (int)items.size();
+    D.n122_slots = (int)sx.size();
+    UP(sets, s122_sets); UP(rowl, s122_rowl); UP(items, s122_items); UP(sx, s122_sx); UP(Ly.s122_sc, s122_sc);
+  }
   UP(Ly.bJ, bJ); UP(Ly.bnp, bnp); UP(Ly.bnb, bnb); UP(Ly.bnh, bnh); UP(Ly.bdim, bdim); UP(boff, boff);
   UP(Ly.pbase, pbase); UP(Ly.pair_a, pair_a); UP(Ly.pair_b, pair_b); UP(Ly.pair_q, pair_q);
   UP(Ly.pidx, pidx);
@@ -387,9 +404,11 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
     UP(ub, ph_ubase);
   }
   {
-    std::vector<double> wr(Ly.npairs), wt((Ly.ph_terms.size() + 1023) / 1024, 1.0);
-    for (int r = 0; r < Ly.npairs; r++)
-      wr[r] = Ly.bdim[Ly.pair_q[r]] * (double)(Ly.nb_start[r + 1] - Ly.nb_start[r] + 1);
+    std::vector<double> wr(Ly.s122_frows.size()), wt((Ly.ph_terms.size() + 1023) / 1024, 1.0);
+    for (size_t k = 0; k < wr.size(); k++) {
+      const int r = Ly.s122_frows[k];
+      wr[k] = Ly.bdim[Ly.pair_q[r]] * (double)(Ly.nb_start[r + 1] - Ly.nb_start[r] + 1);
+    }
     D.xs_rows = make_split(wr);
     D.xs_terms = make_split(wt);
     std::vector<int> arow(Ly.npairs);
@@ -436,6 +455,7 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
   if (!rc) rc = dalloc(c, Ly.ccsize * B, &c->cc[0]);
   if (!rc) rc = dalloc(c, Ly.ccsize * B, &c->cc[1]);
   if (!rc) rc = dalloc(c, Ly.cwsize * B, &c->wph);
+  if (!rc) rc = dalloc(c, Ly.s122_sx.size() * B, &c->w122);
   if (!rc) rc = dalloc(c, Ly.ph_terms.size() * B, &c->tph);
   if (!rc) rc = dalloc(c, kPartials * B, &c->partials);
   if (!rc) rc = dalloc(c, (size_t)std::max(D.nsame, 1) * D.ob_nchunk * B, &c->obpart);
@@ -1215,7 +1235,7 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
   c->fork_mark = nullptr;
   if (mask & (T110 | T220))
     PROFS(c, IMSRG_PROF_E0, s2, launch_e0(s2, D, L->d, R->d, RES->d, hL, hR, mask, c->partials, ad, na, OS));
-  if (mask & T122) PROFS(c, IMSRG_PROF_122, s2, launch_122(s2, D, L->d, R->d, RES->d, hres, ad, na));
+  if (mask & T122) PROFS(c, IMSRG_PROF_122, s2, launch_122(s2, D, L->d, R->d, RES->d, hres, ad, na, c->w122));
   else launch_clear_blocks(s2, D, RES->d, ad, na);
   const bool i8 = c->prec == IMSRG_PREC_INT8X5;
   if ((mask & (T222PPHH | T221)) && i8) {

@@ -59,6 +59,7 @@ struct imsrg_ctx_s {
   double *wpp = nullptr;    // 221 intermediate W1 - pm*W1^T (upper triangles), vsize per instance
   double *cc[2] = {nullptr, nullptr};
   double *wph = nullptr;
+  double *w122 = nullptr;   // 122 set weights (A, -B per slot), per instance
   double *tph = nullptr;    // ph J' terms
   double *partials = nullptr;
   double *obpart = nullptr;  // one-body 121/221 partial sums, nsame * ob_nchunk per instance

@@ -984,11 +984,13 @@ void launch_pandya(hipStream_t s, const DevLayout &L, const double *op, double *
 // whole block (one wavefront per row, coalesced 16-byte row reads, wave-
 // uniform neighbour entries); the diagonal is written already symmetrised,
 // M(r,r)*(1+herm).  k_sym_mirror adds the transpose at the end.
+// The row form: one wavefront per row (the rows of the sets past 32 rows).
 __global__ __launch_bounds__(64) void k_122(DevLayout L, const double *__restrict__ Lop,
                                             const double *__restrict__ Rop, double *__restrict__ RES,
                                             double hres, const int *act) {
-  const int r = xcd_item(L.xs_rows);
-  if (r < 0) return;
+  const int ri = xcd_item(L.xs_rows);
+  if (ri < 0) return;
+  const int r = L.s122_frows[ri];
   {
     const long long inst = inst_id(act);
     if (inst < 0) return;
@@ -1041,10 +1043,153 @@ __global__ __launch_bounds__(64) void k_122(DevLayout L, const double *__restric
     }
   }
 }
+// The set form.  A set holds n <= 32 rows of one block whose neighbour rows
+// all lie in the set, so over any column the set's 122 outputs are a small
+// dense product: M = A R - B L (n x dim), A(k,k') = sum of c_e fL(e) over the
+// entries of row k whose neighbour is row k' (B with fR).  Written as one
+// product M = W X with W(k, 2k'+t) = (A, -B)(k,k') and X(2k'+t, c) = (R, L)
+// rows, it runs on v_mfma_f64_16x16x4.  k_122w forms every set's W (n x 2n,
+// row-major) from its slot entries, one thread per slot.
+__global__ __launch_bounds__(256) void k_122w(DevLayout L, const double *__restrict__ Lop,
+                                              const double *__restrict__ Rop, double *__restrict__ W,
+                                              const int *act) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= L.n122_slots) return;
+  const long long inst = inst_id(act);
+  if (inst < 0) return;
+  const double *FL = Lop + inst * L.op_size + L.f_off, *FR = Rop + inst * L.op_size + L.f_off;
+  const int2 x = L.s122_sx[i];
+  const dpair_t c = *(const dpair_t *)(L.s122_sc + 2 * (long long)i);
+  const dpair_t w = {c.x * FL[x.x] + c.y * FL[x.y], -(c.x * FR[x.x] + c.y * FR[x.y])};
+  *(dpair_t *)(W + inst * 2 * (long long)L.n122_slots + 2 * (long long)i) = w;
+}
+
+// One wavefront per (set, 64 columns) of k_122w's W times the set's R and L
+// rows.  A k-step of 4 covers rows 2kq, 2kq+1, R and L each (k = 4kq +
+// lane/16 reads row 2kq + lane/32, L for odd k); the set is padded to 2*NK
+// rows (pad rows read the last row again with zero weight).  A chunk of 64
+// columns inside the block runs as two 32-column halves, each lane reading
+// columns 2(lane%16), +1 of a half as one 16-byte access: its even and odd
+// columns are two 16-column tiles.  The block's last chunk, when shorter,
+// runs as four 16-column tiles with clamped columns.  Sets past 16 rows run
+// as two 16-row output tiles over two 16-row halves of the rows read.
+template <int NK, int TU, bool FULL>
+__device__ __forceinline__ void s122_wave(const int *__restrict__ RL, const double *__restrict__ Wt, int n, int dim,
+                                          const double *src, int c0, double hres, double *OV, int nhalf) {
+  const int lane = threadIdx.x & 63, cc = lane & 15, g = lane >> 4, n2 = 2 * n;
+  for (int mt = 0; mt < nhalf; mt++) {
+    double4_t acc[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) acc[u] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    for (int kh = 0; kh < nhalf; kh++) {
+      int pr[NK];
+      double a[NK];
+#pragma unroll
+      for (int kq = 0; kq < NK; kq++) {
+        const int r0 = RL[min(16 * kh + 2 * kq, n - 1)], r1 = RL[min(16 * kh + 2 * kq + 1, n - 1)];
+        pr[kq] = (g >> 1) ? r1 : r0;
+        const int m = 16 * mt + cc, j = 32 * kh + 4 * kq + g;
+        const double w = Wt[min(m, n - 1) * n2 + min(j, n2 - 1)];
+        a[kq] = (m < n && j < n2) ? w : 0.0;
+      }
+      if (FULL) {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          dpair_t x[NK];
+#pragma unroll
+          for (int kq = 0; kq < NK; kq++) x[kq] = *(const dpair_t *)(src + (long long)pr[kq] * dim + c0 + 32 * h + 2 * cc);
+#pragma unroll
+          for (int kq = 0; kq < NK; kq++) {
+            acc[2 * h] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kq], x[kq].x, acc[2 * h], 0, 0, 0);
+            acc[2 * h + 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kq], x[kq].y, acc[2 * h + 1], 0, 0, 0);
+          }
+        }
+      } else {
+        int col[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) col[u] = min(c0 + 16 * u + cc, dim - 1);
+#pragma unroll
+        for (int u0 = 0; u0 < 4; u0 += TU) {
+          double b[NK][TU];
+#pragma unroll
+          for (int kq = 0; kq < NK; kq++)
+#pragma unroll
+            for (int u = 0; u < TU; u++) b[kq][u] = src[(long long)pr[kq] * dim + col[u0 + u]];
+#pragma unroll
+          for (int kq = 0; kq < NK; kq++)
+#pragma unroll
+            for (int u = 0; u < TU; u++)
+              acc[u0 + u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kq], b[kq][u], acc[u0 + u], 0, 0, 0);
+        }
+      }
+    }
+    // accumulator element v: row 16 mt + lane/16 + 4v, column lane%16 of its tile
+    const double dg = 1.0 + hres;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int m = 16 * mt + g + 4 * v;
+      if (m < n) {
+        const int rk = RL[m];
+        double *orow = OV + (long long)rk * dim;
+        if (FULL) {
+#pragma unroll
+          for (int h = 0; h < 2; h++) {
+            const int c = c0 + 32 * h + 2 * cc;
+            dpair_t y = {acc[2 * h][v], acc[2 * h + 1][v]};
+            if (c == rk) y.x *= dg;
+            if (c + 1 == rk) y.y *= dg;
+            *(dpair_t *)(orow + c) = y;
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const int c = c0 + 16 * u + cc;
+            if (c < dim) orow[c] = c == rk ? acc[u][v] * dg : acc[u][v];
+          }
+        }
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_122s(
+    DevLayout L, const double *__restrict__ Lop, const double *__restrict__ Rop, double *__restrict__ RES,
+    const double *__restrict__ W, double hres, const int *act) {
+  const int2 it = L.s122_items[blockIdx.x];
+  const long long inst = inst_id(act);
+  if (inst < 0) return;
+  const long long o = inst * L.op_size;
+  const Set122 S = L.s122_sets[it.x];
+  const int n = S.n, dim = S.dim, c0 = 64 * it.y;
+  const long long bo = o + L.v_off + S.bo;
+  const int *RL = L.s122_rowl + S.rs;
+  const double *Wt = W + inst * 2 * (long long)L.n122_slots + S.ab;
+  // k = 4kq + lane/16: odd k read L
+  const double *src = ((threadIdx.x >> 4) & 1) ? Lop + bo : Rop + bo;
+  double *OV = RES + bo;
+  if (c0 + 64 <= dim) {
+    if (n <= 2) s122_wave<1, 4, true>(RL, Wt, n, dim, src, c0, hres, OV, 1);
+    else if (n <= 4) s122_wave<2, 4, true>(RL, Wt, n, dim, src, c0, hres, OV, 1);
+    else if (n <= 8) s122_wave<4, 2, true>(RL, Wt, n, dim, src, c0, hres, OV, 1);
+    else s122_wave<8, 1, true>(RL, Wt, n, dim, src, c0, hres, OV, n > 16 ? 2 : 1);
+  } else {
+    if (n <= 2) s122_wave<1, 4, false>(RL, Wt, n, dim, src, c0, hres, OV, 1);
+    else if (n <= 4) s122_wave<2, 4, false>(RL, Wt, n, dim, src, c0, hres, OV, 1);
+    else if (n <= 8) s122_wave<4, 2, false>(RL, Wt, n, dim, src, c0, hres, OV, 1);
+    else s122_wave<8, 1, false>(RL, Wt, n, dim, src, c0, hres, OV, n > 16 ? 2 : 1);
+  }
+}
+
 void launch_122(hipStream_t s, const DevLayout &L, const double *Lop, const double *Rop, double *RES,
-                int hres, const int *act, int nact) {
+                int hres, const int *act, int nact, double *W) {
   if (L.npairs <= 0 || nact <= 0) return;
-  hipLaunchKernelGGL(k_122, dim3(8 * L.xs_rows.len, nact), dim3(64), 0, s, L, Lop, Rop, RES, (double)hres, act);
+  if (L.n122_items > 0) {
+    hipLaunchKernelGGL(k_122w, dim3((L.n122_slots + 255) / 256, nact), dim3(256), 0, s, L, Lop, Rop, W, act);
+    hipLaunchKernelGGL(k_122s, dim3(L.n122_items, nact), dim3(64), 0, s, L, Lop, Rop, RES, (const double *)W,
+                       (double)hres, act);
+  }
+  if (L.n122_frows > 0)
+    hipLaunchKernelGGL(k_122, dim3(8 * L.xs_rows.len, nact), dim3(64), 0, s, L, Lop, Rop, RES, (double)hres, act);
 }
 
 // zero every channel block (both triangles): the start state when 122 is off

@@ -5,6 +5,7 @@
 #include "layout.hpp"
 
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -180,6 +181,80 @@ std::string Layout::build(int nsp_, const int *n_, const int *l_, const int *jj_
     }
   }
   nb_start[npairs] = (int)nb_pos.size();
+  // 122 row sets: rows grouped by block and unordered group pair, each
+  // neighbour entry's index inside its set, and for the sets the set kernel
+  // takes (at most 32 rows, at most two entries per (row, neighbour) slot)
+  // the slots' entries and the work items
+  {
+    s122_q.clear(); s122_start.clear(); s122_rows.clear(); s122_frows.clear(); s122_items.clear();
+    s122_ab.clear(); s122_sx.clear(); s122_sc.clear();
+    nb_loc.assign(nb_pos.size(), -1);
+    std::vector<int> loc(npairs, -1);
+    for (int qq = 0; qq < nblocks; qq++) {
+      std::map<std::pair<int, int>, std::vector<int>> sets;
+      for (int r = pbase[qq]; r < pbase[qq + 1]; r++) {
+        const int ga = grp_of[pair_a[r]], gb = grp_of[pair_b[r]];
+        sets[{std::min(ga, gb), std::max(ga, gb)}].push_back(r);
+      }
+      for (auto &kv : sets) {
+        s122_q.push_back(qq);
+        s122_start.push_back((int)s122_rows.size());
+        for (int k = 0; k < (int)kv.second.size(); k++) {
+          loc[kv.second[k]] = k;
+          s122_rows.push_back(kv.second[k]);
+        }
+      }
+    }
+    s122_start.push_back((int)s122_rows.size());
+    for (int r = 0; r < npairs; r++) {
+      const int base = pbase[pair_q[r]];
+      for (int e = nb_start[r]; e < nb_start[r + 1]; e++) {
+        const int rn = base + nb_pos[e];
+        const int ga = grp_of[pair_a[r]], gb = grp_of[pair_b[r]], na = grp_of[pair_a[rn]], nb2 = grp_of[pair_b[rn]];
+        if (std::min(ga, gb) != std::min(na, nb2) || std::max(ga, gb) != std::max(na, nb2))
+          return "internal: 122 neighbour outside its row set";
+        nb_loc[e] = loc[rn];
+      }
+    }
+    std::vector<std::pair<int, int>> items;
+    const int nsets = (int)s122_q.size();
+    for (int st = 0; st < nsets; st++) {
+      const int rs = s122_start[st], ns = s122_start[st + 1] - rs, qq = s122_q[st];
+      // slot (k, k'): the entries of row k whose neighbour is row k', in entry order
+      std::vector<int> sx(2 * (size_t)ns * ns, 0), cnt((size_t)ns * ns, 0);
+      std::vector<double> sc(2 * (size_t)ns * ns, 0.0);
+      bool ok = ns <= 32;
+      for (int k = 0; k < ns && ok; k++) {
+        const int r = s122_rows[rs + k];
+        for (int e = nb_start[r]; e < nb_start[r + 1]; e++) {
+          const int sl = k * ns + nb_loc[e];
+          if (cnt[sl] == 2) {
+            ok = false;
+            break;
+          }
+          sx[2 * sl + cnt[sl]] = nb_x[e] * nsp + nb_i[e];
+          sc[2 * sl + cnt[sl]] = nb_coef[e];
+          cnt[sl]++;
+        }
+      }
+      if (!ok) {
+        s122_ab.push_back(-1);
+        for (int k = 0; k < ns; k++) s122_frows.push_back(s122_rows[rs + k]);
+        continue;
+      }
+      s122_ab.push_back((long long)s122_sx.size());  // two values (A, -B) per slot
+      s122_sx.insert(s122_sx.end(), sx.begin(), sx.end());
+      s122_sc.insert(s122_sc.end(), sc.begin(), sc.end());
+      for (int c0 = 0; c0 < bdim[qq]; c0 += 64) items.push_back({st, c0 >> 6});
+    }
+    std::stable_sort(items.begin(), items.end(), [&](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+      return s122_start[x.first + 1] - s122_start[x.first] > s122_start[y.first + 1] - s122_start[y.first];
+    });
+    for (auto &it : items) {
+      s122_items.push_back(it.first);
+      s122_items.push_back(it.second);
+    }
+  }
   // Hartree-Fock tables
   {
     const int ng = (int)grp_start.size();

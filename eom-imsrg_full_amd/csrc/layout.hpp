@@ -57,6 +57,20 @@ struct Layout {
   std::vector<int> nb_start;    // npairs + 1
   std::vector<int> nb_pos, nb_x, nb_i;
   std::vector<double> nb_coef;
+  // 122 row sets: the rows of one block whose two orbits come from one
+  // unordered pair of (l,j,tz) groups (the neighbours of every row of a set
+  // lie in the set).  Set s is rows s122_rows[s122_start[s] .. s122_start[s+1])
+  // (global pair indices) of block s122_q[s]; nb_loc is a neighbour entry's
+  // index inside its row's set.  Sets of at most 32 rows with at most two
+  // entries per slot (row k, neighbour row k') run in the set kernel: their
+  // slots, row-major, start at s122_ab[s] / 2 (-1 for the other sets), slot i
+  // holding f-matrix indices s122_sx[2i], [2i+1] and coefficients s122_sc
+  // (absent entries: index 0, coefficient 0); work items (set, 64-column
+  // chunk) as pairs, larger sets first.  The rows of the other sets go to
+  // the row kernel (s122_frows).
+  std::vector<int> s122_q, s122_start, s122_rows, nb_loc, s122_frows, s122_items, s122_sx;
+  std::vector<long long> s122_ab;
+  std::vector<double> s122_sc;
 
   // Pandya blocks: JC fastest, |Tz|, parity slowest
   int ncc = 0;
