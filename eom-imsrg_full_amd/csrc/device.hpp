@@ -65,7 +65,7 @@ struct DevLayout {
   int nmirror;
   long long f_off, v_off, op_size, vsize, ccsize, cwsize;  // per-instance strides
   const struct PhChunk *ph_chunks;  // k_ph_apply work items, rows in (i, j, J) order
-  XcdSplit xs_rows, xs_terms, xs_apply;  // k_122 rows (of s122_frows), k_ph_terms chunks of 1024 terms, ph_chunks
+  XcdSplit xs_rows, xs_terms, xs_apply;  // k_122 rows (s122_frows), k_ph_terms chunks of 1024 terms, ph_chunks
 };
 
 // A 122 row set: rows s122_rowl[rs .. rs + n) of block q (dimension dim,

@@ -2,7 +2,7 @@
 //
 // Operators are device buffers [E0 | F (nsp x nsp) | V (full channel blocks)]
 // described in layout.hpp.  The two-body terms of a commutator accumulate into
-// the upper triangle of each channel block (k_122 also leaves the source of
+// the upper triangle of each channel block (the 122 kernels also leave the source of
 // its transposed half in the lower triangle); the last pass, k_sym_mirror,
 // adds that half and writes lower = herm * upper, which reproduces the
 // reference's "upper-triangle then mirror" contract for square sub-blocks and
@@ -980,11 +980,12 @@ void launch_pandya(hipStream_t s, const DevLayout &L, const double *op, double *
 // A_X is the pair-space action of X's one-body part,
 //   M(r,s) = sum_{e in nb(r)} c_e [fL(x,i) R(r_e,s) - fR(x,i) L(r_e,s)],
 // the right-multiplication half equals -hL*hR*M(s,r) (hermiticity of f and
-// V), so RES(r,s) = M(r,s) + herm(RES)*M(s,r).  This kernel writes M over the
-// whole block (one wavefront per row, coalesced 16-byte row reads, wave-
-// uniform neighbour entries); the diagonal is written already symmetrised,
+// V), so RES(r,s) = M(r,s) + herm(RES)*M(s,r).  The 122 kernels write M over
+// the whole block; the diagonal is written already symmetrised,
 // M(r,r)*(1+herm).  k_sym_mirror adds the transpose at the end.
-// The row form: one wavefront per row (the rows of the sets past 32 rows).
+// The row form, for the rows of the sets the set form does not take (past 32
+// rows, or a slot with more than two entries): one wavefront per row,
+// coalesced 16-byte row reads, wave-uniform neighbour entries.
 __global__ __launch_bounds__(64) void k_122(DevLayout L, const double *__restrict__ Lop,
                                             const double *__restrict__ Rop, double *__restrict__ RES,
                                             double hres, const int *act) {
@@ -1413,7 +1414,7 @@ void launch_ph_apply(hipStream_t s, const DevLayout &L, const double *Tbuf, doub
 // ------------------------------------------------------------------ symmetrise + mirror
 // Final pass over every channel block: the upper triangle U holds M + the
 // accumulated 222 terms, the strict lower triangle still holds M^T's source
-// M(s,r) from k_122, so for r < s
+// M(s,r) from the 122 kernels, so for r < s
 //   RES(r,s) = U(r,s) + herm*lower(s,r),  RES(s,r) = herm*RES(r,s);
 // the diagonal is kept (zeroed for herm = -1).  This reproduces the
 // reference's "upper triangle then mirror x herm" contract (commutators.f90
