@@ -49,7 +49,7 @@ def parse():
     p.add_argument("--emax", type=int, default=8)
     p.add_argument("--holes", type=int, default=6, help="filled orbits per isospin")
     p.add_argument("--generator", default="white", choices=["white", "atan", "w2"])
-    p.add_argument("--precision", default="all", choices=["all", "fp64", "split32", "bf16", "int8x5"])
+    p.add_argument("--precision", default="all", choices=["all", "fp64", "split32", "bf16", "int8x5", "int8x4"])
     p.add_argument("--cpu-baseline", default="auto", choices=["auto", "reference", "port", "none"])
     p.add_argument("--cpu-reps", type=int, default=3, help="commutators in the CPU sample")
     p.add_argument("--no-concurrency", action="store_true",
@@ -60,7 +60,7 @@ def parse():
     p.add_argument("--batch-emax", type=int, default=6)
     p.add_argument("--batch-holes", type=int, default=3)
     p.add_argument("--batch8", type=int, default=4,
-                   help="instances of the batched emax=8 measurement, FP64 and int8x5 (0: skip)")
+                   help="instances of the batched emax=8 measurement, FP64, int8x5 and int8x4 (0: skip)")
     p.add_argument("--gate-steps", type=int, default=200,
                    help="at most this many lockstep steps in the precision gate (the flow converges earlier)")
     p.add_argument("--tensor-reps", type=int, default=5,
@@ -125,18 +125,19 @@ def roofline_of(cls, work, prof, head):
     ms, n = prof[cls]
     avg_s = ms / 1e3 / n
     extra = {}
-    if bound == "mfma" and head == "int8x5":
+    if bound == "mfma" and head in I8PAIRS:
         # the algorithmic FP64 flop of the product against the int8 dense peak;
         # beside it the same rate against the FP64 MFMA peak, and the int8
-        # slice products actually issued (15 per FP64 multiply-add, tile
-        # padding excluded) against the int8 peak
+        # slice products actually issued (15 per FP64 multiply-add with five
+        # slices, 10 with four; tile padding excluded) against the int8 peak
+        npair = I8PAIRS[head]
         achieved = per_launch / avg_s / 1e12
         peak = PEAK["int8_mfma_tops"]
         unit = "TOP/s"
         extra = {"fp64_equivalent": {"achieved_tflops": achieved, "peak_tflops": PEAK["fp64_mfma_tflops"],
                                      "frac": achieved / PEAK["fp64_mfma_tflops"]},
-                 "int8_slice_ops": {"per_launch": 15 * per_launch, "achieved_tops": 15 * achieved,
-                                    "frac_of_int8_peak": 15 * achieved / PEAK["int8_mfma_tops"]}}
+                 "int8_slice_ops": {"per_launch": npair * per_launch, "achieved_tops": npair * achieved,
+                                    "frac_of_int8_peak": npair * achieved / PEAK["int8_mfma_tops"]}}
     elif bound == "mfma":
         achieved = per_launch / avg_s / 1e12
         peak = {"fp64": PEAK["fp64_mfma_tflops"], "split32": PEAK["fp32_mfma_tflops"] / 3,
@@ -229,9 +230,11 @@ KERNEL_NAME = {
     "gemm_pphh": "k_seg_gemm<false, 0, 0>", "gemm_ph": "k_seg_gemm<true, 0, 1>", "pandya": "k_pandya(",
     "k122": "k_122s(", "ph_terms": "k_ph_terms(", "ph_apply": "k_ph_apply(", "sym_mirror": "k_sym_mirror(",
     "onebody": "k_ob_partial(",
-    # int8x5 products and slicing
-    "gemm_pphh_i8": "k_seg_gemm_i8", "gemm_ph_i8": "k_zprod_i8", "slice_v": "k_slice_v(",
-    "slice_cc": "k_slice_cc(",
+    # int8 products and slicing, five slices (int8x5) and four (int8x4)
+    "gemm_pphh_i8": "k_seg_gemm_i8<5>", "gemm_ph_i8": "k_zprod_i8<5>", "slice_v": "k_slice_v<5>(",
+    "slice_cc": "k_slice_cc<5>(",
+    "gemm_pphh_i8x4": "k_seg_gemm_i8<4>", "gemm_ph_i8x4": "k_zprod_i8<4>", "slice_v_x4": "k_slice_v<4>(",
+    "slice_cc_x4": "k_slice_cc<4>(",
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_emax8.txt")
 
@@ -313,17 +316,20 @@ def load_traffic(head):
         f = line.split()
         if len(f) == 4 and not line.startswith("#") and f[0] in KERNEL_NAME:
             out[f[0]] = float(f[3])
-    if head == "int8x5":
+    if head in I8PAIRS:
+        x = "_x4" if head == "int8x4" else ""
         for cls in ("gemm_pphh", "gemm_ph"):
-            out[cls] = out.get(cls + "_i8")
-        if "slice_v" in out and "slice_cc" in out:
-            out["slice"] = (out["slice_v"] + out["slice_cc"]) / 2
+            out[cls] = out.get(cls + ("_i8x4" if x else "_i8"))
+        if "slice_v" + x in out and "slice_cc" + x in out:
+            out["slice"] = (out["slice_v" + x] + out["slice_cc" + x]) / 2
     return out
 
 
-PRECS = {"fp64": 0, "split32": 1, "bf16": 2, "int8x5": 3}
+PRECS = {"fp64": 0, "split32": 1, "bf16": 2, "int8x5": 3, "int8x4": 4}
 DTYPE = {"fp64": "f64", "split32": "split-f32 (FP64 accum)", "bf16": "split-bf16x3 (FP64 accum)",
-         "int8x5": "int8x5 slices (Ozaki scheme, exact int32, FP64 combine)"}
+         "int8x5": "int8x5 slices (Ozaki scheme, exact int32, FP64 combine)",
+         "int8x4": "int8x4 slices (Ozaki scheme, exact int32, FP64 combine)"}
+I8PAIRS = {"int8x5": 15, "int8x4": 10}  # int8 slice products per FP64 multiply-add
 
 
 def timed_flow(ctx, E, H0vec, a, prec, classes):
@@ -559,14 +565,14 @@ def batch8_bench(a, E, basis, H0vec, single_fp64_steps_per_s):
     each one the benchmark Hamiltonian (seed 0, normal-ordered), flowed as
     independent instances on one batched context (every instance computes its
     whole flow; other seeds at emax=8 stop the reference's BCH_EXPAND with
-    'clipping BCH'), FP64 and int8x5, W untimed + K timed steps;
+    'clipping BCH'), FP64, int8x5 and int8x4, W untimed + K timed steps;
     instance-steps/s against the single-instance FP64 flow."""
     nb = a.batch8
     vecs = [H0vec] * nb
     bctx = E.Context(basis, nbatch=nb)
     out = {"workload": f"{nb} instances of the emax={a.emax} benchmark flow (seed 0, hw 20), {a.generator} "
                        "generator, each computed in full", "instances": nb}
-    for prec in ("fp64", "int8x5"):
+    for prec in ("fp64", "int8x5", "int8x4"):
         bctx.set_precision(PRECS[prec])
         bH = bctx.op(1, np.stack(vecs))
         f = bctx.flow(bH, a.generator, 0.5)
@@ -621,7 +627,7 @@ def main():
     H.normal_order()
     H0vec = H.download()
     eta_vec = ctx.generator(a.generator, H).download() if a.cpu_baseline != "none" else None
-    precs = ["fp64", "split32", "bf16", "int8x5"] if a.precision == "all" else [a.precision]
+    precs = ["fp64", "split32", "bf16", "int8x5", "int8x4"] if a.precision == "all" else [a.precision]
     checks = {}
     for prec in precs:
         if prec == "fp64":
@@ -670,7 +676,7 @@ def main():
     # of every class with a work model (from the fully profiled extra step)
     dom = max((k for k in work if prof[k][1]), key=lambda k: prof[k][0])
     roof = roofline_of(dom, work, prof, head)
-    traffic = load_traffic(head) if (head in ("fp64", "int8x5") and a.emax == 8 and a.holes == 6) else {}
+    traffic = load_traffic(head) if (head in ("fp64", "int8x5", "int8x4") and a.emax == 8 and a.holes == 6) else {}
     roof["traffic"] = traffic.get(dom)
     if breakdown[dom][1]:
         iso = roofline_of(dom, work, breakdown, head)

@@ -120,7 +120,10 @@ struct SegProb {
 // byte ((g * ld / 32 + s) * 5 + i) * 1024 of the block (lane l of a matrix
 // instruction reads the 16 bytes at 16 l: row 32 g + l % 32, columns
 // 32 s + 16 (l / 32) .. +15).  A product keeps the slice pairs with
-// i + j <= 6 (fifteen int8 products, exact int32 sums per level i + j).
+// i + j <= 6 (fifteen int8 products, exact int32 sums per level i + j).  The
+// four-plane form (IMSRG_PREC_INT8X4) keeps slices 1..4, laid out the same
+// way with 4 in place of 5 (each block in the first four fifths of its
+// region), and the ten pairs i + j <= 5.
 constexpr int kI8Planes = 5;
 struct I8Seg {
   int kofs, K, src;  // K window of the segment in the block's rows (multiples of 64)
@@ -293,24 +296,27 @@ void launch_adj_generator(hipStream_t s, const DevLayout &L, const double *H, co
 void launch_adj_normal_order(hipStream_t s, const DevLayout &L, const double *Cb, double *Hb, int herm);
 
 // int8 slices (ozaki.hip): the pp/hh columns of every V row of `op`, and the
-// rows of the Pandya matrices `cc`, into 4 planes + row scales
-void launch_slice_v(hipStream_t s, const DevLayout &L, const I8Layout &I, const double *op, signed char *planes,
-                    double *scale, const int *act, int nact);
-void launch_slice_cc(hipStream_t s, const DevLayout &L, const I8Layout &I, const double *cc, double *pmax,
+// rows of the Pandya matrices `cc`, into ns planes (5, or 4) + row scales;
+// the planes keep the offsets of a kI8Planes-plane layout
+void launch_slice_v(hipStream_t s, const DevLayout &L, const I8Layout &I, int ns, const double *op,
+                    signed char *planes, double *scale, const int *act, int nact);
+void launch_slice_cc(hipStream_t s, const DevLayout &L, const I8Layout &I, int ns, const double *cc, double *pmax,
                      signed char *planes, double *scale, const int *act, int nact);
 // int8-sliced 222_pp_hh GEMM on the 64 x 64 tile lists of the FP64 path
-// (C += upper, W after nmid).  Operand 0 / 1 slices P0 / P1, row scales
-// S0 / S1; segment factor sign * (src ? g1 : g0).
-void launch_seg_gemm_i8(hipStream_t s, const I8Prob *probs, const int4 *tiles, int ntiles, const signed char *P0,
-                        const signed char *P1, const double *S0, const double *S1, double g0, double g1, double *C,
-                        double *W, const int *act, int nact, long long sp, long long ss, long long sc, long long sw);
+// (C += upper, W after nmid).  Operand 0 / 1 slices P0 / P1 (ns planes), row
+// scales S0 / S1; segment factor sign * (src ? g1 : g0).
+void launch_seg_gemm_i8(hipStream_t s, int ns, const I8Prob *probs, const int4 *tiles, int ntiles,
+                        const signed char *P0, const signed char *P1, const double *S0, const double *S1, double g0,
+                        double g1, double *C, double *W, const int *act, int nact, long long sp, long long ss,
+                        long long sc, long long sw);
 // int8-sliced Z product of the ph channel (2x2-blocked upper Z) on the
-// fragment-ordered Pandya planes, over a list of 32 x 32 upper tiles (x =
-// problem, y / z = row / column group; x = -1 a padding slot), four per
-// workgroup.
-void launch_zprod_i8(hipStream_t s, const I8Prob *probs, const int4 *tiles, int ntiles, const signed char *P0,
-                     const signed char *P1, const double *S0, const double *S1, double g0, double g1, double *Z,
-                     double sigma, const int *act, int nact, long long sp, long long ss, long long sz);
+// fragment-ordered Pandya planes (ns planes), over a list of 32 x 32 upper
+// tiles (x = problem, y / z = row / column group; x = -1 a padding slot),
+// four per workgroup.
+void launch_zprod_i8(hipStream_t s, int ns, const I8Prob *probs, const int4 *tiles, int ntiles,
+                     const signed char *P0, const signed char *P1, const double *S0, const double *S1, double g0,
+                     double g1, double *Z, double sigma, const int *act, int nact, long long sp, long long ss,
+                     long long sz);
 
 constexpr int kPartials = 1024;  // reduction partials per instance
 constexpr int kE0Split = 8;      // E0 (220) workgroups per channel block

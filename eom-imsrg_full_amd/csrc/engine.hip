@@ -747,9 +747,9 @@ static int i8_setup(imsrg_ctx_s *c) {
 int imsrg_ctx_set_precision(imsrg_ctx c, int prec) {
   if (!c) return fail(IMSRG_EINVAL, "null context");
   if (prec != IMSRG_PREC_FP64 && prec != IMSRG_PREC_SPLIT_FP32 && prec != IMSRG_PREC_SPLIT_BF16 &&
-      prec != IMSRG_PREC_INT8X5)
+      prec != IMSRG_PREC_INT8X5 && prec != IMSRG_PREC_INT8X4)
     return fail(IMSRG_EINVAL, "unknown precision");
-  if (prec == IMSRG_PREC_INT8X5 && !c->i8_ready) {
+  if ((prec == IMSRG_PREC_INT8X5 || prec == IMSRG_PREC_INT8X4) && !c->i8_ready) {
     int rc = i8_setup(c);
     if (rc) return rc;
   }
@@ -1237,15 +1237,16 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
     PROFS(c, IMSRG_PROF_E0, s2, launch_e0(s2, D, L->d, R->d, RES->d, hL, hR, mask, c->partials, ad, na, OS));
   if (mask & T122) PROFS(c, IMSRG_PROF_122, s2, launch_122(s2, D, L->d, R->d, RES->d, hres, ad, na, c->w122));
   else launch_clear_blocks(s2, D, RES->d, ad, na);
-  const bool i8 = c->prec == IMSRG_PREC_INT8X5;
+  const int ns = i8_planes(c->prec);
+  const bool i8 = ns > 0;
   if ((mask & (T222PPHH | T221)) && i8) {
     PROFS(c, IMSRG_PROF_SLICE, s2, {
-      if (!lprep) launch_slice_v(s2, D, c->i8, L->d, c->i8v[0], c->i8vs[0], ad, na);
-      launch_slice_v(s2, D, c->i8, R->d, c->i8v[1], c->i8vs[1], ad, na);
+      if (!lprep) launch_slice_v(s2, D, c->i8, ns, L->d, c->i8v[0], c->i8vs[0], ad, na);
+      launch_slice_v(s2, D, c->i8, ns, R->d, c->i8v[1], c->i8vs[1], ad, na);
     });
     // B rows are rows of the other operand: X[k][n] = herm * X[n][k]
     PROFK(c, IMSRG_PROF_GEMM_PPHH,
-          launch_seg_gemm_i8(s2, c->pp_i8, c->pp_tiles, c->n_pp_tiles, c->i8v[0], c->i8v[1], c->i8vs[0],
+          launch_seg_gemm_i8(s2, ns, c->pp_i8, c->pp_tiles, c->n_pp_tiles, c->i8v[0], c->i8v[1], c->i8vs[0],
                              c->i8vs[1], (double)hR, (double)hL, (mask & T222PPHH) ? RES->d + D.v_off : nullptr,
                              (mask & T221) ? c->wpp : nullptr, ad, na, c->i8.v_bytes, c->i8.v_sr, OS, D.vsize));
   } else if (mask & (T222PPHH | T221)) {
@@ -1263,11 +1264,11 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
     if (!rcc) PROF(c, IMSRG_PROF_PANDYA, launch_pandya(s, D, R->d, c->cc[1], ad, na));
     if (i8) {
       PROF(c, IMSRG_PROF_SLICE, {
-        if (!lprep) launch_slice_cc(s, D, c->i8, c->cc[0], c->i8pm, c->i8c[0], c->i8cs[0], ad, na);
-        launch_slice_cc(s, D, c->i8, c->cc[1], c->i8pm, c->i8c[1], c->i8cs[1], ad, na);
+        if (!lprep) launch_slice_cc(s, D, c->i8, ns, c->cc[0], c->i8pm, c->i8c[0], c->i8cs[0], ad, na);
+        launch_slice_cc(s, D, c->i8, ns, c->cc[1], c->i8pm, c->i8c[1], c->i8cs[1], ad, na);
       });
       PROFK(c, IMSRG_PROF_GEMM_PH,
-            launch_zprod_i8(s, c->ph_i8, c->ph_i8_tiles, c->n_ph_i8_tiles, c->i8c[0], c->i8c[1], c->i8cs[0],
+            launch_zprod_i8(s, ns, c->ph_i8, c->ph_i8_tiles, c->n_ph_i8_tiles, c->i8c[0], c->i8c[1], c->i8cs[0],
                             c->i8cs[1], (double)hL, -(double)hR, c->wph, -(double)hL * (double)hR, ad, na,
                             c->i8.c_bytes, c->i8.c_sr, D.cwsize));
     } else {
@@ -1491,9 +1492,9 @@ static ActSet series_set(imsrg_ctx_s *c, int slot, const ActSet &a) {
 // those matrices.
 static void prep_L(imsrg_ctx_s *c, imsrg_op_s *G, const ActSet &set) {
   launch_pandya(c->stream, c->dl, G->d, c->cc[0], set.d, set.n());
-  if (c->prec == IMSRG_PREC_INT8X5) {
-    launch_slice_v(c->stream, c->dl, c->i8, G->d, c->i8v[0], c->i8vs[0], set.d, set.n());
-    launch_slice_cc(c->stream, c->dl, c->i8, c->cc[0], c->i8pm, c->i8c[0], c->i8cs[0], set.d, set.n());
+  if (const int ns = i8_planes(c->prec)) {
+    launch_slice_v(c->stream, c->dl, c->i8, ns, G->d, c->i8v[0], c->i8vs[0], set.d, set.n());
+    launch_slice_cc(c->stream, c->dl, c->i8, ns, c->cc[0], c->i8pm, c->i8c[0], c->i8cs[0], set.d, set.n());
   }
 }
 // The series gate after nested commutator n: the test on the device, its

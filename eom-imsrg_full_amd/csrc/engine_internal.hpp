@@ -83,7 +83,7 @@ struct imsrg_ctx_s {
   int shape_pp = -1, shape_ph = -1;
   float tune_ms[4] = {0, 0, 0, 0};                      // pp 64, pp 128x64, ph 64, ph 128x64
   int prec = IMSRG_PREC_FP64;
-  // int8-sliced GEMM operands (IMSRG_PREC_INT8X5), built on first use:
+  // int8-sliced GEMM operands (IMSRG_PREC_INT8X5 / INT8X4), built on first use:
   // planes and row scales of operand 0 (L) and 1 (R), V rows and Pandya rows
   bool i8_ready = false;
   I8Layout i8{};
@@ -135,6 +135,10 @@ struct imsrg_op_s {
 };
 
 // ---------------------------------------------------------------- helpers
+// int8 slices per element of a precision (0: not an int8 precision)
+inline int i8_planes(int prec) {
+  return prec == IMSRG_PREC_INT8X5 ? 5 : prec == IMSRG_PREC_INT8X4 ? 4 : 0;
+}
 template <class T>
 inline int upload_table(imsrg_ctx_s *c, const std::vector<T> &v, const T **out) {
   size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);

@@ -1,17 +1,18 @@
 // ozaki.hip -- the 222 GEMMs on the int8 matrix cores of gfx950 (the Ozaki
 // scheme: error-free slicing into int8 planes, exact int32 products).
 //
-// Every operand row r is held as x = s_r * sum_{i=1..5} a_i 2^(-7i): s_r is a
-// power of two with max|row| / s_r in [1/4, 1/2), and a_i in [-64, 64] are
-// produced by rounding x / s_r * 2^7 to nearest and carrying the remainder
-// (each step exact in FP64; rounding rather than truncation keeps the
-// dropped remainders unbiased, which matters for sums of same-sign terms).
-// A dot product of two rows is then
-//   s_A s_B sum_{L=2..6} 2^(-7L) sum_{i+j=L} sum_k a_i[k] b_j[k],
-// with the inner sums exact in int32 on v_mfma_i32_16x16x64_i8 (|a_i b_j| <=
-// 2^12, at most five pairs per level and K < 2^17); the slice pairs with
-// i + j > 6 are dropped, so the error is about 2^-36 of max|A row| max|B row|
-// per product, against 2^-24 of each K-slab partial sum for split FP32.
+// Every operand row r is held as x = s_r * sum_{i=1..NS} a_i 2^(-7i) (NS = 5
+// slices, or 4): s_r is a power of two with max|row| / s_r in [1/4, 1/2), and
+// a_i in [-64, 64] are produced by rounding x / s_r * 2^7 to nearest and
+// carrying the remainder (each step exact in FP64; rounding rather than
+// truncation keeps the dropped remainders unbiased, which matters for sums of
+// same-sign terms).  A dot product of two rows is then
+//   s_A s_B sum_{L=2..NS+1} 2^(-7L) sum_{i+j=L} sum_k a_i[k] b_j[k],
+// with the inner sums exact in int32 on v_mfma_i32_32x32x32_i8 (|a_i b_j| <=
+// 2^12, at most NS pairs per level and K < 2^17); the slice pairs with
+// i + j > NS + 1 are dropped, so the error is about 2^-36 (NS = 5; 2^-29 for
+// NS = 4) of max|A row| max|B row| per product, against 2^-24 of each K-slab
+// partial sum for split FP32.
 // The levels are combined and scaled in FP64, segment by segment, into the
 // same FP64 accumulators and epilogues (222_pp_hh C += upper, the 221 W
 // intermediate, the 2x2-blocked Z) as the FP64 path.
@@ -25,9 +26,10 @@
 namespace imsrg {
 
 typedef int v4i_t __attribute__((ext_vector_type(4)));
-constexpr int NS = kI8Planes;  // slices per element
+// NS (a template argument): slices per element, 5 or 4
 
 // the NS rounded slices of x / 2^ex (|x| < 2^ex / 2), as bytes
+template <int NS>
 __device__ __forceinline__ void slices_of(double x, int ex, unsigned b[NS]) {
   double t = ldexp(x, 7 - ex);
 #pragma unroll
@@ -52,6 +54,7 @@ __device__ __forceinline__ int row_exp(double m) {
 __device__ __forceinline__ int v_col(int j, int np, int nb, int nh, int P) {
   return j < np ? j : (j >= P && j < P + nh) ? np + nb + (j - P) : -1;
 }
+template <int NS>
 __global__ __launch_bounds__(256) void k_slice_v(DevLayout L, I8Layout I, const double *__restrict__ op,
                                                  signed char *__restrict__ planes, double *__restrict__ scale,
                                                  const int *act) {
@@ -96,7 +99,7 @@ __global__ __launch_bounds__(256) void k_slice_v(DevLayout L, I8Layout I, const 
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       unsigned b[NS] = {};
-      if (m > 0.0) slices_of(x[u], ex, b);
+      if (m > 0.0) slices_of<NS>(x[u], ex, b);
 #pragma unroll
       for (int i = 0; i < NS; i++) w[i] |= b[i] << (8 * u);
     }
@@ -123,6 +126,7 @@ __global__ __launch_bounds__(256) void k_slice_v(DevLayout L, I8Layout I, const 
 // of 32; for group g, K step s (32 columns, nks = ld / 32 per row) and slice
 // i, 1 KB holds lane l's operand of one v_mfma_i32_32x32x32_i8, the 16 bytes
 // of row 32 g + l % 32, columns 32 s + 16 (l / 32) .. +15, at 16 l.
+template <int NS>
 __device__ __forceinline__ long long frag_off(int g, int s, int i, int nks) {
   return ((long long)(g * nks + s) * NS + i) * 1024;
 }
@@ -157,6 +161,7 @@ __global__ __launch_bounds__(256) void k_rowmax_cc(DevLayout L, I8Layout I, cons
   if (kq == 0)
     pmax[it.w + (k0 >> 8) * 64 + rloc] = fmax(fmax(red[0][rloc], red[1][rloc]), fmax(red[2][rloc], red[3][rloc]));
 }
+template <int NS>
 __global__ __launch_bounds__(256) void k_slice_cc(DevLayout L, I8Layout I, const double *__restrict__ cc,
                                                   const double *__restrict__ pmax, signed char *__restrict__ planes,
                                                   double *__restrict__ scale, const int *act) {
@@ -195,7 +200,7 @@ __global__ __launch_bounds__(256) void k_slice_cc(DevLayout L, I8Layout I, const
       for (int u = 0; u < 4; u++) {
         const int k = kb + 4 * g + u;
         unsigned b[NS] = {};
-        if (row < R && k < K && m > 0.0) slices_of(x[4 * g + u], ex, b);
+        if (row < R && k < K && m > 0.0) slices_of<NS>(x[4 * g + u], ex, b);
 #pragma unroll
         for (int i = 0; i < NS; i++) w[i][g] |= b[i] << (8 * u);
       }
@@ -203,21 +208,23 @@ __global__ __launch_bounds__(256) void k_slice_cc(DevLayout L, I8Layout I, const
 #pragma unroll
     for (int i = 0; i < NS; i++) {
       const v4i_t v = {(int)w[i][0], (int)w[i][1], (int)w[i][2], (int)w[i][3]};
-      *(v4i_t *)(dst + frag_off(row >> 5, kb >> 5, i, nks) + (((row & 31) + 32 * ((kb >> 4) & 1)) << 4)) = v;
+      *(v4i_t *)(dst + frag_off<NS>(row >> 5, kb >> 5, i, nks) + (((row & 31) + 32 * ((kb >> 4) & 1)) << 4)) = v;
     }
   }
 }
 
-void launch_slice_v(hipStream_t s, const DevLayout &L, const I8Layout &I, const double *op, signed char *planes,
-                    double *scale, const int *act, int nact) {
+void launch_slice_v(hipStream_t s, const DevLayout &L, const I8Layout &I, int ns, const double *op,
+                    signed char *planes, double *scale, const int *act, int nact) {
   if (nact <= 0 || L.npairs <= 0) return;
-  hipLaunchKernelGGL(k_slice_v, dim3((L.npairs + 3) / 4, nact), dim3(256), 0, s, L, I, op, planes, scale, act);
+  hipLaunchKernelGGL(ns == 4 ? k_slice_v<4> : k_slice_v<5>, dim3((L.npairs + 3) / 4, nact), dim3(256), 0, s, L, I,
+                     op, planes, scale, act);
 }
-void launch_slice_cc(hipStream_t s, const DevLayout &L, const I8Layout &I, const double *cc, double *pmax,
+void launch_slice_cc(hipStream_t s, const DevLayout &L, const I8Layout &I, int ns, const double *cc, double *pmax,
                      signed char *planes, double *scale, const int *act, int nact) {
   if (nact <= 0 || I.n_items <= 0) return;
   hipLaunchKernelGGL(k_rowmax_cc, dim3(I.n_items, nact), dim3(256), 0, s, L, I, cc, pmax, act);
-  hipLaunchKernelGGL(k_slice_cc, dim3(I.n_items, nact), dim3(256), 0, s, L, I, cc, pmax, planes, scale, act);
+  hipLaunchKernelGGL(ns == 4 ? k_slice_cc<4> : k_slice_cc<5>, dim3(I.n_items, nact), dim3(256), 0, s, L, I, cc, pmax,
+                     planes, scale, act);
 }
 
 // ---- the 222_pp_hh product (C += upper, the 221 intermediate W after
@@ -230,6 +237,7 @@ void launch_slice_cc(hipStream_t s, const DevLayout &L, const I8Layout &I, const
 // form issues at half its rate).  Fragments: lane l holds row l % 32 and the
 // K bytes 16 (l / 32) .. +15 of a 32-byte K step, for A and B alike; result
 // element v of lane l is (8 (v / 4) + 4 (l / 32) + v % 4, l % 32).
+template <int NS>
 struct I8Smem {
   v4i_t a[NS][64][4];
   v4i_t b[NS][64][4];
@@ -238,6 +246,7 @@ typedef int v16i_t __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ int swz(int r, int g) { return g ^ ((r >> 2) & 3); }
 
+template <int NS>
 __global__ __launch_bounds__(256, 2) void k_seg_gemm_i8(const I8Prob *__restrict__ probs,
                                                         const int4 *__restrict__ tiles, int ntiles,
                                                         const signed char *__restrict__ P0,
@@ -261,7 +270,7 @@ __global__ __launch_bounds__(256, 2) void k_seg_gemm_i8(const I8Prob *__restrict
   const int4 tl = tiles[t];
   if (tl.x < 0) return;  // padding slot of the XCD-balanced schedule
   const I8Prob &p = probs[tl.x];
-  __shared__ I8Smem S2[2];  // double-buffered chunks: 80 KB, two workgroups fill a CU's 160 KB
+  __shared__ I8Smem<NS> S2[2];  // double-buffered chunks: 80 KB, two workgroups fill a CU's 160 KB
   const int M = p.M, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int m0 = tl.y * 64, n0 = tl.z * 64, wm = wave >> 1, wn = wave & 1;
   const int ra = m0 + wm * 32, cb = n0 + wn * 32;
@@ -307,7 +316,7 @@ __global__ __launch_bounds__(256, 2) void k_seg_gemm_i8(const I8Prob *__restrict
       for (int l = 0; l < NS; l++)
 #pragma unroll
         for (int v = 0; v < 16; v++) lv[l][v] = 0;
-      auto store = [&](I8Smem &S) {
+      auto store = [&](I8Smem<NS> &S) {
 #pragma unroll
         for (int q = 0; q < NS; q++) {
           (&S.a[0][0][0])[sdst[q]] = R[0][q];
@@ -323,7 +332,7 @@ __global__ __launch_bounds__(256, 2) void k_seg_gemm_i8(const I8Prob *__restrict
       __syncthreads();
       int buf = 0;
       for (int kc = 0; kc < K; kc += 64) {  // K is workgroup-uniform: every barrier is reached by all
-        const I8Smem &S = S2[buf];
+        const I8Smem<NS> &S = S2[buf];
         if (active) {
           const int nh = kc + 32 < sgm.K32 ? 2 : 1;  // workgroup-uniform
 #pragma unroll
@@ -394,12 +403,13 @@ __global__ __launch_bounds__(256, 2) void k_seg_gemm_i8(const I8Prob *__restrict
   }
 }
 
-void launch_seg_gemm_i8(hipStream_t s, const I8Prob *probs, const int4 *tiles, int ntiles, const signed char *P0,
-                        const signed char *P1, const double *S0, const double *S1, double g0, double g1, double *C,
-                        double *W, const int *act, int nact, long long sp, long long ss, long long sc, long long sw) {
+void launch_seg_gemm_i8(hipStream_t s, int ns, const I8Prob *probs, const int4 *tiles, int ntiles,
+                        const signed char *P0, const signed char *P1, const double *S0, const double *S1, double g0,
+                        double g1, double *C, double *W, const int *act, int nact, long long sp, long long ss,
+                        long long sc, long long sw) {
   if (ntiles <= 0 || nact <= 0) return;
-  launch_k(k_seg_gemm_i8, dim3(ntiles, nact), dim3(256), 0, s, probs, tiles, ntiles, P0, P1, S0, S1, g0, g1, C, W,
-           act, sp, ss, sc, sw);
+  launch_k(ns == 4 ? k_seg_gemm_i8<4> : k_seg_gemm_i8<5>, dim3(ntiles, nact), dim3(256), 0, s, probs, tiles, ntiles,
+           P0, P1, S0, S1, g0, g1, C, W, act, sp, ss, sc, sw);
 }
 
 // ---- the Z product of the ph channel over each Pandya block,
@@ -420,6 +430,7 @@ void launch_seg_gemm_i8(hipStream_t s, const I8Prob *probs, const int4 *tiles, i
 // half of a 2x2 Z block (the even lane Z(r,c), Z(r+1,c+1), the odd lane
 // Z(r,c+1), Z(r+1,c); see zidx), and the 32 lanes of a row pair write 512
 // contiguous bytes.
+template <int NS>
 __global__ __launch_bounds__(256, 2) void k_zprod_i8(const I8Prob *__restrict__ probs, const int4 *__restrict__ tiles,
                                                   int ntiles, const signed char *__restrict__ P0,
                                                   const signed char *__restrict__ P1, const double *__restrict__ S0,
@@ -456,8 +467,8 @@ __global__ __launch_bounds__(256, 2) void k_zprod_i8(const I8Prob *__restrict__ 
     sy = fh ? sp0[cb + fr] : sp1[cb + fr];  // B rows: R (segment 0), L (segment 1)
   }
   const unsigned lo = 16u * (unsigned)lane;
-  const signed char *A0 = P0 + p.p_off + frag_off(tl.y, 0, 0, nks), *B0 = P1 + p.p_off + frag_off(tl.z, 0, 0, nks);
-  const signed char *A1 = P1 + p.p_off + frag_off(tl.y, 0, 0, nks), *B1 = P0 + p.p_off + frag_off(tl.z, 0, 0, nks);
+  const signed char *A0 = P0 + p.p_off + frag_off<NS>(tl.y, 0, 0, nks), *B0 = P1 + p.p_off + frag_off<NS>(tl.z, 0, 0, nks);
+  const signed char *A1 = P1 + p.p_off + frag_off<NS>(tl.y, 0, 0, nks), *B1 = P0 + p.p_off + frag_off<NS>(tl.z, 0, 0, nks);
   v16i_t lv[NS];
   double acc[16];
   v4i_t fa[2][NS], fb[2][NS];  // two fragment sets (three spill at two wavefronts per SIMD)
@@ -535,12 +546,13 @@ __global__ __launch_bounds__(256, 2) void k_zprod_i8(const I8Prob *__restrict__ 
   }
 }
 
-void launch_zprod_i8(hipStream_t s, const I8Prob *probs, const int4 *tiles, int ntiles, const signed char *P0,
-                     const signed char *P1, const double *S0, const double *S1, double g0, double g1, double *Z,
-                     double sigma, const int *act, int nact, long long sp, long long ss, long long sz) {
+void launch_zprod_i8(hipStream_t s, int ns, const I8Prob *probs, const int4 *tiles, int ntiles,
+                     const signed char *P0, const signed char *P1, const double *S0, const double *S1, double g0,
+                     double g1, double *Z, double sigma, const int *act, int nact, long long sp, long long ss,
+                     long long sz) {
   if (ntiles <= 0 || nact <= 0) return;
-  launch_k(k_zprod_i8, dim3((ntiles + 3) / 4, nact), dim3(256), 0, s, probs, tiles, ntiles, P0, P1, S0, S1, g0, g1,
-           Z, sigma, act, sp, ss, sz);
+  launch_k(ns == 4 ? k_zprod_i8<4> : k_zprod_i8<5>, dim3((ntiles + 3) / 4, nact), dim3(256), 0, s, probs, tiles,
+           ntiles, P0, P1, S0, S1, g0, g1, Z, sigma, act, sp, ss, sz);
 }
 
 }  // namespace imsrg

@@ -25,7 +25,7 @@ TALL = 0xFF
 TS_TERMS = {"111": 1 << 0, "121": 1 << 1, "211": 1 << 2, "122": 1 << 3, "212": 1 << 4,
             "222pphh": 1 << 5, "221": 1 << 6, "222ph": 1 << 7}
 TSALL = 0xFF
-PREC_FP64, PREC_SPLIT_FP32, PREC_SPLIT_BF16, PREC_INT8X5 = 0, 1, 2, 3
+PREC_FP64, PREC_SPLIT_FP32, PREC_SPLIT_BF16, PREC_INT8X5, PREC_INT8X4 = 0, 1, 2, 3, 4
 GEN = {"white": 0, "atan": 1, "w2": 2}
 
 

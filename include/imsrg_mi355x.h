@@ -56,6 +56,8 @@ typedef struct imsrg_op_s *imsrg_op;
 #define IMSRG_PREC_INT8X5 3       /* x = s_row * sum_{i<=5} a_i 2^-7i in rounded int8 slices (Ozaki scheme);
                                      the 15 slice products with i+j <= 6 on int8 MFMA, exact int32 sums,
                                      FP64 combine */
+#define IMSRG_PREC_INT8X4 4       /* the same with four slices and the 10 products with i+j <= 5
+                                     (about 2^-29 of max|A row| max|B row| per product) */
 
 /* generator kinds (generators.f90) */
 #define IMSRG_GEN_WHITE 0         /* build_gs_white  generators.f90:4-106   */
@@ -379,7 +381,7 @@ int imsrg_tensor_commutator_221(imsrg_tensor W1, imsrg_tensor W2, int pm, imsrg_
 #define IMSRG_PROF_NORM 9
 #define IMSRG_PROF_AXPBY 10
 #define IMSRG_PROF_GENERATOR 11
-#define IMSRG_PROF_SLICE 12      /* int8 slicing of the GEMM operands (IMSRG_PREC_INT8X5) */
+#define IMSRG_PROF_SLICE 12      /* int8 slicing of the GEMM operands (IMSRG_PREC_INT8X5 / INT8X4) */
 #define IMSRG_PROF_NCLASS 16
 int imsrg_ctx_profile(imsrg_ctx ctx, int enable);
 /* bracket only the classes in mask (bit k = class k; 0 turns profiling off):

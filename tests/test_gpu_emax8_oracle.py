@@ -9,7 +9,7 @@ size the benchmark times.
   benchmark's normal-ordered Hamiltonian: E0 per step to 1e-10 relative, the
   same MAGNUS_EXPAND and BCH_EXPAND term counts, and H(s), Omega(s) after the
   second step to 1e-10;
-* the reduced-precision 222 products (int8x5, split-FP32, split-BF16) on the
+* the reduced-precision 222 products (int8x5, int8x4, split-FP32, split-BF16) on the
   same two steps: |dE0| per step against the oracle's FP64 steps below
   1e-6 MeV and the FP64 term counts;
 * the sketches (norm, four Gaussian projections, 256 sampled entries) of the
@@ -111,7 +111,7 @@ def test_emax8_two_magnus_steps_fp64(w8, oracle_steps):
     assert_close(g, r["G"].vec(), what="emax8 Omega(s) after two steps")
 
 
-@pytest.mark.parametrize("prec,name", [(3, "int8x5"), (1, "split32"), (2, "bf16")])
+@pytest.mark.parametrize("prec,name", [(3, "int8x5"), (4, "int8x4"), (1, "split32"), (2, "bf16")])
 def test_emax8_reduced_precision_steps(w8, oracle_steps, prec, name):
     B, ctx, ob, E = w8
     h0, r = oracle_steps
