@@ -102,8 +102,9 @@ def kernel_work(info, basis):
     z = z_upper_bytes(basis)                      # Z, the upper 2x2 blocks the product writes
     t = info["nphterm"] * 8.0                     # ph recoupling J' terms
     # int8 slicing: the pp/hh columns of one operator's V rows, and one Pandya
-    # matrix, read in FP64 and written as 5 int8 planes; one span of each per
-    # commutator, so a launch averages the two
+    # matrix, read in FP64 and written as 5 int8 planes (4 for int8x4, scaled
+    # in roofline_of); one span of each per commutator, so a launch averages
+    # the two
     from imsrg_mi355x.basis import pandya_blocks
     vpp = sum(b.dim * (b.npp + b.nhh) for b in basis.blocks)
     ccel = sum(r * nb for (_, r, nb) in pandya_blocks(basis))
@@ -122,6 +123,8 @@ def kernel_work(info, basis):
 
 def roofline_of(cls, work, prof, head):
     bound, per_launch = work[cls]
+    if cls == "slice" and head == "int8x4":
+        per_launch *= (8.0 + 4.0) / (8.0 + 5.0)
     ms, n = prof[cls]
     avg_s = ms / 1e3 / n
     extra = {}
