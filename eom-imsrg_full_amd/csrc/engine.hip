@@ -1222,10 +1222,11 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
     c->fork_mark = nullptr;
     return 0;
   }
-  // The pp/hh chain (E0, 122 or clear, pp/hh GEMM, one-body partials) and the
-  // ph chain (Pandya, Z GEMM, J' terms) touch disjoint outputs and scratch,
-  // so they run on two streams; both end before the ph apply accumulates
-  // into the blocks the pp/hh chain wrote.  A graph capture records one stream.
+  // The pp/hh chain (122 or clear, pp/hh GEMM, one-body partials) and the
+  // ph chain (Pandya, Z GEMM, J' terms), with E0 on one of them, touch
+  // disjoint outputs and scratch, so they run on two streams; both end before
+  // the ph apply accumulates into the blocks the pp/hh chain wrote.  A graph
+  // capture records one stream.
   const bool dual = c->dual && !c->capturing;
   hipStream_t s2 = dual ? c->stream2 : s;
   if (dual) {
@@ -1233,11 +1234,15 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
     HIPCHK(hipStreamWaitEvent(s2, c->fork_mark ? c->fork_mark : c->ev_fork, 0));
   }
   c->fork_mark = nullptr;
-  if (mask & (T110 | T220))
+  // E0 (110, 220) reads only the operands; it joins the chain that ends
+  // first at emax=8: the ph chain with four-slice products, the pp/hh chain
+  // otherwise (the ph chain's Z product is then the longer one)
+  const int ns = i8_planes(c->prec);
+  const bool e0_ph = ns == 4;
+  if ((mask & (T110 | T220)) && !e0_ph)
     PROFS(c, IMSRG_PROF_E0, s2, launch_e0(s2, D, L->d, R->d, RES->d, hL, hR, mask, c->partials, ad, na, OS));
   if (mask & T122) PROFS(c, IMSRG_PROF_122, s2, launch_122(s2, D, L->d, R->d, RES->d, hres, ad, na, c->w122));
   else launch_clear_blocks(s2, D, RES->d, ad, na);
-  const int ns = i8_planes(c->prec);
   const bool i8 = ns > 0;
   if ((mask & (T222PPHH | T221)) && i8) {
     PROFS(c, IMSRG_PROF_SLICE, s2, {
@@ -1280,6 +1285,8 @@ static int comm_launches(imsrg_ctx_s *c, imsrg_op_s *L, imsrg_op_s *R, imsrg_op_
     }
     PROF(c, IMSRG_PROF_PH_TERMS, launch_ph_terms(s, D, c->wph, c->tph, (double)hL, (double)hR, ad, na));
   }
+  if ((mask & (T110 | T220)) && e0_ph)
+    PROF(c, IMSRG_PROF_E0, launch_e0(s, D, L->d, R->d, RES->d, hL, hR, mask, c->partials, ad, na, OS));
   if (dual) {
     HIPCHK(hipEventRecord(c->ev_join, s2));
     HIPCHK(hipStreamWaitEvent(s, c->ev_join, 0));
