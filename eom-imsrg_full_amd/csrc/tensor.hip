@@ -378,34 +378,60 @@ __device__ __forceinline__ bool elem_of(const DevLayout &L, const DevTensor &T, 
   return true;
 }
 
-// TS_commutator_122 (TS_commutators.f90:555-663): overwrites every element
+// TS_commutator_122 (TS_commutators.f90:555-663): overwrites every element.
+// Per output (ab|cd; J1 J2) four sums over the orbits i of the (l,j,tz) group
+// of a, b, c, d: +LF(a,i) T(ib,cd), +LF(b,i) T(ai,cd), -LF(i,c) T(ab,id),
+// -LF(i,d) T(ab,ci).  Each T element lies in the output's own tensor block
+// with its column (kinds 0, 1) or row (kinds 2, 3) fixed, so it costs one
+// pair lookup and one load; the partners go four at a time with every load
+// off any branch (clamped to a valid entry, weight 0 past the group), as in
+// k_t212.
 __global__ void k_t122(DevLayout L, DevTensor T, const double *__restrict__ LF, const double *__restrict__ RV,
                        double *__restrict__ OV, int hR) {
+  (void)hR;  // every element read lies in the output's block (J1 <= J2): no hermitian swap
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   int q, a, b, c, d;
   long long vp;
   if (!elem_of(L, T, e, q, a, b, c, d, vp)) return;
-  const int J1 = T.J1[q], J2 = T.J2[q], n = L.nsp;
+  const int J1 = T.J1[q], J2 = T.J2[q], n = L.nsp, n2 = T.n2[q];
+  const long long u = e - T.tstart[q];
+  const int r1 = (int)(u / n2), s2 = (int)(u % n2);  // row of (ab), column of (cd) in the block
+  const double *Tq = RV + T.off[q];
+  const int *P1 = L.pidx + (size_t)(J1 >> 1) * n * n, *P2 = L.pidx + (size_t)(J2 >> 1) * n * n;
+  const double rab = a == b ? M_SQRT2 : 1.0, rcd = c == d ? M_SQRT2 : 1.0;
   double sm = 0.0;
-  int g = L.grp_of[a];
-  for (int k = 0; k < L.grp_cnt[g]; k++) {
-    const int i = L.grp_list[L.grp_start[g] + k];
-    sm += LF[(size_t)a * n + i] * t_elem(L, T, RV, hR, i, b, c, d, J1, J2);
-  }
-  g = L.grp_of[b];
-  for (int k = 0; k < L.grp_cnt[g]; k++) {
-    const int i = L.grp_list[L.grp_start[g] + k];
-    sm += LF[(size_t)b * n + i] * t_elem(L, T, RV, hR, a, i, c, d, J1, J2);
-  }
-  g = L.grp_of[c];
-  for (int k = 0; k < L.grp_cnt[g]; k++) {
-    const int i = L.grp_list[L.grp_start[g] + k];
-    sm -= LF[(size_t)i * n + c] * t_elem(L, T, RV, hR, a, b, i, d, J1, J2);
-  }
-  g = L.grp_of[d];
-  for (int k = 0; k < L.grp_cnt[g]; k++) {
-    const int i = L.grp_list[L.grp_start[g] + k];
-    sm -= LF[(size_t)i * n + d] * t_elem(L, T, RV, hR, a, b, c, i, J1, J2);
+  // kind 0: x = a, pair (i,b) at J1; 1: x = b, pair (a,i); 2: x = c, pair (i,d) at J2; 3: x = d, pair (c,i)
+#pragma unroll
+  for (int kind = 0; kind < 4; kind++) {
+    const int x = kind == 0 ? a : kind == 1 ? b : kind == 2 ? c : d;
+    const int o = kind == 0 ? b : kind == 1 ? a : kind == 2 ? d : c;  // the pair's other orbit
+    const int jo = L.jj[o], J = kind < 2 ? J1 : J2, g = L.grp_of[x];
+    const int k0 = L.grp_start[g], cnt = L.grp_cnt[g], ji = L.jj[x];  // the group shares x's 2j
+    for (int k = 0; k < cnt; k += 4) {
+      int pi[4];
+#pragma unroll
+      for (int v = 0; v < 4; v++) pi[v] = L.grp_list[k0 + min(k + v, cnt - 1)];
+      long long pos[4];
+      double w[4];
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = pi[v], lo = min(i, o), hi = max(i, o);
+        const int rr = (kind < 2 ? P1 : P2)[lo * n + hi];
+        // i first in kinds 0 and 2, second in 1 and 3: the stored pair is (lo, hi)
+        const bool swp = (kind == 0 || kind == 2) ? i > o : o > i;
+        const double f = swp ? (double)sgnp(1 + ((ji + jo - J) >> 1)) : (i == o ? M_SQRT2 : 1.0);
+        const double lf = kind == 0 ? LF[(size_t)a * n + i] : kind == 1 ? LF[(size_t)b * n + i]
+                        : kind == 2 ? LF[(size_t)i * n + c] : LF[(size_t)i * n + d];
+        const bool ok = k + v < cnt && rr >= 0 && dtri(ji, jo, J);
+        pos[v] = !ok ? 0 : kind < 2 ? (long long)rr * n2 + s2 : (long long)r1 * n2 + rr;
+        w[v] = ok ? f * lf * (kind < 2 ? rcd : rab) : 0.0;
+      }
+      double t[4];
+#pragma unroll
+      for (int v = 0; v < 4; v++) t[v] = Tq[pos[v]];
+#pragma unroll
+      for (int v = 0; v < 4; v++) sm += (kind < 2 ? w[v] : -w[v]) * t[v];
+    }
   }
   sm = sm / sqrt(1.0 + (a == b)) / sqrt(1.0 + (c == d));
   OV[vp] = sm;
