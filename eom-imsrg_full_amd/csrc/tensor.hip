@@ -829,22 +829,43 @@ __global__ __launch_bounds__(256) void k_tgemm(const TProb *__restrict__ probs, 
   double4_t acc[2][2];
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  // each thread stages 4 elements of A and 4 of B per K step of 16; the next
+  // step's elements are loaded into registers (clamped to the operand, so
+  // every load is unconditional and the eight stay in flight together) while
+  // the current step runs on the matrix cores
   for (int sgi = 0; sgi < p.nseg; sgi++) {
     const TSeg sg = p.seg[sgi];
+    if (sg.K <= 0) continue;
     const double *Ap = (sg.asrc ? X1 : X0) + sg.a_off, *Bp = (sg.bsrc ? X1 : X0) + sg.b_off;
     const bool arow = sg.a_cs == 1, brow = sg.b_cs == 1;
+    int am[4], ak[4], bn[4], bk[4];  // element (row/column, k) of each staged value in the tile
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int e = tid + 256 * u;
+      am[u] = arow ? e >> 4 : e & 63;
+      ak[u] = arow ? e & 15 : e >> 6;
+      bn[u] = brow ? e & 63 : e >> 4;
+      bk[u] = brow ? e >> 6 : e & 15;
+    }
+    double ra[4], rb[4];
+    auto gload = [&](int k0) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int m = min(m0 + am[u], p.M - 1), k = min(k0 + ak[u], sg.K - 1);
+        const int n = min(n0 + bn[u], p.N - 1), k2 = min(k0 + bk[u], sg.K - 1);
+        ra[u] = Ap[(long long)m * sg.a_rs + (long long)k * sg.a_cs];
+        rb[u] = Bp[(long long)k2 * sg.b_rs + (long long)n * sg.b_cs];
+      }
+    };
+    gload(0);
     for (int k0 = 0; k0 < sg.K; k0 += 16) {
-      for (int e = tid; e < 16 * 64; e += 256) {
-        int kk, mm;
-        if (arow) { kk = e & 15; mm = e >> 4; } else { mm = e & 63; kk = e >> 6; }
-        const int m = m0 + mm, k = k0 + kk;
-        As[kk][mm] = (m < p.M && k < sg.K) ? sg.alpha * Ap[(long long)m * sg.a_rs + (long long)k * sg.a_cs] : 0.0;
-        int kb, nn;
-        if (brow) { nn = e & 63; kb = e >> 6; } else { kb = e & 15; nn = e >> 4; }
-        const int n = n0 + nn, k2 = k0 + kb;
-        Bs[kb][nn] = (n < p.N && k2 < sg.K) ? Bp[(long long)k2 * sg.b_rs + (long long)n * sg.b_cs] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        As[ak[u]][am[u]] = (m0 + am[u] < p.M && k0 + ak[u] < sg.K) ? sg.alpha * ra[u] : 0.0;
+        Bs[bk[u]][bn[u]] = (n0 + bn[u] < p.N && k0 + bk[u] < sg.K) ? rb[u] : 0.0;
       }
       __syncthreads();
+      gload(k0 + 16);  // past the end: clamped reloads, never used
 #pragma unroll
       for (int kq = 0; kq < 4; kq++) {
         const int k = kq * 4 + (lane >> 4);
