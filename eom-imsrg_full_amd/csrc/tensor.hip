@@ -53,6 +53,10 @@ struct DevTensor {
   const unsigned *quads;          // orbit quadruples x0|x1<<8|x2<<16|x3<<24 (x0 <= x1, x2 <= x3) with outputs
   long long nquad;
   int nslotmax;                   // most gather slots of any quadruple (k_t222ph_q's LDS list)
+  // the quadruples in three runs by slot count (at most 256, 512, nslotmax),
+  // one launch each with an LDS list of that size: run k starts at qrun[k]
+  long long qrun[4];
+  int qcap[3];
   // the orbits i the tensor's one-body part couples to orbit x (same tz, l + dpar/2
   // of the same parity, triangle (ji, jx, R)): part[pstart[x] .. pstart[x+1]),
   // each i | ji << 8
@@ -631,14 +635,14 @@ struct PhSlot {
 };
 __global__ __launch_bounds__(64) void k_t222ph_q(DevLayout L, DevTensor T, const double *__restrict__ Wx,
                                                  const double *__restrict__ Wy, double *__restrict__ OV, int hL,
-                                                 int hR) {
+                                                 int hR, int run) {
   extern __shared__ double ph_lds[];
   PhSlot *slot = reinterpret_cast<PhSlot *>(ph_lds);
   __shared__ int pa[16], pb[16], pc[16], pd[16], pok[16], pab[16], pph[16], pj[16][4], pj3[16], pstart[17];
   __shared__ double pcs[16];
-  const long long qd = blockIdx.x;
-  const int lane = threadIdx.x;
-  if (qd >= T.nquad) return;
+  const long long qd = T.qrun[run] + blockIdx.x;
+  const int lane = threadIdx.x, cap = T.qcap[run];
+  if (qd >= T.qrun[run + 1]) return;
   const unsigned code = T.quads[qd];
   const int x0 = code & 255, x1 = (code >> 8) & 255, x2 = (code >> 16) & 255, x3 = (int)(code >> 24);
   const int R = T.rank, n = L.nsp, JM = L.jtot_max, nd = T.nd, nJ = T.nJ, h = T.nh;
@@ -742,8 +746,8 @@ __global__ __launch_bounds__(64) void k_t222ph_q(DevLayout L, DevTensor T, const
     }
     const unsigned long long m = __ballot(live);
     if (live) {
-      // pos < this quadruple's slot count <= nslotmax (the host counts the same slots)
-      const int pos = min(count + __popcll(m & ((1ull << lane) - 1ull)), T.nslotmax - 1);
+      // pos < this quadruple's slot count <= its run's cap (the host counts the same slots)
+      const int pos = min(count + __popcll(m & ((1ull << lane) - 1ull)), cap - 1);
       slot[pos].c = cval;
       slot[pos].b9 = b9;
       slot[pos].ab = abk;
@@ -1098,7 +1102,7 @@ static int tspace_build(imsrg_ctx_s *c, int rank, int dpar, TensorSpace &S) {
   // column pair x2 <= x3, with at least one stored output (orbits packed in 8 bits)
   if (nsp > 256) return fail(IMSRG_EINVAL, "tensor operators need at most 256 orbits");
   {
-    std::vector<unsigned> quads;
+    std::vector<unsigned> qrun[3];  // by slot count: at most 256, 512, more
     std::vector<int> pa_, pb_;
     int nslotmax = 1;
     for (int a = 0; a < nsp; a++)
@@ -1121,7 +1125,7 @@ static int tspace_build(imsrg_ctx_s *c, int rank, int dpar, TensorSpace &S) {
           }
         }
         if (!any) continue;
-        quads.push_back((unsigned)x0 | (unsigned)x1 << 8 | (unsigned)x2 << 16 | (unsigned)x3 << 24);
+        const unsigned code = (unsigned)x0 | (unsigned)x1 << 8 | (unsigned)x2 << 16 | (unsigned)x3 << 24;
         // the quadruple's gather slots (k_t222ph_q): a run of J3 times the
         // J4 - J3 steps per pattern that passes the branch, parity and Tz tests
         const int xs[4] = {x0, x1, x2, x3};
@@ -1137,11 +1141,21 @@ static int tspace_build(imsrg_ctx_s *c, int rank, int dpar, TensorSpace &S) {
           ns += ((jd + ja - std::abs(jd - ja)) / 2 + 1) * (rank / 2 + 1);
         }
         nslotmax = std::max(nslotmax, ns);
+        qrun[ns <= 256 ? 0 : ns <= 512 ? 1 : 2].push_back(code);
       }
+    }
+    std::vector<unsigned> quads;
+    D.qrun[0] = 0;
+    for (int k = 0; k < 3; k++) {
+      quads.insert(quads.end(), qrun[k].begin(), qrun[k].end());
+      D.qrun[k + 1] = (long long)quads.size();
     }
     if (upload_table(c, quads, &D.quads) != 0) return IMSRG_ENOMEM;
     D.nquad = (long long)quads.size();
     D.nslotmax = nslotmax;
+    D.qcap[0] = std::min(256, nslotmax);
+    D.qcap[1] = std::min(512, nslotmax);
+    D.qcap[2] = nslotmax;
   }
   // products: the pp and hh ladders per tensor block (W1, W2)
   {
@@ -1452,10 +1466,11 @@ static int ts_comm(imsrg_ctx_s *c, const double *Gd, int hG, imsrg_tensor T, ims
     if (S.n_cc_tiles[1] > 0)
       hipLaunchKernelGGL(k_tgemm, dim3(S.n_cc_tiles[1]), dim3(256), 0, s, S.cc_probs[1], S.cc_tiles[1],
                          S.n_cc_tiles[1], (const double *)S.ccx, (const double *)S.gcc, S.wy);
-    if (D.nquad > 0)
-      hipLaunchKernelGGL(k_t222ph_q, dim3((unsigned)D.nquad), dim3(64),
-                         (unsigned)(sizeof(PhSlot) * D.nslotmax), s, L, D, (const double *)S.wx,
-                         (const double *)S.wy, OV, hL, hR);
+    for (int k = 0; k < 3; k++)
+      if (D.qrun[k + 1] > D.qrun[k])
+        hipLaunchKernelGGL(k_t222ph_q, dim3((unsigned)(D.qrun[k + 1] - D.qrun[k])), dim3(64),
+                           (unsigned)(sizeof(PhSlot) * D.qcap[k]), s, L, D, (const double *)S.wx,
+                           (const double *)S.wy, OV, hL, hR, k);
   }
   HIPCHK(hipGetLastError());
   return 0;
