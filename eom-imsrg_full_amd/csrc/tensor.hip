@@ -53,10 +53,11 @@ struct DevTensor {
   const unsigned *quads;          // orbit quadruples x0|x1<<8|x2<<16|x3<<24 (x0 <= x1, x2 <= x3) with outputs
   long long nquad;
   int nslotmax;                   // most gather slots of any quadruple (k_t222ph_q's LDS list)
-  // the quadruples in three runs by slot count (at most 256, 512, nslotmax),
-  // one launch each with an LDS list of that size: run k starts at qrun[k]
-  long long qrun[4];
-  int qcap[3];
+  // the quadruples in four runs by slot count (at most 256, 384, 512,
+  // nslotmax), one launch each with an LDS list of that size: run k starts
+  // at qrun[k]
+  long long qrun[5];
+  int qcap[4];
   // the orbits i the tensor's one-body part couples to orbit x (same tz, l + dpar/2
   // of the same parity, triangle (ji, jx, R)): part[pstart[x] .. pstart[x+1]),
   // each i | ji << 8
@@ -633,9 +634,9 @@ struct PhSlot {
   unsigned b9;   // 9j table index without the (J1, J2) part
   int ab;        // bit 0: (-1)^(J1/2) applies, bit 1: (-1)^(J2/2) applies
 };
-__global__ __launch_bounds__(64) void k_t222ph_q(DevLayout L, DevTensor T, const double *__restrict__ Wx,
-                                                 const double *__restrict__ Wy, double *__restrict__ OV, int hL,
-                                                 int hR, int run) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void k_t222ph_q(
+    DevLayout L, DevTensor T, const double *__restrict__ Wx, const double *__restrict__ Wy, double *__restrict__ OV,
+    int hL, int hR, int run) {
   extern __shared__ double ph_lds[];
   PhSlot *slot = reinterpret_cast<PhSlot *>(ph_lds);
   __shared__ int pa[16], pb[16], pc[16], pd[16], pok[16], pab[16], pph[16], pj[16][4], pj3[16], pstart[17];
@@ -1102,7 +1103,7 @@ static int tspace_build(imsrg_ctx_s *c, int rank, int dpar, TensorSpace &S) {
   // column pair x2 <= x3, with at least one stored output (orbits packed in 8 bits)
   if (nsp > 256) return fail(IMSRG_EINVAL, "tensor operators need at most 256 orbits");
   {
-    std::vector<unsigned> qrun[3];  // by slot count: at most 256, 512, more
+    std::vector<unsigned> qrun[4];  // by slot count: at most 256, 384, 512, more
     std::vector<int> pa_, pb_;
     int nslotmax = 1;
     for (int a = 0; a < nsp; a++)
@@ -1141,12 +1142,12 @@ static int tspace_build(imsrg_ctx_s *c, int rank, int dpar, TensorSpace &S) {
           ns += ((jd + ja - std::abs(jd - ja)) / 2 + 1) * (rank / 2 + 1);
         }
         nslotmax = std::max(nslotmax, ns);
-        qrun[ns <= 256 ? 0 : ns <= 512 ? 1 : 2].push_back(code);
+        qrun[ns <= 256 ? 0 : ns <= 384 ? 1 : ns <= 512 ? 2 : 3].push_back(code);
       }
     }
     std::vector<unsigned> quads;
     D.qrun[0] = 0;
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < 4; k++) {
       quads.insert(quads.end(), qrun[k].begin(), qrun[k].end());
       D.qrun[k + 1] = (long long)quads.size();
     }
@@ -1154,8 +1155,9 @@ static int tspace_build(imsrg_ctx_s *c, int rank, int dpar, TensorSpace &S) {
     D.nquad = (long long)quads.size();
     D.nslotmax = nslotmax;
     D.qcap[0] = std::min(256, nslotmax);
-    D.qcap[1] = std::min(512, nslotmax);
-    D.qcap[2] = nslotmax;
+    D.qcap[1] = std::min(384, nslotmax);
+    D.qcap[2] = std::min(512, nslotmax);
+    D.qcap[3] = nslotmax;
   }
   // products: the pp and hh ladders per tensor block (W1, W2)
   {
@@ -1466,7 +1468,7 @@ static int ts_comm(imsrg_ctx_s *c, const double *Gd, int hG, imsrg_tensor T, ims
     if (S.n_cc_tiles[1] > 0)
       hipLaunchKernelGGL(k_tgemm, dim3(S.n_cc_tiles[1]), dim3(256), 0, s, S.cc_probs[1], S.cc_tiles[1],
                          S.n_cc_tiles[1], (const double *)S.ccx, (const double *)S.gcc, S.wy);
-    for (int k = 0; k < 3; k++)
+    for (int k = 0; k < 4; k++)
       if (D.qrun[k + 1] > D.qrun[k])
         hipLaunchKernelGGL(k_t222ph_q, dim3((unsigned)(D.qrun[k + 1] - D.qrun[k])), dim3(64),
                            (unsigned)(sizeof(PhSlot) * D.qcap[k]), s, L, D, (const double *)S.wx,
