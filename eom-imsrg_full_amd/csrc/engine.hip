@@ -322,6 +322,10 @@ int imsrg_ctx_create_batch(int nsp, const int *n, const int *l, const int *jj, c
     int lds_max = 65536;
     (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
     const size_t need = pandya_lds_bytes(D);
+    if (!Ly.bdim.empty() && *std::max_element(Ly.bdim.begin(), Ly.bdim.end()) >= 32768) {
+      imsrg_ctx_destroy(c);
+      return fail(IMSRG_EINVAL, "basis too large: channel blocks of 2^15 pairs or more");
+    }
     if (need > (size_t)lds_max) {
       imsrg_ctx_destroy(c);
       return fail(IMSRG_EINVAL, "basis too large: the Pandya kernel stages " + std::to_string(need) +

@@ -857,10 +857,13 @@ __global__ __launch_bounds__(256) void k_pandya(DevLayout L, const double *__res
   const size_t NN = (size_t)N * N;
   double *S6 = pan_lds;
   long long *BO = reinterpret_cast<long long *>(S6 + nsl);
-  int *PR = reinterpret_cast<int *>(BO + NB);  // [J'/2][a]: position of pair (h,a) in block J'
-  int *PS = PR + nJ * N;                       // [J'/2][b]: position of pair (p,b)
-  int *BD = PS + nJ * N;
+  int *BD = reinterpret_cast<int *>(BO + NB);
   int *OQ = BD + NB;                           // jj | l << 8 | (2tz + 1) << 16
+  // [J'/2][a]: position of pair (h,a) in block J'; [J'/2][b]: of pair (p,b)
+  // (16 bits: positions below 2^15, -1 absent; the smaller list lets eight
+  // wavefronts per SIMD stay resident)
+  short *PR = reinterpret_cast<short *>(OQ + N);
+  short *PS = PR + nJ * N;
   {
     // every staging load of a batch is issued before the first LDS store, so
     // a thread waits for one memory latency per batch, not one per element
@@ -893,8 +896,8 @@ __global__ __launch_bounds__(256) void k_pandya(DevLayout L, const double *__res
       for (int u = 0; u < 8; u++) {
         const int e = e0 + (int)threadIdx.x + 256 * u;
         if (e < nJ * N) {
-          PR[e] = vr[u];
-          PS[e] = vs[u];
+          PR[e] = (short)vr[u];
+          PS[e] = (short)vs[u];
         }
       }
     }
@@ -963,7 +966,7 @@ __global__ __launch_bounds__(256) void k_pandya(DevLayout L, const double *__res
 }
 size_t pandya_lds_bytes(const DevLayout &L) {
   return sizeof(double) * L.nhalf * L.nhalf * L.nJ + sizeof(long long) * L.nblocks +
-         sizeof(int) * (2 * L.nJ * L.nsp + L.nblocks + L.nsp);
+         sizeof(int) * (L.nblocks + L.nsp) + sizeof(short) * 2 * L.nJ * L.nsp;
 }
 // lets k_pandya take more than 64 KiB of dynamic LDS (up to the device's per-workgroup limit)
 hipError_t pandya_allow_lds(size_t bytes) {
