@@ -62,6 +62,11 @@ struct DevTensor {
   // of the same parity, triangle (ji, jx, R)): part[pstart[x] .. pstart[x+1]),
   // each i | ji << 8
   const int *pstart, *part;
+  // thread orders of k_tpandya inside each scalar Pandya block: columns by
+  // (2j of p, 2j of h), rows by the 2j of their two orbits (zero rows last),
+  // so that the lanes of a wavefront sum (J3', J4') runs of equal length;
+  // entry i is the block-local column / row thread i computes
+  const int *cperm, *rperm;
   long long f_off, v_off;
 };
 
@@ -532,16 +537,19 @@ __global__ void k_tpandya(DevLayout L, DevTensor T, int mode, const double *__re
   const long long u = e - st[x];
   const int J3 = T.cJ3[x], J4 = T.cJ4[x], q3 = T.cq3[x], q4 = T.cq4[x];
   int a, b, p, h;
+  long long o;  // the entry's position in its matrix
   if (mode == 0) {
     const int nb4 = L.ccnph[q4];
-    const int R1 = (int)(u / nb4), k = (int)(u % nb4);
+    const int R1 = (int)(u / nb4), k = T.cperm[L.cccbase[q4] + (int)(u % nb4)];
+    o = (long long)R1 * nb4 + k;
     a = L.ccrow_a[L.ccrbase[q3] + R1];
     b = L.ccrow_b[L.ccrbase[q3] + R1];
     p = L.cccol_p[L.cccbase[q4] + k];
     h = L.cccol_h[L.cccbase[q4] + k];
   } else {
     const int r4 = L.ccrlen[q4];
-    const int k = (int)(u / r4), G = (int)(u % r4);
+    const int k = (int)(u / r4), G = T.rperm[L.ccrbase[q4] + (int)(u % r4)];
+    o = (long long)k * r4 + G;
     p = L.cccol_p[L.cccbase[q3] + k];
     h = L.cccol_h[L.cccbase[q3] + k];
     b = L.ccrow_a[L.ccrbase[q4] + G];  // the row pair is (b, a)
@@ -584,7 +592,7 @@ __global__ void k_tpandya(DevLayout L, DevTensor T, int mode, const double *__re
                     : sm * sgnp((jp + ja + J4) >> 1) * s34 * sgnp((jp + jh) >> 1);
     }
   }
-  out[(mode == 0 ? T.xoff[x] : T.roff[x]) + u] = v;
+  out[(mode == 0 ? T.xoff[x] : T.roff[x]) + o] = v;
 }
 
 // The ph recoupling of TS_commutator_222_ph (TS_commutators.f90:1459-1849) as
@@ -1072,6 +1080,28 @@ static int tspace_build(imsrg_ctx_s *c, int rank, int dpar, TensorSpace &S) {
   TUP(S.cJ3, cJ3); TUP(S.cJ4, cJ4); TUP(S.cq3, cq3); TUP(S.cq4, cq4);
   TUP(S.xoff, xoff); TUP(S.roff, roff); TUP(S.woff, woff); TUP(S.xstart, xstart); TUP(S.rstart, rstart);
   TUP(S.cidx, cidx); TUP(S.ccol, ccol); TUP(S.x6, x6); TUP(S.s6, s6);
+  {
+    std::vector<int> cperm(Ly.cccbase[Ly.ncc]), rperm(Ly.ccrbase[Ly.ncc]);
+    for (int q = 0; q < Ly.ncc; q++) {
+      const int c0 = Ly.cccbase[q], nc = Ly.cccbase[q + 1] - c0, r0 = Ly.ccrbase[q], nr = Ly.ccrlen[q];
+      std::vector<int> ix(nc);
+      for (int k = 0; k < nc; k++) ix[k] = k;
+      std::stable_sort(ix.begin(), ix.end(), [&](int x, int y) {
+        return std::make_pair(Ly.jj[Ly.cccol_p[c0 + x]], Ly.jj[Ly.cccol_h[c0 + x]]) <
+               std::make_pair(Ly.jj[Ly.cccol_p[c0 + y]], Ly.jj[Ly.cccol_h[c0 + y]]);
+      });
+      std::copy(ix.begin(), ix.end(), cperm.begin() + c0);
+      auto rkey = [&](int g) {
+        const int a = Ly.ccrow_a[r0 + g], b = Ly.ccrow_b[r0 + g];
+        return a < 0 ? std::make_pair(1 << 20, 0) : std::make_pair(Ly.jj[a], Ly.jj[b]);
+      };
+      ix.resize(nr);
+      for (int g = 0; g < nr; g++) ix[g] = g;
+      std::stable_sort(ix.begin(), ix.end(), [&](int x, int y) { return rkey(x) < rkey(y); });
+      std::copy(ix.begin(), ix.end(), rperm.begin() + r0);
+    }
+    TUP(cperm, cperm); TUP(rperm, rperm);
+  }
   {
     std::vector<int> ps(nsp + 1, 0), pl;
     for (int x = 0; x < nsp; x++) {
