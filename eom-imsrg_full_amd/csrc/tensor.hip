@@ -67,6 +67,11 @@ struct DevTensor {
   // so that the lanes of a wavefront sum (J3', J4') runs of equal length;
   // entry i is the block-local column / row thread i computes
   const int *cperm, *rperm;
+  // thread order of the 122 and 212 gathers inside each tensor block's rows:
+  // the ket pairs of every scalar block by the (l,j,tz) groups of their two
+  // orbits (the lengths of their partner loops); entry pbase[q] + i is the
+  // column thread i of a row computes
+  const int *sperm;
   long long f_off, v_off;
 };
 
@@ -375,13 +380,19 @@ __global__ __launch_bounds__(64) void k_t_onebody(DevLayout L, DevTensor T, int 
 }
 
 // ---------------------------------------------------------------- two-body gathers
+// flat element e -> block q, orbits, position; with perm, the columns of a
+// row in the sperm order (r, s: row and column in the block)
 __device__ __forceinline__ bool elem_of(const DevLayout &L, const DevTensor &T, long long e, int &q, int &a, int &b,
-                                        int &c, int &d, long long &vpos) {
+                                        int &c, int &d, long long &vpos, bool perm = false, int *rs = nullptr) {
   if (e >= T.nelem) return false;
   q = find_item(T.tstart, T.nblk, e);
   const long long u = e - T.tstart[q];
   const int n2 = T.n2[q];
-  const int r = (int)(u / n2), s = (int)(u % n2);
+  const int r = (int)(u / n2), s0 = (int)(u % n2), s = perm ? T.sperm[L.pbase[T.q2[q]] + s0] : s0;
+  if (rs) {
+    rs[0] = r;
+    rs[1] = s;
+  }
   const int g1 = L.pbase[T.q1[q]] + r, g2 = L.pbase[T.q2[q]] + s;
   a = L.pair_a[g1]; b = L.pair_b[g1]; c = L.pair_a[g2]; d = L.pair_b[g2];
   vpos = T.off[q] + (long long)r * n2 + s;
@@ -400,12 +411,11 @@ __global__ void k_t122(DevLayout L, DevTensor T, const double *__restrict__ LF, 
                        double *__restrict__ OV, int hR) {
   (void)hR;  // every element read lies in the output's block (J1 <= J2): no hermitian swap
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  int q, a, b, c, d;
+  int q, a, b, c, d, rs[2];
   long long vp;
-  if (!elem_of(L, T, e, q, a, b, c, d, vp)) return;
+  if (!elem_of(L, T, e, q, a, b, c, d, vp, true, rs)) return;
   const int J1 = T.J1[q], J2 = T.J2[q], n = L.nsp, n2 = T.n2[q];
-  const long long u = e - T.tstart[q];
-  const int r1 = (int)(u / n2), s2 = (int)(u % n2);  // row of (ab), column of (cd) in the block
+  const int r1 = rs[0], s2 = rs[1];  // row of (ab), column of (cd) in the block
   const double *Tq = RV + T.off[q];
   const int *P1 = L.pidx + (size_t)(J1 >> 1) * n * n, *P2 = L.pidx + (size_t)(J2 >> 1) * n * n;
   const double rab = a == b ? M_SQRT2 : 1.0, rcd = c == d ? M_SQRT2 : 1.0;
@@ -463,14 +473,13 @@ __global__ void k_t122(DevLayout L, DevTensor T, const double *__restrict__ LF, 
 __global__ void k_t212(DevLayout L, DevTensor T, const double *__restrict__ GV, const double *__restrict__ RF,
                        double *__restrict__ OV) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  int q, a, b, c, d;
+  int q, a, b, c, d, rs[2];
   long long vp;
-  if (!elem_of(L, T, e, q, a, b, c, d, vp)) return;
+  if (!elem_of(L, T, e, q, a, b, c, d, vp, true, rs)) return;
   const int J1 = T.J1[q], J2 = T.J2[q], n = L.nsp, R = T.rank;
   const int ja = L.jj[a], jb = L.jj[b], jc = L.jj[c], jd = L.jj[d];
   const int q1 = T.q1[q], q2 = T.q2[q];
-  const long long u = e - T.tstart[q];
-  const int r1 = (int)(u / T.n2[q]), s2 = (int)(u % T.n2[q]);  // row of (ab) in block q1, column of (cd) in q2
+  const int r1 = rs[0], s2 = rs[1];  // row of (ab) in block q1, column of (cd) in q2
   const double *V1 = GV + L.boff[q1], *V2 = GV + L.boff[q2];
   const int d1 = L.bdim[q1], d2 = L.bdim[q2];
   const int *P1 = L.pidx + (size_t)(J1 >> 1) * n * n, *P2 = L.pidx + (size_t)(J2 >> 1) * n * n;
@@ -1101,6 +1110,18 @@ static int tspace_build(imsrg_ctx_s *c, int rank, int dpar, TensorSpace &S) {
       std::copy(ix.begin(), ix.end(), rperm.begin() + r0);
     }
     TUP(cperm, cperm); TUP(rperm, rperm);
+    std::vector<int> sperm(Ly.npairs);
+    for (int q = 0; q < Ly.nblocks; q++) {
+      const int p0 = Ly.pbase[q], np = Ly.pbase[q + 1] - p0;
+      std::vector<int> ix(np);
+      for (int k = 0; k < np; k++) ix[k] = k;
+      std::stable_sort(ix.begin(), ix.end(), [&](int x, int y) {
+        return std::make_pair(Ly.grp_of[Ly.pair_a[p0 + x]], Ly.grp_of[Ly.pair_b[p0 + x]]) <
+               std::make_pair(Ly.grp_of[Ly.pair_a[p0 + y]], Ly.grp_of[Ly.pair_b[p0 + y]]);
+      });
+      std::copy(ix.begin(), ix.end(), sperm.begin() + p0);
+    }
+    TUP(sperm, sperm);
   }
   {
     std::vector<int> ps(nsp + 1, 0), pl;
