@@ -356,13 +356,31 @@ __global__ __launch_bounds__(64) void k_t_onebody(DevLayout L, DevTensor T, int 
       const int c = u < T.nhole ? T.holes[u] : T.parts[u - T.nhole], jc = L.jj[c];
       const double *W = u < T.nhole ? W1 : W2;
       double s1 = 0.0;
+      // J2 over its whole range, four at a time with every load off any
+      // branch: J2 < J1 reads W(cj, ci; J2 J1) (the lower half, by herm), J2 >= J1
+      // W(ci, cj; J1 J2).  The pairs' channel matches the output's, which
+      // passed the tensor's parity and Tz tests.
+      const int J2lo = abs(jc - jj), n2 = (jc + jj - J2lo) / 2 + 1;
       for (int J1 = abs(jc - ji); J1 <= jc + ji; J1 += 2) {
-        for (int J2 = abs(jc - jj); J2 <= min(J1 - 2, jc + jj); J2 += 2)
-          s1 -= sqrt((J1 + 1.0) * (J2 + 1.0)) * tx6(T, J1, J2, jj, ji, jc) *
-                t_elem(L, T, W, hR, c, j, c, i, J2, J1) * sgnp(J2 >> 1) * pm;
-        for (int J2 = max(J1, abs(jc - jj)); J2 <= jc + jj; J2 += 2)
-          s1 += sqrt((J1 + 1.0) * (J2 + 1.0)) * tx6(T, J1, J2, jj, ji, jc) *
-                t_elem(L, T, W, hR, c, i, c, j, J1, J2) * sgnp(J1 >> 1);
+        for (int k0 = 0; k0 < n2; k0 += 4) {
+          long long pos[4];
+          double w[4];
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int J2 = J2lo + 2 * min(k0 + v, n2 - 1);
+            const bool lo = J2 < J1;
+            double f;
+            pos[v] = lo ? t_pos_all(L, T, hR, c, j, c, i, jc, jj, jc, ji, J2, J1, true, f)
+                        : t_pos_all(L, T, hR, c, i, c, j, jc, ji, jc, jj, J1, J2, true, f);
+            const double co = sqrt((J1 + 1.0) * (J2 + 1.0)) * tx6(T, J1, J2, jj, ji, jc);
+            w[v] = k0 + v < n2 ? (lo ? -co * sgnp(J2 >> 1) * pm : co * sgnp(J1 >> 1)) * f : 0.0;
+          }
+          double g[4];
+#pragma unroll
+          for (int v = 0; v < 4; v++) g[v] = W[pos[v]];
+#pragma unroll
+          for (int v = 0; v < 4; v++) s1 += w[v] * g[v];
+        }
       }
       sm += s1 * sgnp((jc + jj) >> 1);
     }
