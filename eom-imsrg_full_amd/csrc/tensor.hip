@@ -681,7 +681,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void k_
   if (qd >= T.qrun[run + 1]) return;
   const unsigned code = T.quads[qd];
   const int x0 = code & 255, x1 = (code >> 8) & 255, x2 = (code >> 16) & 255, x3 = (int)(code >> 24);
-  const int R = T.rank, n = L.nsp, JM = L.jtot_max, nd = T.nd, nJ = T.nJ, h = T.nh;
+  const int R = T.rank, n = L.nsp, JM = L.jtot_max, nd = T.nd, nJ = T.nJ, h = T.nh, dpb = (T.dpar >> 1) & 1;
   if (lane < 16) {
     const int k = lane;
     auto pick = [&](unsigned tbl) {
@@ -749,35 +749,40 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void k_
     double cval = 0.0;
     unsigned b9 = 0;
     int abk = 0;
-    if (s < nslot) {
+    {
+      // every table read is made for every lane, with indices in range
+      // whatever the tests say, so that they are in flight together: the
+      // entry's row IX = (d,a) in the J3 Pandya block, its column JX = (c,b)
+      // in the J4 one (the block index is arithmetic: J4, Tz and the parity of
+      // the J3 pairs with the tensor's), the product's block, then W
+      const int sc = min(s, nslot - 1);
       int k = 0;
-      while (pstart[k + 1] <= s) k++;
-      const int r = s - pstart[k], J3 = pj3[k] + 2 * (r / nd), J4 = J3 + 2 * (r % nd);
+      while (pstart[k + 1] <= sc) k++;
+      const int r = sc - pstart[k], J3 = pj3[k] + 2 * (r / nd), J4 = J3 + 2 * (r % nd);
       const int okc = pok[k];
-      if (okc) {
-        const int a = pa[k], b = pb[k], c = pc[k], d = pd[k], Tz = okc & 1, PAR = (okc >> 1) & 1;
-        const int ja = L.jj[a], jb = L.jj[b], jc = L.jj[c], jd = L.jj[d];
-        const bool isx = k < 8;
-        if (dtri(jd, ja, J3) && J4 <= 2 * JM && dtri(J3, J4, R) && dtri(jc, jb, J4) && (isx || J4 != J3)) {
-          const int q3 = (J3 >> 1) + Tz * (JM + 1) + 2 * PAR * (JM + 1);
-          const int IX = L.ccrow[((size_t)q3 * n + d) * n + a];
-          const int qx = T.cidx[((J3 >> 1) * nJ + (J4 >> 1)) * 4 + Tz * 2 + PAR];
-          if (IX >= 0 && qx >= 0) {
-            const int q4 = T.cq4[qx];
-            const int JX = L.ccrow[((size_t)q4 * n + c) * n + b];
-            if (JX >= 0) {
-              const long long wi = T.woff[qx] + (long long)IX * L.ccrlen[q4] + JX;
-              const double w = isx ? hL * Wx[wi] : Wy[wi];  // Wx carries LCC%herm (TS_commutators.f90:1506)
-              if (fabs(w) > thr) {
-                live = true;
-                cval = pcs[k] * (pph[k] ? (double)sgnp((J3 + J4) >> 1) : 1.0) * sqrt((J3 + 1.0) * (J4 + 1.0)) * w;
-                b9 = (unsigned)((((((pj[k][0] >> 1) * h + (pj[k][1] >> 1)) * h + (pj[k][2] >> 1)) * h +
-                                  (pj[k][3] >> 1)) * nJ + (J3 >> 1)) * nd + ((J4 - J3) >> 1)) * (unsigned)(nJ * nd);
-                abk = pab[k];
-              }
-            }
-          }
-        }
+      const int a = pa[k], b = pb[k], c = pc[k], d = pd[k], Tz = okc & 1, PAR = (okc >> 1) & 1;
+      const int ja = L.jj[a], jb = L.jj[b], jc = L.jj[c], jd = L.jj[d];
+      const bool isx = k < 8;
+      const bool tri = s < nslot && okc && dtri(jd, ja, J3) && J4 <= 2 * JM && dtri(J3, J4, R) &&
+                       dtri(jc, jb, J4) && (isx || J4 != J3);
+      const int J4c = min(J4, 2 * JM);
+      const int q3 = (J3 >> 1) + Tz * (JM + 1) + 2 * PAR * (JM + 1);
+      const int q4 = (J4c >> 1) + Tz * (JM + 1) + 2 * (PAR ^ dpb) * (JM + 1);
+      const int IX = L.ccrow[((size_t)q3 * n + d) * n + a];
+      const int JX = L.ccrow[((size_t)q4 * n + c) * n + b];
+      const int qx = T.cidx[((J3 >> 1) * nJ + (J4c >> 1)) * 4 + Tz * 2 + PAR];
+      const int rl4 = L.ccrlen[q4];
+      const long long wo = T.woff[max(qx, 0)];
+      const bool ok = tri && IX >= 0 && JX >= 0 && qx >= 0;
+      const long long wi = ok ? wo + (long long)IX * rl4 + JX : 0;
+      const double wv = (isx ? Wx : Wy)[wi];
+      const double w = ok ? (isx ? hL * wv : wv) : 0.0;  // Wx carries LCC%herm (TS_commutators.f90:1506)
+      if (fabs(w) > thr) {
+        live = true;
+        cval = pcs[k] * (pph[k] ? (double)sgnp((J3 + J4) >> 1) : 1.0) * sqrt((J3 + 1.0) * (J4 + 1.0)) * w;
+        b9 = (unsigned)((((((pj[k][0] >> 1) * h + (pj[k][1] >> 1)) * h + (pj[k][2] >> 1)) * h +
+                          (pj[k][3] >> 1)) * nJ + (J3 >> 1)) * nd + ((J4 - J3) >> 1)) * (unsigned)(nJ * nd);
+        abk = pab[k];
       }
     }
     const unsigned long long m = __ballot(live);
