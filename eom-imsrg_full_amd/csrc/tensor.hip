@@ -742,7 +742,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void k_
   // gather: slots (pattern, J3 of its run, J4 - J3), compacted to the entries that pass
   const int nslot = pstart[16];
   const double thr = (double)1e-6f;
-  int count = 0;
+  int count = 0, k = 0;  // a lane's pattern: its slot only grows, so k carries over
   for (int s0 = 0; s0 < nslot; s0 += 64) {
     const int s = s0 + lane;
     bool live = false;
@@ -756,7 +756,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void k_
       // in the J4 one (the block index is arithmetic: J4, Tz and the parity of
       // the J3 pairs with the tensor's), the product's block, then W
       const int sc = min(s, nslot - 1);
-      int k = 0;
       while (pstart[k + 1] <= sc) k++;
       const int r = sc - pstart[k], J3 = pj3[k] + 2 * (r / nd), J4 = J3 + 2 * (r % nd);
       const int okc = pok[k];
